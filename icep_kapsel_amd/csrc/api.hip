@@ -1,0 +1,604 @@
+// api.hip -- the C ABI of include/kapsel_gpu.h: context lifetime, host <->
+// device copies, and the reference's call sequences built from the kernels
+// in fft.hip, kspace.hip and particles.hip.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <exception>
+#include <new>
+
+#include "common.hpp"
+
+namespace kapsel {
+
+namespace {
+thread_local char g_err[512] = "";
+}
+
+void set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+}
+
+static hipEvent_t take_event(kapsel_ctx *ctx) {
+  if (!ctx->event_pool.empty()) {
+    hipEvent_t e = ctx->event_pool.back();
+    ctx->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  KAPSEL_HIP(hipEventCreate(&e));
+  return e;
+}
+
+LaunchTimer::LaunchTimer(kapsel_ctx *c, int k, double b) : ctx(c), cls(k), bytes(b) {
+  if (ctx->timing_mask & (1 << k)) {
+    start = take_event(ctx);
+    stop = take_event(ctx);
+    KAPSEL_HIP(hipEventRecord(start, ctx->stream));
+  }
+}
+
+LaunchTimer::~LaunchTimer() noexcept(false) {
+  if (stop) {
+    KAPSEL_HIP(hipEventRecord(stop, ctx->stream));
+    ctx->timing_pending.push_back(TimingRecord{cls, bytes, start, stop});
+  }
+}
+
+static void harvest_timing(kapsel_ctx *ctx) {
+  if (ctx->timing_pending.empty()) return;
+  KAPSEL_HIP(hipStreamSynchronize(ctx->stream));
+  for (const TimingRecord &r : ctx->timing_pending) {
+    float ms = 0.f;
+    KAPSEL_HIP(hipEventElapsedTime(&ms, r.start, r.stop));
+    ctx->timing_ms[r.cls] += ms;
+    ctx->timing_bytes[r.cls] += r.bytes;
+    ctx->timing_count[r.cls] += 1;
+    ctx->event_pool.push_back(r.start);
+    ctx->event_pool.push_back(r.stop);
+  }
+  ctx->timing_pending.clear();
+}
+
+// Particle_domain (profile.cxx:25-74): every integer offset within
+// RADIUS + XI/2 + 1.7321 DX, in the reference's i, j, k loop order
+static void build_sekibun(kapsel_ctx *ctx) {
+  const double dx = ctx->g.dx;
+  const int A = (int)(ctx->P.radius / dx + 1e-9), A_XI = (int)(ctx->P.xi / dx + 1e-9);
+  const int nh = A + A_XI + 2;
+  const double max_radius = ctx->P.radius + ctx->hxi + 1.7321 * dx;
+  ctx->sekibun.clear();
+  for (int i = -nh; i <= nh; i++)
+    for (int j = -nh; j <= nh; j++)
+      for (int k = -nh; k <= nh; k++) {
+        double x0 = (double)i * dx, x1 = (double)j * dx, x2 = (double)k * dx;
+        if (sqrt(x0 * x0 + x1 * x1 + x2 * x2) < max_radius) {
+          ctx->sekibun.push_back(i);
+          ctx->sekibun.push_back(j);
+          ctx->sekibun.push_back(k);
+        }
+      }
+  ctx->np_domain = (int)ctx->sekibun.size() / 3;
+  KAPSEL_HIP(hipMalloc(&ctx->d_sekibun, sizeof(int) * ctx->sekibun.size()));
+  KAPSEL_HIP(hipMemcpy(ctx->d_sekibun, ctx->sekibun.data(), sizeof(int) * ctx->sekibun.size(),
+                       hipMemcpyHostToDevice));
+}
+
+static bool pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+static void check_field(int f) {
+  if (f < 0 || f >= KAPSEL_NFIELDS) {
+    set_error("field id %d out of range", f);
+    throw Error{KAPSEL_ERR_ARG};
+  }
+}
+
+static void destroy(kapsel_ctx *ctx) {
+  if (!ctx) return;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (void *&p : ctx->fields)
+    if (p) (void)hipFree(p);
+  if (ctx->d_uk_dc) (void)hipFree(ctx->d_uk_dc);
+  if (ctx->d_ps) (void)hipFree(ctx->d_ps);
+  if (ctx->d_sekibun) (void)hipFree(ctx->d_sekibun);
+  free_twiddles(ctx);
+  for (auto &r : ctx->timing_pending) {
+    (void)hipEventDestroy(r.start);
+    (void)hipEventDestroy(r.stop);
+  }
+  for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+static void create(const kapsel_params *p, kapsel_ctx **out) {
+  if (!p || !out) {
+    set_error("null argument");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+  for (int n : {p->nx, p->ny, p->nz})
+    if (!pow2(n) || n < 16 || n > 256) {
+      set_error("grid size %d: must be a power of two in [16, 256]", n);
+      throw Error{KAPSEL_ERR_ARG};
+    }
+  if (!(p->dx > 0) || !(p->rho > 0) || !(p->eta > 0) || p->n_particles < 0 ||
+      (p->precision != KAPSEL_FP64 && p->precision != KAPSEL_FP32)) {
+    set_error("invalid physical parameters or precision");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+  if (p->n_particles > 0 && (!(p->radius > 0) || !(p->xi > 0))) {
+    set_error("particles need radius > 0 and xi > 0");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    set_error("no HIP device");
+    throw Error{KAPSEL_ERR_NODEV};
+  }
+  int dev = 0;
+  KAPSEL_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  KAPSEL_HIP(hipGetDeviceProperties(&prop, dev));
+  if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    set_error("device %d is %s; this library is built for gfx950 only", dev, prop.gcnArchName);
+    throw Error{KAPSEL_ERR_NODEV};
+  }
+  kapsel_ctx *ctx = new kapsel_ctx();
+  try {
+    ctx->P = *p;
+    ctx->device = dev;
+    ctx->prec = p->precision;
+    ctx->elem = p->precision == KAPSEL_FP64 ? 8 : 4;
+    Grid &g = ctx->g;
+    g.nx = p->nx;
+    g.ny = p->ny;
+    g.nz = p->nz;
+    g.h = g.nz / 2 + 1;  // HNZ_ (input.cxx:305)
+    g.nz_ = 2 * g.h;     // NZ_ (input.cxx:306)
+    g.hnx = g.nx / 2;
+    g.hny = g.ny / 2;
+    g.trn_x = (g.nx + 2) / 3;  // TRNs (input.cxx:310)
+    g.trn_y = (g.ny + 2) / 3;
+    g.trn_z = (g.nz + 2) / 3;
+    g.dx = p->dx;
+    g.lx = g.nx * g.dx;
+    g.ly = g.ny * g.dx;
+    g.lz = g.nz * g.dx;
+    g.wave_x = 2 * M_PI / g.lx;  // WAVE_X = PI2/LX (input.cxx:326)
+    g.wave_y = 2 * M_PI / g.ly;
+    g.wave_z = 2 * M_PI / g.lz;
+    g.f = (size_t)g.nx * g.ny * g.nz_;
+    g.nc = (size_t)g.nx * g.ny * g.h;
+    g.nr = (size_t)g.nx * g.ny * g.nz;
+    // Set_global_parameters (input.cxx:287-296, 300-301, 404) and A_R_cutoff (input.cxx:1530-1547)
+    ctx->irho = 1. / p->rho;
+    ctx->nu = p->eta * ctx->irho;
+    ctx->hxi = p->xi * 0.5;
+    ctx->mass = p->mass_ratio * p->rho * 4. / 3. * M_PI * p->radius * p->radius * p->radius;
+    ctx->imass = ctx->mass > 0 ? 1. / ctx->mass : 0.0;
+    ctx->moi = 2. / 5. * ctx->mass * p->radius * p->radius;
+    ctx->imoi = ctx->moi > 0 ? 1. / ctx->moi : 0.0;
+    if (p->lj_truncate > 0) {
+      ctx->a_r_cutoff = p->lj_powers == 0 ? pow(2., 1. / 6.) : p->lj_powers == 1 ? pow(2., 1. / 12.)
+                                                                                  : pow(2., 1. / 18.);
+    } else if (p->lj_truncate == 0) {
+      int nmin = g.nx < g.ny ? g.nx : g.ny;
+      nmin = nmin < g.nz ? nmin : g.nz;
+      ctx->a_r_cutoff = fmin(nmin * g.dx * .5 / p->lj_sigma, 2.5);
+    } else {
+      ctx->a_r_cutoff = 0.;
+    }
+    KAPSEL_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    for (int f = 0; f < KAPSEL_NFIELDS; f++) {
+      KAPSEL_HIP(hipMalloc(&ctx->fields[f], g.f * ctx->elem));
+      KAPSEL_HIP(hipMemsetAsync(ctx->fields[f], 0, g.f * ctx->elem, ctx->stream));
+    }
+    KAPSEL_HIP(hipMalloc(&ctx->d_uk_dc, 3 * sizeof(double)));
+    KAPSEL_HIP(hipMemsetAsync(ctx->d_uk_dc, 0, 3 * sizeof(double), ctx->stream));
+    size_t np = p->n_particles > 0 ? (size_t)p->n_particles : 1;
+    KAPSEL_HIP(hipMalloc(&ctx->d_ps, sizeof(double) * KAPSEL_P_NFIELDS * 3 * np));
+    KAPSEL_HIP(hipMemsetAsync(ctx->d_ps, 0, sizeof(double) * KAPSEL_P_NFIELDS * 3 * np, ctx->stream));
+    build_sekibun(ctx);
+    init_twiddles(ctx);
+    KAPSEL_HIP(hipStreamSynchronize(ctx->stream));
+  } catch (...) {
+    destroy(ctx);
+    throw;
+  }
+  *out = ctx;
+}
+
+static void check_ctx(kapsel_ctx *ctx) {
+  if (!ctx) {
+    set_error("null context");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+}
+
+static void check_time(const kapsel_time *t) {
+  if (!t || !(t->dt_fluid > 0)) {
+    set_error("jikan must be non-null with dt_fluid > 0");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+}
+
+// staging of a double host field into a device field of type T
+static void upload(kapsel_ctx *ctx, void *dst, const double *host) {
+  if (ctx->prec == KAPSEL_FP64) {
+    KAPSEL_HIP(hipMemcpyAsync(dst, host, ctx->g.f * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  } else {
+    double *tmp = nullptr;
+    KAPSEL_HIP(hipMalloc(&tmp, ctx->g.f * sizeof(double)));
+    KAPSEL_HIP(hipMemcpyAsync(tmp, host, ctx->g.f * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    copy_convert_in(ctx, dst, tmp);
+    KAPSEL_HIP(hipStreamSynchronize(ctx->stream));
+    KAPSEL_HIP(hipFree(tmp));
+  }
+  KAPSEL_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+static void download(kapsel_ctx *ctx, double *host, const void *src) {
+  if (ctx->prec == KAPSEL_FP64) {
+    KAPSEL_HIP(hipMemcpyAsync(host, src, ctx->g.f * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  } else {
+    double *tmp = nullptr;
+    KAPSEL_HIP(hipMalloc(&tmp, ctx->g.f * sizeof(double)));
+    copy_convert_out(ctx, tmp, src);
+    KAPSEL_HIP(hipMemcpyAsync(host, tmp, ctx->g.f * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    KAPSEL_HIP(hipStreamSynchronize(ctx->stream));
+    KAPSEL_HIP(hipFree(tmp));
+  }
+  KAPSEL_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+static void *fld(kapsel_ctx *ctx, int f) { return ctx->fields[f]; }
+
+// Zeta_k2u (operate_omega.h:138-141), optionally after the 2/3 cut and FIX_CELL
+static void zeta_k2u(kapsel_ctx *ctx, int fix_dc) {
+  zeta_k2u_k(ctx, fix_dc);
+  fft_c2r(ctx, ctx->fields + KAPSEL_U0, 3);
+}
+
+// NS_solver_slavedEuler (fluid_solver.cxx:340-366) with Zeta_k2advection_k
+// (operate_omega.cxx:100-111) and U2advection_k (operate_omega.cxx:12-98)
+static void ns_slaved_euler(kapsel_ctx *ctx, const kapsel_time &t) {
+  zeta_k2u(ctx, 0);
+  advection_products(ctx);
+  void *five[5] = {fld(ctx, KAPSEL_U0), fld(ctx, KAPSEL_U1), fld(ctx, KAPSEL_U2), fld(ctx, KAPSEL_WORK0),
+                   fld(ctx, KAPSEL_WORK1)};
+  fft_r2c(ctx, five, 5);
+  advection_combine_slaved(ctx, t.dt_fluid);
+}
+
+// Solenoidal_u (operate_omega.h:109-116)
+static void solenoidal_u(kapsel_ctx *ctx, void *const *f3) {
+  fft_r2c(ctx, f3, 3);
+  solenoidal_uk(ctx, f3);
+  fft_c2r(ctx, f3, 3);
+}
+
+// Make_f_particle_dt_sole + Add_f_particle (f_particle.h:37-95)
+static void f_particle_dt_sole_add(kapsel_ctx *ctx) {
+  make_f_nonsole(ctx);
+  solenoidal_u(ctx, ctx->fields + KAPSEL_F0);
+  add_f_particle(ctx);
+}
+
+// U2zeta_k (operate_omega.h:212-215)
+static void u2zeta_k(kapsel_ctx *ctx) {
+  fft_r2c(ctx, ctx->fields + KAPSEL_U0, 3);
+  u_k2zeta_k(ctx);
+}
+
+static int fix_mask(const kapsel_ctx *ctx) {
+  int m = 0;
+  for (int d = 0; d < 3; d++)
+    if (ctx->P.fix_cell[d]) m |= 1 << d;
+  return m;
+}
+
+// Time_evolution_hydro, Navier-Stokes branch (sp_3d_ns.cxx:24-186)
+static void spm_step(kapsel_ctx *ctx, kapsel_time &t) {
+  ns_slaved_euler(ctx, t);                    // :27
+  zeta_k2u(ctx, fix_mask(ctx));               // :29-42
+  md_position(ctx, t);                        // :44-50
+  make_phi_particle_sum(ctx);                 // :51-53
+  calc_f_hydro(ctx, t);                       // :99-101
+  force(ctx);                                 // :104-110
+  md_velocity_integrate(ctx, t);
+  make_u_particle_sum(ctx);                   // :166-167
+  f_particle_dt_sole_add(ctx);                // :168-169
+  u2zeta_k(ctx);                              // :184
+  t.time += t.dt_fluid;                       // :525
+  t.ts += 1;
+}
+
+// main, initial rigidity projection (sp_3d_ns.cxx:431-447)
+static void init_rigidity(kapsel_ctx *ctx) {
+  make_phi_particle_sum(ctx);
+  make_u_particle_sum(ctx);
+  zeta_k2u(ctx, -1);
+  f_particle_dt_sole_add(ctx);
+  u2zeta_k(ctx);
+  KAPSEL_HIP(hipMemsetAsync(ctx->d_uk_dc, 0, 3 * sizeof(double), ctx->stream));
+}
+
+}  // namespace kapsel
+
+using namespace kapsel;
+
+#define API_BEGIN try {
+#define API_END                                   \
+  }                                               \
+  catch (const Error &e) {                        \
+    return e.code;                                \
+  }                                               \
+  catch (const std::bad_alloc &) {                \
+    set_error("host allocation failed");          \
+    return KAPSEL_ERR_STATE;                      \
+  }                                               \
+  catch (const std::exception &e) {               \
+    set_error("%s", e.what());                    \
+    return KAPSEL_ERR_STATE;                      \
+  }                                               \
+  return KAPSEL_OK;
+
+extern "C" {
+
+const char *kapsel_last_error(void) { return g_err; }
+
+int kapsel_create(const kapsel_params *params, kapsel_ctx **out) {
+  API_BEGIN
+  create(params, out);
+  API_END
+}
+
+int kapsel_destroy(kapsel_ctx *ctx) {
+  API_BEGIN
+  destroy(ctx);
+  API_END
+}
+
+int kapsel_synchronize(kapsel_ctx *ctx) {
+  API_BEGIN
+  check_ctx(ctx);
+  KAPSEL_HIP(hipStreamSynchronize(ctx->stream));
+  API_END
+}
+
+int kapsel_get_sekibun(kapsel_ctx *ctx, int *np_domain, int *cells) {
+  API_BEGIN
+  check_ctx(ctx);
+  if (!np_domain) {
+    set_error("null np_domain");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+  *np_domain = ctx->np_domain;
+  if (cells) memcpy(cells, ctx->sekibun.data(), sizeof(int) * ctx->sekibun.size());
+  API_END
+}
+
+int kapsel_set_field(kapsel_ctx *ctx, int field, const double *host) {
+  API_BEGIN
+  check_ctx(ctx);
+  check_field(field);
+  if (!host) {
+    set_error("null host buffer");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+  upload(ctx, ctx->fields[field], host);
+  API_END
+}
+
+int kapsel_get_field(kapsel_ctx *ctx, int field, double *host) {
+  API_BEGIN
+  check_ctx(ctx);
+  check_field(field);
+  if (!host) {
+    set_error("null host buffer");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+  download(ctx, host, ctx->fields[field]);
+  API_END
+}
+
+int kapsel_set_uk_dc(kapsel_ctx *ctx, const double uk_dc[3]) {
+  API_BEGIN
+  check_ctx(ctx);
+  KAPSEL_HIP(hipMemcpyAsync(ctx->d_uk_dc, uk_dc, 3 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  KAPSEL_HIP(hipStreamSynchronize(ctx->stream));
+  API_END
+}
+
+int kapsel_get_uk_dc(kapsel_ctx *ctx, double uk_dc[3]) {
+  API_BEGIN
+  check_ctx(ctx);
+  KAPSEL_HIP(hipMemcpyAsync(uk_dc, ctx->d_uk_dc, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  KAPSEL_HIP(hipStreamSynchronize(ctx->stream));
+  API_END
+}
+
+int kapsel_set_particles(kapsel_ctx *ctx, const double *state) {
+  API_BEGIN
+  check_ctx(ctx);
+  size_t n = sizeof(double) * KAPSEL_P_NFIELDS * 3 * ctx->P.n_particles;
+  if (n) {
+    KAPSEL_HIP(hipMemcpyAsync(ctx->d_ps, state, n, hipMemcpyHostToDevice, ctx->stream));
+    KAPSEL_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  API_END
+}
+
+int kapsel_get_particles(kapsel_ctx *ctx, double *state) {
+  API_BEGIN
+  check_ctx(ctx);
+  size_t n = sizeof(double) * KAPSEL_P_NFIELDS * 3 * ctx->P.n_particles;
+  if (n) {
+    KAPSEL_HIP(hipMemcpyAsync(state, ctx->d_ps, n, hipMemcpyDeviceToHost, ctx->stream));
+    KAPSEL_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  API_END
+}
+
+int kapsel_fft_r2c(kapsel_ctx *ctx, int field) {
+  API_BEGIN
+  check_ctx(ctx);
+  check_field(field);
+  fft_r2c(ctx, ctx->fields + field, 1);
+  API_END
+}
+
+int kapsel_fft_c2r(kapsel_ctx *ctx, int field) {
+  API_BEGIN
+  check_ctx(ctx);
+  check_field(field);
+  fft_c2r(ctx, ctx->fields + field, 1);
+  API_END
+}
+
+int kapsel_truncate(kapsel_ctx *ctx, int field) {
+  API_BEGIN
+  check_ctx(ctx);
+  check_field(field);
+  truncate(ctx, ctx->fields + field, 1);
+  API_END
+}
+
+int kapsel_zeta_k2u_k(kapsel_ctx *ctx) {
+  API_BEGIN
+  check_ctx(ctx);
+  zeta_k2u_k(ctx, -1);
+  API_END
+}
+
+int kapsel_u_k2zeta_k(kapsel_ctx *ctx) {
+  API_BEGIN
+  check_ctx(ctx);
+  u_k2zeta_k(ctx);
+  API_END
+}
+
+int kapsel_solenoidal_u(kapsel_ctx *ctx, int field0) {
+  API_BEGIN
+  check_ctx(ctx);
+  check_field(field0);
+  check_field(field0 + 2);
+  solenoidal_u(ctx, ctx->fields + field0);
+  API_END
+}
+
+int kapsel_ns_slaved_euler(kapsel_ctx *ctx, const kapsel_time *jikan) {
+  API_BEGIN
+  check_ctx(ctx);
+  check_time(jikan);
+  ns_slaved_euler(ctx, *jikan);
+  API_END
+}
+
+int kapsel_make_phi_particle_sum(kapsel_ctx *ctx) {
+  API_BEGIN
+  check_ctx(ctx);
+  make_phi_particle_sum(ctx);
+  API_END
+}
+
+int kapsel_make_u_particle_sum(kapsel_ctx *ctx) {
+  API_BEGIN
+  check_ctx(ctx);
+  make_u_particle_sum(ctx);
+  API_END
+}
+
+int kapsel_make_phi_particle(kapsel_ctx *ctx, int field) {
+  API_BEGIN
+  check_ctx(ctx);
+  check_field(field);
+  make_phi_particle(ctx, ctx->fields[field]);
+  API_END
+}
+
+int kapsel_calc_f_hydro(kapsel_ctx *ctx, const kapsel_time *jikan) {
+  API_BEGIN
+  check_ctx(ctx);
+  check_time(jikan);
+  calc_f_hydro(ctx, *jikan);
+  API_END
+}
+
+int kapsel_force(kapsel_ctx *ctx) {
+  API_BEGIN
+  check_ctx(ctx);
+  force(ctx);
+  API_END
+}
+
+int kapsel_md_position(kapsel_ctx *ctx, const kapsel_time *jikan) {
+  API_BEGIN
+  check_ctx(ctx);
+  check_time(jikan);
+  md_position(ctx, *jikan);
+  API_END
+}
+
+int kapsel_md_velocity(kapsel_ctx *ctx, const kapsel_time *jikan) {
+  API_BEGIN
+  check_ctx(ctx);
+  check_time(jikan);
+  force(ctx);
+  md_velocity_integrate(ctx, *jikan);
+  API_END
+}
+
+int kapsel_make_f_particle_dt_sole(kapsel_ctx *ctx) {
+  API_BEGIN
+  check_ctx(ctx);
+  f_particle_dt_sole_add(ctx);
+  API_END
+}
+
+int kapsel_init_rigidity(kapsel_ctx *ctx) {
+  API_BEGIN
+  check_ctx(ctx);
+  init_rigidity(ctx);
+  API_END
+}
+
+int kapsel_spm_step(kapsel_ctx *ctx, kapsel_time *jikan) {
+  API_BEGIN
+  check_ctx(ctx);
+  check_time(jikan);
+  spm_step(ctx, *jikan);
+  API_END
+}
+
+int kapsel_kernel_timing(kapsel_ctx *ctx, int mask) {
+  API_BEGIN
+  check_ctx(ctx);
+  harvest_timing(ctx);
+  ctx->timing_mask = mask;
+  API_END
+}
+
+int kapsel_kernel_timing_get(kapsel_ctx *ctx, int kernel_class, double *total_ms, int *launches,
+                             double *total_bytes) {
+  API_BEGIN
+  check_ctx(ctx);
+  if (kernel_class < 0 || kernel_class >= KAPSEL_K_NCLASSES || !total_ms || !launches || !total_bytes) {
+    set_error("bad kernel class or null output");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+  harvest_timing(ctx);
+  *total_ms = ctx->timing_ms[kernel_class];
+  *launches = ctx->timing_count[kernel_class];
+  *total_bytes = ctx->timing_bytes[kernel_class];
+  ctx->timing_ms[kernel_class] = 0.0;
+  ctx->timing_bytes[kernel_class] = 0.0;
+  ctx->timing_count[kernel_class] = 0;
+  API_END
+}
+
+}  // extern "C"

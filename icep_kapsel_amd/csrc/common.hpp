@@ -1,0 +1,187 @@
+// common.hpp -- shared definitions of the gfx950 SPM library: context, error
+// handling, complex helpers and launch instrumentation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/kapsel_gpu.h"
+
+namespace kapsel {
+
+// ---------------------------------------------------------------- errors
+void set_error(const char *fmt, ...);
+struct Error {
+  int code;
+};
+
+#define KAPSEL_HIP(call)                                                          \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess) {                                                       \
+      ::kapsel::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                          __FILE__, __LINE__);                                    \
+      throw ::kapsel::Error{KAPSEL_ERR_HIP};                                      \
+    }                                                                             \
+  } while (0)
+
+#define KAPSEL_LAUNCH_CHECK() KAPSEL_HIP(hipGetLastError())
+
+// ---------------------------------------------------------------- complex
+template <typename T>
+struct V2 {
+  T re, im;
+};
+template <typename T>
+__device__ __forceinline__ V2<T> operator+(V2<T> a, V2<T> b) {
+  return {a.re + b.re, a.im + b.im};
+}
+template <typename T>
+__device__ __forceinline__ V2<T> operator-(V2<T> a, V2<T> b) {
+  return {a.re - b.re, a.im - b.im};
+}
+template <typename T>
+__device__ __forceinline__ V2<T> cmul(V2<T> a, V2<T> b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename T>
+__device__ __forceinline__ V2<T> conjv(V2<T> a) {
+  return {a.re, -a.im};
+}
+template <typename T>
+__device__ __forceinline__ V2<T> scal(T s, V2<T> a) {
+  return {s * a.re, s * a.im};
+}
+
+// vector types for 2-element loads/stores
+template <typename T>
+struct VecT;
+template <>
+struct VecT<double> {
+  using type = double2;
+};
+template <>
+struct VecT<float> {
+  using type = float2;
+};
+template <typename T>
+__device__ __forceinline__ V2<T> ld2(const T *p) {
+  auto v = *reinterpret_cast<const typename VecT<T>::type *>(p);
+  return {v.x, v.y};
+}
+template <typename T>
+__device__ __forceinline__ void st2(T *p, V2<T> a) {
+  typename VecT<T>::type v;
+  v.x = a.re;
+  v.y = a.im;
+  *reinterpret_cast<typename VecT<T>::type *>(p) = v;
+}
+
+// ---------------------------------------------------------------- grid
+// Scalars every kernel needs, passed by value.
+struct Grid {
+  int nx, ny, nz;     // NX, NY, NZ
+  int nz_;            // NZ_ = 2*(NZ/2+1), row pitch in reals
+  int h;              // NZ/2+1, row pitch in complex values
+  int hnx, hny;       // NX/2, NY/2
+  int trn_x, trn_y, trn_z;
+  double dx;
+  double wave_x, wave_y, wave_z;
+  double lx, ly, lz;  // box lengths L[d]
+  size_t f;           // reals per padded field = NX*NY*NZ_
+  size_t nc;          // complex values per field = NX*NY*h
+  size_t nr;          // real grid points = NX*NY*NZ
+};
+
+// ---------------------------------------------------------------- timing
+struct TimingRecord {
+  int cls;
+  double bytes;
+  hipEvent_t start, stop;
+};
+
+// ---------------------------------------------------------------- context
+struct Twiddles {
+  void *x = nullptr, *y = nullptr, *zh = nullptr, *zrot = nullptr;  // V2<T> tables
+};
+
+}  // namespace kapsel
+
+struct kapsel_ctx {
+  kapsel_params P;
+  kapsel::Grid g;
+  int prec;  // KAPSEL_FP64 or KAPSEL_FP32
+  size_t elem;  // sizeof(T)
+  hipStream_t stream = nullptr;
+  int device = 0;
+  void *fields[KAPSEL_NFIELDS] = {};
+  double *d_uk_dc = nullptr;      // 3 doubles (device)
+  double *d_ps = nullptr;         // particle state (device), SoA
+  int *d_sekibun = nullptr;       // np_domain * 3 ints
+  int np_domain = 0;
+  std::vector<int> sekibun;       // host copy
+  kapsel::Twiddles tw;
+  // derived physics constants (Set_global_parameters, input.cxx:287-430)
+  double nu, irho, hxi, mass, imass, moi, imoi, a_r_cutoff;
+  // kernel timing
+  int timing_mask = 0;
+  std::vector<kapsel::TimingRecord> timing_pending;
+  std::vector<hipEvent_t> event_pool;
+  double timing_ms[KAPSEL_K_NCLASSES] = {};
+  double timing_bytes[KAPSEL_K_NCLASSES] = {};
+  int timing_count[KAPSEL_K_NCLASSES] = {};
+};
+
+namespace kapsel {
+
+// Brackets one kernel launch with HIP events when its class is being timed;
+// `bytes` is the launch's algorithmic HBM traffic (each input read once,
+// each output written once).
+struct LaunchTimer {
+  kapsel_ctx *ctx;
+  int cls;
+  double bytes;
+  hipEvent_t start = nullptr, stop = nullptr;
+  LaunchTimer(kapsel_ctx *c, int k, double b);
+  ~LaunchTimer() noexcept(false);
+};
+
+inline unsigned blocks_for(size_t n, unsigned threads) {
+  return (unsigned)((n + threads - 1) / threads);
+}
+
+// ---- launchers (defined in the .hip files) ----------------------------------
+// FFT (fft.hip)
+void init_twiddles(kapsel_ctx *ctx);
+void free_twiddles(kapsel_ctx *ctx);
+// nf fields transformed together (1..5)
+void fft_r2c(kapsel_ctx *ctx, void *const *fields, int nf);
+void fft_c2r(kapsel_ctx *ctx, void *const *fields, int nf);
+
+// k-space and real-space pointwise (kspace.hip)
+void truncate(kapsel_ctx *ctx, void *const *fields, int nf);
+void zeta_k2u_k(kapsel_ctx *ctx, int fix_dc);
+void u_k2zeta_k(kapsel_ctx *ctx);
+void advection_products(kapsel_ctx *ctx);
+void advection_combine_slaved(kapsel_ctx *ctx, double dt_fluid);
+void solenoidal_uk(kapsel_ctx *ctx, void *const *f3);
+void make_f_nonsole(kapsel_ctx *ctx);
+void add_f_particle(kapsel_ctx *ctx);
+void copy_convert_in(kapsel_ctx *ctx, void *dst, const double *src_dev);
+void copy_convert_out(kapsel_ctx *ctx, double *dst_dev, const void *src);
+
+// particles (particles.hip)
+void md_position(kapsel_ctx *ctx, const kapsel_time &t);
+void make_phi_particle_sum(kapsel_ctx *ctx);
+void make_u_particle_sum(kapsel_ctx *ctx);
+void make_phi_particle(kapsel_ctx *ctx, void *field);
+void make_phi_sin(kapsel_ctx *ctx, void *field);
+void calc_f_hydro(kapsel_ctx *ctx, const kapsel_time &t);
+void force(kapsel_ctx *ctx);
+void md_velocity_integrate(kapsel_ctx *ctx, const kapsel_time &t);
+
+}  // namespace kapsel

@@ -1,0 +1,413 @@
+// fft.hip -- 3-D real FFT on the padded Ooura layout for gfx950.
+//
+// Forward r2c (A2a_k, fft_wrapper.h:681-701): X(k) = sum_x a(x) e^{-ik.x},
+// unscaled, half spectrum kz = 0..NZ/2 stored as (re, im) pairs in rows of
+// NZ_ = NZ+2 reals.  Inverse c2r (A_k2a, fft_wrapper.h:704-723) scales by
+// 1/(NX NY NZ).
+//
+// Three passes, each one read and one write of the field:
+//   z pass  r2c of each row: a length-NZ/2 complex FFT of the even/odd
+//           interleaved row, then the real split (c2r: the split first);
+//   y pass  complex FFT along y, lines at stride h = NZ/2+1 complex values;
+//   x pass  complex FFT along x, lines at stride NY*h.
+// Every line FFT of length L = R1*R2 is a four-step transform: each of R2
+// threads loads R1 strided points straight from HBM into registers and does
+// a radix-R1 DFT, the twiddled partial results cross once through LDS, and
+// R2-point DFTs finish in registers and store straight to HBM.  Adjacent
+// lines are adjacent in memory (lines of a tile differ in the fastest
+// index), so each load/store instruction of a wavefront touches few 64-B
+// segments.  Up to five fields are transformed by one launch (grid.y).
+#include <utility>
+
+#include "common.hpp"
+
+namespace kapsel {
+
+// ---------------------------------------------------------------- constants
+// cos(2 pi m / 16)
+__host__ __device__ constexpr double cos16(int m) {
+  m &= 15;
+  return m == 0    ? 1.0
+         : m == 1  ? 0.92387953251128675613
+         : m == 2  ? 0.70710678118654752440
+         : m == 3  ? 0.38268343236508977173
+         : m == 4  ? 0.0
+         : m == 5  ? -0.38268343236508977173
+         : m == 6  ? -0.70710678118654752440
+         : m == 7  ? -0.92387953251128675613
+         : m == 8  ? -1.0
+         : m == 9  ? -0.92387953251128675613
+         : m == 10 ? -0.70710678118654752440
+         : m == 11 ? -0.38268343236508977173
+         : m == 12 ? 0.0
+         : m == 13 ? 0.38268343236508977173
+         : m == 14 ? 0.70710678118654752440
+                   : 0.92387953251128675613;
+}
+__host__ __device__ constexpr double sin16(int m) { return cos16(m - 4); }
+
+// b * W_16^M with W_16 = exp(DIR 2 pi i / 16); exact for the axis multiples
+template <int DIR, int M, typename T>
+__device__ __forceinline__ V2<T> mulw16(V2<T> b) {
+  constexpr int m = M & 15;
+  if constexpr (m == 0) {
+    return b;
+  } else if constexpr (m == 4) {
+    if constexpr (DIR < 0) return V2<T>{b.im, -b.re};
+    else return V2<T>{-b.im, b.re};
+  } else if constexpr (m == 8) {
+    return V2<T>{-b.re, -b.im};
+  } else if constexpr (m == 12) {
+    if constexpr (DIR < 0) return V2<T>{-b.im, b.re};
+    else return V2<T>{b.im, -b.re};
+  } else {
+    constexpr T c = (T)cos16(m);
+    constexpr T s = (T)(DIR * sin16(m));
+    return V2<T>{b.re * c - b.im * s, b.re * s + b.im * c};
+  }
+}
+
+template <int R, int DIR, typename T, int... K>
+__device__ __forceinline__ void dit_combine(const V2<T> *e, const V2<T> *o, V2<T> *out,
+                                            std::integer_sequence<int, K...>) {
+  ((out[K] = e[K] + mulw16<DIR, K *(16 / R)>(o[K]),
+    out[K + R / 2] = e[K] - mulw16<DIR, K *(16 / R)>(o[K])),
+   ...);
+}
+
+// out[k] = sum_n in[S n] W_R^{nk}, k < R, radix-2 decimation in time in
+// registers (all indices compile-time)
+template <int R, int S, int DIR, typename T>
+__device__ __forceinline__ void dft(const V2<T> *in, V2<T> *out) {
+  if constexpr (R == 1) {
+    out[0] = in[0];
+  } else {
+    V2<T> e[R / 2], o[R / 2];
+    dft<R / 2, 2 * S, DIR>(in, e);
+    dft<R / 2, 2 * S, DIR>(in + S, o);
+    dit_combine<R, DIR>(e, o, out, std::make_integer_sequence<int, R / 2>{});
+  }
+}
+
+// factorisation L = R1 * R2 (R1 >= R2)
+template <int L>
+struct FS;
+template <> struct FS<8> { static constexpr int R1 = 4, R2 = 2; };
+template <> struct FS<16> { static constexpr int R1 = 4, R2 = 4; };
+template <> struct FS<32> { static constexpr int R1 = 8, R2 = 4; };
+template <> struct FS<64> { static constexpr int R1 = 8, R2 = 8; };
+template <> struct FS<128> { static constexpr int R1 = 16, R2 = 8; };
+template <> struct FS<256> { static constexpr int R1 = 16, R2 = 16; };
+
+template <typename T>
+struct FieldPtrs {
+  T *p[5];
+};
+
+// twiddle W_L^m, table holds the forward sign
+template <int DIR, typename T>
+__device__ __forceinline__ V2<T> tw_get(const V2<T> *__restrict__ tw, int m) {
+  V2<T> w = tw[m];
+  if constexpr (DIR > 0) w.im = -w.im;
+  return w;
+}
+
+// ------------------------------------------------------------ c2c line pass
+// Line q (0 <= q < Q) starts at complex offset (q / A) * B + (q % A); its
+// elements are ES complex values apart.  Lines q of one tile are adjacent.
+template <typename T, int L, int DIR, int LPB>
+__global__ __launch_bounds__(LPB *FS<L>::R2) void fft_c2c_lines(FieldPtrs<T> fp, long long Q, long long A,
+                                                                 long long B, long long ES,
+                                                                 const V2<T> *__restrict__ tw) {
+  constexpr int R1 = FS<L>::R1, R2 = FS<L>::R2, TPL = R2;
+  constexpr int LS = R2 * (R1 + 1) + 1;  // LDS line stride (odd)
+  __shared__ V2<T> sm[LPB * LS];
+  T *__restrict__ data = fp.p[blockIdx.y];
+  const int t = threadIdx.x;
+  const int l = t % LPB;
+  const int n2 = t / LPB;
+  const long long q = (long long)blockIdx.x * LPB + l;
+  const bool valid = q < Q;
+  const long long base = valid ? (q / A) * B + (q % A) : 0;
+
+  V2<T> v[R1], o[R1];
+#pragma unroll
+  for (int n1 = 0; n1 < R1; n1++) {
+    v[n1] = valid ? ld2(data + 2 * (base + (long long)(R2 * n1 + n2) * ES)) : V2<T>{0, 0};
+  }
+  dft<R1, 1, DIR>(v, o);
+#pragma unroll
+  for (int k1 = 1; k1 < R1; k1++) o[k1] = cmul(o[k1], tw_get<DIR>(tw, n2 * k1));
+  V2<T> *sl = sm + l * LS;
+#pragma unroll
+  for (int k1 = 0; k1 < R1; k1++) sl[n2 * (R1 + 1) + k1] = o[k1];
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < R1 / TPL; c++) {
+    const int k1 = n2 + TPL * c;
+    V2<T> w[R2], x[R2];
+#pragma unroll
+    for (int m = 0; m < R2; m++) w[m] = sl[m * (R1 + 1) + k1];
+    dft<R2, 1, DIR>(w, x);
+    if (valid) {
+#pragma unroll
+      for (int k2 = 0; k2 < R2; k2++) st2(data + 2 * (base + (long long)(k1 + R1 * k2) * ES), x[k2]);
+    }
+  }
+}
+
+// --------------------------------------------------------------- z passes
+// Row r (NX*NY rows, pitch nz_ reals).  z[n] = a[2n] + i a[2n+1], n < M.
+template <typename T, int NZ, int RPB>
+__global__ __launch_bounds__(RPB *FS<NZ / 2>::R2) void fft_z_r2c(FieldPtrs<T> fp, long long rows, int nz_,
+                                                                  const V2<T> *__restrict__ twm,
+                                                                  const V2<T> *__restrict__ zrot) {
+  constexpr int M = NZ / 2;
+  constexpr int R1 = FS<M>::R1, R2 = FS<M>::R2, TPL = R2;
+  constexpr int RS = R2 * (R1 + 1) + 1;
+  static_assert(RS >= M + 1, "row stride");
+  __shared__ V2<T> sm[RPB * RS];
+  T *__restrict__ data = fp.p[blockIdx.y];
+  const int t = threadIdx.x;
+  const int n2 = t % TPL;
+  const int r = t / TPL;
+  const long long row = (long long)blockIdx.x * RPB + r;
+  const bool valid = row < rows;
+  T *__restrict__ rp = data + (valid ? row : 0) * nz_;
+  V2<T> *sr = sm + r * RS;
+
+  V2<T> v[R1], o[R1];
+#pragma unroll
+  for (int n1 = 0; n1 < R1; n1++) v[n1] = valid ? ld2(rp + 2 * (R2 * n1 + n2)) : V2<T>{0, 0};
+  dft<R1, 1, -1>(v, o);
+#pragma unroll
+  for (int k1 = 1; k1 < R1; k1++) o[k1] = cmul(o[k1], tw_get<-1>(twm, n2 * k1));
+#pragma unroll
+  for (int k1 = 0; k1 < R1; k1++) sr[n2 * (R1 + 1) + k1] = o[k1];
+  __syncthreads();
+  V2<T> zz[R1 / TPL][R2];
+#pragma unroll
+  for (int c = 0; c < R1 / TPL; c++) {
+    const int k1 = n2 + TPL * c;
+    V2<T> w[R2];
+#pragma unroll
+    for (int m = 0; m < R2; m++) w[m] = sr[m * (R1 + 1) + k1];
+    dft<R2, 1, -1>(w, zz[c]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < R1 / TPL; c++) {
+    const int k1 = n2 + TPL * c;
+#pragma unroll
+    for (int k2 = 0; k2 < R2; k2++) sr[k1 + R1 * k2] = zz[c][k2];
+  }
+  __syncthreads();
+  if (valid) {
+    // X[k] = E + W^k O,  E = (Z[k] + conj Z[M-k]) / 2,  O = (Z[k] - conj Z[M-k]) / 2i
+    for (int k = n2; k <= M; k += TPL) {
+      V2<T> zk = sr[k & (M - 1)];
+      V2<T> zm = conjv(sr[(M - k) & (M - 1)]);
+      V2<T> e = scal((T)0.5, zk + zm);
+      V2<T> d = zk - zm;
+      V2<T> od = {(T)0.5 * d.im, (T)-0.5 * d.re};
+      st2(rp + 2 * k, e + cmul(zrot[k], od));
+    }
+  }
+}
+
+template <typename T, int NZ, int RPB>
+__global__ __launch_bounds__(RPB *FS<NZ / 2>::R2) void fft_z_c2r(FieldPtrs<T> fp, long long rows, int nz_,
+                                                                  const V2<T> *__restrict__ twm,
+                                                                  const V2<T> *__restrict__ zrot, T scale) {
+  constexpr int M = NZ / 2;
+  constexpr int R1 = FS<M>::R1, R2 = FS<M>::R2, TPL = R2;
+  constexpr int RS = R2 * (R1 + 1) + 1;
+  __shared__ V2<T> sm[RPB * RS];
+  T *__restrict__ data = fp.p[blockIdx.y];
+  const int t = threadIdx.x;
+  const int n2 = t % TPL;
+  const int r = t / TPL;
+  const long long row = (long long)blockIdx.x * RPB + r;
+  const bool valid = row < rows;
+  T *__restrict__ rp = data + (valid ? row : 0) * nz_;
+  V2<T> *sr = sm + r * RS;
+
+  for (int k = n2; k <= M; k += TPL) {
+    V2<T> x = valid ? ld2(rp + 2 * k) : V2<T>{0, 0};
+    if (k == 0 || k == M) x.im = 0;  // c2r treats the kz = 0, NZ/2 entries as real
+    sr[k] = x;
+  }
+  __syncthreads();
+  V2<T> v[R1], o[R1];
+#pragma unroll
+  for (int n1 = 0; n1 < R1; n1++) {
+    const int n = R2 * n1 + n2;
+    V2<T> xn = sr[n], xm = conjv(sr[M - n]);
+    V2<T> e = xn + xm;                                  // 2 E[n]
+    V2<T> od = cmul(xn - xm, conjv(zrot[n]));           // 2 O[n]
+    v[n1] = V2<T>{e.re - od.im, e.im + od.re};          // E + i O
+  }
+  dft<R1, 1, +1>(v, o);
+#pragma unroll
+  for (int k1 = 1; k1 < R1; k1++) o[k1] = cmul(o[k1], tw_get<+1>(twm, n2 * k1));
+  __syncthreads();
+#pragma unroll
+  for (int k1 = 0; k1 < R1; k1++) sr[n2 * (R1 + 1) + k1] = o[k1];
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < R1 / TPL; c++) {
+    const int k1 = n2 + TPL * c;
+    V2<T> w[R2], x[R2];
+#pragma unroll
+    for (int m = 0; m < R2; m++) w[m] = sr[m * (R1 + 1) + k1];
+    dft<R2, 1, +1>(w, x);
+    if (valid) {
+#pragma unroll
+      for (int k2 = 0; k2 < R2; k2++) st2(rp + 2 * (k1 + R1 * k2), scal(scale, x[k2]));
+    }
+  }
+  if (valid && n2 == 0) {
+    rp[NZ] = 0;
+    rp[NZ + 1] = 0;
+  }
+}
+
+// ------------------------------------------------------------------- host
+namespace {
+
+template <typename T>
+void *make_table(int n, int count) {
+  std::vector<V2<T>> h(count);
+  for (int m = 0; m < count; m++) {
+    long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)n;
+    h[m] = V2<T>{(T)cosl(a), (T)sinl(a)};
+  }
+  void *d = nullptr;
+  KAPSEL_HIP(hipMalloc(&d, sizeof(V2<T>) * count));
+  KAPSEL_HIP(hipMemcpy(d, h.data(), sizeof(V2<T>) * count, hipMemcpyHostToDevice));
+  return d;
+}
+
+template <typename T>
+void init_tw_t(kapsel_ctx *ctx) {
+  const Grid &g = ctx->g;
+  ctx->tw.x = make_table<T>(g.nx, g.nx);
+  ctx->tw.y = make_table<T>(g.ny, g.ny);
+  ctx->tw.zh = make_table<T>(g.nz / 2, g.nz / 2);
+  ctx->tw.zrot = make_table<T>(g.nz, g.nz / 2 + 1);
+}
+
+constexpr int lines_per_block(size_t elem) { return elem == 8 ? 16 : 32; }
+
+template <typename T, int L, int DIR>
+void launch_lines(kapsel_ctx *ctx, int cls, const FieldPtrs<T> &fp, int nf, long long Q, long long A,
+                  long long B, long long ES, const void *tw) {
+  constexpr int LPB = lines_per_block(sizeof(T));
+  dim3 grid((unsigned)((Q + LPB - 1) / LPB), nf);
+  LaunchTimer lt(ctx, cls, 2.0 * nf * ctx->g.f * sizeof(T));
+  hipLaunchKernelGGL((fft_c2c_lines<T, L, DIR, LPB>), grid, dim3(LPB * FS<L>::R2), 0, ctx->stream, fp, Q, A,
+                     B, ES, (const V2<T> *)tw);
+  KAPSEL_LAUNCH_CHECK();
+}
+
+template <typename T, int DIR>
+void lines_dispatch(kapsel_ctx *ctx, int cls, int L, const FieldPtrs<T> &fp, int nf, long long Q, long long A,
+                    long long B, long long ES, const void *tw) {
+  switch (L) {
+    case 16: launch_lines<T, 16, DIR>(ctx, cls, fp, nf, Q, A, B, ES, tw); break;
+    case 32: launch_lines<T, 32, DIR>(ctx, cls, fp, nf, Q, A, B, ES, tw); break;
+    case 64: launch_lines<T, 64, DIR>(ctx, cls, fp, nf, Q, A, B, ES, tw); break;
+    case 128: launch_lines<T, 128, DIR>(ctx, cls, fp, nf, Q, A, B, ES, tw); break;
+    case 256: launch_lines<T, 256, DIR>(ctx, cls, fp, nf, Q, A, B, ES, tw); break;
+    default: set_error("unsupported FFT length %d", L); throw Error{KAPSEL_ERR_ARG};
+  }
+}
+
+constexpr int rows_per_block(size_t elem) { return elem == 8 ? 16 : 32; }
+
+template <typename T, int NZ>
+void launch_z(kapsel_ctx *ctx, const FieldPtrs<T> &fp, int nf, bool fwd) {
+  constexpr int RPB = rows_per_block(sizeof(T));
+  const Grid &g = ctx->g;
+  long long rows = (long long)g.nx * g.ny;
+  dim3 grid((unsigned)((rows + RPB - 1) / RPB), nf);
+  dim3 block(RPB * FS<NZ / 2>::R2);
+  LaunchTimer lt(ctx, KAPSEL_K_FFT_Z, 2.0 * nf * g.f * sizeof(T));
+  if (fwd) {
+    hipLaunchKernelGGL((fft_z_r2c<T, NZ, RPB>), grid, block, 0, ctx->stream, fp, rows, g.nz_,
+                       (const V2<T> *)ctx->tw.zh, (const V2<T> *)ctx->tw.zrot);
+  } else {
+    T scale = (T)(1.0 / ((double)g.nx * g.ny * g.nz));
+    hipLaunchKernelGGL((fft_z_c2r<T, NZ, RPB>), grid, block, 0, ctx->stream, fp, rows, g.nz_,
+                       (const V2<T> *)ctx->tw.zh, (const V2<T> *)ctx->tw.zrot, scale);
+  }
+  KAPSEL_LAUNCH_CHECK();
+}
+
+template <typename T>
+void z_dispatch(kapsel_ctx *ctx, const FieldPtrs<T> &fp, int nf, bool fwd) {
+  switch (ctx->g.nz) {
+    case 16: launch_z<T, 16>(ctx, fp, nf, fwd); break;
+    case 32: launch_z<T, 32>(ctx, fp, nf, fwd); break;
+    case 64: launch_z<T, 64>(ctx, fp, nf, fwd); break;
+    case 128: launch_z<T, 128>(ctx, fp, nf, fwd); break;
+    case 256: launch_z<T, 256>(ctx, fp, nf, fwd); break;
+    default: set_error("unsupported NZ %d", ctx->g.nz); throw Error{KAPSEL_ERR_ARG};
+  }
+}
+
+template <typename T>
+FieldPtrs<T> pack(void *const *fields, int nf) {
+  FieldPtrs<T> fp{};
+  for (int i = 0; i < nf; i++) fp.p[i] = (T *)fields[i];
+  return fp;
+}
+
+template <typename T>
+void r2c_t(kapsel_ctx *ctx, void *const *fields, int nf) {
+  const Grid &g = ctx->g;
+  FieldPtrs<T> fp = pack<T>(fields, nf);
+  z_dispatch<T>(ctx, fp, nf, true);
+  // y lines: q = i*h + kz -> offset i*NY*h + kz, stride h
+  lines_dispatch<T, -1>(ctx, KAPSEL_K_FFT_Y, g.ny, fp, nf, (long long)g.nx * g.h, g.h, (long long)g.ny * g.h,
+                        g.h, ctx->tw.y);
+  // x lines: q = j*h + kz -> offset q, stride NY*h
+  long long plane = (long long)g.ny * g.h;
+  lines_dispatch<T, -1>(ctx, KAPSEL_K_FFT_X, g.nx, fp, nf, plane, plane, 0, plane, ctx->tw.x);
+}
+
+template <typename T>
+void c2r_t(kapsel_ctx *ctx, void *const *fields, int nf) {
+  const Grid &g = ctx->g;
+  FieldPtrs<T> fp = pack<T>(fields, nf);
+  long long plane = (long long)g.ny * g.h;
+  lines_dispatch<T, +1>(ctx, KAPSEL_K_FFT_X, g.nx, fp, nf, plane, plane, 0, plane, ctx->tw.x);
+  lines_dispatch<T, +1>(ctx, KAPSEL_K_FFT_Y, g.ny, fp, nf, (long long)g.nx * g.h, g.h, (long long)g.ny * g.h,
+                        g.h, ctx->tw.y);
+  z_dispatch<T>(ctx, fp, nf, false);
+}
+
+}  // namespace
+
+void init_twiddles(kapsel_ctx *ctx) {
+  if (ctx->prec == KAPSEL_FP64) init_tw_t<double>(ctx);
+  else init_tw_t<float>(ctx);
+}
+
+void free_twiddles(kapsel_ctx *ctx) {
+  for (void *p : {ctx->tw.x, ctx->tw.y, ctx->tw.zh, ctx->tw.zrot})
+    if (p) (void)hipFree(p);
+  ctx->tw = Twiddles{};
+}
+
+void fft_r2c(kapsel_ctx *ctx, void *const *fields, int nf) {
+  if (ctx->prec == KAPSEL_FP64) r2c_t<double>(ctx, fields, nf);
+  else r2c_t<float>(ctx, fields, nf);
+}
+
+void fft_c2r(kapsel_ctx *ctx, void *const *fields, int nf) {
+  if (ctx->prec == KAPSEL_FP64) c2r_t<double>(ctx, fields, nf);
+  else c2r_t<float>(ctx, fields, nf);
+}
+
+}  // namespace kapsel

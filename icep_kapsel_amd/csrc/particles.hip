@@ -1,0 +1,530 @@
+// particles.hip -- particle <-> grid coupling and particle dynamics.
+//
+// Scatter (Make_phi_particle_sum, Make_u_particle_sum, Make_phi_particle,
+// Make_phi_sin): the reference loops over particles and adds into the grid
+// with `omp atomic` (make_phi.cxx:229-370).  Here each grid point sums the
+// particles near it instead: a 4x4x16 tile of points first compacts, in
+// particle order, the particles whose profile can reach the tile into LDS,
+// then each thread adds their profile values for its own point.  No
+// atomics, every field value is written once (no reset pass), and the sum
+// order is fixed, so results are reproducible run to run.  The set of
+// contributions is the reference's: a point of the Sekibun cell of particle
+// i outside the profile support has phi = 0 exactly, and every point with
+// phi > 0 lies in the cell (|o| <= r + |residue| < a + xi/2 + sqrt(3) dx).
+//
+// Gather (Calc_f_hydro_correct_precision, md_force.cxx:340-449): one
+// workgroup per particle walks the Sekibun table in reference order, FP64
+// compensated (TwoSum) partial sums per lane, then a wavefront reduction.
+#include <cfloat>
+
+#include "common.hpp"
+
+namespace kapsel {
+
+namespace {
+
+struct Phys {
+  double dx, idx, radius, hxi, xi;
+  double l[3];
+  int n[3];
+  int np;
+  int rotation;
+};
+
+__device__ __forceinline__ int nint(double a) { return a >= 0.0 ? int(a + .5) : int(a - .5); }
+
+// H and Phi (profile.h:29-49)
+__device__ __forceinline__ double Hf(double x, double dx) {
+  if (x > 0) {
+    double q = dx / x;
+    return exp(-(q * q));
+  }
+  return 0.0;
+}
+__device__ __forceinline__ double phi_smooth(double x, const Phys &p) {
+  double dmy = Hf(p.radius + p.hxi - x, p.dx);
+  return dmy / (dmy + Hf(x - p.radius + p.hxi, p.dx));
+}
+// Phi_compact_sin (profile.h:139-152)
+__device__ __forceinline__ double phi_compact_sin(double x, const Phys &p) {
+  double dmy_x = p.radius - x;
+  if (fabs(dmy_x) <= p.hxi) return 0.5 * sin(M_PI * dmy_x / p.xi) + 0.5;
+  return dmy_x > p.hxi ? 1. : 0.;
+}
+
+// Distance0 min image (interaction.h:122-135), returns squared norm
+__device__ __forceinline__ double min_image(const double *x1, const double *x2, const Phys &p, double *x12) {
+  double s = 0.0;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    double v = x2[d] - x1[d];
+    v -= (double)nint(v / p.l[d]) * p.l[d];
+    x12[d] = v;
+    s += v * v;
+  }
+  return s;
+}
+
+__device__ __forceinline__ double pval(const double *ps, int f, int d, int n, int np) {
+  return ps[((size_t)f * 3 + d) * np + n];
+}
+__device__ __forceinline__ double &pref(double *ps, int f, int d, int n, int np) {
+  return ps[((size_t)f * 3 + d) * np + n];
+}
+
+constexpr int TX = 4, TY = 4, TZ = 16, TP = TX * TY * TZ;
+
+// MODE 0: phi_sum and phi = min(phi_sum, 1) (Make_phi_particle_sum)
+// MODE 1: unclamped smooth-profile sum (Make_phi_particle)
+// MODE 2: unclamped compact-sine sum (Make_phi_sin)
+// MODE 3: phi*u_p with phi_sum read back (Make_u_particle_sum)
+template <typename T, int MODE>
+__global__ __launch_bounds__(TP) void points_kernel(const double *__restrict__ ps, T *__restrict__ out0,
+                                                    T *__restrict__ out1, T *__restrict__ out2,
+                                                    const T *__restrict__ phi_sum, Grid g, Phys p) {
+  constexpr int NV = MODE == 3 ? 9 : 3;
+  __shared__ double cand[NV][TP];
+  __shared__ int wcount[TP / 64];
+  const int t = threadIdx.x;
+  const int ntx = g.nx / TX, nty = g.ny / TY;
+  int b = blockIdx.x;
+  const int bx = b % ntx;
+  b /= ntx;
+  const int by = b % nty;
+  const int bz = b / nty;
+  const int i = bx * TX + t / (TY * TZ);
+  const int j = by * TY + (t / TZ) % TY;
+  const int k = bz * TZ + t % TZ;
+  const double xg[3] = {i * p.dx, j * p.dx, k * p.dx};
+  // tile box centre and half widths, for the conservative reach test
+  const double cen[3] = {(bx * TX + 0.5 * (TX - 1)) * p.dx, (by * TY + 0.5 * (TY - 1)) * p.dx,
+                         (bz * TZ + 0.5 * (TZ - 1)) * p.dx};
+  const double hw[3] = {0.5 * (TX - 1) * p.dx, 0.5 * (TY - 1) * p.dx, 0.5 * (TZ - 1) * p.dx};
+  const double reach = p.radius + p.hxi + 0.5 * p.dx;
+  const size_t im = ((size_t)i * g.ny + j) * g.nz_ + k;
+  double ps_here = 0.0, psat = 1.0;
+  if constexpr (MODE == 3) {
+    ps_here = (double)phi_sum[im];
+    psat = ps_here > 1.0 ? ps_here : 1.0;
+  }
+  double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+  const double rmax = p.radius + p.hxi;
+  for (int base = 0; base < p.np; base += TP) {
+    const int n = base + t;
+    bool c = false;
+    double xp[3] = {0, 0, 0};
+    if (n < p.np) {
+      double s = 0.0;
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        xp[d] = pval(ps, KAPSEL_P_X, d, n, p.np);
+        double dd = xp[d] - cen[d];
+        dd -= p.l[d] * rint(dd / p.l[d]);
+        dd = fabs(dd) - hw[d];
+        if (dd > 0) s += dd * dd;
+      }
+      c = s < reach * reach;
+    }
+    const unsigned long long m = __ballot(c);
+    const int wid = t / 64, lane = t % 64;
+    if (lane == 0) wcount[wid] = __popcll(m);
+    __syncthreads();
+    int off = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < TP / 64; w++) {
+      if (w < wid) off += wcount[w];
+      total += wcount[w];
+    }
+    if (c) {
+      const int slot = off + __popcll(m & ((1ull << lane) - 1ull));
+#pragma unroll
+      for (int d = 0; d < 3; d++) cand[d][slot] = xp[d];
+      if constexpr (MODE == 3) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          cand[3 + d][slot] = pval(ps, KAPSEL_P_V, d, n, p.np);
+          cand[6 + d][slot] = pval(ps, KAPSEL_P_OMEGA, d, n, p.np);
+        }
+      }
+    }
+    __syncthreads();
+    for (int q = 0; q < total; q++) {
+      const double x2[3] = {cand[0][q], cand[1][q], cand[2][q]};
+      double x12[3];
+      const double r2 = min_image(xg, x2, p, x12);
+      if constexpr (MODE == 2) {
+        acc0 += phi_compact_sin(sqrt(r2), p);
+      } else {
+        if (r2 >= rmax * rmax) continue;  // H(a + xi/2 - r) = 0 there: phi = 0
+        const double ph = phi_smooth(sqrt(r2), p);
+        if constexpr (MODE == 3) {
+          // r: particle centre -> point = -x12 (Relative_coord, make_phi.h:162-179)
+          const double r[3] = {-x12[0], -x12[1], -x12[2]};
+          const double dmy_phi = ph / psat;
+          double vr[3] = {0, 0, 0};
+          if (p.rotation) {
+            vr[0] = cand[7][q] * r[2] - cand[8][q] * r[1];
+            vr[1] = cand[8][q] * r[0] - cand[6][q] * r[2];
+            vr[2] = cand[6][q] * r[1] - cand[7][q] * r[0];
+          }
+          acc0 += (cand[3][q] + vr[0]) * dmy_phi;
+          acc1 += (cand[4][q] + vr[1]) * dmy_phi;
+          acc2 += (cand[5][q] + vr[2]) * dmy_phi;
+        } else {
+          acc0 += ph;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if constexpr (MODE == 0) {
+    out0[im] = (T)acc0;                           // phi_sum
+    out1[im] = (T)(acc0 < 1.0 ? acc0 : 1.0);      // phi = MIN(phi_sum, 1)
+  } else if constexpr (MODE == 3) {
+    out0[im] = (T)acc0;
+    out1[im] = (T)acc1;
+    out2[im] = (T)acc2;
+  } else {
+    out0[im] = (T)acc0;
+  }
+}
+
+// TwoSum compensated accumulation
+struct CSum {
+  double s, c;
+};
+__device__ __forceinline__ void cadd(CSum &a, double x) {
+  double t = a.s + x;
+  double bp = t - a.s;
+  double e = (a.s - (t - bp)) + (x - bp);
+  a.s = t;
+  a.c += e;
+}
+__device__ __forceinline__ CSum cmerge(CSum a, CSum b) {
+  CSum r = a;
+  cadd(r, b.s);
+  r.c += b.c;
+  return r;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void hydro_gather_kernel(double *__restrict__ ps, const int *__restrict__ cells,
+                                                           int np_domain, const T *__restrict__ u0,
+                                                           const T *__restrict__ u1, const T *__restrict__ u2,
+                                                           const T *__restrict__ phi_sum, Grid g, Phys p,
+                                                           double dmy) {
+  const int n = blockIdx.x;
+  const int t = threadIdx.x;
+  double xp[3], vp[3], om[3];
+  int x_int[3];
+  double residue[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    xp[d] = pval(ps, KAPSEL_P_X, d, n, p.np);
+    vp[d] = pval(ps, KAPSEL_P_V, d, n, p.np);
+    om[d] = pval(ps, KAPSEL_P_OMEGA, d, n, p.np);
+    // Particle_cell (make_phi.h:120-148)
+    double q = xp[d] * p.idx;
+    x_int[d] = (int)q;
+    residue[d] = (q - (double)x_int[d]) * p.dx;
+  }
+  CSum acc[6];
+#pragma unroll
+  for (int c = 0; c < 6; c++) acc[c] = CSum{0.0, 0.0};
+  for (int mesh = t; mesh < np_domain; mesh += 256) {
+    int rm[3];
+    double r[3], x[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      const int cell = cells[3 * mesh + d];
+      rm[d] = (x_int[d] + cell + p.n[d]) % p.n[d];
+      r[d] = (double)cell * p.dx - residue[d];
+      x[d] = rm[d] * p.dx;
+    }
+    double x12[3];
+    const double dmyR = sqrt(min_image(x, xp, p, x12));
+    double vr[3] = {0, 0, 0};
+    if (p.rotation) {
+      vr[0] = om[1] * r[2] - om[2] * r[1];
+      vr[1] = om[2] * r[0] - om[0] * r[2];
+      vr[2] = om[0] * r[1] - om[1] * r[0];
+    }
+    const size_t im = ((size_t)rm[0] * g.ny + rm[1]) * g.nz_ + rm[2];
+    const double psum = (double)phi_sum[im];
+    const double dmy_phi = phi_smooth(dmyR, p) / (psum > 1.0 ? psum : 1.0);
+    const double uu[3] = {(double)u0[im], (double)u1[im], (double)u2[im]};
+    double fp[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      fp[d] = ((vp[d] + vr[d]) - uu[d]) * dmy_phi;
+      cadd(acc[d], fp[d]);
+    }
+    cadd(acc[3], r[1] * fp[2] - r[2] * fp[1]);
+    cadd(acc[4], r[2] * fp[0] - r[0] * fp[2]);
+    cadd(acc[5], r[0] * fp[1] - r[1] * fp[0]);
+  }
+  // wavefront reduction, then across the 4 wavefronts
+#pragma unroll
+  for (int c = 0; c < 6; c++) {
+    for (int off = 32; off > 0; off >>= 1) {
+      CSum o{__shfl_down(acc[c].s, off), __shfl_down(acc[c].c, off)};
+      acc[c] = cmerge(acc[c], o);
+    }
+  }
+  __shared__ double red[4][6][2];
+  const int wid = t / 64, lane = t % 64;
+  if (lane == 0) {
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+      red[wid][c][0] = acc[c].s;
+      red[wid][c][1] = acc[c].c;
+    }
+  }
+  __syncthreads();
+  if (t == 0) {
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+      CSum s = CSum{red[0][c][0], red[0][c][1]};
+      for (int w = 1; w < 4; w++) s = cmerge(s, CSum{red[w][c][0], red[w][c][1]});
+      const double v = s.s + s.c;
+      if (c < 3) pref(ps, KAPSEL_P_F_HYDRO, c, n, p.np) = dmy * v;
+      else if (p.rotation) pref(ps, KAPSEL_P_TORQUE_HYDRO, c - 3, n, p.np) = dmy * v;
+    }
+  }
+}
+
+struct LJ {
+  double cutoff, sigma, coeff;
+  int powers;
+  int lc[3];
+  double lc_r;
+  double body[3];
+  int on;
+};
+
+// Lennard_Jones_f (interaction.h:175-200)
+__device__ __forceinline__ double lj_f(double x, const LJ &lj) {
+  double dmy = lj.sigma / x;
+  double d2 = dmy * dmy;
+  double d6 = d2 * d2 * d2;
+  if (lj.powers == 0) dmy = d6;
+  else if (lj.powers == 1) dmy = d6 * d6;
+  else dmy = d6 * d6 * d6;
+  return lj.coeff / (x * x) * (2.0 * dmy * dmy - dmy);
+}
+
+// number of times cell b appears among the three wrapped neighbours of
+// cell a along one axis (md_force.cxx:68-81)
+__device__ __forceinline__ int nb_count(int a, int b, int lc) {
+  int cnt = 0;
+#pragma unroll
+  for (int o = -1; o <= 1; o++) {
+    int v = a + o;
+    if (v < 0) v = lc - 1;
+    if (v > lc - 1) v = 0;
+    cnt += (v == b);
+  }
+  return cnt;
+}
+
+// Force(): the cell-list LJ of md_force.cxx:16-171 (pairs i>j visited once
+// per neighbour-cell match, each contributing +-f(r)(x_i - x_j)) plus the
+// constant body force (Add_f_gravity, md_force.cxx:261-280).  One thread per
+// particle sums every pair it takes part in; no write conflicts.
+__global__ void force_kernel(double *__restrict__ ps, Phys p, LJ lj) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= p.np) return;
+  double fi[3] = {0, 0, 0};
+  if (lj.on) {
+    double xi[3];
+    int ci[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      xi[d] = pval(ps, KAPSEL_P_X, d, i, p.np);
+      ci[d] = int(xi[d] / lj.lc_r);
+    }
+    for (int j = 0; j < p.np; j++) {
+      if (j == i) continue;
+      double xj[3];
+      int mult = 1;
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        xj[d] = pval(ps, KAPSEL_P_X, d, j, p.np);
+        mult *= nb_count(ci[d], int(xj[d] / lj.lc_r), lj.lc[d]);
+      }
+      if (!mult) continue;
+      double x12[3];
+      const double r = sqrt(min_image(xi, xj, p, x12));  // x12 = x_j - x_i
+      if (r < lj.cutoff) {
+        const double f = lj_f(r, lj);
+#pragma unroll
+        for (int d = 0; d < 3; d++) fi[d] += mult * (f * (-x12[d]));
+      }
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; d++) pref(ps, KAPSEL_P_FR, d, i, p.np) += fi[d] + lj.body[d];
+}
+
+// MD_solver_position_Euler/_AB2 (particle_solver.cxx:39-91), PBC (periodic_boundary.h:9-19)
+__global__ void md_position_kernel(double *__restrict__ ps, Phys p, int euler, double dt, double hdt) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= p.np) return;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    double x = pval(ps, KAPSEL_P_X, d, n, p.np);
+    pref(ps, KAPSEL_P_X_PREVIOUS, d, n, p.np) = x;
+    const double v = pval(ps, KAPSEL_P_V, d, n, p.np);
+    const double dx = euler ? dt * v : hdt * (3.0 * v - pval(ps, KAPSEL_P_V_OLD, d, n, p.np));
+    pref(ps, KAPSEL_P_X_NOPBC, d, n, p.np) += dx;
+    x += dx;
+    pref(ps, KAPSEL_P_X, d, n, p.np) = fmod(x + p.l[d], p.l[d]);
+  }
+}
+
+// velocity update of MD_solver_velocity_Euler/_AB2_hydro (particle_solver.cxx:
+// 109-150, 252-295) followed by reset_Forces (particle_solver.cxx:11-37)
+__global__ void md_velocity_kernel(double *__restrict__ ps, Phys p, int euler, double dmy, double dmy_rot) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= p.np) return;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    double &v = pref(ps, KAPSEL_P_V, d, n, p.np);
+    double &om = pref(ps, KAPSEL_P_OMEGA, d, n, p.np);
+    pref(ps, KAPSEL_P_V_OLD, d, n, p.np) = v;
+    pref(ps, KAPSEL_P_OMEGA_OLD, d, n, p.np) = om;
+    const double fh = pval(ps, KAPSEL_P_F_HYDRO, d, n, p.np);
+    const double fr = pval(ps, KAPSEL_P_FR, d, n, p.np);
+    const double th = pval(ps, KAPSEL_P_TORQUE_HYDRO, d, n, p.np);
+    const double tr = pval(ps, KAPSEL_P_TORQUE_R, d, n, p.np);
+    if (euler) {
+      v += dmy * (fh + fr);
+      om += dmy_rot * (th + tr);
+    } else {
+      v += dmy * (2.0 * fh + fr + pval(ps, KAPSEL_P_FR_PREVIOUS, d, n, p.np));
+      om += dmy_rot * (2.0 * th + tr + pval(ps, KAPSEL_P_TORQUE_R_PREVIOUS, d, n, p.np));
+    }
+    pref(ps, KAPSEL_P_FR_PREVIOUS, d, n, p.np) = fr;
+    pref(ps, KAPSEL_P_FR, d, n, p.np) = 0.0;
+    pref(ps, KAPSEL_P_F_HYDRO_PREVIOUS, d, n, p.np) = fh;
+    pref(ps, KAPSEL_P_F_HYDRO, d, n, p.np) = 0.0;
+    pref(ps, KAPSEL_P_TORQUE_R_PREVIOUS, d, n, p.np) = tr;
+    pref(ps, KAPSEL_P_TORQUE_R, d, n, p.np) = 0.0;
+    pref(ps, KAPSEL_P_TORQUE_HYDRO_PREVIOUS, d, n, p.np) = th;
+    pref(ps, KAPSEL_P_TORQUE_HYDRO, d, n, p.np) = 0.0;
+  }
+}
+
+Phys phys(kapsel_ctx *ctx) {
+  Phys p;
+  const Grid &g = ctx->g;
+  p.dx = g.dx;
+  p.idx = 1. / g.dx;
+  p.radius = ctx->P.radius;
+  p.xi = ctx->P.xi;
+  p.hxi = ctx->hxi;
+  p.l[0] = g.lx;
+  p.l[1] = g.ly;
+  p.l[2] = g.lz;
+  p.n[0] = g.nx;
+  p.n[1] = g.ny;
+  p.n[2] = g.nz;
+  p.np = ctx->P.n_particles;
+  p.rotation = ctx->P.rotation;
+  return p;
+}
+
+template <typename T, int MODE>
+void launch_points(kapsel_ctx *ctx, void *o0, void *o1, void *o2, const void *phi_sum) {
+  const Grid &g = ctx->g;
+  unsigned nb = (unsigned)((g.nx / TX) * (g.ny / TY) * (g.nz / TZ));
+  const double fields_moved = MODE == 0 ? 2.0 : MODE == 3 ? 4.0 : 1.0;
+  LaunchTimer lt(ctx, KAPSEL_K_PARTICLE, fields_moved * g.nr * sizeof(T));
+  hipLaunchKernelGGL((points_kernel<T, MODE>), dim3(nb), dim3(TP), 0, ctx->stream, ctx->d_ps, (T *)o0, (T *)o1,
+                     (T *)o2, (const T *)phi_sum, g, phys(ctx));
+  KAPSEL_LAUNCH_CHECK();
+}
+
+template <typename T>
+void gather_t(kapsel_ctx *ctx, double dmy) {
+  LaunchTimer lt(ctx, KAPSEL_K_PARTICLE, 4.0 * ctx->P.n_particles * ctx->np_domain * sizeof(T));
+  hipLaunchKernelGGL(hydro_gather_kernel<T>, dim3(ctx->P.n_particles), dim3(256), 0, ctx->stream, ctx->d_ps,
+                     ctx->d_sekibun, ctx->np_domain, (const T *)ctx->fields[KAPSEL_U0],
+                     (const T *)ctx->fields[KAPSEL_U1], (const T *)ctx->fields[KAPSEL_U2],
+                     (const T *)ctx->fields[KAPSEL_PHI_SUM], ctx->g, phys(ctx), dmy);
+  KAPSEL_LAUNCH_CHECK();
+}
+
+}  // namespace
+
+void md_position(kapsel_ctx *ctx, const kapsel_time &t) {
+  if (ctx->P.n_particles <= 0) return;
+  LaunchTimer lt(ctx, KAPSEL_K_PARTICLE, 8.0 * KAPSEL_P_NFIELDS * 3 * ctx->P.n_particles);
+  hipLaunchKernelGGL(md_position_kernel, dim3(blocks_for(ctx->P.n_particles, 64)), dim3(64), 0, ctx->stream,
+                     ctx->d_ps, phys(ctx), t.ts == 0 ? 1 : 0, t.dt_md, t.hdt_md);
+  KAPSEL_LAUNCH_CHECK();
+}
+
+void make_phi_particle_sum(kapsel_ctx *ctx) {
+  if (ctx->prec == KAPSEL_FP64)
+    launch_points<double, 0>(ctx, ctx->fields[KAPSEL_PHI_SUM], ctx->fields[KAPSEL_PHI], nullptr, nullptr);
+  else
+    launch_points<float, 0>(ctx, ctx->fields[KAPSEL_PHI_SUM], ctx->fields[KAPSEL_PHI], nullptr, nullptr);
+}
+
+void make_u_particle_sum(kapsel_ctx *ctx) {
+  void **f = ctx->fields;
+  if (ctx->prec == KAPSEL_FP64)
+    launch_points<double, 3>(ctx, f[KAPSEL_UP0], f[KAPSEL_UP1], f[KAPSEL_UP2], f[KAPSEL_PHI_SUM]);
+  else
+    launch_points<float, 3>(ctx, f[KAPSEL_UP0], f[KAPSEL_UP1], f[KAPSEL_UP2], f[KAPSEL_PHI_SUM]);
+}
+
+void make_phi_particle(kapsel_ctx *ctx, void *field) {
+  if (ctx->prec == KAPSEL_FP64) launch_points<double, 1>(ctx, field, nullptr, nullptr, nullptr);
+  else launch_points<float, 1>(ctx, field, nullptr, nullptr, nullptr);
+}
+
+void make_phi_sin(kapsel_ctx *ctx, void *field) {
+  if (ctx->prec == KAPSEL_FP64) launch_points<double, 2>(ctx, field, nullptr, nullptr, nullptr);
+  else launch_points<float, 2>(ctx, field, nullptr, nullptr, nullptr);
+}
+
+void calc_f_hydro(kapsel_ctx *ctx, const kapsel_time &t) {
+  if (ctx->P.n_particles <= 0) return;
+  const double dmy = -(ctx->g.dx * ctx->g.dx * ctx->g.dx) * ctx->P.rho / t.dt_fluid;  // md_force.cxx:342-343
+  if (ctx->prec == KAPSEL_FP64) gather_t<double>(ctx, dmy);
+  else gather_t<float>(ctx, dmy);
+}
+
+void force(kapsel_ctx *ctx) {
+  if (ctx->P.n_particles <= 0) return;
+  LJ lj;
+  lj.on = ctx->P.lj_truncate >= 0;
+  lj.cutoff = ctx->a_r_cutoff * ctx->P.lj_sigma;
+  lj.sigma = ctx->P.lj_sigma;
+  lj.powers = ctx->P.lj_powers;
+  lj.coeff = (lj.powers == 0 ? 24. : lj.powers == 1 ? 48. : 72.) * ctx->P.lj_epsilon;
+  const int cell_length = 16;  // md_force.cxx:14
+  lj.lc[0] = ctx->g.nx / cell_length;
+  lj.lc[1] = ctx->g.ny / cell_length;
+  lj.lc[2] = ctx->g.nz / cell_length;
+  lj.lc_r = cell_length * ctx->g.dx;
+  for (int d = 0; d < 3; d++) lj.body[d] = ctx->P.body_force[d];
+  LaunchTimer lt(ctx, KAPSEL_K_PARTICLE, 8.0 * KAPSEL_P_NFIELDS * 3 * ctx->P.n_particles);
+  hipLaunchKernelGGL(force_kernel, dim3(blocks_for(ctx->P.n_particles, 64)), dim3(64), 0, ctx->stream, ctx->d_ps,
+                     phys(ctx), lj);
+  KAPSEL_LAUNCH_CHECK();
+}
+
+void md_velocity_integrate(kapsel_ctx *ctx, const kapsel_time &t) {
+  if (ctx->P.n_particles <= 0) return;
+  const int euler = t.ts == 0;
+  const double dtm = euler ? t.dt_md : t.hdt_md;
+  LaunchTimer lt(ctx, KAPSEL_K_PARTICLE, 8.0 * KAPSEL_P_NFIELDS * 3 * ctx->P.n_particles);
+  hipLaunchKernelGGL(md_velocity_kernel, dim3(blocks_for(ctx->P.n_particles, 64)), dim3(64), 0, ctx->stream,
+                     ctx->d_ps, phys(ctx), euler, dtm * ctx->imass, dtm * ctx->imoi);
+  KAPSEL_LAUNCH_CHECK();
+}
+
+}  // namespace kapsel

@@ -1,0 +1,161 @@
+"""Python front end of the C ABI (ctypes), used by the tests and bench.py.
+
+``KapselGPU`` owns one ``kapsel_ctx``.  Its methods are the C entry points
+with host arrays in the reference's padded Ooura layout; every call raises
+``KapselError`` on a non-zero status.  There is no CPU fallback: without the
+built library or a gfx950 device the constructor raises.
+"""
+import ctypes
+
+import numpy as np
+
+from . import abi
+
+
+class KapselError(RuntimeError):
+    pass
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class KapselGPU:
+    def __init__(self, params):
+        self.lib = abi.load_library()
+        self.params = params
+        ctx = ctypes.c_void_p()
+        rc = self.lib.kapsel_create(ctypes.byref(params), ctypes.byref(ctx))
+        if rc != 0:
+            raise KapselError(f"kapsel_create: {rc}: {self.lib.kapsel_last_error().decode()}")
+        self.ctx = ctx
+        self.nx, self.ny, self.nz = params.nx, params.ny, params.nz
+        self.nz_ = 2 * (self.nz // 2 + 1)
+        self.shape = (self.nx, self.ny, self.nz_)
+        self.np = params.n_particles
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.kapsel_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _call(self, name, *args):
+        rc = getattr(self.lib, "kapsel_" + name)(self.ctx, *args)
+        if rc != 0:
+            raise KapselError(f"kapsel_{name}: {rc}: {self.lib.kapsel_last_error().decode()}")
+
+    # host <-> device ----------------------------------------------------
+    def set_field(self, name, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(self.shape)
+        self._call("set_field", abi.FIELD[name], _dp(a))
+
+    def get_field(self, name):
+        a = np.empty(self.shape, dtype=np.float64)
+        self._call("get_field", abi.FIELD[name], _dp(a))
+        return a
+
+    def set_uk_dc(self, v):
+        a = np.ascontiguousarray(v, dtype=np.float64)
+        self._call("set_uk_dc", _dp(a))
+
+    def get_uk_dc(self):
+        a = np.zeros(3)
+        self._call("get_uk_dc", _dp(a))
+        return a
+
+    def set_particles(self, state):
+        a = np.ascontiguousarray(state, dtype=np.float64).reshape(abi.P_NFIELDS, 3, self.np)
+        self._call("set_particles", _dp(a))
+
+    def get_particles(self):
+        a = np.zeros((abi.P_NFIELDS, 3, self.np))
+        self._call("get_particles", _dp(a))
+        return a
+
+    def sekibun(self):
+        n = ctypes.c_int(0)
+        self._call("get_sekibun", ctypes.byref(n), None)
+        cells = np.zeros((n.value, 3), dtype=np.int32)
+        self._call("get_sekibun", ctypes.byref(n), cells.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        return cells
+
+    def synchronize(self):
+        self._call("synchronize")
+
+    # operations ----------------------------------------------------------
+    def fft_r2c(self, name):
+        self._call("fft_r2c", abi.FIELD[name])
+
+    def fft_c2r(self, name):
+        self._call("fft_c2r", abi.FIELD[name])
+
+    def truncate(self, name):
+        self._call("truncate", abi.FIELD[name])
+
+    def zeta_k2u_k(self):
+        self._call("zeta_k2u_k")
+
+    def u_k2zeta_k(self):
+        self._call("u_k2zeta_k")
+
+    def solenoidal_u(self, name0):
+        self._call("solenoidal_u", abi.FIELD[name0])
+
+    def ns_slaved_euler(self, t):
+        self._call("ns_slaved_euler", ctypes.byref(t))
+
+    def make_phi_particle_sum(self):
+        self._call("make_phi_particle_sum")
+
+    def make_u_particle_sum(self):
+        self._call("make_u_particle_sum")
+
+    def make_phi_particle(self, name):
+        self._call("make_phi_particle", abi.FIELD[name])
+
+    def calc_f_hydro(self, t):
+        self._call("calc_f_hydro", ctypes.byref(t))
+
+    def force(self):
+        self._call("force")
+
+    def md_position(self, t):
+        self._call("md_position", ctypes.byref(t))
+
+    def md_velocity(self, t):
+        self._call("md_velocity", ctypes.byref(t))
+
+    def make_f_particle_dt_sole(self):
+        self._call("make_f_particle_dt_sole")
+
+    def init_rigidity(self):
+        self._call("init_rigidity")
+
+    def spm_step(self, t):
+        self._call("spm_step", ctypes.byref(t))
+
+    # timing ---------------------------------------------------------------
+    def kernel_timing(self, classes):
+        mask = 0
+        for c in classes:
+            mask |= 1 << abi.KCLASS[c]
+        self._call("kernel_timing", mask)
+
+    def kernel_timing_get(self, cls):
+        ms = ctypes.c_double(0.0)
+        n = ctypes.c_int(0)
+        b = ctypes.c_double(0.0)
+        self._call("kernel_timing_get", abi.KCLASS[cls], ctypes.byref(ms), ctypes.byref(n), ctypes.byref(b))
+        return ms.value, n.value, b.value

@@ -1,0 +1,195 @@
+/*
+ * kapsel_gpu.h -- C ABI of the MI355X (gfx950) implementation of the KAPSEL
+ * smoothed-profile-method (SPM) step.
+ *
+ * Conventions (all entry points):
+ *   - every function returns an int status: KAPSEL_OK (0) or a negative
+ *     KAPSEL_ERR_* code; kapsel_last_error() returns the message of the last
+ *     failure on the calling thread.  Nothing calls exit().
+ *   - a context owns its device memory and one HIP stream (default priority);
+ *     it is not thread-safe.  Host buffers stay owned by the caller.
+ *   - host field buffers are always double precision and use the reference's
+ *     padded Ooura layout a[(i*NY + j)*NZ_ + k], NZ_ = 2*(NZ/2+1): real space
+ *     uses k < NZ, reciprocal space stores complex (re, im) pairs at
+ *     k = 2*kz, 2*kz+1 for kz = 0..NZ/2 (fft_wrapper.h:681-723,
+ *     fft_wrapper.cxx:47-87).  Device fields are double (KAPSEL_FP64) or float
+ *     (KAPSEL_FP32) in the same layout.
+ *   - particle state is structure-of-arrays doubles:
+ *     state[(f*3 + d)*n_particles + n] for field f (enum kapsel_particle_field),
+ *     component d, particle n.  It is the reference's Particle AoS
+ *     (variable.h:27-77) restricted to what the free-sphere step touches.
+ *
+ * Every entry point cites the reference function it replaces.
+ */
+#ifndef KAPSEL_GPU_H
+#define KAPSEL_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KAPSEL_OK 0
+#define KAPSEL_ERR_ARG (-1)      /* invalid argument */
+#define KAPSEL_ERR_HIP (-2)      /* HIP runtime failure */
+#define KAPSEL_ERR_STATE (-3)    /* call not valid in the current state */
+#define KAPSEL_ERR_NOCONV (-4)   /* iterative solve did not reach the tolerance */
+#define KAPSEL_ERR_NODEV (-5)    /* no usable gfx950 device */
+
+enum kapsel_precision {
+  KAPSEL_FP64 = 0, /* every field and transform in double */
+  KAPSEL_FP32 = 1  /* fields/transforms in float; particle reductions and
+                      particle update stay double */
+};
+
+/* Device-resident fields (host copies via kapsel_set_field/kapsel_get_field). */
+enum kapsel_field {
+  KAPSEL_ZETA0 = 0, KAPSEL_ZETA1,          /* reduced vorticity (k-space) */
+  KAPSEL_U0, KAPSEL_U1, KAPSEL_U2,         /* fluid velocity */
+  KAPSEL_UP0, KAPSEL_UP1, KAPSEL_UP2,      /* particle velocity field phi*u_p */
+  KAPSEL_F0, KAPSEL_F1, KAPSEL_F2,         /* constraint force f_particle */
+  KAPSEL_PHI, KAPSEL_PHI_SUM,              /* clamped / unclamped profile */
+  KAPSEL_EPSILON, KAPSEL_POTENTIAL,        /* permittivity, potential */
+  KAPSEL_RHO_FREE, KAPSEL_RHS,             /* free charge, Poisson RHS */
+  KAPSEL_WORK0, KAPSEL_WORK1, KAPSEL_WORK2, KAPSEL_WORK3, KAPSEL_WORK4,
+  KAPSEL_NFIELDS
+};
+
+enum kapsel_particle_field {
+  KAPSEL_P_X = 0, KAPSEL_P_X_PREVIOUS, KAPSEL_P_X_NOPBC,
+  KAPSEL_P_V, KAPSEL_P_V_OLD, KAPSEL_P_OMEGA, KAPSEL_P_OMEGA_OLD,
+  KAPSEL_P_F_HYDRO, KAPSEL_P_F_HYDRO_PREVIOUS,
+  KAPSEL_P_FR, KAPSEL_P_FR_PREVIOUS,
+  KAPSEL_P_TORQUE_HYDRO, KAPSEL_P_TORQUE_HYDRO_PREVIOUS,
+  KAPSEL_P_TORQUE_R, KAPSEL_P_TORQUE_R_PREVIOUS,
+  KAPSEL_P_NFIELDS
+};
+
+/* Preconditioner choice for kapsel_poisson_solve. */
+enum kapsel_precond {
+  KAPSEL_PRECOND_NONE = 0,     /* plain CG (identity preconditioner) */
+  KAPSEL_PRECOND_SPECTRAL = 1  /* inverse of the mean-permittivity spectral Laplacian */
+};
+
+/* Values of the reference's globals that the hot path reads (input.h). */
+typedef struct kapsel_params {
+  int nx, ny, nz;            /* NX, NY, NZ: powers of two, 16..512 */
+  double dx;                 /* DX */
+  double rho, eta;           /* RHO, ETA (NU = ETA/RHO) */
+  int n_particles;           /* Particle_Number (one species) */
+  double radius, xi;         /* RADIUS = A*DX, XI = A_XI*DX */
+  double mass_ratio;         /* MASS_RATIOS[0] */
+  int rotation;              /* ROTATION: 1 binds Angular2v_rot_on */
+  int lj_truncate;           /* LJ_truncate: 1 cut at the minimum, 0 at 2.5 sigma, -1 no LJ */
+  int lj_powers;             /* LJ_powers: 0 = 12:6, 1 = 24:12, 2 = 36:18 */
+  double lj_epsilon;         /* EPSILON */
+  double lj_sigma;           /* LJ_dia (SIGMA = 2 RADIUS unless sheared) */
+  double body_force[3];      /* constant force added to fr by Force(); the
+                                reference's gravity is body_force[G_direction]
+                                = -G*RHO*(4/3)pi a^3 (MASS_RATIO-1) */
+  int fix_cell[3];           /* FIX_CELLxyz: zero uk_dc[d] each step */
+  int precision;             /* enum kapsel_precision */
+  double eps_fluid;          /* eps_fluid */
+  double eps_particle;       /* eps_particle_top (non-Janus) */
+  double dielectric_cst;     /* Dielectric_cst */
+} kapsel_params;
+
+/* The reference's CTime (variable.h:18-25). */
+typedef struct kapsel_time {
+  int ts;
+  double time;
+  double dt_fluid, hdt_fluid, dt_md, hdt_md;
+} kapsel_time;
+
+typedef struct kapsel_ctx kapsel_ctx;
+
+/* ---- lifetime ---------------------------------------------------------- */
+/* Replaces Init_fft (fft_wrapper.cxx:166-199), Mem_alloc_var
+ * (sp_3d_ns.cxx:295-340) and Particle_domain (profile.cxx:25-74). */
+int kapsel_create(const kapsel_params *params, kapsel_ctx **out);
+int kapsel_destroy(kapsel_ctx *ctx);
+const char *kapsel_last_error(void);
+/* Blocks until all work queued on the context's stream has finished. */
+int kapsel_synchronize(kapsel_ctx *ctx);
+/* Sekibun cell (profile.cxx:25-74): count, and the offsets (3 ints each,
+ * reference order) when cells != NULL. */
+int kapsel_get_sekibun(kapsel_ctx *ctx, int *np_domain, int *cells);
+
+/* ---- host <-> device --------------------------------------------------- */
+int kapsel_set_field(kapsel_ctx *ctx, int field, const double *host);
+int kapsel_get_field(kapsel_ctx *ctx, int field, double *host);
+int kapsel_set_uk_dc(kapsel_ctx *ctx, const double uk_dc[3]);
+int kapsel_get_uk_dc(kapsel_ctx *ctx, double uk_dc[3]);
+int kapsel_set_particles(kapsel_ctx *ctx, const double *state);
+int kapsel_get_particles(kapsel_ctx *ctx, double *state);
+
+/* ---- FFT and k-space algebra ------------------------------------------- */
+/* A2a_k (fft_wrapper.h:681-701): in-place forward r2c, e^{-ik.x}, unscaled. */
+int kapsel_fft_r2c(kapsel_ctx *ctx, int field);
+/* A_k2a (fft_wrapper.h:704-723): in-place inverse c2r, scaled by 1/(NX NY NZ). */
+int kapsel_fft_c2r(kapsel_ctx *ctx, int field);
+/* Truncate_two_third_rule_ooura (fft_wrapper.h:797-839). */
+int kapsel_truncate(kapsel_ctx *ctx, int field);
+/* Zeta_k2u_k (fft_wrapper.cxx:614-711): zeta, uk_dc -> u_k in U0..U2. */
+int kapsel_zeta_k2u_k(kapsel_ctx *ctx);
+/* U_k2zeta_k (fft_wrapper.cxx:428-497): u_k in U0..U2 -> zeta, uk_dc. */
+int kapsel_u_k2zeta_k(kapsel_ctx *ctx);
+/* Solenoidal_u (operate_omega.h:109-116) on the real vector field at
+ * field0..field0+2. */
+int kapsel_solenoidal_u(kapsel_ctx *ctx, int field0);
+/* NS_solver_slavedEuler (fluid_solver.cxx:340-366): advances zeta in place. */
+int kapsel_ns_slaved_euler(kapsel_ctx *ctx, const kapsel_time *jikan);
+
+/* ---- particle <-> grid --------------------------------------------------- */
+/* Make_phi_particle_sum (make_phi.cxx:229-275, 519-523): PHI_SUM, PHI. */
+int kapsel_make_phi_particle_sum(kapsel_ctx *ctx);
+/* Make_u_particle_sum (make_phi.cxx:329-370, 532-536): UP0..UP2 from PHI_SUM. */
+int kapsel_make_u_particle_sum(kapsel_ctx *ctx);
+/* Make_phi_particle (make_phi.cxx:146-227, 506-518 with SW_UP = 0): the
+ * unclamped profile sum into `field`. */
+int kapsel_make_phi_particle(kapsel_ctx *ctx, int field);
+/* Calc_f_hydro_correct_precision (md_force.cxx:340-449): f_hydro and,
+ * with rotation, torque_hydro from U0..U2 and PHI_SUM. */
+int kapsel_calc_f_hydro(kapsel_ctx *ctx, const kapsel_time *jikan);
+/* Calc_f_Lennard_Jones_shear_cap_primitive_lnk (md_force.cxx:16-171) plus
+ * the body force of Add_f_gravity (md_force.cxx:261-280): Force()
+ * (particle_solver.h:92-104), accumulated into fr. */
+int kapsel_force(kapsel_ctx *ctx);
+/* MD_solver_position_Euler/_AB2 (particle_solver.cxx:39-91), by jikan->ts. */
+int kapsel_md_position(kapsel_ctx *ctx, const kapsel_time *jikan);
+/* MD_solver_velocity_Euler/_AB2_hydro (particle_solver.cxx:109-150,
+ * 252-295) including Force() and reset_Forces(). */
+int kapsel_md_velocity(kapsel_ctx *ctx, const kapsel_time *jikan);
+/* Make_f_particle_dt_sole + Add_f_particle (f_particle.h:37-95):
+ * F = UP - PHI*U, projected, then U += F. */
+int kapsel_make_f_particle_dt_sole(kapsel_ctx *ctx);
+
+/* ---- whole step ---------------------------------------------------------- */
+/* Initial rigidity projection of main (sp_3d_ns.cxx:431-447). */
+int kapsel_init_rigidity(kapsel_ctx *ctx);
+/* Time_evolution_hydro, Navier-Stokes branch (sp_3d_ns.cxx:24-186), then
+ * jikan->time += dt_fluid and jikan->ts += 1 (sp_3d_ns.cxx:524-525). */
+int kapsel_spm_step(kapsel_ctx *ctx, kapsel_time *jikan);
+
+/* ---- kernel timing (HIP events on the context stream) --------------------- */
+enum kapsel_kernel_class {
+  KAPSEL_K_FFT_Z = 0,   /* z-direction r2c/c2r pass */
+  KAPSEL_K_FFT_Y,       /* y-direction c2c pass */
+  KAPSEL_K_FFT_X,       /* x-direction c2c pass */
+  KAPSEL_K_POINTWISE,   /* k-space and real-space pointwise kernels */
+  KAPSEL_K_PARTICLE,    /* scatter, gather, LJ, particle update */
+  KAPSEL_K_NCLASSES
+};
+/* Records a HIP event pair around every launch of the kernel classes whose
+ * bit is set in `mask` (0 disables); kapsel_kernel_timing_get returns the
+ * accumulated milliseconds, launch count and algorithmic bytes (each input
+ * read once, each output written once) of one class and resets them.
+ * No reference counterpart: the reference only prints wall-clock block
+ * times (sp_3d_ns.cxx:506-518). */
+int kapsel_kernel_timing(kapsel_ctx *ctx, int mask);
+int kapsel_kernel_timing_get(kapsel_ctx *ctx, int kernel_class, double *total_ms, int *launches,
+                             double *total_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KAPSEL_GPU_H */

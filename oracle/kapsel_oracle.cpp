@@ -1,0 +1,998 @@
+/*
+ * kapsel_oracle.cpp -- CPU FP64 restatement of the KAPSEL SPM hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity checker for the HIP
+ * library (tests/, __graft_entry__.smoke()) and the CPU baseline that
+ * bench.py times beside it.  Nothing in the product links or calls it.
+ *
+ * It restates the reference loops one for one, in the reference's operation
+ * order, FP64 throughout, OpenMP over the same loop nests (scatter with
+ * `omp atomic` like make_phi.cxx:258).  The only part not taken from the
+ * reference is the FFT: the reference calls FFTW/Ooura, this file carries
+ * its own radix-2 transform with the same convention (forward e^{-ik.x}
+ * unscaled, inverse scaled by 1/(NX NY NZ), fft_wrapper.h:681-723), checked
+ * in tests against the reference's own Ooura rdft3d built from source, and
+ * against numpy.fft.
+ *
+ * Pinning of each function (tests/test_oracle_*.py):
+ *   fft_r2c/fft_c2r       Ooura rdft3d+rdft3dsort (oracle/_ref), numpy.fft, DFT closed forms
+ *   truncate              index-set check against |k| >= TRN (closed form)
+ *   zeta_k2u_k/u_k2zeta_k round trip on solenoidal truncated fields, curl/div identities
+ *   u2advection_k         independent curl(u x omega) in numpy; Beltrami (ABC) flow gives 0
+ *   ns_slaved_euler       shear wave decays by exp(-nu k^2 dt) exactly
+ *   solenoidal_u          k.u_k = 0, idempotence, gradient fields removed
+ *   phi / sekibun         closed-form profile values, lattice-point count, brute force O(Np N^3)
+ *   scatter / gather      brute force numpy sums; rigid-motion fluid gives zero force/torque
+ *   LJ                    all-pairs numpy sum (the reference's own brute-force form,
+ *                         md_force.cxx:173-259)
+ *   particle update       closed-form Euler/AB2 increments
+ *   dielectric operator   constant eps: symbol -eps|k|^2; symmetry; residual of the solve
+ */
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../include/kapsel_gpu.h"
+
+typedef std::complex<double> cplx;
+
+namespace {
+
+inline double SQ(double x) { return x * x; }
+
+/* Nint (macro.h:43-52) */
+inline int Nint(double a) { return a >= 0.0 ? int(a + .5) : int(a - .5); }
+
+/* ------------------------------------------------------------------------ */
+/* 1-D radix-2 complex FFT on a contiguous line (own code, not the reference) */
+struct Fft1d {
+  int n = 0;
+  std::vector<cplx> tw;  /* exp(-2 pi i k / n), k < n/2 */
+  std::vector<int> rev;
+  void init(int n_) {
+    n = n_;
+    tw.resize(std::max(1, n / 2));
+    for (int k = 0; k < n / 2; k++) {
+      long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)n;
+      tw[k] = cplx((double)cosl(a), (double)sinl(a));
+    }
+    rev.resize(n);
+    int lg = 0;
+    while ((1 << lg) < n) lg++;
+    for (int i = 0; i < n; i++) {
+      int r = 0;
+      for (int b = 0; b < lg; b++)
+        if (i & (1 << b)) r |= 1 << (lg - 1 - b);
+      rev[i] = r;
+    }
+  }
+  /* sign = -1: forward e^{-i}; sign = +1: inverse e^{+i}, unscaled */
+  void run(cplx *a, int sign) const {
+    for (int i = 0; i < n; i++)
+      if (i < rev[i]) std::swap(a[i], a[rev[i]]);
+    for (int len = 2; len <= n; len <<= 1) {
+      int half = len >> 1, step = n / len;
+      for (int s = 0; s < n; s += len) {
+        for (int k = 0; k < half; k++) {
+          cplx w = tw[k * step];
+          if (sign > 0) w = std::conj(w);
+          cplx u = a[s + k], v = a[s + k + half] * w;
+          a[s + k] = u + v;
+          a[s + k + half] = u - v;
+        }
+      }
+    }
+  }
+};
+
+struct Oracle {
+  kapsel_params P;
+  int NX, NY, NZ, NZ_, HNZ_, HNX, HNY;
+  int TRN_X, TRN_Y, TRN_Z;
+  double DX, DX3, WAVE_X, WAVE_Y, WAVE_Z, NU, IRHO, HXI, RADIUS, XI;
+  double L[3];
+  double MASS, IMASS, MOI, IMOI;
+  double A_R_cutoff;
+  size_t F;
+  Fft1d fx, fy, fzh;        /* lengths NX, NY, NZ/2 */
+  std::vector<cplx> zrot;   /* exp(-2 pi i k / NZ), k <= NZ/2 */
+  std::vector<double *> fields;
+  double uk_dc[3];
+  int Np;
+  std::vector<double> ps;   /* particle state, SoA (kapsel_gpu.h) */
+  int NP_domain;
+  std::vector<int> sekibun; /* 3 ints per cell point */
+
+  double *fld(int f) { return fields[f]; }
+  double *pf(int f, int d) { return ps.data() + ((size_t)f * 3 + d) * Np; }
+
+  /* k maps, Calc_K*_Ooura (fft_wrapper.h:761-773); k index is the double index */
+  inline int KX(int i) const { return i > HNX ? i - NX : i; }
+  inline int KY(int j) const { return j > HNY ? j - NY : j; }
+  inline int KZ(int k) const { return k / 2; }
+  inline double IK2(int i, int j, int k) const {
+    double k2 = SQ(WAVE_X * KX(i)) + SQ(WAVE_Y * KY(j)) + SQ(WAVE_Z * KZ(k));
+    return k2 > 0.0 ? 1. / k2 : 0.0;
+  }
+  inline double K2(int i, int j, int k) const {
+    return SQ(WAVE_X * KX(i)) + SQ(WAVE_Y * KY(j)) + SQ(WAVE_Z * KZ(k));
+  }
+
+  ~Oracle() {
+    for (double *p : fields) free(p);
+  }
+
+  /* ---------------------------------------------------------------- setup */
+  int init(const kapsel_params *p) {
+    P = *p;
+    NX = P.nx; NY = P.ny; NZ = P.nz;
+    HNZ_ = NZ / 2 + 1; NZ_ = 2 * HNZ_;   /* input.cxx:305-306 */
+    HNX = NX / 2; HNY = NY / 2;
+    TRN_X = (NX + 2) / 3; TRN_Y = (NY + 2) / 3; TRN_Z = (NZ + 2) / 3; /* input.cxx:310 */
+    DX = P.dx; DX3 = DX * DX * DX;
+    L[0] = NX * DX; L[1] = NY * DX; L[2] = NZ * DX;
+    WAVE_X = 2 * M_PI / L[0]; WAVE_Y = 2 * M_PI / L[1]; WAVE_Z = 2 * M_PI / L[2];
+    IRHO = 1. / P.rho; NU = P.eta * IRHO;
+    RADIUS = P.radius; XI = P.xi; HXI = XI * 0.5;
+    /* Set_global_parameters (input.cxx:287-296) */
+    MASS = P.mass_ratio * P.rho * 4. / 3. * M_PI * RADIUS * RADIUS * RADIUS;
+    IMASS = 1. / MASS;
+    MOI = 2. / 5. * MASS * SQ(RADIUS);
+    IMOI = 1. / MOI;
+    /* A_R_cutoff (input.cxx:1530-1547) */
+    if (P.lj_truncate > 0) {
+      A_R_cutoff = P.lj_powers == 0 ? pow(2., 1. / 6.) : P.lj_powers == 1 ? pow(2., 1. / 12.) : pow(2., 1. / 18.);
+    } else if (P.lj_truncate == 0) {
+      int Nmin = std::min(NX, std::min(NY, NZ));
+      A_R_cutoff = std::min(Nmin * DX * .5 / P.lj_sigma, 2.5);
+    } else {
+      A_R_cutoff = 0.;
+    }
+    F = (size_t)NX * NY * NZ_;
+    fx.init(NX); fy.init(NY); fzh.init(NZ / 2);
+    zrot.resize(NZ / 2 + 1);
+    for (int k = 0; k <= NZ / 2; k++) {
+      long double a = -2.0L * 3.14159265358979323846264338327950288L * k / (long double)NZ;
+      zrot[k] = cplx((double)cosl(a), (double)sinl(a));
+    }
+    fields.resize(KAPSEL_NFIELDS);
+    for (int f = 0; f < KAPSEL_NFIELDS; f++) {
+      fields[f] = (double *)aligned_alloc(128, ((F * sizeof(double) + 127) / 128) * 128);
+      memset(fields[f], 0, F * sizeof(double));
+    }
+    uk_dc[0] = uk_dc[1] = uk_dc[2] = 0;
+    Np = P.n_particles;
+    ps.assign((size_t)KAPSEL_P_NFIELDS * 3 * std::max(Np, 1), 0.0);
+    particle_domain();
+    return 0;
+  }
+
+  /* Particle_domain (profile.cxx:25-74) */
+  void particle_domain() {
+    /* A, A_XI are the UDF values with RADIUS = A*DX, XI = A_XI*DX (input.cxx:1396-1403) */
+    int A = (int)(RADIUS / DX + 1e-9), A_XI = (int)(XI / DX + 1e-9);
+    int nh = A + A_XI + 2;
+    double MAX_RADIUS = RADIUS + HXI + 1.7321 * DX;
+    sekibun.clear();
+    for (int i = -nh; i <= nh; i++)
+      for (int j = -nh; j <= nh; j++)
+        for (int k = -nh; k <= nh; k++) {
+          double x0 = (double)i * DX, x1 = (double)j * DX, x2 = (double)k * DX;
+          double dmy = sqrt(SQ(x0) + SQ(x1) + SQ(x2));
+          if (dmy < MAX_RADIUS) {
+            sekibun.push_back(i); sekibun.push_back(j); sekibun.push_back(k);
+          }
+        }
+    NP_domain = (int)sekibun.size() / 3;
+  }
+
+  /* ------------------------------------------------------------------ FFT */
+  /* A2a_k: in-place r2c, forward e^{-i}, unscaled (fft_wrapper.h:681-701) */
+  void fft_r2c(double *a) {
+    const int M = NZ / 2;
+    /* z: real rows -> NZ/2+1 complex, half-length complex FFT + split */
+#pragma omp parallel
+    {
+      std::vector<cplx> z(M), X(M + 1);
+#pragma omp for
+      for (int ij = 0; ij < NX * NY; ij++) {
+        double *row = a + (size_t)ij * NZ_;
+        for (int n = 0; n < M; n++) z[n] = cplx(row[2 * n], row[2 * n + 1]);
+        fzh.run(z.data(), -1);
+        /* X[k] = E[k] + W^k O[k]; E = (Z[k]+conj Z[M-k])/2, O = (Z[k]-conj Z[M-k])/(2i) */
+        for (int k = 0; k <= M; k++) {
+          cplx zk = z[k % M], zm = std::conj(z[(M - k) % M]);
+          cplx E = 0.5 * (zk + zm);
+          cplx O = cplx(0, -0.5) * (zk - zm);
+          X[k] = E + zrot[k] * O;
+        }
+        for (int k = 0; k <= M; k++) { row[2 * k] = X[k].real(); row[2 * k + 1] = X[k].imag(); }
+      }
+    }
+    fft_yx(a, -1);
+  }
+
+  /* complex passes along y then x (forward) or x then y (inverse) */
+  void fft_yx(double *a, int sign) {
+    const int H = HNZ_;
+    cplx *c = reinterpret_cast<cplx *>(a);
+    auto ypass = [&]() {
+#pragma omp parallel
+      {
+        std::vector<cplx> line(NY);
+#pragma omp for
+        for (int ik = 0; ik < NX * H; ik++) {
+          int i = ik / H, kz = ik % H;
+          cplx *base = c + (size_t)i * NY * H + kz;
+          for (int j = 0; j < NY; j++) line[j] = base[(size_t)j * H];
+          fy.run(line.data(), sign);
+          for (int j = 0; j < NY; j++) base[(size_t)j * H] = line[j];
+        }
+      }
+    };
+    auto xpass = [&]() {
+#pragma omp parallel
+      {
+        std::vector<cplx> line(NX);
+#pragma omp for
+        for (int jk = 0; jk < NY * H; jk++) {
+          cplx *base = c + jk;
+          for (int i = 0; i < NX; i++) line[i] = base[(size_t)i * NY * H];
+          fx.run(line.data(), sign);
+          for (int i = 0; i < NX; i++) base[(size_t)i * NY * H] = line[i];
+        }
+      }
+    };
+    if (sign < 0) { ypass(); xpass(); }
+    else { xpass(); ypass(); }
+  }
+
+  /* A_k2a: in-place c2r, scaled by 1/(NX NY NZ) (fft_wrapper.h:704-723).
+   * The imaginary parts of the kz = 0 and kz = NZ/2 entries are ignored
+   * after the x and y passes (FFTW's c2r treats them as real). */
+  void fft_c2r(double *a) {
+    fft_yx(a, +1);
+    const int M = NZ / 2;
+    const double scale = 1.0 / ((double)NX * NY * NZ);
+#pragma omp parallel
+    {
+      std::vector<cplx> z(M), X(M + 1);
+#pragma omp for
+      for (int ij = 0; ij < NX * NY; ij++) {
+        double *row = a + (size_t)ij * NZ_;
+        for (int k = 0; k <= M; k++) X[k] = cplx(row[2 * k], row[2 * k + 1]);
+        X[0] = cplx(X[0].real(), 0.0);
+        X[M] = cplx(X[M].real(), 0.0);
+        for (int k = 0; k < M; k++) {
+          cplx xk = X[k], xm = std::conj(X[M - k]);
+          cplx E = xk + xm;                          /* 2 E[k] */
+          cplx O = (xk - xm) * std::conj(zrot[k]);   /* 2 O[k] */
+          z[k] = E + cplx(0, 1) * O;
+        }
+        fzh.run(z.data(), +1);
+        for (int n = 0; n < M; n++) {
+          row[2 * n] = z[n].real() * scale;
+          row[2 * n + 1] = z[n].imag() * scale;
+        }
+        row[NZ] = 0.0; row[NZ + 1] = 0.0;
+      }
+    }
+  }
+
+  /* ------------------------------------------------------------ k-space */
+  /* Truncate_two_third_rule_ooura (fft_wrapper.h:797-839): zero every mode
+   * with |kx| >= TRN_X or |ky| >= TRN_Y or kz >= TRN_Z */
+  void truncate(double *a) {
+    const int trn_z2 = 2 * TRN_Z;
+#pragma omp parallel for
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++)
+        for (int k = 0; k < NZ_; k++) {
+          bool cut = (k >= trn_z2) || (j >= TRN_Y && j <= NY - TRN_Y) ||
+                     (i >= TRN_X && i <= NX - TRN_X);
+          if (cut) a[((size_t)i * NY + j) * NZ_ + k] = 0.0;
+        }
+  }
+
+  /* Zeta_k2u_k (fft_wrapper.cxx:614-711) */
+  void zeta_k2u_k(double **zeta, const double dc[3], double **u) {
+#pragma omp parallel for
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++)
+        for (int k = 0; k < HNZ_; k++) {
+          int k2 = 2 * k;
+          size_t im = ((size_t)i * NY + j) * NZ_ + k2;
+          double kx = WAVE_X * KX(i), ky = WAVE_Y * KY(j), kz = WAVE_Z * KZ(k2);
+          double ik2 = IK2(i, j, k2);
+          double d1r = zeta[0][im], d1i = zeta[0][im + 1];
+          double d2r = zeta[1][im], d2i = zeta[1][im + 1];
+          double orr[3], oi[3];
+          if (KX(i) != 0) {
+            orr[0] = -(1. / kx) * (ky * d1r + kz * d2r);
+            oi[0] = -(1. / kx) * (ky * d1i + kz * d2i);
+            orr[1] = d1r; oi[1] = d1i;
+            orr[2] = d2r; oi[2] = d2i;
+          } else if (KY(j) != 0) {
+            double dmy = -(kz / ky);
+            orr[0] = d2r; oi[0] = d2i;
+            orr[1] = dmy * d1r; oi[1] = dmy * d1i;
+            orr[2] = d1r; oi[2] = d1i;
+          } else {
+            orr[0] = d1r; oi[0] = d1i;
+            orr[1] = d2r; oi[1] = d2i;
+            orr[2] = 0.0; oi[2] = 0.0;
+          }
+          kx *= ik2; ky *= ik2; kz *= ik2;
+          u[0][im] = -(ky * oi[2] - kz * oi[1]);
+          u[0][im + 1] = (ky * orr[2] - kz * orr[1]);
+          u[1][im] = -(kz * oi[0] - kx * oi[2]);
+          u[1][im + 1] = (kz * orr[0] - kx * orr[2]);
+          u[2][im] = -(kx * oi[1] - ky * oi[0]);
+          u[2][im + 1] = (kx * orr[1] - ky * orr[0]);
+        }
+    u[0][0] = dc[0]; u[1][0] = dc[1]; u[2][0] = dc[2];
+  }
+
+  /* U_k2zeta_k (fft_wrapper.cxx:428-497) */
+  void u_k2zeta_k(double **u, double **zeta, double dc[3]) {
+    dc[0] = u[0][0]; dc[1] = u[1][0]; dc[2] = u[2][0];
+#pragma omp parallel for
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++)
+        for (int k = 0; k < HNZ_; k++) {
+          int k2 = 2 * k;
+          size_t im = ((size_t)i * NY + j) * NZ_ + k2;
+          double ks0 = KX(i) * WAVE_X, ks1 = KY(j) * WAVE_Y, ks2 = KZ(k2) * WAVE_Z;
+          double orr[3], oi[3];
+          orr[0] = -(ks1 * u[2][im + 1] - ks2 * u[1][im + 1]);
+          oi[0] = (ks1 * u[2][im] - ks2 * u[1][im]);
+          orr[1] = -(ks2 * u[0][im + 1] - ks0 * u[2][im + 1]);
+          oi[1] = (ks2 * u[0][im] - ks0 * u[2][im]);
+          orr[2] = -(ks0 * u[1][im + 1] - ks1 * u[0][im + 1]);
+          oi[2] = (ks0 * u[1][im] - ks1 * u[0][im]);
+          int a0, a1;
+          if (KX(i) != 0) { a0 = 1; a1 = 2; }
+          else if (KY(j) != 0) { a0 = 2; a1 = 0; }
+          else { a0 = 0; a1 = 1; }
+          zeta[0][im] = orr[a0]; zeta[0][im + 1] = oi[a0];
+          zeta[1][im] = orr[a1]; zeta[1][im + 1] = oi[a1];
+        }
+  }
+
+  /* Omega_k2zeta_k (fft_wrapper.cxx:248-284) */
+  void omega_k2zeta_k(double **omega, double **zetak) {
+#pragma omp parallel for
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++)
+        for (int k = 0; k < NZ_; k++) {
+          size_t im = ((size_t)i * NY + j) * NZ_ + k;
+          if (KX(i) != 0) { zetak[0][im] = omega[1][im]; zetak[1][im] = omega[2][im]; }
+          else if (KY(j) != 0) { zetak[0][im] = omega[2][im]; zetak[1][im] = omega[0][im]; }
+          else { zetak[0][im] = omega[0][im]; zetak[1][im] = omega[1][im]; }
+        }
+  }
+
+  /* U2advection_k (operate_omega.cxx:12-98) */
+  void u2advection_k(double **u, double **adv) {
+#pragma omp parallel for
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++)
+        for (int k = 0; k < NZ; k++) {
+          size_t im = ((size_t)i * NY + j) * NZ_ + k;
+          double u1 = u[0][im], u2 = u[1][im], u3 = u[2][im];
+          u[0][im] = u2 * u3;
+          u[1][im] = u3 * u1;
+          u[2][im] = u1 * u2;
+          adv[0][im] = (u2 - u3) * (u2 + u3);
+          adv[1][im] = (u3 - u1) * (u3 + u1);
+        }
+    for (int d = 0; d < 3; d++) fft_r2c(u[d]);
+    fft_r2c(adv[0]);
+    fft_r2c(adv[1]);
+#pragma omp parallel for
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++)
+        for (int k = 0; k < NZ_; k++) {
+          size_t im = ((size_t)i * NY + j) * NZ_ + k;
+          double k1 = KX(i) * WAVE_X, k2 = KY(j) * WAVE_Y, k3 = KZ(k) * WAVE_Z;
+          double u2u3 = u[0][im], u3u1 = u[1][im], u1u2 = u[2][im];
+          double u22_u32 = adv[0][im], u32_u12 = adv[1][im];
+          double u12_u22 = -u32_u12 - u22_u32;
+          double k1k2 = k1 * k2, k2k3 = k2 * k3, k3k1 = k3 * k1;
+          u[0][im] = -(k2k3 * u22_u32 + (k3 - k2) * (k3 + k2) * u2u3 + k3k1 * u1u2 - k1k2 * u3u1);
+          u[1][im] = -(k3k1 * u32_u12 + (k1 - k3) * (k1 + k3) * u3u1 + k1k2 * u2u3 - k2k3 * u1u2);
+          u[2][im] = -(k1k2 * u12_u22 + (k2 - k1) * (k2 + k1) * u1u2 + k2k3 * u3u1 - k3k1 * u2u3);
+        }
+    omega_k2zeta_k(u, adv);
+  }
+
+  double *U(int d) { return fld(KAPSEL_U0 + d); }
+  double *UP(int d) { return fld(KAPSEL_UP0 + d); }
+  double *FP(int d) { return fld(KAPSEL_F0 + d); }
+
+  /* Zeta_k2advection_k (operate_omega.cxx:100-111); global u is scratch */
+  void zeta_k2advection_k(double **zeta, double **adv) {
+    truncate(zeta[0]);
+    truncate(zeta[1]);
+    double *u[3] = {U(0), U(1), U(2)};
+    zeta_k2u_k(zeta, uk_dc, u);
+    for (int d = 0; d < 3; d++) fft_c2r(u[d]);
+    u2advection_k(u, adv);
+  }
+
+  /* NS_solver_slavedEuler (fluid_solver.cxx:340-366); f_ns0 is WORK0/WORK1 */
+  void ns_slaved_euler(const kapsel_time &jikan) {
+    double *zeta[2] = {fld(KAPSEL_ZETA0), fld(KAPSEL_ZETA1)};
+    double *f_ns0[2] = {fld(KAPSEL_WORK0), fld(KAPSEL_WORK1)};
+    zeta_k2advection_k(zeta, f_ns0);
+    double dmy0 = -NU * jikan.dt_fluid;
+    const double dmy1 = 1. / NU;
+    /* the four kept boxes (fft_wrapper.cxx:185-197) */
+    int rng[4][6] = {{0, TRN_X - 1, 0, TRN_Y - 1, 0, 2 * TRN_Z - 1},
+                     {0, TRN_X - 1, NY - TRN_Y + 1, NY - 1, 0, 2 * TRN_Z - 1},
+                     {NX - TRN_X + 1, NX - 1, 0, TRN_Y - 1, 0, 2 * TRN_Z - 1},
+                     {NX - TRN_X + 1, NX - 1, NY - TRN_Y + 1, NY - 1, 0, 2 * TRN_Z - 1}};
+    for (int n = 0; n < 4; n++) {
+#pragma omp parallel for
+      for (int i = rng[n][0]; i <= rng[n][1]; i++)
+        for (int j = rng[n][2]; j <= rng[n][3]; j++)
+          for (int k = rng[n][4]; k <= rng[n][5]; k++) {
+            size_t im = ((size_t)i * NY + j) * NZ_ + k;
+            double dmy = exp(dmy0 * K2(i, j, k)) - 1.;
+            double ik2 = IK2(i, j, k);
+            zeta[0][im] += dmy * (zeta[0][im] - dmy1 * ik2 * f_ns0[0][im]);
+            zeta[1][im] += dmy * (zeta[1][im] - dmy1 * ik2 * f_ns0[1][im]);
+          }
+    }
+  }
+
+  /* Solenoidal_uk (operate_omega.cxx:187-220) */
+  void solenoidal_uk(double **u) {
+#pragma omp parallel for
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++)
+        for (int k = 0; k < NZ_; k++) {
+          size_t im = ((size_t)i * NY + j) * NZ_ + k;
+          double kx = KX(i) * WAVE_X, ky = KY(j) * WAVE_Y, kz = KZ(k) * WAVE_Z;
+          double dmy = IK2(i, j, k) * (u[0][im] * kx + u[1][im] * ky + u[2][im] * kz);
+          u[0][im] -= dmy * kx;
+          u[1][im] -= dmy * ky;
+          u[2][im] -= dmy * kz;
+        }
+  }
+
+  /* Solenoidal_u (operate_omega.h:109-116) */
+  void solenoidal_u(double **u) {
+    for (int d = 0; d < 3; d++) fft_r2c(u[d]);
+    for (int d = 0; d < 3; d++) truncate(u[d]);
+    solenoidal_uk(u);
+    for (int d = 0; d < 3; d++) fft_c2r(u[d]);
+  }
+
+  /* Zeta_k2u (operate_omega.h:138-141) */
+  void zeta_k2u() {
+    double *zeta[2] = {fld(KAPSEL_ZETA0), fld(KAPSEL_ZETA1)};
+    double *u[3] = {U(0), U(1), U(2)};
+    zeta_k2u_k(zeta, uk_dc, u);
+    for (int d = 0; d < 3; d++) fft_c2r(u[d]);
+  }
+
+  /* U2zeta_k (operate_omega.h:212-215) */
+  void u2zeta_k() {
+    double *zeta[2] = {fld(KAPSEL_ZETA0), fld(KAPSEL_ZETA1)};
+    double *u[3] = {U(0), U(1), U(2)};
+    for (int d = 0; d < 3; d++) fft_r2c(u[d]);
+    u_k2zeta_k(u, zeta, uk_dc);
+  }
+
+  /* ------------------------------------------------------------ profile */
+  /* H, Phi (profile.h:29-49) */
+  inline double H(double x) const { return x > 0 ? exp(-SQ(DX / x)) : 0; }
+  inline double Phi(double x, double radius) const {
+    double dmy = H(radius + HXI - x);
+    return dmy / (dmy + H(x - radius + HXI));
+  }
+  /* Phi_compact_sin (profile.h:139-152) */
+  inline double Phi_compact_sin(double x, double radius) const {
+    double dmy_x = radius - x;
+    if (fabs(dmy_x) <= HXI) return 0.5 * sin(M_PI * dmy_x / XI) + 0.5;
+    return dmy_x > HXI ? 1. : 0.;
+  }
+  /* Particle_cell (make_phi.h:120-148) */
+  inline void Particle_cell(const double *xp, int *x_int, double *residue) const {
+    const double idx = 1. / DX;
+    for (int d = 0; d < 3; d++) {
+      double dmy = xp[d] * idx;
+      x_int[d] = (int)dmy;
+      residue[d] = (dmy - (double)x_int[d]) * DX;
+    }
+  }
+  /* Relative_coord (make_phi.h:162-179) */
+  inline void Relative_coord(const int *cell, const int *x_int, const double *residue,
+                             int *r_mesh, double *r) const {
+    const int Nl[3] = {NX, NY, NZ};
+    for (int d = 0; d < 3; d++) {
+      r_mesh[d] = (x_int[d] + cell[d] + Nl[d]) % Nl[d];
+      r[d] = (double)cell[d] * DX - residue[d];
+    }
+  }
+  /* Distance0/Distance (interaction.h:122-147) */
+  inline double Distance(const double *x1, const double *x2) const {
+    double dmy = 0.0;
+    for (int d = 0; d < 3; d++) {
+      double x12 = x2[d] - x1[d];
+      x12 -= (double)Nint(x12 / L[d]) * L[d];
+      dmy += SQ(x12);
+    }
+    return sqrt(dmy);
+  }
+  inline void Distance0(const double *x1, const double *x2, double &r12, double *x12) const {
+    double dmy = 0.0;
+    for (int d = 0; d < 3; d++) {
+      x12[d] = x2[d] - x1[d];
+      x12[d] -= (double)Nint(x12[d] / L[d]) * L[d];
+      dmy += SQ(x12[d]);
+    }
+    r12 = sqrt(dmy);
+  }
+  /* Angular2v_rot_on / _rot_off (make_phi.h:400-417) */
+  inline void Angular2v(const double *omega, const double *r, double *v) const {
+    if (P.rotation) {
+      v[0] = omega[1] * r[2] - omega[2] * r[1];
+      v[1] = omega[2] * r[0] - omega[0] * r[2];
+      v[2] = omega[0] * r[1] - omega[1] * r[0];
+    } else {
+      v[0] = v[1] = v[2] = 0.0;
+    }
+  }
+  inline void getx(int n, double *xp) { for (int d = 0; d < 3; d++) xp[d] = pf(KAPSEL_P_X, d)[n]; }
+
+  /* Make_phi_particle_sum_primitive (make_phi.cxx:229-275); phi_sum is
+   * accumulated (caller resets, sp_3d_ns.cxx:51-53) */
+  void make_phi_particle_sum(double *phi, double *phi_sum) {
+#pragma omp parallel for
+    for (int n = 0; n < Np; n++) {
+      double xp[3]; getx(n, xp);
+      int x_int[3]; double residue[3];
+      Particle_cell(xp, x_int, residue);
+      int r_mesh[3]; double r[3], x[3];
+      for (int mesh = 0; mesh < NP_domain; mesh++) {
+        Relative_coord(&sekibun[3 * mesh], x_int, residue, r_mesh, r);
+        for (int d = 0; d < 3; d++) x[d] = r_mesh[d] * DX;
+        double dmy = Distance(x, xp);
+        double dmy_phi = Phi(dmy, RADIUS);
+        size_t im = ((size_t)r_mesh[0] * NY + r_mesh[1]) * NZ_ + r_mesh[2];
+#pragma omp atomic
+        phi_sum[im] += dmy_phi;
+      }
+    }
+#pragma omp parallel for
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++)
+        for (int k = 0; k < NZ; k++) {
+          size_t im = ((size_t)i * NY + j) * NZ_ + k;
+          phi[im] = std::min(phi_sum[im], 1.0);
+        }
+  }
+
+  /* Make_phi_u_primitive with SW_UP = 0 (make_phi.cxx:146-227): unclamped sum */
+  void make_phi_particle(double *phi) {
+#pragma omp parallel for
+    for (int n = 0; n < Np; n++) {
+      double xp[3]; getx(n, xp);
+      int x_int[3]; double residue[3];
+      Particle_cell(xp, x_int, residue);
+      int r_mesh[3]; double r[3], x[3];
+      for (int mesh = 0; mesh < NP_domain; mesh++) {
+        Relative_coord(&sekibun[3 * mesh], x_int, residue, r_mesh, r);
+        for (int d = 0; d < 3; d++) x[d] = r_mesh[d] * DX;
+        double dmy = Distance(x, xp);
+        double dmy_phi = Phi(dmy, RADIUS);
+        size_t im = ((size_t)r_mesh[0] * NY + r_mesh[1]) * NZ_ + r_mesh[2];
+#pragma omp atomic
+        phi[im] += dmy_phi;
+      }
+    }
+  }
+
+  /* Make_phi_sin_primitive (operate_dielectric.cxx:78-120) */
+  void make_phi_sin(double *phi) {
+#pragma omp parallel for
+    for (int n = 0; n < Np; n++) {
+      double xp[3]; getx(n, xp);
+      int x_int[3]; double residue[3];
+      Particle_cell(xp, x_int, residue);
+      int r_mesh[3]; double r[3], x[3];
+      for (int mesh = 0; mesh < NP_domain; mesh++) {
+        Relative_coord(&sekibun[3 * mesh], x_int, residue, r_mesh, r);
+        for (int d = 0; d < 3; d++) x[d] = r_mesh[d] * DX;
+        double dmy = Distance(x, xp);
+        double dmy_phi = Phi_compact_sin(dmy, RADIUS);
+        size_t im = ((size_t)r_mesh[0] * NY + r_mesh[1]) * NZ_ + r_mesh[2];
+#pragma omp atomic
+        phi[im] += dmy_phi;
+      }
+    }
+  }
+
+  /* Make_u_particle_sum_primitive (make_phi.cxx:329-370); up accumulated */
+  void make_u_particle_sum(double **up, const double *phi_sum) {
+#pragma omp parallel for
+    for (int n = 0; n < Np; n++) {
+      double xp[3], vp[3], om[3];
+      for (int d = 0; d < 3; d++) {
+        xp[d] = pf(KAPSEL_P_X, d)[n]; vp[d] = pf(KAPSEL_P_V, d)[n]; om[d] = pf(KAPSEL_P_OMEGA, d)[n];
+      }
+      int x_int[3]; double residue[3];
+      Particle_cell(xp, x_int, residue);
+      int r_mesh[3]; double r[3], x[3], v_rot[3];
+      for (int mesh = 0; mesh < NP_domain; mesh++) {
+        Relative_coord(&sekibun[3 * mesh], x_int, residue, r_mesh, r);
+        for (int d = 0; d < 3; d++) x[d] = r_mesh[d] * DX;
+        size_t im = ((size_t)r_mesh[0] * NY + r_mesh[1]) * NZ_ + r_mesh[2];
+        double dmy = Distance(x, xp);
+        double dmy_phi = Phi(dmy, RADIUS) / std::max(phi_sum[im], 1.0);
+        Angular2v(om, r, v_rot);
+#pragma omp atomic
+        up[0][im] += ((vp[0] + v_rot[0]) * dmy_phi);
+#pragma omp atomic
+        up[1][im] += ((vp[1] + v_rot[1]) * dmy_phi);
+#pragma omp atomic
+        up[2][im] += ((vp[2] + v_rot[2]) * dmy_phi);
+      }
+    }
+  }
+
+  /* Calc_f_hydro_correct_precision (md_force.cxx:340-449), free spheres */
+  void calc_f_hydro(const double *phi_sum, double **u, const kapsel_time &jikan) {
+    const double dmy0 = -DX3 * P.rho;
+    double dmy = dmy0 / jikan.dt_fluid;
+#pragma omp parallel for
+    for (int n = 0; n < Np; n++) {
+      double xp[3], vp[3], om[3], force[3] = {0, 0, 0}, torque[3] = {0, 0, 0};
+      for (int d = 0; d < 3; d++) {
+        xp[d] = pf(KAPSEL_P_X, d)[n]; vp[d] = pf(KAPSEL_P_V, d)[n]; om[d] = pf(KAPSEL_P_OMEGA, d)[n];
+      }
+      int x_int[3]; double residue[3];
+      Particle_cell(xp, x_int, residue);
+      int r_mesh[3]; double r[3], x[3], v_rot[3], dmy_fp[3];
+      for (int mesh = 0; mesh < NP_domain; mesh++) {
+        Relative_coord(&sekibun[3 * mesh], x_int, residue, r_mesh, r);
+        for (int d = 0; d < 3; d++) x[d] = r_mesh[d] * DX;
+        double dmyR = Distance(x, xp);
+        Angular2v(om, r, v_rot);
+        size_t im = ((size_t)r_mesh[0] * NY + r_mesh[1]) * NZ_ + r_mesh[2];
+        double dmy_phi = Phi(dmyR, RADIUS) / std::max(phi_sum[im], 1.0);
+        for (int d = 0; d < 3; d++) {
+          dmy_fp[d] = ((vp[d] + v_rot[d]) - u[d][im]) * dmy_phi;
+          force[d] += dmy_fp[d];
+        }
+        torque[0] += (r[1] * dmy_fp[2] - r[2] * dmy_fp[1]);
+        torque[1] += (r[2] * dmy_fp[0] - r[0] * dmy_fp[2]);
+        torque[2] += (r[0] * dmy_fp[1] - r[1] * dmy_fp[0]);
+      }
+      for (int d = 0; d < 3; d++) pf(KAPSEL_P_F_HYDRO, d)[n] = dmy * force[d];
+      if (P.rotation)
+        for (int d = 0; d < 3; d++) pf(KAPSEL_P_TORQUE_HYDRO, d)[n] = dmy * torque[d];
+    }
+  }
+
+  /* Lennard_Jones_f (interaction.h:175-200) */
+  inline double Lennard_Jones_f(double x, double sigma) const {
+    int pw = P.lj_powers;
+    double coeff = pw == 0 ? 24. * P.lj_epsilon : pw == 1 ? 48. * P.lj_epsilon : 72. * P.lj_epsilon;
+    double dmy = sigma / x;
+    double d2 = SQ(dmy);
+    if (pw == 0) dmy = d2 * d2 * d2;
+    else if (pw == 1) dmy = SQ(d2 * d2 * d2);
+    else dmy = SQ(d2 * d2 * d2) * (d2 * d2 * d2);
+    return coeff / SQ(x) * (2.0 * SQ(dmy) - dmy);
+  }
+
+  /* Calc_f_Lennard_Jones_shear_cap_primitive_lnk (md_force.cxx:16-171),
+   * cap = DBL_MAX, free spheres */
+  void lj_lnk() {
+    const int Cell_length = 16;
+    const double pair_cutoff = A_R_cutoff * P.lj_sigma;
+    int lc[3] = {NX / Cell_length, NY / Cell_length, NZ / Cell_length};
+    double lc_r = Cell_length * DX;
+    int lcyz = lc[1] * lc[2], lcxyz = lc[0] * lcyz;
+    std::vector<int> head(lcxyz, -1), lscl(Np);
+    for (int n = 0; n < Np; n++) {
+      int mc[3];
+      for (int d = 0; d < 3; d++) mc[d] = int(pf(KAPSEL_P_X, d)[n] / lc_r);
+      int cn = mc[0] * lcyz + mc[1] * lc[2] + mc[2];
+      lscl[n] = head[cn];
+      head[cn] = n;
+    }
+    int ic[3];
+    for (ic[0] = 0; ic[0] < lc[0]; ic[0]++)
+      for (ic[1] = 0; ic[1] < lc[1]; ic[1]++)
+        for (ic[2] = 0; ic[2] < lc[2]; ic[2]++) {
+          int cn = ic[0] * lcyz + ic[1] * lc[2] + ic[2];
+          int icl[3], icls[3];
+          for (icl[0] = ic[0] - 1; icl[0] <= ic[0] + 1; icl[0]++)
+            for (icl[1] = ic[1] - 1; icl[1] <= ic[1] + 1; icl[1]++)
+              for (icl[2] = ic[2] - 1; icl[2] <= ic[2] + 1; icl[2]++) {
+                for (int d = 0; d < 3; d++) {
+                  icls[d] = icl[d];
+                  if (icls[d] < 0) icls[d] = lc[d] - 1;
+                  if (icls[d] > lc[d] - 1) icls[d] = 0;
+                }
+                int cl = ((icls[0] + lc[0]) % lc[0]) * lcyz + ((icls[1] + lc[1]) % lc[1]) * lc[2] +
+                         ((icls[2] + lc[2]) % lc[2]);
+                int i = head[cn];
+                while (i != -1) {
+                  int j = head[cl];
+                  while (j != -1) {
+                    if (i > j) {
+                      double xi[3], xj[3], rv[3], rij;
+                      getx(i, xi); getx(j, xj);
+                      Distance0(xi, xj, rij, rv);
+                      if (rij < pair_cutoff) {
+                        double dmy_r = Lennard_Jones_f(rij, P.lj_sigma);
+                        for (int d = 0; d < 3; d++) {
+                          double fi = dmy_r * (-rv[d]);
+                          pf(KAPSEL_P_FR, d)[i] += fi;
+                          pf(KAPSEL_P_FR, d)[j] -= fi;
+                        }
+                      }
+                    }
+                    j = lscl[j];
+                  }
+                  i = lscl[i];
+                }
+              }
+        }
+  }
+
+  /* Force (particle_solver.h:92-104) */
+  void force() {
+    if (P.lj_truncate >= 0) lj_lnk();
+    for (int n = 0; n < Np; n++)
+      for (int d = 0; d < 3; d++) pf(KAPSEL_P_FR, d)[n] += P.body_force[d];
+  }
+
+  /* reset_Forces (particle_solver.cxx:11-37) */
+  void reset_forces() {
+    for (int n = 0; n < Np; n++)
+      for (int d = 0; d < 3; d++) {
+        pf(KAPSEL_P_FR_PREVIOUS, d)[n] = pf(KAPSEL_P_FR, d)[n];
+        pf(KAPSEL_P_FR, d)[n] = 0.0;
+        pf(KAPSEL_P_F_HYDRO_PREVIOUS, d)[n] = pf(KAPSEL_P_F_HYDRO, d)[n];
+        pf(KAPSEL_P_F_HYDRO, d)[n] = 0.0;
+        pf(KAPSEL_P_TORQUE_R_PREVIOUS, d)[n] = pf(KAPSEL_P_TORQUE_R, d)[n];
+        pf(KAPSEL_P_TORQUE_R, d)[n] = 0.0;
+        pf(KAPSEL_P_TORQUE_HYDRO_PREVIOUS, d)[n] = pf(KAPSEL_P_TORQUE_HYDRO, d)[n];
+        pf(KAPSEL_P_TORQUE_HYDRO, d)[n] = 0.0;
+      }
+  }
+
+  /* MD_solver_position_Euler/_AB2 (particle_solver.cxx:39-91); PBC
+   * (periodic_boundary.h:9-19) */
+  void md_position(const kapsel_time &jikan) {
+    for (int n = 0; n < Np; n++)
+      for (int d = 0; d < 3; d++) {
+        double &x = pf(KAPSEL_P_X, d)[n];
+        pf(KAPSEL_P_X_PREVIOUS, d)[n] = x;
+        double delta_x = jikan.ts == 0 ? jikan.dt_md * pf(KAPSEL_P_V, d)[n]
+                                       : jikan.hdt_md * (3.0 * pf(KAPSEL_P_V, d)[n] - pf(KAPSEL_P_V_OLD, d)[n]);
+        pf(KAPSEL_P_X_NOPBC, d)[n] += delta_x;
+        x += delta_x;
+        x = fmod(x + L[d], L[d]);
+      }
+  }
+
+  /* MD_solver_velocity_Euler / _AB2_hydro (particle_solver.cxx:109-150, 252-295) */
+  void md_velocity(const kapsel_time &jikan) {
+    force();
+    bool euler = jikan.ts == 0;
+    double dmy = (euler ? jikan.dt_md : jikan.hdt_md) * IMASS;
+    double dmy_rot = (euler ? jikan.dt_md : jikan.hdt_md) * IMOI;
+    for (int n = 0; n < Np; n++)
+      for (int d = 0; d < 3; d++) {
+        double &v = pf(KAPSEL_P_V, d)[n];
+        double &om = pf(KAPSEL_P_OMEGA, d)[n];
+        pf(KAPSEL_P_V_OLD, d)[n] = v;
+        pf(KAPSEL_P_OMEGA_OLD, d)[n] = om;
+        if (euler) {
+          v += dmy * (pf(KAPSEL_P_F_HYDRO, d)[n] + pf(KAPSEL_P_FR, d)[n]);
+          om += dmy_rot * (pf(KAPSEL_P_TORQUE_HYDRO, d)[n] + pf(KAPSEL_P_TORQUE_R, d)[n]);
+        } else {
+          v += dmy * (2.0 * pf(KAPSEL_P_F_HYDRO, d)[n] + pf(KAPSEL_P_FR, d)[n] + pf(KAPSEL_P_FR_PREVIOUS, d)[n]);
+          om += dmy_rot * (2.0 * pf(KAPSEL_P_TORQUE_HYDRO, d)[n] + pf(KAPSEL_P_TORQUE_R, d)[n] +
+                           pf(KAPSEL_P_TORQUE_R_PREVIOUS, d)[n]);
+        }
+      }
+    reset_forces();
+  }
+
+  void reset(double *a) { memset(a, 0, F * sizeof(double)); }
+
+  /* Make_f_particle_dt_sole + Add_f_particle (f_particle.h:37-95) */
+  void make_f_particle_dt_sole_add() {
+    double *f[3] = {FP(0), FP(1), FP(2)};
+    double *u[3] = {U(0), U(1), U(2)};
+    double *up[3] = {UP(0), UP(1), UP(2)};
+    double *phi = fld(KAPSEL_PHI);
+#pragma omp parallel for
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++)
+        for (int k = 0; k < NZ; k++) {
+          size_t im = ((size_t)i * NY + j) * NZ_ + k;
+          for (int d = 0; d < 3; d++) f[d][im] = up[d][im] - phi[im] * u[d][im];
+        }
+    solenoidal_u(f);
+#pragma omp parallel for
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++)
+        for (int k = 0; k < NZ; k++) {
+          size_t im = ((size_t)i * NY + j) * NZ_ + k;
+          for (int d = 0; d < 3; d++) u[d][im] += f[d][im];
+        }
+  }
+
+  /* Time_evolution_hydro, Navier-Stokes branch (sp_3d_ns.cxx:24-186) */
+  void spm_step(kapsel_time &jikan) {
+    ns_slaved_euler(jikan);
+    for (int d = 0; d < 3; d++)
+      if (P.fix_cell[d]) uk_dc[d] = 0.0;
+    truncate(fld(KAPSEL_ZETA0));
+    truncate(fld(KAPSEL_ZETA1));
+    zeta_k2u();
+    md_position(jikan);
+    reset(fld(KAPSEL_PHI));
+    reset(fld(KAPSEL_PHI_SUM));
+    make_phi_particle_sum(fld(KAPSEL_PHI), fld(KAPSEL_PHI_SUM));
+    for (int d = 0; d < 3; d++) reset(UP(d));
+    double *u[3] = {U(0), U(1), U(2)};
+    calc_f_hydro(fld(KAPSEL_PHI_SUM), u, jikan);
+    md_velocity(jikan);
+    for (int d = 0; d < 3; d++) reset(UP(d));
+    double *up[3] = {UP(0), UP(1), UP(2)};
+    make_u_particle_sum(up, fld(KAPSEL_PHI_SUM));
+    make_f_particle_dt_sole_add();
+    u2zeta_k();
+    jikan.time += jikan.dt_fluid;
+    jikan.ts += 1;
+  }
+
+  /* main's initial rigidity projection (sp_3d_ns.cxx:431-447) */
+  void init_rigidity() {
+    reset(fld(KAPSEL_PHI));
+    for (int d = 0; d < 3; d++) reset(UP(d));
+    reset(fld(KAPSEL_PHI_SUM));
+    make_phi_particle_sum(fld(KAPSEL_PHI), fld(KAPSEL_PHI_SUM));
+    double *up[3] = {UP(0), UP(1), UP(2)};
+    make_u_particle_sum(up, fld(KAPSEL_PHI_SUM));
+    zeta_k2u();
+    make_f_particle_dt_sole_add();
+    u2zeta_k();
+    uk_dc[0] = uk_dc[1] = uk_dc[2] = 0.0;
+  }
+};
+
+thread_local char g_err[256] = "";
+
+}  // namespace
+
+/* ------------------------------------------------------------------------ */
+extern "C" {
+
+const char *oracle_last_error(void) { return g_err; }
+
+void *oracle_create(const kapsel_params *p) {
+  if (!p || p->nx < 16 || p->ny < 16 || p->nz < 16) {
+    snprintf(g_err, sizeof g_err, "bad params");
+    return nullptr;
+  }
+  for (int n : {p->nx, p->ny, p->nz})
+    if (n & (n - 1)) {
+      snprintf(g_err, sizeof g_err, "grid not a power of two");
+      return nullptr;
+    }
+  Oracle *o = new Oracle();
+  o->init(p);
+  return o;
+}
+void oracle_destroy(void *h) { delete (Oracle *)h; }
+int oracle_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+void oracle_set_num_threads(int n) {
+#ifdef _OPENMP
+  omp_set_num_threads(n);
+#endif
+}
+
+#define O ((Oracle *)h)
+int oracle_get_sekibun(void *h, int *np, int *cells) {
+  *np = O->NP_domain;
+  if (cells) memcpy(cells, O->sekibun.data(), sizeof(int) * O->sekibun.size());
+  return 0;
+}
+int oracle_set_field(void *h, int f, const double *a) { memcpy(O->fld(f), a, O->F * sizeof(double)); return 0; }
+int oracle_get_field(void *h, int f, double *a) { memcpy(a, O->fld(f), O->F * sizeof(double)); return 0; }
+int oracle_set_uk_dc(void *h, const double *d) { for (int i = 0; i < 3; i++) O->uk_dc[i] = d[i]; return 0; }
+int oracle_get_uk_dc(void *h, double *d) { for (int i = 0; i < 3; i++) d[i] = O->uk_dc[i]; return 0; }
+int oracle_set_particles(void *h, const double *s) {
+  memcpy(O->ps.data(), s, sizeof(double) * KAPSEL_P_NFIELDS * 3 * O->Np); return 0;
+}
+int oracle_get_particles(void *h, double *s) {
+  memcpy(s, O->ps.data(), sizeof(double) * KAPSEL_P_NFIELDS * 3 * O->Np); return 0;
+}
+int oracle_fft_r2c(void *h, int f) { O->fft_r2c(O->fld(f)); return 0; }
+int oracle_fft_c2r(void *h, int f) { O->fft_c2r(O->fld(f)); return 0; }
+int oracle_truncate(void *h, int f) { O->truncate(O->fld(f)); return 0; }
+int oracle_zeta_k2u_k(void *h) {
+  double *z[2] = {O->fld(KAPSEL_ZETA0), O->fld(KAPSEL_ZETA1)};
+  double *u[3] = {O->U(0), O->U(1), O->U(2)};
+  O->zeta_k2u_k(z, O->uk_dc, u);
+  return 0;
+}
+int oracle_u_k2zeta_k(void *h) {
+  double *z[2] = {O->fld(KAPSEL_ZETA0), O->fld(KAPSEL_ZETA1)};
+  double *u[3] = {O->U(0), O->U(1), O->U(2)};
+  O->u_k2zeta_k(u, z, O->uk_dc);
+  return 0;
+}
+/* U2advection_k on U0..U2 (real space); result in WORK0/WORK1 (k-space) */
+int oracle_u2advection_k(void *h) {
+  double *u[3] = {O->U(0), O->U(1), O->U(2)};
+  double *adv[2] = {O->fld(KAPSEL_WORK0), O->fld(KAPSEL_WORK1)};
+  O->u2advection_k(u, adv);
+  return 0;
+}
+int oracle_solenoidal_u(void *h, int f0) {
+  double *u[3] = {O->fld(f0), O->fld(f0 + 1), O->fld(f0 + 2)};
+  O->solenoidal_u(u);
+  return 0;
+}
+int oracle_ns_slaved_euler(void *h, const kapsel_time *t) { O->ns_slaved_euler(*t); return 0; }
+int oracle_make_phi_particle_sum(void *h) {
+  O->reset(O->fld(KAPSEL_PHI));
+  O->reset(O->fld(KAPSEL_PHI_SUM));
+  O->make_phi_particle_sum(O->fld(KAPSEL_PHI), O->fld(KAPSEL_PHI_SUM));
+  return 0;
+}
+int oracle_make_u_particle_sum(void *h) {
+  double *up[3] = {O->UP(0), O->UP(1), O->UP(2)};
+  for (int d = 0; d < 3; d++) O->reset(up[d]);
+  O->make_u_particle_sum(up, O->fld(KAPSEL_PHI_SUM));
+  return 0;
+}
+int oracle_make_phi_particle(void *h, int f) {
+  O->reset(O->fld(f));
+  O->make_phi_particle(O->fld(f));
+  return 0;
+}
+int oracle_make_phi_sin(void *h, int f) {
+  O->reset(O->fld(f));
+  O->make_phi_sin(O->fld(f));
+  return 0;
+}
+int oracle_calc_f_hydro(void *h, const kapsel_time *t) {
+  double *u[3] = {O->U(0), O->U(1), O->U(2)};
+  O->calc_f_hydro(O->fld(KAPSEL_PHI_SUM), u, *t);
+  return 0;
+}
+int oracle_force(void *h) { O->force(); return 0; }
+int oracle_md_position(void *h, const kapsel_time *t) { O->md_position(*t); return 0; }
+int oracle_md_velocity(void *h, const kapsel_time *t) { O->md_velocity(*t); return 0; }
+int oracle_make_f_particle_dt_sole(void *h) { O->make_f_particle_dt_sole_add(); return 0; }
+int oracle_init_rigidity(void *h) { O->init_rigidity(); return 0; }
+int oracle_spm_step(void *h, kapsel_time *t) { O->spm_step(*t); return 0; }
+#undef O
+
+}  // extern "C"

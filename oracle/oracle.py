@@ -1,0 +1,217 @@
+"""ctypes front end of the CPU FP64 oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py.  The method names mirror icep_kapsel_amd.KapselGPU
+so a parity test drives both with the same calls.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from icep_kapsel_amd import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(_HERE, "liboracle.so")
+REF_LIB = os.path.join(_HERE, "_ref", "libooura_ref.so")
+
+_lib = None
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise OSError(f"oracle not built: {LIB} (make -C oracle)")
+        lib = ctypes.CDLL(LIB)
+        vp, dp, ip, i = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int), ctypes.c_int
+        tp = ctypes.POINTER(abi.Time)
+        lib.oracle_create.restype = vp
+        lib.oracle_create.argtypes = [ctypes.POINTER(abi.Params)]
+        lib.oracle_destroy.argtypes = [vp]
+        lib.oracle_destroy.restype = None
+        lib.oracle_last_error.restype = ctypes.c_char_p
+        lib.oracle_num_threads.restype = i
+        lib.oracle_set_num_threads.argtypes = [i]
+        lib.oracle_set_num_threads.restype = None
+        protos = {
+            "oracle_get_sekibun": [ip, ip],
+            "oracle_set_field": [i, dp], "oracle_get_field": [i, dp],
+            "oracle_set_uk_dc": [dp], "oracle_get_uk_dc": [dp],
+            "oracle_set_particles": [dp], "oracle_get_particles": [dp],
+            "oracle_fft_r2c": [i], "oracle_fft_c2r": [i], "oracle_truncate": [i],
+            "oracle_zeta_k2u_k": [], "oracle_u_k2zeta_k": [], "oracle_u2advection_k": [],
+            "oracle_solenoidal_u": [i], "oracle_ns_slaved_euler": [tp],
+            "oracle_make_phi_particle_sum": [], "oracle_make_u_particle_sum": [],
+            "oracle_make_phi_particle": [i], "oracle_make_phi_sin": [i],
+            "oracle_calc_f_hydro": [tp], "oracle_force": [],
+            "oracle_md_position": [tp], "oracle_md_velocity": [tp],
+            "oracle_make_f_particle_dt_sole": [], "oracle_init_rigidity": [],
+            "oracle_spm_step": [tp],
+        }
+        for name, args in protos.items():
+            fn = getattr(lib, name)
+            fn.restype = i
+            fn.argtypes = [vp] + args
+        _lib = lib
+    return _lib
+
+
+def set_threads(n):
+    _load().oracle_set_num_threads(int(n))
+
+
+def num_threads():
+    return _load().oracle_num_threads()
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Oracle:
+    """One parameter set per instance (the reference latches statics per process)."""
+
+    def __init__(self, params):
+        self.lib = _load()
+        self.params = params
+        self.h = self.lib.oracle_create(ctypes.byref(params))
+        if not self.h:
+            raise ValueError(self.lib.oracle_last_error().decode())
+        self.nx, self.ny, self.nz = params.nx, params.ny, params.nz
+        self.nz_ = 2 * (self.nz // 2 + 1)
+        self.shape = (self.nx, self.ny, self.nz_)
+        self.np = params.n_particles
+
+    def close(self):
+        if self.h:
+            self.lib.oracle_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _call(self, name, *args):
+        rc = getattr(self.lib, "oracle_" + name)(self.h, *args)
+        if rc != 0:
+            raise RuntimeError(f"oracle_{name} failed")
+
+    # fields -----------------------------------------------------------
+    def set_field(self, name, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(self.shape)
+        self._call("set_field", abi.FIELD[name], _dp(a))
+
+    def get_field(self, name):
+        a = np.empty(self.shape, dtype=np.float64)
+        self._call("get_field", abi.FIELD[name], _dp(a))
+        return a
+
+    def set_uk_dc(self, v):
+        a = np.ascontiguousarray(v, dtype=np.float64)
+        self._call("set_uk_dc", _dp(a))
+
+    def get_uk_dc(self):
+        a = np.zeros(3)
+        self._call("get_uk_dc", _dp(a))
+        return a
+
+    def set_particles(self, state):
+        a = np.ascontiguousarray(state, dtype=np.float64).reshape(abi.P_NFIELDS, 3, self.np)
+        self._call("set_particles", _dp(a))
+
+    def get_particles(self):
+        a = np.zeros((abi.P_NFIELDS, 3, self.np))
+        self._call("get_particles", _dp(a))
+        return a
+
+    def sekibun(self):
+        n = ctypes.c_int(0)
+        self._call("get_sekibun", ctypes.byref(n), None)
+        cells = np.zeros((n.value, 3), dtype=np.int32)
+        self._call("get_sekibun", ctypes.byref(n), cells.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        return cells
+
+    # operations --------------------------------------------------------
+    def fft_r2c(self, name):
+        self._call("fft_r2c", abi.FIELD[name])
+
+    def fft_c2r(self, name):
+        self._call("fft_c2r", abi.FIELD[name])
+
+    def truncate(self, name):
+        self._call("truncate", abi.FIELD[name])
+
+    def zeta_k2u_k(self):
+        self._call("zeta_k2u_k")
+
+    def u_k2zeta_k(self):
+        self._call("u_k2zeta_k")
+
+    def u2advection_k(self):
+        self._call("u2advection_k")
+
+    def solenoidal_u(self, name0):
+        self._call("solenoidal_u", abi.FIELD[name0])
+
+    def ns_slaved_euler(self, t):
+        self._call("ns_slaved_euler", ctypes.byref(t))
+
+    def make_phi_particle_sum(self):
+        self._call("make_phi_particle_sum")
+
+    def make_u_particle_sum(self):
+        self._call("make_u_particle_sum")
+
+    def make_phi_particle(self, name):
+        self._call("make_phi_particle", abi.FIELD[name])
+
+    def make_phi_sin(self, name):
+        self._call("make_phi_sin", abi.FIELD[name])
+
+    def calc_f_hydro(self, t):
+        self._call("calc_f_hydro", ctypes.byref(t))
+
+    def force(self):
+        self._call("force")
+
+    def md_position(self, t):
+        self._call("md_position", ctypes.byref(t))
+
+    def md_velocity(self, t):
+        self._call("md_velocity", ctypes.byref(t))
+
+    def make_f_particle_dt_sole(self):
+        self._call("make_f_particle_dt_sole")
+
+    def init_rigidity(self):
+        self._call("init_rigidity")
+
+    def spm_step(self, t):
+        self._call("spm_step", ctypes.byref(t))
+
+
+class OouraRef:
+    """The reference's own Ooura rdft3d/rdft3dsort (built from source into _ref/)."""
+
+    def __init__(self):
+        if not os.path.exists(REF_LIB):
+            raise OSError(f"reference FFT not built: {REF_LIB}")
+        self.lib = ctypes.CDLL(REF_LIB)
+        for fn in (self.lib.ref_a2a_k, self.lib.ref_a_k2a):
+            fn.restype = None
+            fn.argtypes = [ctypes.c_int] * 3 + [ctypes.POINTER(ctypes.c_double)]
+
+    def a2a_k(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).copy()
+        nx, ny, nz_ = a.shape
+        self.lib.ref_a2a_k(nx, ny, nz_ - 2, _dp(a))
+        return a
+
+    def a_k2a(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).copy()
+        nx, ny, nz_ = a.shape
+        self.lib.ref_a_k2a(nx, ny, nz_ - 2, _dp(a))
+        return a

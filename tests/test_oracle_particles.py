@@ -1,0 +1,244 @@
+"""Pins the oracle's particle <-> grid coupling and particle dynamics.
+
+profile / Sekibun  -> closed-form values of phi (profile.h:29-49 definition),
+                      lattice-point counts of the Sekibun cell
+scatter            -> brute force over every grid point with min-image distances
+gather             -> brute force; a fluid in rigid motion with the particle
+                      exerts no force or torque
+LJ                 -> all-pairs sum (the reference's own brute-force form,
+                      md_force.cxx:173-259)
+particle update    -> closed-form Euler / AB2 increments
+"""
+import numpy as np
+import pytest
+
+from icep_kapsel_amd import abi, workloads
+
+
+def _oracle(p):
+    from oracle.oracle import Oracle
+    return Oracle(p)
+
+
+def h(x, dx=1.0):
+    x = np.asarray(x, dtype=float)
+    out = np.zeros_like(x)
+    pos = x > 0
+    out[pos] = np.exp(-(dx / x[pos]) ** 2)
+    return out
+
+
+def phi_profile(r, a, xi, dx=1.0):
+    num = h(a + xi / 2 - r, dx)
+    return num / (num + h(r - a + xi / 2, dx))
+
+
+def min_image(d, L):
+    return d - L * np.where(d / L >= 0, np.floor(d / L + 0.5), np.ceil(d / L - 0.5))
+
+
+def grid_vectors(n, xp):
+    """r = grid point - particle, min image, shape (n, n, n, 3)."""
+    g = np.arange(n, dtype=float)
+    G = np.stack(np.meshgrid(g, g, g, indexing="ij"), axis=-1)
+    return min_image(G - xp, float(n))
+
+
+def test_phi_closed_form_values():
+    a, xi = 4.0, 2.0
+    assert phi_profile(np.array([0.0]), a, xi)[0] == 1.0
+    assert phi_profile(np.array([a - xi / 2]), a, xi)[0] == 1.0
+    assert abs(phi_profile(np.array([a]), a, xi)[0] - 0.5) < 1e-15
+    assert phi_profile(np.array([a + xi / 2]), a, xi)[0] == 0.0
+    r = np.linspace(a - xi / 2, a + xi / 2, 101)
+    assert np.all(np.diff(phi_profile(r, a, xi)) <= 0)
+
+
+@pytest.mark.parametrize("a,count", [(5.0, 1935), (4.0, 1309), (6.0, 2801), (3.0, None)])
+def test_sekibun_cell_count(a, count):
+    p = abi.make_params(32, n_particles=1, radius=a, xi=2.0)
+    cells = _oracle(p).sekibun()
+    rmax = a + 1.0 + 1.7321
+    nh = int(a) + 2 + 2
+    o = np.arange(-nh, nh + 1)
+    O = np.stack(np.meshgrid(o, o, o, indexing="ij"), -1).reshape(-1, 3)
+    inside = O[np.sqrt((O.astype(float) ** 2).sum(1)) < rmax]
+    assert len(cells) == len(inside)
+    if count is not None:
+        assert len(cells) == count
+    assert {tuple(c) for c in cells} == {tuple(c) for c in inside}
+
+
+def _setup(n=32, npart=3, a=3.0, seed=3, overlap=True, rotation=1):
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(0, n, size=(npart, 3))
+    if overlap and npart >= 2:
+        pos[1] = (pos[0] + np.array([a * 1.2, 0.3, -0.2])) % n  # overlapping profiles
+    pos[-1] = np.array([0.4, n - 0.3, 5.5])  # straddles the periodic boundary
+    vel = rng.normal(0, 1e-2, size=(npart, 3))
+    om = rng.normal(0, 1e-3, size=(npart, 3))
+    p = abi.make_params(n, n_particles=npart, radius=a, xi=2.0, rotation=rotation)
+    st = workloads.particle_state(pos, vel, om)
+    return p, st, pos, vel, om
+
+
+def _brute_phi(n, pos, a, xi):
+    phis = []
+    for x in pos:
+        r = np.sqrt((grid_vectors(n, x) ** 2).sum(-1))
+        phis.append(phi_profile(r, a, xi))
+    return np.array(phis)
+
+
+def test_scatter_phi_and_up_brute_force():
+    n, a, xi = 32, 3.0, 2.0
+    p, st, pos, vel, om = _setup(n, 3, a)
+    o = _oracle(p)
+    o.set_particles(st)
+    o.make_phi_particle_sum()
+    phis = _brute_phi(n, pos, a, xi)
+    phisum = phis.sum(0)
+    got = o.get_field("PHI_SUM")[:, :, :n]
+    assert np.abs(got - phisum).max() < 1e-13
+    assert np.abs(o.get_field("PHI")[:, :, :n] - np.minimum(phisum, 1.0)).max() < 1e-13
+    # total profile volume close to the sphere volume
+    assert abs(phis[0].sum() / (4 / 3 * np.pi * a ** 3) - 1) < 0.05
+    o.make_u_particle_sum()
+    up = np.zeros((3, n, n, n))
+    for i, x in enumerate(pos):
+        r = grid_vectors(n, x)
+        vr = np.cross(np.broadcast_to(om[i], r.shape), r)
+        w = phis[i] / np.maximum(phisum, 1.0)
+        for d in range(3):
+            up[d] += (vel[i, d] + vr[..., d]) * w
+    for d in range(3):
+        assert np.abs(o.get_field(f"UP{d}")[:, :, :n] - up[d]).max() < 1e-14
+    o.make_phi_particle("WORK0")
+    assert np.abs(o.get_field("WORK0")[:, :, :n] - phisum).max() < 1e-13
+
+
+def _brute_gather(n, pos, vel, om, u, a, xi, dt, rho=1.0, dx=1.0):
+    phis = _brute_phi(n, pos, a, xi)
+    psat = np.maximum(phis.sum(0), 1.0)
+    F, T = [], []
+    for i, x in enumerate(pos):
+        r = grid_vectors(n, x)
+        vr = np.cross(np.broadcast_to(om[i], r.shape), r)
+        w = phis[i] / psat
+        fp = np.stack([(vel[i, d] + vr[..., d] - u[d]) * w for d in range(3)], -1)
+        F.append(-(rho * dx ** 3 / dt) * fp.reshape(-1, 3).sum(0))
+        T.append(-(rho * dx ** 3 / dt) * np.cross(r, fp).reshape(-1, 3).sum(0))
+    return np.array(F), np.array(T)
+
+
+def test_gather_brute_force():
+    n, a, xi, dt = 32, 3.0, 2.0, 0.05
+    p, st, pos, vel, om = _setup(n, 3, a)
+    o = _oracle(p)
+    o.set_particles(st)
+    o.make_phi_particle_sum()
+    rng = np.random.default_rng(9)
+    u = rng.normal(0, 1e-2, size=(3, n, n, n))
+    for d in range(3):
+        o.set_field(f"U{d}", np.pad(u[d], ((0, 0), (0, 0), (0, 2))))
+    o.calc_f_hydro(abi.make_time(dt))
+    out = o.get_particles()
+    F, T = _brute_gather(n, pos, vel, om, u, a, xi, dt)
+    assert np.abs(out[abi.PFIELD["F_HYDRO"]].T - F).max() < 1e-12 * np.abs(F).max()
+    assert np.abs(out[abi.PFIELD["TORQUE_HYDRO"]].T - T).max() < 1e-12 * np.abs(T).max()
+
+
+def test_gather_rigid_motion_gives_zero():
+    n, a, dt = 32, 4.0, 0.05
+    pos = np.array([[10.3, 20.7, 5.1]])
+    vel = np.array([[0.01, -0.02, 0.005]])
+    om = np.array([[0.001, 0.002, -0.003]])
+    p = abi.make_params(n, n_particles=1, radius=a, xi=2.0, rotation=1)
+    o = _oracle(p)
+    o.set_particles(workloads.particle_state(pos, vel, om))
+    o.make_phi_particle_sum()
+    r = grid_vectors(n, pos[0])
+    vr = np.cross(np.broadcast_to(om[0], r.shape), r)
+    for d in range(3):
+        o.set_field(f"U{d}", np.pad(vel[0, d] + vr[..., d], ((0, 0), (0, 0), (0, 2))))
+    o.calc_f_hydro(abi.make_time(dt))
+    out = o.get_particles()
+    scale = (4 / 3 * np.pi * a ** 3) * 0.02 / dt
+    assert np.abs(out[abi.PFIELD["F_HYDRO"]]).max() < 1e-13 * scale
+    assert np.abs(out[abi.PFIELD["TORQUE_HYDRO"]]).max() < 1e-12 * scale * a
+
+
+def lj_brute(pos, L, sigma, eps, cutoff):
+    n = len(pos)
+    F = np.zeros_like(pos)
+    for i in range(n):
+        for j in range(n):
+            if i == j:
+                continue
+            d = min_image(pos[j] - pos[i], L)
+            r = np.sqrt((d * d).sum())
+            if r < cutoff:
+                s6 = (sigma / r) ** 6
+                f = 24 * eps / r ** 2 * (2 * s6 * s6 - s6)
+                F[i] += f * (-d)
+    return F
+
+
+def test_lj_cell_list_matches_all_pairs():
+    n, a = 64, 4.0
+    sigma = 2 * a
+    rng = np.random.default_rng(12)
+    pos = rng.uniform(0, n, size=(40, 3))
+    pos[1] = (pos[0] + [7.5, 1.0, 0.0]) % n      # inside the WCA cutoff 8.98
+    pos[3] = np.array([0.5, 30.0, 30.0])
+    pos[2] = np.array([n - 2.0, 30.5, 29.0])      # pair across the periodic boundary
+    p = abi.make_params(n, n_particles=len(pos), radius=a, xi=2.0, lj_sigma=sigma,
+                        body_force=(0.0, 0.0, 1e-3))
+    o = _oracle(p)
+    o.set_particles(workloads.particle_state(pos))
+    o.force()
+    got = o.get_particles()[abi.PFIELD["FR"]].T
+    ref = lj_brute(pos, float(n), sigma, 1.0, 2 ** (1 / 6) * sigma) + np.array([0, 0, 1e-3])
+    assert np.abs(got - ref).max() < 1e-12 * max(1.0, np.abs(ref).max())
+    assert np.abs(ref[:, :2]).max() > 0  # some pairs interact
+
+
+def test_md_update_closed_form():
+    n, a, dt = 32, 3.0, 0.05
+    pos = np.array([[31.99, 1.0, 5.0], [10.0, 10.0, 10.0]])
+    vel = np.array([[0.5, -0.1, 0.0], [0.0, 0.2, -0.3]])
+    om = np.array([[0.1, 0.0, 0.0], [0.0, 0.0, 0.2]])
+    p = abi.make_params(n, n_particles=2, radius=a, xi=2.0, mass_ratio=1.5, lj_truncate=-1)
+    o = _oracle(p)
+    st = workloads.particle_state(pos, vel, om)
+    st[abi.PFIELD["V_OLD"]] = (vel * 0.5).T
+    fh = np.array([[1.0, 2.0, 3.0], [-1.0, 0.0, 0.5]])
+    th = np.array([[0.1, 0.2, 0.3], [0.0, -0.1, 0.0]])
+    fr_prev = np.array([[0.3, 0.0, 0.0], [0.0, 0.0, -0.2]])
+    st[abi.PFIELD["F_HYDRO"]] = fh.T
+    st[abi.PFIELD["TORQUE_HYDRO"]] = th.T
+    st[abi.PFIELD["FR_PREVIOUS"]] = fr_prev.T
+    o.set_particles(st)
+    t0 = abi.make_time(dt, ts=0)
+    o.md_position(t0)
+    out = o.get_particles()
+    assert np.allclose(out[abi.PFIELD["X"]].T, (pos + dt * vel) % n, atol=1e-14)
+    t1 = abi.make_time(dt, ts=1)
+    o.set_particles(st)
+    o.md_position(t1)
+    out = o.get_particles()
+    assert np.allclose(out[abi.PFIELD["X"]].T, (pos + dt / 2 * (3 * vel - 0.5 * vel)) % n, atol=1e-14)
+    mass = 1.5 * 4 / 3 * np.pi * a ** 3
+    moi = 0.4 * mass * a * a
+    o.set_particles(st)
+    o.md_velocity(t0)
+    out = o.get_particles()
+    assert np.allclose(out[abi.PFIELD["V"]].T, vel + dt / mass * fh, rtol=1e-14, atol=1e-16)
+    assert np.allclose(out[abi.PFIELD["OMEGA"]].T, om + dt / moi * th, rtol=1e-14, atol=1e-16)
+    assert np.all(out[abi.PFIELD["F_HYDRO"]] == 0) and np.all(out[abi.PFIELD["FR"]] == 0)
+    assert np.allclose(out[abi.PFIELD["F_HYDRO_PREVIOUS"]].T, fh)
+    o.set_particles(st)
+    o.md_velocity(t1)
+    out = o.get_particles()
+    assert np.allclose(out[abi.PFIELD["V"]].T, vel + dt / (2 * mass) * (2 * fh + fr_prev), rtol=1e-14, atol=1e-16)
+    assert np.allclose(out[abi.PFIELD["OMEGA"]].T, om + dt / (2 * moi) * 2 * th, rtol=1e-14, atol=1e-16)
