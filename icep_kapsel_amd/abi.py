@@ -127,6 +127,11 @@ _PROTOS = {
     "kapsel_make_f_particle_dt_sole": [],
     "kapsel_init_rigidity": [],
     "kapsel_spm_step": [ctypes.POINTER(Time)],
+    "kapsel_dielectric_field": [],
+    "kapsel_poisson_apply": [_I, _I],
+    "kapsel_poisson_apply_host": [_DP, _DP],
+    "kapsel_poisson_rhs": [_DP],
+    "kapsel_poisson_solve": [_DP, _D, _I, _I, _IP, _DP],
     "kapsel_kernel_timing": [_I],
     "kapsel_kernel_timing_get": [_I, _DP, _IP, _DP],
 }

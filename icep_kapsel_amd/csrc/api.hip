@@ -83,6 +83,10 @@ static void build_sekibun(kapsel_ctx *ctx) {
         }
       }
   ctx->np_domain = (int)ctx->sekibun.size() / 3;
+  if (ctx->P.n_particles > 0 && (2 * nh + 1 > ctx->g.nx || 2 * nh + 1 > ctx->g.ny || 2 * nh + 1 > ctx->g.nz)) {
+    set_error("Sekibun cell (%d points across) wider than the grid", 2 * nh + 1);
+    throw Error{KAPSEL_ERR_ARG};
+  }
   KAPSEL_HIP(hipMalloc(&ctx->d_sekibun, sizeof(int) * ctx->sekibun.size()));
   KAPSEL_HIP(hipMemcpy(ctx->d_sekibun, ctx->sekibun.data(), sizeof(int) * ctx->sekibun.size(),
                        hipMemcpyHostToDevice));
@@ -106,6 +110,7 @@ static void destroy(kapsel_ctx *ctx) {
   if (ctx->d_ps) (void)hipFree(ctx->d_ps);
   if (ctx->d_sekibun) (void)hipFree(ctx->d_sekibun);
   free_twiddles(ctx);
+  poisson_free(ctx);
   for (auto &r : ctx->timing_pending) {
     (void)hipEventDestroy(r.start);
     (void)hipEventDestroy(r.stop);
@@ -426,6 +431,19 @@ int kapsel_set_particles(kapsel_ctx *ctx, const double *state) {
   API_BEGIN
   check_ctx(ctx);
   size_t n = sizeof(double) * KAPSEL_P_NFIELDS * 3 * ctx->P.n_particles;
+  if (n && !state) {
+    set_error("null particle state");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+  const double L[3] = {ctx->g.lx, ctx->g.ly, ctx->g.lz};
+  for (int d = 0; d < 3; d++)
+    for (int i = 0; i < ctx->P.n_particles; i++) {
+      double x = state[((size_t)KAPSEL_P_X * 3 + d) * ctx->P.n_particles + i];
+      if (!(x >= 0.0 && x < L[d])) {
+        set_error("particle %d position[%d] = %g outside [0, %g)", i, d, x, L[d]);
+        throw Error{KAPSEL_ERR_ARG};
+      }
+    }
   if (n) {
     KAPSEL_HIP(hipMemcpyAsync(ctx->d_ps, state, n, hipMemcpyHostToDevice, ctx->stream));
     KAPSEL_HIP(hipStreamSynchronize(ctx->stream));
@@ -572,6 +590,73 @@ int kapsel_spm_step(kapsel_ctx *ctx, kapsel_time *jikan) {
   check_ctx(ctx);
   check_time(jikan);
   spm_step(ctx, *jikan);
+  API_END
+}
+
+int kapsel_dielectric_field(kapsel_ctx *ctx) {
+  API_BEGIN
+  check_ctx(ctx);
+  dielectric_field(ctx);
+  API_END
+}
+
+int kapsel_poisson_apply(kapsel_ctx *ctx, int x_field, int ax_field) {
+  API_BEGIN
+  check_ctx(ctx);
+  check_field(x_field);
+  check_field(ax_field);
+  if (x_field == ax_field) {
+    set_error("poisson_apply needs distinct input and output fields");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+  poisson_apply(ctx, ctx->fields[x_field], ctx->fields[ax_field]);
+  API_END
+}
+
+int kapsel_poisson_apply_host(kapsel_ctx *ctx, const double *x, double *ax) {
+  API_BEGIN
+  check_ctx(ctx);
+  if (!x || !ax) {
+    set_error("null vector");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+  const Grid &g = ctx->g;
+  std::vector<double> pad(g.f, 0.0);
+  for (size_t row = 0; row < (size_t)g.nx * g.ny; row++)
+    memcpy(&pad[row * g.nz_], x + row * g.nz, g.nz * sizeof(double));
+  upload(ctx, ctx->fields[KAPSEL_WORK3], pad.data());
+  poisson_apply(ctx, ctx->fields[KAPSEL_WORK3], ctx->fields[KAPSEL_WORK4]);
+  download(ctx, pad.data(), ctx->fields[KAPSEL_WORK4]);
+  for (size_t row = 0; row < (size_t)g.nx * g.ny; row++)
+    memcpy(ax + row * g.nz, &pad[row * g.nz_], g.nz * sizeof(double));
+  API_END
+}
+
+int kapsel_poisson_rhs(kapsel_ctx *ctx, const double e0[3]) {
+  API_BEGIN
+  check_ctx(ctx);
+  if (!e0) {
+    set_error("null external field");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+  poisson_rhs(ctx, e0);
+  API_END
+}
+
+int kapsel_poisson_solve(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int precond, int *iters,
+                         double *relres) {
+  API_BEGIN
+  check_ctx(ctx);
+  if (!e0 || !iters || !relres || !(tol > 0) || maxit < 1 ||
+      (precond != KAPSEL_PRECOND_NONE && precond != KAPSEL_PRECOND_SPECTRAL)) {
+    set_error("invalid poisson_solve arguments");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+  poisson_solve(ctx, e0, tol, maxit, precond, iters, relres);
+  if (!(*relres < tol)) {
+    set_error("poisson_solve stopped at relative residual %.3e after %d iterations", *relres, *iters);
+    return KAPSEL_ERR_NOCONV;
+  }
   API_END
 }
 

@@ -107,7 +107,13 @@ struct TimingRecord {
 // ---------------------------------------------------------------- context
 struct Twiddles {
   void *x = nullptr, *y = nullptr, *zh = nullptr, *zrot = nullptr;  // V2<T> tables
+  void *shx = nullptr, *shy = nullptr, *shz = nullptr;  // staggered shifts exp(i dx k/2)
 };
+
+template <typename T>
+constexpr int prec_of() {
+  return sizeof(T) == 8 ? KAPSEL_FP64 : KAPSEL_FP32;
+}
 
 }  // namespace kapsel
 
@@ -124,7 +130,15 @@ struct kapsel_ctx {
   int *d_sekibun = nullptr;       // np_domain * 3 ints
   int np_domain = 0;
   std::vector<int> sekibun;       // host copy
-  kapsel::Twiddles tw;
+  kapsel::Twiddles tw[2];          // [KAPSEL_FP64], [KAPSEL_FP32]
+  // Poisson solver storage, allocated on first use: operator scratch of the
+  // context precision, and FP64 shadow fields for the iterative refinement
+  // of an FP32 context
+  void *pscratch[4] = {};
+  double *p64[8] = {};            // eps, x, b, r, 4 operator scratch
+  double *d_partial = nullptr;    // block partial sums (reductions)
+  double *d_scal = nullptr;       // CG scalars
+  int n_partial = 0;
   // derived physics constants (Set_global_parameters, input.cxx:287-430)
   double nu, irho, hxi, mass, imass, moi, imoi, a_r_cutoff;
   // kernel timing
@@ -158,9 +172,13 @@ inline unsigned blocks_for(size_t n, unsigned threads) {
 // FFT (fft.hip)
 void init_twiddles(kapsel_ctx *ctx);
 void free_twiddles(kapsel_ctx *ctx);
-// nf fields transformed together (1..5)
+// nf fields transformed together (1..5), of the context precision or of T
 void fft_r2c(kapsel_ctx *ctx, void *const *fields, int nf);
 void fft_c2r(kapsel_ctx *ctx, void *const *fields, int nf);
+template <typename T>
+void fft_r2c_t(kapsel_ctx *ctx, void *const *fields, int nf);
+template <typename T>
+void fft_c2r_t(kapsel_ctx *ctx, void *const *fields, int nf);
 
 // k-space and real-space pointwise (kspace.hip)
 void truncate(kapsel_ctx *ctx, void *const *fields, int nf);
@@ -180,6 +198,16 @@ void make_phi_particle_sum(kapsel_ctx *ctx);
 void make_u_particle_sum(kapsel_ctx *ctx);
 void make_phi_particle(kapsel_ctx *ctx, void *field);
 void make_phi_sin(kapsel_ctx *ctx, void *field);
+template <typename T>
+void make_dielectric_field_t(kapsel_ctx *ctx, T *eps, T *phi_sin);
+
+// dielectric Poisson (poisson.hip)
+void dielectric_field(kapsel_ctx *ctx);
+void poisson_apply(kapsel_ctx *ctx, const void *x, void *ax);
+void poisson_rhs(kapsel_ctx *ctx, const double e0[3]);
+void poisson_solve(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int precond, int *iters,
+                   double *relres);
+void poisson_free(kapsel_ctx *ctx);
 void calc_f_hydro(kapsel_ctx *ctx, const kapsel_time &t);
 void force(kapsel_ctx *ctx);
 void md_velocity_integrate(kapsel_ctx *ctx, const kapsel_time &t);

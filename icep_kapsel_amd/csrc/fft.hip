@@ -288,13 +288,33 @@ void *make_table(int n, int count) {
   return d;
 }
 
+// exp(i dx wave k / 2) for the signed wavenumbers of one axis (Shift_x/y/z
+// of Init_K, fft_wrapper.cxx:83-85); count = n (x, y) or n/2+1 (z)
+template <typename T>
+void *make_shift(int n, int count) {
+  std::vector<V2<T>> h(count);
+  for (int i = 0; i < count; i++) {
+    int k = (count == n && i > n / 2) ? i - n : i;
+    long double a = 3.14159265358979323846264338327950288L * (long double)k / (long double)n;
+    h[i] = V2<T>{(T)cosl(a), (T)sinl(a)};
+  }
+  void *d = nullptr;
+  KAPSEL_HIP(hipMalloc(&d, sizeof(V2<T>) * count));
+  KAPSEL_HIP(hipMemcpy(d, h.data(), sizeof(V2<T>) * count, hipMemcpyHostToDevice));
+  return d;
+}
+
 template <typename T>
 void init_tw_t(kapsel_ctx *ctx) {
   const Grid &g = ctx->g;
-  ctx->tw.x = make_table<T>(g.nx, g.nx);
-  ctx->tw.y = make_table<T>(g.ny, g.ny);
-  ctx->tw.zh = make_table<T>(g.nz / 2, g.nz / 2);
-  ctx->tw.zrot = make_table<T>(g.nz, g.nz / 2 + 1);
+  Twiddles &tw = ctx->tw[prec_of<T>()];
+  tw.x = make_table<T>(g.nx, g.nx);
+  tw.y = make_table<T>(g.ny, g.ny);
+  tw.zh = make_table<T>(g.nz / 2, g.nz / 2);
+  tw.zrot = make_table<T>(g.nz, g.nz / 2 + 1);
+  tw.shx = make_shift<T>(g.nx, g.nx);
+  tw.shy = make_shift<T>(g.ny, g.ny);
+  tw.shz = make_shift<T>(g.nz, g.nz / 2 + 1);
 }
 
 constexpr int lines_per_block(size_t elem) { return elem == 8 ? 16 : 32; }
@@ -333,13 +353,14 @@ void launch_z(kapsel_ctx *ctx, const FieldPtrs<T> &fp, int nf, bool fwd) {
   dim3 grid((unsigned)((rows + RPB - 1) / RPB), nf);
   dim3 block(RPB * FS<NZ / 2>::R2);
   LaunchTimer lt(ctx, KAPSEL_K_FFT_Z, 2.0 * nf * g.f * sizeof(T));
+  const Twiddles &tw = ctx->tw[prec_of<T>()];
   if (fwd) {
     hipLaunchKernelGGL((fft_z_r2c<T, NZ, RPB>), grid, block, 0, ctx->stream, fp, rows, g.nz_,
-                       (const V2<T> *)ctx->tw.zh, (const V2<T> *)ctx->tw.zrot);
+                       (const V2<T> *)tw.zh, (const V2<T> *)tw.zrot);
   } else {
     T scale = (T)(1.0 / ((double)g.nx * g.ny * g.nz));
     hipLaunchKernelGGL((fft_z_c2r<T, NZ, RPB>), grid, block, 0, ctx->stream, fp, rows, g.nz_,
-                       (const V2<T> *)ctx->tw.zh, (const V2<T> *)ctx->tw.zrot, scale);
+                       (const V2<T> *)tw.zh, (const V2<T> *)tw.zrot, scale);
   }
   KAPSEL_LAUNCH_CHECK();
 }
@@ -363,51 +384,61 @@ FieldPtrs<T> pack(void *const *fields, int nf) {
   return fp;
 }
 
+}  // namespace
+
 template <typename T>
-void r2c_t(kapsel_ctx *ctx, void *const *fields, int nf) {
+void fft_r2c_t(kapsel_ctx *ctx, void *const *fields, int nf) {
   const Grid &g = ctx->g;
+  const Twiddles &tw = ctx->tw[prec_of<T>()];
   FieldPtrs<T> fp = pack<T>(fields, nf);
   z_dispatch<T>(ctx, fp, nf, true);
   // y lines: q = i*h + kz -> offset i*NY*h + kz, stride h
   lines_dispatch<T, -1>(ctx, KAPSEL_K_FFT_Y, g.ny, fp, nf, (long long)g.nx * g.h, g.h, (long long)g.ny * g.h,
-                        g.h, ctx->tw.y);
+                        g.h, tw.y);
   // x lines: q = j*h + kz -> offset q, stride NY*h
   long long plane = (long long)g.ny * g.h;
-  lines_dispatch<T, -1>(ctx, KAPSEL_K_FFT_X, g.nx, fp, nf, plane, plane, 0, plane, ctx->tw.x);
+  lines_dispatch<T, -1>(ctx, KAPSEL_K_FFT_X, g.nx, fp, nf, plane, plane, 0, plane, tw.x);
 }
 
 template <typename T>
-void c2r_t(kapsel_ctx *ctx, void *const *fields, int nf) {
+void fft_c2r_t(kapsel_ctx *ctx, void *const *fields, int nf) {
   const Grid &g = ctx->g;
+  const Twiddles &tw = ctx->tw[prec_of<T>()];
   FieldPtrs<T> fp = pack<T>(fields, nf);
   long long plane = (long long)g.ny * g.h;
-  lines_dispatch<T, +1>(ctx, KAPSEL_K_FFT_X, g.nx, fp, nf, plane, plane, 0, plane, ctx->tw.x);
+  lines_dispatch<T, +1>(ctx, KAPSEL_K_FFT_X, g.nx, fp, nf, plane, plane, 0, plane, tw.x);
   lines_dispatch<T, +1>(ctx, KAPSEL_K_FFT_Y, g.ny, fp, nf, (long long)g.nx * g.h, g.h, (long long)g.ny * g.h,
-                        g.h, ctx->tw.y);
+                        g.h, tw.y);
   z_dispatch<T>(ctx, fp, nf, false);
 }
 
-}  // namespace
+template void fft_r2c_t<double>(kapsel_ctx *, void *const *, int);
+template void fft_r2c_t<float>(kapsel_ctx *, void *const *, int);
+template void fft_c2r_t<double>(kapsel_ctx *, void *const *, int);
+template void fft_c2r_t<float>(kapsel_ctx *, void *const *, int);
 
+// tables of both precisions: an FP32 context refines its Poisson solve in FP64
 void init_twiddles(kapsel_ctx *ctx) {
-  if (ctx->prec == KAPSEL_FP64) init_tw_t<double>(ctx);
-  else init_tw_t<float>(ctx);
+  init_tw_t<double>(ctx);
+  init_tw_t<float>(ctx);
 }
 
 void free_twiddles(kapsel_ctx *ctx) {
-  for (void *p : {ctx->tw.x, ctx->tw.y, ctx->tw.zh, ctx->tw.zrot})
-    if (p) (void)hipFree(p);
-  ctx->tw = Twiddles{};
+  for (Twiddles &tw : ctx->tw) {
+    for (void *p : {tw.x, tw.y, tw.zh, tw.zrot, tw.shx, tw.shy, tw.shz})
+      if (p) (void)hipFree(p);
+    tw = Twiddles{};
+  }
 }
 
 void fft_r2c(kapsel_ctx *ctx, void *const *fields, int nf) {
-  if (ctx->prec == KAPSEL_FP64) r2c_t<double>(ctx, fields, nf);
-  else r2c_t<float>(ctx, fields, nf);
+  if (ctx->prec == KAPSEL_FP64) fft_r2c_t<double>(ctx, fields, nf);
+  else fft_r2c_t<float>(ctx, fields, nf);
 }
 
 void fft_c2r(kapsel_ctx *ctx, void *const *fields, int nf) {
-  if (ctx->prec == KAPSEL_FP64) c2r_t<double>(ctx, fields, nf);
-  else c2r_t<float>(ctx, fields, nf);
+  if (ctx->prec == KAPSEL_FP64) fft_c2r_t<double>(ctx, fields, nf);
+  else fft_c2r_t<float>(ctx, fields, nf);
 }
 
 }  // namespace kapsel

@@ -25,6 +25,7 @@ namespace {
 
 struct Phys {
   double dx, idx, radius, hxi, xi;
+  double eps_f, eps_p, dcst;  // eps_fluid, eps_particle_top, Dielectric_cst
   double l[3];
   int n[3];
   int np;
@@ -78,6 +79,8 @@ constexpr int TX = 4, TY = 4, TZ = 16, TP = TX * TY * TZ;
 // MODE 1: unclamped smooth-profile sum (Make_phi_particle)
 // MODE 2: unclamped compact-sine sum (Make_phi_sin)
 // MODE 3: phi*u_p with phi_sum read back (Make_u_particle_sum)
+// MODE 4: compact-sine sum and the permittivity built from it
+//         (Make_dielectric_field non-Janus branch, operate_dielectric.cxx:298-307)
 template <typename T, int MODE>
 __global__ __launch_bounds__(TP) void points_kernel(const double *__restrict__ ps, T *__restrict__ out0,
                                                     T *__restrict__ out1, T *__restrict__ out2,
@@ -152,7 +155,7 @@ __global__ __launch_bounds__(TP) void points_kernel(const double *__restrict__ p
       const double x2[3] = {cand[0][q], cand[1][q], cand[2][q]};
       double x12[3];
       const double r2 = min_image(xg, x2, p, x12);
-      if constexpr (MODE == 2) {
+      if constexpr (MODE == 2 || MODE == 4) {
         acc0 += phi_compact_sin(sqrt(r2), p);
       } else {
         if (r2 >= rmax * rmax) continue;  // H(a + xi/2 - r) = 0 there: phi = 0
@@ -184,6 +187,9 @@ __global__ __launch_bounds__(TP) void points_kernel(const double *__restrict__ p
     out0[im] = (T)acc0;
     out1[im] = (T)acc1;
     out2[im] = (T)acc2;
+  } else if constexpr (MODE == 4) {
+    out0[im] = (T)acc0;
+    out1[im] = (T)(p.eps_f * p.dcst + (p.eps_p - p.eps_f) * p.dcst * acc0);
   } else {
     out0[im] = (T)acc0;
   }
@@ -223,8 +229,12 @@ __global__ __launch_bounds__(256) void hydro_gather_kernel(double *__restrict__ 
     xp[d] = pval(ps, KAPSEL_P_X, d, n, p.np);
     vp[d] = pval(ps, KAPSEL_P_V, d, n, p.np);
     om[d] = pval(ps, KAPSEL_P_OMEGA, d, n, p.np);
-    // Particle_cell (make_phi.h:120-148)
+    // Particle_cell (make_phi.h:120-148); positions are validated to [0, L)
+    // on upload and wrapped by md_position, the clamp only keeps indexing
+    // in bounds
     double q = xp[d] * p.idx;
+    if (!(q >= 0.0)) q = 0.0;
+    if (q > (double)(p.n[d] - 1) + 0.999999999) q = (double)(p.n[d] - 1);
     x_int[d] = (int)q;
     residue[d] = (q - (double)x_int[d]) * p.dx;
   }
@@ -237,7 +247,7 @@ __global__ __launch_bounds__(256) void hydro_gather_kernel(double *__restrict__ 
 #pragma unroll
     for (int d = 0; d < 3; d++) {
       const int cell = cells[3 * mesh + d];
-      rm[d] = (x_int[d] + cell + p.n[d]) % p.n[d];
+      rm[d] = ((x_int[d] + cell) % p.n[d] + p.n[d]) % p.n[d];
       r[d] = (double)cell * p.dx - residue[d];
       x[d] = rm[d] * p.dx;
     }
@@ -431,6 +441,9 @@ Phys phys(kapsel_ctx *ctx) {
   p.n[2] = g.nz;
   p.np = ctx->P.n_particles;
   p.rotation = ctx->P.rotation;
+  p.eps_f = ctx->P.eps_fluid;
+  p.eps_p = ctx->P.eps_particle;
+  p.dcst = ctx->P.dielectric_cst;
   return p;
 }
 
@@ -438,7 +451,7 @@ template <typename T, int MODE>
 void launch_points(kapsel_ctx *ctx, void *o0, void *o1, void *o2, const void *phi_sum) {
   const Grid &g = ctx->g;
   unsigned nb = (unsigned)((g.nx / TX) * (g.ny / TY) * (g.nz / TZ));
-  const double fields_moved = MODE == 0 ? 2.0 : MODE == 3 ? 4.0 : 1.0;
+  const double fields_moved = (MODE == 0 || MODE == 4) ? 2.0 : MODE == 3 ? 4.0 : 1.0;
   LaunchTimer lt(ctx, KAPSEL_K_PARTICLE, fields_moved * g.nr * sizeof(T));
   hipLaunchKernelGGL((points_kernel<T, MODE>), dim3(nb), dim3(TP), 0, ctx->stream, ctx->d_ps, (T *)o0, (T *)o1,
                      (T *)o2, (const T *)phi_sum, g, phys(ctx));
@@ -489,6 +502,13 @@ void make_phi_sin(kapsel_ctx *ctx, void *field) {
   if (ctx->prec == KAPSEL_FP64) launch_points<double, 2>(ctx, field, nullptr, nullptr, nullptr);
   else launch_points<float, 2>(ctx, field, nullptr, nullptr, nullptr);
 }
+
+template <typename T>
+void make_dielectric_field_t(kapsel_ctx *ctx, T *eps, T *phi_sin) {
+  launch_points<T, 4>(ctx, phi_sin, eps, nullptr, nullptr);
+}
+template void make_dielectric_field_t<double>(kapsel_ctx *, double *, double *);
+template void make_dielectric_field_t<float>(kapsel_ctx *, float *, float *);
 
 void calc_f_hydro(kapsel_ctx *ctx, const kapsel_time &t) {
   if (ctx->P.n_particles <= 0) return;

@@ -146,6 +146,34 @@ class KapselGPU:
     def spm_step(self, t):
         self._call("spm_step", ctypes.byref(t))
 
+    # dielectric Poisson -----------------------------------------------------
+    def make_dielectric_field(self):
+        self._call("dielectric_field")
+
+    def poisson_apply(self, x_name, ax_name):
+        self._call("poisson_apply", abi.FIELD[x_name], abi.FIELD[ax_name])
+
+    def ax_unpadded(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros_like(x)
+        self._call("poisson_apply_host", _dp(x), _dp(out))
+        return out
+
+    def poisson_rhs(self, e0):
+        e = np.ascontiguousarray(e0, dtype=np.float64)
+        self._call("poisson_rhs", _dp(e))
+
+    def poisson_solve(self, e0, tol, maxit, precond=0, check=True):
+        """Returns (iters, relres); raises on non-convergence when check."""
+        e = np.ascontiguousarray(e0, dtype=np.float64)
+        it = ctypes.c_int(0)
+        rr = ctypes.c_double(0.0)
+        rc = self.lib.kapsel_poisson_solve(self.ctx, _dp(e), tol, int(maxit), int(precond), ctypes.byref(it),
+                                           ctypes.byref(rr))
+        if rc != 0 and (check or rc != -4):
+            raise KapselError(f"kapsel_poisson_solve: {rc}: {self.lib.kapsel_last_error().decode()}")
+        return it.value, rr.value
+
     # timing ---------------------------------------------------------------
     def kernel_timing(self, classes):
         mask = 0

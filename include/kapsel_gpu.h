@@ -170,6 +170,30 @@ int kapsel_init_rigidity(kapsel_ctx *ctx);
  * jikan->time += dt_fluid and jikan->ts += 1 (sp_3d_ns.cxx:524-525). */
 int kapsel_spm_step(kapsel_ctx *ctx, kapsel_time *jikan);
 
+/* ---- dielectric Poisson (ICEP) ------------------------------------------- */
+/* Make_dielectric_field, non-Janus branch (operate_dielectric.cxx:270-309,
+ * Make_phi_sin_primitive :78-120): EPSILON = eps_f Dc + (eps_p - eps_f) Dc
+ * sum_i phi_sin_i, the compact-sine sum left in WORK4. */
+int kapsel_dielectric_field(kapsel_ctx *ctx);
+/* Ax (operate_dielectric.cxx:485-522) = D E G x on device fields (padded
+ * layout), E the face average of EPSILON; x_field != ax_field. */
+int kapsel_poisson_apply(kapsel_ctx *ctx, int x_field, int ax_field);
+/* Ax with the reference's Eigen-side signature: unpadded NX*NY*NZ host
+ * vectors (MatrixReplacement mv_function, LinearOperator.h:32). */
+int kapsel_poisson_apply_host(kapsel_ctx *ctx, const double *x, double *ax);
+/* Build_rhs (operate_dielectric.cxx:524-540): RHS = D(eps_face E0) - RHO_FREE. */
+int kapsel_poisson_rhs(kapsel_ctx *ctx, const double e0[3]);
+/* Charge_field2potential_dielectric_LO (operate_dielectric.cxx:542-572):
+ * builds RHS, solves A POTENTIAL = RHS warm-started from POTENTIAL until
+ * ||RHS - A x|| / ||RHS|| < tol, at most maxit iterations, and re-solves
+ * from zero when maxit is reached (:563-570).  Conjugate gradients on -A
+ * (the reference uses BiCGSTAB; A is symmetric negative semidefinite).
+ * precond: enum kapsel_precond.  An FP32 context refines in FP64: *relres
+ * is always an FP64 residual.  Returns KAPSEL_ERR_NOCONV (with *iters and
+ * *relres set) when tol is not reached. */
+int kapsel_poisson_solve(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int precond, int *iters,
+                         double *relres);
+
 /* ---- kernel timing (HIP events on the context stream) --------------------- */
 enum kapsel_kernel_class {
   KAPSEL_K_FFT_Z = 0,   /* z-direction r2c/c2r pass */

@@ -864,6 +864,269 @@ struct Oracle {
     jikan.ts += 1;
   }
 
+  /* ------------------------------------------------ dielectric Poisson */
+  std::vector<double> pw[5];  /* padded scratch of the operator */
+  double *scratch(int i) {
+    if (pw[i].size() != F) pw[i].assign(F, 0.0);
+    return pw[i].data();
+  }
+  /* extract_id (periodic_boundary.h:21-27) */
+  inline size_t extract_id(int i, int j, int k) const {
+    int ii = (i + NX) % NX, jj = (j + NY) % NY, kk = (k + NZ) % NZ;
+    return ((size_t)ii * NY + jj) * NZ_ + kk;
+  }
+  /* Make_dielectric_field, non-Janus branch (operate_dielectric.cxx:270-309) */
+  void make_dielectric_field(double *eps, double *phi_p) {
+    reset(phi_p);
+    reset(eps);
+    make_phi_sin(phi_p);
+    const double ef = P.eps_fluid, ep = P.eps_particle, dc = P.dielectric_cst;
+#pragma omp parallel for
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++)
+        for (int k = 0; k < NZ; k++) {
+          size_t im = ((size_t)i * NY + j) * NZ_ + k;
+          eps[im] = ef * dc + (ep - ef) * dc * phi_p[im];
+        }
+  }
+  /* Interpolate_scalar_on_staggered_grid (operate_dielectric.cxx:57-76) */
+  void interp_staggered(double **vec, const double *s, const double *factor) {
+    for (int d = 0; d < 3; d++) reset(vec[d]);
+#pragma omp parallel for
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++)
+        for (int k = 0; k < NZ; k++) {
+          size_t im = ((size_t)i * NY + j) * NZ_ + k;
+          double sn = 0.5 * s[im];
+          vec[0][im] = (0.5 * s[extract_id(i + 1, j, k)] + sn) * factor[0];
+          vec[1][im] = (0.5 * s[extract_id(i, j + 1, k)] + sn) * factor[1];
+          vec[2][im] = (0.5 * s[extract_id(i, j, k + 1)] + sn) * factor[2];
+        }
+  }
+  /* Shift_d = exp(i DX WAVE_d k_d / 2) (fft_wrapper.cxx:83-85) */
+  inline cplx shift(double wave, int kint) const { return std::exp(cplx(0, 0.5 * DX * wave * (double)kint)); }
+  /* A_k2da_k (fft_wrapper.h:751-755) then Shift_vec_fw_imag (fft_wrapper.h:841-858) */
+  void grad_shift_k(const double *a, double **da) {
+#pragma omp parallel for
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++)
+        for (int k = 0; k < HNZ_; k++) {
+          size_t im = ((size_t)i * NY + j) * NZ_ + 2 * k;
+          int kk[3] = {KX(i), KY(j), k};
+          double wv[3] = {WAVE_X, WAVE_Y, WAVE_Z};
+          for (int d = 0; d < 3; d++) {
+            double wn = kk[d] * wv[d];
+            cplx g(-wn * a[im + 1], wn * a[im]);
+            g *= shift(wv[d], kk[d]);
+            da[d][im] = g.real();
+            da[d][im + 1] = g.imag();
+          }
+        }
+  }
+  /* U_k2divergence_k_shift (fft_wrapper.h:879-900) */
+  void div_shift_k(double **uk, double *out) {
+#pragma omp parallel for
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++)
+        for (int k = 0; k < HNZ_; k++) {
+          size_t im = ((size_t)i * NY + j) * NZ_ + 2 * k;
+          int kk[3] = {KX(i), KY(j), k};
+          double wv[3] = {WAVE_X, WAVE_Y, WAVE_Z};
+          cplx acc(0, 0);
+          for (int d = 0; d < 3; d++) {
+            cplx ks = (double)kk[d] * wv[d] * std::conj(shift(wv[d], kk[d]));
+            acc += cplx(uk[d][im], uk[d][im + 1]) * ks;
+          }
+          acc *= cplx(0, 1);
+          out[im] = acc.real();
+          out[im + 1] = acc.imag();
+        }
+  }
+  /* Ax (operate_dielectric.cxx:485-522) on padded fields; reads EPSILON */
+  void ax(const double *x, double *out) {
+    double *w1 = scratch(0);
+    double *w3[3] = {scratch(1), scratch(2), scratch(3)};
+    reset(w1);
+    for (int d = 0; d < 3; d++) reset(w3[d]);
+#pragma omp parallel for
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++)
+        for (int k = 0; k < NZ; k++) {
+          size_t im = ((size_t)i * NY + j) * NZ_ + k;
+          w1[im] = x[im];
+        }
+    fft_r2c(w1);
+    grad_shift_k(w1, w3);
+    for (int d = 0; d < 3; d++) fft_c2r(w3[d]);
+    const double *eps = fld(KAPSEL_EPSILON);
+#pragma omp parallel for
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++)
+        for (int k = 0; k < NZ; k++) {
+          size_t im = ((size_t)i * NY + j) * NZ_ + k;
+          double en = eps[im];
+          w3[0][im] *= 0.5 * (eps[extract_id(i + 1, j, k)] + en);
+          w3[1][im] *= 0.5 * (eps[extract_id(i, j + 1, k)] + en);
+          w3[2][im] *= 0.5 * (eps[extract_id(i, j, k + 1)] + en);
+        }
+    for (int d = 0; d < 3; d++) fft_r2c(w3[d]);
+    div_shift_k(w3, w1);
+    fft_c2r(w1);
+#pragma omp parallel for
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++)
+        for (int k = 0; k < NZ_; k++) {
+          size_t im = ((size_t)i * NY + j) * NZ_ + k;
+          out[im] = k < NZ ? w1[im] : 0.0;
+        }
+  }
+  /* Build_rhs (operate_dielectric.cxx:524-540) */
+  void build_rhs(double *rhs, const double *rho, const double *eps, const double *e0) {
+    double *w3[3] = {scratch(1), scratch(2), scratch(3)};
+    reset(rhs);
+    interp_staggered(w3, eps, e0);
+    for (int d = 0; d < 3; d++) fft_r2c(w3[d]);
+    div_shift_k(w3, rhs);
+    fft_c2r(rhs);
+#pragma omp parallel for
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++)
+        for (int k = 0; k < NZ; k++) {
+          size_t im = ((size_t)i * NY + j) * NZ_ + k;
+          rhs[im] -= rho[im];
+        }
+  }
+  double dot(const double *a, const double *b) {
+    double s = 0.0;
+#pragma omp parallel for reduction(+ : s)
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++)
+        for (int k = 0; k < NZ; k++) {
+          size_t im = ((size_t)i * NY + j) * NZ_ + k;
+          s += a[im] * b[im];
+        }
+    return s;
+  }
+  void axpy(double a, const double *x, double *y) {
+#pragma omp parallel for
+    for (size_t i = 0; i < F; i++) y[i] += a * x[i];
+  }
+  /* spectral preconditioner z = IFFT(FFT(r) / (eps_mean |k|^2)), mean removed */
+  void precond_spectral(const double *r, double *z, double eps_mean) {
+    memcpy(z, r, F * sizeof(double));
+    fft_r2c(z);
+#pragma omp parallel for
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++)
+        for (int k = 0; k < NZ_; k++) {
+          size_t im = ((size_t)i * NY + j) * NZ_ + k;
+          double k2 = K2(i, j, k);
+          z[im] = k2 > 0 ? z[im] / (eps_mean * k2) : 0.0;
+        }
+    fft_c2r(z);
+  }
+  /* Conjugate gradients on -A x = -b (A = D E G is symmetric negative
+   * semidefinite with the constants as null space); stop when
+   * ||b - A x|| / ||b|| < tol (Eigen's criterion, operate_dielectric.cxx:559-562) */
+  int cg_solve(double *x, const double *b, double tol, int maxit, int precond, double *relres) {
+    double *r = scratch(4);
+    std::vector<double> zv(F), pv(F), apv(F);
+    double *z = zv.data(), *p = pv.data(), *ap = apv.data();
+    double bnorm = sqrt(dot(b, b));
+    if (bnorm == 0.0) {
+      reset(x);
+      *relres = 0.0;
+      return 0;
+    }
+    double eps_mean = 0.0;
+    if (precond) eps_mean = dot(fld(KAPSEL_EPSILON), scratch_ones()) / ((double)NX * NY * NZ);
+    ax(x, ap);
+    for (size_t i = 0; i < F; i++) r[i] = ap[i] - b[i];  /* r = -b - (-A x) */
+    auto apply_m = [&](const double *rr, double *zz) {
+      if (precond) precond_spectral(rr, zz, eps_mean);
+      else memcpy(zz, rr, F * sizeof(double));
+    };
+    apply_m(r, z);
+    memcpy(p, z, F * sizeof(double));
+    double rz = dot(r, z);
+    double rn = sqrt(dot(r, r));
+    int it = 0;
+    while (rn / bnorm >= tol && it < maxit) {
+      ax(p, ap);
+      for (size_t i = 0; i < F; i++) ap[i] = -ap[i];
+      double alpha = rz / dot(p, ap);
+      axpy(alpha, p, x);
+      axpy(-alpha, ap, r);
+      apply_m(r, z);
+      double rz_new = dot(r, z);
+      double beta = rz_new / rz;
+      rz = rz_new;
+#pragma omp parallel for
+      for (size_t i = 0; i < F; i++) p[i] = z[i] + beta * p[i];
+      rn = sqrt(dot(r, r));
+      it++;
+    }
+    /* true residual */
+    ax(x, ap);
+    for (size_t i = 0; i < F; i++) r[i] = b[i] - ap[i];
+    *relres = sqrt(dot(r, r)) / bnorm;
+    return it;
+  }
+  std::vector<double> ones;
+  const double *scratch_ones() {
+    if (ones.size() != F) ones.assign(F, 1.0);
+    return ones.data();
+  }
+  /* BiCGSTAB with identity preconditioner, the reference's solver
+   * (operate_dielectric.cxx:556-562): van der Vorst's algorithm with a
+   * restart when r0.r vanishes, as Eigen implements it */
+  int bicgstab_solve(double *x, const double *b, double tol, int maxit, double *relres) {
+    std::vector<double> rv(F), r0v(F), pv(F, 0.0), vv(F, 0.0), sv(F), tv(F);
+    double *r = rv.data(), *r0 = r0v.data(), *p = pv.data(), *v = vv.data(), *sq = sv.data(), *t = tv.data();
+    ax(x, t);
+    for (size_t i = 0; i < F; i++) r[i] = b[i] - t[i];
+    memcpy(r0, r, F * sizeof(double));
+    double rhs2 = dot(b, b);
+    if (rhs2 == 0.0) {
+      reset(x);
+      *relres = 0.0;
+      return 0;
+    }
+    double r0_sq = dot(r0, r0);
+    double rho = 1, alpha = 1, w = 1;
+    const double tol2 = tol * tol * rhs2, eps2 = 1e-32;
+    int it = 0, restarts = 0;
+    while (dot(r, r) > tol2 && it < maxit) {
+      double rho_old = rho;
+      rho = dot(r0, r);
+      if (fabs(rho) < eps2 * r0_sq) {
+        ax(x, t);
+        for (size_t i = 0; i < F; i++) r[i] = b[i] - t[i];
+        memcpy(r0, r, F * sizeof(double));
+        rho = r0_sq = dot(r, r);
+        if (restarts++ == 0) it = 0;
+      }
+      double beta = (rho / rho_old) * (alpha / w);
+#pragma omp parallel for
+      for (size_t i = 0; i < F; i++) p[i] = r[i] + beta * (p[i] - w * v[i]);
+      ax(p, v);
+      alpha = rho / dot(r0, v);
+#pragma omp parallel for
+      for (size_t i = 0; i < F; i++) sq[i] = r[i] - alpha * v[i];
+      ax(sq, t);
+      double tt = dot(t, t);
+      w = tt > 0 ? dot(t, sq) / tt : 0.0;
+#pragma omp parallel for
+      for (size_t i = 0; i < F; i++) {
+        x[i] += alpha * p[i] + w * sq[i];
+        r[i] = sq[i] - w * t[i];
+      }
+      it++;
+    }
+    *relres = sqrt(dot(r, r) / rhs2);
+    return it;
+  }
+
   /* main's initial rigidity projection (sp_3d_ns.cxx:431-447) */
   void init_rigidity() {
     reset(fld(KAPSEL_PHI));
@@ -991,6 +1254,44 @@ int oracle_force(void *h) { O->force(); return 0; }
 int oracle_md_position(void *h, const kapsel_time *t) { O->md_position(*t); return 0; }
 int oracle_md_velocity(void *h, const kapsel_time *t) { O->md_velocity(*t); return 0; }
 int oracle_make_f_particle_dt_sole(void *h) { O->make_f_particle_dt_sole_add(); return 0; }
+int oracle_make_dielectric_field(void *h) {
+  O->make_dielectric_field(O->fld(KAPSEL_EPSILON), O->fld(KAPSEL_WORK4));
+  return 0;
+}
+int oracle_poisson_apply(void *h, int fx, int fax) {
+  O->ax(O->fld(fx), O->fld(fax));
+  return 0;
+}
+/* Ax on unpadded NX*NY*NZ vectors, the reference's Eigen-side signature */
+int oracle_ax_unpadded(void *h, const double *x, double *ax) {
+  std::vector<double> xp(O->F, 0.0), yp(O->F, 0.0);
+  for (int i = 0; i < O->NX; i++)
+    for (int j = 0; j < O->NY; j++)
+      for (int k = 0; k < O->NZ; k++) xp[((size_t)i * O->NY + j) * O->NZ_ + k] = x[((size_t)i * O->NY + j) * O->NZ + k];
+  O->ax(xp.data(), yp.data());
+  for (int i = 0; i < O->NX; i++)
+    for (int j = 0; j < O->NY; j++)
+      for (int k = 0; k < O->NZ; k++) ax[((size_t)i * O->NY + j) * O->NZ + k] = yp[((size_t)i * O->NY + j) * O->NZ_ + k];
+  return 0;
+}
+int oracle_poisson_rhs(void *h, const double *e0) {
+  O->build_rhs(O->fld(KAPSEL_RHS), O->fld(KAPSEL_RHO_FREE), O->fld(KAPSEL_EPSILON), e0);
+  return 0;
+}
+/* solver: 0 = CG, 1 = spectral-preconditioned CG, 2 = BiCGSTAB (reference) */
+int oracle_poisson_solve(void *h, const double *e0, double tol, int maxit, int solver, int *iters, double *relres) {
+  O->build_rhs(O->fld(KAPSEL_RHS), O->fld(KAPSEL_RHO_FREE), O->fld(KAPSEL_EPSILON), e0);
+  double *x = O->fld(KAPSEL_POTENTIAL);
+  const double *b = O->fld(KAPSEL_RHS);
+  int it = solver == 2 ? O->bicgstab_solve(x, b, tol, maxit, relres) : O->cg_solve(x, b, tol, maxit, solver, relres);
+  if (it >= maxit) {
+    /* the reference's fallback: solve again from a zero initial guess (operate_dielectric.cxx:563-570) */
+    O->reset(x);
+    it = solver == 2 ? O->bicgstab_solve(x, b, tol, maxit, relres) : O->cg_solve(x, b, tol, maxit, solver, relres);
+  }
+  *iters = it;
+  return 0;
+}
 int oracle_init_rigidity(void *h) { O->init_rigidity(); return 0; }
 int oracle_spm_step(void *h, kapsel_time *t) { O->spm_step(*t); return 0; }
 #undef O

@@ -48,6 +48,9 @@ def _load():
             "oracle_md_position": [tp], "oracle_md_velocity": [tp],
             "oracle_make_f_particle_dt_sole": [], "oracle_init_rigidity": [],
             "oracle_spm_step": [tp],
+            "oracle_make_dielectric_field": [], "oracle_poisson_apply": [i, i],
+            "oracle_ax_unpadded": [dp, dp], "oracle_poisson_rhs": [dp],
+            "oracle_poisson_solve": [dp, ctypes.c_double, i, i, ip, dp],
         }
         for name, args in protos.items():
             fn = getattr(lib, name)
@@ -191,6 +194,31 @@ class Oracle:
 
     def spm_step(self, t):
         self._call("spm_step", ctypes.byref(t))
+
+    # dielectric Poisson ---------------------------------------------------
+    def make_dielectric_field(self):
+        self._call("make_dielectric_field")
+
+    def poisson_apply(self, x_name, ax_name):
+        self._call("poisson_apply", abi.FIELD[x_name], abi.FIELD[ax_name])
+
+    def ax_unpadded(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros_like(x)
+        self._call("ax_unpadded", _dp(x), _dp(out))
+        return out
+
+    def poisson_rhs(self, e0):
+        e = np.ascontiguousarray(e0, dtype=np.float64)
+        self._call("poisson_rhs", _dp(e))
+
+    def poisson_solve(self, e0, tol, maxit, solver=0):
+        """solver 0 = CG, 1 = spectral-preconditioned CG, 2 = BiCGSTAB; returns (iters, relres)."""
+        e = np.ascontiguousarray(e0, dtype=np.float64)
+        it = ctypes.c_int(0)
+        rr = ctypes.c_double(0.0)
+        self._call("poisson_solve", _dp(e), tol, int(maxit), int(solver), ctypes.byref(it), ctypes.byref(rr))
+        return it.value, rr.value
 
 
 class OouraRef:
