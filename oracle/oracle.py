@@ -243,3 +243,47 @@ class OouraRef:
         nx, ny, nz_ = a.shape
         self.lib.ref_a_k2a(nx, ny, nz_ - 2, _dp(a))
         return a
+
+
+# ---------------------------------------------------------------- network
+def _pnet():
+    lib = _load()
+    if not hasattr(lib, "_pnet_ready"):
+        dp = ctypes.POINTER(ctypes.c_double)
+        lib.pnet_param_count.restype = ctypes.c_long
+        lib.pnet_layers.argtypes = [ctypes.POINTER(ctypes.c_long)]
+        lib.pnet_layers.restype = None
+        lib.pnet_forward.argtypes = [ctypes.c_int, dp, dp, dp]
+        lib.pnet_forward.restype = None
+        lib.pnet_loss_grad.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp]
+        lib.pnet_loss_grad.restype = ctypes.c_double
+        lib._pnet_ready = True
+    return lib
+
+
+def pnet_param_count():
+    return int(_pnet().pnet_param_count())
+
+
+def pnet_layers():
+    out = (ctypes.c_long * (9 * 6))()
+    _pnet().pnet_layers(out)
+    return [tuple(out[6 * i:6 * i + 6]) for i in range(9)]
+
+
+def pnet_forward(n, params, inp):
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    inp = np.ascontiguousarray(inp, dtype=np.float64).reshape(n ** 3, 2)
+    y = np.zeros((n ** 3, 3))
+    _pnet().pnet_forward(n, _dp(params), _dp(inp), _dp(y))
+    return y
+
+
+def pnet_loss_grad(n, params, inp, e, target):
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    inp = np.ascontiguousarray(inp, dtype=np.float64).reshape(n ** 3, 2)
+    e = np.ascontiguousarray(e, dtype=np.float64)
+    t = np.ascontiguousarray(target, dtype=np.float64).reshape(n ** 3)
+    g = np.zeros(pnet_param_count())
+    loss = _pnet().pnet_loss_grad(n, _dp(params), _dp(inp), _dp(e), _dp(t), _dp(g))
+    return loss, g
