@@ -67,6 +67,19 @@ def run_gpu(precision, steps, warmup, timed_class=None):
     }
 
 
+def copy_calibration(k):
+    from icep_kapsel_amd import abi
+    from icep_kapsel_amd.gpu import KapselGPU
+
+    g = KapselGPU(abi.make_params(128, precision=abi.KAPSEL_FP32))
+    for _ in range(k):
+        g.copy_field("WORK1", "WORK0")
+    g.synchronize()
+    nbytes = 128 * 128 * 130 * 4
+    print(f"copy calibration: {k} copies of {nbytes} bytes read + {nbytes} bytes written", flush=True)
+    g.close()
+
+
 def run_cpu(steps):
     from icep_kapsel_amd import abi, workloads
     from oracle import oracle as omod
@@ -91,7 +104,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="K",
+                    help="only run K plain field copies (128^3 FP32, 8.52 MB each way) to calibrate "
+                         "rocprofv3 FETCH_SIZE/WRITE_SIZE, then exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the SPM step at these sizes runs on a single MI355X).", file=sys.stderr)
@@ -101,9 +121,11 @@ def main():
     log(f"bench: fp32 SPM step 128^3 x 200 spheres, warmup {args.warmup}, steps {args.steps}")
     r32 = run_gpu(abi.KAPSEL_FP32, args.steps, args.warmup)
     log(f"bench: fp32 {r32['steps_per_s']:.1f} steps/s; dominant {r32['dominant']}")
-    log("bench: fp64 SPM step")
-    r64 = run_gpu(abi.KAPSEL_FP64, args.steps, args.warmup, timed_class=r32["dominant"])
-    log(f"bench: fp64 {r64['steps_per_s']:.1f} steps/s")
+    r64 = None
+    if not args.no_fp64:
+        log("bench: fp64 SPM step")
+        r64 = run_gpu(abi.KAPSEL_FP64, args.steps, args.warmup, timed_class=r32["dominant"])
+        log(f"bench: fp64 {r64['steps_per_s']:.1f} steps/s")
     cpu = None
     if not args.no_cpu:
         log(f"bench: CPU oracle baseline, {args.cpu_steps} steps after 1 warm-up")
@@ -135,11 +157,11 @@ def main():
                      "kernel": f"class {r32['dominant']}", "avg_launch_ms": avg_ms,
                      "launches": r32["dominant_launches"]},
         "cpu_baseline": cpu,
-        "fp64_steps_per_s": r64["steps_per_s"],
-        "fp64_ms_per_step": r64["ms_per_step"],
-        "fp32_over_fp64": r32["steps_per_s"] / r64["steps_per_s"],
+        "fp64_steps_per_s": r64["steps_per_s"] if r64 else None,
+        "fp64_ms_per_step": r64["ms_per_step"] if r64 else None,
+        "fp32_over_fp64": r32["steps_per_s"] / r64["steps_per_s"] if r64 else None,
         "kernel_class_ms_per_step_fp32": r32["shares_ms"],
-        "kernel_class_ms_per_step_fp64": r64["shares_ms"],
+        "kernel_class_ms_per_step_fp64": r64["shares_ms"] if r64 else None,
     }
     print(json.dumps(out), flush=True)
 

@@ -110,6 +110,7 @@ _PROTOS = {
     "kapsel_get_uk_dc": [_DP],
     "kapsel_set_particles": [_DP],
     "kapsel_get_particles": [_DP],
+    "kapsel_copy_field": [_I, _I],
     "kapsel_fft_r2c": [_I],
     "kapsel_fft_c2r": [_I],
     "kapsel_truncate": [_I],
@@ -132,6 +133,12 @@ _PROTOS = {
     "kapsel_poisson_apply_host": [_DP, _DP],
     "kapsel_poisson_rhs": [_DP],
     "kapsel_poisson_solve": [_DP, _D, _I, _I, _IP, _DP],
+    "kapsel_pnet_init": [ctypes.POINTER(ctypes.c_long)],
+    "kapsel_pnet_set_params": [ctypes.POINTER(ctypes.c_float)],
+    "kapsel_pnet_get_params": [ctypes.POINTER(ctypes.c_float)],
+    "kapsel_pnet_loss_grad": [_D, _DP, _D, _DP, ctypes.POINTER(ctypes.c_float)],
+    "kapsel_pnet_train_step": [_D, _DP, _D, _D, _DP],
+    "kapsel_pnet_warm_start": [_D, _DP, _D],
     "kapsel_kernel_timing": [_I],
     "kapsel_kernel_timing_get": [_I, _DP, _IP, _DP],
 }

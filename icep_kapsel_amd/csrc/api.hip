@@ -111,6 +111,7 @@ static void destroy(kapsel_ctx *ctx) {
   if (ctx->d_sekibun) (void)hipFree(ctx->d_sekibun);
   free_twiddles(ctx);
   poisson_free(ctx);
+  pnet_free(ctx);
   for (auto &r : ctx->timing_pending) {
     (void)hipEventDestroy(r.start);
     (void)hipEventDestroy(r.stop);
@@ -462,6 +463,15 @@ int kapsel_get_particles(kapsel_ctx *ctx, double *state) {
   API_END
 }
 
+int kapsel_copy_field(kapsel_ctx *ctx, int dst, int src) {
+  API_BEGIN
+  check_ctx(ctx);
+  check_field(dst);
+  check_field(src);
+  copy_field(ctx, ctx->fields[dst], ctx->fields[src]);
+  API_END
+}
+
 int kapsel_fft_r2c(kapsel_ctx *ctx, int field) {
   API_BEGIN
   check_ctx(ctx);
@@ -657,6 +667,80 @@ int kapsel_poisson_solve(kapsel_ctx *ctx, const double e0[3], double tol, int ma
     set_error("poisson_solve stopped at relative residual %.3e after %d iterations", *relres, *iters);
     return KAPSEL_ERR_NOCONV;
   }
+  API_END
+}
+
+int kapsel_pnet_init(kapsel_ctx *ctx, long *n_params) {
+  API_BEGIN
+  check_ctx(ctx);
+  pnet_init(ctx);
+  if (n_params) *n_params = pnet_param_count(ctx);
+  API_END
+}
+
+int kapsel_pnet_set_params(kapsel_ctx *ctx, const float *params) {
+  API_BEGIN
+  check_ctx(ctx);
+  if (!params) {
+    set_error("null parameters");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+  pnet_set_params(ctx, params);
+  API_END
+}
+
+int kapsel_pnet_get_params(kapsel_ctx *ctx, float *params) {
+  API_BEGIN
+  check_ctx(ctx);
+  if (!params) {
+    set_error("null parameters");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+  pnet_get_params(ctx, params);
+  API_END
+}
+
+static void check_unit(const double e[3]) {
+  if (!e || !(fabs(e[0] * e[0] + e[1] * e[1] + e[2] * e[2] - 1.0) < 1e-6)) {
+    set_error("field direction must be a unit vector");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+}
+
+int kapsel_pnet_loss_grad(kapsel_ctx *ctx, double beta, const double e_hat[3], double scale, double *loss,
+                          float *grad) {
+  API_BEGIN
+  check_ctx(ctx);
+  check_unit(e_hat);
+  if (!loss || !(scale > 0)) {
+    set_error("null loss or non-positive scale");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+  *loss = pnet_loss_grad(ctx, beta, e_hat, scale, grad);
+  API_END
+}
+
+int kapsel_pnet_train_step(kapsel_ctx *ctx, double beta, const double e_hat[3], double scale, double lr,
+                           double *loss) {
+  API_BEGIN
+  check_ctx(ctx);
+  check_unit(e_hat);
+  if (!loss || !(scale > 0) || !(lr > 0)) {
+    set_error("null loss, non-positive scale or learning rate");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+  *loss = pnet_train_step(ctx, beta, e_hat, scale, lr);
+  API_END
+}
+
+int kapsel_pnet_warm_start(kapsel_ctx *ctx, double beta, const double e0[3], double scale) {
+  API_BEGIN
+  check_ctx(ctx);
+  if (!e0 || !(scale > 0)) {
+    set_error("null field or non-positive scale");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+  pnet_warm_start(ctx, beta, e0, scale);
   API_END
 }
 

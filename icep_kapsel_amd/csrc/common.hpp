@@ -110,6 +110,8 @@ struct Twiddles {
   void *shx = nullptr, *shy = nullptr, *shz = nullptr;  // staggered shifts exp(i dx k/2)
 };
 
+struct Pnet;  // learned warm start (pnet.hip)
+
 template <typename T>
 constexpr int prec_of() {
   return sizeof(T) == 8 ? KAPSEL_FP64 : KAPSEL_FP32;
@@ -139,6 +141,7 @@ struct kapsel_ctx {
   double *d_partial = nullptr;    // block partial sums (reductions)
   double *d_scal = nullptr;       // CG scalars
   int n_partial = 0;
+  kapsel::Pnet *pnet = nullptr;   // learned warm-start network, on first use
   // derived physics constants (Set_global_parameters, input.cxx:287-430)
   double nu, irho, hxi, mass, imass, moi, imoi, a_r_cutoff;
   // kernel timing
@@ -189,6 +192,7 @@ void advection_combine_slaved(kapsel_ctx *ctx, double dt_fluid);
 void solenoidal_uk(kapsel_ctx *ctx, void *const *f3);
 void make_f_nonsole(kapsel_ctx *ctx);
 void add_f_particle(kapsel_ctx *ctx);
+void copy_field(kapsel_ctx *ctx, void *dst, const void *src);
 void copy_convert_in(kapsel_ctx *ctx, void *dst, const double *src_dev);
 void copy_convert_out(kapsel_ctx *ctx, double *dst_dev, const void *src);
 
@@ -208,6 +212,16 @@ void poisson_rhs(kapsel_ctx *ctx, const double e0[3]);
 void poisson_solve(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int precond, int *iters,
                    double *relres);
 void poisson_free(kapsel_ctx *ctx);
+
+// learned warm start (pnet.hip)
+void pnet_init(kapsel_ctx *ctx);
+void pnet_free(kapsel_ctx *ctx);
+long pnet_param_count(kapsel_ctx *ctx);
+void pnet_set_params(kapsel_ctx *ctx, const float *host);
+void pnet_get_params(kapsel_ctx *ctx, float *host);
+double pnet_loss_grad(kapsel_ctx *ctx, double beta, const double e[3], double scale, float *grad_host);
+double pnet_train_step(kapsel_ctx *ctx, double beta, const double e[3], double scale, double lr);
+void pnet_warm_start(kapsel_ctx *ctx, double beta, const double e0[3], double scale);
 void calc_f_hydro(kapsel_ctx *ctx, const kapsel_time &t);
 void force(kapsel_ctx *ctx);
 void md_velocity_integrate(kapsel_ctx *ctx, const kapsel_time &t);

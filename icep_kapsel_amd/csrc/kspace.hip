@@ -376,6 +376,24 @@ static void add_f_particle_t(kapsel_ctx *ctx) {
 }
 void add_f_particle(kapsel_ctx *ctx) { KAPSEL_DISPATCH(ctx, add_f_particle_t, ctx); }
 
+template <typename T>
+__global__ void copy_kernel(T *__restrict__ dst, const T *__restrict__ src, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+// Copy_v1_k (aux_field.h:82): the whole padded field, a plain streaming copy
+void copy_field(kapsel_ctx *ctx, void *dst, const void *src) {
+  LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, 2.0 * ctx->g.f * ctx->elem);
+  if (ctx->prec == KAPSEL_FP64)
+    hipLaunchKernelGGL(copy_kernel<double>, dim3(blocks_for(ctx->g.f, TPB)), dim3(TPB), 0, ctx->stream,
+                       (double *)dst, (const double *)src, ctx->g.f);
+  else
+    hipLaunchKernelGGL(copy_kernel<float>, dim3(blocks_for(ctx->g.f, TPB)), dim3(TPB), 0, ctx->stream,
+                       (float *)dst, (const float *)src, ctx->g.f);
+  KAPSEL_LAUNCH_CHECK();
+}
+
 void copy_convert_in(kapsel_ctx *ctx, void *dst, const double *src_dev) {
   hipLaunchKernelGGL((convert_kernel<float, double>), dim3(blocks_for(ctx->g.f, TPB)), dim3(TPB), 0, ctx->stream,
                      (float *)dst, src_dev, ctx->g.f);

@@ -73,7 +73,29 @@ __device__ __forceinline__ double &pref(double *ps, int f, int d, int n, int np)
   return ps[((size_t)f * 3 + d) * np + n];
 }
 
-constexpr int TX = 4, TY = 4, TZ = 16, TP = TX * TY * TZ;
+constexpr int TP = 256;  // threads per scatter workgroup
+constexpr int TXL = 8;   // points per thread along x
+
+// profile values in the field precision M (float fields need no FP64 math)
+template <typename M>
+__device__ __forceinline__ M hm(M x, M dx) {
+  if (x > (M)0) {
+    M q = dx / x;
+    return exp(-(q * q));
+  }
+  return (M)0;
+}
+template <typename M>
+__device__ __forceinline__ M phi_m(M x, M a, M hxi, M dx) {
+  M dmy = hm<M>(a + hxi - x, dx);
+  return dmy / (dmy + hm<M>(x - a + hxi, dx));
+}
+template <typename M>
+__device__ __forceinline__ M phi_sin_m(M x, M a, M hxi, M xi) {
+  M dmy_x = a - x;
+  if (fabs(dmy_x) <= hxi) return (M)0.5 * sin((M)M_PI * dmy_x / xi) + (M)0.5;
+  return dmy_x > hxi ? (M)1 : (M)0;
+}
 
 // MODE 0: phi_sum and phi = min(phi_sum, 1) (Make_phi_particle_sum)
 // MODE 1: unclamped smooth-profile sum (Make_phi_particle)
@@ -81,52 +103,55 @@ constexpr int TX = 4, TY = 4, TZ = 16, TP = TX * TY * TZ;
 // MODE 3: phi*u_p with phi_sum read back (Make_u_particle_sum)
 // MODE 4: compact-sine sum and the permittivity built from it
 //         (Make_dielectric_field non-Janus branch, operate_dielectric.cxx:298-307)
+// A workgroup owns a tile of 8 x ty x tz points (ty * tz = 256); each
+// thread walks 8 points along x.  The per-point arithmetic runs in the field
+// precision T; displacements are formed in FP64 from the FP64 positions.
 template <typename T, int MODE>
 __global__ __launch_bounds__(TP) void points_kernel(const double *__restrict__ ps, T *__restrict__ out0,
                                                     T *__restrict__ out1, T *__restrict__ out2,
-                                                    const T *__restrict__ phi_sum, Grid g, Phys p) {
+                                                    const T *__restrict__ phi_sum, Grid g, Phys p, int ty,
+                                                    int tz) {
+  using M = T;
   constexpr int NV = MODE == 3 ? 9 : 3;
   __shared__ double cand[NV][TP];
   __shared__ int wcount[TP / 64];
   const int t = threadIdx.x;
-  const int ntx = g.nx / TX, nty = g.ny / TY;
+  const int ntx = g.nx / TXL, nty = g.ny / ty;
   int b = blockIdx.x;
   const int bx = b % ntx;
   b /= ntx;
   const int by = b % nty;
   const int bz = b / nty;
-  const int i = bx * TX + t / (TY * TZ);
-  const int j = by * TY + (t / TZ) % TY;
-  const int k = bz * TZ + t % TZ;
-  const double xg[3] = {i * p.dx, j * p.dx, k * p.dx};
-  // tile box centre and half widths, for the conservative reach test
-  const double cen[3] = {(bx * TX + 0.5 * (TX - 1)) * p.dx, (by * TY + 0.5 * (TY - 1)) * p.dx,
-                         (bz * TZ + 0.5 * (TZ - 1)) * p.dx};
-  const double hw[3] = {0.5 * (TX - 1) * p.dx, 0.5 * (TY - 1) * p.dx, 0.5 * (TZ - 1) * p.dx};
+  const int j = by * ty + t / tz;
+  const int k = bz * tz + t % tz;
+  const int i0 = bx * TXL;
+  const double il[3] = {1.0 / p.l[0], 1.0 / p.l[1], 1.0 / p.l[2]};
+  // tile box centre and half widths for the conservative reach test
+  const double cen[3] = {(i0 + 0.5 * (TXL - 1)) * p.dx, (by * ty + 0.5 * (ty - 1)) * p.dx,
+                         (bz * tz + 0.5 * (tz - 1)) * p.dx};
+  const double hw[3] = {0.5 * (TXL - 1) * p.dx, 0.5 * (ty - 1) * p.dx, 0.5 * (tz - 1) * p.dx};
   const double reach = p.radius + p.hxi + 0.5 * p.dx;
-  const size_t im = ((size_t)i * g.ny + j) * g.nz_ + k;
-  double ps_here = 0.0, psat = 1.0;
-  if constexpr (MODE == 3) {
-    ps_here = (double)phi_sum[im];
-    psat = ps_here > 1.0 ? ps_here : 1.0;
-  }
-  double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
-  const double rmax = p.radius + p.hxi;
+  const M a = (M)p.radius, hxi = (M)p.hxi, xi = (M)p.xi, dxm = (M)p.dx;
+  const M rmax = a + hxi;
+  M acc0[TXL], acc1[TXL], acc2[TXL];
+#pragma unroll
+  for (int x = 0; x < TXL; x++) acc0[x] = acc1[x] = acc2[x] = (M)0;
+  const double yg = j * p.dx, zg = k * p.dx;
   for (int base = 0; base < p.np; base += TP) {
     const int n = base + t;
     bool c = false;
     double xp[3] = {0, 0, 0};
     if (n < p.np) {
-      double s = 0.0;
+      double s2 = 0.0;
 #pragma unroll
       for (int d = 0; d < 3; d++) {
         xp[d] = pval(ps, KAPSEL_P_X, d, n, p.np);
         double dd = xp[d] - cen[d];
-        dd -= p.l[d] * rint(dd / p.l[d]);
+        dd -= p.l[d] * rint(dd * il[d]);
         dd = fabs(dd) - hw[d];
-        if (dd > 0) s += dd * dd;
+        if (dd > 0) s2 += dd * dd;
       }
-      c = s < reach * reach;
+      c = s2 < reach * reach;
     }
     const unsigned long long m = __ballot(c);
     const int wid = t / 64, lane = t % 64;
@@ -152,46 +177,65 @@ __global__ __launch_bounds__(TP) void points_kernel(const double *__restrict__ p
     }
     __syncthreads();
     for (int q = 0; q < total; q++) {
-      const double x2[3] = {cand[0][q], cand[1][q], cand[2][q]};
-      double x12[3];
-      const double r2 = min_image(xg, x2, p, x12);
-      if constexpr (MODE == 2 || MODE == 4) {
-        acc0 += phi_compact_sin(sqrt(r2), p);
-      } else {
+      // min-image displacement particle -> point (Distance0, interaction.h:122-135);
+      // |x12| < a + xi/2 << L/2 wherever phi > 0, so Nint by multiplication is exact there
+      double dy = yg - cand[1][q], dz = zg - cand[2][q];
+      dy -= (double)nint(dy * il[1]) * p.l[1];
+      dz -= (double)nint(dz * il[2]) * p.l[2];
+      const M ry = (M)dy, rz = (M)dz;
+      const M ryz2 = ry * ry + rz * rz;
+      if (ryz2 >= rmax * rmax) continue;
+#pragma unroll
+      for (int x = 0; x < TXL; x++) {
+        double dxv = (i0 + x) * p.dx - cand[0][q];
+        dxv -= (double)nint(dxv * il[0]) * p.l[0];
+        const M rx = (M)dxv;
+        const M r2 = rx * rx + ryz2;
         if (r2 >= rmax * rmax) continue;  // H(a + xi/2 - r) = 0 there: phi = 0
-        const double ph = phi_smooth(sqrt(r2), p);
-        if constexpr (MODE == 3) {
-          // r: particle centre -> point = -x12 (Relative_coord, make_phi.h:162-179)
-          const double r[3] = {-x12[0], -x12[1], -x12[2]};
-          const double dmy_phi = ph / psat;
-          double vr[3] = {0, 0, 0};
-          if (p.rotation) {
-            vr[0] = cand[7][q] * r[2] - cand[8][q] * r[1];
-            vr[1] = cand[8][q] * r[0] - cand[6][q] * r[2];
-            vr[2] = cand[6][q] * r[1] - cand[7][q] * r[0];
-          }
-          acc0 += (cand[3][q] + vr[0]) * dmy_phi;
-          acc1 += (cand[4][q] + vr[1]) * dmy_phi;
-          acc2 += (cand[5][q] + vr[2]) * dmy_phi;
+        const M r = sqrt(r2);
+        if constexpr (MODE == 2 || MODE == 4) {
+          acc0[x] += phi_sin_m<M>(r, a, hxi, xi);
         } else {
-          acc0 += ph;
+          const M ph = phi_m<M>(r, a, hxi, dxm);
+          if constexpr (MODE == 3) {
+            // r vector: particle centre -> point (Relative_coord, make_phi.h:162-179)
+            M vr[3] = {0, 0, 0};
+            if (p.rotation) {
+              const M o0 = (M)cand[6][q], o1 = (M)cand[7][q], o2 = (M)cand[8][q];
+              vr[0] = o1 * rz - o2 * ry;
+              vr[1] = o2 * rx - o0 * rz;
+              vr[2] = o0 * ry - o1 * rx;
+            }
+            const size_t im = ((size_t)(i0 + x) * g.ny + j) * g.nz_ + k;
+            const M psum = (M)phi_sum[im];
+            const M dmy_phi = ph / (psum > (M)1 ? psum : (M)1);
+            acc0[x] += ((M)cand[3][q] + vr[0]) * dmy_phi;
+            acc1[x] += ((M)cand[4][q] + vr[1]) * dmy_phi;
+            acc2[x] += ((M)cand[5][q] + vr[2]) * dmy_phi;
+          } else {
+            acc0[x] += ph;
+          }
         }
       }
     }
     __syncthreads();
   }
-  if constexpr (MODE == 0) {
-    out0[im] = (T)acc0;                           // phi_sum
-    out1[im] = (T)(acc0 < 1.0 ? acc0 : 1.0);      // phi = MIN(phi_sum, 1)
-  } else if constexpr (MODE == 3) {
-    out0[im] = (T)acc0;
-    out1[im] = (T)acc1;
-    out2[im] = (T)acc2;
-  } else if constexpr (MODE == 4) {
-    out0[im] = (T)acc0;
-    out1[im] = (T)(p.eps_f * p.dcst + (p.eps_p - p.eps_f) * p.dcst * acc0);
-  } else {
-    out0[im] = (T)acc0;
+#pragma unroll
+  for (int x = 0; x < TXL; x++) {
+    const size_t im = ((size_t)(i0 + x) * g.ny + j) * g.nz_ + k;
+    if constexpr (MODE == 0) {
+      out0[im] = (T)acc0[x];                                    // phi_sum
+      out1[im] = (T)(acc0[x] < (M)1 ? acc0[x] : (M)1);          // phi = MIN(phi_sum, 1)
+    } else if constexpr (MODE == 3) {
+      out0[im] = (T)acc0[x];
+      out1[im] = (T)acc1[x];
+      out2[im] = (T)acc2[x];
+    } else if constexpr (MODE == 4) {
+      out0[im] = (T)acc0[x];
+      out1[im] = (T)((M)(p.eps_f * p.dcst) + (M)((p.eps_p - p.eps_f) * p.dcst) * acc0[x]);
+    } else {
+      out0[im] = (T)acc0[x];
+    }
   }
 }
 
@@ -339,11 +383,11 @@ __device__ __forceinline__ int nb_count(int a, int b, int lc) {
 
 // Force(): the cell-list LJ of md_force.cxx:16-171 (pairs i>j visited once
 // per neighbour-cell match, each contributing +-f(r)(x_i - x_j)) plus the
-// constant body force (Add_f_gravity, md_force.cxx:261-280).  One thread per
-// particle sums every pair it takes part in; no write conflicts.
-__global__ void force_kernel(double *__restrict__ ps, Phys p, LJ lj) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= p.np) return;
+// constant body force (Add_f_gravity, md_force.cxx:261-280).  One workgroup
+// per particle i: its lanes take the partners j, a wavefront reduction sums
+// them; every particle's force is written by one workgroup only.
+__global__ __launch_bounds__(256) void force_kernel(double *__restrict__ ps, Phys p, LJ lj) {
+  const int i = blockIdx.x;
   double fi[3] = {0, 0, 0};
   if (lj.on) {
     double xi[3];
@@ -353,7 +397,7 @@ __global__ void force_kernel(double *__restrict__ ps, Phys p, LJ lj) {
       xi[d] = pval(ps, KAPSEL_P_X, d, i, p.np);
       ci[d] = int(xi[d] / lj.lc_r);
     }
-    for (int j = 0; j < p.np; j++) {
+    for (int j = threadIdx.x; j < p.np; j += blockDim.x) {
       if (j == i) continue;
       double xj[3];
       int mult = 1;
@@ -372,8 +416,21 @@ __global__ void force_kernel(double *__restrict__ ps, Phys p, LJ lj) {
       }
     }
   }
+  __shared__ double red[3][4];
 #pragma unroll
-  for (int d = 0; d < 3; d++) pref(ps, KAPSEL_P_FR, d, i, p.np) += fi[d] + lj.body[d];
+  for (int d = 0; d < 3; d++) {
+    for (int off = 32; off > 0; off >>= 1) fi[d] += __shfl_down(fi[d], off);
+    if ((threadIdx.x & 63) == 0) red[d][threadIdx.x >> 6] = fi[d];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      double s = 0.0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += red[d][w];
+      pref(ps, KAPSEL_P_FR, d, i, p.np) += s + lj.body[d];
+    }
+  }
 }
 
 // MD_solver_position_Euler/_AB2 (particle_solver.cxx:39-91), PBC (periodic_boundary.h:9-19)
@@ -450,11 +507,12 @@ Phys phys(kapsel_ctx *ctx) {
 template <typename T, int MODE>
 void launch_points(kapsel_ctx *ctx, void *o0, void *o1, void *o2, const void *phi_sum) {
   const Grid &g = ctx->g;
-  unsigned nb = (unsigned)((g.nx / TX) * (g.ny / TY) * (g.nz / TZ));
+  const int tz = g.nz >= 32 ? 32 : 16, ty = TP / tz;
+  unsigned nb = (unsigned)((g.nx / TXL) * (g.ny / ty) * (g.nz / tz));
   const double fields_moved = (MODE == 0 || MODE == 4) ? 2.0 : MODE == 3 ? 4.0 : 1.0;
   LaunchTimer lt(ctx, KAPSEL_K_PARTICLE, fields_moved * g.nr * sizeof(T));
   hipLaunchKernelGGL((points_kernel<T, MODE>), dim3(nb), dim3(TP), 0, ctx->stream, ctx->d_ps, (T *)o0, (T *)o1,
-                     (T *)o2, (const T *)phi_sum, g, phys(ctx));
+                     (T *)o2, (const T *)phi_sum, g, phys(ctx), ty, tz);
   KAPSEL_LAUNCH_CHECK();
 }
 
@@ -532,8 +590,9 @@ void force(kapsel_ctx *ctx) {
   lj.lc_r = cell_length * ctx->g.dx;
   for (int d = 0; d < 3; d++) lj.body[d] = ctx->P.body_force[d];
   LaunchTimer lt(ctx, KAPSEL_K_PARTICLE, 8.0 * KAPSEL_P_NFIELDS * 3 * ctx->P.n_particles);
-  hipLaunchKernelGGL(force_kernel, dim3(blocks_for(ctx->P.n_particles, 64)), dim3(64), 0, ctx->stream, ctx->d_ps,
-                     phys(ctx), lj);
+  const unsigned threads = ctx->P.n_particles > 128 ? 256 : (ctx->P.n_particles > 64 ? 128 : 64);
+  hipLaunchKernelGGL(force_kernel, dim3(ctx->P.n_particles), dim3(threads), 0, ctx->stream, ctx->d_ps, phys(ctx),
+                     lj);
   KAPSEL_LAUNCH_CHECK();
 }
 

@@ -278,4 +278,484 @@ __global__ void loss_kernel(const float *__restrict__ y, const float *__restrict
 
 }  // namespace
 
+namespace {
+
+// ------------------------------------------------------------ host helpers
+// master FP32 weights (oracle layout W[tap][cin][cout], b[cout]) -> forward
+// kernel kf[tap][coutp][cinp] and input-gradient kernel kb[tap][cinp][coutp]
+// in BF16, and the padded FP32 bias
+__global__ void pack_layer_kernel(const float *__restrict__ master, size_t woff, size_t boff, int cin, int cout,
+                                  int cinp, int coutp, bf16 *__restrict__ kf, bf16 *__restrict__ kb,
+                                  float *__restrict__ bias) {
+  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t total = (size_t)27 * cinp * coutp;
+  if (e < (size_t)coutp) bias[e] = (int)e < cout ? master[boff + e] : 0.f;
+  if (e >= total) return;
+  const int tap = (int)(e / ((size_t)cinp * coutp));
+  const int rem = (int)(e % ((size_t)cinp * coutp));
+  const int ci = rem / coutp, co = rem % coutp;
+  const float w = (ci < cin && co < cout) ? master[woff + ((size_t)tap * cin + ci) * cout + co] : 0.f;
+  const bf16 h = f2bf(w);
+  kb[e] = h;                                                 // [tap][ci][co]
+  kf[((size_t)tap * coutp + co) * cinp + ci] = h;            // [tap][co][ci]
+}
+
+// padded dW [27][cinp][coutp] and db [coutp] -> oracle layout
+__global__ void unpack_grad_kernel(const float *__restrict__ dw, const float *__restrict__ db, size_t woff,
+                                   size_t boff, int cin, int cout, int cinp, int coutp, float *__restrict__ grad) {
+  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < (size_t)cout) grad[boff + e] = db[e];
+  if (e >= (size_t)27 * cin * cout) return;
+  const int tap = (int)(e / ((size_t)cin * cout));
+  const int rem = (int)(e % ((size_t)cin * cout));
+  const int ci = rem / cout, co = rem % cout;
+  grad[woff + e] = dw[((size_t)tap * cinp + ci) * coutp + co];
+}
+
+// network input from the compact-sine sum phi (padded field layout, type T):
+// channel 0 = phi, channel 1 = beta phi, channels 2..15 = 0
+template <typename T>
+__global__ void pnet_input_kernel(const T *__restrict__ phi, Grid g, float beta, bf16 *__restrict__ x0) {
+  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= g.nr * 16) return;
+  const int ch = (int)(e & 15);
+  const size_t v = e >> 4;
+  const size_t row = v / (unsigned)g.nz;
+  const float ph = (float)phi[row * g.nz_ + (v - row * g.nz)];
+  x0[e] = f2bf(ch == 0 ? ph : ch == 1 ? beta * ph : 0.f);
+}
+
+// x[im] = scale * sum_d e_d y[v][d]  (the warm start)
+template <typename T>
+__global__ void pnet_output_kernel(const float *__restrict__ y, Grid g, float e0, float e1, float e2, double scale,
+                                   T *__restrict__ x) {
+  size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= g.nr) return;
+  const size_t row = v / (unsigned)g.nz;
+  const double pred = (double)e0 * y[v * 16] + (double)e1 * y[v * 16 + 1] + (double)e2 * y[v * 16 + 2];
+  x[row * g.nz_ + (v - row * g.nz)] = (T)(scale * pred);
+}
+
+// t[v] = x[im] / scale, mean removed later by the loss being shift-sensitive:
+// the solver's potential is defined up to a constant, so the target is taken
+// with its mean subtracted (mean passed in)
+template <typename T>
+__global__ void pnet_target_kernel(const T *__restrict__ x, Grid g, double scale, double mean,
+                                   float *__restrict__ t) {
+  size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= g.nr) return;
+  const size_t row = v / (unsigned)g.nz;
+  t[v] = (float)(((double)x[row * g.nz_ + (v - row * g.nz)] - mean) / scale);
+}
+
+__global__ void adam_kernel(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
+                            float *__restrict__ v, size_t n, float lr, float b1, float b2, float eps, float c1,
+                            float c2) {
+  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const float gg = g[e];
+  const float mm = b1 * m[e] + (1.f - b1) * gg;
+  const float vv = b2 * v[e] + (1.f - b2) * gg * gg;
+  m[e] = mm;
+  v[e] = vv;
+  p[e] -= lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+}
+
+template <typename T>
+__global__ void field_sum_kernel(const T *__restrict__ x, Grid g, double *__restrict__ partial) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (size_t v = (size_t)blockIdx.x * 256 + threadIdx.x; v < g.nr; v += (size_t)gridDim.x * 256) {
+    const size_t row = v / (unsigned)g.nz;
+    s += (double)x[row * g.nz_ + (v - row * g.nz)];
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+struct PL {
+  int cin, cout, cinp, coutp, level, relu;
+  size_t woff, boff;
+};
+
+constexpr int NLAYER = 9;
+constexpr int RED_BLOCKS = 256;
+constexpr int DW_CHUNK = 4096;
+
+}  // namespace
+
+struct Pnet {
+  int n = 0;
+  PL L[NLAYER];
+  size_t nparam = 0;
+  float *master = nullptr, *am = nullptr, *av = nullptr, *grad = nullptr;
+  bf16 *kf[NLAYER] = {}, *kb[NLAYER] = {};
+  float *bias[NLAYER] = {};
+  // activations
+  bf16 *x0, *a1, *s1, *p1, *a2, *s2, *p2, *a3, *a4, *c2, *d2, *c1, *d1;
+  float *y32, *target;
+  // gradients (pre-activation gradients of each layer's output)
+  bf16 *gy, *gd1, *gc1, *gd2, *gs1, *gc2, *ga4, *gs2, *ga3, *gp2, *ga2, *gp1, *ga1, *scratch;
+  float *partial = nullptr, *dw = nullptr, *db = nullptr;
+  double *red = nullptr;
+  int adam_t = 0;
+  std::vector<void *> allocs;
+  bool params_ready = false;
+};
+
+namespace {
+
+template <typename P>
+P *palloc(Pnet *pn, size_t bytes) {
+  void *p = nullptr;
+  KAPSEL_HIP(hipMalloc(&p, bytes));
+  pn->allocs.push_back(p);
+  return (P *)p;
+}
+
+void pnet_layout(Pnet *pn) {
+  const int spec[NLAYER][4] = {{2, 16, 0, 1},  {16, 16, 0, 1}, {16, 32, 1, 1}, {32, 32, 1, 1}, {32, 32, 2, 1},
+                               {32, 32, 2, 1}, {64, 32, 1, 1}, {48, 16, 0, 1}, {16, 3, 0, 0}};
+  size_t off = 0;
+  for (int l = 0; l < NLAYER; l++) {
+    PL &q = pn->L[l];
+    q.cin = spec[l][0];
+    q.cout = spec[l][1];
+    q.level = spec[l][2];
+    q.relu = spec[l][3];
+    q.cinp = (q.cin + 15) / 16 * 16;
+    q.coutp = (q.cout + 15) / 16 * 16;
+    q.woff = off;
+    off += (size_t)27 * q.cin * q.cout;
+    q.boff = off;
+    off += q.cout;
+  }
+  pn->nparam = off;
+}
+
+size_t nvox(int n) { return (size_t)n * n * n; }
+
+template <int CIN, int COUT>
+void conv_launch(kapsel_ctx *ctx, const bf16 *in, int n, const bf16 *K, const float *bias, int relu, int sign,
+                 const bf16 *mask, bf16 *out, float *out32) {
+  const size_t nv = nvox(n);
+  LaunchTimer lt(ctx, KAPSEL_K_PARTICLE, (double)nv * 2 * (CIN + COUT));
+  hipLaunchKernelGGL((conv_mfma_kernel<CIN, COUT>), dim3((unsigned)((nv + 63) / 64)), dim3(256), 0, ctx->stream,
+                     in, n, K, bias, relu, sign, mask, out, out32);
+  KAPSEL_LAUNCH_CHECK();
+}
+
+// dispatch on (cin, cout) of the convolution as executed
+void conv(kapsel_ctx *ctx, int cin, int cout, const bf16 *in, int n, const bf16 *K, const float *bias, int relu,
+          int sign, const bf16 *mask, bf16 *out, float *out32) {
+#define KAPSEL_CONV_CASE(A, B) \
+  if (cin == A && cout == B) return conv_launch<A, B>(ctx, in, n, K, bias, relu, sign, mask, out, out32);
+  KAPSEL_CONV_CASE(16, 16)
+  KAPSEL_CONV_CASE(16, 32)
+  KAPSEL_CONV_CASE(32, 32)
+  KAPSEL_CONV_CASE(64, 32)
+  KAPSEL_CONV_CASE(48, 16)
+  KAPSEL_CONV_CASE(32, 16)
+  KAPSEL_CONV_CASE(32, 64)
+  KAPSEL_CONV_CASE(16, 48)
+#undef KAPSEL_CONV_CASE
+  set_error("no convolution kernel for %d -> %d", cin, cout);
+  throw Error{KAPSEL_ERR_STATE};
+}
+
+template <int CIN, int COUT>
+void dw_launch(kapsel_ctx *ctx, Pnet *pn, const bf16 *in, const bf16 *dy, int n) {
+  const size_t nv = nvox(n);
+  const int nchunk = (int)((nv + DW_CHUNK - 1) / DW_CHUNK);
+  const int chunk = (int)((nv + nchunk - 1) / nchunk + 15) / 16 * 16;
+  LaunchTimer lt(ctx, KAPSEL_K_PARTICLE, (double)nv * 2 * (CIN + COUT) * 27);
+  hipLaunchKernelGGL((conv_dw_kernel<CIN, COUT>), dim3(27, nchunk), dim3(256), 0, ctx->stream, in, dy, n, chunk,
+                     pn->partial);
+  KAPSEL_LAUNCH_CHECK();
+  const size_t cnt = (size_t)27 * CIN * COUT;
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(blocks_for(cnt, 256)), dim3(256), 0, ctx->stream, pn->partial,
+                     nchunk, cnt, cnt, pn->dw);
+  KAPSEL_LAUNCH_CHECK();
+}
+
+// dW, db of layer l from its input and its pre-activation output gradient dy
+void layer_grad(kapsel_ctx *ctx, Pnet *pn, int l, const bf16 *in, const bf16 *dy, int n) {
+  const PL &q = pn->L[l];
+#define KAPSEL_DW_CASE(A, B) \
+  if (q.cinp == A && q.coutp == B) dw_launch<A, B>(ctx, pn, in, dy, n); else
+  KAPSEL_DW_CASE(16, 16)
+  KAPSEL_DW_CASE(16, 32)
+  KAPSEL_DW_CASE(32, 32)
+  KAPSEL_DW_CASE(64, 32)
+  KAPSEL_DW_CASE(48, 16) {
+    set_error("no weight-gradient kernel for %d -> %d", q.cinp, q.coutp);
+    throw Error{KAPSEL_ERR_STATE};
+  }
+#undef KAPSEL_DW_CASE
+  hipLaunchKernelGGL(bias_grad_kernel, dim3(q.coutp), dim3(256), 0, ctx->stream, dy, (int)nvox(n), q.coutp,
+                     pn->db);
+  KAPSEL_LAUNCH_CHECK();
+  const size_t cnt = (size_t)27 * q.cin * q.cout;
+  hipLaunchKernelGGL(unpack_grad_kernel, dim3(blocks_for(cnt > (size_t)q.cout ? cnt : q.cout, 256)), dim3(256), 0,
+                     ctx->stream, pn->dw, pn->db, q.woff, q.boff, q.cin, q.cout, q.cinp, q.coutp, pn->grad);
+  KAPSEL_LAUNCH_CHECK();
+}
+
+void pack_params(kapsel_ctx *ctx, Pnet *pn) {
+  for (int l = 0; l < NLAYER; l++) {
+    const PL &q = pn->L[l];
+    const size_t total = (size_t)27 * q.cinp * q.coutp;
+    hipLaunchKernelGGL(pack_layer_kernel, dim3(blocks_for(total, 256)), dim3(256), 0, ctx->stream, pn->master,
+                       q.woff, q.boff, q.cin, q.cout, q.cinp, q.coutp, pn->kf[l], pn->kb[l], pn->bias[l]);
+    KAPSEL_LAUNCH_CHECK();
+  }
+}
+
+unsigned eb(size_t n) { return blocks_for(n, 256); }
+
+void forward(kapsel_ctx *ctx, Pnet *pn) {
+  const int n0 = pn->n, n1 = n0 / 2, n2 = n0 / 4;
+  const PL *L = pn->L;
+  conv(ctx, 16, 16, pn->x0, n0, pn->kf[0], pn->bias[0], 1, 1, nullptr, pn->a1, nullptr);
+  conv(ctx, 16, 16, pn->a1, n0, pn->kf[1], pn->bias[1], 1, 1, nullptr, pn->s1, nullptr);
+  hipLaunchKernelGGL(pool_kernel, dim3(eb(nvox(n1) * 16)), dim3(256), 0, ctx->stream, pn->s1, n0, 16, pn->p1);
+  conv(ctx, 16, 32, pn->p1, n1, pn->kf[2], pn->bias[2], 1, 1, nullptr, pn->a2, nullptr);
+  conv(ctx, 32, 32, pn->a2, n1, pn->kf[3], pn->bias[3], 1, 1, nullptr, pn->s2, nullptr);
+  hipLaunchKernelGGL(pool_kernel, dim3(eb(nvox(n2) * 32)), dim3(256), 0, ctx->stream, pn->s2, n1, 32, pn->p2);
+  conv(ctx, 32, 32, pn->p2, n2, pn->kf[4], pn->bias[4], 1, 1, nullptr, pn->a3, nullptr);
+  conv(ctx, 32, 32, pn->a3, n2, pn->kf[5], pn->bias[5], 1, 1, nullptr, pn->a4, nullptr);
+  hipLaunchKernelGGL(up_concat_kernel, dim3(eb(nvox(n1) * 64)), dim3(256), 0, ctx->stream, pn->a4, n2, 32, pn->s2,
+                     32, pn->c2);
+  conv(ctx, 64, 32, pn->c2, n1, pn->kf[6], pn->bias[6], 1, 1, nullptr, pn->d2, nullptr);
+  hipLaunchKernelGGL(up_concat_kernel, dim3(eb(nvox(n0) * 48)), dim3(256), 0, ctx->stream, pn->d2, n1, 32, pn->s1,
+                     16, pn->c1);
+  conv(ctx, 48, 16, pn->c1, n0, pn->kf[7], pn->bias[7], 1, 1, nullptr, pn->d1, nullptr);
+  conv(ctx, 16, 16, pn->d1, n0, pn->kf[8], pn->bias[8], L[8].relu, 1, nullptr, pn->scratch, pn->y32);
+  KAPSEL_LAUNCH_CHECK();
+}
+
+// backward from gy (gradient of y, 16 padded channels); fills pn->grad
+void backward(kapsel_ctx *ctx, Pnet *pn) {
+  const int n0 = pn->n, n1 = n0 / 2, n2 = n0 / 4;
+  // out layer (8): input d1
+  layer_grad(ctx, pn, 8, pn->d1, pn->gy, n0);
+  conv(ctx, 16, 16, pn->gy, n0, pn->kb[8], nullptr, 0, -1, pn->d1, pn->gd1, nullptr);
+  // d1 (7): input c1 = [up(d2) | s1]
+  layer_grad(ctx, pn, 7, pn->c1, pn->gd1, n0);
+  conv(ctx, 16, 48, pn->gd1, n0, pn->kb[7], nullptr, 0, -1, pn->c1, pn->gc1, nullptr);
+  hipLaunchKernelGGL(up_concat_bwd_kernel, dim3(eb(nvox(n1) * 32 + nvox(n0) * 16)), dim3(256), 0, ctx->stream,
+                     pn->gc1, n1, 32, 16, pn->gd2, pn->gs1);
+  // d2 (6): input c2 = [up(a4) | s2]
+  layer_grad(ctx, pn, 6, pn->c2, pn->gd2, n1);
+  conv(ctx, 32, 64, pn->gd2, n1, pn->kb[6], nullptr, 0, -1, pn->c2, pn->gc2, nullptr);
+  hipLaunchKernelGGL(up_concat_bwd_kernel, dim3(eb(nvox(n2) * 32 + nvox(n1) * 32)), dim3(256), 0, ctx->stream,
+                     pn->gc2, n2, 32, 32, pn->ga4, pn->gs2);
+  // e3b (5): input a3
+  layer_grad(ctx, pn, 5, pn->a3, pn->ga4, n2);
+  conv(ctx, 32, 32, pn->ga4, n2, pn->kb[5], nullptr, 0, -1, pn->a3, pn->ga3, nullptr);
+  // e3a (4): input p2 (pooled, not a ReLU output: no mask)
+  layer_grad(ctx, pn, 4, pn->p2, pn->ga3, n2);
+  conv(ctx, 32, 32, pn->ga3, n2, pn->kb[4], nullptr, 0, -1, nullptr, pn->gp2, nullptr);
+  hipLaunchKernelGGL(pool_bwd_kernel, dim3(eb(nvox(n1) * 32)), dim3(256), 0, ctx->stream, pn->gp2, n1, 32, pn->s2,
+                     pn->gs2);
+  // e2b (3): input a2
+  layer_grad(ctx, pn, 3, pn->a2, pn->gs2, n1);
+  conv(ctx, 32, 32, pn->gs2, n1, pn->kb[3], nullptr, 0, -1, pn->a2, pn->ga2, nullptr);
+  // e2a (2): input p1
+  layer_grad(ctx, pn, 2, pn->p1, pn->ga2, n1);
+  conv(ctx, 32, 16, pn->ga2, n1, pn->kb[2], nullptr, 0, -1, nullptr, pn->gp1, nullptr);
+  hipLaunchKernelGGL(pool_bwd_kernel, dim3(eb(nvox(n0) * 16)), dim3(256), 0, ctx->stream, pn->gp1, n0, 16, pn->s1,
+                     pn->gs1);
+  // e1b (1): input a1
+  layer_grad(ctx, pn, 1, pn->a1, pn->gs1, n0);
+  conv(ctx, 16, 16, pn->gs1, n0, pn->kb[1], nullptr, 0, -1, pn->a1, pn->ga1, nullptr);
+  // e1a (0): input x0
+  layer_grad(ctx, pn, 0, pn->x0, pn->ga1, n0);
+  KAPSEL_LAUNCH_CHECK();
+}
+
+Pnet *get_pnet(kapsel_ctx *ctx) {
+  if (!ctx->pnet) {
+    set_error("network not initialised (kapsel_pnet_init)");
+    throw Error{KAPSEL_ERR_STATE};
+  }
+  return ctx->pnet;
+}
+
+template <typename T>
+void make_input(kapsel_ctx *ctx, Pnet *pn, double beta) {
+  hipLaunchKernelGGL(pnet_input_kernel<T>, dim3(eb(ctx->g.nr * 16)), dim3(256), 0, ctx->stream,
+                     (const T *)ctx->fields[KAPSEL_WORK4], ctx->g, (float)beta, pn->x0);
+  KAPSEL_LAUNCH_CHECK();
+}
+
+template <typename T>
+double field_mean(kapsel_ctx *ctx, Pnet *pn, const T *x) {
+  hipLaunchKernelGGL(field_sum_kernel<T>, dim3(RED_BLOCKS), dim3(256), 0, ctx->stream, x, ctx->g, pn->red);
+  KAPSEL_LAUNCH_CHECK();
+  std::vector<double> h(RED_BLOCKS);
+  KAPSEL_HIP(hipMemcpyAsync(h.data(), pn->red, sizeof(double) * RED_BLOCKS, hipMemcpyDeviceToHost, ctx->stream));
+  KAPSEL_HIP(hipStreamSynchronize(ctx->stream));
+  double s = 0;
+  for (double v : h) s += v;
+  return s / (double)ctx->g.nr;
+}
+
+// loss and gradient for target POTENTIAL / scale (mean removed), unit field e
+template <typename T>
+double loss_grad_t(kapsel_ctx *ctx, Pnet *pn, double beta, const double e[3], double scale) {
+  make_input<T>(ctx, pn, beta);
+  const T *x = (const T *)ctx->fields[KAPSEL_POTENTIAL];
+  const double mean = field_mean<T>(ctx, pn, x);
+  hipLaunchKernelGGL(pnet_target_kernel<T>, dim3(eb(ctx->g.nr)), dim3(256), 0, ctx->stream, x, ctx->g, scale, mean,
+                     pn->target);
+  forward(ctx, pn);
+  hipLaunchKernelGGL(loss_kernel, dim3(RED_BLOCKS), dim3(256), 0, ctx->stream, pn->y32, pn->target,
+                     (int)ctx->g.nr, (float)e[0], (float)e[1], (float)e[2], pn->gy, pn->red);
+  KAPSEL_LAUNCH_CHECK();
+  std::vector<double> h(RED_BLOCKS);
+  KAPSEL_HIP(hipMemcpyAsync(h.data(), pn->red, sizeof(double) * RED_BLOCKS, hipMemcpyDeviceToHost, ctx->stream));
+  backward(ctx, pn);
+  KAPSEL_HIP(hipStreamSynchronize(ctx->stream));
+  double s = 0;
+  for (double v : h) s += v;
+  return s / (double)ctx->g.nr;
+}
+
+template <typename T>
+void warm_start_t(kapsel_ctx *ctx, Pnet *pn, double beta, const double e0[3], double scale) {
+  make_input<T>(ctx, pn, beta);
+  forward(ctx, pn);
+  hipLaunchKernelGGL(pnet_output_kernel<T>, dim3(eb(ctx->g.nr)), dim3(256), 0, ctx->stream, pn->y32, ctx->g,
+                     (float)e0[0], (float)e0[1], (float)e0[2], scale, (T *)ctx->fields[KAPSEL_POTENTIAL]);
+  KAPSEL_LAUNCH_CHECK();
+}
+
+}  // namespace
+
+void pnet_init(kapsel_ctx *ctx) {
+  const Grid &g = ctx->g;
+  if (g.nx != g.ny || g.nx != g.nz || g.nx % 4) {
+    set_error("the network needs a cubic grid divisible by 4 (got %d x %d x %d)", g.nx, g.ny, g.nz);
+    throw Error{KAPSEL_ERR_ARG};
+  }
+  if (ctx->pnet) return;
+  Pnet *pn = new Pnet();
+  ctx->pnet = pn;
+  pn->n = g.nx;
+  pnet_layout(pn);
+  const size_t v0 = nvox(pn->n), v1 = nvox(pn->n / 2), v2 = nvox(pn->n / 4);
+  pn->master = palloc<float>(pn, pn->nparam * 4);
+  pn->am = palloc<float>(pn, pn->nparam * 4);
+  pn->av = palloc<float>(pn, pn->nparam * 4);
+  pn->grad = palloc<float>(pn, pn->nparam * 4);
+  KAPSEL_HIP(hipMemsetAsync(pn->master, 0, pn->nparam * 4, ctx->stream));
+  KAPSEL_HIP(hipMemsetAsync(pn->am, 0, pn->nparam * 4, ctx->stream));
+  KAPSEL_HIP(hipMemsetAsync(pn->av, 0, pn->nparam * 4, ctx->stream));
+  size_t maxw = 0;
+  for (int l = 0; l < NLAYER; l++) {
+    const PL &q = pn->L[l];
+    const size_t w = (size_t)27 * q.cinp * q.coutp;
+    pn->kf[l] = palloc<bf16>(pn, w * 2);
+    pn->kb[l] = palloc<bf16>(pn, w * 2);
+    pn->bias[l] = palloc<float>(pn, q.coutp * 4);
+    maxw = w > maxw ? w : maxw;
+  }
+  auto act = [&](size_t v, int c) { return palloc<bf16>(pn, v * c * 2); };
+  pn->x0 = act(v0, 16); pn->a1 = act(v0, 16); pn->s1 = act(v0, 16); pn->p1 = act(v1, 16);
+  pn->a2 = act(v1, 32); pn->s2 = act(v1, 32); pn->p2 = act(v2, 32); pn->a3 = act(v2, 32); pn->a4 = act(v2, 32);
+  pn->c2 = act(v1, 64); pn->d2 = act(v1, 32); pn->c1 = act(v0, 48); pn->d1 = act(v0, 16);
+  pn->scratch = act(v0, 16);
+  pn->y32 = palloc<float>(pn, v0 * 16 * 4);
+  pn->target = palloc<float>(pn, v0 * 4);
+  pn->gy = act(v0, 16); pn->gd1 = act(v0, 16); pn->gc1 = act(v0, 48); pn->gd2 = act(v1, 32);
+  pn->gs1 = act(v0, 16); pn->gc2 = act(v1, 64); pn->ga4 = act(v2, 32); pn->gs2 = act(v1, 32);
+  pn->ga3 = act(v2, 32); pn->gp2 = act(v2, 32); pn->ga2 = act(v1, 32); pn->gp1 = act(v1, 16); pn->ga1 = act(v0, 16);
+  KAPSEL_HIP(hipMemsetAsync(pn->gy, 0, v0 * 16 * 2, ctx->stream));  // channels 3..15 stay zero
+  size_t maxpart = 0;
+  for (int l = 0; l < NLAYER; l++) {
+    const PL &q = pn->L[l];
+    const size_t nv = nvox(pn->n >> q.level);
+    const size_t nchunk = (nv + DW_CHUNK - 1) / DW_CHUNK;
+    const size_t need = nchunk * 27 * q.cinp * q.coutp;
+    maxpart = need > maxpart ? need : maxpart;
+  }
+  pn->partial = palloc<float>(pn, maxpart * 4);
+  pn->dw = palloc<float>(pn, maxw * 4);
+  pn->db = palloc<float>(pn, 64 * 4);
+  pn->red = palloc<double>(pn, RED_BLOCKS * 8);
+  KAPSEL_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+void pnet_free(kapsel_ctx *ctx) {
+  if (!ctx->pnet) return;
+  for (void *p : ctx->pnet->allocs) (void)hipFree(p);
+  delete ctx->pnet;
+  ctx->pnet = nullptr;
+}
+
+long pnet_param_count(kapsel_ctx *ctx) { return (long)get_pnet(ctx)->nparam; }
+
+void pnet_set_params(kapsel_ctx *ctx, const float *host) {
+  Pnet *pn = get_pnet(ctx);
+  KAPSEL_HIP(hipMemcpyAsync(pn->master, host, pn->nparam * 4, hipMemcpyHostToDevice, ctx->stream));
+  KAPSEL_HIP(hipMemsetAsync(pn->am, 0, pn->nparam * 4, ctx->stream));
+  KAPSEL_HIP(hipMemsetAsync(pn->av, 0, pn->nparam * 4, ctx->stream));
+  pn->adam_t = 0;
+  pack_params(ctx, pn);
+  pn->params_ready = true;
+  KAPSEL_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+void pnet_get_params(kapsel_ctx *ctx, float *host) {
+  Pnet *pn = get_pnet(ctx);
+  KAPSEL_HIP(hipMemcpyAsync(host, pn->master, pn->nparam * 4, hipMemcpyDeviceToHost, ctx->stream));
+  KAPSEL_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+static void need_params(Pnet *pn) {
+  if (!pn->params_ready) {
+    set_error("network parameters not set (kapsel_pnet_set_params)");
+    throw Error{KAPSEL_ERR_STATE};
+  }
+}
+
+double pnet_loss_grad(kapsel_ctx *ctx, double beta, const double e[3], double scale, float *grad_host) {
+  Pnet *pn = get_pnet(ctx);
+  need_params(pn);
+  double loss = ctx->prec == KAPSEL_FP64 ? loss_grad_t<double>(ctx, pn, beta, e, scale)
+                                         : loss_grad_t<float>(ctx, pn, beta, e, scale);
+  if (grad_host) {
+    KAPSEL_HIP(hipMemcpyAsync(grad_host, pn->grad, pn->nparam * 4, hipMemcpyDeviceToHost, ctx->stream));
+    KAPSEL_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  return loss;
+}
+
+double pnet_train_step(kapsel_ctx *ctx, double beta, const double e[3], double scale, double lr) {
+  Pnet *pn = get_pnet(ctx);
+  need_params(pn);
+  double loss = pnet_loss_grad(ctx, beta, e, scale, nullptr);
+  pn->adam_t += 1;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.f - powf(b1, (float)pn->adam_t), c2 = 1.f - powf(b2, (float)pn->adam_t);
+  hipLaunchKernelGGL(adam_kernel, dim3(eb(pn->nparam)), dim3(256), 0, ctx->stream, pn->master, pn->grad, pn->am,
+                     pn->av, pn->nparam, (float)lr, b1, b2, 1e-8f, c1, c2);
+  KAPSEL_LAUNCH_CHECK();
+  pack_params(ctx, pn);
+  return loss;
+}
+
+void pnet_warm_start(kapsel_ctx *ctx, double beta, const double e0[3], double scale) {
+  Pnet *pn = get_pnet(ctx);
+  need_params(pn);
+  if (ctx->prec == KAPSEL_FP64) warm_start_t<double>(ctx, pn, beta, e0, scale);
+  else warm_start_t<float>(ctx, pn, beta, e0, scale);
+}
+
 }  // namespace kapsel

@@ -95,6 +95,9 @@ class KapselGPU:
         self._call("synchronize")
 
     # operations ----------------------------------------------------------
+    def copy_field(self, dst, src):
+        self._call("copy_field", abi.FIELD[dst], abi.FIELD[src])
+
     def fft_r2c(self, name):
         self._call("fft_r2c", abi.FIELD[name])
 
@@ -173,6 +176,39 @@ class KapselGPU:
         if rc != 0 and (check or rc != -4):
             raise KapselError(f"kapsel_poisson_solve: {rc}: {self.lib.kapsel_last_error().decode()}")
         return it.value, rr.value
+
+    # learned warm start -----------------------------------------------------
+    def pnet_init(self):
+        n = ctypes.c_long(0)
+        self._call("pnet_init", ctypes.byref(n))
+        return n.value
+
+    def pnet_set_params(self, params):
+        a = np.ascontiguousarray(params, dtype=np.float32)
+        self._call("pnet_set_params", a.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+
+    def pnet_get_params(self, n):
+        a = np.zeros(n, dtype=np.float32)
+        self._call("pnet_get_params", a.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+        return a
+
+    def pnet_loss_grad(self, beta, e_hat, scale, n_params=None):
+        e = np.ascontiguousarray(e_hat, dtype=np.float64)
+        loss = ctypes.c_double(0.0)
+        g = np.zeros(n_params, dtype=np.float32) if n_params else None
+        gp = g.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) if g is not None else None
+        self._call("pnet_loss_grad", float(beta), _dp(e), float(scale), ctypes.byref(loss), gp)
+        return loss.value, g
+
+    def pnet_train_step(self, beta, e_hat, scale, lr):
+        e = np.ascontiguousarray(e_hat, dtype=np.float64)
+        loss = ctypes.c_double(0.0)
+        self._call("pnet_train_step", float(beta), _dp(e), float(scale), float(lr), ctypes.byref(loss))
+        return loss.value
+
+    def pnet_warm_start(self, beta, e0, scale):
+        e = np.ascontiguousarray(e0, dtype=np.float64)
+        self._call("pnet_warm_start", float(beta), _dp(e), float(scale))
 
     # timing ---------------------------------------------------------------
     def kernel_timing(self, classes):

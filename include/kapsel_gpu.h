@@ -122,6 +122,9 @@ int kapsel_get_uk_dc(kapsel_ctx *ctx, double uk_dc[3]);
 int kapsel_set_particles(kapsel_ctx *ctx, const double *state);
 int kapsel_get_particles(kapsel_ctx *ctx, double *state);
 
+/* Copy_v1_k (aux_field.h:82): dst = src over the whole padded field. */
+int kapsel_copy_field(kapsel_ctx *ctx, int dst, int src);
+
 /* ---- FFT and k-space algebra ------------------------------------------- */
 /* A2a_k (fft_wrapper.h:681-701): in-place forward r2c, e^{-ik.x}, unscaled. */
 int kapsel_fft_r2c(kapsel_ctx *ctx, int field);
@@ -193,6 +196,29 @@ int kapsel_poisson_rhs(kapsel_ctx *ctx, const double e0[3]);
  * *relres set) when tol is not reached. */
 int kapsel_poisson_solve(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int precond, int *iters,
                          double *relres);
+
+/* ---- learned warm start (no counterpart in the reference) ----------------- */
+/* A 3-level periodic convolutional encoder-decoder (182,083 parameters) maps
+ * (phi_sin, beta phi_sin) -- the compact-sine sum left in WORK4 by
+ * kapsel_dielectric_field and the Clausius-Mossotti factor
+ * beta = (eps_p - eps_f) / (eps_p + 2 eps_f) -- to three potentials psi_d,
+ * the response to a unit external field along d.  BF16 weights and
+ * activations on MFMA, FP32 accumulation, FP32 master weights and Adam.
+ * Needs a cubic grid with N divisible by 4. */
+int kapsel_pnet_init(kapsel_ctx *ctx, long *n_params);
+/* parameters: FP32, per layer W[tap][cin][cout] then b[cout] (layer order
+ * e1a e1b e2a e2b e3a e3b d2 d1 out) */
+int kapsel_pnet_set_params(kapsel_ctx *ctx, const float *params);
+int kapsel_pnet_get_params(kapsel_ctx *ctx, float *params);
+/* loss = mean_v (sum_d e_d psi_d - (POTENTIAL - mean) / scale)^2 and its
+ * gradient (grad may be NULL), no update; e_hat a unit vector */
+int kapsel_pnet_loss_grad(kapsel_ctx *ctx, double beta, const double e_hat[3], double scale, double *loss,
+                          float *grad);
+/* one Adam step on that loss (learning rate lr) */
+int kapsel_pnet_train_step(kapsel_ctx *ctx, double beta, const double e_hat[3], double scale, double lr,
+                           double *loss);
+/* POTENTIAL = scale * sum_d e0_d psi_d: the warm start of kapsel_poisson_solve */
+int kapsel_pnet_warm_start(kapsel_ctx *ctx, double beta, const double e0[3], double scale);
 
 /* ---- kernel timing (HIP events on the context stream) --------------------- */
 enum kapsel_kernel_class {
