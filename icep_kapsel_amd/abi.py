@@ -133,6 +133,8 @@ _PROTOS = {
     "kapsel_poisson_apply_host": [_DP, _DP],
     "kapsel_poisson_rhs": [_DP],
     "kapsel_poisson_solve": [_DP, _D, _I, _I, _IP, _DP],
+    "kapsel_maxwell_force": [_DP],
+    "kapsel_icep_step": [ctypes.POINTER(Time), _DP, _D, _I, _I, _IP, _DP],
     "kapsel_pnet_init": [ctypes.POINTER(ctypes.c_long)],
     "kapsel_pnet_set_params": [ctypes.POINTER(ctypes.c_float)],
     "kapsel_pnet_get_params": [ctypes.POINTER(ctypes.c_float)],

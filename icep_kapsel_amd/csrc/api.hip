@@ -323,6 +323,31 @@ static void spm_step(kapsel_ctx *ctx, kapsel_time &t) {
   t.ts += 1;
 }
 
+// ICEP step without ions: Time_evolution_hydro's Navier-Stokes step with the
+// dielectric force of the Electrolyte+Dielectric branch inserted after the
+// profile scatter (sp_3d_ns.cxx:51-95): eps, Poisson solve warm-started from
+// the previous potential, Maxwell force, u += f dt / rho, projection
+static void icep_step(kapsel_ctx *ctx, kapsel_time &t, const double e0[3], double tol, int maxit, int precond,
+                      int *iters, double *relres) {
+  ns_slaved_euler(ctx, t);
+  zeta_k2u(ctx, fix_mask(ctx));
+  md_position(ctx, t);
+  make_phi_particle_sum(ctx);
+  dielectric_field(ctx);
+  poisson_solve(ctx, e0, tol, maxit, precond, iters, relres);
+  maxwell_force(ctx, e0);
+  add_force_to_u(ctx, t.dt_fluid * ctx->irho);
+  solenoidal_u(ctx, ctx->fields + KAPSEL_U0);
+  calc_f_hydro(ctx, t);
+  force(ctx);
+  md_velocity_integrate(ctx, t);
+  make_u_particle_sum(ctx);
+  f_particle_dt_sole_add(ctx);
+  u2zeta_k(ctx);
+  t.time += t.dt_fluid;
+  t.ts += 1;
+}
+
 // main, initial rigidity projection (sp_3d_ns.cxx:431-447)
 static void init_rigidity(kapsel_ctx *ctx) {
   make_phi_particle_sum(ctx);
@@ -741,6 +766,35 @@ int kapsel_pnet_warm_start(kapsel_ctx *ctx, double beta, const double e0[3], dou
     throw Error{KAPSEL_ERR_ARG};
   }
   pnet_warm_start(ctx, beta, e0, scale);
+  API_END
+}
+
+int kapsel_maxwell_force(kapsel_ctx *ctx, const double e0[3]) {
+  API_BEGIN
+  check_ctx(ctx);
+  if (!e0) {
+    set_error("null external field");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+  maxwell_force(ctx, e0);
+  API_END
+}
+
+int kapsel_icep_step(kapsel_ctx *ctx, kapsel_time *jikan, const double e0[3], double tol, int maxit, int precond,
+                     int *iters, double *relres) {
+  API_BEGIN
+  check_ctx(ctx);
+  check_time(jikan);
+  if (!e0 || !iters || !relres || !(tol > 0) || maxit < 1 ||
+      (precond != KAPSEL_PRECOND_NONE && precond != KAPSEL_PRECOND_SPECTRAL)) {
+    set_error("invalid icep_step arguments");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+  icep_step(ctx, *jikan, e0, tol, maxit, precond, iters, relres);
+  if (!(*relres < tol)) {
+    set_error("icep_step: Poisson solve stopped at relative residual %.3e", *relres);
+    return KAPSEL_ERR_NOCONV;
+  }
   API_END
 }
 

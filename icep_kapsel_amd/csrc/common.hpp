@@ -136,7 +136,7 @@ struct kapsel_ctx {
   // Poisson solver storage, allocated on first use: operator scratch of the
   // context precision, and FP64 shadow fields for the iterative refinement
   // of an FP32 context
-  void *pscratch[4] = {};
+  void *pscratch[5] = {};          // [0..3] operator scratch, [4] refinement correction
   double *p64[8] = {};            // eps, x, b, r, 4 operator scratch
   double *d_partial = nullptr;    // block partial sums (reductions)
   double *d_scal = nullptr;       // CG scalars
@@ -212,6 +212,8 @@ void poisson_rhs(kapsel_ctx *ctx, const double e0[3]);
 void poisson_solve(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int precond, int *iters,
                    double *relres);
 void poisson_free(kapsel_ctx *ctx);
+void maxwell_force(kapsel_ctx *ctx, const double e0[3]);
+void add_force_to_u(kapsel_ctx *ctx, double s);
 
 // learned warm start (pnet.hip)
 void pnet_init(kapsel_ctx *ctx);

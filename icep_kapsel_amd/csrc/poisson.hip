@@ -69,6 +69,69 @@ __device__ __forceinline__ size_t nbr(const Grid &g, int i, int j, int k) {
   return ((size_t)i * g.ny + j) * g.nz_ + k;
 }
 
+// extract_id for -1 neighbours
+__device__ __forceinline__ size_t nbm(const Grid &g, int i, int j, int k) {
+  i = i < 0 ? g.nx - 1 : i;
+  j = j < 0 ? g.ny - 1 : j;
+  k = k < 0 ? g.nz - 1 : k;
+  return ((size_t)i * g.ny + j) * g.nz_ + k;
+}
+
+// E^2 at the cell centres from the face values E_d = -dpsi_d + E0_d
+// (operate_dielectric.cxx:676-702)
+template <typename T>
+__global__ void efield2_kernel(const T *__restrict__ g0, const T *__restrict__ g1, const T *__restrict__ g2,
+                               double e0x, double e0y, double e0z, T *__restrict__ e2, Grid g) {
+  size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= g.nr) return;
+  const RI p = ridx(g, r);
+  const T fx = -g0[p.im] + (T)e0x, fy = -g1[p.im] + (T)e0y, fz = -g2[p.im] + (T)e0z;
+  const T bx = -g0[nbm(g, p.i - 1, p.j, p.k)] + (T)e0x;
+  const T by = -g1[nbm(g, p.i, p.j - 1, p.k)] + (T)e0y;
+  const T bz = -g2[nbm(g, p.i, p.j, p.k - 1)] + (T)e0z;
+  const T ax = (fx + bx) / (T)2, ay = (fy + by) / (T)2, az = (fz + bz) / (T)2;
+  e2[p.im] = ax * ax + ay * ay + az * az;
+}
+
+// face force rho_f E - E2_f grad(eps) / 2 (operate_dielectric.cxx:704-757)
+// averaged back to the cell centre (Interpolate_vec_on_normal_grid, :32-55)
+template <typename T>
+__device__ __forceinline__ T face_force(const T *gpsi, const T *geps, const T *rho, const T *e2, double e0d,
+                                        size_t at, size_t up) {
+  const T efw = -gpsi[at] + (T)e0d;
+  const T rho_f = (T)0.5 * rho[up] + (T)0.5 * rho[at];
+  const T e2f = (T)0.5 * e2[at] + (T)0.5 * e2[up];
+  return rho_f * efw - e2f * geps[at] / (T)2;
+}
+
+template <typename T>
+__global__ void maxwell_kernel(const T *__restrict__ p0, const T *__restrict__ p1, const T *__restrict__ p2,
+                               const T *__restrict__ q0, const T *__restrict__ q1, const T *__restrict__ q2,
+                               const T *__restrict__ rho, const T *__restrict__ e2, double e0x, double e0y,
+                               double e0z, T *__restrict__ f0, T *__restrict__ f1, T *__restrict__ f2, Grid g) {
+  size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= g.nr) return;
+  const RI p = ridx(g, r);
+  const size_t xm = nbm(g, p.i - 1, p.j, p.k), ym = nbm(g, p.i, p.j - 1, p.k), zm = nbm(g, p.i, p.j, p.k - 1);
+  const size_t xp = nbr(g, p.i + 1, p.j, p.k), yp = nbr(g, p.i, p.j + 1, p.k), zp = nbr(g, p.i, p.j, p.k + 1);
+  f0[p.im] = (T)0.5 * face_force(p0, q0, rho, e2, e0x, p.im, xp) + (T)0.5 * face_force(p0, q0, rho, e2, e0x, xm, p.im);
+  f1[p.im] = (T)0.5 * face_force(p1, q1, rho, e2, e0y, p.im, yp) + (T)0.5 * face_force(p1, q1, rho, e2, e0y, ym, p.im);
+  f2[p.im] = (T)0.5 * face_force(p2, q2, rho, e2, e0z, p.im, zp) + (T)0.5 * face_force(p2, q2, rho, e2, e0z, zm, p.im);
+}
+
+// u += f s (s = dt / rho, sp_3d_ns.cxx:83-93)
+template <typename T>
+__global__ void add_force_kernel(T *__restrict__ u0, T *__restrict__ u1, T *__restrict__ u2,
+                                 const T *__restrict__ f0, const T *__restrict__ f1, const T *__restrict__ f2,
+                                 double s, Grid g) {
+  size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= g.nr) return;
+  const RI p = ridx(g, r);
+  u0[p.im] += f0[p.im] * (T)s;
+  u1[p.im] += f1[p.im] * (T)s;
+  u2[p.im] += f2[p.im] * (T)s;
+}
+
 template <typename T>
 __global__ void copy_real_kernel(const T *__restrict__ x, T *__restrict__ y, Grid g) {
   size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -503,7 +566,7 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
                    (float *)ctx->pscratch[3]};
   float *w32[4] = {fld<float>(ctx, KAPSEL_WORK0), fld<float>(ctx, KAPSEL_WORK1), fld<float>(ctx, KAPSEL_WORK2),
                    fld<float>(ctx, KAPSEL_WORK3)};
-  float *d32 = fld<float>(ctx, KAPSEL_WORK4), *eps32 = fld<float>(ctx, KAPSEL_EPSILON);
+  float *d32 = (float *)ctx->pscratch[4], *eps32 = fld<float>(ctx, KAPSEL_EPSILON);
   // FP64 copies of the FP32 inputs: eps and rho enter the residual in FP64
   hipLaunchKernelGGL((convert_real_kernel<double, float>), dim3(blocks_for(g.f, TPB)), dim3(TPB), 0, ctx->stream,
                      eps64, eps32, g);
@@ -558,7 +621,66 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
   return total;
 }
 
+// staggered gradient of a real field a into g[0..2] (s0 scratch)
+template <typename T>
+void staggered_grad_t(kapsel_ctx *ctx, const T *a, T *s0, T *const *gout) {
+  const Grid &g = ctx->g;
+  {
+    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 2, sizeof(T)));
+    hipLaunchKernelGGL(copy_real_kernel<T>, dim3(blocks_for(g.f, TPB)), dim3(TPB), 0, ctx->stream, a, s0, g);
+    KAPSEL_LAUNCH_CHECK();
+  }
+  void *s1[1] = {s0};
+  fft_r2c_t<T>(ctx, s1, 1);
+  {
+    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 4, sizeof(T)));
+    hipLaunchKernelGGL(grad_shift_kernel<T>, dim3(blocks_for(g.nc, TPB)), dim3(TPB), 0, ctx->stream, s0, gout[0],
+                       gout[1], gout[2], sh<T>(ctx, 0), sh<T>(ctx, 1), sh<T>(ctx, 2), g);
+    KAPSEL_LAUNCH_CHECK();
+  }
+  void *g3[3] = {gout[0], gout[1], gout[2]};
+  fft_c2r_t<T>(ctx, g3, 3);
+}
+
+template <typename T>
+void maxwell_t(kapsel_ctx *ctx, const double e0[3]) {
+  const Grid &g = ctx->g;
+  T *s0 = (T *)ctx->pscratch[0];
+  T *ge[3] = {(T *)ctx->pscratch[1], (T *)ctx->pscratch[2], (T *)ctx->pscratch[3]};
+  T *gp[3] = {fld<T>(ctx, KAPSEL_WORK1), fld<T>(ctx, KAPSEL_WORK2), fld<T>(ctx, KAPSEL_WORK3)};
+  T *e2 = fld<T>(ctx, KAPSEL_WORK0);
+  staggered_grad_t<T>(ctx, fld<T>(ctx, KAPSEL_EPSILON), s0, ge);
+  staggered_grad_t<T>(ctx, fld<T>(ctx, KAPSEL_POTENTIAL), s0, gp);
+  LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 15, sizeof(T)));
+  hipLaunchKernelGGL(efield2_kernel<T>, dim3(blocks_for(g.nr, TPB)), dim3(TPB), 0, ctx->stream, gp[0], gp[1], gp[2],
+                     e0[0], e0[1], e0[2], e2, g);
+  hipLaunchKernelGGL(maxwell_kernel<T>, dim3(blocks_for(g.nr, TPB)), dim3(TPB), 0, ctx->stream, gp[0], gp[1], gp[2],
+                     ge[0], ge[1], ge[2], fld<T>(ctx, KAPSEL_RHO_FREE), e2, e0[0], e0[1], e0[2],
+                     fld<T>(ctx, KAPSEL_F0), fld<T>(ctx, KAPSEL_F1), fld<T>(ctx, KAPSEL_F2), g);
+  KAPSEL_LAUNCH_CHECK();
+}
+
+template <typename T>
+void add_force_t(kapsel_ctx *ctx, double s) {
+  LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 9, sizeof(T)));
+  hipLaunchKernelGGL(add_force_kernel<T>, dim3(blocks_for(ctx->g.nr, TPB)), dim3(TPB), 0, ctx->stream,
+                     fld<T>(ctx, KAPSEL_U0), fld<T>(ctx, KAPSEL_U1), fld<T>(ctx, KAPSEL_U2), fld<T>(ctx, KAPSEL_F0),
+                     fld<T>(ctx, KAPSEL_F1), fld<T>(ctx, KAPSEL_F2), s, ctx->g);
+  KAPSEL_LAUNCH_CHECK();
+}
+
 }  // namespace
+
+void maxwell_force(kapsel_ctx *ctx, const double e0[3]) {
+  ensure_storage(ctx);
+  if (ctx->prec == KAPSEL_FP64) maxwell_t<double>(ctx, e0);
+  else maxwell_t<float>(ctx, e0);
+}
+
+void add_force_to_u(kapsel_ctx *ctx, double s) {
+  if (ctx->prec == KAPSEL_FP64) add_force_t<double>(ctx, s);
+  else add_force_t<float>(ctx, s);
+}
 
 void dielectric_field(kapsel_ctx *ctx) {
   if (ctx->prec == KAPSEL_FP64)

@@ -177,6 +177,20 @@ class KapselGPU:
             raise KapselError(f"kapsel_poisson_solve: {rc}: {self.lib.kapsel_last_error().decode()}")
         return it.value, rr.value
 
+    def maxwell_force(self, e0):
+        e = np.ascontiguousarray(e0, dtype=np.float64)
+        self._call("maxwell_force", _dp(e))
+
+    def icep_step(self, t, e0, tol, maxit, precond=0, check=True):
+        e = np.ascontiguousarray(e0, dtype=np.float64)
+        it = ctypes.c_int(0)
+        rr = ctypes.c_double(0.0)
+        rc = self.lib.kapsel_icep_step(self.ctx, ctypes.byref(t), _dp(e), tol, int(maxit), int(precond),
+                                       ctypes.byref(it), ctypes.byref(rr))
+        if rc != 0 and (check or rc != -4):
+            raise KapselError(f"kapsel_icep_step: {rc}: {self.lib.kapsel_last_error().decode()}")
+        return it.value, rr.value
+
     # learned warm start -----------------------------------------------------
     def pnet_init(self):
         n = ctypes.c_long(0)

@@ -197,6 +197,20 @@ int kapsel_poisson_rhs(kapsel_ctx *ctx, const double e0[3]);
 int kapsel_poisson_solve(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int precond, int *iters,
                          double *relres);
 
+/* Make_Maxwell_force_x_on_fluid after the solve (operate_dielectric.cxx:646-763,
+ * AC off, External_field on): staggered gradients of EPSILON and POTENTIAL,
+ * face force rho E - E^2 grad(eps) / 2, averaged to the cell centres into
+ * F0..F2 (scratch: WORK0..WORK3). */
+int kapsel_maxwell_force(kapsel_ctx *ctx, const double e0[3]);
+/* One ICEP step without ions: kapsel_spm_step's sequence with, after the
+ * profile scatter, kapsel_dielectric_field, kapsel_poisson_solve (warm start:
+ * the potential of the previous step), kapsel_maxwell_force, u += f dt/rho
+ * and Solenoidal_u(u) -- the dielectric part of the Electrolyte+Dielectric
+ * branch of Time_evolution_hydro (sp_3d_ns.cxx:55-95).  Returns
+ * KAPSEL_ERR_NOCONV (step completed) when the solve missed tol. */
+int kapsel_icep_step(kapsel_ctx *ctx, kapsel_time *jikan, const double e0[3], double tol, int maxit, int precond,
+                     int *iters, double *relres);
+
 /* ---- learned warm start (no counterpart in the reference) ----------------- */
 /* A 3-level periodic convolutional encoder-decoder (182,083 parameters) maps
  * (phi_sin, beta phi_sin) -- the compact-sine sum left in WORK4 by

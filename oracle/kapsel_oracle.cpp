@@ -1127,6 +1127,117 @@ struct Oracle {
     return it;
   }
 
+  /* staggered gradient of a real field: IFFT(i k e^{i k dx/2} FFT(a))
+   * (A2a_k, A_k2da_k, Shift_vec_fw_imag, U_k2u; operate_dielectric.cxx:649-662) */
+  void staggered_grad(const double *a, double **g) {
+    double *w = scratch(0);
+    memcpy(w, a, F * sizeof(double));
+    fft_r2c(w);
+    grad_shift_k(w, g);
+    for (int d = 0; d < 3; d++) fft_c2r(g[d]);
+  }
+  /* Make_Maxwell_force_x_on_fluid after the solve (operate_dielectric.cxx:646-763), AC off,
+   * External_field on: cell-centred force in F0..F2; grad eps in WORK1..3, E^2 in WORK0 */
+  void maxwell_force(const double *e0) {
+    double *ge[3] = {fld(KAPSEL_WORK1), fld(KAPSEL_WORK2), fld(KAPSEL_WORK3)};
+    double *fc[3] = {FP(0), FP(1), FP(2)};
+    std::vector<double> ev[3];
+    double *ef[3];
+    for (int d = 0; d < 3; d++) {
+      ev[d].assign(F, 0.0);
+      ef[d] = ev[d].data();
+    }
+    staggered_grad(fld(KAPSEL_EPSILON), ge);
+    staggered_grad(fld(KAPSEL_POTENTIAL), ef);  /* force[] holds grad(potential) at the faces */
+    double *e2 = fld(KAPSEL_WORK0);
+    reset(e2);
+#pragma omp parallel for
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++)
+        for (int k = 0; k < NZ; k++) {
+          size_t im = ((size_t)i * NY + j) * NZ_ + k;
+          double fx = -ef[0][im] + e0[0], fy = -ef[1][im] + e0[1], fz = -ef[2][im] + e0[2];
+          double bx = -ef[0][extract_id(i - 1, j, k)] + e0[0];
+          double by = -ef[1][extract_id(i, j - 1, k)] + e0[1];
+          double bz = -ef[2][extract_id(i, j, k - 1)] + e0[2];
+          e2[im] = SQ((fx + bx) / 2.0) + SQ((fy + by) / 2.0) + SQ((fz + bz) / 2.0);
+        }
+    const double *rho = fld(KAPSEL_RHO_FREE);
+    std::vector<double> ff[3];
+    for (int d = 0; d < 3; d++) ff[d].assign(F, 0.0);
+#pragma omp parallel for
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++)
+        for (int k = 0; k < NZ; k++) {
+          size_t im = ((size_t)i * NY + j) * NZ_ + k;
+          size_t nb[3] = {extract_id(i + 1, j, k), extract_id(i, j + 1, k), extract_id(i, j, k + 1)};
+          for (int d = 0; d < 3; d++) {
+            double efw = -ef[d][im] + e0[d];
+            double rho_f = 0.5 * rho[nb[d]] + 0.5 * rho[im];
+            double e2f = 0.5 * e2[im] + 0.5 * e2[nb[d]];
+            ff[d][im] = rho_f * efw - e2f * ge[d][im] / 2.0;
+          }
+        }
+    /* Interpolate_vec_on_normal_grid (operate_dielectric.cxx:32-55) */
+    for (int d = 0; d < 3; d++) reset(fc[d]);
+#pragma omp parallel for
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++)
+        for (int k = 0; k < NZ; k++) {
+          size_t im = ((size_t)i * NY + j) * NZ_ + k;
+          fc[0][im] = 0.5 * ff[0][im] + 0.5 * ff[0][extract_id(i - 1, j, k)];
+          fc[1][im] = 0.5 * ff[1][im] + 0.5 * ff[1][extract_id(i, j - 1, k)];
+          fc[2][im] = 0.5 * ff[2][im] + 0.5 * ff[2][extract_id(i, j, k - 1)];
+        }
+  }
+  /* u += f dt / rho, then Solenoidal_u(u) (sp_3d_ns.cxx:83-94) */
+  void add_force_to_fluid(const kapsel_time &jikan) {
+    double dmy = jikan.dt_fluid * IRHO;
+#pragma omp parallel for
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++)
+        for (int k = 0; k < NZ; k++) {
+          size_t im = ((size_t)i * NY + j) * NZ_ + k;
+          for (int d = 0; d < 3; d++) U(d)[im] += FP(d)[im] * dmy;
+        }
+    double *u[3] = {U(0), U(1), U(2)};
+    solenoidal_u(u);
+  }
+  /* ICEP step without ions: the Navier-Stokes step of Time_evolution_hydro with the
+   * dielectric force of the Electrolyte+Dielectric branch inserted after the profile
+   * scatter (sp_3d_ns.cxx:51-95): eps field, Poisson solve warm-started from the
+   * previous potential, Maxwell force, u += f dt / rho, projection */
+  int icep_step(kapsel_time &jikan, const double *e0, double tol, int maxit, int solver, double *relres) {
+    ns_slaved_euler(jikan);
+    for (int d = 0; d < 3; d++)
+      if (P.fix_cell[d]) uk_dc[d] = 0.0;
+    truncate(fld(KAPSEL_ZETA0));
+    truncate(fld(KAPSEL_ZETA1));
+    zeta_k2u();
+    md_position(jikan);
+    reset(fld(KAPSEL_PHI));
+    reset(fld(KAPSEL_PHI_SUM));
+    make_phi_particle_sum(fld(KAPSEL_PHI), fld(KAPSEL_PHI_SUM));
+    make_dielectric_field(fld(KAPSEL_EPSILON), fld(KAPSEL_WORK4));
+    build_rhs(fld(KAPSEL_RHS), fld(KAPSEL_RHO_FREE), fld(KAPSEL_EPSILON), e0);
+    double *x = fld(KAPSEL_POTENTIAL);
+    int it = solver == 2 ? bicgstab_solve(x, fld(KAPSEL_RHS), tol, maxit, relres)
+                         : cg_solve(x, fld(KAPSEL_RHS), tol, maxit, solver, relres);
+    maxwell_force(e0);
+    add_force_to_fluid(jikan);
+    double *u[3] = {U(0), U(1), U(2)};
+    calc_f_hydro(fld(KAPSEL_PHI_SUM), u, jikan);
+    md_velocity(jikan);
+    for (int d = 0; d < 3; d++) reset(UP(d));
+    double *up[3] = {UP(0), UP(1), UP(2)};
+    make_u_particle_sum(up, fld(KAPSEL_PHI_SUM));
+    make_f_particle_dt_sole_add();
+    u2zeta_k();
+    jikan.time += jikan.dt_fluid;
+    jikan.ts += 1;
+    return it;
+  }
+
   /* main's initial rigidity projection (sp_3d_ns.cxx:431-447) */
   void init_rigidity() {
     reset(fld(KAPSEL_PHI));
@@ -1290,6 +1401,19 @@ int oracle_poisson_solve(void *h, const double *e0, double tol, int maxit, int s
     it = solver == 2 ? O->bicgstab_solve(x, b, tol, maxit, relres) : O->cg_solve(x, b, tol, maxit, solver, relres);
   }
   *iters = it;
+  return 0;
+}
+int oracle_maxwell_force(void *h, const double *e0) {
+  O->maxwell_force(e0);
+  return 0;
+}
+int oracle_add_force_to_fluid(void *h, const kapsel_time *t) {
+  O->add_force_to_fluid(*t);
+  return 0;
+}
+int oracle_icep_step(void *h, kapsel_time *t, const double *e0, double tol, int maxit, int solver, int *iters,
+                     double *relres) {
+  *iters = O->icep_step(*t, e0, tol, maxit, solver, relres);
   return 0;
 }
 int oracle_init_rigidity(void *h) { O->init_rigidity(); return 0; }

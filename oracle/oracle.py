@@ -51,6 +51,8 @@ def _load():
             "oracle_make_dielectric_field": [], "oracle_poisson_apply": [i, i],
             "oracle_ax_unpadded": [dp, dp], "oracle_poisson_rhs": [dp],
             "oracle_poisson_solve": [dp, ctypes.c_double, i, i, ip, dp],
+            "oracle_maxwell_force": [dp], "oracle_add_force_to_fluid": [tp],
+            "oracle_icep_step": [tp, dp, ctypes.c_double, i, i, ip, dp],
         }
         for name, args in protos.items():
             fn = getattr(lib, name)
@@ -211,6 +213,21 @@ class Oracle:
     def poisson_rhs(self, e0):
         e = np.ascontiguousarray(e0, dtype=np.float64)
         self._call("poisson_rhs", _dp(e))
+
+    def maxwell_force(self, e0):
+        e = np.ascontiguousarray(e0, dtype=np.float64)
+        self._call("maxwell_force", _dp(e))
+
+    def add_force_to_fluid(self, t):
+        self._call("add_force_to_fluid", ctypes.byref(t))
+
+    def icep_step(self, t, e0, tol, maxit, solver=0):
+        e = np.ascontiguousarray(e0, dtype=np.float64)
+        it = ctypes.c_int(0)
+        rr = ctypes.c_double(0.0)
+        self._call("icep_step", ctypes.byref(t), _dp(e), tol, int(maxit), int(solver), ctypes.byref(it),
+                   ctypes.byref(rr))
+        return it.value, rr.value
 
     def poisson_solve(self, e0, tol, maxit, solver=0):
         """solver 0 = CG, 1 = spectral-preconditioned CG, 2 = BiCGSTAB; returns (iters, relres)."""

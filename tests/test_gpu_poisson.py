@@ -114,3 +114,51 @@ def test_poisson_solve_128_dielectric_spheres():
     assert rr0 < 1e-10 and rr1 < 1e-10
     assert it1 * 5 < it0
     assert rel(x1 - x1.mean(), x0 - x0.mean()) < 1e-8
+
+
+def test_maxwell_force_matches_oracle():
+    n = 64
+    print("case maxwell force 64^3", flush=True)
+    p, st = _case(n, 6, 6.0, 0.1, 21)
+    k, o = gpu(p), oracle(p)
+    e0 = [0.02, 0.0, 0.1]
+    o.set_particles(st)
+    o.make_dielectric_field()
+    o.poisson_solve(e0, 1e-10, 5000, 1)
+    rng = np.random.default_rng(8)
+    rho = rng.standard_normal((n, n, n)) * 1e-3
+    for s in (k, o):
+        s.set_particles(st)
+        s.make_dielectric_field()
+        s.set_field("POTENTIAL", o.get_field("POTENTIAL"))
+        s.set_field("RHO_FREE", sp.pad_real(rho))
+        s.maxwell_force(e0)
+    for f in ("F0", "F1", "F2"):
+        assert rel(k.get_field(f)[:, :, :n], o.get_field(f)[:, :, :n]) < 1e-12, f
+
+
+def test_icep_step_fp64_matches_oracle():
+    """Two ICEP steps (64^3, 6 dielectric spheres a=6): fields and particles
+    agree to 1e-9 (the solves stop at 1e-12, not at bit-identical iterates)."""
+    n = 64
+    print("case icep step 64^3 fp64", flush=True)
+    p, st = _case(n, 6, 6.0, 0.1, 22)
+    k, o = gpu(p), oracle(p)
+    e0 = [0.0, 0.05, 0.1]
+    for s in (k, o):
+        s.set_particles(st)
+        s.init_rigidity()
+    tk, to = abi.make_time(0.05), abi.make_time(0.05)
+    for _ in range(2):
+        itk, rrk = k.icep_step(tk, e0, 1e-12, 5000, 1)
+        ito, rro = o.icep_step(to, e0, 1e-12, 5000, 1)
+        print(f"icep step: gpu {itk} it {rrk:.1e}, oracle {ito} it {rro:.1e}", flush=True)
+        assert rrk < 1e-12
+    for f in ("ZETA0", "ZETA1", "EPSILON", "F0", "F1", "F2"):
+        assert rel(k.get_field(f), o.get_field(f)) < 1e-9, f
+    xk, xo = k.get_field("POTENTIAL")[:, :, :n], o.get_field("POTENTIAL")[:, :, :n]
+    assert rel(xk - xk.mean(), xo - xo.mean()) < 1e-9
+    pk, po = k.get_particles(), o.get_particles()
+    for f in ("X", "V", "OMEGA"):
+        i = abi.PFIELD[f]
+        assert rel(pk[i], po[i]) < 1e-9, f

@@ -119,3 +119,52 @@ def test_solvers_agree_and_converge():
     assert np.abs(sols[1][0] - sols[0][0]).max() < 1e-8 * scale
     assert np.abs(sols[2][0] - sols[0][0]).max() < 1e-8 * scale
     assert sols[1][1] < sols[0][1] / 4  # the spectral preconditioner cuts iterations
+
+
+def test_maxwell_force_closed_forms():
+    """f = rho E - E^2 grad(eps) / 2 on the staggered grid, then averaged to the
+    cell centres (operate_dielectric.cxx:646-763), in two closed forms."""
+    n, m, d, e0z = 32, 3, 0.4, 0.7
+    z = np.arange(n)
+    o = _oracle(abi.make_params(n))
+    # (1) zero potential, eps = 1.3 + d cos(2 pi m z / N), E0 along z:
+    #     f_z = (E0^2 / 2) d (2 pi m / N) sin(2 pi m z / N) cos(pi m / N)
+    eps = np.broadcast_to(1.3 + d * np.cos(2 * np.pi * m * z / n), (n, n, n))
+    o.set_field("EPSILON", sp.pad_real(eps))
+    o.set_field("POTENTIAL", np.zeros(o.shape))
+    o.set_field("RHO_FREE", np.zeros(o.shape))
+    o.maxwell_force([0.0, 0.0, e0z])
+    fz = o.get_field("F2")[:, :, :n]
+    expect = 0.5 * e0z ** 2 * d * (2 * np.pi * m / n) * np.sin(2 * np.pi * m * z / n) * np.cos(np.pi * m / n)
+    assert np.abs(fz - np.broadcast_to(expect, fz.shape)).max() < 1e-13
+    assert np.abs(o.get_field("F0")[:, :, :n]).max() < 1e-13
+    # (2) uniform eps, charge rho(z): f_z = (rho(z+1) + 2 rho(z) + rho(z-1)) / 4 * E0
+    rng = np.random.default_rng(4)
+    rho1 = rng.standard_normal(n)
+    o.set_field("EPSILON", np.full(o.shape, 2.0))
+    o.set_field("RHO_FREE", sp.pad_real(np.broadcast_to(rho1, (n, n, n))))
+    o.maxwell_force([0.0, 0.0, e0z])
+    fz = o.get_field("F2")[:, :, :n]
+    expect = 0.25 * (np.roll(rho1, -1) + 2 * rho1 + np.roll(rho1, 1)) * e0z
+    assert np.abs(fz - np.broadcast_to(expect, fz.shape)).max() < 1e-13
+
+
+def test_icep_step_runs_and_keeps_solenoidal_velocity():
+    n = 32
+    p = abi.make_params(n, n_particles=3, radius=4.0, xi=2.0, eps_fluid=1.0, eps_particle=0.1)
+    pos = workloads.random_positions(workloads.Xoshiro256(9), 3, (n, n, n), 8.0)
+    o = _oracle(p)
+    o.set_particles(workloads.particle_state(pos))
+    o.init_rigidity()
+    t = abi.make_time(0.05)
+    for _ in range(2):
+        it, rr = o.icep_step(t, [0.0, 0.0, 0.1], 1e-10, 5000, 1)
+        assert rr < 1e-10 and it > 0
+    # U2zeta_k leaves u in reciprocal space (operate_omega.h:203-215)
+    uk = np.stack([sp.to_complex(o.get_field(f"U{d}")) for d in range(3)])
+    u = uk
+    _, (kx, ky, kz) = sp.wavevectors((n, n, n))
+    div = kx * uk[0] + ky * uk[1] + kz * uk[2]
+    assert np.abs(div).max() < 1e-10 * max(1e-30, np.abs(uk).max())
+    # the dielectric force moved the fluid (it is zero without a field gradient)
+    assert np.abs(u).max() > 0
