@@ -133,6 +133,7 @@ _PROTOS = {
     "kapsel_poisson_apply_host": [_DP, _DP],
     "kapsel_poisson_rhs": [_DP],
     "kapsel_poisson_solve": [_DP, _D, _I, _I, _IP, _DP],
+    "kapsel_set_dielectric": [_D, _D, _D],
     "kapsel_maxwell_force": [_DP],
     "kapsel_icep_step": [ctypes.POINTER(Time), _DP, _D, _I, _I, _IP, _DP],
     "kapsel_pnet_init": [ctypes.POINTER(ctypes.c_long)],

@@ -769,6 +769,19 @@ int kapsel_pnet_warm_start(kapsel_ctx *ctx, double beta, const double e0[3], dou
   API_END
 }
 
+int kapsel_set_dielectric(kapsel_ctx *ctx, double eps_fluid, double eps_particle, double dielectric_cst) {
+  API_BEGIN
+  check_ctx(ctx);
+  if (!(eps_fluid > 0) || !(eps_particle > 0) || !(dielectric_cst > 0)) {
+    set_error("permittivities must be positive");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+  ctx->P.eps_fluid = eps_fluid;
+  ctx->P.eps_particle = eps_particle;
+  ctx->P.dielectric_cst = dielectric_cst;
+  API_END
+}
+
 int kapsel_maxwell_force(kapsel_ctx *ctx, const double e0[3]) {
   API_BEGIN
   check_ctx(ctx);

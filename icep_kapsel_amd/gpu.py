@@ -177,6 +177,9 @@ class KapselGPU:
             raise KapselError(f"kapsel_poisson_solve: {rc}: {self.lib.kapsel_last_error().decode()}")
         return it.value, rr.value
 
+    def set_dielectric(self, eps_fluid, eps_particle, dielectric_cst=1.0):
+        self._call("set_dielectric", float(eps_fluid), float(eps_particle), float(dielectric_cst))
+
     def maxwell_force(self, e0):
         e = np.ascontiguousarray(e0, dtype=np.float64)
         self._call("maxwell_force", _dp(e))

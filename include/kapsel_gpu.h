@@ -197,6 +197,9 @@ int kapsel_poisson_rhs(kapsel_ctx *ctx, const double e0[3]);
 int kapsel_poisson_solve(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int precond, int *iters,
                          double *relres);
 
+/* Sets eps_fluid, eps_particle_top and Dielectric_cst (input.cxx:688-736)
+ * for the following kapsel_dielectric_field calls. */
+int kapsel_set_dielectric(kapsel_ctx *ctx, double eps_fluid, double eps_particle, double dielectric_cst);
 /* Make_Maxwell_force_x_on_fluid after the solve (operate_dielectric.cxx:646-763,
  * AC off, External_field on): staggered gradients of EPSILON and POTENTIAL,
  * face force rho E - E^2 grad(eps) / 2, averaged to the cell centres into
