@@ -1,14 +1,29 @@
-"""SPM step throughput on one MI355X.
+"""ICEP step throughput on one MI355X: FP32 + learned warm start vs FP64 plain.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W]
 
-Workload: the 128^3 SPM Navier-Stokes step with 200 spheres (a=4, xi=2,
-Gaussian particle velocities std 1e-3, random non-overlapping positions):
-slaved-Euler NS update, profile scatter, hydrodynamic gather, LJ, particle
-update, rigidity constraint and projection -- Time_evolution_hydro of the
-reference (sp_3d_ns.cxx:24-186).  Inputs are resident in HBM before timing.
-The FP32 path (float fields, FP64 particle reductions) is the headline value;
-the FP64 path is timed the same way beside it.  Prints one JSON line.
+Workload: the ICEP box -- 128^3 grid, 64 dielectric spheres a=6 xi=2
+(eps_f = 1, eps_p = 0.1, no free charge), E0 of magnitude 0.1 along a seeded
+random direction, eta = rho = 1, dt = 0.05, WCA LJ.  One step is the
+reference's Navier-Stokes step with the dielectric force (sp_3d_ns.cxx:24-186
+with the Electrolyte+Dielectric insert :51-95 and no ions): slaved-Euler NS
+update, profile scatter, permittivity field, variable-permittivity Poisson
+solve warm-started from the previous potential, Maxwell force, projection,
+hydrodynamic gather, LJ, particle update and rigidity constraint.  Both
+paths stop the Poisson solve at the same bound, relative residual 1e-8
+checked in FP64.
+
+  value : FP32 context (FP32 fields and FFTs, FP64 particle reductions),
+          spectral-preconditioned CG in FP32 inside FP64 iterative
+          refinement, first (cold) solve warm-started by the trained network
+  base  : FP64 context, plain CG (no preconditioner), cold solve from zero
+
+Both runs start from the same state and do the same W + K steps; the line
+reports their particle-velocity and vorticity differences.  Beside them:
+cold solves of held-out configurations (zero vs network warm start), and the
+FP64 CPU oracle (the reference's BiCGSTAB) timed on one step from the GPU's
+state after the warm-up.  Inputs are resident in HBM before timing.  Prints
+one JSON line.
 """
 import argparse
 import json
@@ -21,13 +36,158 @@ sys.path.insert(0, ROOT)
 
 METRIC = "SPM steps/s at 128^3: FP32+learned-precond vs FP64 plain, same error bound"
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+TOL = 1e-8
+MAXIT = 20000
+WEIGHTS = os.path.join(ROOT, "icep_kapsel_amd", "weights", "pnet_icep_128.f32")
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "icep_step_128_fp32_pmc_traffic.md")
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_gpu(precision, steps, warmup, timed_class=None):
+def beta_of(ratio):
+    return (ratio - 1.0) / (ratio + 2.0)
+
+
+def load_network(k):
+    import numpy as np
+    if not os.path.exists(WEIGHTS):
+        return False
+    n = k.pnet_init()
+    w = np.fromfile(WEIGHTS, dtype=np.float32)
+    if w.size != n:
+        raise RuntimeError(f"{WEIGHTS}: {w.size} parameters, the network has {n}")
+    k.pnet_set_params(w)
+    return True
+
+
+def run_icep(precision, precond, steps, warmup, learned, timed_class=None, snapshot=False):
+    import numpy as np
+    from icep_kapsel_amd import abi, workloads
+    from icep_kapsel_amd.gpu import KapselGPU
+
+    p, st, e0, dt = workloads.icep_config(precision=precision)
+    ratio = p.eps_particle / p.eps_fluid
+    k = KapselGPU(p)
+    k.set_particles(st)
+    k.init_rigidity()
+    t = abi.make_time(dt)
+    used_net = load_network(k) if learned else False
+    k.synchronize()
+    iters = []
+    t_cold = time.perf_counter()
+    if used_net:
+        # cold start: the network's potential for the initial configuration
+        k.make_dielectric_field()
+        k.pnet_warm_start(beta_of(ratio), e0, p.radius)
+        k.poisson_galerkin_start(e0)
+    for i in range(warmup):
+        it, rr = k.icep_step(t, e0, TOL, MAXIT, precond)
+        iters.append(it)
+        if i == 0:
+            k.synchronize()
+            t_cold = time.perf_counter() - t_cold
+    k.synchronize()
+    snap = None
+    if snapshot:
+        snap = {"ZETA0": k.get_field("ZETA0"), "ZETA1": k.get_field("ZETA1"), "uk_dc": k.get_uk_dc(),
+                "particles": k.get_particles(), "POTENTIAL": k.get_field("POTENTIAL"), "ts": t.ts,
+                "time": t.time}
+    # the kernel class with the largest share of device time in one step
+    k.kernel_timing(abi.KERNEL_CLASSES)
+    it, rr = k.icep_step(t, e0, TOL, MAXIT, precond)
+    iters.append(it)
+    shares = {c: k.kernel_timing_get(c) for c in abi.KERNEL_CLASSES}
+    k.kernel_timing([])
+    dominant = timed_class or max(shares, key=lambda c: shares[c][0])
+    # timed region: only the dominant class carries HIP events
+    k.kernel_timing([dominant])
+    k.synchronize()
+    timed_iters, relres = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        it, rr = k.icep_step(t, e0, TOL, MAXIT, precond)
+        timed_iters.append(it)
+        relres.append(rr)
+    k.synchronize()
+    t1 = time.perf_counter()
+    ms, launches, nbytes = k.kernel_timing_get(dominant)
+    k.kernel_timing([])
+    v = k.get_particles()[abi.PFIELD["V"]]
+    zeta = np.stack([k.get_field("ZETA0"), k.get_field("ZETA1")])
+    k.close()
+    return {
+        "steps_per_s": steps / (t1 - t0),
+        "ms_per_step": 1e3 * (t1 - t0) / steps,
+        "first_step_s": t_cold,
+        "warmup_iters": iters,
+        "iters_mean": float(np.mean(timed_iters)),
+        "relres_max": float(np.max(relres)),
+        "dominant": dominant,
+        "dominant_ms": ms,
+        "dominant_launches": launches,
+        "dominant_bytes": nbytes,
+        "shares_ms": {c: v_[0] for c, v_ in shares.items()},
+        "used_network": used_net,
+        "v": v,
+        "zeta": zeta,
+        "snapshot": snap,
+    }
+
+
+def cold_solves(n_samples):
+    """Held-out configurations of the training distribution (seed 4104): cold
+    solves to 1e-8 from zero and from the network, FP32 spectral PCG with
+    FP64 refinement; FP64 plain CG from zero beside them."""
+    import numpy as np
+    from icep_kapsel_amd import abi, train_pnet
+    from icep_kapsel_amd.gpu import KapselGPU
+
+    k32 = train_pnet.make_context(128)
+    if not load_network(k32):
+        k32.close()
+        return None
+    p64 = abi.make_params(128, n_particles=train_pnet.NPART, radius=train_pnet.A, xi=train_pnet.XI,
+                          eps_fluid=1.0, eps_particle=0.1, precision=abi.KAPSEL_FP64)
+    k64 = KapselGPU(p64)
+    rec = []
+    for s in range(n_samples):
+        pos, ratio, e_hat = train_pnet.sample(4104 * 100000 + s, 128)
+        e0 = train_pnet.E_MAG * e_hat
+        r = {"ratio": ratio}
+        train_pnet.prepare(k32, pos, ratio)
+        for mode in ("zero", "network", "network_galerkin"):
+            k32.set_field("POTENTIAL", np.zeros(k32.shape))
+            k32.synchronize()
+            t0 = time.perf_counter()
+            if mode != "zero":
+                k32.pnet_warm_start(beta_of(ratio), e0, train_pnet.A)
+            if mode == "network_galerkin":
+                k32.poisson_galerkin_start(e0)
+            it, rr = k32.poisson_solve(e0, TOL, MAXIT, abi.PRECOND_SPECTRAL)
+            k32.synchronize()
+            r[f"fp32_pcg_{mode}_iters"] = it
+            r[f"fp32_pcg_{mode}_s"] = time.perf_counter() - t0
+        train_pnet.prepare(k64, pos, ratio)
+        k64.set_field("POTENTIAL", np.zeros(k64.shape))
+        k64.synchronize()
+        t0 = time.perf_counter()
+        it, rr = k64.poisson_solve(e0, TOL, MAXIT, abi.PRECOND_NONE)
+        k64.synchronize()
+        r["fp64_cg_zero_iters"] = it
+        r["fp64_cg_zero_s"] = time.perf_counter() - t0
+        rec.append(r)
+        log(f"bench: cold solve {s}: " + ", ".join(f"{a} {b:.4g}" for a, b in r.items()))
+    k32.close()
+    k64.close()
+    keys = [k_ for k_ in rec[0] if k_ != "ratio"]
+    return {"samples": n_samples, "seed": 4104, "tol": TOL,
+            "mean": {k_: float(np.mean([r[k_] for r in rec])) for k_ in keys}, "per_sample": rec}
+
+
+def run_spm(precision, steps, warmup):
+    """The 128^3 x 200-sphere SPM Navier-Stokes step without the Poisson solve."""
     from icep_kapsel_amd import abi, workloads
     from icep_kapsel_amd.gpu import KapselGPU
 
@@ -39,32 +199,47 @@ def run_gpu(precision, steps, warmup, timed_class=None):
     for _ in range(warmup):
         k.spm_step(t)
     k.synchronize()
-    # find the kernel class with the largest share of device time
-    k.kernel_timing(abi.KERNEL_CLASSES)
-    k.spm_step(t)
-    shares = {c: k.kernel_timing_get(c) for c in abi.KERNEL_CLASSES}
-    k.kernel_timing([])
-    dominant = timed_class or max(shares, key=lambda c: shares[c][0])
-    # timed region: only the dominant class carries HIP events
-    k.kernel_timing([dominant])
-    k.synchronize()
     t0 = time.perf_counter()
-    for i in range(steps):
+    for _ in range(steps):
         k.spm_step(t)
     k.synchronize()
     t1 = time.perf_counter()
-    ms, launches, nbytes = k.kernel_timing_get(dominant)
-    k.kernel_timing([])
     k.close()
-    return {
-        "steps_per_s": steps / (t1 - t0),
-        "ms_per_step": 1e3 * (t1 - t0) / steps,
-        "dominant": dominant,
-        "dominant_ms": ms,
-        "dominant_launches": launches,
-        "dominant_bytes": nbytes,
-        "shares_ms": {c: v[0] for c, v in shares.items()},
-    }
+    return steps / (t1 - t0)
+
+
+def run_cpu(snap):
+    """One FP64 oracle ICEP step (reference BiCGSTAB, identity preconditioner)
+    from the GPU FP64 state after the warm-up steps."""
+    from icep_kapsel_amd import abi, workloads
+    from oracle import oracle as omod
+
+    p, st, e0, dt = workloads.icep_config()
+    o = omod.Oracle(p)
+    o.set_particles(snap["particles"])
+    o.set_field("ZETA0", snap["ZETA0"])
+    o.set_field("ZETA1", snap["ZETA1"])
+    o.set_uk_dc(snap["uk_dc"])
+    o.set_field("POTENTIAL", snap["POTENTIAL"])
+    t = abi.make_time(dt, ts=snap["ts"], time=snap["time"])
+    t0 = time.perf_counter()
+    it, rr = o.icep_step(t, e0, TOL, MAXIT, 2)
+    t1 = time.perf_counter()
+    return 1.0 / (t1 - t0), omod.num_threads(), it
+
+
+def pmc_traffic(kernel_class):
+    """HBM bytes per launch of the dominant class from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (lines 'traffic <class> <bytes>')."""
+    try:
+        with open(PMC_SUMMARY) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic" and parts[1] == kernel_class:
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
 
 
 def copy_calibration(k):
@@ -80,31 +255,20 @@ def copy_calibration(k):
     g.close()
 
 
-def run_cpu(steps):
-    from icep_kapsel_amd import abi, workloads
-    from oracle import oracle as omod
-
-    p, st, dt = workloads.spm_config("c2")
-    o = omod.Oracle(p)
-    o.set_particles(st)
-    o.init_rigidity()
-    t = abi.make_time(dt)
-    o.spm_step(t)  # warm-up
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        o.spm_step(t)
-    t1 = time.perf_counter()
-    return steps / (t1 - t0), omod.num_threads()
+def rel_diff(a, b):
+    import numpy as np
+    return float(np.abs(a - b).max() / max(1e-300, np.abs(b).max()))
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cold", type=int, default=3, help="held-out cold solves (zero vs network warm start)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 plain run")
+    ap.add_argument("--no-spm", action="store_true", help="skip the SPM-only step timings")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="K",
                     help="only run K plain field copies (128^3 FP32, 8.52 MB each way) to calibrate "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE, then exit")
@@ -114,28 +278,39 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the SPM step at these sizes runs on a single MI355X).", file=sys.stderr)
+              "(the 128^3 ICEP step runs on a single MI355X).", file=sys.stderr)
         sys.exit(2)
     from icep_kapsel_amd import abi
 
-    log(f"bench: fp32 SPM step 128^3 x 200 spheres, warmup {args.warmup}, steps {args.steps}")
-    r32 = run_gpu(abi.KAPSEL_FP32, args.steps, args.warmup)
-    log(f"bench: fp32 {r32['steps_per_s']:.1f} steps/s; dominant {r32['dominant']}")
+    log(f"bench: FP32 + learned ICEP step 128^3 x 64 spheres, warmup {args.warmup}, steps {args.steps}")
+    r32 = run_icep(abi.KAPSEL_FP32, abi.PRECOND_SPECTRAL, args.steps, args.warmup, learned=True)
+    log(f"bench: fp32 {r32['steps_per_s']:.2f} steps/s, {r32['iters_mean']:.1f} inner it/step, "
+        f"warm-up iterations {r32['warmup_iters']}, dominant {r32['dominant']}")
     r64 = None
     if not args.no_fp64:
-        log("bench: fp64 SPM step")
-        r64 = run_gpu(abi.KAPSEL_FP64, args.steps, args.warmup, timed_class=r32["dominant"])
-        log(f"bench: fp64 {r64['steps_per_s']:.1f} steps/s")
+        log("bench: FP64 plain CG ICEP step")
+        r64 = run_icep(abi.KAPSEL_FP64, abi.PRECOND_NONE, args.steps, args.warmup, learned=False,
+                       timed_class=r32["dominant"], snapshot=not args.no_cpu)
+        log(f"bench: fp64 plain {r64['steps_per_s']:.2f} steps/s, {r64['iters_mean']:.1f} it/step, "
+            f"warm-up iterations {r64['warmup_iters']}")
+    cold = cold_solves(args.cold) if args.cold else None
+    spm = None
+    if not args.no_spm:
+        log("bench: SPM step (128^3 x 200 spheres, no Poisson) FP32 / FP64")
+        spm = {"fp32_steps_per_s": run_spm(abi.KAPSEL_FP32, 50, 5),
+               "fp64_steps_per_s": run_spm(abi.KAPSEL_FP64, 50, 5)}
     cpu = None
-    if not args.no_cpu:
-        log(f"bench: CPU oracle baseline, {args.cpu_steps} steps after 1 warm-up")
-        cps, cores = run_cpu(args.cpu_steps)
+    if not args.no_cpu and r64 is not None:
+        log("bench: CPU oracle, one ICEP step from the GPU FP64 state after the warm-up")
+        cps, cores, cit = run_cpu(r64["snapshot"])
         cpu = {"value": cps, "unit": "steps/s", "cores": cores, "kind": "port",
-               "sample": f"{args.cpu_steps} FP64 SPM steps of the same 128^3 x 200-sphere workload "
-                         "(after 1 warm-up step), OpenMP C++ oracle"}
-        log(f"bench: cpu {cps:.3f} steps/s on {cores} threads")
+               "sample": f"1 FP64 ICEP step of the same 128^3 x 64-sphere workload from the state after "
+                         f"{args.warmup} warm-up steps, reference BiCGSTAB to 1e-8 ({cit} iterations), "
+                         "OpenMP C++ oracle"}
+        log(f"bench: cpu {cps:.4f} steps/s on {cores} threads ({cit} BiCGSTAB iterations)")
     avg_ms = r32["dominant_ms"] / max(1, r32["dominant_launches"])
     achieved = (r32["dominant_bytes"] / 1e9) / (r32["dominant_ms"] / 1e3) if r32["dominant_ms"] > 0 else 0.0
+    traffic = pmc_traffic(r32["dominant"])
     out = {
         "metric": METRIC,
         "value": r32["steps_per_s"],
@@ -149,20 +324,40 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": "spm_ns_step_128^3_200_spheres_a4_xi2", "grid": 128, "particles": 200,
-                   "radius": 4.0, "xi": 2.0, "dt": 0.05, "poisson": "not in this step (NS branch)",
-                   "parallelism": "single_gpu"},
+        "config": {"workload": "icep_step_128^3_64_dielectric_spheres_a6_xi2_eps0.1_E0.1", "grid": 128,
+                   "particles": 64, "radius": 6.0, "xi": 2.0, "eps_ratio": 0.1, "e0_magnitude": 0.1,
+                   "dt": 0.05, "poisson_tol": TOL, "seed": 3003,
+                   "solver": "fp32 spectral PCG in fp64 refinement, learned cold start",
+                   "baseline_solver": "fp64 plain CG", "parallelism": "single_gpu"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic,
                      "kernel": f"class {r32['dominant']}", "avg_launch_ms": avg_ms,
+                     "algorithmic_bytes_per_launch": r32["dominant_bytes"] / max(1, r32["dominant_launches"]),
                      "launches": r32["dominant_launches"]},
         "cpu_baseline": cpu,
-        "fp64_steps_per_s": r64["steps_per_s"] if r64 else None,
-        "fp64_ms_per_step": r64["ms_per_step"] if r64 else None,
-        "fp32_over_fp64": r32["steps_per_s"] / r64["steps_per_s"] if r64 else None,
+        "fp32_inner_iters_per_step": r32["iters_mean"],
+        "fp32_relres_max": r32["relres_max"],
+        "fp32_first_step_s": r32["first_step_s"],
+        "learned_warm_start_used": r32["used_network"],
         "kernel_class_ms_per_step_fp32": r32["shares_ms"],
-        "kernel_class_ms_per_step_fp64": r64["shares_ms"] if r64 else None,
     }
+    if r64:
+        out.update({
+            "fp64_plain_steps_per_s": r64["steps_per_s"],
+            "fp64_plain_ms_per_step": r64["ms_per_step"],
+            "fp64_plain_iters_per_step": r64["iters_mean"],
+            "fp64_plain_relres_max": r64["relres_max"],
+            "fp64_plain_first_step_s": r64["first_step_s"],
+            "speedup_vs_fp64_plain": r32["steps_per_s"] / r64["steps_per_s"],
+            "particle_velocity_rel_err_vs_fp64": rel_diff(r32["v"], r64["v"]),
+            "vorticity_rel_err_vs_fp64": rel_diff(r32["zeta"], r64["zeta"]),
+            "kernel_class_ms_per_step_fp64": r64["shares_ms"],
+        })
+    if cold:
+        out["cold_solve_1e-8"] = cold["mean"]
+    if spm:
+        out["spm_ns_step_128_200_spheres"] = spm
     print(json.dumps(out), flush=True)
 
 

@@ -142,6 +142,7 @@ _PROTOS = {
     "kapsel_pnet_loss_grad": [_D, _DP, _D, _DP, ctypes.POINTER(ctypes.c_float)],
     "kapsel_pnet_train_step": [_D, _DP, _D, _D, _DP],
     "kapsel_pnet_warm_start": [_D, _DP, _D],
+    "kapsel_poisson_galerkin_start": [_DP, _DP],
     "kapsel_kernel_timing": [_I],
     "kapsel_kernel_timing_get": [_I, _DP, _IP, _DP],
 }

@@ -101,6 +101,11 @@ static void check_field(int f) {
   }
 }
 
+// a write to the FP32 potential through the API supersedes its FP64 shadow
+static void written(kapsel_ctx *ctx, int f) {
+  if (f == KAPSEL_POTENTIAL) ctx->pot64_valid = false;
+}
+
 static void destroy(kapsel_ctx *ctx) {
   if (!ctx) return;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
@@ -417,6 +422,7 @@ int kapsel_set_field(kapsel_ctx *ctx, int field, const double *host) {
   API_BEGIN
   check_ctx(ctx);
   check_field(field);
+  written(ctx, field);
   if (!host) {
     set_error("null host buffer");
     throw Error{KAPSEL_ERR_ARG};
@@ -493,6 +499,7 @@ int kapsel_copy_field(kapsel_ctx *ctx, int dst, int src) {
   check_ctx(ctx);
   check_field(dst);
   check_field(src);
+  written(ctx, dst);
   copy_field(ctx, ctx->fields[dst], ctx->fields[src]);
   API_END
 }
@@ -501,6 +508,7 @@ int kapsel_fft_r2c(kapsel_ctx *ctx, int field) {
   API_BEGIN
   check_ctx(ctx);
   check_field(field);
+  written(ctx, field);
   fft_r2c(ctx, ctx->fields + field, 1);
   API_END
 }
@@ -509,6 +517,7 @@ int kapsel_fft_c2r(kapsel_ctx *ctx, int field) {
   API_BEGIN
   check_ctx(ctx);
   check_field(field);
+  written(ctx, field);
   fft_c2r(ctx, ctx->fields + field, 1);
   API_END
 }
@@ -517,6 +526,7 @@ int kapsel_truncate(kapsel_ctx *ctx, int field) {
   API_BEGIN
   check_ctx(ctx);
   check_field(field);
+  written(ctx, field);
   truncate(ctx, ctx->fields + field, 1);
   API_END
 }
@@ -644,6 +654,7 @@ int kapsel_poisson_apply(kapsel_ctx *ctx, int x_field, int ax_field) {
     set_error("poisson_apply needs distinct input and output fields");
     throw Error{KAPSEL_ERR_ARG};
   }
+  written(ctx, ax_field);
   poisson_apply(ctx, ctx->fields[x_field], ctx->fields[ax_field]);
   API_END
 }
@@ -765,7 +776,20 @@ int kapsel_pnet_warm_start(kapsel_ctx *ctx, double beta, const double e0[3], dou
     set_error("null field or non-positive scale");
     throw Error{KAPSEL_ERR_ARG};
   }
+  written(ctx, KAPSEL_POTENTIAL);
   pnet_warm_start(ctx, beta, e0, scale);
+  API_END
+}
+
+int kapsel_poisson_galerkin_start(kapsel_ctx *ctx, const double e0[3], double *gamma) {
+  API_BEGIN
+  check_ctx(ctx);
+  if (!e0) {
+    set_error("null field");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+  const double g = galerkin_scale_potential(ctx, e0);
+  if (gamma) *gamma = g;
   API_END
 }
 

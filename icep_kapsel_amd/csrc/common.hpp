@@ -138,6 +138,9 @@ struct kapsel_ctx {
   // of an FP32 context
   void *pscratch[5] = {};          // [0..3] operator scratch, [4] refinement correction
   double *p64[8] = {};            // eps, x, b, r, 4 operator scratch
+  // FP32 context: p64[1] holds the FP64 potential of the last solve; it seeds
+  // the next solve unless the FP32 POTENTIAL field was written since
+  bool pot64_valid = false;
   double *d_partial = nullptr;    // block partial sums (reductions)
   double *d_scal = nullptr;       // CG scalars
   int n_partial = 0;
@@ -212,6 +215,7 @@ void poisson_rhs(kapsel_ctx *ctx, const double e0[3]);
 void poisson_solve(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int precond, int *iters,
                    double *relres);
 void poisson_free(kapsel_ctx *ctx);
+double galerkin_scale_potential(kapsel_ctx *ctx, const double e0[3]);
 void maxwell_force(kapsel_ctx *ctx, const double e0[3]);
 void add_force_to_u(kapsel_ctx *ctx, double s);
 

@@ -10,6 +10,9 @@
 //   e3a 32->32,    e3b 32->32        (N/4)^3 | up, concat [up | s2]
 //   d2  64->32                       (N/2)^3 | up, concat [up | s1]
 //   d1  48->16, out 16->3 (linear)   N^3
+// Hidden layers use a leaky ReLU, x > 0 ? x : LEAK x: with a plain ReLU,
+// training let whole layers die (the input is exactly zero in the fluid, so
+// a layer's output there is its activated bias).
 // The input's 2 channels are stored padded to 16 (zero channels and zero
 // weights) so every convolution is a 16-multiple implicit GEMM.
 //
@@ -34,6 +37,8 @@ typedef short s4v __attribute__((ext_vector_type(4)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef uint16_t bf16;
 
+constexpr float LEAK = 0.1f;
+
 __device__ __forceinline__ float bf2f(bf16 h) { return __uint_as_float((uint32_t)h << 16); }
 __device__ __forceinline__ bf16 f2bf(float f) {
   uint32_t u = __float_as_uint(f);
@@ -52,9 +57,10 @@ __device__ __forceinline__ int shifted(int v, int n, int ox, int oy, int oz) {
   return (wrapi(i + ox, n) * n + wrapi(j + oy, n)) * n + wrapi(k + oz, n);
 }
 
-// out[v][co] = sum_tap sum_c in[v + s*o_tap][c] K[tap][co][c] (+ bias, relu),
-// s = +1 (forward) or -1 (input gradient).  mask (optional): zero where the
-// mask tensor (same layout as out) is <= 0.  out32 (optional) FP32 copy.
+// out[v][co] = sum_tap sum_c in[v + s*o_tap][c] K[tap][co][c] (+ bias, leaky
+// ReLU), s = +1 (forward) or -1 (input gradient).  mask (optional): scale by
+// LEAK where the mask tensor (an activation output, same layout as out) is
+// <= 0.  out32 (optional) FP32 copy.
 template <int CIN, int COUT>
 __global__ __launch_bounds__(256) void conv_mfma_kernel(const bf16 *__restrict__ in, int n,
                                                         const bf16 *__restrict__ K, const float *__restrict__ bias,
@@ -95,9 +101,9 @@ __global__ __launch_bounds__(256) void conv_mfma_kernel(const bf16 *__restrict__
     for (int r = 0; r < 4; r++) {
       const int v = v0 + 4 * kq + r;
       float x = acc[t][r] + bv;
-      if (relu && !(x > 0.f)) x = 0.f;
+      if (relu && !(x > 0.f)) x *= LEAK;
       const size_t o = (size_t)v * COUT + co;
-      if (mask && !(bf2f(mask[o]) > 0.f)) x = 0.f;
+      if (mask && !(bf2f(mask[o]) > 0.f)) x *= LEAK;
       out[o] = f2bf(x);
       if (out32) out32[o] = x;
     }
@@ -199,8 +205,9 @@ __global__ void pool_kernel(const bf16 *__restrict__ in, int n, int c, bf16 *__r
   out[e] = f2bf(0.125f * s);
 }
 
-// din (n^3 x c) += 1/8 dout(parent), masked by act > 0 (act = the pooled
-// tensor's producer output), written as din = mask * (din + ...)
+// din (n^3 x c) += act'(act) / 8 dout(parent), act = the pooled tensor's
+// producer output; din arrives holding the skip-connection gradient, which
+// the concat's input-gradient convolution has already passed through act'
 __global__ void pool_bwd_kernel(const bf16 *__restrict__ dout, int n, int c, const bf16 *__restrict__ act,
                                 bf16 *__restrict__ din) {
   const int m = n / 2;
@@ -209,8 +216,8 @@ __global__ void pool_bwd_kernel(const bf16 *__restrict__ dout, int n, int c, con
   const int ch = (int)(e % c);
   const int v = (int)(e / c);
   const int k = v % n, j = (v / n) % n, i = v / (n * n);
-  const float g = bf2f(din[e]) + 0.125f * bf2f(dout[((size_t)((i / 2) * m + j / 2) * m + k / 2) * c + ch]);
-  din[e] = f2bf(bf2f(act[e]) > 0.f ? g : 0.f);
+  const float slope = bf2f(act[e]) > 0.f ? 0.125f : 0.125f * LEAK;
+  din[e] = f2bf(bf2f(din[e]) + slope * bf2f(dout[((size_t)((i / 2) * m + j / 2) * m + k / 2) * c + ch]));
 }
 
 // concat [up(src_m) | skip]: out ((2m)^3 x (cu + cs))

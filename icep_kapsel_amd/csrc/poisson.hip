@@ -315,6 +315,159 @@ __global__ void add_correction_kernel(double *__restrict__ x, const T *__restric
   x[pt.im] -= (double)d[pt.im];
 }
 
+// ---------------------------------------------------------- k-space PCG
+// The FP32 correction solve keeps every CG vector as the half spectrum of a
+// real field.  A maps spectra to spectra with 6 real 3-D transforms (the
+// gradient's 3 inverse and the flux's 3 forward transforms); the
+// preconditioner is a pointwise scale; dot products follow from Parseval:
+//   sum_x a(x) b(x) = N^-3 sum_c w(kz) Re(conj(a_c) b_c),
+// w = 1 on the kz = 0 and kz = NZ/2 planes (their conjugate partners are
+// stored) and 2 elsewhere.  The sums below omit the N^-3 (it cancels in
+// alpha and beta; the host applies it to ||r||^2).
+__device__ __forceinline__ double kweight(const Grid &g, int kz) {
+  return (kz == 0 || 2 * kz == g.nz) ? 1.0 : 2.0;
+}
+
+template <typename T>
+__device__ __forceinline__ T precond_factor(const Grid &g, const KI &k, double inv_eps) {
+  const double q0 = k.kx * g.wave_x, q1 = k.ky * g.wave_y, q2 = k.kz * g.wave_z;
+  const double k2 = q0 * q0 + q1 * q1 + q2 * q2;
+  return k2 > 0 ? (T)(inv_eps / k2) : (T)0;
+}
+
+// r = A 0 - b = -b, x = 0, z = M^{-1} r (z may alias r: no preconditioner);
+// partial[b] = <r, z>, partial[NPART + b] = <r, r>
+template <typename T>
+__global__ __launch_bounds__(TPB) void kcg_init_kernel(const T *__restrict__ b, T *__restrict__ x, T *r, T *z,
+                                                       double inv_eps, int precond, double *__restrict__ partial,
+                                                       Grid g) {
+  double s_rz = 0.0, s_rr = 0.0;
+  for (size_t c = (size_t)blockIdx.x * TPB + threadIdx.x; c < g.nc; c += (size_t)gridDim.x * TPB) {
+    const KI k = kidx(g, c);
+    const V2<T> bv = ld2(b + 2 * c);
+    const V2<T> rv{-bv.re, -bv.im};
+    st2(r + 2 * c, rv);
+    st2(x + 2 * c, V2<T>{(T)0, (T)0});
+    V2<T> zv = rv;
+    if (precond) {
+      zv = scal(precond_factor<T>(g, k, inv_eps), rv);
+      st2(z + 2 * c, zv);
+    }
+    const double w = kweight(g, k.kz);
+    s_rz += w * ((double)rv.re * zv.re + (double)rv.im * zv.im);
+    s_rr += w * ((double)rv.re * rv.re + (double)rv.im * rv.im);
+  }
+  s_rz = block_sum(s_rz);
+  s_rr = block_sum(s_rr);
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = s_rz;
+    partial[NPART + blockIdx.x] = s_rr;
+  }
+}
+
+// p = z + beta p (p = z on the first iteration), then g_d = i k_d e^{i k_d dx/2} p
+template <typename T>
+__global__ void kcg_dir_grad_kernel(const T *z, T *p, const double *__restrict__ sc, int first,
+                                    T *__restrict__ g0, T *__restrict__ g1, T *__restrict__ g2,
+                                    const V2<T> *__restrict__ shx, const V2<T> *__restrict__ shy,
+                                    const V2<T> *__restrict__ shz, Grid g) {
+  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  const KI k = kidx(g, c);
+  V2<T> pv = ld2(z + 2 * c);
+  if (!first) pv = pv + scal((T)sc[S_BETA], ld2(p + 2 * c));
+  st2(p + 2 * c, pv);
+  const T w0 = (T)(k.kx * g.wave_x), w1 = (T)(k.ky * g.wave_y), w2 = (T)(k.kz * g.wave_z);
+  st2(g0 + 2 * c, cmul(V2<T>{-w0 * pv.im, w0 * pv.re}, shx[k.i]));
+  st2(g1 + 2 * c, cmul(V2<T>{-w1 * pv.im, w1 * pv.re}, shy[k.j]));
+  st2(g2 + 2 * c, cmul(V2<T>{-w2 * pv.im, w2 * pv.re}, shz[k.kz]));
+}
+
+// ap = i sum_d k_d e^{-i k_d dx/2} w_d (the staggered divergence of the
+// fluxes); partial[b] = <p, ap>
+template <typename T>
+__global__ __launch_bounds__(TPB) void kcg_div_dot_kernel(const T *__restrict__ w0, const T *__restrict__ w1,
+                                                          const T *__restrict__ w2, const T *__restrict__ p,
+                                                          T *__restrict__ ap, const V2<T> *__restrict__ shx,
+                                                          const V2<T> *__restrict__ shy,
+                                                          const V2<T> *__restrict__ shz,
+                                                          double *__restrict__ partial, Grid g) {
+  double s = 0.0;
+  for (size_t c = (size_t)blockIdx.x * TPB + threadIdx.x; c < g.nc; c += (size_t)gridDim.x * TPB) {
+    const KI k = kidx(g, c);
+    const T q0 = (T)(k.kx * g.wave_x), q1 = (T)(k.ky * g.wave_y), q2 = (T)(k.kz * g.wave_z);
+    V2<T> a = cmul(ld2(w0 + 2 * c), scal(q0, conjv(shx[k.i])));
+    a = a + cmul(ld2(w1 + 2 * c), scal(q1, conjv(shy[k.j])));
+    a = a + cmul(ld2(w2 + 2 * c), scal(q2, conjv(shz[k.kz])));
+    const V2<T> av{-a.im, a.re};
+    st2(ap + 2 * c, av);
+    const V2<T> pv = ld2(p + 2 * c);
+    s += kweight(g, k.kz) * ((double)pv.re * av.re + (double)pv.im * av.im);
+  }
+  s = block_sum(s);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// alpha = <r,z> / <p, S p> with S p = -A p;  x += alpha p, r += alpha ap,
+// z = M^{-1} r; partial[b] = <r, z>, partial[NPART + b] = <r, r>
+template <typename T>
+__global__ __launch_bounds__(TPB) void kcg_update_kernel(T *__restrict__ x, T *r, T *z, const T *__restrict__ p,
+                                                         const T *__restrict__ ap, const double *__restrict__ sc,
+                                                         double inv_eps, int precond, double *__restrict__ partial,
+                                                         Grid g) {
+  const double pap = -sc[S_PAP];
+  const T alpha = (T)(pap != 0.0 ? sc[S_RZ] / pap : 0.0);
+  double s_rz = 0.0, s_rr = 0.0;
+  for (size_t c = (size_t)blockIdx.x * TPB + threadIdx.x; c < g.nc; c += (size_t)gridDim.x * TPB) {
+    const KI k = kidx(g, c);
+    const V2<T> pv = ld2(p + 2 * c);
+    st2(x + 2 * c, ld2(x + 2 * c) + scal(alpha, pv));
+    const V2<T> rv = ld2(r + 2 * c) + scal(alpha, ld2(ap + 2 * c));
+    st2(r + 2 * c, rv);
+    V2<T> zv = rv;
+    if (precond) {
+      zv = scal(precond_factor<T>(g, k, inv_eps), rv);
+      st2(z + 2 * c, zv);
+    }
+    const double w = kweight(g, k.kz);
+    s_rz += w * ((double)rv.re * zv.re + (double)rv.im * zv.im);
+    s_rr += w * ((double)rv.re * rv.re + (double)rv.im * rv.im);
+  }
+  s_rz = block_sum(s_rz);
+  s_rr = block_sum(s_rr);
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = s_rz;
+    partial[NPART + blockIdx.x] = s_rr;
+  }
+}
+
+// sums of the two partial arrays: scal[S_RR] = <r,r>; mode 0 sets
+// scal[S_RZ] = <r,z>, mode 1 also beta = <r,z> / old scal[S_RZ]
+__global__ __launch_bounds__(TPB) void finish2_kernel(const double *__restrict__ partial, double *__restrict__ scal,
+                                                      int mode) {
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < NPART; i += TPB) {
+    a += partial[i];
+    b += partial[NPART + i];
+  }
+  a = block_sum(a);
+  b = block_sum(b);
+  if (threadIdx.x == 0) {
+    const double old = scal[S_RZ];
+    scal[S_BETA] = (mode == 1 && old != 0.0) ? a / old : 0.0;
+    scal[S_RZ] = a;
+    scal[S_RR] = b;
+  }
+}
+
+template <typename T>
+__global__ void scale_real_kernel(T *__restrict__ x, double s, Grid g) {
+  size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= g.nr) return;
+  RI pt = ridx(g, q);
+  x[pt.im] = (T)(s * (double)x[pt.im]);
+}
+
 // ------------------------------------------------------------------ host
 template <typename T>
 const V2<T> *sh(kapsel_ctx *ctx, int d) {
@@ -387,7 +540,7 @@ void build_rhs_t(kapsel_ctx *ctx, T *rhs, const T *rho, const T *eps, const doub
 
 void ensure_storage(kapsel_ctx *ctx) {
   if (!ctx->d_partial) {
-    KAPSEL_HIP(hipMalloc(&ctx->d_partial, sizeof(double) * NPART));
+    KAPSEL_HIP(hipMalloc(&ctx->d_partial, sizeof(double) * 2 * NPART));
     KAPSEL_HIP(hipMalloc(&ctx->d_scal, sizeof(double) * S_N));
     KAPSEL_HIP(hipMemsetAsync(ctx->d_scal, 0, sizeof(double) * S_N, ctx->stream));
     ctx->n_partial = NPART;
@@ -502,6 +655,68 @@ int cg_t(kapsel_ctx *ctx, T *x, const T *b, const T *eps, double tol_abs, int ma
   return it;
 }
 
+// PCG on S = -A for A x = b with every vector in k-space: b and x are the
+// half spectra (x written, b read); w = {r, p, ap, z} k-space work vectors,
+// s = {g0, g1, g2} flux scratch.  Stops when the real-space ||A x - b|| is
+// below tol_abs (checked every iteration); returns the iteration count and
+// *rr_out = that ||A x - b||^2.
+template <typename T>
+int kcg_t(kapsel_ctx *ctx, T *x, const T *b, const T *eps, double tol_abs, int maxit, int precond, double inv_eps,
+          T *const *w, T *const *s, double *rr_out) {
+  const Grid &g = ctx->g;
+  T *r = w[0], *p = w[1], *ap = w[2], *z = precond ? w[3] : w[0];
+  const double norm = 1.0 / (double)g.nr;  // Parseval factor of the k-space sums
+  {
+    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, precond ? 4 : 3, sizeof(T)));
+    hipLaunchKernelGGL(kcg_init_kernel<T>, dim3(NPART), dim3(TPB), 0, ctx->stream, b, x, r, z, inv_eps, precond,
+                       ctx->d_partial, g);
+    KAPSEL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(finish2_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->d_partial, ctx->d_scal, 0);
+    KAPSEL_LAUNCH_CHECK();
+  }
+  double rr = read_scal(ctx, S_RR) * norm;
+  const double tol2 = tol_abs * tol_abs;
+  void *s3[3] = {s[0], s[1], s[2]};
+  int it = 0;
+  while (rr >= tol2 && it < maxit) {
+    {
+      LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, it ? 6 : 5, sizeof(T)));
+      hipLaunchKernelGGL(kcg_dir_grad_kernel<T>, dim3(blocks_for(g.nc, TPB)), dim3(TPB), 0, ctx->stream, z, p,
+                         ctx->d_scal, it == 0, s[0], s[1], s[2], sh<T>(ctx, 0), sh<T>(ctx, 1), sh<T>(ctx, 2), g);
+      KAPSEL_LAUNCH_CHECK();
+    }
+    fft_c2r_t<T>(ctx, s3, 3);
+    {
+      LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 7, sizeof(T)));
+      hipLaunchKernelGGL(face_mul_kernel<T>, dim3(blocks_for(g.nr, TPB)), dim3(TPB), 0, ctx->stream, s[0], s[1],
+                         s[2], eps, g);
+      KAPSEL_LAUNCH_CHECK();
+    }
+    fft_r2c_t<T>(ctx, s3, 3);
+    {
+      LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 5, sizeof(T)));
+      hipLaunchKernelGGL(kcg_div_dot_kernel<T>, dim3(NPART), dim3(TPB), 0, ctx->stream, s[0], s[1], s[2], p, ap,
+                         sh<T>(ctx, 0), sh<T>(ctx, 1), sh<T>(ctx, 2), ctx->d_partial, g);
+      KAPSEL_LAUNCH_CHECK();
+      hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->d_partial, NPART, ctx->d_scal,
+                         S_PAP, 0);
+      KAPSEL_LAUNCH_CHECK();
+    }
+    {
+      LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, precond ? 7 : 6, sizeof(T)));
+      hipLaunchKernelGGL(kcg_update_kernel<T>, dim3(NPART), dim3(TPB), 0, ctx->stream, x, r, z, p, ap,
+                         ctx->d_scal, inv_eps, precond, ctx->d_partial, g);
+      KAPSEL_LAUNCH_CHECK();
+      hipLaunchKernelGGL(finish2_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->d_partial, ctx->d_scal, 1);
+      KAPSEL_LAUNCH_CHECK();
+    }
+    it++;
+    rr = read_scal(ctx, S_RR) * norm;
+  }
+  *rr_out = rr;
+  return it;
+}
+
 template <typename T>
 T *fld(kapsel_ctx *ctx, int f) {
   return (T *)ctx->fields[f];
@@ -572,14 +787,16 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
                      eps64, eps32, g);
   hipLaunchKernelGGL((convert_real_kernel<double, float>), dim3(blocks_for(g.f, TPB)), dim3(TPB), 0, ctx->stream,
                      r64, fld<float>(ctx, KAPSEL_RHO_FREE), g);
-  hipLaunchKernelGGL((convert_real_kernel<double, float>), dim3(blocks_for(g.f, TPB)), dim3(TPB), 0, ctx->stream,
-                     x64, fld<float>(ctx, KAPSEL_POTENTIAL), g);
+  if (!ctx->pot64_valid)
+    hipLaunchKernelGGL((convert_real_kernel<double, float>), dim3(blocks_for(g.f, TPB)), dim3(TPB), 0, ctx->stream,
+                       x64, fld<float>(ctx, KAPSEL_POTENTIAL), g);
   KAPSEL_LAUNCH_CHECK();
   build_rhs_t<double>(ctx, b64, r64, eps64, e0, s64);
   dot_t<double, double>(ctx, b64, b64, S_BB);
   const double bb = read_scal(ctx, S_BB);
   if (bb == 0.0) {
     KAPSEL_HIP(hipMemsetAsync(ctx->fields[KAPSEL_POTENTIAL], 0, g.f * sizeof(float), ctx->stream));
+    ctx->pot64_valid = false;
     *relres = 0.0;
     return 0;
   }
@@ -597,16 +814,22 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
     const double rr = read_scal(ctx, S_RR);
     rel = sqrt(rr) / bnorm;
     if (rel < tol || total >= maxit) break;
-    // correction: A d = r in FP32 (RHS holds r meanwhile), then x -= d;
-    // inner tolerance 1e-4 of the current residual, which FP32 certifies
+    // correction: A d = r in FP32 (RHS holds r meanwhile), solved in k-space,
+    // then x -= d.  Inner tolerance: what the outer bound still needs (with a
+    // factor 2 of margin), but never below 1e-4 of the current residual,
+    // the reduction FP32 certifies
     float *b32 = fld<float>(ctx, KAPSEL_RHS);
     hipLaunchKernelGGL((convert_real_kernel<float, double>), dim3(blocks_for(g.f, TPB)), dim3(TPB), 0,
                        ctx->stream, b32, r64, g);
     KAPSEL_LAUNCH_CHECK();
-    KAPSEL_HIP(hipMemsetAsync(d32, 0, g.f * sizeof(float), ctx->stream));
+    void *bk[1] = {b32};
+    fft_r2c_t<float>(ctx, bk, 1);
     double rr_in = 0.0;
-    int it = cg_t<float>(ctx, d32, b32, eps32, 1e-4 * sqrt(rr), maxit - total, precond, inv_eps, w32, s32, &rr_in);
+    const double tol_in = fmax(1e-4 * sqrt(rr), 0.5 * tol * bnorm);
+    int it = kcg_t<float>(ctx, d32, b32, eps32, tol_in, maxit - total, precond, inv_eps, w32, s32 + 1, &rr_in);
     total += it;
+    void *dk[1] = {d32};
+    fft_c2r_t<float>(ctx, dk, 1);
     hipLaunchKernelGGL(add_correction_kernel<float>, dim3(blocks_for(g.nr, TPB)), dim3(TPB), 0, ctx->stream, x64,
                        d32, g);
     KAPSEL_LAUNCH_CHECK();
@@ -617,6 +840,7 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
   hipLaunchKernelGGL((convert_real_kernel<float, double>), dim3(blocks_for(g.f, TPB)), dim3(TPB), 0, ctx->stream,
                      fld<float>(ctx, KAPSEL_RHS), b64, g);
   KAPSEL_LAUNCH_CHECK();
+  ctx->pot64_valid = true;
   *relres = rel;
   return total;
 }
@@ -715,6 +939,33 @@ void poisson_rhs(kapsel_ctx *ctx, const double e0[3]) {
     build_rhs_t<float>(ctx, fld<float>(ctx, KAPSEL_RHS), fld<float>(ctx, KAPSEL_RHO_FREE),
                        fld<float>(ctx, KAPSEL_EPSILON), e0, s);
   }
+}
+
+// x <- gamma x, gamma = (x . b) / (x . A x): of all multiples of the warm
+// start x, the one closest to the solution in the energy norm of S = -A, so
+// the start is never worse than x = 0 in that norm.  Builds RHS for e0.
+template <typename T>
+double galerkin_scale_t(kapsel_ctx *ctx, const double e0[3]) {
+  const Grid &g = ctx->g;
+  T *s[4] = {(T *)ctx->pscratch[0], (T *)ctx->pscratch[1], (T *)ctx->pscratch[2], (T *)ctx->pscratch[3]};
+  T *x = fld<T>(ctx, KAPSEL_POTENTIAL), *b = fld<T>(ctx, KAPSEL_RHS), *ax = fld<T>(ctx, KAPSEL_WORK0);
+  const T *eps = fld<T>(ctx, KAPSEL_EPSILON);
+  build_rhs_t<T>(ctx, b, fld<T>(ctx, KAPSEL_RHO_FREE), eps, e0, s);
+  ax_t<T>(ctx, x, ax, eps, s);
+  dot_t<T, T>(ctx, x, b, S_TMP);
+  const double xb = read_scal(ctx, S_TMP);
+  dot_t<T, T>(ctx, x, ax, S_TMP);
+  const double xax = read_scal(ctx, S_TMP);
+  const double gamma = xax != 0.0 ? xb / xax : 0.0;
+  hipLaunchKernelGGL(scale_real_kernel<T>, dim3(blocks_for(g.nr, TPB)), dim3(TPB), 0, ctx->stream, x, gamma, g);
+  KAPSEL_LAUNCH_CHECK();
+  return gamma;
+}
+
+double galerkin_scale_potential(kapsel_ctx *ctx, const double e0[3]) {
+  ensure_storage(ctx);
+  ctx->pot64_valid = false;
+  return ctx->prec == KAPSEL_FP64 ? galerkin_scale_t<double>(ctx, e0) : galerkin_scale_t<float>(ctx, e0);
 }
 
 void poisson_solve(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int precond, int *iters,

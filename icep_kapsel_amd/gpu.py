@@ -227,6 +227,13 @@ class KapselGPU:
         e = np.ascontiguousarray(e0, dtype=np.float64)
         self._call("pnet_warm_start", float(beta), _dp(e), float(scale))
 
+    def poisson_galerkin_start(self, e0):
+        """Rescales the warm start in POTENTIAL (energy-norm optimal); returns gamma."""
+        e = np.ascontiguousarray(e0, dtype=np.float64)
+        g = ctypes.c_double(0.0)
+        self._call("poisson_galerkin_start", _dp(e), ctypes.byref(g))
+        return g.value
+
     # timing ---------------------------------------------------------------
     def kernel_timing(self, classes):
         mask = 0

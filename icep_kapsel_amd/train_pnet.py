@@ -79,9 +79,10 @@ def load_weights(path):
 
 
 def evaluate(k, params, n_samples, seed=4104, tol=1e-8, n=128):
-    """Cold solves of held-out configurations, warm-started from zero and from
-    the network: FP32 spectral PCG refined to an FP64 residual tol.  Returns a
-    list of dicts (iterations, seconds, initial relative residual)."""
+    """Cold solves of held-out configurations, warm-started from zero, from
+    the network, and from the network rescaled by the energy-norm (Galerkin)
+    factor: FP32 spectral PCG refined to an FP64 residual tol.  Returns a
+    list of dicts (iterations, seconds, final relative residual)."""
     import time as _t
     k.pnet_init()
     k.pnet_set_params(params)
@@ -91,15 +92,14 @@ def evaluate(k, params, n_samples, seed=4104, tol=1e-8, n=128):
         prepare(k, pos, ratio)
         e0 = E_MAG * e_hat
         rec = {"ratio": ratio}
-        for mode in ("zero", "network"):
-            if mode == "zero":
-                k.set_field("POTENTIAL", np.zeros(k.shape))
-                k.synchronize()
-                t0 = _t.perf_counter()
-            else:
-                k.synchronize()
-                t0 = _t.perf_counter()
+        for mode in ("zero", "network", "network_galerkin"):
+            k.set_field("POTENTIAL", np.zeros(k.shape))
+            k.synchronize()
+            t0 = _t.perf_counter()
+            if mode != "zero":
                 k.pnet_warm_start(beta_of(ratio), e0, A)
+            if mode == "network_galerkin":
+                rec["gamma"] = k.poisson_galerkin_start(e0)
             it, rr = k.poisson_solve(e0, tol, 8000, abi.PRECOND_SPECTRAL)
             k.synchronize()
             rec[mode + "_iters"] = it
@@ -113,14 +113,25 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--samples", type=int, default=2000)
-    ap.add_argument("--steps-per-sample", type=int, default=2)
-    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--steps-per-sample", type=int, default=3)
+    ap.add_argument("--lr", type=float, default=5e-4)
     ap.add_argument("--seed", type=int, default=4004)
-    ap.add_argument("--out", required=True)
+    ap.add_argument("--out", help="weights file to write")
     ap.add_argument("--eval", type=int, default=0, help="after training, evaluate on this many held-out samples")
+    ap.add_argument("--eval-only", metavar="WEIGHTS", help="evaluate these weights (no training)")
     args = ap.parse_args(argv)
     offs, nparam = layer_offsets()
     k = make_context(args.n)
+    if args.eval_only:
+        ev = evaluate(k, load_weights(args.eval_only), args.eval or 8, n=args.n)
+        for e in ev:
+            print("eval:", json.dumps(e), flush=True)
+        for mode in ("zero", "network", "network_galerkin"):
+            print(f"eval: mean {mode} iterations {np.mean([e[mode + '_iters'] for e in ev]):.2f} "
+                  f"seconds {np.mean([e[mode + '_seconds'] for e in ev]):.4f}", flush=True)
+        return 0
+    if not args.out:
+        ap.error("--out is required for training")
     assert k.pnet_init() == nparam
     k.pnet_set_params(he_init(args.seed, offs, nparam))
     t0 = time.time()
@@ -154,6 +165,7 @@ def main(argv=None):
         rec["eval"] = ev
         rec["eval_mean_iters_zero"] = float(np.mean([e["zero_iters"] for e in ev]))
         rec["eval_mean_iters_network"] = float(np.mean([e["network_iters"] for e in ev]))
+        rec["eval_mean_iters_network_galerkin"] = float(np.mean([e["network_galerkin_iters"] for e in ev]))
     with open(args.out + ".json", "w") as f:
         json.dump(rec, f, indent=1)
     print("train: done", json.dumps(rec), flush=True)

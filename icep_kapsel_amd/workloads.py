@@ -121,3 +121,21 @@ def spm_config(name, precision=abi.KAPSEL_FP64, seed=None, n_particles=None, gri
     vel = np.array([[rng.normal() * vstd for _ in range(3)] for _ in range(npart)]) if vstd else None
     st = particle_state(pos, vel)
     return p, st, 0.05
+
+
+def icep_config(run=0, precision=abi.KAPSEL_FP64, ratio=0.1, grid=128, n_particles=64):
+    """The ICEP box: 128^3, 64 dielectric spheres a=6 xi=2 at rest in a fluid
+    at rest, eps_f=1, eps_p=ratio (0.1), no free charge, eta=rho=1, dt=0.05,
+    WCA LJ sigma=2a.  E0 = R (0, 0, 0.1) with R a seeded uniform random
+    rotation, i.e. |E0| = 0.1 along a uniformly random direction (normal
+    deviates, normalised).  Seed 3003 + run; positions are drawn first.
+    Returns (params, particle state, e0, dt)."""
+    n, a = grid, 6.0
+    rng = Xoshiro256(3003 + run)
+    p = abi.make_params(n, n_particles=n_particles, radius=a, xi=2.0, rotation=1, lj_truncate=1,
+                        lj_powers=0, lj_epsilon=1.0, lj_sigma=2.0 * a, eps_fluid=1.0, eps_particle=ratio,
+                        precision=precision)
+    pos = random_positions(rng, n_particles, (n, n, n), 2.0 * a)
+    v = np.array([rng.normal() for _ in range(3)])
+    e0 = 0.1 * v / np.linalg.norm(v)
+    return p, particle_state(pos), e0, 0.05

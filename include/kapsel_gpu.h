@@ -236,6 +236,12 @@ int kapsel_pnet_train_step(kapsel_ctx *ctx, double beta, const double e_hat[3], 
                            double *loss);
 /* POTENTIAL = scale * sum_d e0_d psi_d: the warm start of kapsel_poisson_solve */
 int kapsel_pnet_warm_start(kapsel_ctx *ctx, double beta, const double e0[3], double scale);
+/* Rescales the warm start in POTENTIAL by gamma = (x . b) / (x . A x), b the
+ * RHS for e0 (built into RHS; WORK0 scratch): the multiple of x nearest the
+ * solution in the energy norm of -A, never worse than a zero start in that
+ * norm.  No reference counterpart (the reference starts from the previous
+ * potential, operate_dielectric.cxx:558-562).  *gamma may be NULL. */
+int kapsel_poisson_galerkin_start(kapsel_ctx *ctx, const double e0[3], double *gamma);
 
 /* ---- kernel timing (HIP events on the context stream) --------------------- */
 enum kapsel_kernel_class {

@@ -35,7 +35,7 @@ def _init(rng):
 def _setup(n=16, seed=0):
     from icep_kapsel_amd.gpu import KapselGPU
     p = abi.make_params(n, n_particles=2, radius=3.0, xi=2.0, eps_fluid=1.0, eps_particle=0.1)
-    pos = np.array([[4.3, 5.1, 6.7], [11.2, 10.4, 3.3]])
+    pos = np.array([[4.3, 5.1, 6.7], [11.2, 10.4, 3.3]]) * (n / 16)
     k = KapselGPU(p)
     k.set_particles(workloads.particle_state(pos))
     k.make_dielectric_field()
@@ -47,10 +47,13 @@ def _setup(n=16, seed=0):
     return k, rng, t
 
 
-def test_pnet_gradient_matches_oracle():
+@pytest.mark.parametrize("n", [16, 32])
+def test_pnet_gradient_matches_oracle(n):
+    """16^3: one weight-gradient chunk per layer; 32^3: the full-resolution
+    layers reduce 8 chunk partials."""
     from oracle import oracle as om
-    n, beta, scale = 16, -0.4285714, 0.5
-    print("case pnet gradient 16^3", flush=True)
+    beta, scale = -0.4285714, 0.5
+    print(f"case pnet gradient {n}^3", flush=True)
     k, rng, t = _setup(n)
     P = k.pnet_init()
     assert P == om.pnet_param_count()
