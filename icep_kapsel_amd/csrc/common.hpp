@@ -143,6 +143,12 @@ struct kapsel_ctx {
   bool pot64_valid = false;
   double *d_partial = nullptr;    // block partial sums (reductions)
   double *d_scal = nullptr;       // CG scalars
+  double *h_slots = nullptr;      // pinned host copies of ||r||^2 (pipelined convergence checks)
+  hipEvent_t slot_ev[2] = {};
+  // one k-space PCG iteration captured as a HIP graph, replayed while the
+  // buffers it was captured with (kcg_key) stay the same
+  hipGraphExec_t kcg_exec = nullptr;
+  const void *kcg_key[10] = {};
   int n_partial = 0;
   kapsel::Pnet *pnet = nullptr;   // learned warm-start network, on first use
   // derived physics constants (Set_global_parameters, input.cxx:287-430)

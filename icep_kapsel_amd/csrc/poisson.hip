@@ -28,7 +28,7 @@ namespace kapsel {
 
 namespace {
 
-enum { S_BB = 0, S_RZ, S_PAP, S_RR, S_RZN, S_BETA, S_TMP, S_N };
+enum { S_BB = 0, S_RZ, S_PAP, S_RR, S_RZN, S_BETA, S_TMP, S_INVEPS, S_N };
 constexpr int NPART = 1024;  // blocks of the partial-sum reductions (fixed: deterministic sums)
 constexpr unsigned TPB = 256;
 
@@ -244,8 +244,10 @@ __global__ __launch_bounds__(TPB) void dot_partial_kernel(const T *__restrict__ 
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// scal[slot] = sum(partial); mode 1 also sets beta = new / scal[S_RZ] and
-// scal[S_RZ] = new (the rz update of CG)
+// Second stage of every reduction (a separate launch: a device-wide fence
+// per block to let the last block finish costs more than the launch):
+// scal[slot] = sum(partial) in block order; mode 1 also sets
+// beta = new / scal[S_RZ] and scal[S_RZ] = new (the rz update of CG)
 __global__ __launch_bounds__(TPB) void finish_kernel(const double *__restrict__ partial, int n,
                                                      double *__restrict__ scal, int slot, int mode) {
   double s = 0.0;
@@ -258,6 +260,25 @@ __global__ __launch_bounds__(TPB) void finish_kernel(const double *__restrict__ 
       scal[S_RZ] = s;
     }
     scal[slot] = s;
+  }
+}
+
+// two sums at once: scal[S_RR] = <r,r> (partial[NPART..]); scal[S_RZ] =
+// <r,z>; mode 1 also beta = <r,z> / old scal[S_RZ] (mode 0: beta = 0)
+__global__ __launch_bounds__(TPB) void finish2_kernel(const double *__restrict__ partial, double *__restrict__ scal,
+                                                      int mode) {
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < NPART; i += TPB) {
+    a += partial[i];
+    b += partial[NPART + i];
+  }
+  a = block_sum(a);
+  b = block_sum(b);
+  if (threadIdx.x == 0) {
+    const double old = scal[S_RZ];
+    scal[S_BETA] = (mode == 1 && old != 0.0) ? a / old : 0.0;
+    scal[S_RZ] = a;
+    scal[S_RR] = b;
   }
 }
 
@@ -335,12 +356,14 @@ __device__ __forceinline__ T precond_factor(const Grid &g, const KI &k, double i
   return k2 > 0 ? (T)(inv_eps / k2) : (T)0;
 }
 
-// r = A 0 - b = -b, x = 0, z = M^{-1} r (z may alias r: no preconditioner);
-// partial[b] = <r, z>, partial[NPART + b] = <r, r>
+// r = A 0 - b = -b, x = 0, z = M^{-1} r (z may alias r: no preconditioner),
+// p = z; partial[b] = <r, z>, partial[NPART + b] = <r, r>.  The
+// preconditioner scale 1 / eps_mean is read from sc[S_INVEPS].
 template <typename T>
 __global__ __launch_bounds__(TPB) void kcg_init_kernel(const T *__restrict__ b, T *__restrict__ x, T *r, T *z,
-                                                       double inv_eps, int precond, double *__restrict__ partial,
-                                                       Grid g) {
+                                                       T *__restrict__ p, const double *__restrict__ sc,
+                                                       int precond, double *__restrict__ partial, Grid g) {
+  const double inv_eps = sc[S_INVEPS];
   double s_rz = 0.0, s_rr = 0.0;
   for (size_t c = (size_t)blockIdx.x * TPB + threadIdx.x; c < g.nc; c += (size_t)gridDim.x * TPB) {
     const KI k = kidx(g, c);
@@ -353,6 +376,7 @@ __global__ __launch_bounds__(TPB) void kcg_init_kernel(const T *__restrict__ b, 
       zv = scal(precond_factor<T>(g, k, inv_eps), rv);
       st2(z + 2 * c, zv);
     }
+    st2(p + 2 * c, zv);
     const double w = kweight(g, k.kz);
     s_rz += w * ((double)rv.re * zv.re + (double)rv.im * zv.im);
     s_rr += w * ((double)rv.re * rv.re + (double)rv.im * rv.im);
@@ -365,7 +389,8 @@ __global__ __launch_bounds__(TPB) void kcg_init_kernel(const T *__restrict__ b, 
   }
 }
 
-// p = z + beta p (p = z on the first iteration), then g_d = i k_d e^{i k_d dx/2} p
+// p = z + beta p (beta = 0 before the first update: p = z), then
+// g_d = i k_d e^{i k_d dx/2} p
 template <typename T>
 __global__ void kcg_dir_grad_kernel(const T *z, T *p, const double *__restrict__ sc, int first,
                                     T *__restrict__ g0, T *__restrict__ g1, T *__restrict__ g2,
@@ -413,8 +438,8 @@ __global__ __launch_bounds__(TPB) void kcg_div_dot_kernel(const T *__restrict__ 
 template <typename T>
 __global__ __launch_bounds__(TPB) void kcg_update_kernel(T *__restrict__ x, T *r, T *z, const T *__restrict__ p,
                                                          const T *__restrict__ ap, const double *__restrict__ sc,
-                                                         double inv_eps, int precond, double *__restrict__ partial,
-                                                         Grid g) {
+                                                         int precond, double *__restrict__ partial, Grid g) {
+  const double inv_eps = sc[S_INVEPS];
   const double pap = -sc[S_PAP];
   const T alpha = (T)(pap != 0.0 ? sc[S_RZ] / pap : 0.0);
   double s_rz = 0.0, s_rr = 0.0;
@@ -441,23 +466,9 @@ __global__ __launch_bounds__(TPB) void kcg_update_kernel(T *__restrict__ x, T *r
   }
 }
 
-// sums of the two partial arrays: scal[S_RR] = <r,r>; mode 0 sets
-// scal[S_RZ] = <r,z>, mode 1 also beta = <r,z> / old scal[S_RZ]
-__global__ __launch_bounds__(TPB) void finish2_kernel(const double *__restrict__ partial, double *__restrict__ scal,
-                                                      int mode) {
-  double a = 0.0, b = 0.0;
-  for (int i = threadIdx.x; i < NPART; i += TPB) {
-    a += partial[i];
-    b += partial[NPART + i];
-  }
-  a = block_sum(a);
-  b = block_sum(b);
-  if (threadIdx.x == 0) {
-    const double old = scal[S_RZ];
-    scal[S_BETA] = (mode == 1 && old != 0.0) ? a / old : 0.0;
-    scal[S_RZ] = a;
-    scal[S_RR] = b;
-  }
+// sc[out] = count / sc[sum] (1 if the sum is not positive)
+__global__ void inv_mean_kernel(double *__restrict__ sc, int sum, int out, double count) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) sc[out] = sc[sum] > 0.0 ? count / sc[sum] : 1.0;
 }
 
 template <typename T>
@@ -545,6 +556,10 @@ void ensure_storage(kapsel_ctx *ctx) {
     KAPSEL_HIP(hipMemsetAsync(ctx->d_scal, 0, sizeof(double) * S_N, ctx->stream));
     ctx->n_partial = NPART;
   }
+  if (!ctx->h_slots) {
+    KAPSEL_HIP(hipHostMalloc((void **)&ctx->h_slots, sizeof(double) * 2, hipHostMallocDefault));
+    for (hipEvent_t &e : ctx->slot_ev) KAPSEL_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
   if (!ctx->pscratch[0]) {
     for (void *&p : ctx->pscratch) {
       KAPSEL_HIP(hipMalloc(&p, ctx->g.f * ctx->elem));
@@ -627,7 +642,8 @@ int cg_t(kapsel_ctx *ctx, T *x, const T *b, const T *eps, double tol_abs, int ma
   dot_t<T, T>(ctx, r, r, S_RR);
   double rr = read_scal(ctx, S_RR);
   const double tol2 = tol_abs * tol_abs;
-  int it = 0;
+  // convergence test pipelined as in kcg_t (one iteration of lag)
+  int it = 0, known = 0;
   while (rr >= tol2 && it < maxit) {
     ax_t<T>(ctx, p, ap, eps, s);
     dot_t<T, T>(ctx, p, ap, S_PAP);
@@ -649,69 +665,143 @@ int cg_t(kapsel_ctx *ctx, T *x, const T *b, const T *eps, double tol_abs, int ma
       KAPSEL_LAUNCH_CHECK();
     }
     it++;
-    rr = read_scal(ctx, S_RR);
+    KAPSEL_HIP(hipMemcpyAsync(ctx->h_slots + (it & 1), ctx->d_scal + S_RR, sizeof(double), hipMemcpyDeviceToHost,
+                              ctx->stream));
+    KAPSEL_HIP(hipEventRecord(ctx->slot_ev[it & 1], ctx->stream));
+    if (it - known == 2 || it == maxit) {
+      known++;
+      KAPSEL_HIP(hipEventSynchronize(ctx->slot_ev[known & 1]));
+      rr = ctx->h_slots[known & 1];
+    }
+  }
+  while (known < it) {
+    known++;
+    KAPSEL_HIP(hipEventSynchronize(ctx->slot_ev[known & 1]));
+    rr = ctx->h_slots[known & 1];
   }
   *rr_out = rr;
   return it;
 }
 
-// PCG on S = -A for A x = b with every vector in k-space: b and x are the
-// half spectra (x written, b read); w = {r, p, ap, z} k-space work vectors,
-// s = {g0, g1, g2} flux scratch.  Stops when the real-space ||A x - b|| is
-// below tol_abs (checked every iteration); returns the iteration count and
-// *rr_out = that ||A x - b||^2.
+// One k-space PCG iteration: p = z + beta p, ap = A p (3 inverse and 3
+// forward transforms), alpha, x += alpha p, r += alpha ap, z = M^{-1} r,
+// beta.  Every scalar lives in device memory, so the sequence can be
+// captured once and replayed.
 template <typename T>
-int kcg_t(kapsel_ctx *ctx, T *x, const T *b, const T *eps, double tol_abs, int maxit, int precond, double inv_eps,
-          T *const *w, T *const *s, double *rr_out) {
+void kcg_iteration(kapsel_ctx *ctx, T *x, const T *eps, int precond, T *const *w, T *const *s) {
   const Grid &g = ctx->g;
   T *r = w[0], *p = w[1], *ap = w[2], *z = precond ? w[3] : w[0];
+  void *s3[3] = {s[0], s[1], s[2]};
+  {
+    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 6, sizeof(T)));
+    hipLaunchKernelGGL(kcg_dir_grad_kernel<T>, dim3(blocks_for(g.nc, TPB)), dim3(TPB), 0, ctx->stream, z, p,
+                       ctx->d_scal, 0, s[0], s[1], s[2], sh<T>(ctx, 0), sh<T>(ctx, 1), sh<T>(ctx, 2), g);
+    KAPSEL_LAUNCH_CHECK();
+  }
+  fft_c2r_t<T>(ctx, s3, 3);
+  {
+    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 7, sizeof(T)));
+    hipLaunchKernelGGL(face_mul_kernel<T>, dim3(blocks_for(g.nr, TPB)), dim3(TPB), 0, ctx->stream, s[0], s[1],
+                       s[2], eps, g);
+    KAPSEL_LAUNCH_CHECK();
+  }
+  fft_r2c_t<T>(ctx, s3, 3);
+  {
+    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 5, sizeof(T)));
+    hipLaunchKernelGGL(kcg_div_dot_kernel<T>, dim3(NPART), dim3(TPB), 0, ctx->stream, s[0], s[1], s[2], p, ap,
+                       sh<T>(ctx, 0), sh<T>(ctx, 1), sh<T>(ctx, 2), ctx->d_partial, g);
+    KAPSEL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->d_partial, NPART, ctx->d_scal,
+                       S_PAP, 0);
+    KAPSEL_LAUNCH_CHECK();
+  }
+  {
+    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, precond ? 7 : 6, sizeof(T)));
+    hipLaunchKernelGGL(kcg_update_kernel<T>, dim3(NPART), dim3(TPB), 0, ctx->stream, x, r, z, p, ap, ctx->d_scal,
+                       precond, ctx->d_partial, g);
+    KAPSEL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(finish2_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->d_partial, ctx->d_scal, 1);
+    KAPSEL_LAUNCH_CHECK();
+  }
+}
+
+// Queues one iteration: replays the captured graph when the buffers match
+// the capture (and no kernel timing is on), else captures it anew.
+template <typename T>
+void kcg_enqueue(kapsel_ctx *ctx, T *x, const T *eps, int precond, T *const *w, T *const *s) {
+  if (ctx->timing_mask) {
+    kcg_iteration<T>(ctx, x, eps, precond, w, s);
+    return;
+  }
+  const void *key[10] = {x, eps, w[0], w[1], w[2], w[3], s[0], s[1], s[2], (const void *)(intptr_t)(precond + 1)};
+  bool same = ctx->kcg_exec != nullptr;
+  for (int i = 0; i < 10 && same; i++) same = key[i] == ctx->kcg_key[i];
+  if (!same) {
+    if (ctx->kcg_exec) {
+      hipGraphExec_t old = ctx->kcg_exec;
+      ctx->kcg_exec = nullptr;
+      KAPSEL_HIP(hipGraphExecDestroy(old));
+    }
+    hipGraph_t graph = nullptr;
+    KAPSEL_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    try {
+      kcg_iteration<T>(ctx, x, eps, precond, w, s);
+    } catch (...) {
+      (void)hipStreamEndCapture(ctx->stream, &graph);
+      if (graph) (void)hipGraphDestroy(graph);
+      throw;
+    }
+    KAPSEL_HIP(hipStreamEndCapture(ctx->stream, &graph));
+    hipError_t e = hipGraphInstantiate(&ctx->kcg_exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    KAPSEL_HIP(e);
+    for (int i = 0; i < 10; i++) ctx->kcg_key[i] = key[i];
+  }
+  KAPSEL_HIP(hipGraphLaunch(ctx->kcg_exec, ctx->stream));
+}
+
+// PCG on S = -A for A x = b with every vector in k-space: b and x are the
+// half spectra (x written, b read); w = {r, p, ap, z} k-space work vectors,
+// s = {g0, g1, g2} flux scratch; sc[S_INVEPS] holds 1 / eps_mean.  Stops
+// when the real-space ||A x - b|| is below tol_abs; returns the iteration
+// count and *rr_out = that ||A x - b||^2.
+template <typename T>
+int kcg_t(kapsel_ctx *ctx, T *x, const T *b, const T *eps, double tol_abs, int maxit, int precond, T *const *w,
+          T *const *s, double *rr_out) {
+  const Grid &g = ctx->g;
+  T *r = w[0], *p = w[1], *z = precond ? w[3] : w[0];
   const double norm = 1.0 / (double)g.nr;  // Parseval factor of the k-space sums
   {
-    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, precond ? 4 : 3, sizeof(T)));
-    hipLaunchKernelGGL(kcg_init_kernel<T>, dim3(NPART), dim3(TPB), 0, ctx->stream, b, x, r, z, inv_eps, precond,
-                       ctx->d_partial, g);
+    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, precond ? 5 : 4, sizeof(T)));
+    hipLaunchKernelGGL(kcg_init_kernel<T>, dim3(NPART), dim3(TPB), 0, ctx->stream, b, x, r, z, p, ctx->d_scal,
+                       precond, ctx->d_partial, g);
     KAPSEL_LAUNCH_CHECK();
     hipLaunchKernelGGL(finish2_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->d_partial, ctx->d_scal, 0);
     KAPSEL_LAUNCH_CHECK();
   }
   double rr = read_scal(ctx, S_RR) * norm;
   const double tol2 = tol_abs * tol_abs;
-  void *s3[3] = {s[0], s[1], s[2]};
-  int it = 0;
+  // Pipelined convergence test: iteration it copies its ||r||^2 into pinned
+  // slot it % 2; the host reads iteration it - 1's value only after queueing
+  // iteration it, so the GPU never drains between iterations.  The test thus
+  // lags one iteration: at most one extra iteration runs past the bound.
+  int it = 0, known = 0;
   while (rr >= tol2 && it < maxit) {
-    {
-      LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, it ? 6 : 5, sizeof(T)));
-      hipLaunchKernelGGL(kcg_dir_grad_kernel<T>, dim3(blocks_for(g.nc, TPB)), dim3(TPB), 0, ctx->stream, z, p,
-                         ctx->d_scal, it == 0, s[0], s[1], s[2], sh<T>(ctx, 0), sh<T>(ctx, 1), sh<T>(ctx, 2), g);
-      KAPSEL_LAUNCH_CHECK();
-    }
-    fft_c2r_t<T>(ctx, s3, 3);
-    {
-      LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 7, sizeof(T)));
-      hipLaunchKernelGGL(face_mul_kernel<T>, dim3(blocks_for(g.nr, TPB)), dim3(TPB), 0, ctx->stream, s[0], s[1],
-                         s[2], eps, g);
-      KAPSEL_LAUNCH_CHECK();
-    }
-    fft_r2c_t<T>(ctx, s3, 3);
-    {
-      LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 5, sizeof(T)));
-      hipLaunchKernelGGL(kcg_div_dot_kernel<T>, dim3(NPART), dim3(TPB), 0, ctx->stream, s[0], s[1], s[2], p, ap,
-                         sh<T>(ctx, 0), sh<T>(ctx, 1), sh<T>(ctx, 2), ctx->d_partial, g);
-      KAPSEL_LAUNCH_CHECK();
-      hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->d_partial, NPART, ctx->d_scal,
-                         S_PAP, 0);
-      KAPSEL_LAUNCH_CHECK();
-    }
-    {
-      LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, precond ? 7 : 6, sizeof(T)));
-      hipLaunchKernelGGL(kcg_update_kernel<T>, dim3(NPART), dim3(TPB), 0, ctx->stream, x, r, z, p, ap,
-                         ctx->d_scal, inv_eps, precond, ctx->d_partial, g);
-      KAPSEL_LAUNCH_CHECK();
-      hipLaunchKernelGGL(finish2_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->d_partial, ctx->d_scal, 1);
-      KAPSEL_LAUNCH_CHECK();
-    }
+    kcg_enqueue<T>(ctx, x, eps, precond, w, s);
     it++;
-    rr = read_scal(ctx, S_RR) * norm;
+    KAPSEL_HIP(hipMemcpyAsync(ctx->h_slots + (it & 1), ctx->d_scal + S_RR, sizeof(double), hipMemcpyDeviceToHost,
+                              ctx->stream));
+    KAPSEL_HIP(hipEventRecord(ctx->slot_ev[it & 1], ctx->stream));
+    if (it - known == 2 || it == maxit) {
+      known++;
+      KAPSEL_HIP(hipEventSynchronize(ctx->slot_ev[known & 1]));
+      rr = ctx->h_slots[known & 1] * norm;
+    }
+  }
+  while (known < it) {  // the iteration still in flight
+    known++;
+    KAPSEL_HIP(hipEventSynchronize(ctx->slot_ev[known & 1]));
+    rr = ctx->h_slots[known & 1] * norm;
   }
   *rr_out = rr;
   return it;
@@ -733,6 +823,18 @@ double true_relres(kapsel_ctx *ctx, const T *x, const T *b, const T *eps, T *r, 
   dot_t<T, T>(ctx, b, b, S_BB);
   double rr = read_scal(ctx, S_RR), bb = read_scal(ctx, S_BB);
   return bb > 0 ? sqrt(rr / bb) : 0.0;
+}
+
+// sc[S_INVEPS] = 1 / mean(eps), without a host round trip
+template <typename T>
+void inv_eps_device(kapsel_ctx *ctx, const T *eps, T *scratch) {
+  const Grid &g = ctx->g;
+  hipLaunchKernelGGL(fill_kernel<T>, dim3(blocks_for(g.f, TPB)), dim3(TPB), 0, ctx->stream, scratch, (T)1, g.f);
+  KAPSEL_LAUNCH_CHECK();
+  dot_t<T, T>(ctx, eps, scratch, S_TMP);
+  hipLaunchKernelGGL(inv_mean_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->d_scal, S_TMP, S_INVEPS,
+                     (double)g.nr);
+  KAPSEL_LAUNCH_CHECK();
 }
 
 template <typename T>
@@ -800,7 +902,7 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
     *relres = 0.0;
     return 0;
   }
-  const double inv_eps = precond ? inv_eps_of<float>(ctx, eps32, w32[3]) : 1.0;
+  if (precond) inv_eps_device<float>(ctx, eps32, w32[3]);
   const double bnorm = sqrt(bb);
   int total = 0;
   double rel = 1.0;
@@ -826,7 +928,7 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
     fft_r2c_t<float>(ctx, bk, 1);
     double rr_in = 0.0;
     const double tol_in = fmax(1e-4 * sqrt(rr), 0.5 * tol * bnorm);
-    int it = kcg_t<float>(ctx, d32, b32, eps32, tol_in, maxit - total, precond, inv_eps, w32, s32 + 1, &rr_in);
+    int it = kcg_t<float>(ctx, d32, b32, eps32, tol_in, maxit - total, precond, w32, s32 + 1, &rr_in);
     total += it;
     void *dk[1] = {d32};
     fft_c2r_t<float>(ctx, dk, 1);
@@ -981,6 +1083,10 @@ void poisson_free(kapsel_ctx *ctx) {
   for (double *&p : ctx->p64)
     if (p) (void)hipFree(p), p = nullptr;
   if (ctx->d_partial) (void)hipFree(ctx->d_partial), ctx->d_partial = nullptr;
+  if (ctx->h_slots) (void)hipHostFree(ctx->h_slots), ctx->h_slots = nullptr;
+  if (ctx->kcg_exec) (void)hipGraphExecDestroy(ctx->kcg_exec), ctx->kcg_exec = nullptr;
+  for (hipEvent_t &e : ctx->slot_ev)
+    if (e) (void)hipEventDestroy(e), e = nullptr;
   if (ctx->d_scal) (void)hipFree(ctx->d_scal), ctx->d_scal = nullptr;
 }
 

@@ -162,3 +162,29 @@ def test_icep_step_fp64_matches_oracle():
     for f in ("X", "V", "OMEGA"):
         i = abi.PFIELD[f]
         assert rel(pk[i], po[i]) < 1e-9, f
+
+
+def test_fp32_solve_graph_replay_matches_eager():
+    """The FP32 correction solve replays one captured k-space CG iteration as
+    a HIP graph; with kernel timing on, the same iteration is launched
+    eagerly.  Repeated solves (eager FP64 refinement work between the
+    replays) give bit-identical iterates either way."""
+    n = 64
+    print("case fp32 solve graph vs eager 64^3", flush=True)
+    p, st = _case(n, 6, 6.0, 0.1, 14, precision=abi.KAPSEL_FP32)
+    k = gpu(p)
+    k.set_particles(st)
+    k.make_dielectric_field()
+    e0 = [0.0, 0.03, 0.1]
+    out = []
+    for mode in ("graph", "eager", "graph", "eager", "graph"):
+        k.kernel_timing(["POINTWISE"] if mode == "eager" else [])
+        k.set_field("POTENTIAL", np.zeros(k.shape))
+        it, rr = k.poisson_solve(e0, 1e-10, 5000, 1)
+        out.append((it, rr, k.get_field("POTENTIAL")[:, :, :n]))
+        print(f"{mode}: {it} iterations, relres {rr:.3e}", flush=True)
+    k.kernel_timing([])
+    for it, rr, x in out[1:]:
+        assert it == out[0][0] and rr == out[0][1]
+        assert np.array_equal(x, out[0][2])
+    assert out[0][1] < 1e-10
