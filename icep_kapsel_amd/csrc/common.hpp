@@ -139,8 +139,12 @@ struct kapsel_ctx {
   void *pscratch[5] = {};          // [0..3] operator scratch, [4] refinement correction
   double *p64[8] = {};            // eps, x, b, r, 4 operator scratch
   // FP32 context: p64[1] holds the FP64 potential of the last solve; it seeds
-  // the next solve unless the FP32 POTENTIAL field was written since
+  // the next solve unless the FP32 POTENTIAL field was written since.
+  // x64_prev holds the solve before it (when prev_valid), so the next start
+  // can be the linear extrapolation 2 x_n - x_{n-1} in time
   bool pot64_valid = false;
+  bool prev_valid = false;
+  double *x64_prev = nullptr;
   double *d_partial = nullptr;    // block partial sums (reductions)
   double *d_scal = nullptr;       // CG scalars
   double *h_slots = nullptr;      // pinned host copies of ||r||^2 (pipelined convergence checks)

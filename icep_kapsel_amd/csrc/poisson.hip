@@ -471,6 +471,16 @@ __global__ void inv_mean_kernel(double *__restrict__ sc, int sum, int out, doubl
   if (threadIdx.x == 0 && blockIdx.x == 0) sc[out] = sc[sum] > 0.0 ? count / sc[sum] : 1.0;
 }
 
+// x <- 2 x - prev, prev <- old x  (linear extrapolation of the potential in time)
+__global__ void extrapolate_kernel(double *__restrict__ x, double *__restrict__ prev, Grid g) {
+  size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= g.nr) return;
+  RI pt = ridx(g, q);
+  const double xn = x[pt.im];
+  x[pt.im] = 2.0 * xn - prev[pt.im];
+  prev[pt.im] = xn;
+}
+
 template <typename T>
 __global__ void scale_real_kernel(T *__restrict__ x, double s, Grid g) {
   size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -571,6 +581,8 @@ void ensure_storage(kapsel_ctx *ctx) {
       KAPSEL_HIP(hipMalloc(&p, ctx->g.f * sizeof(double)));
       KAPSEL_HIP(hipMemsetAsync(p, 0, ctx->g.f * sizeof(double), ctx->stream));
     }
+    KAPSEL_HIP(hipMalloc(&ctx->x64_prev, ctx->g.f * sizeof(double)));
+    KAPSEL_HIP(hipMemsetAsync(ctx->x64_prev, 0, ctx->g.f * sizeof(double), ctx->stream));
   }
 }
 
@@ -604,6 +616,50 @@ __global__ void fill_kernel(T *__restrict__ a, T v, size_t n) {
   size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e < n) a[e] = v;
 }
+
+// Pipelined convergence test of the CG loops.  Each queued iteration copies
+// its ||r||^2 into pinned slot it % 2 and records an event; the host keeps
+// one more iteration queued while it waits for the older one, so the GPU
+// does not drain between iterations.  It stops queueing ahead when the
+// geometric trend of the last two values says the iteration in flight
+// meets the bound, so normally no iteration runs past it.
+struct ConvergenceWatch {
+  kapsel_ctx *ctx;
+  double scale;
+  double rr, rr_prev = -1.0;
+  int it = 0, known = 0;
+  ConvergenceWatch(kapsel_ctx *c, double s, double rr0) : ctx(c), scale(s), rr(rr0) {}
+  void queued() {
+    it++;
+    KAPSEL_HIP(hipMemcpyAsync(ctx->h_slots + (it & 1), ctx->d_scal + S_RR, sizeof(double), hipMemcpyDeviceToHost,
+                              ctx->stream));
+    KAPSEL_HIP(hipEventRecord(ctx->slot_ev[it & 1], ctx->stream));
+  }
+  void wait_one() {
+    known++;
+    KAPSEL_HIP(hipEventSynchronize(ctx->slot_ev[known & 1]));
+    rr_prev = rr;
+    rr = ctx->h_slots[known & 1] * scale;
+  }
+  // runs enqueue() until ||r||^2 < tol2 or maxit iterations
+  template <typename F>
+  void run(double tol2, int maxit, F enqueue) {
+    for (;;) {
+      if (known == it) {
+        if (rr < tol2 || it >= maxit) return;
+        enqueue();
+        queued();
+      } else {
+        const bool ahead = it < maxit && rr_prev > 0.0 && rr * (rr / rr_prev) >= tol2;
+        if (ahead) {
+          enqueue();
+          queued();
+        }
+        wait_one();
+      }
+    }
+  }
+};
 
 // z = M^{-1} r
 template <typename T>
@@ -642,9 +698,8 @@ int cg_t(kapsel_ctx *ctx, T *x, const T *b, const T *eps, double tol_abs, int ma
   dot_t<T, T>(ctx, r, r, S_RR);
   double rr = read_scal(ctx, S_RR);
   const double tol2 = tol_abs * tol_abs;
-  // convergence test pipelined as in kcg_t (one iteration of lag)
-  int it = 0, known = 0;
-  while (rr >= tol2 && it < maxit) {
+  ConvergenceWatch cw(ctx, 1.0, rr);
+  cw.run(tol2, maxit, [&]() {
     ax_t<T>(ctx, p, ap, eps, s);
     dot_t<T, T>(ctx, p, ap, S_PAP);
     {
@@ -664,21 +719,9 @@ int cg_t(kapsel_ctx *ctx, T *x, const T *b, const T *eps, double tol_abs, int ma
                          ctx->d_scal, g);
       KAPSEL_LAUNCH_CHECK();
     }
-    it++;
-    KAPSEL_HIP(hipMemcpyAsync(ctx->h_slots + (it & 1), ctx->d_scal + S_RR, sizeof(double), hipMemcpyDeviceToHost,
-                              ctx->stream));
-    KAPSEL_HIP(hipEventRecord(ctx->slot_ev[it & 1], ctx->stream));
-    if (it - known == 2 || it == maxit) {
-      known++;
-      KAPSEL_HIP(hipEventSynchronize(ctx->slot_ev[known & 1]));
-      rr = ctx->h_slots[known & 1];
-    }
-  }
-  while (known < it) {
-    known++;
-    KAPSEL_HIP(hipEventSynchronize(ctx->slot_ev[known & 1]));
-    rr = ctx->h_slots[known & 1];
-  }
+  });
+  rr = cw.rr;
+  const int it = cw.it;
   *rr_out = rr;
   return it;
 }
@@ -781,28 +824,10 @@ int kcg_t(kapsel_ctx *ctx, T *x, const T *b, const T *eps, double tol_abs, int m
   }
   double rr = read_scal(ctx, S_RR) * norm;
   const double tol2 = tol_abs * tol_abs;
-  // Pipelined convergence test: iteration it copies its ||r||^2 into pinned
-  // slot it % 2; the host reads iteration it - 1's value only after queueing
-  // iteration it, so the GPU never drains between iterations.  The test thus
-  // lags one iteration: at most one extra iteration runs past the bound.
-  int it = 0, known = 0;
-  while (rr >= tol2 && it < maxit) {
-    kcg_enqueue<T>(ctx, x, eps, precond, w, s);
-    it++;
-    KAPSEL_HIP(hipMemcpyAsync(ctx->h_slots + (it & 1), ctx->d_scal + S_RR, sizeof(double), hipMemcpyDeviceToHost,
-                              ctx->stream));
-    KAPSEL_HIP(hipEventRecord(ctx->slot_ev[it & 1], ctx->stream));
-    if (it - known == 2 || it == maxit) {
-      known++;
-      KAPSEL_HIP(hipEventSynchronize(ctx->slot_ev[known & 1]));
-      rr = ctx->h_slots[known & 1] * norm;
-    }
-  }
-  while (known < it) {  // the iteration still in flight
-    known++;
-    KAPSEL_HIP(hipEventSynchronize(ctx->slot_ev[known & 1]));
-    rr = ctx->h_slots[known & 1] * norm;
-  }
+  ConvergenceWatch cw(ctx, norm, rr);
+  cw.run(tol2, maxit, [&]() { kcg_enqueue<T>(ctx, x, eps, precond, w, s); });
+  rr = cw.rr;
+  const int it = cw.it;
   *rr_out = rr;
   return it;
 }
@@ -889,9 +914,21 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
                      eps64, eps32, g);
   hipLaunchKernelGGL((convert_real_kernel<double, float>), dim3(blocks_for(g.f, TPB)), dim3(TPB), 0, ctx->stream,
                      r64, fld<float>(ctx, KAPSEL_RHO_FREE), g);
-  if (!ctx->pot64_valid)
+  // start: the POTENTIAL field if it was written since the last solve; else
+  // the last solution, extrapolated linearly in time once two solutions of
+  // consecutive solves are known
+  if (!ctx->pot64_valid) {
     hipLaunchKernelGGL((convert_real_kernel<double, float>), dim3(blocks_for(g.f, TPB)), dim3(TPB), 0, ctx->stream,
                        x64, fld<float>(ctx, KAPSEL_POTENTIAL), g);
+    ctx->prev_valid = false;
+  } else if (ctx->prev_valid) {
+    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 4, sizeof(double)));
+    hipLaunchKernelGGL(extrapolate_kernel, dim3(blocks_for(g.nr, TPB)), dim3(TPB), 0, ctx->stream, x64,
+                       ctx->x64_prev, g);
+  } else {
+    KAPSEL_HIP(hipMemcpyAsync(ctx->x64_prev, x64, g.f * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    ctx->prev_valid = true;
+  }
   KAPSEL_LAUNCH_CHECK();
   build_rhs_t<double>(ctx, b64, r64, eps64, e0, s64);
   dot_t<double, double>(ctx, b64, b64, S_BB);
@@ -1082,6 +1119,7 @@ void poisson_free(kapsel_ctx *ctx) {
     if (p) (void)hipFree(p), p = nullptr;
   for (double *&p : ctx->p64)
     if (p) (void)hipFree(p), p = nullptr;
+  if (ctx->x64_prev) (void)hipFree(ctx->x64_prev), ctx->x64_prev = nullptr;
   if (ctx->d_partial) (void)hipFree(ctx->d_partial), ctx->d_partial = nullptr;
   if (ctx->h_slots) (void)hipHostFree(ctx->h_slots), ctx->h_slots = nullptr;
   if (ctx->kcg_exec) (void)hipGraphExecDestroy(ctx->kcg_exec), ctx->kcg_exec = nullptr;
