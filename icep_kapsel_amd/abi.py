@@ -33,7 +33,8 @@ P_NFIELDS = len(PFIELDS)
 PRECOND_NONE = 0
 PRECOND_SPECTRAL = 1
 
-KERNEL_CLASSES = ["FFT_Z", "FFT_Y", "FFT_X", "POINTWISE", "PARTICLE"]
+KERNEL_CLASSES = ["FFT_Z_R2C", "FFT_Z_C2R", "FFT_LINES_FWD", "FFT_LINES_INV", "POINTWISE", "PARTICLE",
+                  "NETWORK", "REFINE64"]
 KCLASS = {name: i for i, name in enumerate(KERNEL_CLASSES)}
 
 

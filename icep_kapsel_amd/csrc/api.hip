@@ -34,8 +34,9 @@ static hipEvent_t take_event(kapsel_ctx *ctx) {
   return e;
 }
 
-LaunchTimer::LaunchTimer(kapsel_ctx *c, int k, double b) : ctx(c), cls(k), bytes(b) {
-  if (ctx->timing_mask & (1 << k)) {
+LaunchTimer::LaunchTimer(kapsel_ctx *c, int k, double b)
+    : ctx(c), cls(c->refine64 ? KAPSEL_K_REFINE64 : k), bytes(b) {
+  if (ctx->timing_mask & (1 << cls)) {
     start = take_event(ctx);
     stop = take_event(ctx);
     KAPSEL_HIP(hipEventRecord(start, ctx->stream));

@@ -143,6 +143,7 @@ struct kapsel_ctx {
   // x64_prev holds the solve before it (when prev_valid), so the next start
   // can be the linear extrapolation 2 x_n - x_{n-1} in time
   bool pot64_valid = false;
+  bool refine64 = false;  // inside the FP64 part of an FP32 solve: launches time as KAPSEL_K_REFINE64
   bool prev_valid = false;
   double *x64_prev = nullptr;
   double *d_partial = nullptr;    // block partial sums (reductions)

@@ -104,6 +104,22 @@ struct FieldPtrs {
   T *p[5];
 };
 
+// XCD-aware tile order.  Workgroups are dispatched round-robin over the 8
+// XCDs by linear id (blockIdx.y * gridDim.x + blockIdx.x), and each XCD has
+// its own L2; remapping gives each XCD one contiguous range of (field, tile)
+// ids, so the cache lines that two neighbouring tiles share are fetched once,
+// by one L2.  Returns the remapped linear id; field = id / gridDim.x.
+__device__ __forceinline__ unsigned xcd_tile(unsigned &field) {
+  constexpr unsigned NXCD = 8;
+  const unsigned total = gridDim.x * gridDim.y;
+  const unsigned lin = blockIdx.y * gridDim.x + blockIdx.x;
+  const unsigned q = total / NXCD, r = total % NXCD;
+  const unsigned x = lin % NXCD, k = lin / NXCD;
+  const unsigned id = x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+  field = id / gridDim.x;
+  return id % gridDim.x;
+}
+
 // twiddle W_L^m, table holds the forward sign
 template <int DIR, typename T>
 __device__ __forceinline__ V2<T> tw_get(const V2<T> *__restrict__ tw, int m) {
@@ -122,11 +138,13 @@ __global__ __launch_bounds__(LPB *FS<L>::R2) void fft_c2c_lines(FieldPtrs<T> fp,
   constexpr int R1 = FS<L>::R1, R2 = FS<L>::R2, TPL = R2;
   constexpr int LS = R2 * (R1 + 1) + 1;  // LDS line stride (odd)
   __shared__ V2<T> sm[LPB * LS];
-  T *__restrict__ data = fp.p[blockIdx.y];
+  unsigned field;
+  const unsigned tile = xcd_tile(field);
+  T *__restrict__ data = fp.p[field];
   const int t = threadIdx.x;
   const int l = t % LPB;
   const int n2 = t / LPB;
-  const long long q = (long long)blockIdx.x * LPB + l;
+  const long long q = (long long)tile * LPB + l;
   const bool valid = q < Q;
   const long long base = valid ? (q / A) * B + (q % A) : 0;
 
@@ -167,11 +185,13 @@ __global__ __launch_bounds__(RPB *FS<NZ / 2>::R2) void fft_z_r2c(FieldPtrs<T> fp
   constexpr int RS = R2 * (R1 + 1) + 1;
   static_assert(RS >= M + 1, "row stride");
   __shared__ V2<T> sm[RPB * RS];
-  T *__restrict__ data = fp.p[blockIdx.y];
+  unsigned field;
+  const unsigned tile = xcd_tile(field);
+  T *__restrict__ data = fp.p[field];
   const int t = threadIdx.x;
   const int n2 = t % TPL;
   const int r = t / TPL;
-  const long long row = (long long)blockIdx.x * RPB + r;
+  const long long row = (long long)tile * RPB + r;
   const bool valid = row < rows;
   T *__restrict__ rp = data + (valid ? row : 0) * nz_;
   V2<T> *sr = sm + r * RS;
@@ -223,11 +243,13 @@ __global__ __launch_bounds__(RPB *FS<NZ / 2>::R2) void fft_z_c2r(FieldPtrs<T> fp
   constexpr int R1 = FS<M>::R1, R2 = FS<M>::R2, TPL = R2;
   constexpr int RS = R2 * (R1 + 1) + 1;
   __shared__ V2<T> sm[RPB * RS];
-  T *__restrict__ data = fp.p[blockIdx.y];
+  unsigned field;
+  const unsigned tile = xcd_tile(field);
+  T *__restrict__ data = fp.p[field];
   const int t = threadIdx.x;
   const int n2 = t % TPL;
   const int r = t / TPL;
-  const long long row = (long long)blockIdx.x * RPB + r;
+  const long long row = (long long)tile * RPB + r;
   const bool valid = row < rows;
   T *__restrict__ rp = data + (valid ? row : 0) * nz_;
   V2<T> *sr = sm + r * RS;
@@ -270,6 +292,209 @@ __global__ __launch_bounds__(RPB *FS<NZ / 2>::R2) void fft_z_c2r(FieldPtrs<T> fp
     rp[NZ] = 0;
     rp[NZ + 1] = 0;
   }
+}
+
+// ------------------------------------------------------------ plane passes
+// The z and y transforms of one x-plane in one workgroup.  Plane i is the
+// NY rows of H = NZ/2+1 complex values at complex offset i*NY*H, contiguous
+// in memory; it is read once into LDS, the row (z) and column (y)
+// transforms run out of LDS, and it is written once: one read and one write
+// of the field for two of the three axes.  Each line's four-step exchange
+// goes through the line's own LDS slots (its inputs are all in registers by
+// then), so the plane is the only LDS a workgroup holds.  Used where the
+// plane fits (gfx950: 160 KB of LDS per workgroup).
+template <typename T, int NY, int NZ, int THREADS>
+struct PlaneCfg {
+  static constexpr int M = NZ / 2, H = NZ / 2 + 1;
+  static constexpr int ZR1 = FS<M>::R1, ZR2 = FS<M>::R2;
+  static constexpr int ZROWS = THREADS / ZR2;  // rows per round
+  static constexpr int YR1 = FS<NY>::R1, YR2 = FS<NY>::R2;
+  static constexpr int YCOLS = THREADS / YR2;  // columns per round
+  static constexpr int PLANE = NY * H;
+  static constexpr size_t LDS_BYTES = sizeof(V2<T>) * PLANE;
+};
+
+// y transform of every column of the plane in P (in place), direction DIR
+template <typename T, int NY, int NZ, int THREADS, int DIR>
+__device__ __forceinline__ void plane_columns(V2<T> *P, const V2<T> *__restrict__ twy) {
+  using C = PlaneCfg<T, NY, NZ, THREADS>;
+  constexpr int H = C::H, R1 = C::YR1, R2 = C::YR2;
+  // consecutive lanes take consecutive columns: the lanes of a wavefront
+  // touch consecutive LDS words (the column stride H would alias banks)
+  const int t = threadIdx.x;
+  const int cl = t % C::YCOLS, n2 = t / C::YCOLS;
+  for (int c0 = 0; c0 < H; c0 += C::YCOLS) {
+    const int kz = c0 + cl;
+    const bool valid = kz < H;
+    V2<T> *col = P + (valid ? kz : 0);
+    V2<T> v[R1], o[R1];
+#pragma unroll
+    for (int n1 = 0; n1 < R1; n1++) v[n1] = valid ? col[(R2 * n1 + n2) * H] : V2<T>{0, 0};
+    dft<R1, 1, DIR>(v, o);
+#pragma unroll
+    for (int k1 = 1; k1 < R1; k1++) o[k1] = cmul(o[k1], tw_get<DIR>(twy, n2 * k1));
+    __syncthreads();  // every column of the round is in registers
+    if (valid) {
+#pragma unroll
+      for (int k1 = 0; k1 < R1; k1++) col[(n2 * R1 + k1) * H] = o[k1];
+    }
+    __syncthreads();
+    if (valid) {
+      // the slots read for k1 are the slots written for k1 (same thread)
+#pragma unroll
+      for (int c = 0; c < R1 / R2; c++) {
+        const int k1 = n2 + R2 * c;
+        V2<T> w[R2], x[R2];
+#pragma unroll
+        for (int m = 0; m < R2; m++) w[m] = col[(m * R1 + k1) * H];
+        dft<R2, 1, DIR>(w, x);
+#pragma unroll
+        for (int k2 = 0; k2 < R2; k2++) col[(k1 + R1 * k2) * H] = x[k2];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T, int NY, int NZ, int THREADS>
+__global__ __launch_bounds__(THREADS) void fft_plane_r2c(FieldPtrs<T> fp, const V2<T> *__restrict__ twm,
+                                                         const V2<T> *__restrict__ zrot,
+                                                         const V2<T> *__restrict__ twy) {
+  using C = PlaneCfg<T, NY, NZ, THREADS>;
+  constexpr int M = C::M, H = C::H, R1 = C::ZR1, R2 = C::ZR2;
+  constexpr int KPT = M / R2 + 1;  // split outputs per thread (at most)
+  __shared__ V2<T> P[C::PLANE];
+  unsigned field;
+  const unsigned plane = xcd_tile(field);
+  T *__restrict__ gp = fp.p[field] + (size_t)plane * 2 * C::PLANE;
+  const int t = threadIdx.x;
+#pragma unroll 4
+  for (int e = t; e < C::PLANE; e += THREADS) P[e] = ld2(gp + 2 * e);
+  __syncthreads();
+  // z: r2c of each row -- a length-M complex FFT of the packed reals, then
+  // X[k] = E + W^k O, E = (Z[k] + conj Z[M-k]) / 2, O = (Z[k] - conj Z[M-k]) / 2i
+  // (consecutive lanes take consecutive rows)
+  const int rl = t % C::ZROWS, n2 = t / C::ZROWS;
+  for (int r0 = 0; r0 < NY; r0 += C::ZROWS) {
+    const int j = r0 + rl;
+    const bool valid = j < NY;
+    V2<T> *row = P + (valid ? j : 0) * H;
+    V2<T> v[R1], o[R1];
+#pragma unroll
+    for (int n1 = 0; n1 < R1; n1++) v[n1] = valid ? row[R2 * n1 + n2] : V2<T>{0, 0};
+    dft<R1, 1, -1>(v, o);
+#pragma unroll
+    for (int k1 = 1; k1 < R1; k1++) o[k1] = cmul(o[k1], tw_get<-1>(twm, n2 * k1));
+    __syncthreads();
+    if (valid) {
+#pragma unroll
+      for (int k1 = 0; k1 < R1; k1++) row[n2 * R1 + k1] = o[k1];
+    }
+    __syncthreads();
+    if (valid) {
+#pragma unroll
+      for (int c = 0; c < R1 / R2; c++) {
+        const int k1 = n2 + R2 * c;
+        V2<T> w[R2], x[R2];
+#pragma unroll
+        for (int m = 0; m < R2; m++) w[m] = row[m * R1 + k1];
+        dft<R2, 1, -1>(w, x);
+#pragma unroll
+        for (int k2 = 0; k2 < R2; k2++) row[k1 + R1 * k2] = x[k2];  // Z[k], natural order
+      }
+    }
+    __syncthreads();
+    V2<T> zk[KPT], zm[KPT];
+#pragma unroll
+    for (int q = 0; q < KPT; q++) {
+      const int k = n2 + R2 * q;
+      if (valid && k <= M) {
+        zk[q] = row[k & (M - 1)];
+        zm[q] = conjv(row[(M - k) & (M - 1)]);
+      }
+    }
+    __syncthreads();
+    if (valid) {
+#pragma unroll
+      for (int q = 0; q < KPT; q++) {
+        const int k = n2 + R2 * q;
+        if (k <= M) {
+          const V2<T> e = scal((T)0.5, zk[q] + zm[q]);
+          const V2<T> d = zk[q] - zm[q];
+          const V2<T> od = {(T)0.5 * d.im, (T)-0.5 * d.re};
+          row[k] = e + cmul(zrot[k], od);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  plane_columns<T, NY, NZ, THREADS, -1>(P, twy);
+#pragma unroll 4
+  for (int e = t; e < C::PLANE; e += THREADS) st2(gp + 2 * e, P[e]);
+}
+
+template <typename T, int NY, int NZ, int THREADS>
+__global__ __launch_bounds__(THREADS) void fft_plane_c2r(FieldPtrs<T> fp, const V2<T> *__restrict__ twm,
+                                                         const V2<T> *__restrict__ zrot,
+                                                         const V2<T> *__restrict__ twy, T scale) {
+  using C = PlaneCfg<T, NY, NZ, THREADS>;
+  constexpr int M = C::M, H = C::H, R1 = C::ZR1, R2 = C::ZR2;
+  __shared__ V2<T> P[C::PLANE];
+  unsigned field;
+  const unsigned plane = xcd_tile(field);
+  T *__restrict__ gp = fp.p[field] + (size_t)plane * 2 * C::PLANE;
+  const int t = threadIdx.x;
+#pragma unroll 4
+  for (int e = t; e < C::PLANE; e += THREADS) P[e] = ld2(gp + 2 * e);
+  __syncthreads();
+  plane_columns<T, NY, NZ, THREADS, +1>(P, twy);
+  // z: c2r of each row (the kz = 0 and kz = NZ/2 entries are taken as real)
+  const int rl = t % C::ZROWS, n2 = t / C::ZROWS;
+  for (int r0 = 0; r0 < NY; r0 += C::ZROWS) {
+    const int j = r0 + rl;
+    const bool valid = j < NY;
+    V2<T> *row = P + (valid ? j : 0) * H;
+    V2<T> v[R1], o[R1];
+#pragma unroll
+    for (int n1 = 0; n1 < R1; n1++) {
+      const int n = R2 * n1 + n2;
+      V2<T> xn = valid ? row[n] : V2<T>{0, 0};
+      V2<T> xm = valid ? row[M - n] : V2<T>{0, 0};
+      if (n == 0) {
+        xn.im = 0;
+        xm.im = 0;  // M - n = M
+      }
+      xm = conjv(xm);
+      const V2<T> e = xn + xm;                         // 2 E[n]
+      const V2<T> od = cmul(xn - xm, conjv(zrot[n]));  // 2 O[n]
+      v[n1] = V2<T>{e.re - od.im, e.im + od.re};       // E + i O
+    }
+    dft<R1, 1, +1>(v, o);
+#pragma unroll
+    for (int k1 = 1; k1 < R1; k1++) o[k1] = cmul(o[k1], tw_get<+1>(twm, n2 * k1));
+    __syncthreads();  // every row of the round is in registers
+    if (valid) {
+#pragma unroll
+      for (int k1 = 0; k1 < R1; k1++) row[n2 * R1 + k1] = o[k1];
+    }
+    __syncthreads();
+    if (valid) {
+#pragma unroll
+      for (int c = 0; c < R1 / R2; c++) {
+        const int k1 = n2 + R2 * c;
+        V2<T> w[R2], x[R2];
+#pragma unroll
+        for (int m = 0; m < R2; m++) w[m] = row[m * R1 + k1];
+        dft<R2, 1, +1>(w, x);
+#pragma unroll
+        for (int k2 = 0; k2 < R2; k2++) row[k1 + R1 * k2] = scal(scale, x[k2]);
+      }
+      if (n2 == 0) row[M] = V2<T>{0, 0};
+    }
+    __syncthreads();
+  }
+#pragma unroll 4
+  for (int e = t; e < C::PLANE; e += THREADS) st2(gp + 2 * e, P[e]);
 }
 
 // ------------------------------------------------------------------- host
@@ -352,7 +577,7 @@ void launch_z(kapsel_ctx *ctx, const FieldPtrs<T> &fp, int nf, bool fwd) {
   long long rows = (long long)g.nx * g.ny;
   dim3 grid((unsigned)((rows + RPB - 1) / RPB), nf);
   dim3 block(RPB * FS<NZ / 2>::R2);
-  LaunchTimer lt(ctx, KAPSEL_K_FFT_Z, 2.0 * nf * g.f * sizeof(T));
+  LaunchTimer lt(ctx, fwd ? KAPSEL_K_FFT_Z_R2C : KAPSEL_K_FFT_Z_C2R, 2.0 * nf * g.f * sizeof(T));
   const Twiddles &tw = ctx->tw[prec_of<T>()];
   if (fwd) {
     hipLaunchKernelGGL((fft_z_r2c<T, NZ, RPB>), grid, block, 0, ctx->stream, fp, rows, g.nz_,
@@ -377,6 +602,44 @@ void z_dispatch(kapsel_ctx *ctx, const FieldPtrs<T> &fp, int nf, bool fwd) {
   }
 }
 
+template <typename T, int N, int THREADS>
+void launch_plane(kapsel_ctx *ctx, const FieldPtrs<T> &fp, int nf, bool fwd) {
+  const Grid &g = ctx->g;
+  static_assert(PlaneCfg<T, N, N, THREADS>::LDS_BYTES <= 160 * 1024, "plane does not fit LDS");
+  dim3 grid((unsigned)g.nx, nf);
+  LaunchTimer lt(ctx, fwd ? KAPSEL_K_FFT_Z_R2C : KAPSEL_K_FFT_Z_C2R, 2.0 * nf * g.f * sizeof(T));
+  const Twiddles &tw = ctx->tw[prec_of<T>()];
+  if (fwd) {
+    hipLaunchKernelGGL((fft_plane_r2c<T, N, N, THREADS>), grid, dim3(THREADS), 0, ctx->stream, fp,
+                       (const V2<T> *)tw.zh, (const V2<T> *)tw.zrot, (const V2<T> *)tw.y);
+  } else {
+    T scale = (T)(1.0 / ((double)g.nx * g.ny * g.nz));
+    hipLaunchKernelGGL((fft_plane_c2r<T, N, N, THREADS>), grid, dim3(THREADS), 0, ctx->stream, fp,
+                       (const V2<T> *)tw.zh, (const V2<T> *)tw.zrot, (const V2<T> *)tw.y, scale);
+  }
+  KAPSEL_LAUNCH_CHECK();
+}
+
+// the z and y transforms as one plane pass (NY = NZ = 32, 64; 128 in FP32);
+// false: use the z and y line passes
+template <typename T>
+bool plane_dispatch(kapsel_ctx *ctx, const FieldPtrs<T> &fp, int nf, bool fwd) {
+  const Grid &g = ctx->g;
+  if (g.ny != g.nz) return false;
+  switch (g.nz) {
+    // workgroup sizes: every column of the plane in one round
+    case 32: launch_plane<T, 32, 128>(ctx, fp, nf, fwd); return true;
+    case 64: launch_plane<T, 64, 320>(ctx, fp, nf, fwd); return true;
+    case 128:
+      // an FP64 128 x 65 plane (133 KB) leaves one workgroup per CU, slower
+      // than the separate z and y passes
+      if constexpr (sizeof(T) == 8) return false;
+      launch_plane<T, 128, 576>(ctx, fp, nf, fwd);
+      return true;
+    default: return false;
+  }
+}
+
 template <typename T>
 FieldPtrs<T> pack(void *const *fields, int nf) {
   FieldPtrs<T> fp{};
@@ -391,13 +654,15 @@ void fft_r2c_t(kapsel_ctx *ctx, void *const *fields, int nf) {
   const Grid &g = ctx->g;
   const Twiddles &tw = ctx->tw[prec_of<T>()];
   FieldPtrs<T> fp = pack<T>(fields, nf);
-  z_dispatch<T>(ctx, fp, nf, true);
-  // y lines: q = i*h + kz -> offset i*NY*h + kz, stride h
-  lines_dispatch<T, -1>(ctx, KAPSEL_K_FFT_Y, g.ny, fp, nf, (long long)g.nx * g.h, g.h, (long long)g.ny * g.h,
-                        g.h, tw.y);
+  if (!plane_dispatch<T>(ctx, fp, nf, true)) {
+    z_dispatch<T>(ctx, fp, nf, true);
+    // y lines: q = i*h + kz -> offset i*NY*h + kz, stride h
+    lines_dispatch<T, -1>(ctx, KAPSEL_K_FFT_LINES_FWD, g.ny, fp, nf, (long long)g.nx * g.h, g.h,
+                          (long long)g.ny * g.h, g.h, tw.y);
+  }
   // x lines: q = j*h + kz -> offset q, stride NY*h
   long long plane = (long long)g.ny * g.h;
-  lines_dispatch<T, -1>(ctx, KAPSEL_K_FFT_X, g.nx, fp, nf, plane, plane, 0, plane, tw.x);
+  lines_dispatch<T, -1>(ctx, KAPSEL_K_FFT_LINES_FWD, g.nx, fp, nf, plane, plane, 0, plane, tw.x);
 }
 
 template <typename T>
@@ -406,10 +671,12 @@ void fft_c2r_t(kapsel_ctx *ctx, void *const *fields, int nf) {
   const Twiddles &tw = ctx->tw[prec_of<T>()];
   FieldPtrs<T> fp = pack<T>(fields, nf);
   long long plane = (long long)g.ny * g.h;
-  lines_dispatch<T, +1>(ctx, KAPSEL_K_FFT_X, g.nx, fp, nf, plane, plane, 0, plane, tw.x);
-  lines_dispatch<T, +1>(ctx, KAPSEL_K_FFT_Y, g.ny, fp, nf, (long long)g.nx * g.h, g.h, (long long)g.ny * g.h,
-                        g.h, tw.y);
-  z_dispatch<T>(ctx, fp, nf, false);
+  lines_dispatch<T, +1>(ctx, KAPSEL_K_FFT_LINES_INV, g.nx, fp, nf, plane, plane, 0, plane, tw.x);
+  if (!plane_dispatch<T>(ctx, fp, nf, false)) {
+    lines_dispatch<T, +1>(ctx, KAPSEL_K_FFT_LINES_INV, g.ny, fp, nf, (long long)g.nx * g.h, g.h,
+                          (long long)g.ny * g.h, g.h, tw.y);
+    z_dispatch<T>(ctx, fp, nf, false);
+  }
 }
 
 template void fft_r2c_t<double>(kapsel_ctx *, void *const *, int);

@@ -451,7 +451,7 @@ template <int CIN, int COUT>
 void conv_launch(kapsel_ctx *ctx, const bf16 *in, int n, const bf16 *K, const float *bias, int relu, int sign,
                  const bf16 *mask, bf16 *out, float *out32) {
   const size_t nv = nvox(n);
-  LaunchTimer lt(ctx, KAPSEL_K_PARTICLE, (double)nv * 2 * (CIN + COUT));
+  LaunchTimer lt(ctx, KAPSEL_K_NETWORK, (double)nv * 2 * (CIN + COUT));
   hipLaunchKernelGGL((conv_mfma_kernel<CIN, COUT>), dim3((unsigned)((nv + 63) / 64)), dim3(256), 0, ctx->stream,
                      in, n, K, bias, relu, sign, mask, out, out32);
   KAPSEL_LAUNCH_CHECK();
@@ -480,7 +480,7 @@ void dw_launch(kapsel_ctx *ctx, Pnet *pn, const bf16 *in, const bf16 *dy, int n)
   const size_t nv = nvox(n);
   const int nchunk = (int)((nv + DW_CHUNK - 1) / DW_CHUNK);
   const int chunk = (int)((nv + nchunk - 1) / nchunk + 15) / 16 * 16;
-  LaunchTimer lt(ctx, KAPSEL_K_PARTICLE, (double)nv * 2 * (CIN + COUT) * 27);
+  LaunchTimer lt(ctx, KAPSEL_K_NETWORK, (double)nv * 2 * (CIN + COUT) * 27);
   hipLaunchKernelGGL((conv_dw_kernel<CIN, COUT>), dim3(27, nchunk), dim3(256), 0, ctx->stream, in, dy, n, chunk,
                      pn->partial);
   KAPSEL_LAUNCH_CHECK();

@@ -899,9 +899,17 @@ int solve_fp64(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
   return it;
 }
 
+// marks the FP64 part of an FP32 context's solve for the kernel timing
+struct RefineScope {
+  kapsel_ctx *c;
+  explicit RefineScope(kapsel_ctx *x) : c(x) { c->refine64 = true; }
+  ~RefineScope() { c->refine64 = false; }
+};
+
 // FP32 context: FP64 iterative refinement around FP32 (P)CG corrections
 int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int precond, double *relres) {
   const Grid &g = ctx->g;
+  RefineScope refine(ctx);
   double *eps64 = ctx->p64[0], *x64 = ctx->p64[1], *b64 = ctx->p64[2], *r64 = ctx->p64[3];
   double *s64[4] = {ctx->p64[4], ctx->p64[5], ctx->p64[6], ctx->p64[7]};
   float *s32[4] = {(float *)ctx->pscratch[0], (float *)ctx->pscratch[1], (float *)ctx->pscratch[2],
@@ -962,6 +970,7 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
                        ctx->stream, b32, r64, g);
     KAPSEL_LAUNCH_CHECK();
     void *bk[1] = {b32};
+    ctx->refine64 = false;  // the FP32 correction solve
     fft_r2c_t<float>(ctx, bk, 1);
     double rr_in = 0.0;
     const double tol_in = fmax(1e-4 * sqrt(rr), 0.5 * tol * bnorm);
@@ -969,6 +978,7 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
     total += it;
     void *dk[1] = {d32};
     fft_c2r_t<float>(ctx, dk, 1);
+    ctx->refine64 = true;
     hipLaunchKernelGGL(add_correction_kernel<float>, dim3(blocks_for(g.nr, TPB)), dim3(TPB), 0, ctx->stream, x64,
                        d32, g);
     KAPSEL_LAUNCH_CHECK();

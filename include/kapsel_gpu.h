@@ -245,11 +245,14 @@ int kapsel_poisson_galerkin_start(kapsel_ctx *ctx, const double e0[3], double *g
 
 /* ---- kernel timing (HIP events on the context stream) --------------------- */
 enum kapsel_kernel_class {
-  KAPSEL_K_FFT_Z = 0,   /* z-direction r2c/c2r pass */
-  KAPSEL_K_FFT_Y,       /* y-direction c2c pass */
-  KAPSEL_K_FFT_X,       /* x-direction c2c pass */
-  KAPSEL_K_POINTWISE,   /* k-space and real-space pointwise kernels */
-  KAPSEL_K_PARTICLE,    /* scatter, gather, LJ, particle update */
+  KAPSEL_K_FFT_Z_R2C = 0, /* z-direction r2c pass (one kernel) */
+  KAPSEL_K_FFT_Z_C2R,     /* z-direction c2r pass (one kernel) */
+  KAPSEL_K_FFT_LINES_FWD, /* forward c2c line pass, y and x (one kernel on a cubic grid) */
+  KAPSEL_K_FFT_LINES_INV, /* inverse c2c line pass, y and x (one kernel on a cubic grid) */
+  KAPSEL_K_POINTWISE,     /* k-space and real-space pointwise kernels */
+  KAPSEL_K_PARTICLE,      /* scatter, gather, LJ, particle update */
+  KAPSEL_K_NETWORK,       /* warm-start network kernels */
+  KAPSEL_K_REFINE64,      /* FP64 work of an FP32 context's iterative refinement */
   KAPSEL_K_NCLASSES
 };
 /* Records a HIP event pair around every launch of the kernel classes whose

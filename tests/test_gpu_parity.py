@@ -48,7 +48,8 @@ def test_fft_matches_reference_golden(path, prec, tol):
     assert rel(k.get_field("U1")[:, :, :nz], g["real_out"]) < tol
 
 
-@pytest.mark.parametrize("shape", [(16, 16, 16), (64, 64, 64), (128, 128, 128), (32, 64, 256), (256, 16, 32)])
+@pytest.mark.parametrize("shape", [(16, 16, 16), (32, 32, 32), (64, 64, 64), (128, 128, 128), (32, 64, 256),
+                                   (256, 16, 32)])
 def test_fft_matches_oracle(shape):
     print(f"case fft oracle {shape}", flush=True)
     nx, ny, nz = shape
@@ -68,6 +69,38 @@ def test_fft_matches_oracle(shape):
         x.fft_c2r("U1")
     assert rel(k.get_field("U1")[:, :, :nz], o.get_field("U1")[:, :, :nz]) < 1e-13
     assert np.abs(k.get_field("U1")[:, :, :nz] - real).max() < 1e-12
+
+
+@pytest.mark.parametrize("shape", [(32, 32, 32), (64, 64, 64), (128, 128, 128), (64, 32, 32)])
+def test_fft_fp32_matches_oracle(shape):
+    """FP32 transforms (the fused z+y plane pass where a plane fits in LDS:
+    NY = NZ = 32, 64, 128) against the FP64 oracle, relative 3e-6."""
+    print(f"case fft fp32 oracle {shape}", flush=True)
+    nx, ny, nz = shape
+    rng = np.random.default_rng(7 + sum(shape))
+    real = rng.standard_normal(shape)
+    k = gpu(abi.make_params(shape, precision=abi.KAPSEL_FP32))
+    o = oracle(abi.make_params(shape))
+    for x in (k, o):
+        x.set_field("U0", sp.pad_real(real))
+        x.fft_r2c("U0")
+    spec_o = o.get_field("U0")
+    assert rel(k.get_field("U0"), spec_o) < 3e-6
+    for x in (k, o):
+        x.set_field("U1", spec_o)
+        x.fft_c2r("U1")
+    assert rel(k.get_field("U1")[:, :, :nz], o.get_field("U1")[:, :, :nz]) < 3e-6
+    # three fields in one launch
+    fields = ["U0", "U1", "U2"]
+    reals = [rng.standard_normal(shape) for _ in fields]
+    for f, r in zip(fields, reals):
+        k.set_field(f, sp.pad_real(r))
+    k.solenoidal_u("U0")  # r2c of 3 fields, projection, c2r of 3 fields
+    for f, r in zip(fields, reals):
+        o.set_field(f, sp.pad_real(r))
+    o.solenoidal_u("U0")
+    for f in fields:
+        assert rel(k.get_field(f)[:, :, :nz], o.get_field(f)[:, :, :nz]) < 3e-6, f
 
 
 @pytest.mark.parametrize("prec,tol", [(abi.KAPSEL_FP64, 1e-13), (abi.KAPSEL_FP32, 3e-6)])
