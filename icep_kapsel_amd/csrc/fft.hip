@@ -311,8 +311,56 @@ struct PlaneCfg {
   static constexpr int YR1 = FS<NY>::R1, YR2 = FS<NY>::R2;
   static constexpr int YCOLS = THREADS / YR2;  // columns per round
   static constexpr int PLANE = NY * H;
-  static constexpr size_t LDS_BYTES = sizeof(V2<T>) * PLANE;
+  static constexpr size_t LDS_BYTES = sizeof(V2<T>) * (PLANE + NZ + 1 + NY);
 };
+
+// Plane <-> LDS copies with 16-byte accesses, every load of a thread issued
+// before the first LDS store (the copy is latency-bound otherwise)
+template <typename T, int PLANE, int THREADS>
+__device__ __forceinline__ void plane_load(const T *__restrict__ gp, V2<T> *P) {
+  constexpr int NV = PLANE * (int)sizeof(V2<T>) / 16;
+  static_assert(PLANE * sizeof(V2<T>) % 16 == 0, "plane size");
+  constexpr int IT = (NV + THREADS - 1) / THREADS;
+  const int4 *src = reinterpret_cast<const int4 *>(gp);
+  int4 *dst = reinterpret_cast<int4 *>(P);
+  int4 r[IT];
+#pragma unroll
+  for (int i = 0; i < IT; i++) {
+    const int idx = threadIdx.x + i * THREADS;
+    if (idx < NV) r[i] = src[idx];
+  }
+#pragma unroll
+  for (int i = 0; i < IT; i++) {
+    const int idx = threadIdx.x + i * THREADS;
+    if (idx < NV) dst[idx] = r[i];
+  }
+}
+
+template <typename T, int PLANE, int THREADS>
+__device__ __forceinline__ void plane_store(T *__restrict__ gp, const V2<T> *P) {
+  constexpr int NV = PLANE * (int)sizeof(V2<T>) / 16;
+  constexpr int IT = (NV + THREADS - 1) / THREADS;
+  const int4 *src = reinterpret_cast<const int4 *>(P);
+  int4 *dst = reinterpret_cast<int4 *>(gp);
+#pragma unroll
+  for (int i = 0; i < IT; i++) {
+    const int idx = threadIdx.x + i * THREADS;
+    if (idx < NV) dst[idx] = src[idx];
+  }
+}
+
+// the twiddle tables of a plane pass, copied to LDS
+template <typename T, int NY, int NZ>
+struct PlaneTw {
+  V2<T> twm[NZ / 2], zrot[NZ / 2 + 1], twy[NY];
+};
+template <typename T, int NY, int NZ, int THREADS>
+__device__ __forceinline__ void plane_tw_load(PlaneTw<T, NY, NZ> &tw, const V2<T> *__restrict__ twm,
+                                              const V2<T> *__restrict__ zrot, const V2<T> *__restrict__ twy) {
+  for (int e = threadIdx.x; e < NZ / 2; e += THREADS) tw.twm[e] = twm[e];
+  for (int e = threadIdx.x; e < NZ / 2 + 1; e += THREADS) tw.zrot[e] = zrot[e];
+  for (int e = threadIdx.x; e < NY; e += THREADS) tw.twy[e] = twy[e];
+}
 
 // y transform of every column of the plane in P (in place), direction DIR
 template <typename T, int NY, int NZ, int THREADS, int DIR>
@@ -363,13 +411,14 @@ __global__ __launch_bounds__(THREADS) void fft_plane_r2c(FieldPtrs<T> fp, const 
   using C = PlaneCfg<T, NY, NZ, THREADS>;
   constexpr int M = C::M, H = C::H, R1 = C::ZR1, R2 = C::ZR2;
   constexpr int KPT = M / R2 + 1;  // split outputs per thread (at most)
-  __shared__ V2<T> P[C::PLANE];
+  __shared__ __align__(16) V2<T> P[C::PLANE];
+  __shared__ PlaneTw<T, NY, NZ> tw;
   unsigned field;
   const unsigned plane = xcd_tile(field);
   T *__restrict__ gp = fp.p[field] + (size_t)plane * 2 * C::PLANE;
   const int t = threadIdx.x;
-#pragma unroll 4
-  for (int e = t; e < C::PLANE; e += THREADS) P[e] = ld2(gp + 2 * e);
+  plane_load<T, C::PLANE, THREADS>(gp, P);
+  plane_tw_load<T, NY, NZ, THREADS>(tw, twm, zrot, twy);
   __syncthreads();
   // z: r2c of each row -- a length-M complex FFT of the packed reals, then
   // X[k] = E + W^k O, E = (Z[k] + conj Z[M-k]) / 2, O = (Z[k] - conj Z[M-k]) / 2i
@@ -384,7 +433,7 @@ __global__ __launch_bounds__(THREADS) void fft_plane_r2c(FieldPtrs<T> fp, const 
     for (int n1 = 0; n1 < R1; n1++) v[n1] = valid ? row[R2 * n1 + n2] : V2<T>{0, 0};
     dft<R1, 1, -1>(v, o);
 #pragma unroll
-    for (int k1 = 1; k1 < R1; k1++) o[k1] = cmul(o[k1], tw_get<-1>(twm, n2 * k1));
+    for (int k1 = 1; k1 < R1; k1++) o[k1] = cmul(o[k1], tw_get<-1>(tw.twm, n2 * k1));
     __syncthreads();
     if (valid) {
 #pragma unroll
@@ -422,15 +471,14 @@ __global__ __launch_bounds__(THREADS) void fft_plane_r2c(FieldPtrs<T> fp, const 
           const V2<T> e = scal((T)0.5, zk[q] + zm[q]);
           const V2<T> d = zk[q] - zm[q];
           const V2<T> od = {(T)0.5 * d.im, (T)-0.5 * d.re};
-          row[k] = e + cmul(zrot[k], od);
+          row[k] = e + cmul(tw.zrot[k], od);
         }
       }
     }
     __syncthreads();
   }
-  plane_columns<T, NY, NZ, THREADS, -1>(P, twy);
-#pragma unroll 4
-  for (int e = t; e < C::PLANE; e += THREADS) st2(gp + 2 * e, P[e]);
+  plane_columns<T, NY, NZ, THREADS, -1>(P, tw.twy);
+  plane_store<T, C::PLANE, THREADS>(gp, P);
 }
 
 template <typename T, int NY, int NZ, int THREADS>
@@ -439,15 +487,16 @@ __global__ __launch_bounds__(THREADS) void fft_plane_c2r(FieldPtrs<T> fp, const 
                                                          const V2<T> *__restrict__ twy, T scale) {
   using C = PlaneCfg<T, NY, NZ, THREADS>;
   constexpr int M = C::M, H = C::H, R1 = C::ZR1, R2 = C::ZR2;
-  __shared__ V2<T> P[C::PLANE];
+  __shared__ __align__(16) V2<T> P[C::PLANE];
+  __shared__ PlaneTw<T, NY, NZ> tw;
   unsigned field;
   const unsigned plane = xcd_tile(field);
   T *__restrict__ gp = fp.p[field] + (size_t)plane * 2 * C::PLANE;
   const int t = threadIdx.x;
-#pragma unroll 4
-  for (int e = t; e < C::PLANE; e += THREADS) P[e] = ld2(gp + 2 * e);
+  plane_load<T, C::PLANE, THREADS>(gp, P);
+  plane_tw_load<T, NY, NZ, THREADS>(tw, twm, zrot, twy);
   __syncthreads();
-  plane_columns<T, NY, NZ, THREADS, +1>(P, twy);
+  plane_columns<T, NY, NZ, THREADS, +1>(P, tw.twy);
   // z: c2r of each row (the kz = 0 and kz = NZ/2 entries are taken as real)
   const int rl = t % C::ZROWS, n2 = t / C::ZROWS;
   for (int r0 = 0; r0 < NY; r0 += C::ZROWS) {
@@ -466,12 +515,12 @@ __global__ __launch_bounds__(THREADS) void fft_plane_c2r(FieldPtrs<T> fp, const 
       }
       xm = conjv(xm);
       const V2<T> e = xn + xm;                         // 2 E[n]
-      const V2<T> od = cmul(xn - xm, conjv(zrot[n]));  // 2 O[n]
+      const V2<T> od = cmul(xn - xm, conjv(tw.zrot[n]));  // 2 O[n]
       v[n1] = V2<T>{e.re - od.im, e.im + od.re};       // E + i O
     }
     dft<R1, 1, +1>(v, o);
 #pragma unroll
-    for (int k1 = 1; k1 < R1; k1++) o[k1] = cmul(o[k1], tw_get<+1>(twm, n2 * k1));
+    for (int k1 = 1; k1 < R1; k1++) o[k1] = cmul(o[k1], tw_get<+1>(tw.twm, n2 * k1));
     __syncthreads();  // every row of the round is in registers
     if (valid) {
 #pragma unroll
@@ -493,8 +542,7 @@ __global__ __launch_bounds__(THREADS) void fft_plane_c2r(FieldPtrs<T> fp, const 
     }
     __syncthreads();
   }
-#pragma unroll 4
-  for (int e = t; e < C::PLANE; e += THREADS) st2(gp + 2 * e, P[e]);
+  plane_store<T, C::PLANE, THREADS>(gp, P);
 }
 
 // ------------------------------------------------------------------- host
