@@ -34,7 +34,7 @@ PRECOND_NONE = 0
 PRECOND_SPECTRAL = 1
 
 KERNEL_CLASSES = ["FFT_Z_R2C", "FFT_Z_C2R", "FFT_LINES_FWD", "FFT_LINES_INV", "POINTWISE", "PARTICLE",
-                  "NETWORK", "REFINE64"]
+                  "NETWORK", "REFINE64", "FFT_PLANE_FACE"]
 KCLASS = {name: i for i, name in enumerate(KERNEL_CLASSES)}
 
 

@@ -136,7 +136,7 @@ struct kapsel_ctx {
   // Poisson solver storage, allocated on first use: operator scratch of the
   // context precision, and FP64 shadow fields for the iterative refinement
   // of an FP32 context
-  void *pscratch[5] = {};          // [0..3] operator scratch, [4] refinement correction
+  void *pscratch[8] = {};          // [0..3] operator scratch, [4] refinement correction, [5..7] face eps
   double *p64[8] = {};            // eps, x, b, r, 4 operator scratch
   // FP32 context: p64[1] holds the FP64 potential of the last solve; it seeds
   // the next solve unless the FP32 POTENTIAL field was written since.
@@ -153,7 +153,7 @@ struct kapsel_ctx {
   // one k-space PCG iteration captured as a HIP graph, replayed while the
   // buffers it was captured with (kcg_key) stay the same
   hipGraphExec_t kcg_exec = nullptr;
-  const void *kcg_key[10] = {};
+  const void *kcg_key[11] = {};
   int n_partial = 0;
   kapsel::Pnet *pnet = nullptr;   // learned warm-start network, on first use
   // derived physics constants (Set_global_parameters, input.cxx:287-430)
@@ -196,6 +196,11 @@ template <typename T>
 void fft_r2c_t(kapsel_ctx *ctx, void *const *fields, int nf);
 template <typename T>
 void fft_c2r_t(kapsel_ctx *ctx, void *const *fields, int nf);
+// g_d <- r2c(face_d * c2r(g_d)) for the three staggered-gradient spectra,
+// the z and y transforms and the product fused in one plane pass; false
+// (nothing done) where the grid has no plane pass
+template <typename T>
+bool fft_face_t(kapsel_ctx *ctx, void *const *g3, void *const *face3);
 
 // k-space and real-space pointwise (kspace.hip)
 void truncate(kapsel_ctx *ctx, void *const *fields, int nf);

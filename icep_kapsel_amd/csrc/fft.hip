@@ -326,8 +326,10 @@ __device__ __forceinline__ void plane_load(const T *__restrict__ gp, V2<T> *P) {
   int4 r[IT];
 #pragma unroll
   for (int i = 0; i < IT; i++) {
+    // unconditional (clamped) loads: a conditionally assigned array is
+    // placed in scratch memory and its loads serialise
     const int idx = threadIdx.x + i * THREADS;
-    if (idx < NV) r[i] = src[idx];
+    r[i] = src[idx < NV ? idx : NV - 1];
   }
 #pragma unroll
   for (int i = 0; i < IT; i++) {
@@ -404,25 +406,15 @@ __device__ __forceinline__ void plane_columns(V2<T> *P, const V2<T> *__restrict_
   }
 }
 
+// z: r2c of every row of the plane in P (in place) -- a length-M complex FFT
+// of the packed reals, then X[k] = E + W^k O with E = (Z[k] + conj Z[M-k]) / 2,
+// O = (Z[k] - conj Z[M-k]) / 2i.  Consecutive lanes take consecutive rows.
 template <typename T, int NY, int NZ, int THREADS>
-__global__ __launch_bounds__(THREADS) void fft_plane_r2c(FieldPtrs<T> fp, const V2<T> *__restrict__ twm,
-                                                         const V2<T> *__restrict__ zrot,
-                                                         const V2<T> *__restrict__ twy) {
+__device__ __forceinline__ void plane_rows_r2c(V2<T> *P, const PlaneTw<T, NY, NZ> &tw) {
   using C = PlaneCfg<T, NY, NZ, THREADS>;
   constexpr int M = C::M, H = C::H, R1 = C::ZR1, R2 = C::ZR2;
   constexpr int KPT = M / R2 + 1;  // split outputs per thread (at most)
-  __shared__ __align__(16) V2<T> P[C::PLANE];
-  __shared__ PlaneTw<T, NY, NZ> tw;
-  unsigned field;
-  const unsigned plane = xcd_tile(field);
-  T *__restrict__ gp = fp.p[field] + (size_t)plane * 2 * C::PLANE;
   const int t = threadIdx.x;
-  plane_load<T, C::PLANE, THREADS>(gp, P);
-  plane_tw_load<T, NY, NZ, THREADS>(tw, twm, zrot, twy);
-  __syncthreads();
-  // z: r2c of each row -- a length-M complex FFT of the packed reals, then
-  // X[k] = E + W^k O, E = (Z[k] + conj Z[M-k]) / 2, O = (Z[k] - conj Z[M-k]) / 2i
-  // (consecutive lanes take consecutive rows)
   const int rl = t % C::ZROWS, n2 = t / C::ZROWS;
   for (int r0 = 0; r0 < NY; r0 += C::ZROWS) {
     const int j = r0 + rl;
@@ -477,27 +469,15 @@ __global__ __launch_bounds__(THREADS) void fft_plane_r2c(FieldPtrs<T> fp, const 
     }
     __syncthreads();
   }
-  plane_columns<T, NY, NZ, THREADS, -1>(P, tw.twy);
-  plane_store<T, C::PLANE, THREADS>(gp, P);
 }
 
+// z: c2r of every row of the plane in P (in place), scaled; the kz = 0 and
+// kz = NZ/2 entries are taken as real and the row padding is zeroed
 template <typename T, int NY, int NZ, int THREADS>
-__global__ __launch_bounds__(THREADS) void fft_plane_c2r(FieldPtrs<T> fp, const V2<T> *__restrict__ twm,
-                                                         const V2<T> *__restrict__ zrot,
-                                                         const V2<T> *__restrict__ twy, T scale) {
+__device__ __forceinline__ void plane_rows_c2r(V2<T> *P, const PlaneTw<T, NY, NZ> &tw, T scale) {
   using C = PlaneCfg<T, NY, NZ, THREADS>;
   constexpr int M = C::M, H = C::H, R1 = C::ZR1, R2 = C::ZR2;
-  __shared__ __align__(16) V2<T> P[C::PLANE];
-  __shared__ PlaneTw<T, NY, NZ> tw;
-  unsigned field;
-  const unsigned plane = xcd_tile(field);
-  T *__restrict__ gp = fp.p[field] + (size_t)plane * 2 * C::PLANE;
   const int t = threadIdx.x;
-  plane_load<T, C::PLANE, THREADS>(gp, P);
-  plane_tw_load<T, NY, NZ, THREADS>(tw, twm, zrot, twy);
-  __syncthreads();
-  plane_columns<T, NY, NZ, THREADS, +1>(P, tw.twy);
-  // z: c2r of each row (the kz = 0 and kz = NZ/2 entries are taken as real)
   const int rl = t % C::ZROWS, n2 = t / C::ZROWS;
   for (int r0 = 0; r0 < NY; r0 += C::ZROWS) {
     const int j = r0 + rl;
@@ -514,9 +494,9 @@ __global__ __launch_bounds__(THREADS) void fft_plane_c2r(FieldPtrs<T> fp, const 
         xm.im = 0;  // M - n = M
       }
       xm = conjv(xm);
-      const V2<T> e = xn + xm;                         // 2 E[n]
+      const V2<T> e = xn + xm;                            // 2 E[n]
       const V2<T> od = cmul(xn - xm, conjv(tw.zrot[n]));  // 2 O[n]
-      v[n1] = V2<T>{e.re - od.im, e.im + od.re};       // E + i O
+      v[n1] = V2<T>{e.re - od.im, e.im + od.re};          // E + i O
     }
     dft<R1, 1, +1>(v, o);
 #pragma unroll
@@ -542,6 +522,89 @@ __global__ __launch_bounds__(THREADS) void fft_plane_c2r(FieldPtrs<T> fp, const 
     }
     __syncthreads();
   }
+}
+
+template <typename T, int NY, int NZ, int THREADS>
+__global__ __launch_bounds__(THREADS) void fft_plane_r2c(FieldPtrs<T> fp, const V2<T> *__restrict__ twm,
+                                                         const V2<T> *__restrict__ zrot,
+                                                         const V2<T> *__restrict__ twy) {
+  using C = PlaneCfg<T, NY, NZ, THREADS>;
+  __shared__ __align__(16) V2<T> P[C::PLANE];
+  __shared__ PlaneTw<T, NY, NZ> tw;
+  unsigned field;
+  const unsigned plane = xcd_tile(field);
+  T *__restrict__ gp = fp.p[field] + (size_t)plane * 2 * C::PLANE;
+  plane_load<T, C::PLANE, THREADS>(gp, P);
+  plane_tw_load<T, NY, NZ, THREADS>(tw, twm, zrot, twy);
+  __syncthreads();
+  plane_rows_r2c<T, NY, NZ, THREADS>(P, tw);
+  plane_columns<T, NY, NZ, THREADS, -1>(P, tw.twy);
+  plane_store<T, C::PLANE, THREADS>(gp, P);
+}
+
+template <typename T, int NY, int NZ, int THREADS>
+__global__ __launch_bounds__(THREADS) void fft_plane_c2r(FieldPtrs<T> fp, const V2<T> *__restrict__ twm,
+                                                         const V2<T> *__restrict__ zrot,
+                                                         const V2<T> *__restrict__ twy, T scale) {
+  using C = PlaneCfg<T, NY, NZ, THREADS>;
+  __shared__ __align__(16) V2<T> P[C::PLANE];
+  __shared__ PlaneTw<T, NY, NZ> tw;
+  unsigned field;
+  const unsigned plane = xcd_tile(field);
+  T *__restrict__ gp = fp.p[field] + (size_t)plane * 2 * C::PLANE;
+  plane_load<T, C::PLANE, THREADS>(gp, P);
+  plane_tw_load<T, NY, NZ, THREADS>(tw, twm, zrot, twy);
+  __syncthreads();
+  plane_columns<T, NY, NZ, THREADS, +1>(P, tw.twy);
+  plane_rows_c2r<T, NY, NZ, THREADS>(P, tw, scale);
+  plane_store<T, C::PLANE, THREADS>(gp, P);
+}
+
+// The middle of the dielectric operator on one plane of the staggered
+// gradient field d (fields 0, 1, 2 = x, y, z): inverse z and y transforms,
+// times the face permittivity (a real field of the same layout), forward z
+// and y transforms -- one read and one write where two plane passes and a
+// pointwise pass would make three of each.
+template <typename T, int NY, int NZ, int THREADS>
+__global__ __launch_bounds__(THREADS) void fft_plane_face(FieldPtrs<T> fp, const V2<T> *__restrict__ twm,
+                                                          const V2<T> *__restrict__ zrot,
+                                                          const V2<T> *__restrict__ twy, T scale,
+                                                          FieldPtrs<T> face) {
+  using C = PlaneCfg<T, NY, NZ, THREADS>;
+  __shared__ __align__(16) V2<T> P[C::PLANE];
+  __shared__ PlaneTw<T, NY, NZ> tw;
+  unsigned field;
+  const unsigned plane = xcd_tile(field);
+  T *__restrict__ gp = fp.p[field] + (size_t)plane * 2 * C::PLANE;
+  plane_load<T, C::PLANE, THREADS>(gp, P);
+  plane_tw_load<T, NY, NZ, THREADS>(tw, twm, zrot, twy);
+  __syncthreads();
+  plane_columns<T, NY, NZ, THREADS, +1>(P, tw.twy);
+  // the face permittivity of this plane, in flight during the row transforms
+  constexpr int NV = C::PLANE * (int)sizeof(V2<T>) / 16;
+  constexpr int IT = (NV + THREADS - 1) / THREADS;
+  const int4 *fsrc = reinterpret_cast<const int4 *>(face.p[field] + (size_t)plane * 2 * C::PLANE);
+  int4 fr[IT];
+#pragma unroll
+  for (int i = 0; i < IT; i++) {
+    const int idx = threadIdx.x + i * THREADS;
+    fr[i] = fsrc[idx < NV ? idx : NV - 1];
+  }
+  plane_rows_c2r<T, NY, NZ, THREADS>(P, tw, scale);
+  constexpr int PER = 16 / (int)sizeof(T);
+#pragma unroll
+  for (int i = 0; i < IT; i++) {
+    const int idx = threadIdx.x + i * THREADS;
+    if (idx < NV) {
+      T *pv = reinterpret_cast<T *>(P) + (size_t)idx * PER;
+      const T *fv = reinterpret_cast<const T *>(&fr[i]);
+#pragma unroll
+      for (int e = 0; e < PER; e++) pv[e] *= fv[e];
+    }
+  }
+  __syncthreads();
+  plane_rows_r2c<T, NY, NZ, THREADS>(P, tw);
+  plane_columns<T, NY, NZ, THREADS, -1>(P, tw.twy);
   plane_store<T, C::PLANE, THREADS>(gp, P);
 }
 
@@ -688,6 +751,18 @@ bool plane_dispatch(kapsel_ctx *ctx, const FieldPtrs<T> &fp, int nf, bool fwd) {
   }
 }
 
+template <typename T, int N, int THREADS>
+void launch_plane_face(kapsel_ctx *ctx, const FieldPtrs<T> &fp, const FieldPtrs<T> &face) {
+  const Grid &g = ctx->g;
+  dim3 grid((unsigned)g.nx, 3);
+  LaunchTimer lt(ctx, KAPSEL_K_FFT_PLANE_FACE, 2.0 * 3 * g.f * sizeof(T) + 3.0 * g.f * sizeof(T));
+  const Twiddles &tw = ctx->tw[prec_of<T>()];
+  T scale = (T)(1.0 / ((double)g.nx * g.ny * g.nz));
+  hipLaunchKernelGGL((fft_plane_face<T, N, N, THREADS>), grid, dim3(THREADS), 0, ctx->stream, fp,
+                     (const V2<T> *)tw.zh, (const V2<T> *)tw.zrot, (const V2<T> *)tw.y, scale, face);
+  KAPSEL_LAUNCH_CHECK();
+}
+
 template <typename T>
 FieldPtrs<T> pack(void *const *fields, int nf) {
   FieldPtrs<T> fp{};
@@ -727,6 +802,34 @@ void fft_c2r_t(kapsel_ctx *ctx, void *const *fields, int nf) {
   }
 }
 
+template <typename T>
+bool fft_face_t(kapsel_ctx *ctx, void *const *g3, void *const *face3) {
+  const Grid &g = ctx->g;
+  if (g.ny != g.nz) return false;
+  const Twiddles &tw = ctx->tw[prec_of<T>()];
+  FieldPtrs<T> fp = pack<T>(g3, 3), fc = pack<T>(face3, 3);
+  const long long plane = (long long)g.ny * g.h;
+  auto xpass = [&](bool inv) {
+    if (inv) lines_dispatch<T, +1>(ctx, KAPSEL_K_FFT_LINES_INV, g.nx, fp, 3, plane, plane, 0, plane, tw.x);
+    else lines_dispatch<T, -1>(ctx, KAPSEL_K_FFT_LINES_FWD, g.nx, fp, 3, plane, plane, 0, plane, tw.x);
+  };
+  switch (g.nz) {
+    case 32: xpass(true); launch_plane_face<T, 32, 128>(ctx, fp, fc); xpass(false); return true;
+    case 64: xpass(true); launch_plane_face<T, 64, 320>(ctx, fp, fc); xpass(false); return true;
+    case 128:
+      if constexpr (sizeof(T) == 4) {
+        xpass(true);
+        launch_plane_face<T, 128, 576>(ctx, fp, fc);
+        xpass(false);
+        return true;
+      }
+      return false;
+    default: return false;
+  }
+}
+
+template bool fft_face_t<double>(kapsel_ctx *, void *const *, void *const *);
+template bool fft_face_t<float>(kapsel_ctx *, void *const *, void *const *);
 template void fft_r2c_t<double>(kapsel_ctx *, void *const *, int);
 template void fft_r2c_t<float>(kapsel_ctx *, void *const *, int);
 template void fft_c2r_t<double>(kapsel_ctx *, void *const *, int);

@@ -168,6 +168,26 @@ __global__ void face_mul_kernel(T *__restrict__ g0, T *__restrict__ g1, T *__res
   g2[p.im] *= (T)0.5 * (eps[nbr(g, p.i, p.j, p.k + 1)] + en);
 }
 
+// f_d = 0.5 (eps_i + eps_{i+e_d}) at the real points, 0 in the row padding:
+// the factors face_mul_kernel applies, kept for the fused plane pass
+template <typename T>
+__global__ void face_fields_kernel(const T *__restrict__ eps, T *__restrict__ f0, T *__restrict__ f1,
+                                   T *__restrict__ f2, Grid g) {
+  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= g.f) return;
+  const int k = (int)(e % (unsigned)g.nz_);
+  if (k >= g.nz) {
+    f0[e] = f1[e] = f2[e] = (T)0;
+    return;
+  }
+  const size_t row = e / (unsigned)g.nz_;
+  const int j = (int)(row % (unsigned)g.ny), i = (int)(row / (unsigned)g.ny);
+  const T en = eps[e];
+  f0[e] = (T)0.5 * (eps[nbr(g, i + 1, j, k)] + en);
+  f1[e] = (T)0.5 * (eps[nbr(g, i, j + 1, k)] + en);
+  f2[e] = (T)0.5 * (eps[nbr(g, i, j, k + 1)] + en);
+}
+
 // out = i sum_d k_d conj(shift_d) w_d
 template <typename T>
 __global__ void div_shift_kernel(const T *__restrict__ w0, const T *__restrict__ w1, const T *__restrict__ w2,
@@ -731,24 +751,28 @@ int cg_t(kapsel_ctx *ctx, T *x, const T *b, const T *eps, double tol_abs, int ma
 // beta.  Every scalar lives in device memory, so the sequence can be
 // captured once and replayed.
 template <typename T>
-void kcg_iteration(kapsel_ctx *ctx, T *x, const T *eps, int precond, T *const *w, T *const *s) {
+void kcg_iteration(kapsel_ctx *ctx, T *x, const T *eps, int precond, T *const *w, T *const *s,
+                   T *const *face) {
   const Grid &g = ctx->g;
   T *r = w[0], *p = w[1], *ap = w[2], *z = precond ? w[3] : w[0];
   void *s3[3] = {s[0], s[1], s[2]};
+  void *f3[3] = {face[0], face[1], face[2]};
   {
     LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 6, sizeof(T)));
     hipLaunchKernelGGL(kcg_dir_grad_kernel<T>, dim3(blocks_for(g.nc, TPB)), dim3(TPB), 0, ctx->stream, z, p,
                        ctx->d_scal, 0, s[0], s[1], s[2], sh<T>(ctx, 0), sh<T>(ctx, 1), sh<T>(ctx, 2), g);
     KAPSEL_LAUNCH_CHECK();
   }
-  fft_c2r_t<T>(ctx, s3, 3);
-  {
-    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 7, sizeof(T)));
-    hipLaunchKernelGGL(face_mul_kernel<T>, dim3(blocks_for(g.nr, TPB)), dim3(TPB), 0, ctx->stream, s[0], s[1],
-                       s[2], eps, g);
-    KAPSEL_LAUNCH_CHECK();
+  if (!fft_face_t<T>(ctx, s3, f3)) {
+    fft_c2r_t<T>(ctx, s3, 3);
+    {
+      LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 7, sizeof(T)));
+      hipLaunchKernelGGL(face_mul_kernel<T>, dim3(blocks_for(g.nr, TPB)), dim3(TPB), 0, ctx->stream, s[0], s[1],
+                         s[2], eps, g);
+      KAPSEL_LAUNCH_CHECK();
+    }
+    fft_r2c_t<T>(ctx, s3, 3);
   }
-  fft_r2c_t<T>(ctx, s3, 3);
   {
     LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 5, sizeof(T)));
     hipLaunchKernelGGL(kcg_div_dot_kernel<T>, dim3(NPART), dim3(TPB), 0, ctx->stream, s[0], s[1], s[2], p, ap,
@@ -771,14 +795,15 @@ void kcg_iteration(kapsel_ctx *ctx, T *x, const T *eps, int precond, T *const *w
 // Queues one iteration: replays the captured graph when the buffers match
 // the capture (and no kernel timing is on), else captures it anew.
 template <typename T>
-void kcg_enqueue(kapsel_ctx *ctx, T *x, const T *eps, int precond, T *const *w, T *const *s) {
+void kcg_enqueue(kapsel_ctx *ctx, T *x, const T *eps, int precond, T *const *w, T *const *s, T *const *face) {
   if (ctx->timing_mask) {
-    kcg_iteration<T>(ctx, x, eps, precond, w, s);
+    kcg_iteration<T>(ctx, x, eps, precond, w, s, face);
     return;
   }
-  const void *key[10] = {x, eps, w[0], w[1], w[2], w[3], s[0], s[1], s[2], (const void *)(intptr_t)(precond + 1)};
+  const void *key[11] = {x,    eps,  w[0], w[1], w[2], w[3], s[0], s[1], s[2], (const void *)(intptr_t)(precond + 1),
+                         face[0]};
   bool same = ctx->kcg_exec != nullptr;
-  for (int i = 0; i < 10 && same; i++) same = key[i] == ctx->kcg_key[i];
+  for (int i = 0; i < 11 && same; i++) same = key[i] == ctx->kcg_key[i];
   if (!same) {
     if (ctx->kcg_exec) {
       hipGraphExec_t old = ctx->kcg_exec;
@@ -788,7 +813,7 @@ void kcg_enqueue(kapsel_ctx *ctx, T *x, const T *eps, int precond, T *const *w, 
     hipGraph_t graph = nullptr;
     KAPSEL_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     try {
-      kcg_iteration<T>(ctx, x, eps, precond, w, s);
+      kcg_iteration<T>(ctx, x, eps, precond, w, s, face);
     } catch (...) {
       (void)hipStreamEndCapture(ctx->stream, &graph);
       if (graph) (void)hipGraphDestroy(graph);
@@ -798,7 +823,7 @@ void kcg_enqueue(kapsel_ctx *ctx, T *x, const T *eps, int precond, T *const *w, 
     hipError_t e = hipGraphInstantiate(&ctx->kcg_exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     KAPSEL_HIP(e);
-    for (int i = 0; i < 10; i++) ctx->kcg_key[i] = key[i];
+    for (int i = 0; i < 11; i++) ctx->kcg_key[i] = key[i];
   }
   KAPSEL_HIP(hipGraphLaunch(ctx->kcg_exec, ctx->stream));
 }
@@ -810,7 +835,7 @@ void kcg_enqueue(kapsel_ctx *ctx, T *x, const T *eps, int precond, T *const *w, 
 // count and *rr_out = that ||A x - b||^2.
 template <typename T>
 int kcg_t(kapsel_ctx *ctx, T *x, const T *b, const T *eps, double tol_abs, int maxit, int precond, T *const *w,
-          T *const *s, double *rr_out) {
+          T *const *s, T *const *face, double *rr_out) {
   const Grid &g = ctx->g;
   T *r = w[0], *p = w[1], *z = precond ? w[3] : w[0];
   const double norm = 1.0 / (double)g.nr;  // Parseval factor of the k-space sums
@@ -825,7 +850,7 @@ int kcg_t(kapsel_ctx *ctx, T *x, const T *b, const T *eps, double tol_abs, int m
   double rr = read_scal(ctx, S_RR) * norm;
   const double tol2 = tol_abs * tol_abs;
   ConvergenceWatch cw(ctx, norm, rr);
-  cw.run(tol2, maxit, [&]() { kcg_enqueue<T>(ctx, x, eps, precond, w, s); });
+  cw.run(tol2, maxit, [&]() { kcg_enqueue<T>(ctx, x, eps, precond, w, s, face); });
   rr = cw.rr;
   const int it = cw.it;
   *rr_out = rr;
@@ -948,6 +973,10 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
     return 0;
   }
   if (precond) inv_eps_device<float>(ctx, eps32, w32[3]);
+  float *face32[3] = {(float *)ctx->pscratch[5], (float *)ctx->pscratch[6], (float *)ctx->pscratch[7]};
+  hipLaunchKernelGGL(face_fields_kernel<float>, dim3(blocks_for(g.f, TPB)), dim3(TPB), 0, ctx->stream, eps32,
+                     face32[0], face32[1], face32[2], g);
+  KAPSEL_LAUNCH_CHECK();
   const double bnorm = sqrt(bb);
   int total = 0;
   double rel = 1.0;
@@ -974,7 +1003,7 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
     fft_r2c_t<float>(ctx, bk, 1);
     double rr_in = 0.0;
     const double tol_in = fmax(1e-4 * sqrt(rr), 0.5 * tol * bnorm);
-    int it = kcg_t<float>(ctx, d32, b32, eps32, tol_in, maxit - total, precond, w32, s32 + 1, &rr_in);
+    int it = kcg_t<float>(ctx, d32, b32, eps32, tol_in, maxit - total, precond, w32, s32 + 1, face32, &rr_in);
     total += it;
     void *dk[1] = {d32};
     fft_c2r_t<float>(ctx, dk, 1);
