@@ -137,7 +137,7 @@ struct kapsel_ctx {
   // context precision, and FP64 shadow fields for the iterative refinement
   // of an FP32 context
   void *pscratch[8] = {};          // [0..3] operator scratch, [4] refinement correction, [5..7] face eps
-  double *p64[8] = {};            // eps, x, b, r, 4 operator scratch
+  double *p64[11] = {};           // eps, x (k), b, r (k), 4 scratch (b spectrum first), 3 face eps
   // FP32 context: p64[1] holds the FP64 potential of the last solve; it seeds
   // the next solve unless the FP32 POTENTIAL field was written since.
   // x64_prev holds the solve before it (when prev_valid), so the next start
@@ -196,11 +196,11 @@ template <typename T>
 void fft_r2c_t(kapsel_ctx *ctx, void *const *fields, int nf);
 template <typename T>
 void fft_c2r_t(kapsel_ctx *ctx, void *const *fields, int nf);
-// g_d <- r2c(face_d * c2r(g_d)) for the three staggered-gradient spectra,
-// the z and y transforms and the product fused in one plane pass; false
-// (nothing done) where the grid has no plane pass
+// g_d <- r2c(face_d * c2r(g_d)) for the three staggered-gradient spectra
+// (face_d real fields, 0 in the row padding): the inverse and forward z (and,
+// where a plane fits in LDS, y) transforms and the product fused in one pass
 template <typename T>
-bool fft_face_t(kapsel_ctx *ctx, void *const *g3, void *const *face3);
+void fft_face_t(kapsel_ctx *ctx, void *const *g3, void *const *face3);
 
 // k-space and real-space pointwise (kspace.hip)
 void truncate(kapsel_ctx *ctx, void *const *fields, int nf);

@@ -608,6 +608,116 @@ __global__ __launch_bounds__(THREADS) void fft_plane_face(FieldPtrs<T> fp, const
   plane_store<T, C::PLANE, THREADS>(gp, P);
 }
 
+// One z row of each of the three staggered-gradient spectra (after the x
+// and y inverse passes): c2r along z, times the face permittivity row, r2c
+// along z -- the middle of the dielectric operator where no plane pass
+// applies.  The row stays in LDS between the two transforms.
+template <typename T, int NZ, int RPB>
+__global__ __launch_bounds__(RPB *FS<NZ / 2>::R2) void fft_z_face(FieldPtrs<T> fp, long long rows, int nz_,
+                                                                   const V2<T> *__restrict__ twm,
+                                                                   const V2<T> *__restrict__ zrot, T scale,
+                                                                   FieldPtrs<T> face) {
+  constexpr int M = NZ / 2;
+  constexpr int R1 = FS<M>::R1, R2 = FS<M>::R2, TPL = R2;
+  constexpr int RS = R2 * (R1 + 1) + 1;
+  static_assert(RS >= M + 1, "row stride");
+  __shared__ V2<T> sm[RPB * RS];
+  unsigned field;
+  const unsigned tile = xcd_tile(field);
+  const int t = threadIdx.x;
+  const int n2 = t % TPL;
+  const int r = t / TPL;
+  const long long row = (long long)tile * RPB + r;
+  const bool valid = row < rows;
+  T *__restrict__ rp = fp.p[field] + (valid ? row : 0) * nz_;
+  const T *__restrict__ fr = face.p[field] + (valid ? row : 0) * nz_;
+  V2<T> *sr = sm + r * RS;
+
+  // c2r (the kz = 0 and kz = NZ/2 entries are taken as real)
+  for (int k = n2; k <= M; k += TPL) {
+    V2<T> x = valid ? ld2(rp + 2 * k) : V2<T>{0, 0};
+    if (k == 0 || k == M) x.im = 0;
+    sr[k] = x;
+  }
+  __syncthreads();
+  V2<T> v[R1], o[R1];
+#pragma unroll
+  for (int n1 = 0; n1 < R1; n1++) {
+    const int n = R2 * n1 + n2;
+    V2<T> xn = sr[n], xm = conjv(sr[M - n]);
+    V2<T> e = xn + xm;
+    V2<T> od = cmul(xn - xm, conjv(zrot[n]));
+    v[n1] = V2<T>{e.re - od.im, e.im + od.re};
+  }
+  dft<R1, 1, +1>(v, o);
+#pragma unroll
+  for (int k1 = 1; k1 < R1; k1++) o[k1] = cmul(o[k1], tw_get<+1>(twm, n2 * k1));
+  __syncthreads();
+#pragma unroll
+  for (int k1 = 0; k1 < R1; k1++) sr[n2 * (R1 + 1) + k1] = o[k1];
+  __syncthreads();
+  V2<T> zz[R1 / TPL][R2];
+#pragma unroll
+  for (int c = 0; c < R1 / TPL; c++) {
+    const int k1 = n2 + TPL * c;
+    V2<T> w[R2], x[R2];
+#pragma unroll
+    for (int m = 0; m < R2; m++) w[m] = sr[m * (R1 + 1) + k1];
+    dft<R2, 1, +1>(w, x);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; k2++) {
+      // reals 2n, 2n+1 of the row, times the face permittivity
+      const int n = k1 + R1 * k2;
+      const V2<T> sx = scal(scale, x[k2]);
+      zz[c][k2] = valid ? V2<T>{sx.re * fr[2 * n], sx.im * fr[2 * n + 1]} : V2<T>{0, 0};
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < R1 / TPL; c++) {
+    const int k1 = n2 + TPL * c;
+#pragma unroll
+    for (int k2 = 0; k2 < R2; k2++) sr[k1 + R1 * k2] = zz[c][k2];
+  }
+  __syncthreads();
+  // r2c of the product
+#pragma unroll
+  for (int n1 = 0; n1 < R1; n1++) v[n1] = sr[R2 * n1 + n2];
+  dft<R1, 1, -1>(v, o);
+#pragma unroll
+  for (int k1 = 1; k1 < R1; k1++) o[k1] = cmul(o[k1], tw_get<-1>(twm, n2 * k1));
+  __syncthreads();
+#pragma unroll
+  for (int k1 = 0; k1 < R1; k1++) sr[n2 * (R1 + 1) + k1] = o[k1];
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < R1 / TPL; c++) {
+    const int k1 = n2 + TPL * c;
+    V2<T> w[R2];
+#pragma unroll
+    for (int m = 0; m < R2; m++) w[m] = sr[m * (R1 + 1) + k1];
+    dft<R2, 1, -1>(w, zz[c]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < R1 / TPL; c++) {
+    const int k1 = n2 + TPL * c;
+#pragma unroll
+    for (int k2 = 0; k2 < R2; k2++) sr[k1 + R1 * k2] = zz[c][k2];
+  }
+  __syncthreads();
+  if (valid) {
+    for (int k = n2; k <= M; k += TPL) {
+      V2<T> zk = sr[k & (M - 1)];
+      V2<T> zm = conjv(sr[(M - k) & (M - 1)]);
+      V2<T> e = scal((T)0.5, zk + zm);
+      V2<T> d = zk - zm;
+      V2<T> od = {(T)0.5 * d.im, (T)-0.5 * d.re};
+      st2(rp + 2 * k, e + cmul(zrot[k], od));
+    }
+  }
+}
+
 // ------------------------------------------------------------------- host
 namespace {
 
@@ -731,8 +841,17 @@ void launch_plane(kapsel_ctx *ctx, const FieldPtrs<T> &fp, int nf, bool fwd) {
   KAPSEL_LAUNCH_CHECK();
 }
 
-// the z and y transforms as one plane pass (NY = NZ = 32, 64; 128 in FP32);
-// false: use the z and y line passes
+// whether the z and y transforms run as one plane pass: NY = NZ = 32, 64;
+// 128 in FP32 (an FP64 128 x 65 plane, 133 KB, leaves one workgroup per CU
+// and is slower than the separate z and y passes)
+template <typename T>
+bool plane_ok(const Grid &g) {
+  if (g.ny != g.nz) return false;
+  return g.nz == 32 || g.nz == 64 || (g.nz == 128 && sizeof(T) == 4);
+}
+
+// the z and y transforms as one plane pass where plane_ok; false: use the z
+// and y line passes
 template <typename T>
 bool plane_dispatch(kapsel_ctx *ctx, const FieldPtrs<T> &fp, int nf, bool fwd) {
   const Grid &g = ctx->g;
@@ -761,6 +880,32 @@ void launch_plane_face(kapsel_ctx *ctx, const FieldPtrs<T> &fp, const FieldPtrs<
   hipLaunchKernelGGL((fft_plane_face<T, N, N, THREADS>), grid, dim3(THREADS), 0, ctx->stream, fp,
                      (const V2<T> *)tw.zh, (const V2<T> *)tw.zrot, (const V2<T> *)tw.y, scale, face);
   KAPSEL_LAUNCH_CHECK();
+}
+
+template <typename T, int NZ>
+void launch_z_face(kapsel_ctx *ctx, const FieldPtrs<T> &fp, const FieldPtrs<T> &face) {
+  constexpr int RPB = rows_per_block(sizeof(T));
+  const Grid &g = ctx->g;
+  long long rows = (long long)g.nx * g.ny;
+  dim3 grid((unsigned)((rows + RPB - 1) / RPB), 3);
+  LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, 2.0 * 3 * g.f * sizeof(T) + 3.0 * g.f * sizeof(T));
+  const Twiddles &tw = ctx->tw[prec_of<T>()];
+  T scale = (T)(1.0 / ((double)g.nx * g.ny * g.nz));
+  hipLaunchKernelGGL((fft_z_face<T, NZ, RPB>), grid, dim3(RPB * FS<NZ / 2>::R2), 0, ctx->stream, fp, rows, g.nz_,
+                     (const V2<T> *)tw.zh, (const V2<T> *)tw.zrot, scale, face);
+  KAPSEL_LAUNCH_CHECK();
+}
+
+template <typename T>
+void z_face_dispatch(kapsel_ctx *ctx, const FieldPtrs<T> &fp, const FieldPtrs<T> &face) {
+  switch (ctx->g.nz) {
+    case 16: launch_z_face<T, 16>(ctx, fp, face); break;
+    case 32: launch_z_face<T, 32>(ctx, fp, face); break;
+    case 64: launch_z_face<T, 64>(ctx, fp, face); break;
+    case 128: launch_z_face<T, 128>(ctx, fp, face); break;
+    case 256: launch_z_face<T, 256>(ctx, fp, face); break;
+    default: set_error("unsupported NZ %d", ctx->g.nz); throw Error{KAPSEL_ERR_ARG};
+  }
 }
 
 template <typename T>
@@ -803,33 +948,38 @@ void fft_c2r_t(kapsel_ctx *ctx, void *const *fields, int nf) {
 }
 
 template <typename T>
-bool fft_face_t(kapsel_ctx *ctx, void *const *g3, void *const *face3) {
+void fft_face_t(kapsel_ctx *ctx, void *const *g3, void *const *face3) {
   const Grid &g = ctx->g;
-  if (g.ny != g.nz) return false;
   const Twiddles &tw = ctx->tw[prec_of<T>()];
   FieldPtrs<T> fp = pack<T>(g3, 3), fc = pack<T>(face3, 3);
   const long long plane = (long long)g.ny * g.h;
-  auto xpass = [&](bool inv) {
-    if (inv) lines_dispatch<T, +1>(ctx, KAPSEL_K_FFT_LINES_INV, g.nx, fp, 3, plane, plane, 0, plane, tw.x);
-    else lines_dispatch<T, -1>(ctx, KAPSEL_K_FFT_LINES_FWD, g.nx, fp, 3, plane, plane, 0, plane, tw.x);
-  };
-  switch (g.nz) {
-    case 32: xpass(true); launch_plane_face<T, 32, 128>(ctx, fp, fc); xpass(false); return true;
-    case 64: xpass(true); launch_plane_face<T, 64, 320>(ctx, fp, fc); xpass(false); return true;
-    case 128:
-      if constexpr (sizeof(T) == 4) {
-        xpass(true);
-        launch_plane_face<T, 128, 576>(ctx, fp, fc);
-        xpass(false);
-        return true;
-      }
-      return false;
-    default: return false;
+  lines_dispatch<T, +1>(ctx, KAPSEL_K_FFT_LINES_INV, g.nx, fp, 3, plane, plane, 0, plane, tw.x);
+  bool done = false;
+  if (plane_ok<T>(g)) {
+    switch (g.nz) {
+      case 32: launch_plane_face<T, 32, 128>(ctx, fp, fc); done = true; break;
+      case 64: launch_plane_face<T, 64, 320>(ctx, fp, fc); done = true; break;
+      case 128:
+        if constexpr (sizeof(T) == 4) {
+          launch_plane_face<T, 128, 576>(ctx, fp, fc);
+          done = true;
+        }
+        break;
+      default: break;
+    }
   }
+  if (!done) {
+    lines_dispatch<T, +1>(ctx, KAPSEL_K_FFT_LINES_INV, g.ny, fp, 3, (long long)g.nx * g.h, g.h,
+                          (long long)g.ny * g.h, g.h, tw.y);
+    z_face_dispatch<T>(ctx, fp, fc);
+    lines_dispatch<T, -1>(ctx, KAPSEL_K_FFT_LINES_FWD, g.ny, fp, 3, (long long)g.nx * g.h, g.h,
+                          (long long)g.ny * g.h, g.h, tw.y);
+  }
+  lines_dispatch<T, -1>(ctx, KAPSEL_K_FFT_LINES_FWD, g.nx, fp, 3, plane, plane, 0, plane, tw.x);
 }
 
-template bool fft_face_t<double>(kapsel_ctx *, void *const *, void *const *);
-template bool fft_face_t<float>(kapsel_ctx *, void *const *, void *const *);
+template void fft_face_t<double>(kapsel_ctx *, void *const *, void *const *);
+template void fft_face_t<float>(kapsel_ctx *, void *const *, void *const *);
 template void fft_r2c_t<double>(kapsel_ctx *, void *const *, int);
 template void fft_r2c_t<float>(kapsel_ctx *, void *const *, int);
 template void fft_c2r_t<double>(kapsel_ctx *, void *const *, int);

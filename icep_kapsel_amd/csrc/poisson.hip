@@ -486,6 +486,44 @@ __global__ __launch_bounds__(TPB) void kcg_update_kernel(T *__restrict__ x, T *r
   }
 }
 
+// r = i sum_d k_d e^{-i k_d dx/2} w_d - b: the k-space residual A x - b of
+// the refinement; partial[blk] = Parseval-weighted <r, r> (without N^-3)
+template <typename T>
+__global__ __launch_bounds__(TPB) void kres_kernel(const T *__restrict__ w0, const T *__restrict__ w1,
+                                                   const T *__restrict__ w2, const T *__restrict__ b,
+                                                   T *__restrict__ r, const V2<T> *__restrict__ shx,
+                                                   const V2<T> *__restrict__ shy, const V2<T> *__restrict__ shz,
+                                                   double *__restrict__ partial, Grid g) {
+  double s = 0.0;
+  for (size_t c = (size_t)blockIdx.x * TPB + threadIdx.x; c < g.nc; c += (size_t)gridDim.x * TPB) {
+    const KI k = kidx(g, c);
+    const T q0 = (T)(k.kx * g.wave_x), q1 = (T)(k.ky * g.wave_y), q2 = (T)(k.kz * g.wave_z);
+    V2<T> a = cmul(ld2(w0 + 2 * c), scal(q0, conjv(shx[k.i])));
+    a = a + cmul(ld2(w1 + 2 * c), scal(q1, conjv(shy[k.j])));
+    a = a + cmul(ld2(w2 + 2 * c), scal(q2, conjv(shz[k.kz])));
+    const V2<T> rv = V2<T>{-a.im, a.re} - ld2(b + 2 * c);
+    st2(r + 2 * c, rv);
+    s += kweight(g, k.kz) * ((double)rv.re * rv.re + (double)rv.im * rv.im);
+  }
+  s = block_sum(s);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// over every entry of a field (a half spectrum here): x <- 2 x - prev, prev <- old x
+__global__ void extrapolate_all_kernel(double *__restrict__ x, double *__restrict__ prev, size_t n) {
+  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const double xn = x[e];
+  x[e] = 2.0 * xn - prev[e];
+  prev[e] = xn;
+}
+
+// over every entry: x64 -= (double) d   (r = A x - b: the correction enters with a minus sign)
+__global__ void sub_all_kernel(double *__restrict__ x, const float *__restrict__ d, size_t n) {
+  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) x[e] -= (double)d[e];
+}
+
 // sc[out] = count / sc[sum] (1 if the sum is not positive)
 __global__ void inv_mean_kernel(double *__restrict__ sc, int sum, int out, double count) {
   if (threadIdx.x == 0 && blockIdx.x == 0) sc[out] = sc[sum] > 0.0 ? count / sc[sum] : 1.0;
@@ -534,15 +572,18 @@ void ax_t(kapsel_ctx *ctx, const T *x, T *out, const T *eps, T *const *s) {
                        s[2], s[3], sh<T>(ctx, 0), sh<T>(ctx, 1), sh<T>(ctx, 2), g);
     KAPSEL_LAUNCH_CHECK();
   }
+  // inverse transforms, face permittivity product, forward transforms: one
+  // fused pass over the z (and y) axes (the face factors of the context
+  // precision go to the operator scratch slots 5..7)
   void *s3[3] = {s[1], s[2], s[3]};
-  fft_c2r_t<T>(ctx, s3, 3);
+  void *f3[3] = {ctx->pscratch[5], ctx->pscratch[6], ctx->pscratch[7]};
   {
-    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 7, sizeof(T)));
-    hipLaunchKernelGGL(face_mul_kernel<T>, dim3(blocks_for(g.nr, TPB)), dim3(TPB), 0, ctx->stream, s[1], s[2],
-                       s[3], eps, g);
+    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 4, sizeof(T)));
+    hipLaunchKernelGGL(face_fields_kernel<T>, dim3(blocks_for(g.f, TPB)), dim3(TPB), 0, ctx->stream, eps,
+                       (T *)f3[0], (T *)f3[1], (T *)f3[2], g);
     KAPSEL_LAUNCH_CHECK();
   }
-  fft_r2c_t<T>(ctx, s3, 3);
+  fft_face_t<T>(ctx, s3, f3);
   {
     LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 4, sizeof(T)));
     hipLaunchKernelGGL(div_shift_kernel<T>, dim3(blocks_for(g.nc, TPB)), dim3(TPB), 0, ctx->stream, s[1], s[2],
@@ -763,16 +804,7 @@ void kcg_iteration(kapsel_ctx *ctx, T *x, const T *eps, int precond, T *const *w
                        ctx->d_scal, 0, s[0], s[1], s[2], sh<T>(ctx, 0), sh<T>(ctx, 1), sh<T>(ctx, 2), g);
     KAPSEL_LAUNCH_CHECK();
   }
-  if (!fft_face_t<T>(ctx, s3, f3)) {
-    fft_c2r_t<T>(ctx, s3, 3);
-    {
-      LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 7, sizeof(T)));
-      hipLaunchKernelGGL(face_mul_kernel<T>, dim3(blocks_for(g.nr, TPB)), dim3(TPB), 0, ctx->stream, s[0], s[1],
-                         s[2], eps, g);
-      KAPSEL_LAUNCH_CHECK();
-    }
-    fft_r2c_t<T>(ctx, s3, 3);
-  }
+  fft_face_t<T>(ctx, s3, f3);
   {
     LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 5, sizeof(T)));
     hipLaunchKernelGGL(kcg_div_dot_kernel<T>, dim3(NPART), dim3(TPB), 0, ctx->stream, s[0], s[1], s[2], p, ap,
@@ -935,35 +967,43 @@ struct RefineScope {
 int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int precond, double *relres) {
   const Grid &g = ctx->g;
   RefineScope refine(ctx);
-  double *eps64 = ctx->p64[0], *x64 = ctx->p64[1], *b64 = ctx->p64[2], *r64 = ctx->p64[3];
+  // FP64 shadow: eps and b (real), the potential and the residual as half
+  // spectra, the RHS spectrum, operator scratch, face permittivities
+  double *eps64 = ctx->p64[0], *xk = ctx->p64[1], *b64 = ctx->p64[2], *rk = ctx->p64[3];
+  double *bk = ctx->p64[4];
   double *s64[4] = {ctx->p64[4], ctx->p64[5], ctx->p64[6], ctx->p64[7]};
+  double *face64[3] = {ctx->p64[8], ctx->p64[9], ctx->p64[10]};
   float *s32[4] = {(float *)ctx->pscratch[0], (float *)ctx->pscratch[1], (float *)ctx->pscratch[2],
                    (float *)ctx->pscratch[3]};
   float *w32[4] = {fld<float>(ctx, KAPSEL_WORK0), fld<float>(ctx, KAPSEL_WORK1), fld<float>(ctx, KAPSEL_WORK2),
                    fld<float>(ctx, KAPSEL_WORK3)};
   float *d32 = (float *)ctx->pscratch[4], *eps32 = fld<float>(ctx, KAPSEL_EPSILON);
+  const unsigned fb = blocks_for(g.f, TPB);
   // FP64 copies of the FP32 inputs: eps and rho enter the residual in FP64
-  hipLaunchKernelGGL((convert_real_kernel<double, float>), dim3(blocks_for(g.f, TPB)), dim3(TPB), 0, ctx->stream,
-                     eps64, eps32, g);
-  hipLaunchKernelGGL((convert_real_kernel<double, float>), dim3(blocks_for(g.f, TPB)), dim3(TPB), 0, ctx->stream,
-                     r64, fld<float>(ctx, KAPSEL_RHO_FREE), g);
+  // (rk holds rho until the RHS is built)
+  hipLaunchKernelGGL((convert_real_kernel<double, float>), dim3(fb), dim3(TPB), 0, ctx->stream, eps64, eps32, g);
+  hipLaunchKernelGGL((convert_real_kernel<double, float>), dim3(fb), dim3(TPB), 0, ctx->stream, rk,
+                     fld<float>(ctx, KAPSEL_RHO_FREE), g);
+  KAPSEL_LAUNCH_CHECK();
   // start: the POTENTIAL field if it was written since the last solve; else
   // the last solution, extrapolated linearly in time once two solutions of
   // consecutive solves are known
   if (!ctx->pot64_valid) {
-    hipLaunchKernelGGL((convert_real_kernel<double, float>), dim3(blocks_for(g.f, TPB)), dim3(TPB), 0, ctx->stream,
-                       x64, fld<float>(ctx, KAPSEL_POTENTIAL), g);
+    hipLaunchKernelGGL((convert_real_kernel<double, float>), dim3(fb), dim3(TPB), 0, ctx->stream, xk,
+                       fld<float>(ctx, KAPSEL_POTENTIAL), g);
+    KAPSEL_LAUNCH_CHECK();
+    void *x1[1] = {xk};
+    fft_r2c_t<double>(ctx, x1, 1);
     ctx->prev_valid = false;
   } else if (ctx->prev_valid) {
     LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 4, sizeof(double)));
-    hipLaunchKernelGGL(extrapolate_kernel, dim3(blocks_for(g.nr, TPB)), dim3(TPB), 0, ctx->stream, x64,
-                       ctx->x64_prev, g);
+    hipLaunchKernelGGL(extrapolate_all_kernel, dim3(fb), dim3(TPB), 0, ctx->stream, xk, ctx->x64_prev, g.f);
+    KAPSEL_LAUNCH_CHECK();
   } else {
-    KAPSEL_HIP(hipMemcpyAsync(ctx->x64_prev, x64, g.f * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    KAPSEL_HIP(hipMemcpyAsync(ctx->x64_prev, xk, g.f * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     ctx->prev_valid = true;
   }
-  KAPSEL_LAUNCH_CHECK();
-  build_rhs_t<double>(ctx, b64, r64, eps64, e0, s64);
+  build_rhs_t<double>(ctx, b64, rk, eps64, e0, s64);
   dot_t<double, double>(ctx, b64, b64, S_BB);
   const double bb = read_scal(ctx, S_BB);
   if (bb == 0.0) {
@@ -972,52 +1012,70 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
     *relres = 0.0;
     return 0;
   }
+  {
+    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 2, sizeof(double)));
+    hipLaunchKernelGGL(copy_real_kernel<double>, dim3(fb), dim3(TPB), 0, ctx->stream, b64, bk, g);
+    KAPSEL_LAUNCH_CHECK();
+  }
+  void *b1[1] = {bk};
+  fft_r2c_t<double>(ctx, b1, 1);
   if (precond) inv_eps_device<float>(ctx, eps32, w32[3]);
   float *face32[3] = {(float *)ctx->pscratch[5], (float *)ctx->pscratch[6], (float *)ctx->pscratch[7]};
-  hipLaunchKernelGGL(face_fields_kernel<float>, dim3(blocks_for(g.f, TPB)), dim3(TPB), 0, ctx->stream, eps32,
-                     face32[0], face32[1], face32[2], g);
+  hipLaunchKernelGGL(face_fields_kernel<float>, dim3(fb), dim3(TPB), 0, ctx->stream, eps32, face32[0], face32[1],
+                     face32[2], g);
+  hipLaunchKernelGGL(face_fields_kernel<double>, dim3(fb), dim3(TPB), 0, ctx->stream, eps64, face64[0],
+                     face64[1], face64[2], g);
   KAPSEL_LAUNCH_CHECK();
-  const double bnorm = sqrt(bb);
+  void *g3[3] = {s64[1], s64[2], s64[3]};
+  void *f3[3] = {face64[0], face64[1], face64[2]};
+  const double bnorm = sqrt(bb), norm = 1.0 / (double)g.nr;
+  float *b32 = fld<float>(ctx, KAPSEL_RHS);  // the correction's right-hand side spectrum meanwhile
   int total = 0;
   double rel = 1.0;
   for (int outer = 0; outer < 30; outer++) {
-    // r64 = A x64 - b64
-    ax_t<double>(ctx, x64, r64, eps64, s64);
-    hipLaunchKernelGGL(resid_kernel<double>, dim3(blocks_for(g.nr, TPB)), dim3(TPB), 0, ctx->stream, r64, b64,
-                       r64, g);
-    KAPSEL_LAUNCH_CHECK();
-    dot_t<double, double>(ctx, r64, r64, S_RR);
-    const double rr = read_scal(ctx, S_RR);
+    // rk = A xk - bk in k-space, ||r||^2 by Parseval
+    {
+      LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 4, sizeof(double)));
+      hipLaunchKernelGGL(grad_shift_kernel<double>, dim3(blocks_for(g.nc, TPB)), dim3(TPB), 0, ctx->stream, xk,
+                         s64[1], s64[2], s64[3], sh<double>(ctx, 0), sh<double>(ctx, 1), sh<double>(ctx, 2), g);
+      KAPSEL_LAUNCH_CHECK();
+    }
+    fft_face_t<double>(ctx, g3, f3);
+    {
+      LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 5, sizeof(double)));
+      hipLaunchKernelGGL(kres_kernel<double>, dim3(NPART), dim3(TPB), 0, ctx->stream, s64[1], s64[2], s64[3], bk,
+                         rk, sh<double>(ctx, 0), sh<double>(ctx, 1), sh<double>(ctx, 2), ctx->d_partial, g);
+      KAPSEL_LAUNCH_CHECK();
+      hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->d_partial, NPART, ctx->d_scal,
+                         S_RR, 0);
+      KAPSEL_LAUNCH_CHECK();
+    }
+    const double rr = read_scal(ctx, S_RR) * norm;
     rel = sqrt(rr) / bnorm;
     if (rel < tol || total >= maxit) break;
-    // correction: A d = r in FP32 (RHS holds r meanwhile), solved in k-space,
-    // then x -= d.  Inner tolerance: what the outer bound still needs (with a
-    // factor 2 of margin), but never below 1e-4 of the current residual,
-    // the reduction FP32 certifies
-    float *b32 = fld<float>(ctx, KAPSEL_RHS);
-    hipLaunchKernelGGL((convert_real_kernel<float, double>), dim3(blocks_for(g.f, TPB)), dim3(TPB), 0,
-                       ctx->stream, b32, r64, g);
+    // correction: A d = r in FP32 k-space, then x -= d.  Inner tolerance:
+    // what the outer bound still needs (with a factor 2 of margin), but
+    // never below 1e-4 of the current residual, the reduction FP32 certifies
+    hipLaunchKernelGGL((convert_real_kernel<float, double>), dim3(fb), dim3(TPB), 0, ctx->stream, b32, rk, g);
     KAPSEL_LAUNCH_CHECK();
-    void *bk[1] = {b32};
     ctx->refine64 = false;  // the FP32 correction solve
-    fft_r2c_t<float>(ctx, bk, 1);
     double rr_in = 0.0;
     const double tol_in = fmax(1e-4 * sqrt(rr), 0.5 * tol * bnorm);
     int it = kcg_t<float>(ctx, d32, b32, eps32, tol_in, maxit - total, precond, w32, s32 + 1, face32, &rr_in);
     total += it;
-    void *dk[1] = {d32};
-    fft_c2r_t<float>(ctx, dk, 1);
     ctx->refine64 = true;
-    hipLaunchKernelGGL(add_correction_kernel<float>, dim3(blocks_for(g.nr, TPB)), dim3(TPB), 0, ctx->stream, x64,
-                       d32, g);
+    hipLaunchKernelGGL(sub_all_kernel, dim3(fb), dim3(TPB), 0, ctx->stream, xk, d32, g.f);
     KAPSEL_LAUNCH_CHECK();
     if (it == 0) break;
   }
-  hipLaunchKernelGGL((convert_real_kernel<float, double>), dim3(blocks_for(g.f, TPB)), dim3(TPB), 0, ctx->stream,
-                     fld<float>(ctx, KAPSEL_POTENTIAL), x64, g);
-  hipLaunchKernelGGL((convert_real_kernel<float, double>), dim3(blocks_for(g.f, TPB)), dim3(TPB), 0, ctx->stream,
-                     fld<float>(ctx, KAPSEL_RHS), b64, g);
+  // outputs: the potential (FP32, real) and the RHS b
+  float *pot = fld<float>(ctx, KAPSEL_POTENTIAL);
+  hipLaunchKernelGGL((convert_real_kernel<float, double>), dim3(fb), dim3(TPB), 0, ctx->stream, pot, xk, g);
+  hipLaunchKernelGGL((convert_real_kernel<float, double>), dim3(fb), dim3(TPB), 0, ctx->stream, b32, b64, g);
   KAPSEL_LAUNCH_CHECK();
+  ctx->refine64 = false;
+  void *p1[1] = {pot};
+  fft_c2r_t<float>(ctx, p1, 1);
   ctx->pot64_valid = true;
   *relres = rel;
   return total;
