@@ -34,7 +34,7 @@ PRECOND_NONE = 0
 PRECOND_SPECTRAL = 1
 
 KERNEL_CLASSES = ["FFT_Z_R2C", "FFT_Z_C2R", "FFT_LINES_FWD", "FFT_LINES_INV", "POINTWISE", "PARTICLE",
-                  "NETWORK", "REFINE64", "FFT_PLANE_FACE"]
+                  "NETWORK", "REFINE64", "FFT_PLANE_FACE", "FFT_X_GRAD", "FFT_X_DIV"]
 KCLASS = {name: i for i, name in enumerate(KERNEL_CLASSES)}
 
 

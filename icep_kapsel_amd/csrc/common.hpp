@@ -201,6 +201,18 @@ void fft_c2r_t(kapsel_ctx *ctx, void *const *fields, int nf);
 // where a plane fits in LDS, y) transforms and the product fused in one pass
 template <typename T>
 void fft_face_t(kapsel_ctx *ctx, void *const *g3, void *const *face3);
+// fft_face_t without its first (x inverse) and last (x forward) passes
+template <typename T>
+void fft_face_mid_t(kapsel_ctx *ctx, void *const *g3, void *const *face3);
+// the operator's first pass on a CG direction: p = z + sc[beta_slot] p, then
+// g_d = x-inverse of (i k_d e^{i k_d dx/2} p); false: NX has no fused pass
+template <typename T>
+bool fft_x_grad_t(kapsel_ctx *ctx, const T *z, T *p, const double *sc, int beta_slot, void *const *g3);
+// the operator's last pass: ap = staggered divergence of the x-forward of
+// g_d; one Parseval-weighted <p, ap> partial per workgroup; returns the
+// number of partials (0: NX has no fused pass)
+template <typename T>
+int fft_x_div_t(kapsel_ctx *ctx, void *const *g3, const T *p, T *ap, double *partial);
 
 // k-space and real-space pointwise (kspace.hip)
 void truncate(kapsel_ctx *ctx, void *const *fields, int nf);

@@ -718,6 +718,172 @@ __global__ __launch_bounds__(RPB *FS<NZ / 2>::R2) void fft_z_face(FieldPtrs<T> f
   }
 }
 
+// ---------------------------------------- x passes fused with the CG algebra
+// The dielectric operator applied to a CG direction p in k-space: the first
+// pass (x, inverse) also forms p = z + beta p and the staggered gradient
+// symbols; the last pass (x, forward) also forms the staggered divergence and
+// <p, A p>.  x line q = j*h + kz: elements q + i*plane, i = 0..NX-1.
+
+__device__ __forceinline__ int signed_k(int i, int n) { return i > n / 2 ? i - n : i; }
+
+template <int THREADS>
+__device__ __forceinline__ double block_sum_fft(double v) {
+  __shared__ double red[THREADS / 64];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+  const int lane = threadIdx.x % 64, wid = threadIdx.x / 64;
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < THREADS / 64; w++) s += red[w];
+  return s;
+}
+
+// p = z + sc[beta] p (stored), then g_d = IFFT_x(i k_d e^{i k_d dx/2} p), d = x, y, z
+template <typename T, int L, int LPB>
+__global__ __launch_bounds__(LPB *FS<L>::R2) void fft_x_grad(const T *z, T *p, const double *__restrict__ sc,
+                                                             int beta_slot, FieldPtrs<T> g3, long long Q,
+                                                             long long plane, int h, int ny, double wx, double wy,
+                                                             double wz, const V2<T> *__restrict__ shx,
+                                                             const V2<T> *__restrict__ shy,
+                                                             const V2<T> *__restrict__ shz,
+                                                             const V2<T> *__restrict__ tw) {
+  constexpr int R1 = FS<L>::R1, R2 = FS<L>::R2;
+  constexpr int LS = R2 * (R1 + 1) + 1;
+  __shared__ V2<T> sm[LPB * LS];
+  unsigned field;
+  const unsigned tile = xcd_tile(field);
+  const int t = threadIdx.x;
+  const int l = t % LPB, n2 = t / LPB;
+  const long long q = (long long)tile * LPB + l;
+  const bool valid = q < Q;
+  const long long base = valid ? q : 0;
+  const int j = (int)(base / h), kz = (int)(base % h);
+  const T beta = (T)sc[beta_slot];
+  V2<T> pv[R1];
+#pragma unroll
+  for (int n1 = 0; n1 < R1; n1++) {
+    const long long off = base + (long long)(R2 * n1 + n2) * plane;
+    if (valid) {
+      pv[n1] = ld2(z + 2 * off) + scal(beta, ld2(p + 2 * off));
+      st2(p + 2 * off, pv[n1]);
+    } else {
+      pv[n1] = V2<T>{0, 0};
+    }
+  }
+  V2<T> *sl = sm + l * LS;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    V2<T> v[R1], o[R1];
+#pragma unroll
+    for (int n1 = 0; n1 < R1; n1++) {
+      const int i = R2 * n1 + n2;
+      const T w = d == 0 ? (T)(signed_k(i, L) * wx) : d == 1 ? (T)(signed_k(j, ny) * wy) : (T)(kz * wz);
+      const V2<T> sft = d == 0 ? shx[i] : d == 1 ? shy[j] : shz[kz];
+      v[n1] = cmul(V2<T>{-w * pv[n1].im, w * pv[n1].re}, sft);
+    }
+    dft<R1, 1, +1>(v, o);
+#pragma unroll
+    for (int k1 = 1; k1 < R1; k1++) o[k1] = cmul(o[k1], tw_get<+1>(tw, n2 * k1));
+    if (d > 0) __syncthreads();  // the previous direction's LDS reads are done
+#pragma unroll
+    for (int k1 = 0; k1 < R1; k1++) sl[n2 * (R1 + 1) + k1] = o[k1];
+    __syncthreads();
+    T *__restrict__ gd = g3.p[d];
+#pragma unroll
+    for (int c = 0; c < R1 / R2; c++) {
+      const int k1 = n2 + R2 * c;
+      V2<T> w[R2], x[R2];
+#pragma unroll
+      for (int m = 0; m < R2; m++) w[m] = sl[m * (R1 + 1) + k1];
+      dft<R2, 1, +1>(w, x);
+      if (valid) {
+#pragma unroll
+        for (int k2 = 0; k2 < R2; k2++) st2(gd + 2 * (base + (long long)(k1 + R1 * k2) * plane), x[k2]);
+      }
+    }
+  }
+}
+
+// w_d = FFT_x(g_d), ap = i sum_d k_d e^{-i k_d dx/2} w_d (stored);
+// partial[tile] = sum over the tile of w(kz) Re(conj(p) ap), w = 1 on the
+// kz = 0 and NZ/2 planes and 2 elsewhere (Parseval, without N^-3)
+template <typename T, int L, int LPB>
+__global__ __launch_bounds__(LPB *FS<L>::R2) void fft_x_div(FieldPtrs<T> g3, const T *__restrict__ p,
+                                                            T *__restrict__ ap, double *__restrict__ partial,
+                                                            long long Q, long long plane, int h, int ny, int nz,
+                                                            double wx, double wy, double wz,
+                                                            const V2<T> *__restrict__ shx,
+                                                            const V2<T> *__restrict__ shy,
+                                                            const V2<T> *__restrict__ shz,
+                                                            const V2<T> *__restrict__ tw) {
+  constexpr int R1 = FS<L>::R1, R2 = FS<L>::R2;
+  constexpr int LS = R2 * (R1 + 1) + 1;
+  __shared__ V2<T> sm[LPB * LS];
+  unsigned field;
+  const unsigned tile = xcd_tile(field);
+  const int t = threadIdx.x;
+  const int l = t % LPB, n2 = t / LPB;
+  const long long q = (long long)tile * LPB + l;
+  const bool valid = q < Q;
+  const long long base = valid ? q : 0;
+  const int j = (int)(base / h), kz = (int)(base % h);
+  V2<T> acc[R1 / R2][R2];
+#pragma unroll
+  for (int c = 0; c < R1 / R2; c++)
+#pragma unroll
+    for (int k2 = 0; k2 < R2; k2++) acc[c][k2] = V2<T>{0, 0};
+  V2<T> *sl = sm + l * LS;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const T *__restrict__ gd = g3.p[d];
+    V2<T> v[R1], o[R1];
+#pragma unroll
+    for (int n1 = 0; n1 < R1; n1++)
+      v[n1] = valid ? ld2(gd + 2 * (base + (long long)(R2 * n1 + n2) * plane)) : V2<T>{0, 0};
+    dft<R1, 1, -1>(v, o);
+#pragma unroll
+    for (int k1 = 1; k1 < R1; k1++) o[k1] = cmul(o[k1], tw_get<-1>(tw, n2 * k1));
+    if (d > 0) __syncthreads();
+#pragma unroll
+    for (int k1 = 0; k1 < R1; k1++) sl[n2 * (R1 + 1) + k1] = o[k1];
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < R1 / R2; c++) {
+      const int k1 = n2 + R2 * c;
+      V2<T> w[R2], x[R2];
+#pragma unroll
+      for (int m = 0; m < R2; m++) w[m] = sl[m * (R1 + 1) + k1];
+      dft<R2, 1, -1>(w, x);
+#pragma unroll
+      for (int k2 = 0; k2 < R2; k2++) {
+        const int i = k1 + R1 * k2;
+        const T qd = d == 0 ? (T)(signed_k(i, L) * wx) : d == 1 ? (T)(signed_k(j, ny) * wy) : (T)(kz * wz);
+        const V2<T> sft = d == 0 ? shx[i] : d == 1 ? shy[j] : shz[kz];
+        acc[c][k2] = acc[c][k2] + cmul(x[k2], scal(qd, conjv(sft)));
+      }
+    }
+  }
+  const double wk = (kz == 0 || 2 * kz == nz) ? 1.0 : 2.0;
+  double s = 0.0;
+  if (valid) {
+#pragma unroll
+    for (int c = 0; c < R1 / R2; c++) {
+      const int k1 = n2 + R2 * c;
+#pragma unroll
+      for (int k2 = 0; k2 < R2; k2++) {
+        const long long off = base + (long long)(k1 + R1 * k2) * plane;
+        const V2<T> av{-acc[c][k2].im, acc[c][k2].re};
+        st2(ap + 2 * off, av);
+        const V2<T> pp = ld2(p + 2 * off);
+        s += wk * ((double)pp.re * av.re + (double)pp.im * av.im);
+      }
+    }
+  }
+  s = block_sum_fft<LPB * FS<L>::R2>(s);
+  if (threadIdx.x == 0) partial[tile] = s;
+}
+
 // ------------------------------------------------------------------- host
 namespace {
 
@@ -978,6 +1144,85 @@ void fft_face_t(kapsel_ctx *ctx, void *const *g3, void *const *face3) {
   lines_dispatch<T, -1>(ctx, KAPSEL_K_FFT_LINES_FWD, g.nx, fp, 3, plane, plane, 0, plane, tw.x);
 }
 
+// the middle of fft_face_t: everything but the two x passes
+template <typename T>
+void fft_face_mid_t(kapsel_ctx *ctx, void *const *g3, void *const *face3) {
+  const Grid &g = ctx->g;
+  const Twiddles &tw = ctx->tw[prec_of<T>()];
+  FieldPtrs<T> fp = pack<T>(g3, 3), fc = pack<T>(face3, 3);
+  if (plane_ok<T>(g)) {
+    switch (g.nz) {
+      case 32: launch_plane_face<T, 32, 128>(ctx, fp, fc); return;
+      case 64: launch_plane_face<T, 64, 320>(ctx, fp, fc); return;
+      case 128:
+        if constexpr (sizeof(T) == 4) {
+          launch_plane_face<T, 128, 576>(ctx, fp, fc);
+          return;
+        }
+        break;
+      default: break;
+    }
+  }
+  lines_dispatch<T, +1>(ctx, KAPSEL_K_FFT_LINES_INV, g.ny, fp, 3, (long long)g.nx * g.h, g.h, (long long)g.ny * g.h,
+                        g.h, tw.y);
+  z_face_dispatch<T>(ctx, fp, fc);
+  lines_dispatch<T, -1>(ctx, KAPSEL_K_FFT_LINES_FWD, g.ny, fp, 3, (long long)g.nx * g.h, g.h, (long long)g.ny * g.h,
+                        g.h, tw.y);
+}
+
+template <typename T, int L>
+void launch_x_grad(kapsel_ctx *ctx, const T *z, T *p, const double *sc, int beta_slot, void *const *g3) {
+  constexpr int LPB = lines_per_block(sizeof(T));
+  const Grid &g = ctx->g;
+  const Twiddles &tw = ctx->tw[prec_of<T>()];
+  const long long Q = (long long)g.ny * g.h;
+  LaunchTimer lt(ctx, KAPSEL_K_FFT_X_GRAD, 6.0 * g.f * sizeof(T));
+  hipLaunchKernelGGL((fft_x_grad<T, L, LPB>), dim3((unsigned)((Q + LPB - 1) / LPB)), dim3(LPB * FS<L>::R2), 0,
+                     ctx->stream, z, p, sc, beta_slot, pack<T>(g3, 3), Q, Q, g.h, g.ny, g.wave_x, g.wave_y,
+                     g.wave_z, (const V2<T> *)tw.shx, (const V2<T> *)tw.shy, (const V2<T> *)tw.shz,
+                     (const V2<T> *)tw.x);
+  KAPSEL_LAUNCH_CHECK();
+}
+
+template <typename T, int L>
+int launch_x_div(kapsel_ctx *ctx, void *const *g3, const T *p, T *ap, double *partial) {
+  constexpr int LPB = lines_per_block(sizeof(T));
+  const Grid &g = ctx->g;
+  const Twiddles &tw = ctx->tw[prec_of<T>()];
+  const long long Q = (long long)g.ny * g.h;
+  const unsigned nb = (unsigned)((Q + LPB - 1) / LPB);
+  LaunchTimer lt(ctx, KAPSEL_K_FFT_X_DIV, 5.0 * g.f * sizeof(T));
+  hipLaunchKernelGGL((fft_x_div<T, L, LPB>), dim3(nb), dim3(LPB * FS<L>::R2), 0, ctx->stream, pack<T>(g3, 3), p,
+                     ap, partial, Q, Q, g.h, g.ny, g.nz, g.wave_x, g.wave_y, g.wave_z, (const V2<T> *)tw.shx,
+                     (const V2<T> *)tw.shy, (const V2<T> *)tw.shz, (const V2<T> *)tw.x);
+  KAPSEL_LAUNCH_CHECK();
+  return (int)nb;
+}
+
+template <typename T>
+bool fft_x_grad_t(kapsel_ctx *ctx, const T *z, T *p, const double *sc, int beta_slot, void *const *g3) {
+  switch (ctx->g.nx) {
+    case 64: launch_x_grad<T, 64>(ctx, z, p, sc, beta_slot, g3); return true;
+    case 128: launch_x_grad<T, 128>(ctx, z, p, sc, beta_slot, g3); return true;
+    default: return false;
+  }
+}
+
+template <typename T>
+int fft_x_div_t(kapsel_ctx *ctx, void *const *g3, const T *p, T *ap, double *partial) {
+  switch (ctx->g.nx) {
+    case 64: return launch_x_div<T, 64>(ctx, g3, p, ap, partial);
+    case 128: return launch_x_div<T, 128>(ctx, g3, p, ap, partial);
+    default: return 0;
+  }
+}
+
+template void fft_face_mid_t<double>(kapsel_ctx *, void *const *, void *const *);
+template void fft_face_mid_t<float>(kapsel_ctx *, void *const *, void *const *);
+template bool fft_x_grad_t<float>(kapsel_ctx *, const float *, float *, const double *, int, void *const *);
+template bool fft_x_grad_t<double>(kapsel_ctx *, const double *, double *, const double *, int, void *const *);
+template int fft_x_div_t<float>(kapsel_ctx *, void *const *, const float *, float *, double *);
+template int fft_x_div_t<double>(kapsel_ctx *, void *const *, const double *, double *, double *);
 template void fft_face_t<double>(kapsel_ctx *, void *const *, void *const *);
 template void fft_face_t<float>(kapsel_ctx *, void *const *, void *const *);
 template void fft_r2c_t<double>(kapsel_ctx *, void *const *, int);

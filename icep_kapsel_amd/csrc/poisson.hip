@@ -798,21 +798,31 @@ void kcg_iteration(kapsel_ctx *ctx, T *x, const T *eps, int precond, T *const *w
   T *r = w[0], *p = w[1], *ap = w[2], *z = precond ? w[3] : w[0];
   void *s3[3] = {s[0], s[1], s[2]};
   void *f3[3] = {face[0], face[1], face[2]};
-  {
-    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 6, sizeof(T)));
-    hipLaunchKernelGGL(kcg_dir_grad_kernel<T>, dim3(blocks_for(g.nc, TPB)), dim3(TPB), 0, ctx->stream, z, p,
-                       ctx->d_scal, 0, s[0], s[1], s[2], sh<T>(ctx, 0), sh<T>(ctx, 1), sh<T>(ctx, 2), g);
+  if (fft_x_grad_t<T>(ctx, z, p, ctx->d_scal, S_BETA, s3)) {
+    // direction update and gradient in the first pass, divergence and
+    // <p, Ap> in the last
+    fft_face_mid_t<T>(ctx, s3, f3);
+    const int nb = fft_x_div_t<T>(ctx, s3, p, ap, ctx->d_partial);
+    hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->d_partial, nb, ctx->d_scal, S_PAP,
+                       0);
     KAPSEL_LAUNCH_CHECK();
-  }
-  fft_face_t<T>(ctx, s3, f3);
-  {
-    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 5, sizeof(T)));
-    hipLaunchKernelGGL(kcg_div_dot_kernel<T>, dim3(NPART), dim3(TPB), 0, ctx->stream, s[0], s[1], s[2], p, ap,
-                       sh<T>(ctx, 0), sh<T>(ctx, 1), sh<T>(ctx, 2), ctx->d_partial, g);
-    KAPSEL_LAUNCH_CHECK();
-    hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->d_partial, NPART, ctx->d_scal,
-                       S_PAP, 0);
-    KAPSEL_LAUNCH_CHECK();
+  } else {
+    {
+      LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 6, sizeof(T)));
+      hipLaunchKernelGGL(kcg_dir_grad_kernel<T>, dim3(blocks_for(g.nc, TPB)), dim3(TPB), 0, ctx->stream, z, p,
+                         ctx->d_scal, 0, s[0], s[1], s[2], sh<T>(ctx, 0), sh<T>(ctx, 1), sh<T>(ctx, 2), g);
+      KAPSEL_LAUNCH_CHECK();
+    }
+    fft_face_t<T>(ctx, s3, f3);
+    {
+      LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 5, sizeof(T)));
+      hipLaunchKernelGGL(kcg_div_dot_kernel<T>, dim3(NPART), dim3(TPB), 0, ctx->stream, s[0], s[1], s[2], p, ap,
+                         sh<T>(ctx, 0), sh<T>(ctx, 1), sh<T>(ctx, 2), ctx->d_partial, g);
+      KAPSEL_LAUNCH_CHECK();
+      hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->d_partial, NPART, ctx->d_scal,
+                         S_PAP, 0);
+      KAPSEL_LAUNCH_CHECK();
+    }
   }
   {
     LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, precond ? 7 : 6, sizeof(T)));

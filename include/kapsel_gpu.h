@@ -254,6 +254,8 @@ enum kapsel_kernel_class {
   KAPSEL_K_NETWORK,       /* warm-start network kernels */
   KAPSEL_K_REFINE64,      /* FP64 work of an FP32 context's iterative refinement */
   KAPSEL_K_FFT_PLANE_FACE, /* inverse z+y, face-permittivity product, forward z+y of the operator (one kernel) */
+  KAPSEL_K_FFT_X_GRAD,     /* CG direction update + gradient symbols + x inverse pass (one kernel) */
+  KAPSEL_K_FFT_X_DIV,      /* x forward pass + divergence symbols + <p, Ap> partials (one kernel) */
   KAPSEL_K_NCLASSES
 };
 /* Records a HIP event pair around every launch of the kernel classes whose
