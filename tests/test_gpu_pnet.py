@@ -103,3 +103,46 @@ def test_pnet_forward_matches_oracle_and_training_reduces_loss():
     assert losses[-1] < 0.7 * losses[0]
     p2 = k.pnet_get_params(P)
     assert np.abs(p2 - params).max() > 0
+
+
+def test_trained_network_gain_on_held_out_solves():
+    """What the trained network (icep_kapsel_amd/weights) gains on held-out
+    configurations of the ICEP box (seed 4104): the energy-norm error of its
+    start, (e, S e) with e = x0 - x* and S = -A, against that of a zero start
+    (= (x*, S x*)), and the spectral-PCG iterations to 1e-8 from each start.
+    The start must remove most of the energy-norm error (ratio < 0.35; about
+    0.1-0.2 is measured) and must not cost more than 2 iterations."""
+    import os
+    from icep_kapsel_amd import train_pnet
+    n = 128
+    path = os.path.join(os.path.dirname(__file__), "..", "icep_kapsel_amd", "weights", "pnet_icep_128.f32")
+    k = train_pnet.make_context(n)
+    w = np.fromfile(path, dtype=np.float32)
+    assert k.pnet_init() == w.size
+    k.pnet_set_params(w)
+
+    def energy(x):
+        k.set_field("WORK3", x)
+        k.poisson_apply("WORK3", "WORK2")
+        return -float(np.sum(x[:, :, :n].astype(np.float64) * k.get_field("WORK2")[:, :, :n]))
+
+    for s in range(3):
+        print(f"case network gain sample {s}", flush=True)
+        pos, ratio, e_hat = train_pnet.sample(4104 * 100000 + 50 + s, n)
+        e0 = train_pnet.E_MAG * e_hat
+        train_pnet.prepare(k, pos, ratio)
+        k.set_field("POTENTIAL", np.zeros(k.shape))
+        k.poisson_solve(e0, 1e-10, 20000, abi.PRECOND_SPECTRAL)
+        xs = k.get_field("POTENTIAL")
+        xs[:, :, :n] -= xs[:, :, :n].mean()
+        k.pnet_warm_start(train_pnet.beta_of(ratio), e0, train_pnet.A)
+        gamma = k.poisson_galerkin_start(e0)
+        ratio_e = energy(k.get_field("POTENTIAL") - xs) / energy(xs)
+        it_net, rr_net = k.poisson_solve(e0, 1e-8, 20000, abi.PRECOND_SPECTRAL)
+        k.set_field("POTENTIAL", np.zeros(k.shape))
+        it_zero, rr_zero = k.poisson_solve(e0, 1e-8, 20000, abi.PRECOND_SPECTRAL)
+        print(f"eps ratio {ratio:.3f}: energy-norm error of the start {ratio_e:.3f} of a zero start "
+              f"(gamma {gamma:.3f}); PCG iterations {it_net} from the network, {it_zero} from zero", flush=True)
+        assert rr_net < 1e-8 and rr_zero < 1e-8
+        assert ratio_e < 0.35
+        assert it_net <= it_zero + 2
