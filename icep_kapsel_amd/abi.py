@@ -18,6 +18,7 @@ FIELDS = [
     "ZETA0", "ZETA1", "U0", "U1", "U2", "UP0", "UP1", "UP2", "F0", "F1", "F2",
     "PHI", "PHI_SUM", "EPSILON", "POTENTIAL", "RHO_FREE", "RHS",
     "WORK0", "WORK1", "WORK2", "WORK3", "WORK4",
+    "CONC0", "CONC1",
 ]
 FIELD = {name: i for i, name in enumerate(FIELDS)}
 NFIELDS = len(FIELDS)
@@ -63,6 +64,31 @@ class Time(ctypes.Structure):
         ("dt_fluid", ctypes.c_double), ("hdt_fluid", ctypes.c_double),
         ("dt_md", ctypes.c_double), ("hdt_md", ctypes.c_double),
     ]
+
+
+class Electrolyte(ctypes.Structure):
+    _fields_ = [
+        ("kbt", ctypes.c_double), ("elementary_charge", ctypes.c_double),
+        ("dielectric_cst", ctypes.c_double), ("surface_charge", ctypes.c_double),
+        ("valency", ctypes.c_double * 2), ("onsager_coeff", ctypes.c_double * 2),
+        ("debye_length", ctypes.c_double), ("e_ext", ctypes.c_double * 3),
+        ("ac", ctypes.c_int), ("angular_frequency", ctypes.c_double),
+    ]
+
+
+def make_electrolyte(kbt=1.0, elementary_charge=1.0, dielectric_cst=1.0, surface_charge=100.0,
+                     valency=(1.0, -1.0), onsager_coeff=(7.24, 7.24), debye_length=10.0,
+                     e_ext=(0.0, 0.0, 0.0), ac=0, angular_frequency=0.0):
+    """Defaults are the salt section of the reference's UDF/input.udf:53-100."""
+    e = Electrolyte()
+    e.kbt, e.elementary_charge, e.dielectric_cst = kbt, elementary_charge, dielectric_cst
+    e.surface_charge = surface_charge
+    e.valency[:] = list(valency)
+    e.onsager_coeff[:] = list(onsager_coeff)
+    e.debye_length = debye_length
+    e.e_ext[:] = list(e_ext)
+    e.ac, e.angular_frequency = ac, angular_frequency
+    return e
 
 
 def make_time(dt, ts=0, time=0.0):
@@ -137,6 +163,17 @@ _PROTOS = {
     "kapsel_set_dielectric": [_D, _D, _D],
     "kapsel_maxwell_force": [_DP],
     "kapsel_icep_step": [ctypes.POINTER(Time), _DP, _D, _I, _I, _IP, _DP],
+    "kapsel_set_electrolyte": [ctypes.POINTER(Electrolyte)],
+    "kapsel_init_rho_ion": [],
+    "kapsel_get_total_solute": [_DP],
+    "kapsel_set_total_solute": [_DP],
+    "kapsel_make_phi_qq": [_I, _I],
+    "kapsel_make_surface_normal": [_I],
+    "kapsel_charge_field": [_I],
+    "kapsel_nssolute_euler": [ctypes.POINTER(Time)],
+    "kapsel_rescale_solute": [_DP],
+    "kapsel_coulomb_force": [ctypes.POINTER(Time)],
+    "kapsel_electrolyte_step": [ctypes.POINTER(Time)],
     "kapsel_pnet_init": [ctypes.POINTER(ctypes.c_long)],
     "kapsel_pnet_set_params": [ctypes.POINTER(ctypes.c_float)],
     "kapsel_pnet_get_params": [ctypes.POINTER(ctypes.c_float)],

@@ -118,6 +118,7 @@ static void destroy(kapsel_ctx *ctx) {
   free_twiddles(ctx);
   poisson_free(ctx);
   pnet_free(ctx);
+  electrolyte_free(ctx);
   for (auto &r : ctx->timing_pending) {
     (void)hipEventDestroy(r.start);
     (void)hipEventDestroy(r.stop);
@@ -350,6 +351,30 @@ static void icep_step(kapsel_ctx *ctx, kapsel_time &t, const double e0[3], doubl
   make_u_particle_sum(ctx);
   f_particle_dt_sole_add(ctx);
   u2zeta_k(ctx);
+  t.time += t.dt_fluid;
+  t.ts += 1;
+}
+
+// Time_evolution_hydro, Electrolyte branch without Dielectric
+// (sp_3d_ns.cxx:24-186): the Nernst-Planck Euler step replaces the slaved
+// step, Rescale_solute follows the profile scatter, and the Coulomb force is
+// applied to u before the hydrodynamic force.  The branch's first
+// Calc_f_hydro (:60-68) only fills the diagnostic f_hydro1; it is skipped
+// (the second call overwrites f_hydro and torque_hydro before any use).
+static void electrolyte_step(kapsel_ctx *ctx, kapsel_time &t) {
+  nssolute_euler(ctx, t);                     // :20
+  zeta_k2u(ctx, fix_mask(ctx));               // :29-42
+  md_position(ctx, t);                        // :44-50
+  make_phi_particle_sum(ctx);                 // :51-53
+  rescale_and_coulomb_force(ctx, t);          // :55-72
+  add_force_to_u(ctx, t.dt_fluid * ctx->irho);  // :75-84
+  solenoidal_u(ctx, ctx->fields + KAPSEL_U0);   // :85
+  calc_f_hydro(ctx, t);                       // :99-101
+  force(ctx);                                 // :104-110
+  md_velocity_integrate(ctx, t);
+  make_u_particle_sum(ctx);                   // :166-167
+  f_particle_dt_sole_add(ctx);                // :168-169
+  u2zeta_k(ctx);                              // :184
   t.time += t.dt_fluid;
   t.ts += 1;
 }
@@ -833,6 +858,118 @@ int kapsel_icep_step(kapsel_ctx *ctx, kapsel_time *jikan, const double e0[3], do
     set_error("icep_step: Poisson solve stopped at relative residual %.3e", *relres);
     return KAPSEL_ERR_NOCONV;
   }
+  API_END
+}
+
+int kapsel_set_electrolyte(kapsel_ctx *ctx, const kapsel_electrolyte *el) {
+  API_BEGIN
+  check_ctx(ctx);
+  if (!el) {
+    set_error("null electrolyte parameters");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+  electrolyte_set(ctx, *el);
+  API_END
+}
+
+int kapsel_init_rho_ion(kapsel_ctx *ctx) {
+  API_BEGIN
+  check_ctx(ctx);
+  init_rho_ion(ctx);
+  written(ctx, KAPSEL_PHI);
+  API_END
+}
+
+int kapsel_get_total_solute(kapsel_ctx *ctx, double total[2]) {
+  API_BEGIN
+  check_ctx(ctx);
+  if (!total) {
+    set_error("null output");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+  total[0] = ctx->el_total[0];
+  total[1] = ctx->el_total[1];
+  API_END
+}
+
+int kapsel_set_total_solute(kapsel_ctx *ctx, const double total[2]) {
+  API_BEGIN
+  check_ctx(ctx);
+  if (!total || !(total[0] > 0) || !(total[1] > 0)) {
+    set_error("totals must be positive");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+  ctx->el_total[0] = total[0];
+  ctx->el_total[1] = total[1];
+  API_END
+}
+
+int kapsel_make_phi_qq(kapsel_ctx *ctx, int phi_field, int surface_field) {
+  API_BEGIN
+  check_ctx(ctx);
+  check_field(phi_field);
+  check_field(surface_field);
+  if (phi_field == surface_field) {
+    set_error("phi and surface fields must differ");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+  make_phi_qq(ctx, fld(ctx, phi_field), fld(ctx, surface_field));
+  written(ctx, phi_field);
+  written(ctx, surface_field);
+  API_END
+}
+
+int kapsel_make_surface_normal(kapsel_ctx *ctx, int field0) {
+  API_BEGIN
+  check_ctx(ctx);
+  check_field(field0);
+  check_field(field0 + 2);
+  make_surface_normal(ctx, ctx->fields + field0);
+  for (int d = 0; d < 3; d++) written(ctx, field0 + d);
+  API_END
+}
+
+int kapsel_charge_field(kapsel_ctx *ctx, int field) {
+  API_BEGIN
+  check_ctx(ctx);
+  check_field(field);
+  charge_field(ctx, fld(ctx, field));
+  written(ctx, field);
+  API_END
+}
+
+int kapsel_nssolute_euler(kapsel_ctx *ctx, const kapsel_time *jikan) {
+  API_BEGIN
+  check_ctx(ctx);
+  check_time(jikan);
+  nssolute_euler(ctx, *jikan);
+  API_END
+}
+
+int kapsel_rescale_solute(kapsel_ctx *ctx, double factor[2]) {
+  API_BEGIN
+  check_ctx(ctx);
+  rescale_solute(ctx, factor);
+  API_END
+}
+
+int kapsel_coulomb_force(kapsel_ctx *ctx, const kapsel_time *jikan) {
+  API_BEGIN
+  check_ctx(ctx);
+  check_time(jikan);
+  coulomb_force(ctx, *jikan);
+  API_END
+}
+
+int kapsel_electrolyte_step(kapsel_ctx *ctx, kapsel_time *jikan) {
+  API_BEGIN
+  check_ctx(ctx);
+  check_time(jikan);
+  if (!ctx->el_set) {
+    set_error("electrolyte parameters not set (kapsel_set_electrolyte)");
+    throw Error{KAPSEL_ERR_STATE};
+  }
+  electrolyte_step(ctx, *jikan);
   API_END
 }
 

@@ -97,6 +97,45 @@ struct Grid {
   size_t nr;          // real grid points = NX*NY*NZ
 };
 
+// Index of complex value c of the half spectrum (one thread per value): the
+// row (i, j), kz, and the signed wavenumbers of Calc_K*_Ooura
+// (fft_wrapper.h:761-773), +N/2 kept for the Nyquist index
+struct KIdx {
+  int i, j, kz;
+  int kx, ky;
+};
+__device__ __forceinline__ KIdx kidx(const Grid &g, size_t c) {
+  KIdx r;
+  const unsigned plane = (unsigned)g.ny * (unsigned)g.h;
+  unsigned cc = (unsigned)c;
+  r.i = cc / plane;
+  unsigned rem = cc - r.i * plane;
+  r.j = rem / (unsigned)g.h;
+  r.kz = rem - r.j * g.h;
+  r.kx = r.i > g.hnx ? r.i - g.nx : r.i;
+  r.ky = r.j > g.hny ? r.j - g.ny : r.j;
+  return r;
+}
+// Truncate_two_third_rule_ooura (fft_wrapper.h:797-839): the cut set
+__device__ __forceinline__ bool is_cut(const Grid &g, const KIdx &k) {
+  return (k.kz >= g.trn_z) || (k.j >= g.trn_y && k.j <= g.ny - g.trn_y) ||
+         (k.i >= g.trn_x && k.i <= g.nx - g.trn_x);
+}
+// Real grid point r (k < NZ) -> its coordinates and padded index
+struct RI {
+  int i, j, k;
+  size_t im;
+};
+__device__ __forceinline__ RI ridx(const Grid &g, size_t r) {
+  RI o;
+  size_t row = r / (unsigned)g.nz;
+  o.k = (int)(r - row * g.nz);
+  o.j = (int)(row % (unsigned)g.ny);
+  o.i = (int)(row / (unsigned)g.ny);
+  o.im = row * g.nz_ + o.k;
+  return o;
+}
+
 // ---------------------------------------------------------------- timing
 struct TimingRecord {
   int cls;
@@ -156,6 +195,17 @@ struct kapsel_ctx {
   const void *kcg_key[11] = {};
   int n_partial = 0;
   kapsel::Pnet *pnet = nullptr;   // learned warm-start network, on first use
+  // electrolyte (electrolyte.hip): parameters, the conserved ion amounts, and
+  // scratch fields allocated by kapsel_set_electrolyte: surface normal (3),
+  // profile sum and raw surface charge of Make_phi_qq_particle (2), the two
+  // ion concentrations in real space (2), the charge density to transform and
+  // the real charge density kept for the force (2)
+  kapsel_electrolyte el{};
+  bool el_set = false;
+  double el_total[2] = {0.0, 0.0};
+  void *el_f[9] = {};
+  double *el_partial = nullptr;   // block partial sums of the electrolyte reductions
+  double *el_scal = nullptr;      // device scalars (electrolyte.hip)
   // derived physics constants (Set_global_parameters, input.cxx:287-430)
   double nu, irho, hxi, mass, imass, moi, imoi, a_r_cutoff;
   // kernel timing
@@ -259,5 +309,26 @@ void pnet_warm_start(kapsel_ctx *ctx, double beta, const double e0[3], double sc
 void calc_f_hydro(kapsel_ctx *ctx, const kapsel_time &t);
 void force(kapsel_ctx *ctx);
 void md_velocity_integrate(kapsel_ctx *ctx, const kapsel_time &t);
+// Make_phi_qq_particle before its normalisation: unclamped profile sum into
+// phi, sum of Q_e DPhi_compact_sin into surface (particles.hip)
+void make_phi_qq_raw(kapsel_ctx *ctx, void *phi, void *surface, double q_e);
+// Make_surface_normal into n3[0..2] (particles.hip)
+void make_surface_normal(kapsel_ctx *ctx, void *const *n3);
+// NSsolute_solver_Euler's vorticity update: U2advection_k combination and
+// the forward Euler step with the viscous term on the kept modes (kspace.hip)
+void advection_combine_euler(kapsel_ctx *ctx, double dt_fluid);
+
+// electrolyte (electrolyte.hip)
+void electrolyte_set(kapsel_ctx *ctx, const kapsel_electrolyte &el);
+void electrolyte_free(kapsel_ctx *ctx);
+void init_rho_ion(kapsel_ctx *ctx);
+void make_phi_qq(kapsel_ctx *ctx, void *phi, void *surface);
+void charge_field(kapsel_ctx *ctx, void *out);
+void nssolute_euler(kapsel_ctx *ctx, const kapsel_time &t);
+void rescale_solute(kapsel_ctx *ctx, double *factor_host);
+void coulomb_force(kapsel_ctx *ctx, const kapsel_time &t);
+// Rescale_solute right after the profile scatter (PHI_SUM current) and the
+// Coulomb force of the rescaled ions into F0..F2
+void rescale_and_coulomb_force(kapsel_ctx *ctx, const kapsel_time &t);
 
 }  // namespace kapsel

@@ -1,0 +1,608 @@
+// electrolyte.hip -- two ion species: Nernst-Planck transport, the Coulomb
+// field of particles and ions, and the Coulomb force on the fluid.
+//
+// The reference's electrolyte step (fluid_solver.cxx:184-338,
+// operate_electrolyte.cxx:45-242, solute_rhs.cxx:42-233) is a chain of
+// whole-field loops, each reading and writing full fields.  Here the chain is
+// regrouped into a few passes, each reading its inputs once:
+//   charge      rho = surface * s + sum_n z_n e c_n (1 - phi_qq)  (one pass,
+//               s the surface normalisation computed on the device)
+//   potential   psi = rho_k / (eps k^2) (optionally 2/3-cut), grad = i k psi
+//               and the applied-field shift of the k = 0 mode, one pass
+//               producing the three gradient spectra
+//   flux (k)    per species: the 2/3 cut of c_k, kBT L i k c_k and c_k for the
+//               inverse transforms, one pass
+//   flux (x)    electromigration, impermeable band, advection, one pass
+//   update      divergence of the flux spectrum and the Euler step on the
+//               kept modes, one pass
+// Reductions (surface normalisation, ion amounts) accumulate in FP64 with a
+// fixed block count, so their sums are reproducible.
+#include <cmath>
+
+#include "common.hpp"
+
+namespace kapsel {
+
+namespace {
+
+constexpr unsigned TPB = 256;
+constexpr int NBR = 1024;  // blocks of a reduction (fixed: deterministic sums)
+
+// device scalars
+enum { S_SURF = 0, S_F0, S_F1, S_C0, S_C1, S_VOL, S_N };
+// scratch fields
+enum { E_N0 = 0, E_N1, E_N2, E_QQ, E_SURF, E_CX0, E_CX1, E_RHOK, E_RHO, E_NF };
+
+__device__ __forceinline__ double block_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+  __shared__ double w[TPB / 64];
+  const int lane = threadIdx.x % 64, wid = threadIdx.x / 64;
+  __syncthreads();
+  if (lane == 0) w[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(TPB / 64); i++) s += w[i];
+  return s;  // valid in thread 0
+}
+
+// OP 0: sum |a|                          (Make_phi_qq_particle, :218-229)
+// OP 1: sums (1 - phi) a, (1 - phi) b    (Count_single_solute, solute_rhs.cxx:130-148)
+// OP 2: sum (1 - phi)                    (Set_uniform_ion_charge_density_salt, :581-591)
+template <typename T, int OP>
+__global__ __launch_bounds__(TPB) void reduce_kernel(const T *__restrict__ a, const T *__restrict__ b,
+                                                     const T *__restrict__ phi, double *__restrict__ partial,
+                                                     Grid g) {
+  double s0 = 0.0, s1 = 0.0;
+  for (size_t r = (size_t)blockIdx.x * TPB + threadIdx.x; r < g.nr; r += (size_t)gridDim.x * TPB) {
+    const RI p = ridx(g, r);
+    if constexpr (OP == 0) {
+      s0 += fabs((double)a[p.im]);
+    } else if constexpr (OP == 1) {
+      const double w = 1. - (double)phi[p.im];
+      s0 += w * (double)a[p.im];
+      s1 += w * (double)b[p.im];
+    } else {
+      s0 += 1. - (double)phi[p.im];
+    }
+  }
+  s0 = block_sum(s0);
+  if constexpr (OP == 1) s1 = block_sum(s1);
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = s0;
+    partial[NBR + blockIdx.x] = s1;
+  }
+}
+
+// MODE 0: scal[S_SURF] = abs_total / (sum dx^3)     (Make_phi_qq_particle, :230)
+// MODE 1: counts c_n = sum_n dx^3, factors total_n / c_n   (Rescale_solute, :117-118)
+// MODE 2: scal[S_VOL] = sum
+__global__ __launch_bounds__(TPB) void finish_kernel(const double *__restrict__ partial, double *__restrict__ scal,
+                                                     int mode, double dx3, double abs_total, double t0,
+                                                     double t1) {
+  double s0 = 0.0, s1 = 0.0;
+  for (int i = threadIdx.x; i < NBR; i += TPB) {
+    s0 += partial[i];
+    s1 += partial[NBR + i];
+  }
+  s0 = block_sum(s0);
+  s1 = block_sum(s1);
+  if (threadIdx.x != 0) return;
+  if (mode == 0) {
+    scal[S_SURF] = abs_total / (s0 * dx3);
+  } else if (mode == 1) {
+    const double c0 = s0 * dx3, c1 = s1 * dx3;
+    scal[S_C0] = c0;
+    scal[S_C1] = c1;
+    scal[S_F0] = t0 / c0;
+    scal[S_F1] = t1 / c1;
+  } else {
+    scal[S_VOL] = s0;
+  }
+}
+
+// a *= scal[slot] over the real points (the surface normalisation, :231-240)
+template <typename T>
+__global__ void scale_real_kernel(T *__restrict__ a, const double *__restrict__ scal, int slot, Grid g) {
+  size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= g.nr) return;
+  const RI p = ridx(g, r);
+  a[p.im] *= (T)scal[slot];
+}
+
+// c_k *= factor over the whole spectrum (Rescale_solute, :119-126); the real
+// copy of the same species follows, so the Coulomb force can reuse it
+template <typename T>
+__global__ void rescale_kernel(T *__restrict__ c0k, T *__restrict__ c1k, const double *__restrict__ scal, Grid g) {
+  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= g.f) return;
+  c0k[e] *= (T)scal[S_F0];
+  c1k[e] *= (T)scal[S_F1];
+}
+
+// Conc_k2charge_field (operate_electrolyte.cxx:45-74): the normalised surface
+// charge plus sum_n Valency_e[n] c_n (1 - phi_qq); with scaled, the ion
+// fields are multiplied by the rescale factors first (they hold the inverse
+// transforms of the spectra before Rescale_solute scaled them).  The result
+// goes to out (to be transformed) and, when keep is set, to keep as well.
+template <typename T>
+__global__ void charge_kernel(const T *__restrict__ qq, const T *__restrict__ surf, const T *__restrict__ c0,
+                              const T *__restrict__ c1, const double *__restrict__ scal, double ze0, double ze1,
+                              int scaled, T *__restrict__ out, T *__restrict__ keep, Grid g) {
+  size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= g.nr) return;
+  const RI p = ridx(g, r);
+  const T one_phi = (T)1 - qq[p.im];
+  T v0 = c0[p.im], v1 = c1[p.im];
+  if (scaled) {
+    v0 *= (T)scal[S_F0];
+    v1 *= (T)scal[S_F1];
+  }
+  T rho = surf[p.im] * (T)scal[S_SURF];
+  rho += (T)ze0 * v0 * one_phi;
+  rho += (T)ze1 * v1 * one_phi;
+  out[p.im] = rho;
+  if (keep) keep[p.im] = rho;
+}
+
+// Charge_field_k2Coulomb_potential_k_PBC (operate_electrolyte.cxx:76-88),
+// optionally Truncate_two_third_rule, A_k2da_k (fft_wrapper.h:755-759) and
+// Add_constant_field_k (fluid_solver.cxx:231-244, dc = E_ext amp, 0 for the
+// force): g_d = i k_d psi, g_d(k=0) -= dc_d
+template <typename T>
+__global__ void coulomb_grad_kernel(const T *__restrict__ rhok, T *__restrict__ g0, T *__restrict__ g1,
+                                    T *__restrict__ g2, double idc, int trunc, double dc0, double dc1, double dc2,
+                                    Grid g) {
+  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  const KIdx k = kidx(g, c);
+  const double wx = k.kx * g.wave_x, wy = k.ky * g.wave_y, wz = k.kz * g.wave_z;
+  const double k2 = wx * wx + wy * wy + wz * wz;
+  const double ik2 = k2 > 0.0 ? 1. / k2 : 0.0;  // IK2 of Init_K (fft_wrapper.cxx:71-80)
+  V2<T> a = ld2(rhok + 2 * c);
+  const T s = (T)(ik2 * idc);
+  a = V2<T>{a.re * s, a.im * s};
+  if (trunc && is_cut(g, k)) a = V2<T>{0, 0};
+  V2<T> d0{-(T)wx * a.im, (T)wx * a.re}, d1{-(T)wy * a.im, (T)wy * a.re}, d2{-(T)wz * a.im, (T)wz * a.re};
+  if (c == 0) {
+    d0.re -= (T)dc0;
+    d1.re -= (T)dc1;
+    d2.re -= (T)dc2;
+  }
+  st2(g0 + 2 * c, d0);
+  st2(g1 + 2 * c, d1);
+  st2(g2 + 2 * c, d2);
+}
+
+// Per species, k-space half of the flux: Truncate_two_third_rule of c_k
+// (Rhs_NS_solute, fluid_solver.cxx:209), Diffusion_flux_single's
+// i k kBT L c_k (solute_rhs.cxx:212-233), and c_k for A_k2a_out (:211)
+template <typename T>
+__global__ void flux_k_kernel(T *__restrict__ ck, T *__restrict__ f0, T *__restrict__ f1, T *__restrict__ f2,
+                              T *__restrict__ cx, double kl, Grid g) {
+  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  const KIdx k = kidx(g, c);
+  V2<T> a = ld2(ck + 2 * c);
+  if (is_cut(g, k)) {
+    a = V2<T>{0, 0};
+    st2(ck + 2 * c, a);
+  }
+  const V2<T> w{(T)kl * a.re, (T)kl * a.im};
+  const T wx = (T)(k.kx * g.wave_x), wy = (T)(k.ky * g.wave_y), wz = (T)(k.kz * g.wave_z);
+  st2(f0 + 2 * c, V2<T>{-wx * w.im, wx * w.re});
+  st2(f1 + 2 * c, V2<T>{-wy * w.im, wy * w.re});
+  st2(f2 + 2 * c, V2<T>{-wz * w.im, wz * w.re});
+  st2(cx + 2 * c, a);
+}
+
+// Per species, real-space half of the flux: Solute_solver_rhs_nonlinear_x_single
+// (solute_rhs.cxx:150-188), Solute_impermeability (:42-97) on every point
+// whose surface normal is nonzero (exactly the points the reference's
+// particle loop projects; the projection is idempotent, so points of two
+// particles' cells need it once), Add_advection_flux (:190-210)
+template <typename T>
+__global__ void flux_x_kernel(T *__restrict__ f0, T *__restrict__ f1, T *__restrict__ f2, const T *__restrict__ cx,
+                              const T *__restrict__ g0, const T *__restrict__ g1, const T *__restrict__ g2,
+                              const T *__restrict__ n0, const T *__restrict__ n1, const T *__restrict__ n2,
+                              const T *__restrict__ u0, const T *__restrict__ u1, const T *__restrict__ u2,
+                              double onsager, double ze, Grid g) {
+  size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= g.nr) return;
+  const size_t im = ridx(g, r).im;
+  const T c = cx[im];
+  const T L = (T)onsager, z = (T)ze;
+  T a0 = f0[im] + c * (L * (g0[im] * z));
+  T a1 = f1[im] + c * (L * (g1[im] * z));
+  T a2 = f2[im] + c * (L * (g2[im] * z));
+  const T m0 = n0[im], m1 = n1[im], m2 = n2[im];
+  const T norm2 = m0 * m0 + m1 * m1 + m2 * m2;
+  if (norm2 > (T)0) {
+    const T dmy = (a0 * m0 + a1 * m1 + a2 * m2) / norm2;
+    a0 -= m0 * dmy;
+    a1 -= m1 * dmy;
+    a2 -= m2 * dmy;
+  }
+  f0[im] = a0 + c * -u0[im];
+  f1[im] = a1 + c * -u1[im];
+  f2[im] = a2 + c * -u2[im];
+}
+
+// U_k2divergence_k (fft_wrapper.cxx:896-920) of the flux spectrum and
+// Field_solver_Euler (fluid_solver.cxx:121-136) of c_k on the kept modes
+// (the cut modes were zeroed by flux_k_kernel)
+template <typename T>
+__global__ void div_euler_kernel(const T *__restrict__ f0, const T *__restrict__ f1, const T *__restrict__ f2,
+                                 T *__restrict__ ck, double dt, Grid g) {
+  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  const KIdx k = kidx(g, c);
+  if (is_cut(g, k)) return;
+  const T kx = (T)(k.kx * g.wave_x), ky = (T)(k.ky * g.wave_y), kz = (T)(k.kz * g.wave_z);
+  const V2<T> a = ld2(f0 + 2 * c), b = ld2(f1 + 2 * c), e = ld2(f2 + 2 * c);
+  const V2<T> div{-(kx * a.im + ky * b.im + kz * e.im), (kx * a.re + ky * b.re + kz * e.re)};
+  V2<T> v = ld2(ck + 2 * c);
+  v.re += (T)dt * div.re;
+  v.im += (T)dt * div.im;
+  st2(ck + 2 * c, v);
+}
+
+// Make_Coulomb_force_x_on_fluid, last loops (operate_electrolyte.cxx:143-176):
+// f_d = rho (-d_d psi + E_d)
+template <typename T>
+__global__ void coulomb_force_kernel(T *__restrict__ f0, T *__restrict__ f1, T *__restrict__ f2,
+                                     const T *__restrict__ rho, double e0, double e1, double e2, Grid g) {
+  size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= g.nr) return;
+  const size_t im = ridx(g, r).im;
+  const T q = rho[im];
+  T v = -f0[im];
+  v += (T)e0;
+  f0[im] = q * v;
+  v = -f1[im];
+  v += (T)e1;
+  f1[im] = q * v;
+  v = -f2[im];
+  v += (T)e2;
+  f2[im] = q * v;
+}
+
+// uniform ion densities on the real points, 0 in the row padding
+// (Set_uniform_ion_charge_density_salt, :604-613)
+template <typename T>
+__global__ void fill_kernel(T *__restrict__ c0, T *__restrict__ c1, double v0, double v1, Grid g) {
+  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= g.f) return;
+  const bool real = (int)(e % (unsigned)g.nz_) < g.nz;
+  c0[e] = real ? (T)v0 : (T)0;
+  c1[e] = real ? (T)v1 : (T)0;
+}
+
+double kept_count_el(const Grid &g) { return (double)(2 * g.trn_x - 1) * (2 * g.trn_y - 1) * g.trn_z; }
+double cut_count_el(const Grid &g) { return (double)g.nc - kept_count_el(g); }
+
+template <typename T>
+T *fl(kapsel_ctx *ctx, int f) {
+  return (T *)ctx->fields[f];
+}
+template <typename T>
+T *ef(kapsel_ctx *ctx, int f) {
+  return (T *)ctx->el_f[f];
+}
+
+double valency_e(const kapsel_ctx *ctx, int n) { return ctx->el.valency[n] * ctx->el.elementary_charge; }
+double ac_factor(const kapsel_ctx *ctx, const kapsel_time &t) {
+  return ctx->el.ac ? sin(ctx->el.angular_frequency * t.time) : 1.0;
+}
+
+void need_el(const kapsel_ctx *ctx) {
+  if (!ctx->el_set) {
+    set_error("electrolyte parameters not set (kapsel_set_electrolyte)");
+    throw Error{KAPSEL_ERR_STATE};
+  }
+}
+
+template <typename T, int OP>
+void reduce(kapsel_ctx *ctx, const void *a, const void *b, const void *phi) {
+  const double fields = OP == 1 ? 3.0 : 1.0;
+  LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, fields * ctx->g.nr * sizeof(T));
+  hipLaunchKernelGGL((reduce_kernel<T, OP>), dim3(NBR), dim3(TPB), 0, ctx->stream, (const T *)a, (const T *)b,
+                     (const T *)phi, ctx->el_partial, ctx->g);
+  KAPSEL_LAUNCH_CHECK();
+}
+
+void finish(kapsel_ctx *ctx, int mode, double abs_total = 0.0, double t0 = 0.0, double t1 = 0.0) {
+  const double dx3 = ctx->g.dx * ctx->g.dx * ctx->g.dx;
+  hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->el_partial, ctx->el_scal, mode, dx3,
+                     abs_total, t0, t1);
+  KAPSEL_LAUNCH_CHECK();
+}
+
+double read_scal(kapsel_ctx *ctx, int slot) {
+  double v = 0.0;
+  KAPSEL_HIP(hipMemcpyAsync(&v, ctx->el_scal + slot, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  KAPSEL_HIP(hipStreamSynchronize(ctx->stream));
+  return v;
+}
+
+// Make_phi_qq_particle (operate_electrolyte.cxx:179-242) without its final
+// scaling pass: the profile sum and the raw surface charge, then the
+// normalisation factor into scal[S_SURF]
+template <typename T>
+void phi_qq_t(kapsel_ctx *ctx, T *phi, T *surf) {
+  const double qe = ctx->el.surface_charge * ctx->el.elementary_charge;
+  make_phi_qq_raw(ctx, phi, surf, qe);
+  reduce<T, 0>(ctx, surf, nullptr, nullptr);
+  finish(ctx, 0, ctx->P.n_particles * fabs(qe));
+}
+
+// ion concentrations in real space: E_CX0/E_CX1 = c2r(CONC0/CONC1)
+template <typename T>
+void ions_real_t(kapsel_ctx *ctx) {
+  copy_field(ctx, ctx->el_f[E_CX0], ctx->fields[KAPSEL_CONC0]);
+  copy_field(ctx, ctx->el_f[E_CX1], ctx->fields[KAPSEL_CONC1]);
+  void *two[2] = {ctx->el_f[E_CX0], ctx->el_f[E_CX1]};
+  fft_c2r_t<T>(ctx, two, 2);
+}
+
+// Conc_k2charge_field into out (and keep); the ion fields must hold the
+// inverse transforms already (scaled: times the rescale factors)
+template <typename T>
+void charge_t(kapsel_ctx *ctx, T *out, T *keep, int scaled) {
+  phi_qq_t<T>(ctx, ef<T>(ctx, E_QQ), ef<T>(ctx, E_SURF));
+  LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, (keep ? 6.0 : 5.0) * ctx->g.nr * sizeof(T));
+  hipLaunchKernelGGL(charge_kernel<T>, dim3(blocks_for(ctx->g.nr, TPB)), dim3(TPB), 0, ctx->stream,
+                     ef<T>(ctx, E_QQ), ef<T>(ctx, E_SURF), ef<T>(ctx, E_CX0), ef<T>(ctx, E_CX1), ctx->el_scal,
+                     valency_e(ctx, 0), valency_e(ctx, 1), scaled, out, keep, ctx->g);
+  KAPSEL_LAUNCH_CHECK();
+}
+
+template <typename T>
+void coulomb_grad_t(kapsel_ctx *ctx, const T *rhok, T *const *g3, int trunc, const double dc[3]) {
+  LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, 4.0 * ctx->g.f * sizeof(T));
+  hipLaunchKernelGGL(coulomb_grad_kernel<T>, dim3(blocks_for(ctx->g.nc, TPB)), dim3(TPB), 0, ctx->stream, rhok,
+                     g3[0], g3[1], g3[2], 1. / ctx->el.dielectric_cst, trunc, dc[0], dc[1], dc[2], ctx->g);
+  KAPSEL_LAUNCH_CHECK();
+}
+
+// Rhs_NS_Nernst_Planck + Rhs_NS_solute + Field_solver_Euler
+// (fluid_solver.cxx:184-338), non-Dielectric branch.  Buffers: grad_potential
+// in F0..F2 and solute_flux in UP0..UP2 as in the reference's call
+// (:328), the real c_n in WORK2, the right-hand side of zeta in WORK0/WORK1.
+template <typename T>
+void nssolute_t(kapsel_ctx *ctx, const kapsel_time &t) {
+  const Grid &g = ctx->g;
+  // charge density of the current state (concentrations not yet cut)
+  ions_real_t<T>(ctx);
+  T *rhok = ef<T>(ctx, E_RHOK);
+  charge_t<T>(ctx, rhok, nullptr, 0);
+  void *one[1] = {rhok};
+  fft_r2c_t<T>(ctx, one, 1);
+  const double amp = (double)(g.nx * g.ny * g.nz) * ac_factor(ctx, t);
+  const double dc[3] = {ctx->el.e_ext[0] * amp, ctx->el.e_ext[1] * amp, ctx->el.e_ext[2] * amp};
+  T *grad[3] = {fl<T>(ctx, KAPSEL_F0), fl<T>(ctx, KAPSEL_F1), fl<T>(ctx, KAPSEL_F2)};
+  coulomb_grad_t<T>(ctx, rhok, grad, 1, dc);
+  fft_c2r_t<T>(ctx, (void *const *)grad, 3);
+  // u of the truncated zeta (Rhs_NS_solute :203-207)
+  zeta_k2u_k(ctx, 0);
+  fft_c2r_t<T>(ctx, ctx->fields + KAPSEL_U0, 3);
+  make_surface_normal(ctx, ctx->el_f + E_N0);
+  T *flux[3] = {fl<T>(ctx, KAPSEL_UP0), fl<T>(ctx, KAPSEL_UP1), fl<T>(ctx, KAPSEL_UP2)};
+  T *cx = fl<T>(ctx, KAPSEL_WORK2);
+  for (int n = 0; n < 2; n++) {
+    T *ck = fl<T>(ctx, KAPSEL_CONC0 + n);
+    {
+      LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, (2.0 * g.f + 2.0 * cut_count_el(g) + 3.0 * g.f) * sizeof(T));
+      hipLaunchKernelGGL(flux_k_kernel<T>, dim3(blocks_for(g.nc, TPB)), dim3(TPB), 0, ctx->stream, ck, flux[0],
+                         flux[1], flux[2], cx, ctx->el.kbt * ctx->el.onsager_coeff[n], g);
+      KAPSEL_LAUNCH_CHECK();
+    }
+    void *four[4] = {flux[0], flux[1], flux[2], cx};
+    fft_c2r_t<T>(ctx, four, 4);
+    {
+      LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, 16.0 * g.nr * sizeof(T));
+      hipLaunchKernelGGL(flux_x_kernel<T>, dim3(blocks_for(g.nr, TPB)), dim3(TPB), 0, ctx->stream, flux[0],
+                         flux[1], flux[2], cx, grad[0], grad[1], grad[2], ef<T>(ctx, E_N0), ef<T>(ctx, E_N1),
+                         ef<T>(ctx, E_N2), fl<T>(ctx, KAPSEL_U0), fl<T>(ctx, KAPSEL_U1), fl<T>(ctx, KAPSEL_U2),
+                         ctx->el.onsager_coeff[n], valency_e(ctx, n), g);
+      KAPSEL_LAUNCH_CHECK();
+    }
+    fft_r2c_t<T>(ctx, (void *const *)flux, 3);
+    {
+      LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, 5.0 * 2.0 * kept_count_el(g) * sizeof(T));
+      hipLaunchKernelGGL(div_euler_kernel<T>, dim3(blocks_for(g.nc, TPB)), dim3(TPB), 0, ctx->stream, flux[0],
+                         flux[1], flux[2], ck, t.dt_fluid, g);
+      KAPSEL_LAUNCH_CHECK();
+    }
+  }
+  // Rhs_solvent (U2advection_k) + viscous term + Euler step of zeta
+  advection_products(ctx);
+  void *five[5] = {ctx->fields[KAPSEL_U0], ctx->fields[KAPSEL_U1], ctx->fields[KAPSEL_U2],
+                   ctx->fields[KAPSEL_WORK0], ctx->fields[KAPSEL_WORK1]};
+  fft_r2c_t<T>(ctx, five, 5);
+  advection_combine_euler(ctx, t.dt_fluid);
+}
+
+// Rescale_solute (solute_rhs.cxx:99-128).  PHI becomes the unclamped profile
+// sum, as the reference's Make_phi_particle leaves it; at the positions of
+// the last profile scatter that is PHI_SUM exactly (same kernel, same sums)
+// when phi_sum_current is set.  The ion fields keep the unscaled inverse
+// transforms; charge_t(scaled = 1) applies the factors.
+template <typename T>
+void rescale_t(kapsel_ctx *ctx, bool phi_sum_current) {
+  if (phi_sum_current) copy_field(ctx, ctx->fields[KAPSEL_PHI], ctx->fields[KAPSEL_PHI_SUM]);
+  else make_phi_particle(ctx, ctx->fields[KAPSEL_PHI]);
+  ions_real_t<T>(ctx);
+  reduce<T, 1>(ctx, ctx->el_f[E_CX0], ctx->el_f[E_CX1], ctx->fields[KAPSEL_PHI]);
+  finish(ctx, 1, 0.0, ctx->el_total[0], ctx->el_total[1]);
+  LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, 4.0 * ctx->g.f * sizeof(T));
+  hipLaunchKernelGGL(rescale_kernel<T>, dim3(blocks_for(ctx->g.f, TPB)), dim3(TPB), 0, ctx->stream,
+                     fl<T>(ctx, KAPSEL_CONC0), fl<T>(ctx, KAPSEL_CONC1), ctx->el_scal, ctx->g);
+  KAPSEL_LAUNCH_CHECK();
+}
+
+// Make_Coulomb_force_x_on_fluid (operate_electrolyte.cxx:90-177), free
+// particles; scaled: the ion fields hold the transforms from before the last
+// rescale (see rescale_t), otherwise they are recomputed
+template <typename T>
+void coulomb_force_t(kapsel_ctx *ctx, const kapsel_time &t, int scaled) {
+  if (!scaled) ions_real_t<T>(ctx);
+  T *rhok = ef<T>(ctx, E_RHOK), *rho = ef<T>(ctx, E_RHO);
+  charge_t<T>(ctx, rhok, rho, scaled);
+  void *one[1] = {rhok};
+  fft_r2c_t<T>(ctx, one, 1);
+  T *f[3] = {fl<T>(ctx, KAPSEL_F0), fl<T>(ctx, KAPSEL_F1), fl<T>(ctx, KAPSEL_F2)};
+  const double zero[3] = {0.0, 0.0, 0.0};
+  coulomb_grad_t<T>(ctx, rhok, f, 0, zero);
+  fft_c2r_t<T>(ctx, (void *const *)f, 3);
+  const double s = ac_factor(ctx, t);
+  LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, 7.0 * ctx->g.nr * sizeof(T));
+  hipLaunchKernelGGL(coulomb_force_kernel<T>, dim3(blocks_for(ctx->g.nr, TPB)), dim3(TPB), 0, ctx->stream, f[0],
+                     f[1], f[2], rho, ctx->el.e_ext[0] * s, ctx->el.e_ext[1] * s, ctx->el.e_ext[2] * s, ctx->g);
+  KAPSEL_LAUNCH_CHECK();
+}
+
+// Init_rho_ion, N_spec = 2, uniform salt (operate_electrolyte.cxx:559-621, 860)
+template <typename T>
+void init_rho_ion_t(kapsel_ctx *ctx) {
+  const kapsel_electrolyte &e = ctx->el;
+  make_phi_particle(ctx, ctx->fields[KAPSEL_PHI]);
+  reduce<T, 2>(ctx, nullptr, nullptr, ctx->fields[KAPSEL_PHI]);
+  finish(ctx, 2);
+  const double volume_phi = read_scal(ctx, S_VOL);
+  double positive_ion_number = 0., negative_ion_number = 0.;
+  const double q = e.surface_charge, npart = (double)ctx->P.n_particles;
+  if (q < 0.) positive_ion_number += fabs(q * npart / e.valency[0]);
+  else if (q > 0.) negative_ion_number += fabs(q * npart / e.valency[1]);
+  const double dx3 = ctx->g.dx * ctx->g.dx * ctx->g.dx;
+  const double pd = positive_ion_number / (volume_phi * dx3);
+  const double nd = negative_ion_number / (volume_phi * dx3);
+  const double rho_inf = e.kbt * e.dielectric_cst / ((e.elementary_charge * e.debye_length) *
+                                                      (e.elementary_charge * e.debye_length));
+  const double rp = rho_inf / (e.valency[0] * (e.valency[0] - e.valency[1]));
+  const double rn = -rho_inf / (e.valency[1] * (e.valency[0] - e.valency[1]));
+  {
+    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, 2.0 * ctx->g.f * sizeof(T));
+    hipLaunchKernelGGL(fill_kernel<T>, dim3(blocks_for(ctx->g.f, TPB)), dim3(TPB), 0, ctx->stream,
+                       fl<T>(ctx, KAPSEL_CONC0), fl<T>(ctx, KAPSEL_CONC1), pd + rp, nd + rn, ctx->g);
+    KAPSEL_LAUNCH_CHECK();
+  }
+  void *two[2] = {ctx->fields[KAPSEL_CONC0], ctx->fields[KAPSEL_CONC1]};
+  fft_r2c_t<T>(ctx, two, 2);
+  // Count_solute_each (solute_rhs.h:143-156): the totals are the amounts of
+  // the transformed fields; PHI already holds Make_phi_particle
+  ions_real_t<T>(ctx);
+  reduce<T, 1>(ctx, ctx->el_f[E_CX0], ctx->el_f[E_CX1], ctx->fields[KAPSEL_PHI]);
+  finish(ctx, 1, 0.0, 1.0, 1.0);
+  ctx->el_total[0] = read_scal(ctx, S_C0);
+  ctx->el_total[1] = read_scal(ctx, S_C1);
+}
+
+#define EL_DISPATCH(ctx, FN, ...) \
+  ((ctx)->prec == KAPSEL_FP64 ? FN<double>(__VA_ARGS__) : FN<float>(__VA_ARGS__))
+
+}  // namespace
+
+void electrolyte_set(kapsel_ctx *ctx, const kapsel_electrolyte &el) {
+  if (ctx->P.n_particles < 1) {
+    set_error("the electrolyte needs particles: the surface charge is normalised by their total "
+              "(Make_phi_qq_particle)");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+  if (el.surface_charge == 0.0 || !(el.valency[0] > 0.0) || !(el.valency[1] < 0.0) || !(el.kbt > 0.0) ||
+      !(el.dielectric_cst > 0.0) || !(el.debye_length > 0.0) || !(el.elementary_charge > 0.0) ||
+      !(el.onsager_coeff[0] >= 0.0) || !(el.onsager_coeff[1] >= 0.0)) {
+    set_error("invalid electrolyte parameters (nonzero surface charge, valencies +/-, positive kBT, "
+              "Dielectric_cst, Debye length, elementary charge; Onsager coefficients >= 0)");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+  if (!ctx->el_f[0]) {
+    for (int i = 0; i < E_NF; i++) {
+      KAPSEL_HIP(hipMalloc(&ctx->el_f[i], ctx->g.f * ctx->elem));
+      KAPSEL_HIP(hipMemsetAsync(ctx->el_f[i], 0, ctx->g.f * ctx->elem, ctx->stream));
+    }
+    KAPSEL_HIP(hipMalloc(&ctx->el_partial, sizeof(double) * 2 * NBR));
+    KAPSEL_HIP(hipMalloc(&ctx->el_scal, sizeof(double) * S_N));
+    KAPSEL_HIP(hipMemsetAsync(ctx->el_scal, 0, sizeof(double) * S_N, ctx->stream));
+  }
+  ctx->el = el;
+  ctx->el_set = true;
+}
+
+void electrolyte_free(kapsel_ctx *ctx) {
+  for (void *&p : ctx->el_f)
+    if (p) {
+      (void)hipFree(p);
+      p = nullptr;
+    }
+  if (ctx->el_partial) (void)hipFree(ctx->el_partial);
+  if (ctx->el_scal) (void)hipFree(ctx->el_scal);
+  ctx->el_partial = ctx->el_scal = nullptr;
+}
+
+void init_rho_ion(kapsel_ctx *ctx) {
+  need_el(ctx);
+  EL_DISPATCH(ctx, init_rho_ion_t, ctx);
+}
+
+template <typename T>
+static void make_phi_qq_t(kapsel_ctx *ctx, void *phi, void *surface) {
+  phi_qq_t<T>(ctx, (T *)phi, (T *)surface);
+  LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, 2.0 * ctx->g.nr * sizeof(T));
+  hipLaunchKernelGGL(scale_real_kernel<T>, dim3(blocks_for(ctx->g.nr, TPB)), dim3(TPB), 0, ctx->stream,
+                     (T *)surface, ctx->el_scal, (int)S_SURF, ctx->g);
+  KAPSEL_LAUNCH_CHECK();
+}
+void make_phi_qq(kapsel_ctx *ctx, void *phi, void *surface) {
+  need_el(ctx);
+  EL_DISPATCH(ctx, make_phi_qq_t, ctx, phi, surface);
+}
+
+template <typename T>
+static void charge_field_t(kapsel_ctx *ctx, void *out) {
+  ions_real_t<T>(ctx);
+  charge_t<T>(ctx, (T *)out, nullptr, 0);
+}
+void charge_field(kapsel_ctx *ctx, void *out) {
+  need_el(ctx);
+  EL_DISPATCH(ctx, charge_field_t, ctx, out);
+}
+
+void nssolute_euler(kapsel_ctx *ctx, const kapsel_time &t) {
+  need_el(ctx);
+  EL_DISPATCH(ctx, nssolute_t, ctx, t);
+}
+
+template <typename T>
+static void rescale_solute_t(kapsel_ctx *ctx, double *factor_host) {
+  rescale_t<T>(ctx, false);
+  if (factor_host) {
+    factor_host[0] = read_scal(ctx, S_F0);
+    factor_host[1] = read_scal(ctx, S_F1);
+  }
+}
+void rescale_solute(kapsel_ctx *ctx, double *factor_host) {
+  need_el(ctx);
+  EL_DISPATCH(ctx, rescale_solute_t, ctx, factor_host);
+}
+
+void coulomb_force(kapsel_ctx *ctx, const kapsel_time &t) {
+  need_el(ctx);
+  EL_DISPATCH(ctx, coulomb_force_t, ctx, t, 0);
+}
+
+// the Electrolyte branch's middle part (sp_3d_ns.cxx:55-95): Rescale_solute
+// right after the profile scatter (PHI_SUM current), then the Coulomb force
+// from the rescaled ions
+void rescale_and_coulomb_force(kapsel_ctx *ctx, const kapsel_time &t) {
+  need_el(ctx);
+  if (ctx->prec == KAPSEL_FP64) {
+    rescale_t<double>(ctx, true);
+    coulomb_force_t<double>(ctx, t, 1);
+  } else {
+    rescale_t<float>(ctx, true);
+    coulomb_force_t<float>(ctx, t, 1);
+  }
+}
+
+}  // namespace kapsel

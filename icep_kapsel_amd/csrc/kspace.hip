@@ -12,30 +12,6 @@ namespace kapsel {
 
 namespace {
 
-struct KIdx {
-  int i, j, kz;
-  int kx, ky;  // signed wavenumbers
-};
-
-__device__ __forceinline__ KIdx kidx(const Grid &g, size_t c) {
-  KIdx r;
-  const unsigned plane = (unsigned)g.ny * (unsigned)g.h;
-  unsigned cc = (unsigned)c;
-  r.i = cc / plane;
-  unsigned rem = cc - r.i * plane;
-  r.j = rem / (unsigned)g.h;
-  r.kz = rem - r.j * g.h;
-  r.kx = r.i > g.hnx ? r.i - g.nx : r.i;
-  r.ky = r.j > g.hny ? r.j - g.ny : r.j;
-  return r;
-}
-
-// Truncate_two_third_rule_ooura (fft_wrapper.h:797-839): the cut set
-__device__ __forceinline__ bool is_cut(const Grid &g, const KIdx &k) {
-  return (k.kz >= g.trn_z) || (k.j >= g.trn_y && k.j <= g.ny - g.trn_y) ||
-         (k.i >= g.trn_x && k.i <= g.nx - g.trn_x);
-}
-
 template <typename T>
 struct P5 {
   T *p[5];
@@ -155,10 +131,16 @@ __global__ void advection_products_kernel(T *__restrict__ u0, T *__restrict__ u1
 }
 
 // U2advection_k k-space combination + Omega_k2zeta_k (operate_omega.cxx:56-97,
-// fft_wrapper.cxx:248-284) fused with the slaved-Euler update of
-// NS_solver_slavedEuler (fluid_solver.cxx:346-365) on the kept modes.
-// f_ns0 is left in a0/a1 as the reference leaves it in its global f_ns0.
-template <typename T>
+// fft_wrapper.cxx:248-284) fused with the update of zeta on the kept modes:
+// EULER 0: the slaved-Euler step of NS_solver_slavedEuler
+//          (fluid_solver.cxx:346-365), dmy0 = -NU dt, dmy1 = 1/NU;
+// EULER 1: Add_zeta_viscous_term (operate_omega.cxx:170-185) and the forward
+//          Euler step Field_solver_Euler (fluid_solver.cxx:121-136) of
+//          NSsolute_solver_Euler, dmy0 = NU, dmy1 = dt.
+// The advection term is left in a0/a1 as the reference leaves it in its
+// global f_ns0 (slaved); the Euler variant leaves the full right-hand side
+// there, as Rhs_NS_solute leaves it in f_ns1.
+template <typename T, int EULER>
 __global__ void advection_combine_slaved_kernel(const T *__restrict__ u0, const T *__restrict__ u1,
                                                 const T *__restrict__ u2, T *__restrict__ a0,
                                                 T *__restrict__ a1, T *__restrict__ z0, T *__restrict__ z1,
@@ -190,6 +172,21 @@ __global__ void advection_combine_slaved_kernel(const T *__restrict__ u0, const 
   } else {
     f0 = w[0];
     f1 = w[1];
+  }
+  if constexpr (EULER) {
+    if (!is_cut(g, k)) {
+      const double q1 = k.kx * g.wave_x, q2 = k.ky * g.wave_y, q3 = k.kz * g.wave_z;
+      const T nk2 = (T)(dmy0 * (q1 * q1 + q2 * q2 + q3 * q3));  // NU * K2[im]
+      const T dt = (T)dmy1;
+      V2<T> za = ld2(z0 + 2 * c), zb = ld2(z1 + 2 * c);
+      f0 = V2<T>{f0.re + -(nk2 * za.re), f0.im + -(nk2 * za.im)};
+      f1 = V2<T>{f1.re + -(nk2 * zb.re), f1.im + -(nk2 * zb.im)};
+      st2(z0 + 2 * c, V2<T>{za.re + dt * f0.re, za.im + dt * f0.im});
+      st2(z1 + 2 * c, V2<T>{zb.re + dt * f1.re, zb.im + dt * f1.im});
+    }
+    st2(a0 + 2 * c, f0);
+    st2(a1 + 2 * c, f1);
+    return;
   }
   st2(a0 + 2 * c, f0);
   st2(a1 + 2 * c, f1);
@@ -337,7 +334,7 @@ static void advection_combine_slaved_t(kapsel_ctx *ctx, double dt_fluid) {
   double dmy0 = -ctx->nu * dt_fluid;
   double dmy1 = 1. / ctx->nu;
   LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, (7.0 * ctx->g.f + 8.0 * kept_count(ctx->g)) * sizeof(T));
-  hipLaunchKernelGGL(advection_combine_slaved_kernel<T>, dim3(blocks_for(ctx->g.nc, TPB)), dim3(TPB), 0,
+  hipLaunchKernelGGL((advection_combine_slaved_kernel<T, 0>), dim3(blocks_for(ctx->g.nc, TPB)), dim3(TPB), 0,
                      ctx->stream, F<T>(ctx, KAPSEL_U0), F<T>(ctx, KAPSEL_U1), F<T>(ctx, KAPSEL_U2),
                      F<T>(ctx, KAPSEL_WORK0), F<T>(ctx, KAPSEL_WORK1), F<T>(ctx, KAPSEL_ZETA0),
                      F<T>(ctx, KAPSEL_ZETA1), ctx->g, dmy0, dmy1);
@@ -345,6 +342,19 @@ static void advection_combine_slaved_t(kapsel_ctx *ctx, double dt_fluid) {
 }
 void advection_combine_slaved(kapsel_ctx *ctx, double dt_fluid) {
   KAPSEL_DISPATCH(ctx, advection_combine_slaved_t, ctx, dt_fluid);
+}
+
+template <typename T>
+static void advection_combine_euler_t(kapsel_ctx *ctx, double dt_fluid) {
+  LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, (7.0 * ctx->g.f + 4.0 * kept_count(ctx->g)) * sizeof(T));
+  hipLaunchKernelGGL((advection_combine_slaved_kernel<T, 1>), dim3(blocks_for(ctx->g.nc, TPB)), dim3(TPB), 0,
+                     ctx->stream, F<T>(ctx, KAPSEL_U0), F<T>(ctx, KAPSEL_U1), F<T>(ctx, KAPSEL_U2),
+                     F<T>(ctx, KAPSEL_WORK0), F<T>(ctx, KAPSEL_WORK1), F<T>(ctx, KAPSEL_ZETA0),
+                     F<T>(ctx, KAPSEL_ZETA1), ctx->g, ctx->nu, dt_fluid);
+  KAPSEL_LAUNCH_CHECK();
+}
+void advection_combine_euler(kapsel_ctx *ctx, double dt_fluid) {
+  KAPSEL_DISPATCH(ctx, advection_combine_euler_t, ctx, dt_fluid);
 }
 
 template <typename T>

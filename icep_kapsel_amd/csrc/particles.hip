@@ -16,6 +16,7 @@
 // workgroup per particle walks the Sekibun table in reference order, FP64
 // compensated (TwoSum) partial sums per lane, then a wavefront reduction.
 #include <cfloat>
+#include <type_traits>
 
 #include "common.hpp"
 
@@ -26,6 +27,7 @@ namespace {
 struct Phys {
   double dx, idx, radius, hxi, xi;
   double eps_f, eps_p, dcst;  // eps_fluid, eps_particle_top, Dielectric_cst
+  double qe;                  // Surface_charge_e of the particle species (Make_phi_qq_particle)
   double l[3];
   int n[3];
   int np;
@@ -103,6 +105,13 @@ __device__ __forceinline__ M phi_sin_m(M x, M a, M hxi, M xi) {
 // MODE 3: phi*u_p with phi_sum read back (Make_u_particle_sum)
 // MODE 4: compact-sine sum and the permittivity built from it
 //         (Make_dielectric_field non-Janus branch, operate_dielectric.cxx:298-307)
+// MODE 5: unclamped smooth-profile sum and sum of Q_e DPhi_compact_sin, the
+//         surface charge before its normalisation (Make_phi_qq_particle,
+//         operate_electrolyte.cxx:179-224)
+// MODE 6: sum of the unit vectors particle -> point over the band
+//         |r - a| < xi/2 (Make_surface_normal, make_phi.cxx:17-61); computed
+//         in FP64 for every field precision, since the band test is a
+//         discontinuity a rounding difference could move
 // A workgroup owns a tile of 8 x ty x tz points (ty * tz = 256); each
 // thread walks 8 points along x.  The per-point arithmetic runs in the field
 // precision T; displacements are formed in FP64 from the FP64 positions.
@@ -111,7 +120,7 @@ __global__ __launch_bounds__(TP) void points_kernel(const double *__restrict__ p
                                                     T *__restrict__ out1, T *__restrict__ out2,
                                                     const T *__restrict__ phi_sum, Grid g, Phys p, int ty,
                                                     int tz) {
-  using M = T;
+  using M = typename std::conditional<MODE == 6, double, T>::type;
   constexpr int NV = MODE == 3 ? 9 : 3;
   __shared__ double cand[NV][TP];
   __shared__ int wcount[TP / 64];
@@ -195,6 +204,21 @@ __global__ __launch_bounds__(TP) void points_kernel(const double *__restrict__ p
         const M r = sqrt(r2);
         if constexpr (MODE == 2 || MODE == 4) {
           acc0[x] += phi_sin_m<M>(r, a, hxi, xi);
+        } else if constexpr (MODE == 6) {
+          if (fabs(r - a) < hxi) {
+            const M ir = (M)1 / r;
+            acc0[x] += rx * ir;
+            acc1[x] += ry * ir;
+            acc2[x] += rz * ir;
+          }
+        } else if constexpr (MODE == 5) {
+          acc0[x] += phi_m<M>(r, a, hxi, dxm);
+          // DPhi_compact_sin (profile.h:168-181)
+          const M dmy_x = a - r;
+          if (fabs(dmy_x) < hxi) {
+            const M ixi = (M)1 / xi;
+            acc1[x] += (M)p.qe * ((M)M_PI * (M)0.5 * ixi * cos((M)M_PI * dmy_x * ixi));
+          }
         } else {
           const M ph = phi_m<M>(r, a, hxi, dxm);
           if constexpr (MODE == 3) {
@@ -226,10 +250,13 @@ __global__ __launch_bounds__(TP) void points_kernel(const double *__restrict__ p
     if constexpr (MODE == 0) {
       out0[im] = (T)acc0[x];                                    // phi_sum
       out1[im] = (T)(acc0[x] < (M)1 ? acc0[x] : (M)1);          // phi = MIN(phi_sum, 1)
-    } else if constexpr (MODE == 3) {
+    } else if constexpr (MODE == 3 || MODE == 6) {
       out0[im] = (T)acc0[x];
       out1[im] = (T)acc1[x];
       out2[im] = (T)acc2[x];
+    } else if constexpr (MODE == 5) {
+      out0[im] = (T)acc0[x];
+      out1[im] = (T)acc1[x];
     } else if constexpr (MODE == 4) {
       out0[im] = (T)acc0[x];
       out1[im] = (T)((M)(p.eps_f * p.dcst) + (M)((p.eps_p - p.eps_f) * p.dcst) * acc0[x]);
@@ -501,18 +528,21 @@ Phys phys(kapsel_ctx *ctx) {
   p.eps_f = ctx->P.eps_fluid;
   p.eps_p = ctx->P.eps_particle;
   p.dcst = ctx->P.dielectric_cst;
+  p.qe = 0.0;
   return p;
 }
 
 template <typename T, int MODE>
-void launch_points(kapsel_ctx *ctx, void *o0, void *o1, void *o2, const void *phi_sum) {
+void launch_points(kapsel_ctx *ctx, void *o0, void *o1, void *o2, const void *phi_sum, double qe = 0.0) {
   const Grid &g = ctx->g;
   const int tz = g.nz >= 32 ? 32 : 16, ty = TP / tz;
   unsigned nb = (unsigned)((g.nx / TXL) * (g.ny / ty) * (g.nz / tz));
-  const double fields_moved = (MODE == 0 || MODE == 4) ? 2.0 : MODE == 3 ? 4.0 : 1.0;
+  const double fields_moved = (MODE == 0 || MODE == 4 || MODE == 5) ? 2.0 : (MODE == 3 || MODE == 6) ? 4.0 : 1.0;
   LaunchTimer lt(ctx, KAPSEL_K_PARTICLE, fields_moved * g.nr * sizeof(T));
+  Phys p = phys(ctx);
+  p.qe = qe;
   hipLaunchKernelGGL((points_kernel<T, MODE>), dim3(nb), dim3(TP), 0, ctx->stream, ctx->d_ps, (T *)o0, (T *)o1,
-                     (T *)o2, (const T *)phi_sum, g, phys(ctx), ty, tz);
+                     (T *)o2, (const T *)phi_sum, g, p, ty, tz);
   KAPSEL_LAUNCH_CHECK();
 }
 
@@ -559,6 +589,16 @@ void make_phi_particle(kapsel_ctx *ctx, void *field) {
 void make_phi_sin(kapsel_ctx *ctx, void *field) {
   if (ctx->prec == KAPSEL_FP64) launch_points<double, 2>(ctx, field, nullptr, nullptr, nullptr);
   else launch_points<float, 2>(ctx, field, nullptr, nullptr, nullptr);
+}
+
+void make_phi_qq_raw(kapsel_ctx *ctx, void *phi, void *surface, double q_e) {
+  if (ctx->prec == KAPSEL_FP64) launch_points<double, 5>(ctx, phi, surface, nullptr, nullptr, q_e);
+  else launch_points<float, 5>(ctx, phi, surface, nullptr, nullptr, q_e);
+}
+
+void make_surface_normal(kapsel_ctx *ctx, void *const *n3) {
+  if (ctx->prec == KAPSEL_FP64) launch_points<double, 6>(ctx, n3[0], n3[1], n3[2], nullptr);
+  else launch_points<float, 6>(ctx, n3[0], n3[1], n3[2], nullptr);
 }
 
 template <typename T>

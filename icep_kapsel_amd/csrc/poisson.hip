@@ -32,35 +32,6 @@ enum { S_BB = 0, S_RZ, S_PAP, S_RR, S_RZN, S_BETA, S_TMP, S_INVEPS, S_N };
 constexpr int NPART = 1024;  // blocks of the partial-sum reductions (fixed: deterministic sums)
 constexpr unsigned TPB = 256;
 
-struct KI {
-  int i, j, kz, kx, ky;
-};
-__device__ __forceinline__ KI kidx(const Grid &g, size_t c) {
-  KI r;
-  const unsigned plane = (unsigned)g.ny * (unsigned)g.h;
-  unsigned cc = (unsigned)c;
-  r.i = cc / plane;
-  unsigned rem = cc - r.i * plane;
-  r.j = rem / (unsigned)g.h;
-  r.kz = rem - r.j * g.h;
-  r.kx = r.i > g.hnx ? r.i - g.nx : r.i;
-  r.ky = r.j > g.hny ? r.j - g.ny : r.j;
-  return r;
-}
-
-struct RI {
-  int i, j, k;
-  size_t im;
-};
-__device__ __forceinline__ RI ridx(const Grid &g, size_t r) {
-  RI o;
-  size_t row = r / (unsigned)g.nz;
-  o.k = (int)(r - row * g.nz);
-  o.j = (int)(row % (unsigned)g.ny);
-  o.i = (int)(row / (unsigned)g.ny);
-  o.im = row * g.nz_ + o.k;
-  return o;
-}
 // extract_id (periodic_boundary.h:21-27) for +1 neighbours
 __device__ __forceinline__ size_t nbr(const Grid &g, int i, int j, int k) {
   i = i == g.nx ? 0 : i;
@@ -147,7 +118,7 @@ __global__ void grad_shift_kernel(const T *__restrict__ xk, T *__restrict__ g0, 
                                   const V2<T> *__restrict__ shz, Grid g) {
   size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.nc) return;
-  KI k = kidx(g, c);
+  KIdx k = kidx(g, c);
   V2<T> x = ld2(xk + 2 * c);
   const T w0 = (T)(k.kx * g.wave_x), w1 = (T)(k.ky * g.wave_y), w2 = (T)(k.kz * g.wave_z);
   st2(g0 + 2 * c, cmul(V2<T>{-w0 * x.im, w0 * x.re}, shx[k.i]));
@@ -195,7 +166,7 @@ __global__ void div_shift_kernel(const T *__restrict__ w0, const T *__restrict__
                                  const V2<T> *__restrict__ shy, const V2<T> *__restrict__ shz, Grid g) {
   size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.nc) return;
-  KI k = kidx(g, c);
+  KIdx k = kidx(g, c);
   const T q0 = (T)(k.kx * g.wave_x), q1 = (T)(k.ky * g.wave_y), q2 = (T)(k.kz * g.wave_z);
   V2<T> a = cmul(ld2(w0 + 2 * c), scal(q0, conjv(shx[k.i])));
   a = a + cmul(ld2(w1 + 2 * c), scal(q1, conjv(shy[k.j])));
@@ -229,7 +200,7 @@ template <typename T>
 __global__ void precond_scale_kernel(T *__restrict__ zk, double inv_eps, Grid g) {
   size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.nc) return;
-  KI k = kidx(g, c);
+  KIdx k = kidx(g, c);
   const double q0 = k.kx * g.wave_x, q1 = k.ky * g.wave_y, q2 = k.kz * g.wave_z;
   const double k2 = q0 * q0 + q1 * q1 + q2 * q2;
   const T s = k2 > 0 ? (T)(inv_eps / k2) : (T)0;
@@ -370,7 +341,7 @@ __device__ __forceinline__ double kweight(const Grid &g, int kz) {
 }
 
 template <typename T>
-__device__ __forceinline__ T precond_factor(const Grid &g, const KI &k, double inv_eps) {
+__device__ __forceinline__ T precond_factor(const Grid &g, const KIdx &k, double inv_eps) {
   const double q0 = k.kx * g.wave_x, q1 = k.ky * g.wave_y, q2 = k.kz * g.wave_z;
   const double k2 = q0 * q0 + q1 * q1 + q2 * q2;
   return k2 > 0 ? (T)(inv_eps / k2) : (T)0;
@@ -386,7 +357,7 @@ __global__ __launch_bounds__(TPB) void kcg_init_kernel(const T *__restrict__ b, 
   const double inv_eps = sc[S_INVEPS];
   double s_rz = 0.0, s_rr = 0.0;
   for (size_t c = (size_t)blockIdx.x * TPB + threadIdx.x; c < g.nc; c += (size_t)gridDim.x * TPB) {
-    const KI k = kidx(g, c);
+    const KIdx k = kidx(g, c);
     const V2<T> bv = ld2(b + 2 * c);
     const V2<T> rv{-bv.re, -bv.im};
     st2(r + 2 * c, rv);
@@ -418,7 +389,7 @@ __global__ void kcg_dir_grad_kernel(const T *z, T *p, const double *__restrict__
                                     const V2<T> *__restrict__ shz, Grid g) {
   size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.nc) return;
-  const KI k = kidx(g, c);
+  const KIdx k = kidx(g, c);
   V2<T> pv = ld2(z + 2 * c);
   if (!first) pv = pv + scal((T)sc[S_BETA], ld2(p + 2 * c));
   st2(p + 2 * c, pv);
@@ -439,7 +410,7 @@ __global__ __launch_bounds__(TPB) void kcg_div_dot_kernel(const T *__restrict__ 
                                                           double *__restrict__ partial, Grid g) {
   double s = 0.0;
   for (size_t c = (size_t)blockIdx.x * TPB + threadIdx.x; c < g.nc; c += (size_t)gridDim.x * TPB) {
-    const KI k = kidx(g, c);
+    const KIdx k = kidx(g, c);
     const T q0 = (T)(k.kx * g.wave_x), q1 = (T)(k.ky * g.wave_y), q2 = (T)(k.kz * g.wave_z);
     V2<T> a = cmul(ld2(w0 + 2 * c), scal(q0, conjv(shx[k.i])));
     a = a + cmul(ld2(w1 + 2 * c), scal(q1, conjv(shy[k.j])));
@@ -464,7 +435,7 @@ __global__ __launch_bounds__(TPB) void kcg_update_kernel(T *__restrict__ x, T *r
   const T alpha = (T)(pap != 0.0 ? sc[S_RZ] / pap : 0.0);
   double s_rz = 0.0, s_rr = 0.0;
   for (size_t c = (size_t)blockIdx.x * TPB + threadIdx.x; c < g.nc; c += (size_t)gridDim.x * TPB) {
-    const KI k = kidx(g, c);
+    const KIdx k = kidx(g, c);
     const V2<T> pv = ld2(p + 2 * c);
     st2(x + 2 * c, ld2(x + 2 * c) + scal(alpha, pv));
     const V2<T> rv = ld2(r + 2 * c) + scal(alpha, ld2(ap + 2 * c));
@@ -496,7 +467,7 @@ __global__ __launch_bounds__(TPB) void kres_kernel(const T *__restrict__ w0, con
                                                    double *__restrict__ partial, Grid g) {
   double s = 0.0;
   for (size_t c = (size_t)blockIdx.x * TPB + threadIdx.x; c < g.nc; c += (size_t)gridDim.x * TPB) {
-    const KI k = kidx(g, c);
+    const KIdx k = kidx(g, c);
     const T q0 = (T)(k.kx * g.wave_x), q1 = (T)(k.ky * g.wave_y), q2 = (T)(k.kz * g.wave_z);
     V2<T> a = cmul(ld2(w0 + 2 * c), scal(q0, conjv(shx[k.i])));
     a = a + cmul(ld2(w1 + 2 * c), scal(q1, conjv(shy[k.j])));

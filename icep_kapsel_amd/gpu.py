@@ -234,6 +234,45 @@ class KapselGPU:
         self._call("poisson_galerkin_start", _dp(e), ctypes.byref(g))
         return g.value
 
+    # electrolyte -------------------------------------------------------------
+    def set_electrolyte(self, el):
+        self._call("set_electrolyte", ctypes.byref(el))
+
+    def init_rho_ion(self):
+        self._call("init_rho_ion")
+
+    def get_total_solute(self):
+        a = np.zeros(2)
+        self._call("get_total_solute", _dp(a))
+        return a
+
+    def set_total_solute(self, v):
+        a = np.ascontiguousarray(v, dtype=np.float64)
+        self._call("set_total_solute", _dp(a))
+
+    def make_phi_qq(self, phi_name, surface_name):
+        self._call("make_phi_qq", abi.FIELD[phi_name], abi.FIELD[surface_name])
+
+    def make_surface_normal(self, name0):
+        self._call("make_surface_normal", abi.FIELD[name0])
+
+    def charge_field(self, name):
+        self._call("charge_field", abi.FIELD[name])
+
+    def nssolute_euler(self, t):
+        self._call("nssolute_euler", ctypes.byref(t))
+
+    def rescale_solute(self):
+        a = np.zeros(2)
+        self._call("rescale_solute", _dp(a))
+        return a
+
+    def coulomb_force(self, t):
+        self._call("coulomb_force", ctypes.byref(t))
+
+    def electrolyte_step(self, t):
+        self._call("electrolyte_step", ctypes.byref(t))
+
     # timing ---------------------------------------------------------------
     def kernel_timing(self, classes):
         mask = 0

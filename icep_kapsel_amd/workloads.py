@@ -139,3 +139,34 @@ def icep_config(run=0, precision=abi.KAPSEL_FP64, ratio=0.1, grid=128, n_particl
     v = np.array([rng.normal() for _ in range(3)])
     e0 = 0.1 * v / np.linalg.norm(v)
     return p, particle_state(pos), e0, 0.05
+
+
+def electrolyte_auto_dt(p, el):
+    """Time_increment "auto" with factor 1 for the Electrolyte equation
+    (input.cxx:329-379): min(1 / (nu KMAX2), 1 / (kBT max(L_+, L_-) KMAX2)),
+    KMAX2 = sum_d (WAVE_d TRN_d)^2; with AC, also at most 1e-2 / Frequency."""
+    n = (p.nx, p.ny, p.nz)
+    kmax2 = sum((2.0 * math.pi / (m * p.dx) * ((m + 2) // 3)) ** 2 for m in n)
+    nu = p.eta / p.rho
+    dt = min(1.0 / (nu * kmax2), 1.0 / (el.kbt * max(el.onsager_coeff[0], el.onsager_coeff[1]) * kmax2))
+    if el.ac and el.angular_frequency > 0:
+        dt = min(dt, 1e-2 * 2.0 * math.pi / el.angular_frequency)
+    return dt
+
+
+def electrolyte_config(precision=abi.KAPSEL_FP64, grid=256, n_particles=2000, radius=4.0, xi=2.0,
+                       surface_charge=100.0, debye_length=4.0, e_ext=(0.01, 0.0, 0.0), seed=5005):
+    """Scale + electrolyte workload: 256^3, 2000 spheres a=4 xi=2 carrying
+    surface charge 100 e each, two monovalent ion species (Onsager
+    coefficients 7.24, kBT = e = Dielectric_cst = 1), Debye length 4, DC field
+    E_ext = (0.01, 0, 0) (the reference UDF's salt and Electric_field
+    sections, UDF/input.udf:53-100), eta = rho = 1, WCA LJ sigma = 2a, spheres
+    at rest from seeded non-overlapping positions, auto time step.
+    Returns (params, particle state, electrolyte parameters, dt)."""
+    rng = Xoshiro256(seed)
+    n, a = grid, radius
+    p = abi.make_params(n, n_particles=n_particles, radius=a, xi=xi, rotation=1, lj_truncate=1,
+                        lj_powers=0, lj_epsilon=1.0, lj_sigma=2.0 * a, precision=precision)
+    pos = random_positions(rng, n_particles, (n, n, n), 2.0 * a)
+    el = abi.make_electrolyte(surface_charge=surface_charge, debye_length=debye_length, e_ext=e_ext)
+    return p, particle_state(pos), el, electrolyte_auto_dt(p, el)

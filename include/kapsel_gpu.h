@@ -51,6 +51,8 @@ enum kapsel_field {
   KAPSEL_EPSILON, KAPSEL_POTENTIAL,        /* permittivity, potential */
   KAPSEL_RHO_FREE, KAPSEL_RHS,             /* free charge, Poisson RHS */
   KAPSEL_WORK0, KAPSEL_WORK1, KAPSEL_WORK2, KAPSEL_WORK3, KAPSEL_WORK4,
+  KAPSEL_CONC0, KAPSEL_CONC1,              /* ion concentrations (k-space),
+                                              Concentration[0..1] (solute_rhs.cxx:16) */
   KAPSEL_NFIELDS
 };
 
@@ -242,6 +244,62 @@ int kapsel_pnet_warm_start(kapsel_ctx *ctx, double beta, const double e0[3], dou
  * norm.  No reference counterpart (the reference starts from the previous
  * potential, operate_dielectric.cxx:558-562).  *gamma may be NULL. */
 int kapsel_poisson_galerkin_start(kapsel_ctx *ctx, const double e0[3], double *gamma);
+
+/* ---- electrolyte: two ion species, Nernst-Planck, Coulomb force ----------- */
+/* The globals the Electrolyte equation reads (input.cxx:249-265,
+ * operate_electrolyte.cxx:10-14), for N_spec = 2 ("salt") without
+ * Dielectric: species 0 is the positive ion, species 1 the negative ion. */
+typedef struct kapsel_electrolyte {
+  double kbt;                /* kBT */
+  double elementary_charge;  /* Elementary_charge */
+  double dielectric_cst;     /* Dielectric_cst */
+  double surface_charge;     /* Surface_charge[0]: charge of one sphere, in units of
+                                Elementary_charge (nonzero) */
+  double valency[2];         /* Valency_positive_ion (> 0), Valency_negative_ion (< 0) */
+  double onsager_coeff[2];   /* Onsager_coeff_positive_ion, Onsager_coeff_negative_ion */
+  double debye_length;       /* Debye_length */
+  double e_ext[3];           /* E_ext; all zero when External_field is OFF */
+  int ac;                    /* AC: the applied field is E_ext sin(angular_frequency * time) */
+  double angular_frequency;  /* Angular_Frequency (2 pi Frequency) */
+} kapsel_electrolyte;
+
+/* Sets the electrolyte parameters; checks of Init_rho_ion
+ * (operate_electrolyte.cxx:740-745, 820-827). */
+int kapsel_set_electrolyte(kapsel_ctx *ctx, const kapsel_electrolyte *el);
+/* Init_rho_ion, N_spec = 2 without Poisson_Boltzmann
+ * (operate_electrolyte.cxx:738-861): Set_uniform_ion_charge_density_salt
+ * (:559-621) fills CONC0/CONC1 (k-space) from the particle positions, then
+ * Count_solute_each (solute_rhs.h:143-156) sets the conserved totals. */
+int kapsel_init_rho_ion(kapsel_ctx *ctx);
+/* Total_solute[0..1] (solute_rhs.cxx:13): the amounts Rescale_solute keeps. */
+int kapsel_get_total_solute(kapsel_ctx *ctx, double total[2]);
+int kapsel_set_total_solute(kapsel_ctx *ctx, const double total[2]);
+/* Make_phi_qq_particle (operate_electrolyte.cxx:179-242): unclamped profile
+ * sum into phi_field, normalised surface charge density into surface_field. */
+int kapsel_make_phi_qq(kapsel_ctx *ctx, int phi_field, int surface_field);
+/* Make_surface_normal (make_phi.cxx:17-61) into field0..field0+2. */
+int kapsel_make_surface_normal(kapsel_ctx *ctx, int field0);
+/* Conc_k2charge_field (operate_electrolyte.cxx:45-74): real-space charge
+ * density of the particles and the ions (CONC0/CONC1) into `field`. */
+int kapsel_charge_field(kapsel_ctx *ctx, int field);
+/* NSsolute_solver_Euler (fluid_solver.cxx:318-338) with Rhs_NS_Nernst_Planck
+ * (:246-292, non-Dielectric branch), Rhs_NS_solute (:184-229) and the
+ * solute_rhs.cxx fluxes (:42-233): one Euler step of zeta and CONC0/CONC1. */
+int kapsel_nssolute_euler(kapsel_ctx *ctx, const kapsel_time *jikan);
+/* Rescale_solute (solute_rhs.cxx:99-128): PHI <- unclamped profile sum,
+ * CONC_n *= Total_solute[n] / Count_single_solute; factors may be NULL. */
+int kapsel_rescale_solute(kapsel_ctx *ctx, double factor[2]);
+/* Make_Coulomb_force_x_on_fluid (operate_electrolyte.cxx:90-177), free
+ * particles: the force density into F0..F2. */
+int kapsel_coulomb_force(kapsel_ctx *ctx, const kapsel_time *jikan);
+/* Time_evolution_hydro, Electrolyte branch without Dielectric
+ * (sp_3d_ns.cxx:24-186): NSsolute_solver_Euler, FIX_CELL, Zeta_k2u, particle
+ * position update, profile scatter, Rescale_solute, Coulomb force,
+ * u += f dt / RHO, Solenoidal_u, hydrodynamic force, particle velocity update
+ * (no thermostat in this branch), rigidity constraint, U2zeta_k; then
+ * time += dt_fluid, ts += 1.  The reference's first Calc_f_hydro of the
+ * branch (:60-68) only fills the diagnostic f_hydro1 and is not repeated. */
+int kapsel_electrolyte_step(kapsel_ctx *ctx, kapsel_time *jikan);
 
 /* ---- kernel timing (HIP events on the context stream) --------------------- */
 enum kapsel_kernel_class {

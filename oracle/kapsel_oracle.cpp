@@ -1238,6 +1238,414 @@ struct Oracle {
     return it;
   }
 
+  /* ------------------------------------------------------- electrolyte */
+  /* Two ion species without Dielectric (N_spec = 2, "salt").  Scratch
+   * buffers stand in for the reference's globals used as working memory
+   * (u, up, f_particle, Surface_normal, Concentration_rhs0); none of their
+   * contents leaks into a result, so the mapping is free. */
+  kapsel_electrolyte EL{};
+  bool el_set = false;
+  double total_solute[2] = {0.0, 0.0};
+  enum { EB_NORMAL = 0, EB_GRAD = 3, EB_FLUX = 6, EB_CHARGE = 9, EB_PHIQQ = 10, EB_WORK = 11,
+         EB_RHS = 12, EB_POT = 14, EB_COUNT = 15 };
+  std::vector<double> eb[EB_COUNT];
+  double *EB(int i) {
+    if (eb[i].size() != F) eb[i].assign(F, 0.0);
+    return eb[i].data();
+  }
+  double *C(int n) { return fld(KAPSEL_CONC0 + n); }
+  double valency_e(int n) const { return EL.valency[n] * EL.elementary_charge; }
+  double ac_factor(const kapsel_time &t) const { return EL.ac ? sin(EL.angular_frequency * t.time) : 1.0; }
+
+  /* DPhi_compact_sin (profile.h:168-181) */
+  inline double DPhi_compact_sin(double x) const {
+    const double ixi = 1. / XI;
+    double dmy_x = RADIUS - x;
+    if (fabs(dmy_x) < HXI) return M_PI * .5 * ixi * cos(M_PI * dmy_x * ixi);
+    return 0.;
+  }
+
+  /* Make_phi_qq_particle (operate_electrolyte.cxx:179-242); phi and surface
+   * accumulate (callers reset them); one particle species */
+  void make_phi_qq_particle(double *phi, double *surface) {
+    const double q = EL.surface_charge * EL.elementary_charge; /* Surface_charge_e (:740-746) */
+    double abs_total_surface_charge = 0.;
+    for (int n = 0; n < Np; n++) {
+      double xp[3]; getx(n, xp);
+      int x_int[3]; double residue[3];
+      Particle_cell(xp, x_int, residue);
+      int r_mesh[3]; double r[3], x[3];
+      for (int mesh = 0; mesh < NP_domain; mesh++) {
+        Relative_coord(&sekibun[3 * mesh], x_int, residue, r_mesh, r);
+        for (int d = 0; d < 3; d++) x[d] = r_mesh[d] * DX;
+        double dmy = Distance(x, xp);
+        size_t im = ((size_t)r_mesh[0] * NY + r_mesh[1]) * NZ_ + r_mesh[2];
+        phi[im] += Phi(dmy, RADIUS);
+        surface[im] += q * DPhi_compact_sin(dmy);
+      }
+      abs_total_surface_charge += fabs(q);
+    }
+    double dmy = 0.0;
+#pragma omp parallel for reduction(+ : dmy)
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++)
+        for (int k = 0; k < NZ; k++) dmy += fabs(surface[((size_t)i * NY + j) * NZ_ + k]);
+    double rescale = abs_total_surface_charge / (dmy * DX3);
+#pragma omp parallel for
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++)
+        for (int k = 0; k < NZ; k++) surface[((size_t)i * NY + j) * NZ_ + k] *= rescale;
+  }
+
+  /* Conc_k2charge_field (operate_electrolyte.cxx:45-74) */
+  void conc_k2charge_field(double *charge, double *phi_p, double *work) {
+    reset(phi_p);
+    reset(charge);
+    make_phi_qq_particle(phi_p, charge);
+    for (int n = 0; n < 2; n++) {
+      memcpy(work, C(n), F * sizeof(double)); /* A_k2a_out */
+      fft_c2r(work);
+      const double ze = valency_e(n);
+#pragma omp parallel for
+      for (int i = 0; i < NX; i++)
+        for (int j = 0; j < NY; j++)
+          for (int k = 0; k < NZ; k++) {
+            size_t im = ((size_t)i * NY + j) * NZ_ + k;
+            charge[im] += ze * work[im] * (1. - phi_p[im]);
+          }
+    }
+  }
+
+  /* Charge_field_k2Coulomb_potential_k_PBC (operate_electrolyte.cxx:76-88) */
+  void coulomb_potential_k(double *a) {
+    const double idc = 1. / EL.dielectric_cst;
+#pragma omp parallel for
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++)
+        for (int k = 0; k < NZ_; k++) a[((size_t)i * NY + j) * NZ_ + k] *= (IK2(i, j, k) * idc);
+  }
+
+  /* A_k2da_k (fft_wrapper.h:755-759, fft_wrapper.cxx:214-246): da_d = i k_d a */
+  void grad_k(const double *a, double **da) {
+#pragma omp parallel for
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++)
+        for (int k = 0; k < HNZ_; k++) {
+          int k2 = 2 * k;
+          size_t im = ((size_t)i * NY + j) * NZ_ + k2;
+          const double wn[3] = {KX(i) * WAVE_X, KY(j) * WAVE_Y, KZ(k2) * WAVE_Z};
+          for (int d = 0; d < 3; d++) {
+            da[d][im] = -wn[d] * a[im + 1];
+            da[d][im + 1] = wn[d] * a[im];
+          }
+        }
+  }
+
+  /* U_k2divergence_k (fft_wrapper.cxx:896-920): div = i k . u */
+  void divergence_k(double **u, double *div) {
+#pragma omp parallel for
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++)
+        for (int k = 0; k < HNZ_; k++) {
+          int k2 = 2 * k;
+          size_t im0 = ((size_t)i * NY + j) * NZ_ + k2, im1 = im0 + 1;
+          const double ks[3] = {KX(i) * WAVE_X, KY(j) * WAVE_Y, KZ(k2) * WAVE_Z};
+          div[im0] = -(ks[0] * u[0][im1] + ks[1] * u[1][im1] + ks[2] * u[2][im1]);
+          div[im1] = (ks[0] * u[0][im0] + ks[1] * u[1][im0] + ks[2] * u[2][im0]);
+        }
+  }
+
+  /* Make_surface_normal (make_phi.cxx:17-61) */
+  void make_surface_normal(double **nrm) {
+    for (int d = 0; d < 3; d++) reset(nrm[d]);
+    for (int n = 0; n < Np; n++) {
+      double xp[3]; getx(n, xp);
+      int x_int[3]; double residue[3];
+      Particle_cell(xp, x_int, residue);
+      int r_mesh[3]; double r[3];
+      for (int mesh = 0; mesh < NP_domain; mesh++) {
+        Relative_coord(&sekibun[3 * mesh], x_int, residue, r_mesh, r);
+        double dmy_r = sqrt(SQ(r[0]) + SQ(r[1]) + SQ(r[2]));
+        double dmy = fabs(dmy_r - RADIUS);
+        if (dmy < HXI) {
+          double ir = 1. / dmy_r;
+          size_t im = ((size_t)r_mesh[0] * NY + r_mesh[1]) * NZ_ + r_mesh[2];
+          for (int d = 0; d < 3; d++) nrm[d][im] += r[d] * ir;
+        }
+      }
+    }
+  }
+
+  /* Solute_impermeability (solute_rhs.cxx:42-97), particles in order */
+  void solute_impermeability(double **flux, double **nrm) {
+    for (int n = 0; n < Np; n++) {
+      double xp[3]; getx(n, xp);
+      int x_int[3]; double residue[3];
+      Particle_cell(xp, x_int, residue);
+      int r_mesh[3]; double r[3];
+      for (int mesh = 0; mesh < NP_domain; mesh++) {
+        Relative_coord(&sekibun[3 * mesh], x_int, residue, r_mesh, r);
+        size_t im = ((size_t)r_mesh[0] * NY + r_mesh[1]) * NZ_ + r_mesh[2];
+        double normal[3], norm2 = 0.;
+        for (int d = 0; d < 3; d++) {
+          normal[d] = nrm[d][im];
+          norm2 += SQ(normal[d]);
+        }
+        if (norm2 > 0.) {
+          double dmy = 0.;
+          for (int d = 0; d < 3; d++) dmy += flux[d][im] * normal[d];
+          dmy /= norm2;
+          for (int d = 0; d < 3; d++) flux[d][im] -= normal[d] * dmy;
+        }
+      }
+    }
+  }
+
+  /* Count_single_solute (solute_rhs.cxx:130-148) */
+  double count_single_solute(const double *c, const double *phi) {
+    double dmy = 0.;
+#pragma omp parallel for reduction(+ : dmy)
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++)
+        for (int k = 0; k < NZ; k++) {
+          size_t im = ((size_t)i * NY + j) * NZ_ + k;
+          dmy += (1. - phi[im]) * c[im];
+        }
+    return dmy * DX3;
+  }
+
+  /* Count_solute_each (solute_rhs.h:143-156) */
+  void count_solute_each(double *n_solute, double *phi, double *work) {
+    reset(phi);
+    make_phi_particle(phi);
+    for (int n = 0; n < 2; n++) {
+      memcpy(work, C(n), F * sizeof(double));
+      fft_c2r(work);
+      n_solute[n] = count_single_solute(work, phi);
+    }
+  }
+
+  /* Init_rho_ion (operate_electrolyte.cxx:738-861), N_spec = 2 without
+   * Poisson_Boltzmann: Set_uniform_ion_charge_density_salt (:559-621) */
+  void init_rho_ion() {
+    double *phi = fld(KAPSEL_PHI);
+    reset(phi);
+    make_phi_particle(phi);
+    double positive_ion_number = 0., negative_ion_number = 0.;
+    const double q = EL.surface_charge, npart = (double)Np;
+    if (q < 0.) positive_ion_number += fabs(q * npart / EL.valency[0]);
+    else if (q > 0.) negative_ion_number += fabs(q * npart / EL.valency[1]);
+    double volume_phi = 0.;
+#pragma omp parallel for reduction(+ : volume_phi)
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++)
+        for (int k = 0; k < NZ; k++) volume_phi += (1. - phi[((size_t)i * NY + j) * NZ_ + k]);
+    double pd = positive_ion_number / (volume_phi * DX3);
+    double nd = negative_ion_number / (volume_phi * DX3);
+    double rho_inf = EL.kbt * EL.dielectric_cst / SQ(EL.elementary_charge * EL.debye_length);
+    double rp = rho_inf / (EL.valency[0] * (EL.valency[0] - EL.valency[1]));
+    double rn = -rho_inf / (EL.valency[1] * (EL.valency[0] - EL.valency[1]));
+    reset(C(0));
+    reset(C(1));
+#pragma omp parallel for
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++)
+        for (int k = 0; k < NZ; k++) {
+          size_t im = ((size_t)i * NY + j) * NZ_ + k;
+          C(0)[im] = pd + rp;
+          C(1)[im] = nd + rn;
+        }
+    total_solute[0] = count_single_solute(C(0), phi);
+    total_solute[1] = count_single_solute(C(1), phi);
+    fft_r2c(C(0));
+    fft_r2c(C(1));
+    count_solute_each(total_solute, phi, EB(EB_WORK));
+  }
+
+  /* Rescale_solute (solute_rhs.cxx:99-128) */
+  void rescale_solute(double *factor) {
+    double *phi = fld(KAPSEL_PHI), *work = EB(EB_WORK);
+    reset(phi);
+    make_phi_particle(phi);
+    for (int n = 0; n < 2; n++) {
+      memcpy(work, C(n), F * sizeof(double));
+      fft_c2r(work);
+      double dmy = count_single_solute(work, phi);
+      factor[n] = total_solute[n] / dmy;
+      double *c = C(n);
+#pragma omp parallel for
+      for (size_t im = 0; im < F; im++) c[im] *= factor[n];
+    }
+  }
+
+  /* the four kept boxes of the 2/3 rule (fft_wrapper.cxx:185-197) */
+  void kept_boxes(int rng[4][6]) const {
+    int r[4][6] = {{0, TRN_X - 1, 0, TRN_Y - 1, 0, 2 * TRN_Z - 1},
+                   {0, TRN_X - 1, NY - TRN_Y + 1, NY - 1, 0, 2 * TRN_Z - 1},
+                   {NX - TRN_X + 1, NX - 1, 0, TRN_Y - 1, 0, 2 * TRN_Z - 1},
+                   {NX - TRN_X + 1, NX - 1, NY - TRN_Y + 1, NY - 1, 0, 2 * TRN_Z - 1}};
+    memcpy(rng, r, sizeof r);
+  }
+
+  /* Rhs_NS_solute (fluid_solver.cxx:184-229) with Diffusion_flux_single,
+   * Solute_solver_rhs_nonlinear_x_single, Add_advection_flux
+   * (solute_rhs.cxx:150-233) and Rhs_solvent (:172-182) */
+  void rhs_ns_solute(double **grad, double **rhs_ns, double **rhs_solute) {
+    truncate(fld(KAPSEL_ZETA0));
+    truncate(fld(KAPSEL_ZETA1));
+    zeta_k2u();
+    double *u[3] = {U(0), U(1), U(2)};
+    double *nrm[3] = {EB(EB_NORMAL), EB(EB_NORMAL + 1), EB(EB_NORMAL + 2)};
+    make_surface_normal(nrm);
+    double *flux[3] = {EB(EB_FLUX), EB(EB_FLUX + 1), EB(EB_FLUX + 2)};
+    for (int n = 0; n < 2; n++) {
+      double *ck = C(n);
+      truncate(ck);
+      /* Diffusion_flux_single: rhs_ns[0] is its working memory */
+      double *w = rhs_ns[0];
+      const double kl = EL.kbt * EL.onsager_coeff[n];
+#pragma omp parallel for
+      for (size_t im = 0; im < F; im++) w[im] = kl * ck[im];
+      grad_k(w, flux);
+      for (int d = 0; d < 3; d++) fft_c2r(flux[d]);
+      double *c = rhs_solute[n];
+      memcpy(c, ck, F * sizeof(double));
+      fft_c2r(c);
+      const double ze = valency_e(n), L = EL.onsager_coeff[n];
+#pragma omp parallel for
+      for (int i = 0; i < NX; i++)
+        for (int j = 0; j < NY; j++)
+          for (int k = 0; k < NZ; k++) {
+            size_t im = ((size_t)i * NY + j) * NZ_ + k;
+            for (int d = 0; d < 3; d++) flux[d][im] += c[im] * (L * (grad[d][im] * ze));
+          }
+      solute_impermeability(flux, nrm);
+#pragma omp parallel for
+      for (int i = 0; i < NX; i++)
+        for (int j = 0; j < NY; j++)
+          for (int k = 0; k < NZ; k++) {
+            size_t im = ((size_t)i * NY + j) * NZ_ + k;
+            for (int d = 0; d < 3; d++) flux[d][im] += c[im] * -u[d][im];
+          }
+      for (int d = 0; d < 3; d++) fft_r2c(flux[d]);
+      divergence_k(flux, rhs_solute[n]);
+    }
+    u2advection_k(u, rhs_ns);
+    int rng[4][6];
+    kept_boxes(rng);
+    double *zeta[2] = {fld(KAPSEL_ZETA0), fld(KAPSEL_ZETA1)};
+    for (int b = 0; b < 4; b++) {
+#pragma omp parallel for
+      for (int i = rng[b][0]; i <= rng[b][1]; i++)
+        for (int j = rng[b][2]; j <= rng[b][3]; j++)
+          for (int k = rng[b][4]; k <= rng[b][5]; k++) {
+            size_t im = ((size_t)i * NY + j) * NZ_ + k;
+            double k2 = K2(i, j, k);
+            rhs_ns[0][im] += -(NU * k2 * zeta[0][im]);
+            rhs_ns[1][im] += -(NU * k2 * zeta[1][im]);
+          }
+    }
+  }
+
+  /* Rhs_NS_Nernst_Planck, non-Dielectric branch (fluid_solver.cxx:246-292)
+   * with Add_constant_field_k (:231-244) */
+  void rhs_ns_nernst_planck(const kapsel_time &jikan, double **rhs_ns, double **rhs_solute) {
+    double *pot = EB(EB_CHARGE);
+    conc_k2charge_field(pot, EB(EB_PHIQQ), EB(EB_WORK));
+    fft_r2c(pot);
+    coulomb_potential_k(pot);
+    truncate(pot);
+    double *grad[3] = {EB(EB_GRAD), EB(EB_GRAD + 1), EB(EB_GRAD + 2)};
+    grad_k(pot, grad);
+    double amp = (double)(NX * NY * NZ) * ac_factor(jikan);
+    for (int d = 0; d < 3; d++) grad[d][0] -= EL.e_ext[d] * amp;
+    for (int d = 0; d < 3; d++) fft_c2r(grad[d]);
+    rhs_ns_solute(grad, rhs_ns, rhs_solute);
+  }
+
+  /* NSsolute_solver_Euler (fluid_solver.cxx:318-338) with Field_solver_Euler
+   * (:121-136); rhs_ns is WORK0/WORK1 */
+  void nssolute_euler(const kapsel_time &jikan) {
+    double *rhs_ns[2] = {fld(KAPSEL_WORK0), fld(KAPSEL_WORK1)};
+    double *rhs_solute[2] = {EB(EB_RHS), EB(EB_RHS + 1)};
+    rhs_ns_nernst_planck(jikan, rhs_ns, rhs_solute);
+    double *zeta[2] = {fld(KAPSEL_ZETA0), fld(KAPSEL_ZETA1)};
+    int rng[4][6];
+    kept_boxes(rng);
+    for (int b = 0; b < 4; b++) {
+      for (int d = 0; d < 2; d++)
+#pragma omp parallel for
+        for (int i = rng[b][0]; i <= rng[b][1]; i++)
+          for (int j = rng[b][2]; j <= rng[b][3]; j++)
+            for (int k = rng[b][4]; k <= rng[b][5]; k++) {
+              size_t im = ((size_t)i * NY + j) * NZ_ + k;
+              zeta[d][im] += (jikan.dt_fluid * rhs_ns[d][im]);
+            }
+      for (int n = 0; n < 2; n++)
+#pragma omp parallel for
+        for (int i = rng[b][0]; i <= rng[b][1]; i++)
+          for (int j = rng[b][2]; j <= rng[b][3]; j++)
+            for (int k = rng[b][4]; k <= rng[b][5]; k++) {
+              size_t im = ((size_t)i * NY + j) * NZ_ + k;
+              C(n)[im] += (jikan.dt_fluid * rhs_solute[n][im]);
+            }
+    }
+    /* uk_dc += dt * dc_rhs with dc_rhs = 0 (Rhs_solvent) */
+  }
+
+  /* Make_Coulomb_force_x_on_fluid (operate_electrolyte.cxx:90-177), free particles */
+  void coulomb_force(const kapsel_time &jikan) {
+    double *charge = EB(EB_CHARGE), *pot = EB(EB_POT);
+    double *force[3] = {FP(0), FP(1), FP(2)};
+    conc_k2charge_field(charge, force[0], force[1]);
+    memcpy(pot, charge, F * sizeof(double)); /* A2a_k_out */
+    fft_r2c(pot);
+    coulomb_potential_k(pot);
+    grad_k(pot, force);
+    for (int d = 0; d < 3; d++) fft_c2r(force[d]);
+    double external[3];
+    const double s = ac_factor(jikan);
+    for (int d = 0; d < 3; d++) external[d] = EL.e_ext[d] * s;
+    for (int d = 0; d < 3; d++)
+#pragma omp parallel for
+      for (int i = 0; i < NX; i++)
+        for (int j = 0; j < NY; j++)
+          for (int k = 0; k < NZ; k++) {
+            size_t im = ((size_t)i * NY + j) * NZ_ + k;
+            double electric_field = -force[d][im];
+            electric_field += external[d];
+            force[d][im] = charge[im] * electric_field;
+          }
+  }
+
+  /* Time_evolution_hydro, Electrolyte branch without Dielectric (sp_3d_ns.cxx:24-186) */
+  void electrolyte_step(kapsel_time &jikan) {
+    nssolute_euler(jikan);
+    for (int d = 0; d < 3; d++)
+      if (P.fix_cell[d]) uk_dc[d] = 0.0;
+    truncate(fld(KAPSEL_ZETA0));
+    truncate(fld(KAPSEL_ZETA1));
+    zeta_k2u();
+    md_position(jikan);
+    reset(fld(KAPSEL_PHI));
+    reset(fld(KAPSEL_PHI_SUM));
+    make_phi_particle_sum(fld(KAPSEL_PHI), fld(KAPSEL_PHI_SUM));
+    double factor[2];
+    rescale_solute(factor);
+    coulomb_force(jikan);
+    add_force_to_fluid(jikan);
+    double *u[3] = {U(0), U(1), U(2)};
+    calc_f_hydro(fld(KAPSEL_PHI_SUM), u, jikan);
+    md_velocity(jikan);
+    for (int d = 0; d < 3; d++) reset(UP(d));
+    double *up[3] = {UP(0), UP(1), UP(2)};
+    make_u_particle_sum(up, fld(KAPSEL_PHI_SUM));
+    make_f_particle_dt_sole_add();
+    u2zeta_k();
+    jikan.time += jikan.dt_fluid;
+    jikan.ts += 1;
+  }
+
   /* main's initial rigidity projection (sp_3d_ns.cxx:431-447) */
   void init_rigidity() {
     reset(fld(KAPSEL_PHI));
@@ -1418,6 +1826,49 @@ int oracle_icep_step(void *h, kapsel_time *t, const double *e0, double tol, int 
 }
 int oracle_init_rigidity(void *h) { O->init_rigidity(); return 0; }
 int oracle_spm_step(void *h, kapsel_time *t) { O->spm_step(*t); return 0; }
+
+/* electrolyte */
+int oracle_set_electrolyte(void *h, const kapsel_electrolyte *el) {
+  O->EL = *el;
+  O->el_set = true;
+  return 0;
+}
+#define NEED_EL                                                     \
+  if (!O->el_set) {                                                 \
+    snprintf(g_err, sizeof g_err, "electrolyte parameters not set"); \
+    return -3;                                                      \
+  }
+int oracle_init_rho_ion(void *h) { NEED_EL O->init_rho_ion(); return 0; }
+int oracle_get_total_solute(void *h, double *t) { t[0] = O->total_solute[0]; t[1] = O->total_solute[1]; return 0; }
+int oracle_set_total_solute(void *h, const double *t) { O->total_solute[0] = t[0]; O->total_solute[1] = t[1]; return 0; }
+int oracle_make_phi_qq(void *h, int fphi, int fsurf) {
+  NEED_EL
+  O->reset(O->fld(fphi));
+  O->reset(O->fld(fsurf));
+  O->make_phi_qq_particle(O->fld(fphi), O->fld(fsurf));
+  return 0;
+}
+int oracle_make_surface_normal(void *h, int f0) {
+  double *n[3] = {O->fld(f0), O->fld(f0 + 1), O->fld(f0 + 2)};
+  O->make_surface_normal(n);
+  return 0;
+}
+int oracle_charge_field(void *h, int f) {
+  NEED_EL
+  O->conc_k2charge_field(O->fld(f), O->EB(Oracle::EB_PHIQQ), O->EB(Oracle::EB_WORK));
+  return 0;
+}
+int oracle_nssolute_euler(void *h, const kapsel_time *t) { NEED_EL O->nssolute_euler(*t); return 0; }
+int oracle_rescale_solute(void *h, double *factor) {
+  NEED_EL
+  double f[2];
+  O->rescale_solute(f);
+  if (factor) { factor[0] = f[0]; factor[1] = f[1]; }
+  return 0;
+}
+int oracle_coulomb_force(void *h, const kapsel_time *t) { NEED_EL O->coulomb_force(*t); return 0; }
+int oracle_electrolyte_step(void *h, kapsel_time *t) { NEED_EL O->electrolyte_step(*t); return 0; }
+#undef NEED_EL
 #undef O
 
 }  // extern "C"

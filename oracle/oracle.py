@@ -53,6 +53,11 @@ def _load():
             "oracle_poisson_solve": [dp, ctypes.c_double, i, i, ip, dp],
             "oracle_maxwell_force": [dp], "oracle_add_force_to_fluid": [tp],
             "oracle_icep_step": [tp, dp, ctypes.c_double, i, i, ip, dp],
+            "oracle_set_electrolyte": [ctypes.POINTER(abi.Electrolyte)], "oracle_init_rho_ion": [],
+            "oracle_get_total_solute": [dp], "oracle_set_total_solute": [dp],
+            "oracle_make_phi_qq": [i, i], "oracle_make_surface_normal": [i], "oracle_charge_field": [i],
+            "oracle_nssolute_euler": [tp], "oracle_rescale_solute": [dp], "oracle_coulomb_force": [tp],
+            "oracle_electrolyte_step": [tp],
         }
         for name, args in protos.items():
             fn = getattr(lib, name)
@@ -236,6 +241,46 @@ class Oracle:
         rr = ctypes.c_double(0.0)
         self._call("poisson_solve", _dp(e), tol, int(maxit), int(solver), ctypes.byref(it), ctypes.byref(rr))
         return it.value, rr.value
+
+
+    # electrolyte -------------------------------------------------------
+    def set_electrolyte(self, el):
+        self._call("set_electrolyte", ctypes.byref(el))
+
+    def init_rho_ion(self):
+        self._call("init_rho_ion")
+
+    def get_total_solute(self):
+        a = np.zeros(2)
+        self._call("get_total_solute", _dp(a))
+        return a
+
+    def set_total_solute(self, v):
+        a = np.ascontiguousarray(v, dtype=np.float64)
+        self._call("set_total_solute", _dp(a))
+
+    def make_phi_qq(self, phi_name, surface_name):
+        self._call("make_phi_qq", abi.FIELD[phi_name], abi.FIELD[surface_name])
+
+    def make_surface_normal(self, name0):
+        self._call("make_surface_normal", abi.FIELD[name0])
+
+    def charge_field(self, name):
+        self._call("charge_field", abi.FIELD[name])
+
+    def nssolute_euler(self, t):
+        self._call("nssolute_euler", ctypes.byref(t))
+
+    def rescale_solute(self):
+        a = np.zeros(2)
+        self._call("rescale_solute", _dp(a))
+        return a
+
+    def coulomb_force(self, t):
+        self._call("coulomb_force", ctypes.byref(t))
+
+    def electrolyte_step(self, t):
+        self._call("electrolyte_step", ctypes.byref(t))
 
 
 class OouraRef:

@@ -1,0 +1,232 @@
+"""GPU parity of the electrolyte step (two ion species, Nernst-Planck,
+Coulomb force) against the CPU oracle, and its invariants at full size.
+
+Tolerances (relative to the field's max magnitude):
+  FP64  1e-12 for every piece and after one full step; 1e-10 after 10 steps;
+        the mean flow (k = 0 mode of u), a sum with heavy cancellation, 1e-8
+        of itself
+  FP32  against FP64 after 10 steps: particle velocities 1e-5, fluid
+        velocity 1e-5, concentrations 1e-5 (bounds written in the test)
+Full size (256^3, 2000 spheres a=4, surface charge 100 e, Debye length 4):
+the ion amounts held by Rescale_solute and electroneutrality, FP32.
+"""
+import numpy as np
+import pytest
+
+from icep_kapsel_amd import abi, workloads
+from tests import spectral as sp
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu(p):
+    from icep_kapsel_amd.gpu import KapselGPU
+    return KapselGPU(p)
+
+
+def oracle(p):
+    from oracle.oracle import Oracle
+    return Oracle(p)
+
+
+def rel(a, b):
+    s = np.abs(b).max()
+    return float(np.abs(a - b).max() / (s if s > 0 else 1.0))
+
+
+def _random_conc(n, rng, base, amp, kmax=5):
+    f = rng.standard_normal((n, n, n))
+    fk = np.fft.rfftn(f) * sp.band_mask((n, n, n), kmax)
+    f = np.fft.irfftn(fk, s=(n, n, n), axes=(0, 1, 2))
+    return base + amp * f / np.abs(f).max()
+
+
+def _pair(n, npart, q=40.0, e_ext=(0.02, -0.01, 0.005), seed=17, prec=abi.KAPSEL_FP64, ac=0):
+    rng = workloads.Xoshiro256(seed)
+    p = abi.make_params(n, n_particles=npart, radius=4.0, xi=2.0, precision=prec)
+    pos = workloads.random_positions(rng, npart, (n, n, n), 8.0)
+    el = abi.make_electrolyte(surface_charge=q, debye_length=4.0, e_ext=e_ext, ac=ac, angular_frequency=0.7)
+    el.onsager_coeff[:] = [7.24, 5.0]
+    st = workloads.particle_state(pos)
+    return p, st, el
+
+
+def _load(xs, st, el, conc_k=None, uk=None):
+    for x in xs:
+        x.set_particles(st)
+        x.set_electrolyte(el)
+        if conc_k is not None:
+            for s in (0, 1):
+                x.set_field(f"CONC{s}", conc_k[s])
+        if uk is not None:
+            for d in range(3):
+                x.set_field(f"U{d}", uk[d])
+            x.u_k2zeta_k()
+
+
+def _state(n, npart, seed):
+    rng = np.random.default_rng(seed)
+    conc = [_random_conc(n, rng, 0.05, 0.02), _random_conc(n, rng, 0.04, 0.02)]
+    ck = [np.fft.rfftn(c) for c in conc]
+    for s in (0, 1):
+        ck[s][n // 3 + 2, 3, n // 3 + 1] += 0.3  # above the 2/3 cut
+    u = sp.random_solenoidal((n, n, n), rng, kmax=5, amp=0.01)
+    uk = [sp.from_complex(np.fft.rfftn(u[d])) for d in range(3)]
+    return [sp.from_complex(c) for c in ck], uk
+
+
+@pytest.mark.parametrize("n,npart", [(32, 3), (64, 8)])
+def test_electrolyte_pieces_match_oracle(n, npart):
+    print(f"case electrolyte pieces {n}^3 {npart} spheres", flush=True)
+    p, st, el = _pair(n, npart)
+    conc_k, uk = _state(n, npart, 100 + n)
+    k, o = gpu(p), oracle(p)
+    _load((k, o), st, el, conc_k, uk)
+    worst = {}
+    for x in (k, o):
+        x.make_phi_qq("WORK3", "WORK4")
+        x.make_surface_normal("WORK0")
+    for f in ("WORK3", "WORK4", "WORK0", "WORK1", "WORK2"):
+        worst["qq/normal " + f] = rel(k.get_field(f)[:, :, :n], o.get_field(f)[:, :, :n])
+    for x in (k, o):
+        x.charge_field("RHO_FREE")
+    worst["charge"] = rel(k.get_field("RHO_FREE")[:, :, :n], o.get_field("RHO_FREE")[:, :, :n])
+    t = abi.make_time(0.01, ts=2, time=0.3)
+    for x in (k, o):
+        x.coulomb_force(t)
+    for d in range(3):
+        worst[f"coulomb F{d}"] = rel(k.get_field(f"F{d}")[:, :, :n], o.get_field(f"F{d}")[:, :, :n])
+    for x in (k, o):
+        x.set_total_solute([0.05 * n ** 3, 0.04 * n ** 3])
+    fk, fo = k.rescale_solute(), o.rescale_solute()
+    worst["rescale factors"] = float(np.abs(fk - fo).max() / np.abs(fo).max())
+    worst["PHI after rescale"] = rel(k.get_field("PHI")[:, :, :n], o.get_field("PHI")[:, :, :n])
+    for s in (0, 1):
+        worst[f"CONC{s} rescaled"] = rel(k.get_field(f"CONC{s}"), o.get_field(f"CONC{s}"))
+    for x in (k, o):
+        x.nssolute_euler(t)
+    for f in ("ZETA0", "ZETA1", "CONC0", "CONC1"):
+        worst["nssolute " + f] = rel(k.get_field(f), o.get_field(f))
+    for x in (k, o):
+        x.init_rho_ion()
+    for f in ("CONC0", "CONC1"):
+        worst["init " + f] = rel(k.get_field(f), o.get_field(f))
+    worst["init PHI"] = rel(k.get_field("PHI")[:, :, :n], o.get_field("PHI")[:, :, :n])
+    tk, to = k.get_total_solute(), o.get_total_solute()
+    worst["init totals"] = float(np.abs(tk - to).max() / np.abs(to).max())
+    print({f: f"{v:.1e}" for f, v in worst.items()}, flush=True)
+    bad = {f: v for f, v in worst.items() if not v < 1e-12}
+    assert not bad, bad
+
+
+def _compare(k, o, n):
+    """Relative deviations.  After the step U holds the spectrum of u
+    (U2zeta_k transforms in place).  Its k = 0 mode, the mean flow, is the
+    grid sum of a force made of large opposite contributions (surface charge
+    against the ion clouds), so its rounding is amplified by that
+    cancellation: it is compared relative to its own size, the other modes
+    relative to the largest mode."""
+    worst = {}
+    for f in ("ZETA0", "ZETA1", "CONC0", "CONC1"):
+        worst[f] = rel(k.get_field(f), o.get_field(f))
+    for f in ("PHI", "PHI_SUM", "F0", "F1", "F2"):
+        worst[f] = rel(k.get_field(f)[:, :, :n], o.get_field(f)[:, :, :n])
+    means = {}
+    for f in ("U0", "U1", "U2"):
+        a, b = k.get_field(f).copy(), o.get_field(f).copy()
+        means[f] = float(abs(a[0, 0, 0] - b[0, 0, 0]) / abs(b[0, 0, 0]))
+        a[0, 0, 0] = b[0, 0, 0] = 0.0
+        worst[f] = rel(a, b)
+    pk, po = k.get_particles(), o.get_particles()
+    for f in ("X", "V", "OMEGA", "F_HYDRO_PREVIOUS", "TORQUE_HYDRO_PREVIOUS"):
+        i = abi.PFIELD[f]
+        worst[f] = rel(pk[i], po[i])
+    worst["totals"] = float(np.abs(k.get_total_solute() - o.get_total_solute()).max()
+                            / np.abs(o.get_total_solute()).max())
+    return worst, means
+
+
+@pytest.mark.parametrize("ac", [0, 1])
+def test_electrolyte_step_fp64_matches_oracle(ac):
+    n, npart = 64, 8
+    print(f"case electrolyte step fp64 {n}^3 ac={ac}", flush=True)
+    p, st, el = _pair(n, npart, q=60.0, e_ext=(0.05, 0.0, 0.02), ac=ac)
+    k, o = gpu(p), oracle(p)
+    _load((k, o), st, el)
+    for x in (k, o):
+        x.init_rigidity()
+        x.init_rho_ion()
+    dt = workloads.electrolyte_auto_dt(p, el)
+    tk, to = abi.make_time(dt), abi.make_time(dt)
+    k.electrolyte_step(tk)
+    o.electrolyte_step(to)
+    assert tk.ts == to.ts == 1
+    w1, m1 = _compare(k, o, n)
+    print("one step", {f: f"{v:.1e}" for f, v in w1.items()}, "mean flow", m1, flush=True)
+    for _ in range(9):
+        k.electrolyte_step(tk)
+        o.electrolyte_step(to)
+    w10, m10 = _compare(k, o, n)
+    print("ten steps", {f: f"{v:.1e}" for f, v in w10.items()}, "mean flow", m10, flush=True)
+    assert max(w1.values()) < 1e-12, w1
+    assert max(w10.values()) < 1e-10, w10
+    assert max(m1.values()) < 1e-8 and max(m10.values()) < 1e-8
+
+
+def test_electrolyte_fp32_against_fp64():
+    n, npart = 128, 120
+    print(f"case electrolyte fp32 vs fp64 {n}^3 {npart} spheres", flush=True)
+    ks = []
+    for prec in (abi.KAPSEL_FP64, abi.KAPSEL_FP32):
+        p, st, el = _pair(n, npart, q=100.0, e_ext=(0.01, 0.0, 0.0), prec=prec)
+        k = gpu(p)
+        _load((k,), st, el)
+        k.init_rigidity()
+        k.init_rho_ion()
+        t = abi.make_time(workloads.electrolyte_auto_dt(p, el))
+        for _ in range(10):
+            k.electrolyte_step(t)
+        ks.append(k)
+    k64, k32 = ks
+    e = {}
+    for f in ("CONC0", "CONC1", "ZETA0", "ZETA1"):
+        e[f] = rel(k32.get_field(f), k64.get_field(f))
+    for f in ("U0", "U1", "U2"):
+        # spectrum of u; its k = 0 mode (mean flow) separately, relative to itself
+        a, b = k32.get_field(f).copy(), k64.get_field(f).copy()
+        e[f + " k=0"] = float(abs(a[0, 0, 0] - b[0, 0, 0]) / abs(b[0, 0, 0]))
+        a[0, 0, 0] = b[0, 0, 0] = 0.0
+        e[f] = rel(a, b)
+    p64, p32 = k64.get_particles(), k32.get_particles()
+    for f in ("V", "OMEGA", "X"):
+        i = abi.PFIELD[f]
+        e[f] = rel(p32[i], p64[i])
+    print("fp32 vs fp64 after 10 steps", {f: f"{v:.1e}" for f, v in e.items()}, flush=True)
+    assert e["V"] < 1e-5 and max(e["U0"], e["U1"], e["U2"]) < 1e-5
+    assert max(e["CONC0"], e["CONC1"]) < 1e-5
+
+
+def test_electrolyte_full_size_invariants_fp32():
+    """256^3, 2000 spheres: after 5 FP32 steps the (1 - phi)-weighted ion
+    amounts equal the totals Rescale_solute holds, and the box stays neutral."""
+    print("case electrolyte 256^3 x 2000 fp32 invariants", flush=True)
+    p, st, el, dt = workloads.electrolyte_config(precision=abi.KAPSEL_FP32)
+    k = gpu(p)
+    k.set_particles(st)
+    k.set_electrolyte(el)
+    k.init_rigidity()
+    k.init_rho_ion()
+    tot = k.get_total_solute()
+    t = abi.make_time(dt)
+    for _ in range(5):
+        k.electrolyte_step(t)
+    f = k.rescale_solute()  # recounts at the current positions
+    print("rescale factors after 5 steps", f, flush=True)
+    assert np.abs(f - 1.0).max() < 1e-5
+    k.charge_field("WORK0")
+    rho = k.get_field("WORK0")[:, :, :p.nz]
+    q = p.n_particles * el.surface_charge
+    print("net charge / particle charge", rho.sum() / q, flush=True)
+    assert abs(rho.sum()) < 1e-4 * q
+    v = k.get_particles()[abi.PFIELD["V"]]
+    assert np.all(np.isfinite(v)) and v[0].mean() > 0  # positive spheres drift along E
