@@ -218,6 +218,70 @@ def run_spm(precision, steps, warmup):
     return steps / (t1 - t0)
 
 
+def run_electrolyte(precision, steps, warmup, snapshot=False):
+    """The scale + electrolyte workload: 256^3, 2000 spheres a=4 xi=2 with
+    surface charge 100 e, two monovalent ion species (Debye length 4), DC
+    field 0.01 along x, auto time step; uniform salt start, rigidity
+    projection, W warm-up steps, then K timed steps of the Electrolyte branch
+    of Time_evolution_hydro (Nernst-Planck Euler step, profile scatter,
+    solute rescaling, Coulomb force, projection, gather, particle update)."""
+    import numpy as np
+    from icep_kapsel_amd import abi, workloads
+    from icep_kapsel_amd.gpu import KapselGPU
+
+    p, st, el, dt = workloads.electrolyte_config(precision=precision)
+    k = KapselGPU(p)
+    k.set_particles(st)
+    k.set_electrolyte(el)
+    k.init_rigidity()
+    k.init_rho_ion()
+    t = abi.make_time(dt)
+    for _ in range(warmup):
+        k.electrolyte_step(t)
+    k.synchronize()
+    snap = None
+    if snapshot:
+        snap = {f: k.get_field(f) for f in ("ZETA0", "ZETA1", "CONC0", "CONC1")}
+        snap.update({"uk_dc": k.get_uk_dc(), "particles": k.get_particles(), "totals": k.get_total_solute(),
+                     "ts": t.ts, "time": t.time})
+    k.kernel_timing(abi.KERNEL_CLASSES)
+    k.electrolyte_step(t)
+    shares = {c: k.kernel_timing_get(c) for c in abi.KERNEL_CLASSES}
+    k.kernel_timing([])
+    k.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        k.electrolyte_step(t)
+    k.synchronize()
+    t1 = time.perf_counter()
+    v = k.get_particles()[abi.PFIELD["V"]]
+    k.close()
+    return {"steps_per_s": steps / (t1 - t0), "ms_per_step": 1e3 * (t1 - t0) / steps, "dt": dt,
+            "shares_ms": {c: v_[0] for c, v_ in shares.items() if v_[1]},
+            "shares_launches": {c: v_[1] for c, v_ in shares.items() if v_[1]},
+            "v": v, "snapshot": snap}
+
+
+def run_cpu_electrolyte(snap):
+    """One FP64 oracle electrolyte step from the GPU FP64 state after the warm-up."""
+    from icep_kapsel_amd import abi, workloads
+    from oracle import oracle as omod
+
+    p, st, el, dt = workloads.electrolyte_config()
+    o = omod.Oracle(p)
+    o.set_particles(snap["particles"])
+    o.set_electrolyte(el)
+    for f in ("ZETA0", "ZETA1", "CONC0", "CONC1"):
+        o.set_field(f, snap[f])
+    o.set_uk_dc(snap["uk_dc"])
+    o.set_total_solute(snap["totals"])
+    t = abi.make_time(dt, ts=snap["ts"], time=snap["time"])
+    t0 = time.perf_counter()
+    o.electrolyte_step(t)
+    t1 = time.perf_counter()
+    return 1.0 / (t1 - t0), omod.num_threads()
+
+
 def run_cpu(snap):
     """One FP64 oracle ICEP step (reference BiCGSTAB, identity preconditioner)
     from the GPU FP64 state after the warm-up steps."""
@@ -279,6 +343,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 plain run")
     ap.add_argument("--no-spm", action="store_true", help="skip the SPM-only step timings")
+    ap.add_argument("--el-steps", type=int, default=50, help="timed steps of the 256^3 electrolyte workload")
+    ap.add_argument("--no-electrolyte", action="store_true", help="skip the 256^3 electrolyte workload")
+    ap.add_argument("--only-electrolyte", action="store_true", help="print the electrolyte record only")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="K",
                     help="only run K plain field copies (128^3 FP32, 8.52 MB each way) to calibrate "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE, then exit")
@@ -292,6 +359,12 @@ def main():
         sys.exit(2)
     from icep_kapsel_amd import abi
 
+    electrolyte = None
+    if not args.no_electrolyte:
+        electrolyte = electrolyte_record(args)
+        if args.only_electrolyte:
+            print(json.dumps(electrolyte), flush=True)
+            return
     log(f"bench: FP32 + learned ICEP step 128^3 x 64 spheres, warmup {args.warmup}, steps {args.steps}")
     r32 = run_icep(abi.KAPSEL_FP32, abi.PRECOND_SPECTRAL, args.steps, args.warmup, learned=True)
     log(f"bench: fp32 {r32['steps_per_s']:.2f} steps/s, {r32['iters_mean']:.1f} inner it/step, "
@@ -369,7 +442,37 @@ def main():
         out["cold_solve_1e-8"] = cold["mean"]
     if spm:
         out["spm_ns_step_128_200_spheres"] = spm
+    if electrolyte:
+        out["electrolyte_256_2000_spheres"] = electrolyte
     print(json.dumps(out), flush=True)
+
+
+def electrolyte_record(args):
+    """The 256^3 x 2000-sphere electrolyte workload, FP32 and FP64, and one
+    FP64 oracle step from the same state."""
+    from icep_kapsel_amd import abi
+
+    log(f"bench: electrolyte step 256^3 x 2000 charged spheres, warmup 3, steps {args.el_steps}")
+    e32 = run_electrolyte(abi.KAPSEL_FP32, args.el_steps, 3)
+    log(f"bench: electrolyte fp32 {e32['steps_per_s']:.2f} steps/s")
+    e64 = run_electrolyte(abi.KAPSEL_FP64, args.el_steps, 3, snapshot=not args.no_cpu)
+    log(f"bench: electrolyte fp64 {e64['steps_per_s']:.2f} steps/s")
+    rec = {"workload": "electrolyte_step_256^3_2000_spheres_a4_xi2_q100_debye4_E0.01",
+           "fp32_steps_per_s": e32["steps_per_s"], "fp32_ms_per_step": e32["ms_per_step"],
+           "fp64_steps_per_s": e64["steps_per_s"], "fp64_ms_per_step": e64["ms_per_step"],
+           "dt": e32["dt"], "steps": args.el_steps, "warmup": 3,
+           "particle_velocity_rel_err_fp32_vs_fp64": rel_diff(e32["v"], e64["v"]),
+           "kernel_class_ms_per_step_fp32": e32["shares_ms"],
+           "kernel_class_launches_per_step_fp32": e32["shares_launches"],
+           "kernel_class_ms_per_step_fp64": e64["shares_ms"]}
+    if not args.no_cpu:
+        log("bench: CPU oracle, one electrolyte step from the GPU FP64 state after the warm-up")
+        cps, cores = run_cpu_electrolyte(e64["snapshot"])
+        rec["cpu_baseline"] = {"value": cps, "unit": "steps/s", "cores": cores, "kind": "port",
+                               "sample": "1 FP64 electrolyte step of the same 256^3 x 2000-sphere workload "
+                                         "from the state after 3 warm-up steps, OpenMP C++ oracle"}
+        log(f"bench: electrolyte cpu {cps:.4f} steps/s on {cores} threads")
+    return rec
 
 
 if __name__ == "__main__":

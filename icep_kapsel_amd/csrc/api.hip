@@ -225,11 +225,18 @@ static void create(const kapsel_params *p, kapsel_ctx **out) {
   *out = ctx;
 }
 
-static void check_ctx(kapsel_ctx *ctx) {
+// entry points that only read state
+static void check_ctx_ro(kapsel_ctx *ctx) {
   if (!ctx) {
     set_error("null context");
     throw Error{KAPSEL_ERR_ARG};
   }
+}
+
+// every other entry point: may change state
+static void check_ctx(kapsel_ctx *ctx) {
+  check_ctx_ro(ctx);
+  ctx->api_gen++;
 }
 
 static void check_time(const kapsel_time *t) {
@@ -294,11 +301,12 @@ static void solenoidal_u(kapsel_ctx *ctx, void *const *f3) {
   fft_c2r(ctx, f3, 3);
 }
 
-// Make_f_particle_dt_sole + Add_f_particle (f_particle.h:37-95)
-static void f_particle_dt_sole_add(kapsel_ctx *ctx) {
+// Make_f_particle_dt_sole + Add_f_particle (f_particle.h:37-95); keep3: also
+// store the new real velocity there
+static void f_particle_dt_sole_add(kapsel_ctx *ctx, void *const *keep3 = nullptr) {
   make_f_nonsole(ctx);
   solenoidal_u(ctx, ctx->fields + KAPSEL_F0);
-  add_f_particle(ctx);
+  add_f_particle(ctx, keep3);
 }
 
 // U2zeta_k (operate_omega.h:212-215)
@@ -362,7 +370,8 @@ static void icep_step(kapsel_ctx *ctx, kapsel_time &t, const double e0[3], doubl
 // Calc_f_hydro (:60-68) only fills the diagnostic f_hydro1; it is skipped
 // (the second call overwrites f_hydro and torque_hydro before any use).
 static void electrolyte_step(kapsel_ctx *ctx, kapsel_time &t) {
-  nssolute_euler(ctx, t);                     // :20
+  const bool reuse = ctx->el_reuse_gen == ctx->api_gen;
+  nssolute_step(ctx, t, reuse);               // :20
   zeta_k2u(ctx, fix_mask(ctx));               // :29-42
   md_position(ctx, t);                        // :44-50
   make_phi_particle_sum(ctx);                 // :51-53
@@ -373,10 +382,13 @@ static void electrolyte_step(kapsel_ctx *ctx, kapsel_time &t) {
   force(ctx);                                 // :104-110
   md_velocity_integrate(ctx, t);
   make_u_particle_sum(ctx);                   // :166-167
-  f_particle_dt_sole_add(ctx);                // :168-169
+  f_particle_dt_sole_add(ctx, electrolyte_u_keep(ctx));  // :168-169
   u2zeta_k(ctx);                              // :184
   t.time += t.dt_fluid;
   t.ts += 1;
+  // the next entry point sees api_gen + 1; if it is this step again, what
+  // this step left (charge spectrum, real ion fields, real velocity) is current
+  ctx->el_reuse_gen = ctx->api_gen + 1;
 }
 
 // main, initial rigidity projection (sp_3d_ns.cxx:431-447)
@@ -427,14 +439,14 @@ int kapsel_destroy(kapsel_ctx *ctx) {
 
 int kapsel_synchronize(kapsel_ctx *ctx) {
   API_BEGIN
-  check_ctx(ctx);
+  check_ctx_ro(ctx);
   KAPSEL_HIP(hipStreamSynchronize(ctx->stream));
   API_END
 }
 
 int kapsel_get_sekibun(kapsel_ctx *ctx, int *np_domain, int *cells) {
   API_BEGIN
-  check_ctx(ctx);
+  check_ctx_ro(ctx);
   if (!np_domain) {
     set_error("null np_domain");
     throw Error{KAPSEL_ERR_ARG};
@@ -459,7 +471,7 @@ int kapsel_set_field(kapsel_ctx *ctx, int field, const double *host) {
 
 int kapsel_get_field(kapsel_ctx *ctx, int field, double *host) {
   API_BEGIN
-  check_ctx(ctx);
+  check_ctx_ro(ctx);
   check_field(field);
   if (!host) {
     set_error("null host buffer");
@@ -479,7 +491,7 @@ int kapsel_set_uk_dc(kapsel_ctx *ctx, const double uk_dc[3]) {
 
 int kapsel_get_uk_dc(kapsel_ctx *ctx, double uk_dc[3]) {
   API_BEGIN
-  check_ctx(ctx);
+  check_ctx_ro(ctx);
   KAPSEL_HIP(hipMemcpyAsync(uk_dc, ctx->d_uk_dc, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   KAPSEL_HIP(hipStreamSynchronize(ctx->stream));
   API_END
@@ -511,7 +523,7 @@ int kapsel_set_particles(kapsel_ctx *ctx, const double *state) {
 
 int kapsel_get_particles(kapsel_ctx *ctx, double *state) {
   API_BEGIN
-  check_ctx(ctx);
+  check_ctx_ro(ctx);
   size_t n = sizeof(double) * KAPSEL_P_NFIELDS * 3 * ctx->P.n_particles;
   if (n) {
     KAPSEL_HIP(hipMemcpyAsync(state, ctx->d_ps, n, hipMemcpyDeviceToHost, ctx->stream));
@@ -882,7 +894,7 @@ int kapsel_init_rho_ion(kapsel_ctx *ctx) {
 
 int kapsel_get_total_solute(kapsel_ctx *ctx, double total[2]) {
   API_BEGIN
-  check_ctx(ctx);
+  check_ctx_ro(ctx);
   if (!total) {
     set_error("null output");
     throw Error{KAPSEL_ERR_ARG};
@@ -975,7 +987,7 @@ int kapsel_electrolyte_step(kapsel_ctx *ctx, kapsel_time *jikan) {
 
 int kapsel_kernel_timing(kapsel_ctx *ctx, int mask) {
   API_BEGIN
-  check_ctx(ctx);
+  check_ctx_ro(ctx);
   harvest_timing(ctx);
   ctx->timing_mask = mask;
   API_END
@@ -984,7 +996,7 @@ int kapsel_kernel_timing(kapsel_ctx *ctx, int mask) {
 int kapsel_kernel_timing_get(kapsel_ctx *ctx, int kernel_class, double *total_ms, int *launches,
                              double *total_bytes) {
   API_BEGIN
-  check_ctx(ctx);
+  check_ctx_ro(ctx);
   if (kernel_class < 0 || kernel_class >= KAPSEL_K_NCLASSES || !total_ms || !launches || !total_bytes) {
     set_error("bad kernel class or null output");
     throw Error{KAPSEL_ERR_ARG};

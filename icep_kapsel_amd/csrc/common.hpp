@@ -199,11 +199,17 @@ struct kapsel_ctx {
   // scratch fields allocated by kapsel_set_electrolyte: surface normal (3),
   // profile sum and raw surface charge of Make_phi_qq_particle (2), the two
   // ion concentrations in real space (2), the charge density to transform and
-  // the real charge density kept for the force (2)
+  // the real charge density kept for the force (2), the real velocity at the
+  // end of a step (3)
   kapsel_electrolyte el{};
   bool el_set = false;
   double el_total[2] = {0.0, 0.0};
-  void *el_f[9] = {};
+  void *el_f[12] = {};
+  // Every entry point that can change state counts api_gen up.  An
+  // electrolyte step sets el_reuse_gen to the count its successor will see
+  // if nothing runs in between; that successor then reuses what the step
+  // left: the charge spectrum, the real ion fields and the real velocity.
+  unsigned long long api_gen = 0, el_reuse_gen = ~0ull;
   double *el_partial = nullptr;   // block partial sums of the electrolyte reductions
   double *el_scal = nullptr;      // device scalars (electrolyte.hip)
   // derived physics constants (Set_global_parameters, input.cxx:287-430)
@@ -272,7 +278,8 @@ void advection_products(kapsel_ctx *ctx);
 void advection_combine_slaved(kapsel_ctx *ctx, double dt_fluid);
 void solenoidal_uk(kapsel_ctx *ctx, void *const *f3);
 void make_f_nonsole(kapsel_ctx *ctx);
-void add_f_particle(kapsel_ctx *ctx);
+// u += f over the real points; keep3: also store the sum there
+void add_f_particle(kapsel_ctx *ctx, void *const *keep3 = nullptr);
 void copy_field(kapsel_ctx *ctx, void *dst, const void *src);
 void copy_convert_in(kapsel_ctx *ctx, void *dst, const double *src_dev);
 void copy_convert_out(kapsel_ctx *ctx, double *dst_dev, const void *src);
@@ -325,6 +332,12 @@ void init_rho_ion(kapsel_ctx *ctx);
 void make_phi_qq(kapsel_ctx *ctx, void *phi, void *surface);
 void charge_field(kapsel_ctx *ctx, void *out);
 void nssolute_euler(kapsel_ctx *ctx, const kapsel_time &t);
+// NSsolute_solver_Euler inside an electrolyte step; reuse: the previous step
+// ended right before (see api_gen), so its charge spectrum, real ion fields
+// and real velocity are current
+void nssolute_step(kapsel_ctx *ctx, const kapsel_time &t, bool reuse);
+// where the electrolyte step keeps the real velocity (3 fields)
+void *const *electrolyte_u_keep(kapsel_ctx *ctx);
 void rescale_solute(kapsel_ctx *ctx, double *factor_host);
 void coulomb_force(kapsel_ctx *ctx, const kapsel_time &t);
 // Rescale_solute right after the profile scatter (PHI_SUM current) and the

@@ -31,7 +31,7 @@ constexpr int NBR = 1024;  // blocks of a reduction (fixed: deterministic sums)
 // device scalars
 enum { S_SURF = 0, S_F0, S_F1, S_C0, S_C1, S_VOL, S_N };
 // scratch fields
-enum { E_N0 = 0, E_N1, E_N2, E_QQ, E_SURF, E_CX0, E_CX1, E_RHOK, E_RHO, E_NF };
+enum { E_N0 = 0, E_N1, E_N2, E_QQ, E_SURF, E_CX0, E_CX1, E_RHOK, E_RHO, E_UX0, E_UX1, E_UX2, E_NF };
 
 __device__ __forceinline__ double block_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
@@ -193,24 +193,27 @@ __global__ void flux_k_kernel(T *__restrict__ ck, T *__restrict__ f0, T *__restr
   st2(f0 + 2 * c, V2<T>{-wx * w.im, wx * w.re});
   st2(f1 + 2 * c, V2<T>{-wy * w.im, wy * w.re});
   st2(f2 + 2 * c, V2<T>{-wz * w.im, wz * w.re});
-  st2(cx + 2 * c, a);
+  if (cx) st2(cx + 2 * c, a);
 }
 
 // Per species, real-space half of the flux: Solute_solver_rhs_nonlinear_x_single
 // (solute_rhs.cxx:150-188), Solute_impermeability (:42-97) on every point
 // whose surface normal is nonzero (exactly the points the reference's
 // particle loop projects; the projection is idempotent, so points of two
-// particles' cells need it once), Add_advection_flux (:190-210)
+// particles' cells need it once), Add_advection_flux (:190-210).  With fac,
+// the real ion field is the one kept from before the last rescale and is
+// multiplied by its factor.
 template <typename T>
 __global__ void flux_x_kernel(T *__restrict__ f0, T *__restrict__ f1, T *__restrict__ f2, const T *__restrict__ cx,
-                              const T *__restrict__ g0, const T *__restrict__ g1, const T *__restrict__ g2,
-                              const T *__restrict__ n0, const T *__restrict__ n1, const T *__restrict__ n2,
-                              const T *__restrict__ u0, const T *__restrict__ u1, const T *__restrict__ u2,
-                              double onsager, double ze, Grid g) {
+                              const double *__restrict__ fac, const T *__restrict__ g0, const T *__restrict__ g1,
+                              const T *__restrict__ g2, const T *__restrict__ n0, const T *__restrict__ n1,
+                              const T *__restrict__ n2, const T *__restrict__ u0, const T *__restrict__ u1,
+                              const T *__restrict__ u2, double onsager, double ze, Grid g) {
   size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= g.nr) return;
   const size_t im = ridx(g, r).im;
-  const T c = cx[im];
+  T c = cx[im];
+  if (fac) c *= (T)*fac;
   const T L = (T)onsager, z = (T)ze;
   T a0 = f0[im] + c * (L * (g0[im] * z));
   T a1 = f1[im] + c * (L * (g1[im] * z));
@@ -265,6 +268,23 @@ __global__ void coulomb_force_kernel(T *__restrict__ f0, T *__restrict__ f1, T *
   v = -f2[im];
   v += (T)e2;
   f2[im] = q * v;
+}
+
+// U2advection_k, real-space part (operate_omega.cxx:30-47), from the kept
+// real velocity: products into U0..U2 and WORK0/WORK1
+template <typename T>
+__global__ void products_kernel(const T *__restrict__ x0, const T *__restrict__ x1, const T *__restrict__ x2,
+                                T *__restrict__ u0, T *__restrict__ u1, T *__restrict__ u2, T *__restrict__ a0,
+                                T *__restrict__ a1, Grid g) {
+  size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= g.nr) return;
+  const size_t im = ridx(g, r).im;
+  const T v1 = x0[im], v2 = x1[im], v3 = x2[im];
+  u0[im] = v2 * v3;
+  u1[im] = v3 * v1;
+  u2[im] = v1 * v2;
+  a0[im] = (v2 - v3) * (v2 + v3);
+  a1[im] = (v3 - v1) * (v3 + v1);
 }
 
 // uniform ion densities on the real points, 0 in the row padding
@@ -369,42 +389,57 @@ void coulomb_grad_t(kapsel_ctx *ctx, const T *rhok, T *const *g3, int trunc, con
 // (fluid_solver.cxx:184-338), non-Dielectric branch.  Buffers: grad_potential
 // in F0..F2 and solute_flux in UP0..UP2 as in the reference's call
 // (:328), the real c_n in WORK2, the right-hand side of zeta in WORK0/WORK1.
+// reuse: the previous electrolyte step ended right before, so E_RHOK holds
+// the charge spectrum of the current ions and particles (its Coulomb force),
+// E_CX0/E_CX1 times the rescale factors are the real ion fields (the spectra
+// are 2/3-cut already: the Euler step only touches kept modes and the cut
+// modes were zeroed), and E_UX0..2 the real velocity whose spectrum zeta
+// holds (solenoidal and cut, so Zeta_k2u gives it back).
 template <typename T>
-void nssolute_t(kapsel_ctx *ctx, const kapsel_time &t) {
+void nssolute_t(kapsel_ctx *ctx, const kapsel_time &t, bool reuse) {
   const Grid &g = ctx->g;
-  // charge density of the current state (concentrations not yet cut)
-  ions_real_t<T>(ctx);
   T *rhok = ef<T>(ctx, E_RHOK);
-  charge_t<T>(ctx, rhok, nullptr, 0);
-  void *one[1] = {rhok};
-  fft_r2c_t<T>(ctx, one, 1);
+  if (!reuse) {
+    // charge density of the current state (concentrations not yet cut)
+    ions_real_t<T>(ctx);
+    charge_t<T>(ctx, rhok, nullptr, 0);
+    void *one[1] = {rhok};
+    fft_r2c_t<T>(ctx, one, 1);
+  }
   const double amp = (double)(g.nx * g.ny * g.nz) * ac_factor(ctx, t);
   const double dc[3] = {ctx->el.e_ext[0] * amp, ctx->el.e_ext[1] * amp, ctx->el.e_ext[2] * amp};
   T *grad[3] = {fl<T>(ctx, KAPSEL_F0), fl<T>(ctx, KAPSEL_F1), fl<T>(ctx, KAPSEL_F2)};
   coulomb_grad_t<T>(ctx, rhok, grad, 1, dc);
   fft_c2r_t<T>(ctx, (void *const *)grad, 3);
   // u of the truncated zeta (Rhs_NS_solute :203-207)
-  zeta_k2u_k(ctx, 0);
-  fft_c2r_t<T>(ctx, ctx->fields + KAPSEL_U0, 3);
+  T *u[3];
+  if (reuse) {
+    for (int d = 0; d < 3; d++) u[d] = ef<T>(ctx, E_UX0 + d);
+  } else {
+    zeta_k2u_k(ctx, 0);
+    fft_c2r_t<T>(ctx, ctx->fields + KAPSEL_U0, 3);
+    for (int d = 0; d < 3; d++) u[d] = fl<T>(ctx, KAPSEL_U0 + d);
+  }
   make_surface_normal(ctx, ctx->el_f + E_N0);
   T *flux[3] = {fl<T>(ctx, KAPSEL_UP0), fl<T>(ctx, KAPSEL_UP1), fl<T>(ctx, KAPSEL_UP2)};
-  T *cx = fl<T>(ctx, KAPSEL_WORK2);
   for (int n = 0; n < 2; n++) {
     T *ck = fl<T>(ctx, KAPSEL_CONC0 + n);
+    T *cx = reuse ? ef<T>(ctx, E_CX0 + n) : fl<T>(ctx, KAPSEL_WORK2);
+    const double *fac = reuse ? ctx->el_scal + S_F0 + n : nullptr;
     {
-      LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, (2.0 * g.f + 2.0 * cut_count_el(g) + 3.0 * g.f) * sizeof(T));
+      LaunchTimer lt(ctx, KAPSEL_K_POINTWISE,
+                     (2.0 * g.f + 2.0 * cut_count_el(g) + (reuse ? 3.0 : 4.0) * g.f) * sizeof(T));
       hipLaunchKernelGGL(flux_k_kernel<T>, dim3(blocks_for(g.nc, TPB)), dim3(TPB), 0, ctx->stream, ck, flux[0],
-                         flux[1], flux[2], cx, ctx->el.kbt * ctx->el.onsager_coeff[n], g);
+                         flux[1], flux[2], reuse ? nullptr : cx, ctx->el.kbt * ctx->el.onsager_coeff[n], g);
       KAPSEL_LAUNCH_CHECK();
     }
     void *four[4] = {flux[0], flux[1], flux[2], cx};
-    fft_c2r_t<T>(ctx, four, 4);
+    fft_c2r_t<T>(ctx, four, reuse ? 3 : 4);
     {
       LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, 16.0 * g.nr * sizeof(T));
       hipLaunchKernelGGL(flux_x_kernel<T>, dim3(blocks_for(g.nr, TPB)), dim3(TPB), 0, ctx->stream, flux[0],
-                         flux[1], flux[2], cx, grad[0], grad[1], grad[2], ef<T>(ctx, E_N0), ef<T>(ctx, E_N1),
-                         ef<T>(ctx, E_N2), fl<T>(ctx, KAPSEL_U0), fl<T>(ctx, KAPSEL_U1), fl<T>(ctx, KAPSEL_U2),
-                         ctx->el.onsager_coeff[n], valency_e(ctx, n), g);
+                         flux[1], flux[2], cx, fac, grad[0], grad[1], grad[2], ef<T>(ctx, E_N0), ef<T>(ctx, E_N1),
+                         ef<T>(ctx, E_N2), u[0], u[1], u[2], ctx->el.onsager_coeff[n], valency_e(ctx, n), g);
       KAPSEL_LAUNCH_CHECK();
     }
     fft_r2c_t<T>(ctx, (void *const *)flux, 3);
@@ -416,7 +451,15 @@ void nssolute_t(kapsel_ctx *ctx, const kapsel_time &t) {
     }
   }
   // Rhs_solvent (U2advection_k) + viscous term + Euler step of zeta
-  advection_products(ctx);
+  if (reuse) {
+    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, 8.0 * g.nr * sizeof(T));
+    hipLaunchKernelGGL(products_kernel<T>, dim3(blocks_for(g.nr, TPB)), dim3(TPB), 0, ctx->stream, u[0], u[1],
+                       u[2], fl<T>(ctx, KAPSEL_U0), fl<T>(ctx, KAPSEL_U1), fl<T>(ctx, KAPSEL_U2),
+                       fl<T>(ctx, KAPSEL_WORK0), fl<T>(ctx, KAPSEL_WORK1), g);
+    KAPSEL_LAUNCH_CHECK();
+  } else {
+    advection_products(ctx);
+  }
   void *five[5] = {ctx->fields[KAPSEL_U0], ctx->fields[KAPSEL_U1], ctx->fields[KAPSEL_U2],
                    ctx->fields[KAPSEL_WORK0], ctx->fields[KAPSEL_WORK1]};
   fft_r2c_t<T>(ctx, five, 5);
@@ -570,8 +613,15 @@ void charge_field(kapsel_ctx *ctx, void *out) {
 
 void nssolute_euler(kapsel_ctx *ctx, const kapsel_time &t) {
   need_el(ctx);
-  EL_DISPATCH(ctx, nssolute_t, ctx, t);
+  EL_DISPATCH(ctx, nssolute_t, ctx, t, false);
 }
+
+void nssolute_step(kapsel_ctx *ctx, const kapsel_time &t, bool reuse) {
+  need_el(ctx);
+  EL_DISPATCH(ctx, nssolute_t, ctx, t, reuse);
+}
+
+void *const *electrolyte_u_keep(kapsel_ctx *ctx) { return ctx->el_f + E_UX0; }
 
 template <typename T>
 static void rescale_solute_t(kapsel_ctx *ctx, double *factor_host) {

@@ -183,6 +183,10 @@ __global__ void advection_combine_slaved_kernel(const T *__restrict__ u0, const 
       f1 = V2<T>{f1.re + -(nk2 * zb.re), f1.im + -(nk2 * zb.im)};
       st2(z0 + 2 * c, V2<T>{za.re + dt * f0.re, za.im + dt * f0.im});
       st2(z1 + 2 * c, V2<T>{zb.re + dt * f1.re, zb.im + dt * f1.im});
+    } else {
+      // Rhs_NS_solute cut zeta before (fluid_solver.cxx:201-203)
+      st2(z0 + 2 * c, V2<T>{0, 0});
+      st2(z1 + 2 * c, V2<T>{0, 0});
     }
     st2(a0 + 2 * c, f0);
     st2(a1 + 2 * c, f1);
@@ -243,13 +247,17 @@ __global__ void f_nonsole_kernel(P5<T> u, P5<T> up, P5<T> f, const T *__restrict
 
 // Add_f_particle (f_particle.h:79-95): u += f
 template <typename T>
-__global__ void add_f_kernel(P5<T> u, P5<T> f, Grid g) {
+__global__ void add_f_kernel(P5<T> u, P5<T> f, P5<T> keep, Grid g) {
   size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= g.nr) return;
   size_t row = r / (unsigned)g.nz;
   size_t im = row * g.nz_ + (r - row * g.nz);
 #pragma unroll
-  for (int d = 0; d < 3; d++) u.p[d][im] += f.p[d][im];
+  for (int d = 0; d < 3; d++) {
+    const T v = u.p[d][im] + f.p[d][im];
+    u.p[d][im] = v;
+    if (keep.p[0]) keep.p[d][im] = v;
+  }
 }
 
 template <typename D, typename S>
@@ -378,13 +386,14 @@ static void make_f_nonsole_t(kapsel_ctx *ctx) {
 void make_f_nonsole(kapsel_ctx *ctx) { KAPSEL_DISPATCH(ctx, make_f_nonsole_t, ctx); }
 
 template <typename T>
-static void add_f_particle_t(kapsel_ctx *ctx) {
-  LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, 9.0 * ctx->g.nr * sizeof(T));
+static void add_f_particle_t(kapsel_ctx *ctx, void *const *keep3) {
+  LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, (keep3 ? 12.0 : 9.0) * ctx->g.nr * sizeof(T));
   hipLaunchKernelGGL(add_f_kernel<T>, dim3(blocks_for(ctx->g.nr, TPB)), dim3(TPB), 0, ctx->stream,
-                     p5<T>(ctx->fields + KAPSEL_U0, 3), p5<T>(ctx->fields + KAPSEL_F0, 3), ctx->g);
+                     p5<T>(ctx->fields + KAPSEL_U0, 3), p5<T>(ctx->fields + KAPSEL_F0, 3),
+                     keep3 ? p5<T>(keep3, 3) : P5<T>{}, ctx->g);
   KAPSEL_LAUNCH_CHECK();
 }
-void add_f_particle(kapsel_ctx *ctx) { KAPSEL_DISPATCH(ctx, add_f_particle_t, ctx); }
+void add_f_particle(kapsel_ctx *ctx, void *const *keep3) { KAPSEL_DISPATCH(ctx, add_f_particle_t, ctx, keep3); }
 
 template <typename T>
 __global__ void copy_kernel(T *__restrict__ dst, const T *__restrict__ src, size_t n) {

@@ -173,6 +173,35 @@ def test_electrolyte_step_fp64_matches_oracle(ac):
     assert max(m1.values()) < 1e-8 and max(m10.values()) < 1e-8
 
 
+def test_electrolyte_step_reuse_matches_recompute():
+    """Consecutive steps reuse what the previous step left (charge spectrum,
+    real ion fields, real velocity); a state-changing call in between makes
+    the next step recompute them.  Both ways agree to rounding."""
+    n, npart = 64, 8
+    print(f"case electrolyte reuse vs recompute {n}^3", flush=True)
+    p, st, el = _pair(n, npart, q=60.0, e_ext=(0.05, 0.0, 0.02))
+    ks = [gpu(p), gpu(p)]
+    _load(ks, st, el)
+    for x in ks:
+        x.init_rigidity()
+        x.init_rho_ion()
+    dt = workloads.electrolyte_auto_dt(p, el)
+    ts = [abi.make_time(dt), abi.make_time(dt)]
+    for _ in range(6):
+        ks[0].electrolyte_step(ts[0])
+        uk = ks[1].get_uk_dc()
+        ks[1].set_uk_dc(uk)  # changes nothing, but forces the recompute path
+        ks[1].electrolyte_step(ts[1])
+    e = {}
+    for f in ("ZETA0", "ZETA1", "CONC0", "CONC1", "PHI"):
+        e[f] = rel(ks[0].get_field(f), ks[1].get_field(f))
+    pa, pb = ks[0].get_particles(), ks[1].get_particles()
+    for f in ("X", "V", "OMEGA"):
+        e[f] = rel(pa[abi.PFIELD[f]], pb[abi.PFIELD[f]])
+    print({f: f"{v:.1e}" for f, v in e.items()}, flush=True)
+    assert max(e.values()) < 1e-12, e
+
+
 def test_electrolyte_fp32_against_fp64():
     n, npart = 128, 120
     print(f"case electrolyte fp32 vs fp64 {n}^3 {npart} spheres", flush=True)
