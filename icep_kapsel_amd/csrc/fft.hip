@@ -254,11 +254,17 @@ __global__ __launch_bounds__(RPB *FS<NZ / 2>::R2) void fft_z_c2r(FieldPtrs<T> fp
   T *__restrict__ rp = data + (valid ? row : 0) * nz_;
   V2<T> *sr = sm + r * RS;
 
-  for (int k = n2; k <= M; k += TPL) {
-    V2<T> x = valid ? ld2(rp + 2 * k) : V2<T>{0, 0};
-    if (k == 0 || k == M) x.im = 0;  // c2r treats the kz = 0, NZ/2 entries as real
-    sr[k] = x;
-  }
+  // all loads of the row first (rp points at row 0 for an invalid row, so
+  // every load is in bounds), then the LDS stores: no load waits on a store
+  constexpr int NJ = M / TPL;
+  V2<T> xin[NJ];
+#pragma unroll
+  for (int j = 0; j < NJ; j++) xin[j] = ld2(rp + 2 * (n2 + TPL * j));
+  V2<T> xlast = ld2(rp + 2 * M);
+  if (n2 == 0) xin[0].im = 0;  // c2r treats the kz = 0, NZ/2 entries as real
+#pragma unroll
+  for (int j = 0; j < NJ; j++) sr[n2 + TPL * j] = xin[j];
+  if (n2 == 0) sr[M] = V2<T>{xlast.re, (T)0};
   __syncthreads();
   V2<T> v[R1], o[R1];
 #pragma unroll
