@@ -91,6 +91,17 @@ def make_electrolyte(kbt=1.0, elementary_charge=1.0, dielectric_cst=1.0, surface
     return e
 
 
+class RestartData(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("n_particles", ctypes.c_int), ("n_spec", ctypes.c_int),
+        ("ts", ctypes.c_int), ("time", ctypes.c_double), ("uk_dc", ctypes.c_double * 3),
+        ("zeta0", ctypes.POINTER(ctypes.c_double)), ("zeta1", ctypes.POINTER(ctypes.c_double)),
+        ("particles", ctypes.POINTER(ctypes.c_double)), ("orientations", ctypes.POINTER(ctypes.c_double)),
+        ("conc0", ctypes.POINTER(ctypes.c_double)), ("conc1", ctypes.POINTER(ctypes.c_double)),
+    ]
+
+
 def make_time(dt, ts=0, time=0.0):
     """CTime as main() initialises it (sp_3d_ns.cxx:412)."""
     return Time(ts, time, dt, 0.5 * dt, dt, 0.5 * dt)
@@ -137,6 +148,8 @@ _PROTOS = {
     "kapsel_get_uk_dc": [_DP],
     "kapsel_set_particles": [_DP],
     "kapsel_get_particles": [_DP],
+    "kapsel_set_orientations": [_DP],
+    "kapsel_get_orientations": [_DP],
     "kapsel_copy_field": [_I, _I],
     "kapsel_fft_r2c": [_I],
     "kapsel_fft_c2r": [_I],
@@ -163,6 +176,8 @@ _PROTOS = {
     "kapsel_set_dielectric": [_D, _D, _D],
     "kapsel_maxwell_force": [_DP],
     "kapsel_icep_step": [ctypes.POINTER(Time), _DP, _D, _I, _I, _IP, _DP],
+    "kapsel_save_restart": [ctypes.c_char_p, ctypes.POINTER(Time), ctypes.c_char_p],
+    "kapsel_load_restart": [ctypes.c_char_p, ctypes.POINTER(Time)],
     "kapsel_set_electrolyte": [ctypes.POINTER(Electrolyte)],
     "kapsel_init_rho_ion": [],
     "kapsel_get_total_solute": [_DP],
@@ -205,6 +220,11 @@ def load_library(path=None):
         fn = getattr(lib, name)
         fn.restype = _I
         fn.argtypes = [_VP] + args
+    # host-only entry points (no context)
+    lib.kapsel_restart_write.restype = _I
+    lib.kapsel_restart_write.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(RestartData)]
+    lib.kapsel_restart_read.restype = _I
+    lib.kapsel_restart_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(RestartData)]
     if path == LIB_PATH:
         _lib = lib
     return lib

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <exception>
 #include <new>
+#include <vector>
 
 #include "common.hpp"
 
@@ -114,6 +115,7 @@ static void destroy(kapsel_ctx *ctx) {
     if (p) (void)hipFree(p);
   if (ctx->d_uk_dc) (void)hipFree(ctx->d_uk_dc);
   if (ctx->d_ps) (void)hipFree(ctx->d_ps);
+  if (ctx->d_q) (void)hipFree(ctx->d_q);
   if (ctx->d_sekibun) (void)hipFree(ctx->d_sekibun);
   free_twiddles(ctx);
   poisson_free(ctx);
@@ -215,6 +217,14 @@ static void create(const kapsel_params *p, kapsel_ctx **out) {
     size_t np = p->n_particles > 0 ? (size_t)p->n_particles : 1;
     KAPSEL_HIP(hipMalloc(&ctx->d_ps, sizeof(double) * KAPSEL_P_NFIELDS * 3 * np));
     KAPSEL_HIP(hipMemsetAsync(ctx->d_ps, 0, sizeof(double) * KAPSEL_P_NFIELDS * 3 * np, ctx->stream));
+    {
+      // orientations: q = q_old = identity (1, 0, 0, 0)
+      std::vector<double> q(8 * np, 0.0);
+      for (size_t i = 0; i < np; i++) q[i] = q[4 * np + i] = 1.0;
+      KAPSEL_HIP(hipMalloc(&ctx->d_q, sizeof(double) * 8 * np));
+      KAPSEL_HIP(hipMemcpyAsync(ctx->d_q, q.data(), sizeof(double) * 8 * np, hipMemcpyHostToDevice, ctx->stream));
+      KAPSEL_HIP(hipStreamSynchronize(ctx->stream));
+    }
     build_sekibun(ctx);
     init_twiddles(ctx);
     KAPSEL_HIP(hipStreamSynchronize(ctx->stream));
@@ -527,6 +537,45 @@ int kapsel_get_particles(kapsel_ctx *ctx, double *state) {
   size_t n = sizeof(double) * KAPSEL_P_NFIELDS * 3 * ctx->P.n_particles;
   if (n) {
     KAPSEL_HIP(hipMemcpyAsync(state, ctx->d_ps, n, hipMemcpyDeviceToHost, ctx->stream));
+    KAPSEL_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  API_END
+}
+
+int kapsel_set_orientations(kapsel_ctx *ctx, const double *q) {
+  API_BEGIN
+  check_ctx(ctx);
+  const int np = ctx->P.n_particles;
+  if (!q && np > 0) {
+    set_error("null orientations");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+  for (int i = 0; i < np; i++)
+    for (int h = 0; h < 2; h++) {
+      double s2 = 0.0;
+      for (int c = 0; c < 4; c++) s2 += q[(size_t)(4 * h + c) * np + i] * q[(size_t)(4 * h + c) * np + i];
+      if (!(fabs(sqrt(s2) - 1.0) < 1e-6)) {  // qtn_isnormal (quaternion.h:272-281)
+        set_error("particle %d: %s is not a unit quaternion (norm %g)", i, h ? "q_old" : "q", sqrt(s2));
+        throw Error{KAPSEL_ERR_ARG};
+      }
+    }
+  if (np > 0) {
+    KAPSEL_HIP(hipMemcpyAsync(ctx->d_q, q, sizeof(double) * 8 * np, hipMemcpyHostToDevice, ctx->stream));
+    KAPSEL_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  API_END
+}
+
+int kapsel_get_orientations(kapsel_ctx *ctx, double *q) {
+  API_BEGIN
+  check_ctx_ro(ctx);
+  const int np = ctx->P.n_particles;
+  if (!q && np > 0) {
+    set_error("null output");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+  if (np > 0) {
+    KAPSEL_HIP(hipMemcpyAsync(q, ctx->d_q, sizeof(double) * 8 * np, hipMemcpyDeviceToHost, ctx->stream));
     KAPSEL_HIP(hipStreamSynchronize(ctx->stream));
   }
   API_END
@@ -981,7 +1030,124 @@ int kapsel_electrolyte_step(kapsel_ctx *ctx, kapsel_time *jikan) {
     set_error("electrolyte parameters not set (kapsel_set_electrolyte)");
     throw Error{KAPSEL_ERR_STATE};
   }
+  if (!(ctx->el_total[0] > 0) || !(ctx->el_total[1] > 0)) {
+    set_error("ion totals not set (kapsel_init_rho_ion, kapsel_set_total_solute, or kapsel_load_restart "
+              "after kapsel_set_electrolyte)");
+    throw Error{KAPSEL_ERR_STATE};
+  }
   electrolyte_step(ctx, *jikan);
+  API_END
+}
+
+int kapsel_restart_write(const char *path, const char *template_path, const kapsel_restart_data *data) {
+  API_BEGIN
+  restart_write(path, template_path, data);
+  API_END
+}
+
+int kapsel_restart_read(const char *path, kapsel_restart_data *data) {
+  API_BEGIN
+  restart_read(path, data);
+  API_END
+}
+
+// host buffers of a context's restart state
+struct RestartBuffers {
+  std::vector<double> z0, z1, ps, q, c0, c1;
+  kapsel_restart_data d{};
+  explicit RestartBuffers(kapsel_ctx *ctx) {
+    const size_t f = ctx->g.f;
+    const int np = ctx->P.n_particles;
+    z0.resize(f);
+    z1.resize(f);
+    ps.resize((size_t)KAPSEL_P_NFIELDS * 3 * (np > 0 ? np : 1));
+    q.resize((size_t)8 * (np > 0 ? np : 1));
+    d.nx = ctx->g.nx;
+    d.ny = ctx->g.ny;
+    d.nz = ctx->g.nz;
+    d.n_particles = np;
+    d.n_spec = ctx->el_set ? 2 : 0;
+    d.zeta0 = z0.data();
+    d.zeta1 = z1.data();
+    d.particles = ps.data();
+    d.orientations = q.data();
+    if (ctx->el_set) {
+      c0.resize(f);
+      c1.resize(f);
+      d.conc0 = c0.data();
+      d.conc1 = c1.data();
+    }
+  }
+};
+
+int kapsel_save_restart(kapsel_ctx *ctx, const char *path, const kapsel_time *jikan, const char *template_path) {
+  API_BEGIN
+  check_ctx_ro(ctx);
+  if (!jikan) {
+    set_error("null jikan");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+  RestartBuffers b(ctx);
+  download(ctx, b.d.zeta0, fld(ctx, KAPSEL_ZETA0));
+  download(ctx, b.d.zeta1, fld(ctx, KAPSEL_ZETA1));
+  KAPSEL_HIP(hipMemcpyAsync(b.d.uk_dc, ctx->d_uk_dc, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  const int np = ctx->P.n_particles;
+  if (np > 0) {
+    KAPSEL_HIP(hipMemcpyAsync(b.d.particles, ctx->d_ps, sizeof(double) * KAPSEL_P_NFIELDS * 3 * np,
+                              hipMemcpyDeviceToHost, ctx->stream));
+    KAPSEL_HIP(hipMemcpyAsync(b.d.orientations, ctx->d_q, sizeof(double) * 8 * np, hipMemcpyDeviceToHost,
+                              ctx->stream));
+  }
+  if (ctx->el_set) {
+    download(ctx, b.d.conc0, fld(ctx, KAPSEL_CONC0));
+    download(ctx, b.d.conc1, fld(ctx, KAPSEL_CONC1));
+  }
+  KAPSEL_HIP(hipStreamSynchronize(ctx->stream));
+  b.d.ts = jikan->ts;
+  b.d.time = jikan->time;
+  restart_write(path, template_path, &b.d);
+  API_END
+}
+
+int kapsel_load_restart(kapsel_ctx *ctx, const char *path, kapsel_time *jikan) {
+  API_BEGIN
+  check_ctx(ctx);
+  if (!jikan) {
+    set_error("null jikan");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+  RestartBuffers b(ctx);
+  restart_read(path, &b.d);
+  const double L[3] = {ctx->g.lx, ctx->g.ly, ctx->g.lz};
+  const int np = ctx->P.n_particles;
+  for (int d = 0; d < 3; d++)
+    for (int i = 0; i < np; i++) {
+      const double x = b.d.particles[((size_t)KAPSEL_P_X * 3 + d) * np + i];
+      if (!(x >= 0.0 && x < L[d])) {
+        set_error("%s: particle %d position[%d] = %g outside [0, %g)", path, i, d, x, L[d]);
+        throw Error{KAPSEL_ERR_ARG};
+      }
+    }
+  upload(ctx, fld(ctx, KAPSEL_ZETA0), b.d.zeta0);
+  upload(ctx, fld(ctx, KAPSEL_ZETA1), b.d.zeta1);
+  KAPSEL_HIP(hipMemcpyAsync(ctx->d_uk_dc, b.d.uk_dc, 3 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  if (np > 0) {
+    KAPSEL_HIP(hipMemcpyAsync(ctx->d_ps, b.d.particles, sizeof(double) * KAPSEL_P_NFIELDS * 3 * np,
+                              hipMemcpyHostToDevice, ctx->stream));
+    KAPSEL_HIP(hipMemcpyAsync(ctx->d_q, b.d.orientations, sizeof(double) * 8 * np, hipMemcpyHostToDevice,
+                              ctx->stream));
+  }
+  if (ctx->el_set) {
+    upload(ctx, fld(ctx, KAPSEL_CONC0), b.d.conc0);
+    upload(ctx, fld(ctx, KAPSEL_CONC1), b.d.conc1);
+    recount_total_solute(ctx);
+  }
+  KAPSEL_HIP(hipStreamSynchronize(ctx->stream));
+  // the Poisson warm starts belong to the state before the load
+  ctx->pot64_valid = false;
+  ctx->prev_valid = false;
+  jikan->ts = b.d.ts;
+  jikan->time = b.d.time;
   API_END
 }
 

@@ -168,6 +168,7 @@ struct kapsel_ctx {
   void *fields[KAPSEL_NFIELDS] = {};
   double *d_uk_dc = nullptr;      // 3 doubles (device)
   double *d_ps = nullptr;         // particle state (device), SoA
+  double *d_q = nullptr;          // orientations q, q_old (device): [c*np + n], c < 8
   int *d_sekibun = nullptr;       // np_domain * 3 ints
   int np_domain = 0;
   std::vector<int> sekibun;       // host copy
@@ -195,7 +196,11 @@ struct kapsel_ctx {
   const void *kcg_key[11] = {};
   int n_partial = 0;
   kapsel::Pnet *pnet = nullptr;   // learned warm-start network, on first use
-  // electrolyte (electrolyte.hip): parameters, the conserved ion amounts, and
+  // restart files (restart.hip), host only
+void restart_write(const char *path, const char *template_path, const kapsel_restart_data *d);
+void restart_read(const char *path, kapsel_restart_data *d);
+
+// electrolyte (electrolyte.hip): parameters, the conserved ion amounts, and
   // scratch fields allocated by kapsel_set_electrolyte: surface normal (3),
   // profile sum and raw surface charge of Make_phi_qq_particle (2), the two
   // ion concentrations in real space (2), the charge density to transform and
@@ -325,6 +330,10 @@ void make_surface_normal(kapsel_ctx *ctx, void *const *n3);
 // the forward Euler step with the viscous term on the kept modes (kspace.hip)
 void advection_combine_euler(kapsel_ctx *ctx, double dt_fluid);
 
+// restart files (restart.hip), host only
+void restart_write(const char *path, const char *template_path, const kapsel_restart_data *d);
+void restart_read(const char *path, kapsel_restart_data *d);
+
 // electrolyte (electrolyte.hip)
 void electrolyte_set(kapsel_ctx *ctx, const kapsel_electrolyte &el);
 void electrolyte_free(kapsel_ctx *ctx);
@@ -343,5 +352,7 @@ void coulomb_force(kapsel_ctx *ctx, const kapsel_time &t);
 // Rescale_solute right after the profile scatter (PHI_SUM current) and the
 // Coulomb force of the rescaled ions into F0..F2
 void rescale_and_coulomb_force(kapsel_ctx *ctx, const kapsel_time &t);
+// Count_solute_each (solute_rhs.h:143-156) into the conserved totals
+void recount_total_solute(kapsel_ctx *ctx);
 
 }  // namespace kapsel

@@ -572,6 +572,20 @@ void electrolyte_set(kapsel_ctx *ctx, const kapsel_electrolyte &el) {
   ctx->el_set = true;
 }
 
+template <typename T>
+static void recount_t(kapsel_ctx *ctx) {
+  make_phi_particle(ctx, ctx->fields[KAPSEL_PHI]);
+  ions_real_t<T>(ctx);
+  reduce<T, 1>(ctx, ctx->el_f[E_CX0], ctx->el_f[E_CX1], ctx->fields[KAPSEL_PHI]);
+  finish(ctx, 1, 0.0, 1.0, 1.0);
+  ctx->el_total[0] = read_scal(ctx, S_C0);
+  ctx->el_total[1] = read_scal(ctx, S_C1);
+}
+void recount_total_solute(kapsel_ctx *ctx) {
+  need_el(ctx);
+  EL_DISPATCH(ctx, recount_t, ctx);
+}
+
 void electrolyte_free(kapsel_ctx *ctx) {
   for (void *&p : ctx->el_f)
     if (p) {

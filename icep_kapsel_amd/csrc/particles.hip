@@ -461,7 +461,28 @@ __global__ __launch_bounds__(256) void force_kernel(double *__restrict__ ps, Phy
 }
 
 // MD_solver_position_Euler/_AB2 (particle_solver.cxx:39-91), PBC (periodic_boundary.h:9-19)
-__global__ void md_position_kernel(double *__restrict__ ps, Phys p, int euler, double dt, double hdt) {
+// quaternion product alpha a.b (quaternion.h:173-183), (s, v) layout
+__device__ __forceinline__ void qprod(double *q, const double *a, const double *b, double alpha) {
+  q[0] = alpha * (a[0] * b[0] - (a[1] * b[1] + a[2] * b[2] + a[3] * b[3]));
+  q[1] = alpha * (a[0] * b[1] + b[0] * a[1] + a[2] * b[3] - a[3] * b[2]);
+  q[2] = alpha * (a[0] * b[2] + b[0] * a[2] + a[3] * b[1] - a[1] * b[3]);
+  q[3] = alpha * (a[0] * b[3] + b[0] * a[3] + a[1] * b[2] - a[2] * b[1]);
+}
+// q.(0,v).q* (qtn_qvq_prod, quaternion.h:207-216)
+__device__ __forceinline__ void qvq(double *out, const double *q, const double *v) {
+  const double qs2 = q[0] * q[0] - (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+  const double qdotp = q[1] * v[0] + q[2] * v[1] + q[3] * v[2];
+  out[0] = qs2 * v[0] + 2.0 * (qdotp * q[1] + q[0] * (q[2] * v[2] - q[3] * v[1]));
+  out[1] = qs2 * v[1] + 2.0 * (qdotp * q[2] + q[0] * (q[3] * v[0] - q[1] * v[2]));
+  out[2] = qs2 * v[2] + 2.0 * (qdotp * q[3] + q[0] * (q[1] * v[1] - q[2] * v[0]));
+}
+
+// MD_solver_position_Euler/_AB2 (particle_solver.cxx:39-91) and, with
+// ROTATION, MD_solver_orientation_Euler (space-frame qdot) / _AB2 (body-frame
+// qdot of 3 w_b - w_b,old) (particle_rotation_solver.h:15-39, rigid_body.cxx:
+// 10-24, 151-165), each followed by qtn_normalize
+__global__ void md_position_kernel(double *__restrict__ ps, double *__restrict__ qa, Phys p, int euler, double dt,
+                                   double hdt) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= p.np) return;
 #pragma unroll
@@ -473,6 +494,44 @@ __global__ void md_position_kernel(double *__restrict__ ps, Phys p, int euler, d
     pref(ps, KAPSEL_P_X_NOPBC, d, n, p.np) += dx;
     x += dx;
     pref(ps, KAPSEL_P_X, d, n, p.np) = fmod(x + p.l[d], p.l[d]);
+  }
+  if (!p.rotation) return;
+  double q[4], qo[4], om[3], dq[4];
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    q[c] = qa[(size_t)c * p.np + n];
+    qo[c] = qa[(size_t)(4 + c) * p.np + n];
+  }
+#pragma unroll
+  for (int d = 0; d < 3; d++) om[d] = pval(ps, KAPSEL_P_OMEGA, d, n, p.np);
+  if (euler) {
+#pragma unroll
+    for (int c = 0; c < 4; c++) qo[c] = q[c];
+    const double qw[4] = {0.0, om[0], om[1], om[2]};
+    qprod(dq, qw, q, 0.5);
+#pragma unroll
+    for (int c = 0; c < 4; c++) q[c] += dq[c] * dt;
+  } else {
+    double omo[3], wb[3], wbo[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) omo[d] = pval(ps, KAPSEL_P_OMEGA_OLD, d, n, p.np);
+    const double qc[4] = {q[0], -q[1], -q[2], -q[3]}, qoc[4] = {qo[0], -qo[1], -qo[2], -qo[3]};
+    qvq(wb, qc, om);
+    qvq(wbo, qoc, omo);
+#pragma unroll
+    for (int d = 0; d < 3; d++) wb[d] = 3.0 * wb[d] - wbo[d];
+#pragma unroll
+    for (int c = 0; c < 4; c++) qo[c] = q[c];
+    const double qw[4] = {0.0, wb[0], wb[1], wb[2]};
+    qprod(dq, q, qw, 0.5);
+#pragma unroll
+    for (int c = 0; c < 4; c++) q[c] += dq[c] * hdt;
+  }
+  const double qnorm = 1.0 / sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    qa[(size_t)c * p.np + n] = q[c] * qnorm;
+    qa[(size_t)(4 + c) * p.np + n] = qo[c];
   }
 }
 
@@ -562,7 +621,7 @@ void md_position(kapsel_ctx *ctx, const kapsel_time &t) {
   if (ctx->P.n_particles <= 0) return;
   LaunchTimer lt(ctx, KAPSEL_K_PARTICLE, 8.0 * KAPSEL_P_NFIELDS * 3 * ctx->P.n_particles);
   hipLaunchKernelGGL(md_position_kernel, dim3(blocks_for(ctx->P.n_particles, 64)), dim3(64), 0, ctx->stream,
-                     ctx->d_ps, phys(ctx), t.ts == 0 ? 1 : 0, t.dt_md, t.hdt_md);
+                     ctx->d_ps, ctx->d_q, phys(ctx), t.ts == 0 ? 1 : 0, t.dt_md, t.hdt_md);
   KAPSEL_LAUNCH_CHECK();
 }
 

@@ -75,6 +75,16 @@ class KapselGPU:
         self._call("get_uk_dc", _dp(a))
         return a
 
+    def set_orientations(self, q):
+        """q: (8, Np) -- q0..q3 then q_old0..q_old3 (scalar part first)."""
+        a = np.ascontiguousarray(q, dtype=np.float64).reshape(8, self.np)
+        self._call("set_orientations", _dp(a))
+
+    def get_orientations(self):
+        a = np.zeros((8, self.np))
+        self._call("get_orientations", _dp(a))
+        return a
+
     def set_particles(self, state):
         a = np.ascontiguousarray(state, dtype=np.float64).reshape(abi.P_NFIELDS, 3, self.np)
         self._call("set_particles", _dp(a))
@@ -233,6 +243,16 @@ class KapselGPU:
         g = ctypes.c_double(0.0)
         self._call("poisson_galerkin_start", _dp(e), ctypes.byref(g))
         return g.value
+
+    # restart files -------------------------------------------------------------
+    def save_restart(self, path, t, template=None):
+        """Save_Restart_udf: the context state as a UDF resume record."""
+        tp = template.encode() if template else None
+        self._call("save_restart", str(path).encode(), ctypes.byref(t), tp)
+
+    def load_restart(self, path, t):
+        """Force_restore_parameters: state from a UDF resume record; sets t.ts, t.time."""
+        self._call("load_restart", str(path).encode(), ctypes.byref(t))
 
     # electrolyte -------------------------------------------------------------
     def set_electrolyte(self, el):

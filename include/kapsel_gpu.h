@@ -123,6 +123,13 @@ int kapsel_set_uk_dc(kapsel_ctx *ctx, const double uk_dc[3]);
 int kapsel_get_uk_dc(kapsel_ctx *ctx, double uk_dc[3]);
 int kapsel_set_particles(kapsel_ctx *ctx, const double *state);
 int kapsel_get_particles(kapsel_ctx *ctx, double *state);
+/* Orientation quaternions Particle.q and q_old (variable.h:27-77), scalar
+ * part first: q[c*n_particles + i] for c = 0..3, q_old at c = 4..7.
+ * kapsel_md_position advances them when rotation is on
+ * (MD_solver_orientation_Euler/_AB2, particle_rotation_solver.h:15-39);
+ * kapsel_create sets both to the identity. */
+int kapsel_set_orientations(kapsel_ctx *ctx, const double *q);
+int kapsel_get_orientations(kapsel_ctx *ctx, double *q);
 
 /* Copy_v1_k (aux_field.h:82): dst = src over the whole padded field. */
 int kapsel_copy_field(kapsel_ctx *ctx, int dst, int src);
@@ -300,6 +307,46 @@ int kapsel_coulomb_force(kapsel_ctx *ctx, const kapsel_time *jikan);
  * time += dt_fluid, ts += 1.  The reference's first Calc_f_hydro of the
  * branch (:60-68) only fills the diagnostic f_hydro1 and is not repeated. */
 int kapsel_electrolyte_step(kapsel_ctx *ctx, kapsel_time *jikan);
+
+/* ---- restart files: the reference's UDF restart record -------------------- */
+/* Saved state of Save_Restart_udf / Force_restore_parameters (resume.cxx:
+ * 10-78, 308-398): zeta and the ion spectra in the padded k-space layout
+ * (NX*NY*NZ_ doubles each), uk_dc, the particle state (layout of
+ * kapsel_get_particles) and orientations (layout of kapsel_get_orientations),
+ * jikan.  Members of the reference's sParticle this library does not keep
+ * (f_hydro1, torque_hydro1, slip forces and torques) are written as zero and
+ * skipped on reading; x_previous is set to x on reading, as the reference
+ * does (resume.cxx:409-410).  Buffers stay owned by the caller. */
+typedef struct kapsel_restart_data {
+  int nx, ny, nz;              /* grid */
+  int n_particles;
+  int n_spec;                  /* 0: no ion concentrations, 2: conc0/conc1 */
+  int ts;                      /* jikan.ts */
+  double time;                 /* jikan.time */
+  double uk_dc[3];
+  double *zeta0, *zeta1;       /* NX*NY*NZ_ each */
+  double *particles;           /* KAPSEL_P_NFIELDS*3*n_particles */
+  double *orientations;        /* 8*n_particles */
+  double *conc0, *conc1;       /* NX*NY*NZ_ each when n_spec == 2 */
+} kapsel_restart_data;
+/* Host only (no device needed).  Writes the record resume.CONTINUE.Saved_Data
+ * in UDF data syntax (UDF/define.udf:317-338).  template_path == NULL: a
+ * file holding only that record; otherwise the template UDF with its resume
+ * record replaced (or added before \end{data}), e.g. the run's input UDF. */
+int kapsel_restart_write(const char *path, const char *template_path, const kapsel_restart_data *data);
+/* Host only.  Reads resume.CONTINUE.Saved_Data of a UDF file into the
+ * caller's buffers; the grid, particle count and (for n_spec == 2) the
+ * concentrations must match the file. */
+int kapsel_restart_read(const char *path, kapsel_restart_data *data);
+/* Save_Restart_udf of a context: its zeta, uk_dc, particles, orientations
+ * and (once kapsel_set_electrolyte has run) ion concentrations, with jikan. */
+int kapsel_save_restart(kapsel_ctx *ctx, const char *path, const kapsel_time *jikan, const char *template_path);
+/* Force_restore_parameters into a context: sets zeta, uk_dc, particles,
+ * orientations, jikan->ts and jikan->time (the time steps are the caller's)
+ * and, for an electrolyte context, the ion concentrations, whose conserved
+ * totals (not part of the record) are recounted from them
+ * (Count_solute_each, solute_rhs.h:143-156). */
+int kapsel_load_restart(kapsel_ctx *ctx, const char *path, kapsel_time *jikan);
 
 /* ---- kernel timing (HIP events on the context stream) --------------------- */
 enum kapsel_kernel_class {

@@ -107,6 +107,7 @@ struct Oracle {
   double uk_dc[3];
   int Np;
   std::vector<double> ps;   /* particle state, SoA (kapsel_gpu.h) */
+  std::vector<double> qs;   /* orientations q (c < 4) and q_old (c = 4..7) at [c*Np + n] */
   int NP_domain;
   std::vector<int> sekibun; /* 3 ints per cell point */
 
@@ -170,6 +171,8 @@ struct Oracle {
     uk_dc[0] = uk_dc[1] = uk_dc[2] = 0;
     Np = P.n_particles;
     ps.assign((size_t)KAPSEL_P_NFIELDS * 3 * std::max(Np, 1), 0.0);
+    qs.assign((size_t)8 * std::max(Np, 1), 0.0);
+    for (int n = 0; n < Np; n++) qs[n] = qs[(size_t)4 * Np + n] = 1.0;
     particle_domain();
     return 0;
   }
@@ -775,10 +778,58 @@ struct Oracle {
       }
   }
 
+  /* quaternion product q = alpha qa.qb (quaternion.h:173-183); q = (s, v) */
+  static void qprod(double *q, const double *a, const double *b, double alpha) {
+    q[0] = alpha * (a[0] * b[0] - (a[1] * b[1] + a[2] * b[2] + a[3] * b[3]));
+    q[1] = alpha * (a[0] * b[1] + b[0] * a[1] + a[2] * b[3] - a[3] * b[2]);
+    q[2] = alpha * (a[0] * b[2] + b[0] * a[2] + a[3] * b[1] - a[1] * b[3]);
+    q[3] = alpha * (a[0] * b[3] + b[0] * a[3] + a[1] * b[2] - a[2] * b[1]);
+  }
+  /* v' = q.(0,v).q* (qtn_qvq_prod, quaternion.h:207-216) */
+  static void qvq(double *out, const double *q, const double *v) {
+    double qs2 = q[0] * q[0] - (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    double qdotp = q[1] * v[0] + q[2] * v[1] + q[3] * v[2];
+    out[0] = qs2 * v[0] + 2.0 * (qdotp * q[1] + q[0] * (q[2] * v[2] - q[3] * v[1]));
+    out[1] = qs2 * v[1] + 2.0 * (qdotp * q[2] + q[0] * (q[3] * v[0] - q[1] * v[2]));
+    out[2] = qs2 * v[2] + 2.0 * (qdotp * q[3] + q[0] * (q[1] * v[1] - q[2] * v[0]));
+  }
+  /* qtn_normalize (quaternion.h:283-292) */
+  static void qnormalize(double *q) {
+    double qnorm = 1.0 / sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    for (int c = 0; c < 4; c++) q[c] *= qnorm;
+  }
+  /* MD_solver_orientation_Euler / _AB2 (particle_rotation_solver.h:15-39)
+   * with qdot (rigid_body.cxx:151-165) and rigid_body_rotation SPACE2BODY
+   * (rigid_body.cxx:10-24) */
+  void md_orientation(int n, bool euler, double dt, double hdt) {
+    double q[4], qo[4], om[3], omo[3];
+    for (int c = 0; c < 4; c++) { q[c] = qs[(size_t)c * Np + n]; qo[c] = qs[(size_t)(4 + c) * Np + n]; }
+    for (int d = 0; d < 3; d++) { om[d] = pf(KAPSEL_P_OMEGA, d)[n]; omo[d] = pf(KAPSEL_P_OMEGA_OLD, d)[n]; }
+    double dq[4];
+    if (euler) {
+      for (int c = 0; c < 4; c++) qo[c] = q[c];
+      const double qw[4] = {0.0, om[0], om[1], om[2]};
+      qprod(dq, qw, q, 0.5);                       /* SPACE_FRAME */
+      for (int c = 0; c < 4; c++) q[c] += dq[c] * dt;
+    } else {
+      double wb[3], wbo[3];
+      const double qc[4] = {q[0], -q[1], -q[2], -q[3]}, qoc[4] = {qo[0], -qo[1], -qo[2], -qo[3]};
+      qvq(wb, qc, om);
+      qvq(wbo, qoc, omo);
+      for (int d = 0; d < 3; d++) wb[d] = 3.0 * wb[d] - wbo[d];
+      for (int c = 0; c < 4; c++) qo[c] = q[c];
+      const double qw[4] = {0.0, wb[0], wb[1], wb[2]};
+      qprod(dq, q, qw, 0.5);                       /* BODY_FRAME */
+      for (int c = 0; c < 4; c++) q[c] += dq[c] * hdt;
+    }
+    qnormalize(q);
+    for (int c = 0; c < 4; c++) { qs[(size_t)c * Np + n] = q[c]; qs[(size_t)(4 + c) * Np + n] = qo[c]; }
+  }
+
   /* MD_solver_position_Euler/_AB2 (particle_solver.cxx:39-91); PBC
-   * (periodic_boundary.h:9-19) */
+   * (periodic_boundary.h:9-19); orientations with ROTATION */
   void md_position(const kapsel_time &jikan) {
-    for (int n = 0; n < Np; n++)
+    for (int n = 0; n < Np; n++) {
       for (int d = 0; d < 3; d++) {
         double &x = pf(KAPSEL_P_X, d)[n];
         pf(KAPSEL_P_X_PREVIOUS, d)[n] = x;
@@ -788,6 +839,8 @@ struct Oracle {
         x += delta_x;
         x = fmod(x + L[d], L[d]);
       }
+      if (P.rotation) md_orientation(n, jikan.ts == 0, jikan.dt_md, jikan.hdt_md);
+    }
   }
 
   /* MD_solver_velocity_Euler / _AB2_hydro (particle_solver.cxx:109-150, 252-295) */
@@ -1825,6 +1878,14 @@ int oracle_icep_step(void *h, kapsel_time *t, const double *e0, double tol, int 
   return 0;
 }
 int oracle_init_rigidity(void *h) { O->init_rigidity(); return 0; }
+int oracle_set_orientations(void *h, const double *q) {
+  memcpy(O->qs.data(), q, sizeof(double) * 8 * O->Np);
+  return 0;
+}
+int oracle_get_orientations(void *h, double *q) {
+  memcpy(q, O->qs.data(), sizeof(double) * 8 * O->Np);
+  return 0;
+}
 int oracle_spm_step(void *h, kapsel_time *t) { O->spm_step(*t); return 0; }
 
 /* electrolyte */

@@ -53,6 +53,7 @@ def _load():
             "oracle_poisson_solve": [dp, ctypes.c_double, i, i, ip, dp],
             "oracle_maxwell_force": [dp], "oracle_add_force_to_fluid": [tp],
             "oracle_icep_step": [tp, dp, ctypes.c_double, i, i, ip, dp],
+            "oracle_set_orientations": [dp], "oracle_get_orientations": [dp],
             "oracle_set_electrolyte": [ctypes.POINTER(abi.Electrolyte)], "oracle_init_rho_ion": [],
             "oracle_get_total_solute": [dp], "oracle_set_total_solute": [dp],
             "oracle_make_phi_qq": [i, i], "oracle_make_surface_normal": [i], "oracle_charge_field": [i],
@@ -135,6 +136,16 @@ class Oracle:
     def get_particles(self):
         a = np.zeros((abi.P_NFIELDS, 3, self.np))
         self._call("get_particles", _dp(a))
+        return a
+
+    def set_orientations(self, q):
+        """q: (8, Np) -- q0..q3 then q_old0..q_old3 (scalar part first)."""
+        a = np.ascontiguousarray(q, dtype=np.float64).reshape(8, self.np)
+        self._call("set_orientations", _dp(a))
+
+    def get_orientations(self):
+        a = np.zeros((8, self.np))
+        self._call("get_orientations", _dp(a))
         return a
 
     def sekibun(self):

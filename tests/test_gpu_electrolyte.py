@@ -141,6 +141,7 @@ def _compare(k, o, n):
     for f in ("X", "V", "OMEGA", "F_HYDRO_PREVIOUS", "TORQUE_HYDRO_PREVIOUS"):
         i = abi.PFIELD[f]
         worst[f] = rel(pk[i], po[i])
+    worst["orientation"] = rel(k.get_orientations(), o.get_orientations())
     worst["totals"] = float(np.abs(k.get_total_solute() - o.get_total_solute()).max()
                             / np.abs(o.get_total_solute()).max())
     return worst, means

@@ -212,16 +212,22 @@ def test_particle_ops_match_oracle(n, npart, a):
         x.force()
     i = abi.PFIELD["FR"]
     assert rel(k.get_particles()[i], o.get_particles()[i]) < 1e-12
+    # random unit orientations (q and q_old) for the quaternion update
+    q = rng.standard_normal((8, npart))
+    q[:4] /= np.linalg.norm(q[:4], axis=0)
+    q[4:] /= np.linalg.norm(q[4:], axis=0)
     for ts in (0, 3):
         tt = abi.make_time(0.05, ts=ts)
         for x in (k, o):
             x.set_particles(st)
+            x.set_orientations(q)
             x.md_position(tt)
             x.md_velocity(tt)
         pk, po = k.get_particles(), o.get_particles()
         for f in abi.PFIELDS:
             i = abi.PFIELD[f]
             assert np.abs(pk[i] - po[i]).max() <= 1e-12 * max(1e-30, np.abs(po[i]).max()), (ts, f)
+        assert np.abs(k.get_orientations() - o.get_orientations()).max() < 1e-14, ts
 
 
 def _compare_step_state(k, o, n, tol):
@@ -235,6 +241,7 @@ def _compare_step_state(k, o, n, tol):
     for f in ("X", "V", "OMEGA", "F_HYDRO_PREVIOUS", "TORQUE_HYDRO_PREVIOUS", "FR_PREVIOUS"):
         i = abi.PFIELD[f]
         worst[f] = rel(pk[i], po[i])
+    worst["orientation q, q_old"] = rel(k.get_orientations(), o.get_orientations())
     bad = {f: v for f, v in worst.items() if not v < tol}
     return worst, bad
 

@@ -242,3 +242,58 @@ def test_md_update_closed_form():
     out = o.get_particles()
     assert np.allclose(out[abi.PFIELD["V"]].T, vel + dt / (2 * mass) * (2 * fh + fr_prev), rtol=1e-14, atol=1e-16)
     assert np.allclose(out[abi.PFIELD["OMEGA"]].T, om + dt / (2 * moi) * 2 * th, rtol=1e-14, atol=1e-16)
+
+
+def _qmul(a, b):
+    """Hamilton product of quaternions (s, v) -- written from the definition."""
+    s = a[0] * b[0] - np.dot(a[1:], b[1:])
+    v = a[0] * b[1:] + b[0] * a[1:] + np.cross(a[1:], b[1:])
+    return np.concatenate([[s], v])
+
+
+def _rotate_body(q, w):
+    """q* (0, w) q: a space-frame vector seen in the body frame."""
+    qc = np.concatenate([[q[0]], -q[1:]])
+    return _qmul(_qmul(qc, np.concatenate([[0.0], w])), q)[1:]
+
+
+def test_orientation_update_closed_form():
+    """Constant angular velocity: one Euler step then AB2 steps.  Both
+    schemes advance q by q <- normalize(q (1, dt/2 w_b)) with w_b the body-frame
+    angular velocity (constant here, since q turns about w_b itself), so after
+    N steps q_N = q0 (cos N beta, sin N beta w_b/|w_b|), tan beta = |w| dt/2."""
+    n = 32
+    rng = np.random.default_rng(12)
+    p = abi.make_params(n, n_particles=2, radius=3.0, xi=2.0, rotation=1)
+    o = _oracle(p)
+    pos = np.array([[5.0, 6.0, 7.0], [20.0, 21.0, 22.0]])
+    om = np.array([[0.03, -0.02, 0.05], [-0.01, 0.04, 0.02]])
+    st = workloads.particle_state(pos, omegas=om)
+    st[abi.PFIELD["OMEGA_OLD"]] = om.T
+    o.set_particles(st)
+    q0 = rng.standard_normal((2, 4))
+    q0 /= np.linalg.norm(q0, axis=1, keepdims=True)
+    q = np.zeros((8, 2))
+    q[:4] = q0.T
+    q[4:] = q0.T
+    o.set_orientations(q)
+    dt, steps = 0.3, 25
+    for ts in range(steps):
+        o.md_position(abi.make_time(dt, ts=ts))
+    got = o.get_orientations()
+    for i in range(2):
+        wb = _rotate_body(q0[i], om[i])
+        beta = np.arctan(np.linalg.norm(om[i]) * dt / 2)
+        r = np.concatenate([[np.cos(steps * beta)], np.sin(steps * beta) * wb / np.linalg.norm(wb)])
+        expect = _qmul(q0[i], r)
+        assert np.abs(got[:4, i] - expect).max() < 1e-13
+        r_prev = np.concatenate([[np.cos((steps - 1) * beta)],
+                                 np.sin((steps - 1) * beta) * wb / np.linalg.norm(wb)])
+        assert np.abs(got[4:, i] - _qmul(q0[i], r_prev)).max() < 1e-13
+    # rotation off: orientations untouched
+    p2 = abi.make_params(n, n_particles=2, radius=3.0, xi=2.0, rotation=0)
+    o2 = _oracle(p2)
+    o2.set_particles(st)
+    o2.set_orientations(q)
+    o2.md_position(abi.make_time(dt, ts=3))
+    assert np.array_equal(o2.get_orientations(), q)
