@@ -121,6 +121,7 @@ static void destroy(kapsel_ctx *ctx) {
   poisson_free(ctx);
   pnet_free(ctx);
   electrolyte_free(ctx);
+  thermostat_free(ctx);
   for (auto &r : ctx->timing_pending) {
     (void)hipEventDestroy(r.start);
     (void)hipEventDestroy(r.stop);
@@ -341,6 +342,7 @@ static void spm_step(kapsel_ctx *ctx, kapsel_time &t) {
   calc_f_hydro(ctx, t);                       // :99-101
   force(ctx);                                 // :104-110
   md_velocity_integrate(ctx, t);
+  thermostat(ctx, t);                         // :155-157, with kBT > 0
   make_u_particle_sum(ctx);                   // :166-167
   f_particle_dt_sole_add(ctx);                // :168-169
   u2zeta_k(ctx);                              // :184
@@ -1036,6 +1038,28 @@ int kapsel_electrolyte_step(kapsel_ctx *ctx, kapsel_time *jikan) {
     throw Error{KAPSEL_ERR_STATE};
   }
   electrolyte_step(ctx, *jikan);
+  API_END
+}
+
+int kapsel_set_thermostat(kapsel_ctx *ctx, double kbt, double alpha_v, double alpha_o, unsigned long seed) {
+  API_BEGIN
+  check_ctx(ctx);
+  if (!(kbt >= 0.0) || !(alpha_v >= 0.0) || !(alpha_o >= 0.0)) {
+    set_error("kBT, alpha_v and alpha_o must be >= 0");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+  ctx->thermo_kbt = kbt;
+  ctx->thermo_alpha_v = alpha_v;
+  ctx->thermo_alpha_o = alpha_o;
+  ctx->mt.seed((std::mt19937::result_type)(seed & 0xffffffffUL));  // init_genrand (32-bit seed)
+  API_END
+}
+
+int kapsel_thermostat(kapsel_ctx *ctx, const kapsel_time *jikan) {
+  API_BEGIN
+  check_ctx(ctx);
+  check_time(jikan);
+  thermostat(ctx, *jikan);
   API_END
 }
 

@@ -6,6 +6,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -206,6 +207,14 @@ void restart_read(const char *path, kapsel_restart_data *d);
   // ion concentrations in real space (2), the charge density to transform and
   // the real charge density kept for the force (2), the real velocity at the
   // end of a step (3)
+  // thermostat (particles.hip): parameters, the MT19937 stream (host), pinned
+  // double-buffered deviates and their device copy
+  double thermo_kbt = 0.0, thermo_alpha_v = 0.0, thermo_alpha_o = 0.0;
+  std::mt19937 mt{5489u};
+  double *h_noise[2] = {};
+  hipEvent_t noise_ev[2] = {};
+  double *d_noise = nullptr;
+  int noise_slot = 0;
   kapsel_electrolyte el{};
   bool el_set = false;
   double el_total[2] = {0.0, 0.0};
@@ -321,6 +330,9 @@ void pnet_warm_start(kapsel_ctx *ctx, double beta, const double e0[3], double sc
 void calc_f_hydro(kapsel_ctx *ctx, const kapsel_time &t);
 void force(kapsel_ctx *ctx);
 void md_velocity_integrate(kapsel_ctx *ctx, const kapsel_time &t);
+// Add_random_force_thermostat (fluct.cxx:9-40)
+void thermostat(kapsel_ctx *ctx, const kapsel_time &t);
+void thermostat_free(kapsel_ctx *ctx);
 // Make_phi_qq_particle before its normalisation: unclamped profile sum into
 // phi, sum of Q_e DPhi_compact_sin into surface (particles.hip)
 void make_phi_qq_raw(kapsel_ctx *ctx, void *phi, void *surface, double q_e);

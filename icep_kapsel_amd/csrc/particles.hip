@@ -535,6 +535,19 @@ __global__ void md_position_kernel(double *__restrict__ ps, double *__restrict__
   }
 }
 
+// Add_random_force_thermostat (fluct.cxx:22-35): the kicks of one step,
+// deviates noise[k*np + n] (k < 6: velocity then angular velocity)
+__global__ void thermostat_kernel(double *__restrict__ ps, const double *__restrict__ noise, Phys p, double nv,
+                                  double imass, double no, double imoi) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= p.np) return;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    pref(ps, KAPSEL_P_V, d, n, p.np) += noise[(size_t)d * p.np + n] * nv * imass;
+    if (p.rotation) pref(ps, KAPSEL_P_OMEGA, d, n, p.np) += noise[(size_t)(d + 3) * p.np + n] * no * imoi;
+  }
+}
+
 // velocity update of MD_solver_velocity_Euler/_AB2_hydro (particle_solver.cxx:
 // 109-150, 252-295) followed by reset_Forces (particle_solver.cxx:11-37)
 __global__ void md_velocity_kernel(double *__restrict__ ps, Phys p, int euler, double dmy, double dmy_rot) {
@@ -693,6 +706,72 @@ void force(kapsel_ctx *ctx) {
   hipLaunchKernelGGL(force_kernel, dim3(ctx->P.n_particles), dim3(threads), 0, ctx->stream, ctx->d_ps, phys(ctx),
                      lj);
   KAPSEL_LAUNCH_CHECK();
+}
+
+namespace {
+// genrand_real3 of mt19937ar.c: (int32 + 0.5) / 2^32, in (0, 1)
+double real3(std::mt19937 &mt) { return ((double)mt() + 0.5) * (1.0 / 4294967296.0); }
+// Gauss2 (fluct.h:52-63): polar method, two deviates
+void gauss2(std::mt19937 &mt, double *r) {
+  double x1, x2, rsq;
+  do {
+    x1 = 2. * real3(mt) - 1.;
+    x2 = 2. * real3(mt) - 1.;
+    rsq = x1 * x1 + x2 * x2;
+  } while (rsq >= 1.0 || rsq == 0.0);
+  const double factor = sqrt(-2. * log(rsq) / rsq);
+  r[0] = factor * x1;
+  r[1] = factor * x2;
+}
+}  // namespace
+
+void thermostat(kapsel_ctx *ctx, const kapsel_time &t) {
+  const int np = ctx->P.n_particles;
+  if (np <= 0 || !(ctx->thermo_kbt > 0.0)) return;
+  const size_t bytes = sizeof(double) * 6 * np;
+  if (!ctx->d_noise) {
+    KAPSEL_HIP(hipMalloc(&ctx->d_noise, bytes));
+    for (int s = 0; s < 2; s++) {
+      KAPSEL_HIP(hipHostMalloc(&ctx->h_noise[s], bytes, hipHostMallocDefault));
+      KAPSEL_HIP(hipEventCreateWithFlags(&ctx->noise_ev[s], hipEventDisableTiming));
+    }
+  }
+  // the copy that last read this host buffer has finished before it is refilled
+  const int s = ctx->noise_slot;
+  ctx->noise_slot ^= 1;
+  KAPSEL_HIP(hipEventSynchronize(ctx->noise_ev[s]));
+  double *h = ctx->h_noise[s];
+  for (int n = 0; n < np; n++) {
+    double dmy[6];
+    gauss2(ctx->mt, dmy);
+    gauss2(ctx->mt, dmy + 2);
+    gauss2(ctx->mt, dmy + 4);
+    for (int k = 0; k < 6; k++) h[(size_t)k * np + n] = dmy[k];
+  }
+  KAPSEL_HIP(hipMemcpyAsync(ctx->d_noise, h, bytes, hipMemcpyHostToDevice, ctx->stream));
+  KAPSEL_HIP(hipEventRecord(ctx->noise_ev[s], ctx->stream));
+  // Zeta_drag, Zeta_drag_rot, sdv_*, noise_intensity_* (fluct.cxx:10-15;
+  // kT_snap_v/_o = 1.439/1.22, input.cxx:276-278)
+  const double a = ctx->P.radius;
+  const double zeta_drag = 6. * M_PI * ctx->P.eta * a;
+  const double zeta_drag_rot = 8. * M_PI * ctx->P.eta * a * a * a;
+  const double nv = 1.439 * sqrt(zeta_drag * t.dt_md * ctx->thermo_kbt * ctx->thermo_alpha_v);
+  const double no = 1.22 * sqrt(zeta_drag_rot * t.dt_md * ctx->thermo_kbt * ctx->thermo_alpha_o);
+  LaunchTimer lt(ctx, KAPSEL_K_PARTICLE, 8.0 * 12 * np);
+  hipLaunchKernelGGL(thermostat_kernel, dim3(blocks_for(np, 64)), dim3(64), 0, ctx->stream, ctx->d_ps,
+                     ctx->d_noise, phys(ctx), nv, ctx->imass, no, ctx->imoi);
+  KAPSEL_LAUNCH_CHECK();
+}
+
+void thermostat_free(kapsel_ctx *ctx) {
+  if (ctx->d_noise) (void)hipFree(ctx->d_noise);
+  for (int s = 0; s < 2; s++) {
+    if (ctx->h_noise[s]) (void)hipHostFree(ctx->h_noise[s]);
+    if (ctx->noise_ev[s]) (void)hipEventDestroy(ctx->noise_ev[s]);
+    ctx->h_noise[s] = nullptr;
+    ctx->noise_ev[s] = nullptr;
+  }
+  ctx->d_noise = nullptr;
 }
 
 void md_velocity_integrate(kapsel_ctx *ctx, const kapsel_time &t) {

@@ -244,6 +244,13 @@ class KapselGPU:
         self._call("poisson_galerkin_start", _dp(e), ctypes.byref(g))
         return g.value
 
+    # thermal fluctuations -------------------------------------------------------
+    def set_thermostat(self, kbt, alpha_v, alpha_o, seed=0):
+        self._call("set_thermostat", float(kbt), float(alpha_v), float(alpha_o), int(seed))
+
+    def thermostat(self, t):
+        self._call("thermostat", ctypes.byref(t))
+
     # restart files -------------------------------------------------------------
     def save_restart(self, path, t, template=None):
         """Save_Restart_udf: the context state as a UDF resume record."""

@@ -308,6 +308,19 @@ int kapsel_coulomb_force(kapsel_ctx *ctx, const kapsel_time *jikan);
  * branch (:60-68) only fills the diagnostic f_hydro1 and is not repeated. */
 int kapsel_electrolyte_step(kapsel_ctx *ctx, kapsel_time *jikan);
 
+/* ---- thermal fluctuations ------------------------------------------------ */
+/* kBT, alpha_v, alpha_o of the Navier_Stokes equation (input.cxx:280-281,
+ * 539-540) and the seed of the random stream (MT_seed, fluct.h:34-46:
+ * init_genrand of MT19937; the reference seeds 0, sp_3d_ns.cxx:402).  With
+ * kbt > 0, kapsel_spm_step applies Add_random_force_thermostat after the
+ * velocity update (sp_3d_ns.cxx:155-157); the ICEP and electrolyte steps do
+ * not (the reference's Electrolyte branch skips it).  kbt = 0 turns it off. */
+int kapsel_set_thermostat(kapsel_ctx *ctx, double kbt, double alpha_v, double alpha_o, unsigned long seed);
+/* Add_random_force_thermostat (fluct.cxx:9-40), free particles: Gauss2
+ * deviates (fluct.h:52-63) of the context's MT19937 stream, three pairs per
+ * particle in particle order, drawn on the host and applied on the device. */
+int kapsel_thermostat(kapsel_ctx *ctx, const kapsel_time *jikan);
+
 /* ---- restart files: the reference's UDF restart record -------------------- */
 /* Saved state of Save_Restart_udf / Force_restore_parameters (resume.cxx:
  * 10-78, 308-398): zeta and the ion spectra in the padded k-space layout

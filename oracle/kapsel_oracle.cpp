@@ -30,6 +30,7 @@
  */
 #include <algorithm>
 #include <cmath>
+#include <random>
 #include <complex>
 #include <cstdio>
 #include <cstdlib>
@@ -908,6 +909,7 @@ struct Oracle {
     double *u[3] = {U(0), U(1), U(2)};
     calc_f_hydro(fld(KAPSEL_PHI_SUM), u, jikan);
     md_velocity(jikan);
+    if (thermo_on && thermo_kbt > 0.) thermostat(jikan);  /* sp_3d_ns.cxx:155-157 */
     for (int d = 0; d < 3; d++) reset(UP(d));
     double *up[3] = {UP(0), UP(1), UP(2)};
     make_u_particle_sum(up, fld(KAPSEL_PHI_SUM));
@@ -1289,6 +1291,49 @@ struct Oracle {
     jikan.time += jikan.dt_fluid;
     jikan.ts += 1;
     return it;
+  }
+
+  /* -------------------------------------------------------- thermostat */
+  /* kBT, alpha_v, alpha_o (input.cxx:280-281, 539-540) and the MT19937
+   * stream of mt19937ar.c seeded by init_genrand (MT_seed, fluct.h:34-46);
+   * std::mt19937 is the same generator with the same seeding */
+  bool thermo_on = false;
+  double thermo_kbt = 0.0, thermo_alpha_v = 0.0, thermo_alpha_o = 0.0;
+  std::mt19937 mt{5489u};
+  /* genrand_real3: (int32 + 0.5) / 2^32, in (0, 1) */
+  double real3() { return ((double)mt() + 0.5) * (1.0 / 4294967296.0); }
+  /* Gauss2 (fluct.h:52-63): polar method, two deviates */
+  void gauss2(double *r) {
+    double x1, x2, rsq;
+    do {
+      x1 = 2. * real3() - 1.;
+      x2 = 2. * real3() - 1.;
+      rsq = x1 * x1 + x2 * x2;
+    } while (rsq >= 1.0 || rsq == 0.0);
+    double factor = sqrt(-2. * log(rsq) / rsq);
+    r[0] = factor * x1;
+    r[1] = factor * x2;
+  }
+  /* Add_random_force_thermostat (fluct.cxx:9-40), free particles, in
+   * particle order; kT_snap_v/_o of input.cxx:276-278 */
+  void thermostat(const kapsel_time &jikan) {
+    const double kT_snap_v = 1.439, kT_snap_o = 1.22;
+    const double zeta_drag = 6. * M_PI * P.eta * RADIUS;
+    const double zeta_drag_rot = 8. * M_PI * P.eta * RADIUS * RADIUS * RADIUS;
+    const double sdv_v = sqrt(zeta_drag * jikan.dt_md * thermo_kbt * thermo_alpha_v);
+    const double sdv_omega = sqrt(zeta_drag_rot * jikan.dt_md * thermo_kbt * thermo_alpha_o);
+    const double noise_intensity_v = kT_snap_v * sdv_v;
+    const double noise_intensity_o = kT_snap_o * sdv_omega;
+    for (int n = 0; n < Np; n++) {
+      double dmy[6];
+      gauss2(dmy);
+      gauss2(dmy + 2);
+      gauss2(dmy + 4);
+      for (int d = 0; d < 3; d++) {
+        pf(KAPSEL_P_V, d)[n] += dmy[d] * noise_intensity_v * IMASS;
+        if (P.rotation) pf(KAPSEL_P_OMEGA, d)[n] += dmy[d + 3] * noise_intensity_o * IMOI;
+      }
+    }
   }
 
   /* ------------------------------------------------------- electrolyte */
@@ -1878,6 +1923,20 @@ int oracle_icep_step(void *h, kapsel_time *t, const double *e0, double tol, int 
   return 0;
 }
 int oracle_init_rigidity(void *h) { O->init_rigidity(); return 0; }
+int oracle_set_thermostat(void *h, double kbt, double alpha_v, double alpha_o, unsigned long seed) {
+  O->thermo_on = true;
+  O->thermo_kbt = kbt;
+  O->thermo_alpha_v = alpha_v;
+  O->thermo_alpha_o = alpha_o;
+  O->mt.seed((std::mt19937::result_type)(seed & 0xffffffffUL));
+  return 0;
+}
+int oracle_thermostat(void *h, const kapsel_time *t) { O->thermostat(*t); return 0; }
+/* n values of the oracle's genrand_real3 stream (tests of the generator) */
+int oracle_mt_real3(void *h, int n, double *out) {
+  for (int i = 0; i < n; i++) out[i] = O->real3();
+  return 0;
+}
 int oracle_set_orientations(void *h, const double *q) {
   memcpy(O->qs.data(), q, sizeof(double) * 8 * O->Np);
   return 0;

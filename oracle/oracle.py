@@ -54,6 +54,8 @@ def _load():
             "oracle_maxwell_force": [dp], "oracle_add_force_to_fluid": [tp],
             "oracle_icep_step": [tp, dp, ctypes.c_double, i, i, ip, dp],
             "oracle_set_orientations": [dp], "oracle_get_orientations": [dp],
+            "oracle_set_thermostat": [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_ulong],
+            "oracle_thermostat": [tp], "oracle_mt_real3": [i, dp],
             "oracle_set_electrolyte": [ctypes.POINTER(abi.Electrolyte)], "oracle_init_rho_ion": [],
             "oracle_get_total_solute": [dp], "oracle_set_total_solute": [dp],
             "oracle_make_phi_qq": [i, i], "oracle_make_surface_normal": [i], "oracle_charge_field": [i],
@@ -136,6 +138,17 @@ class Oracle:
     def get_particles(self):
         a = np.zeros((abi.P_NFIELDS, 3, self.np))
         self._call("get_particles", _dp(a))
+        return a
+
+    def set_thermostat(self, kbt, alpha_v, alpha_o, seed=0):
+        self._call("set_thermostat", float(kbt), float(alpha_v), float(alpha_o), int(seed))
+
+    def thermostat(self, t):
+        self._call("thermostat", ctypes.byref(t))
+
+    def mt_real3(self, n):
+        a = np.zeros(int(n))
+        self._call("mt_real3", int(n), _dp(a))
         return a
 
     def set_orientations(self, q):
@@ -292,6 +305,26 @@ class Oracle:
 
     def electrolyte_step(self, t):
         self._call("electrolyte_step", ctypes.byref(t))
+
+
+class MTRef:
+    """The reference's own mt19937ar.c (built from source into _ref/)."""
+
+    def __init__(self):
+        path = os.path.join(_HERE, "_ref", "libmt_ref.so")
+        if not os.path.exists(path):
+            raise OSError(f"reference generator not built: {path}")
+        self.lib = ctypes.CDLL(path)
+        self.lib.init_genrand.argtypes = [ctypes.c_ulong]
+        self.lib.init_genrand.restype = None
+        self.lib.genrand_real3.argtypes = []
+        self.lib.genrand_real3.restype = ctypes.c_double
+
+    def seed(self, s):
+        self.lib.init_genrand(int(s))
+
+    def real3(self, n):
+        return np.array([self.lib.genrand_real3() for _ in range(int(n))])
 
 
 class OouraRef:

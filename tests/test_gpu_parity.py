@@ -288,3 +288,25 @@ def test_spm_step_fp64_matches_oracle_c2():
     worst, bad = _compare_step_state(k, o, n, 1e-12)
     print("c2 one-step worst", {f: f"{v:.1e}" for f, v in worst.items()}, flush=True)
     assert not bad, bad
+
+
+def test_spm_step_thermostat_matches_oracle():
+    """64^3, 8 spheres, kBT = 0.4 (alpha_v = alpha_o = 1, seed 11): the same
+    MT19937 stream drives both; 5 steps at 1e-12."""
+    p, st, dt = workloads.spm_config("c1")
+    n = p.nx
+    print("case spm step thermostat c1 fp64", flush=True)
+    k, o = gpu(p), oracle(p)
+    for x in (k, o):
+        x.set_particles(st)
+        x.set_thermostat(0.4, 1.0, 1.0, 11)
+        x.init_rigidity()
+    tk, to = abi.make_time(dt), abi.make_time(dt)
+    for _ in range(5):
+        k.spm_step(tk)
+        o.spm_step(to)
+    worst, bad = _compare_step_state(k, o, n, 1e-12)
+    print("thermostat 5-step worst", {f: f"{v:.1e}" for f, v in worst.items()}, flush=True)
+    assert not bad, bad
+    v = k.get_particles()[abi.PFIELD["V"]]
+    assert np.abs(v).max() > 1e-3  # the kicks moved the spheres

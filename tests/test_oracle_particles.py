@@ -297,3 +297,79 @@ def test_orientation_update_closed_form():
     o2.set_orientations(q)
     o2.md_position(abi.make_time(dt, ts=3))
     assert np.array_equal(o2.get_orientations(), q)
+
+
+def _mt_ref():
+    from oracle.oracle import MTRef
+    try:
+        return MTRef()
+    except OSError:
+        pytest.skip("reference generator not built (oracle/_ref)")
+
+
+@pytest.mark.parametrize("seed", [0, 5489, 987654321])
+def test_generator_is_the_reference_mt19937(seed):
+    """The oracle's stream equals the reference's own mt19937ar.c (built from
+    source into oracle/_ref): init_genrand(seed), genrand_real3."""
+    ref = _mt_ref()
+    p = abi.make_params(16, n_particles=1, radius=3.0, xi=2.0)
+    o = _oracle(p)
+    o.set_thermostat(1.0, 1.0, 1.0, seed)
+    ref.seed(seed)
+    a, b = o.mt_real3(20000), ref.real3(20000)
+    assert np.array_equal(a, b)
+    assert a.min() > 0.0 and a.max() < 1.0
+
+
+def _polar(u):
+    """Gauss2 pairs from a stream of uniforms (polar method, rejection)."""
+    out, i = [], 0
+    while True:
+        x1, x2 = 2 * u[i] - 1, 2 * u[i + 1] - 1
+        i += 2
+        rsq = x1 * x1 + x2 * x2
+        if 0 < rsq < 1:
+            f = np.sqrt(-2 * np.log(rsq) / rsq)
+            yield f * x1, f * x2
+
+
+def test_thermostat_kicks_from_reference_stream():
+    """Velocity and angular-velocity kicks: Gauss2 deviates of the reference
+    generator, three pairs per particle in particle order, times
+    kT_snap sqrt(zeta dt kBT alpha) / m (or / I), zeta = 6 pi eta a
+    (8 pi eta a^3 for rotation), kT_snap = 1.439 / 1.22."""
+    ref = _mt_ref()
+    n, npart, a, eta, rho, mr = 32, 50, 3.0, 0.7, 1.3, 1.5
+    kbt, av, ao, dt, seed = 0.3, 0.8, 0.6, 0.05, 77
+    p = abi.make_params(n, n_particles=npart, radius=a, xi=2.0, eta=eta, rho=rho, mass_ratio=mr, rotation=1)
+    o = _oracle(p)
+    rng = np.random.default_rng(2)
+    pos = rng.uniform(0, n, (npart, 3))
+    o.set_particles(workloads.particle_state(pos))
+    o.set_thermostat(kbt, av, ao, seed)
+    o.thermostat(abi.make_time(dt))
+    st = o.get_particles()
+    ref.seed(seed)
+    g = _polar(ref.real3(20 * npart))
+    mass = mr * rho * 4 / 3 * np.pi * a ** 3
+    moi = 0.4 * mass * a * a
+    nv = 1.439 * np.sqrt(6 * np.pi * eta * a * dt * kbt * av)
+    no = 1.22 * np.sqrt(8 * np.pi * eta * a ** 3 * dt * kbt * ao)
+    dv = np.zeros((npart, 3))
+    dw = np.zeros((npart, 3))
+    for i in range(npart):
+        d = [x for _ in range(3) for x in next(g)]
+        dv[i] = np.array(d[:3]) * nv / mass
+        dw[i] = np.array(d[3:]) * no / moi
+    assert np.abs(st[abi.PFIELD["V"]].T - dv).max() < 1e-15 * np.abs(dv).max() * 10
+    assert np.abs(st[abi.PFIELD["OMEGA"]].T - dw).max() < 1e-15 * np.abs(dw).max() * 10
+    # deviates are standard normal: variance of the kicks over many particles
+    p2 = abi.make_params(n, n_particles=4000, radius=a, xi=2.0, eta=eta, rho=rho, mass_ratio=mr, rotation=0)
+    o2 = _oracle(p2)
+    o2.set_particles(workloads.particle_state(rng.uniform(0, n, (4000, 3))))
+    o2.set_thermostat(kbt, av, ao, 3)
+    o2.thermostat(abi.make_time(dt))
+    st2 = o2.get_particles()
+    var = np.var(st2[abi.PFIELD["V"]]) / (nv / mass) ** 2
+    assert abs(var - 1.0) < 0.05
+    assert np.all(st2[abi.PFIELD["OMEGA"]] == 0.0)  # rotation off: no angular kicks
