@@ -182,6 +182,8 @@ _PROTOS = {
     "kapsel_load_restart": [ctypes.c_char_p, ctypes.POINTER(Time)],
     "kapsel_set_electrolyte": [ctypes.POINTER(Electrolyte)],
     "kapsel_init_rho_ion": [],
+    "kapsel_init_rho_ion_pb": [ctypes.POINTER(Time), _I, _IP, _DP],
+    "kapsel_free_energy_pb": [ctypes.POINTER(Time), _DP],
     "kapsel_get_total_solute": [_DP],
     "kapsel_set_total_solute": [_DP],
     "kapsel_make_phi_qq": [_I, _I],

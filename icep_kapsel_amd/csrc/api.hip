@@ -943,6 +943,35 @@ int kapsel_init_rho_ion(kapsel_ctx *ctx) {
   API_END
 }
 
+int kapsel_init_rho_ion_pb(kapsel_ctx *ctx, const kapsel_time *jikan, int max_iter, int *iters,
+                           double free_energy[3]) {
+  API_BEGIN
+  check_ctx(ctx);
+  check_time(jikan);
+  if (!iters) {
+    set_error("null iters");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+  double fe[3];
+  *iters = init_rho_ion_pb(ctx, *jikan, max_iter > 0 ? max_iter : 10000, fe);
+  if (free_energy)
+    for (int c = 0; c < 3; c++) free_energy[c] = fe[c];
+  written(ctx, KAPSEL_PHI);
+  API_END
+}
+
+int kapsel_free_energy_pb(kapsel_ctx *ctx, const kapsel_time *jikan, double free_energy[3]) {
+  API_BEGIN
+  check_ctx(ctx);
+  check_time(jikan);
+  if (!free_energy) {
+    set_error("null output");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+  free_energy_pb(ctx, *jikan, free_energy);
+  API_END
+}
+
 int kapsel_get_total_solute(kapsel_ctx *ctx, double total[2]) {
   API_BEGIN
   check_ctx_ro(ctx);

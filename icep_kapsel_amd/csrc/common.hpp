@@ -366,5 +366,10 @@ void coulomb_force(kapsel_ctx *ctx, const kapsel_time &t);
 void rescale_and_coulomb_force(kapsel_ctx *ctx, const kapsel_time &t);
 // Count_solute_each (solute_rhs.h:143-156) into the conserved totals
 void recount_total_solute(kapsel_ctx *ctx);
+// Init_rho_ion with Poisson_Boltzmann, salt, several particles; returns the
+// relaxation steps taken; fe: the final free energy {total, ideal, electrostatic}
+int init_rho_ion_pb(kapsel_ctx *ctx, const kapsel_time &t, int max_iter, double *fe);
+// Calc_free_energy_PB, non-Dielectric branch
+void free_energy_pb(kapsel_ctx *ctx, const kapsel_time &t, double *fe);
 
 }  // namespace kapsel

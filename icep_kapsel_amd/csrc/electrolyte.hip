@@ -29,7 +29,7 @@ constexpr unsigned TPB = 256;
 constexpr int NBR = 1024;  // blocks of a reduction (fixed: deterministic sums)
 
 // device scalars
-enum { S_SURF = 0, S_F0, S_F1, S_C0, S_C1, S_VOL, S_N };
+enum { S_SURF = 0, S_F0, S_F1, S_C0, S_C1, S_VOL, S_FE_IDEAL, S_FE_EL, S_N };
 // scratch fields
 enum { E_N0 = 0, E_N1, E_N2, E_QQ, E_SURF, E_CX0, E_CX1, E_RHOK, E_RHO, E_UX0, E_UX1, E_UX2, E_NF };
 
@@ -49,10 +49,13 @@ __device__ __forceinline__ double block_sum(double v) {
 // OP 0: sum |a|                          (Make_phi_qq_particle, :218-229)
 // OP 1: sums (1 - phi) a, (1 - phi) b    (Count_single_solute, solute_rhs.cxx:130-148)
 // OP 2: sum (1 - phi)                    (Set_uniform_ion_charge_density_salt, :581-591)
+// OP 3: sum_n (1 - phi) c_n (log c_n - 1), a = c_0, b = c_1 (Calc_free_energy_PB, :370-381)
+// OP 4: sum rho (psi / 2 - E.(i,j,k) dx), a = rho, b = psi   (Calc_free_energy_PB, :388-411)
 template <typename T, int OP>
 __global__ __launch_bounds__(TPB) void reduce_kernel(const T *__restrict__ a, const T *__restrict__ b,
                                                      const T *__restrict__ phi, double *__restrict__ partial,
-                                                     Grid g) {
+                                                     Grid g, double e0 = 0.0, double e1 = 0.0, double e2 = 0.0,
+                                                     int external = 0) {
   double s0 = 0.0, s1 = 0.0;
   for (size_t r = (size_t)blockIdx.x * TPB + threadIdx.x; r < g.nr; r += (size_t)gridDim.x * TPB) {
     const RI p = ridx(g, r);
@@ -62,8 +65,17 @@ __global__ __launch_bounds__(TPB) void reduce_kernel(const T *__restrict__ a, co
       const double w = 1. - (double)phi[p.im];
       s0 += w * (double)a[p.im];
       s1 += w * (double)b[p.im];
-    } else {
+    } else if constexpr (OP == 2) {
       s0 += 1. - (double)phi[p.im];
+    } else if constexpr (OP == 3) {
+      const double w = 1. - (double)phi[p.im];
+      const double c0 = (double)a[p.im], c1 = (double)b[p.im];
+      s0 += w * c0 * (log(c0) - 1.);
+      s0 += w * c1 * (log(c1) - 1.);
+    } else {
+      double v = (double)b[p.im] * 0.5;
+      if (external) v -= ((e0 * (double)p.i + e1 * (double)p.j + e2 * (double)p.k) * g.dx);
+      s0 += (double)a[p.im] * v;
     }
   }
   s0 = block_sum(s0);
@@ -96,8 +108,12 @@ __global__ __launch_bounds__(TPB) void finish_kernel(const double *__restrict__ 
     scal[S_C1] = c1;
     scal[S_F0] = t0 / c0;
     scal[S_F1] = t1 / c1;
-  } else {
+  } else if (mode == 2) {
     scal[S_VOL] = s0;
+  } else if (mode == 3) {
+    scal[S_FE_IDEAL] = s0 * abs_total;  // abs_total carries kBT DX3 here
+  } else {
+    scal[S_FE_EL] = s0 * dx3;
   }
 }
 
@@ -208,7 +224,7 @@ __global__ void flux_x_kernel(T *__restrict__ f0, T *__restrict__ f1, T *__restr
                               const double *__restrict__ fac, const T *__restrict__ g0, const T *__restrict__ g1,
                               const T *__restrict__ g2, const T *__restrict__ n0, const T *__restrict__ n1,
                               const T *__restrict__ n2, const T *__restrict__ u0, const T *__restrict__ u1,
-                              const T *__restrict__ u2, double onsager, double ze, Grid g) {
+                              const T *__restrict__ u2, double onsager, double ze, int advect, Grid g) {
   size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= g.nr) return;
   const size_t im = ridx(g, r).im;
@@ -226,9 +242,28 @@ __global__ void flux_x_kernel(T *__restrict__ f0, T *__restrict__ f1, T *__restr
     a1 -= m1 * dmy;
     a2 -= m2 * dmy;
   }
-  f0[im] = a0 + c * -u0[im];
-  f1[im] = a1 + c * -u1[im];
-  f2[im] = a2 + c * -u2[im];
+  if (advect) {
+    a0 += c * -u0[im];
+    a1 += c * -u1[im];
+    a2 += c * -u2[im];
+  }
+  f0[im] = a0;
+  f1[im] = a1;
+  f2[im] = a2;
+}
+
+// psi_k = rho_k IK2 / Dielectric_cst (Charge_field_k2Coulomb_potential_k_PBC,
+// operate_electrolyte.cxx:76-88), in place
+template <typename T>
+__global__ void potential_k_kernel(T *__restrict__ a, double idc, Grid g) {
+  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  const KIdx k = kidx(g, c);
+  const double wx = k.kx * g.wave_x, wy = k.ky * g.wave_y, wz = k.kz * g.wave_z;
+  const double k2 = wx * wx + wy * wy + wz * wz;
+  const T s = (T)((k2 > 0.0 ? 1. / k2 : 0.0) * idc);
+  V2<T> v = ld2(a + 2 * c);
+  st2(a + 2 * c, V2<T>{v.re * s, v.im * s});
 }
 
 // U_k2divergence_k (fft_wrapper.cxx:896-920) of the flux spectrum and
@@ -394,9 +429,13 @@ void coulomb_grad_t(kapsel_ctx *ctx, const T *rhok, T *const *g3, int trunc, con
 // E_CX0/E_CX1 times the rescale factors are the real ion fields (the spectra
 // are 2/3-cut already: the Euler step only touches kept modes and the cut
 // modes were zeroed), and E_UX0..2 the real velocity whose spectrum zeta
-// holds (solenoidal and cut, so Zeta_k2u gives it back).
+// holds (solenoidal and cut, so Zeta_k2u gives it back).  ions_only: the
+// Nernst-Planck part with the fluid at rest and zeta untouched, as
+// Ion_diffusion_solver_Euler runs for Set_steadystate_ion_density
+// (fluid_solver.cxx:293-316, operate_electrolyte.cxx:436-444; the solvent
+// right-hand side it also computes is discarded there).
 template <typename T>
-void nssolute_t(kapsel_ctx *ctx, const kapsel_time &t, bool reuse) {
+void nssolute_t(kapsel_ctx *ctx, const kapsel_time &t, bool reuse, bool ions_only = false) {
   const Grid &g = ctx->g;
   T *rhok = ef<T>(ctx, E_RHOK);
   if (!reuse) {
@@ -412,8 +451,10 @@ void nssolute_t(kapsel_ctx *ctx, const kapsel_time &t, bool reuse) {
   coulomb_grad_t<T>(ctx, rhok, grad, 1, dc);
   fft_c2r_t<T>(ctx, (void *const *)grad, 3);
   // u of the truncated zeta (Rhs_NS_solute :203-207)
-  T *u[3];
-  if (reuse) {
+  T *u[3] = {nullptr, nullptr, nullptr};
+  if (ions_only) {
+    // u = 0: the advective flux vanishes
+  } else if (reuse) {
     for (int d = 0; d < 3; d++) u[d] = ef<T>(ctx, E_UX0 + d);
   } else {
     zeta_k2u_k(ctx, 0);
@@ -439,7 +480,8 @@ void nssolute_t(kapsel_ctx *ctx, const kapsel_time &t, bool reuse) {
       LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, 16.0 * g.nr * sizeof(T));
       hipLaunchKernelGGL(flux_x_kernel<T>, dim3(blocks_for(g.nr, TPB)), dim3(TPB), 0, ctx->stream, flux[0],
                          flux[1], flux[2], cx, fac, grad[0], grad[1], grad[2], ef<T>(ctx, E_N0), ef<T>(ctx, E_N1),
-                         ef<T>(ctx, E_N2), u[0], u[1], u[2], ctx->el.onsager_coeff[n], valency_e(ctx, n), g);
+                         ef<T>(ctx, E_N2), u[0], u[1], u[2], ctx->el.onsager_coeff[n], valency_e(ctx, n),
+                         ions_only ? 0 : 1, g);
       KAPSEL_LAUNCH_CHECK();
     }
     fft_r2c_t<T>(ctx, (void *const *)flux, 3);
@@ -450,6 +492,7 @@ void nssolute_t(kapsel_ctx *ctx, const kapsel_time &t, bool reuse) {
       KAPSEL_LAUNCH_CHECK();
     }
   }
+  if (ions_only) return;
   // Rhs_solvent (U2advection_k) + viscous term + Euler step of zeta
   if (reuse) {
     LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, 8.0 * g.nr * sizeof(T));
@@ -503,6 +546,70 @@ void coulomb_force_t(kapsel_ctx *ctx, const kapsel_time &t, int scaled) {
   hipLaunchKernelGGL(coulomb_force_kernel<T>, dim3(blocks_for(ctx->g.nr, TPB)), dim3(TPB), 0, ctx->stream, f[0],
                      f[1], f[2], rho, ctx->el.e_ext[0] * s, ctx->el.e_ext[1] * s, ctx->el.e_ext[2] * s, ctx->g);
   KAPSEL_LAUNCH_CHECK();
+}
+
+// Calc_free_energy_PB, non-Dielectric branch (operate_electrolyte.cxx:302-421):
+// scal[S_FE_IDEAL], scal[S_FE_EL].  phi is the unclamped profile sum, which
+// the phi_qq scatter of the charge computes as well.
+template <typename T>
+void free_energy_t(kapsel_ctx *ctx, const kapsel_time &t) {
+  const Grid &g = ctx->g;
+  ions_real_t<T>(ctx);
+  T *rhok = ef<T>(ctx, E_RHOK), *rho = ef<T>(ctx, E_RHO);
+  charge_t<T>(ctx, rhok, rho, 0);  // E_QQ: profile sum
+  {
+    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, 3.0 * g.nr * sizeof(T));
+    hipLaunchKernelGGL((reduce_kernel<T, 3>), dim3(NBR), dim3(TPB), 0, ctx->stream, ef<T>(ctx, E_CX0),
+                       ef<T>(ctx, E_CX1), ef<T>(ctx, E_QQ), ctx->el_partial, g, 0.0, 0.0, 0.0, 0);
+    KAPSEL_LAUNCH_CHECK();
+  }
+  const double dx3 = g.dx * g.dx * g.dx;
+  finish(ctx, 3, ctx->el.kbt * dx3);
+  void *one[1] = {rhok};
+  fft_r2c_t<T>(ctx, one, 1);
+  {
+    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, 2.0 * g.f * sizeof(T));
+    hipLaunchKernelGGL(potential_k_kernel<T>, dim3(blocks_for(g.nc, TPB)), dim3(TPB), 0, ctx->stream, rhok,
+                       1. / ctx->el.dielectric_cst, g);
+    KAPSEL_LAUNCH_CHECK();
+  }
+  fft_c2r_t<T>(ctx, one, 1);
+  const double s = ac_factor(ctx, t);
+  const int external = ctx->el.e_ext[0] != 0. || ctx->el.e_ext[1] != 0. || ctx->el.e_ext[2] != 0.;
+  {
+    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, 2.0 * g.nr * sizeof(T));
+    hipLaunchKernelGGL((reduce_kernel<T, 4>), dim3(NBR), dim3(TPB), 0, ctx->stream, rho, rhok, (const T *)nullptr,
+                       ctx->el_partial, g, ctx->el.e_ext[0] * s, ctx->el.e_ext[1] * s, ctx->el.e_ext[2] * s,
+                       external);
+    KAPSEL_LAUNCH_CHECK();
+  }
+  finish(ctx, 4);
+}
+
+template <typename T>
+void free_energy_host_t(kapsel_ctx *ctx, const kapsel_time &t, double *fe) {
+  free_energy_t<T>(ctx, t);
+  fe[1] = read_scal(ctx, S_FE_IDEAL);
+  fe[2] = read_scal(ctx, S_FE_EL);
+  fe[0] = fe[1] + fe[2];
+}
+
+// Set_steadystate_ion_density (operate_electrolyte.cxx:424-457): relax the
+// ions with the fluid at rest until the free energy changes by less than
+// TOL = 1e-4 (operate_electrolyte.h:22), at most max_iter - 1 steps
+template <typename T>
+int steadystate_t(kapsel_ctx *ctx, const kapsel_time &t, int max_iter, double *fe) {
+  free_energy_host_t<T>(ctx, t, fe);
+  double previous = fe[0];
+  int m;
+  for (m = 1; m < max_iter; m++) {
+    nssolute_t<T>(ctx, t, false, true);
+    rescale_t<T>(ctx, false);
+    free_energy_host_t<T>(ctx, t, fe);
+    if (fabs(fe[0] - previous) < 1.e-4) break;
+    previous = fe[0];
+  }
+  return m;
 }
 
 // Init_rho_ion, N_spec = 2, uniform salt (operate_electrolyte.cxx:559-621, 860)
@@ -572,6 +679,18 @@ void electrolyte_set(kapsel_ctx *ctx, const kapsel_electrolyte &el) {
   ctx->el_set = true;
 }
 
+
+void electrolyte_free(kapsel_ctx *ctx) {
+  for (void *&p : ctx->el_f)
+    if (p) {
+      (void)hipFree(p);
+      p = nullptr;
+    }
+  if (ctx->el_partial) (void)hipFree(ctx->el_partial);
+  if (ctx->el_scal) (void)hipFree(ctx->el_scal);
+  ctx->el_partial = ctx->el_scal = nullptr;
+}
+
 template <typename T>
 static void recount_t(kapsel_ctx *ctx) {
   make_phi_particle(ctx, ctx->fields[KAPSEL_PHI]);
@@ -586,20 +705,26 @@ void recount_total_solute(kapsel_ctx *ctx) {
   EL_DISPATCH(ctx, recount_t, ctx);
 }
 
-void electrolyte_free(kapsel_ctx *ctx) {
-  for (void *&p : ctx->el_f)
-    if (p) {
-      (void)hipFree(p);
-      p = nullptr;
-    }
-  if (ctx->el_partial) (void)hipFree(ctx->el_partial);
-  if (ctx->el_scal) (void)hipFree(ctx->el_scal);
-  ctx->el_partial = ctx->el_scal = nullptr;
-}
-
 void init_rho_ion(kapsel_ctx *ctx) {
   need_el(ctx);
   EL_DISPATCH(ctx, init_rho_ion_t, ctx);
+}
+
+template <typename T>
+static int init_rho_ion_pb_t(kapsel_ctx *ctx, const kapsel_time &t, int max_iter, double *fe) {
+  init_rho_ion_t<T>(ctx);
+  const int it = steadystate_t<T>(ctx, t, max_iter, fe);
+  recount_t<T>(ctx);  // Count_solute_each (operate_electrolyte.cxx:860)
+  return it;
+}
+int init_rho_ion_pb(kapsel_ctx *ctx, const kapsel_time &t, int max_iter, double *fe) {
+  need_el(ctx);
+  return EL_DISPATCH(ctx, init_rho_ion_pb_t, ctx, t, max_iter, fe);
+}
+
+void free_energy_pb(kapsel_ctx *ctx, const kapsel_time &t, double *fe) {
+  need_el(ctx);
+  EL_DISPATCH(ctx, free_energy_host_t, ctx, t, fe);
 }
 
 template <typename T>

@@ -268,6 +268,17 @@ class KapselGPU:
     def init_rho_ion(self):
         self._call("init_rho_ion")
 
+    def init_rho_ion_pb(self, t, max_iter=10000):
+        it = ctypes.c_int(0)
+        fe = np.zeros(3)
+        self._call("init_rho_ion_pb", ctypes.byref(t), int(max_iter), ctypes.byref(it), _dp(fe))
+        return it.value, fe
+
+    def free_energy_pb(self, t):
+        fe = np.zeros(3)
+        self._call("free_energy_pb", ctypes.byref(t), _dp(fe))
+        return fe
+
     def get_total_solute(self):
         a = np.zeros(2)
         self._call("get_total_solute", _dp(a))

@@ -1691,6 +1691,126 @@ struct Oracle {
     /* uk_dc += dt * dc_rhs with dc_rhs = 0 (Rhs_solvent) */
   }
 
+  /* Calc_free_energy_PB, non-Dielectric branch (operate_electrolyte.cxx:
+   * 302-421): fe = {total, ideal, electrostatic} */
+  void free_energy_pb(const kapsel_time &jikan, double *fe) {
+    double *phi = EB(EB_PHIQQ), *work = EB(EB_WORK), *charge = EB(EB_CHARGE), *pot = EB(EB_POT);
+    double ideal = 0., electrostatic = 0.;
+    reset(phi);
+    make_phi_particle(phi);
+    for (int n = 0; n < 2; n++) {
+      memcpy(work, C(n), F * sizeof(double));
+      fft_c2r(work);
+#pragma omp parallel for reduction(+ : ideal)
+      for (int i = 0; i < NX; i++)
+        for (int j = 0; j < NY; j++)
+          for (int k = 0; k < NZ; k++) {
+            size_t im = ((size_t)i * NY + j) * NZ_ + k;
+            double c = work[im];
+            ideal += (1. - phi[im]) * c * (log(c) - 1.);
+          }
+    }
+    ideal *= (EL.kbt * DX3);
+    conc_k2charge_field(charge, phi, work);
+    memcpy(pot, charge, F * sizeof(double));
+    fft_r2c(pot);
+    coulomb_potential_k(pot);
+    fft_c2r(pot);
+    const double s = ac_factor(jikan);
+    const double ext[3] = {EL.e_ext[0] * s, EL.e_ext[1] * s, EL.e_ext[2] * s};
+    const bool external = EL.e_ext[0] != 0. || EL.e_ext[1] != 0. || EL.e_ext[2] != 0.;
+#pragma omp parallel for reduction(+ : electrostatic)
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++)
+        for (int k = 0; k < NZ; k++) {
+          size_t im = ((size_t)i * NY + j) * NZ_ + k;
+          double v = pot[im] * 0.5;
+          if (external) v -= ((ext[0] * (double)i + ext[1] * (double)j + ext[2] * (double)k) * DX);
+          electrostatic += charge[im] * v;
+        }
+    electrostatic *= DX3;
+    fe[1] = ideal;
+    fe[2] = electrostatic;
+    fe[0] = fe[1] + fe[2];
+  }
+
+  /* Ion_diffusion_solver_Euler (fluid_solver.cxx:293-316) as Set_steadystate_ion_density
+   * calls it (operate_electrolyte.cxx:436-444): zero velocity and vorticity, the
+   * Nernst-Planck right-hand side, the Euler step of the ions, Rescale_solute.  The
+   * solvent right-hand side the reference computes there is discarded, so it is not. */
+  void ion_diffusion_euler(const kapsel_time &jikan) {
+    for (int d = 0; d < 3; d++) reset(U(d));
+    double *pot = EB(EB_CHARGE);
+    conc_k2charge_field(pot, EB(EB_PHIQQ), EB(EB_WORK));
+    fft_r2c(pot);
+    coulomb_potential_k(pot);
+    truncate(pot);
+    double *grad[3] = {EB(EB_GRAD), EB(EB_GRAD + 1), EB(EB_GRAD + 2)};
+    grad_k(pot, grad);
+    double amp = (double)(NX * NY * NZ) * ac_factor(jikan);
+    for (int d = 0; d < 3; d++) grad[d][0] -= EL.e_ext[d] * amp;
+    for (int d = 0; d < 3; d++) fft_c2r(grad[d]);
+    double *nrm[3] = {EB(EB_NORMAL), EB(EB_NORMAL + 1), EB(EB_NORMAL + 2)};
+    make_surface_normal(nrm);
+    double *flux[3] = {EB(EB_FLUX), EB(EB_FLUX + 1), EB(EB_FLUX + 2)};
+    double *rhs_solute[2] = {EB(EB_RHS), EB(EB_RHS + 1)};
+    for (int n = 0; n < 2; n++) {
+      double *ck = C(n);
+      truncate(ck);
+      double *w = fld(KAPSEL_WORK0);
+      const double kl = EL.kbt * EL.onsager_coeff[n];
+#pragma omp parallel for
+      for (size_t im = 0; im < F; im++) w[im] = kl * ck[im];
+      grad_k(w, flux);
+      for (int d = 0; d < 3; d++) fft_c2r(flux[d]);
+      double *c = rhs_solute[n];
+      memcpy(c, ck, F * sizeof(double));
+      fft_c2r(c);
+      const double ze = valency_e(n), L = EL.onsager_coeff[n];
+#pragma omp parallel for
+      for (int i = 0; i < NX; i++)
+        for (int j = 0; j < NY; j++)
+          for (int k = 0; k < NZ; k++) {
+            size_t im = ((size_t)i * NY + j) * NZ_ + k;
+            for (int d = 0; d < 3; d++) flux[d][im] += c[im] * (L * (grad[d][im] * ze));
+          }
+      solute_impermeability(flux, nrm);
+      for (int d = 0; d < 3; d++) fft_r2c(flux[d]);
+      divergence_k(flux, rhs_solute[n]);
+    }
+    int rng[4][6];
+    kept_boxes(rng);
+    for (int b = 0; b < 4; b++)
+      for (int n = 0; n < 2; n++)
+#pragma omp parallel for
+        for (int i = rng[b][0]; i <= rng[b][1]; i++)
+          for (int j = rng[b][2]; j <= rng[b][3]; j++)
+            for (int k = rng[b][4]; k <= rng[b][5]; k++) {
+              size_t im = ((size_t)i * NY + j) * NZ_ + k;
+              C(n)[im] += (jikan.dt_fluid * rhs_solute[n][im]);
+            }
+    double f[2];
+    rescale_solute(f);
+  }
+
+  /* Set_steadystate_ion_density (operate_electrolyte.cxx:424-457), TOL = 1e-4
+   * (operate_electrolyte.h:22), at most max_iter - 1 relaxation steps (STEP = 10000) */
+  int steadystate_ion_density(const kapsel_time &jikan, int max_iter, double *fe_out) {
+    double fe[3];
+    free_energy_pb(jikan, fe);
+    double previous = fe[0];
+    int m;
+    for (m = 1; m < max_iter; m++) {
+      ion_diffusion_euler(jikan);
+      free_energy_pb(jikan, fe);
+      double dum = fabs(fe[0] - previous);
+      if (dum < 1.e-4) break;
+      previous = fe[0];
+    }
+    if (fe_out) for (int c = 0; c < 3; c++) fe_out[c] = fe[c];
+    return m;
+  }
+
   /* Make_Coulomb_force_x_on_fluid (operate_electrolyte.cxx:90-177), free particles */
   void coulomb_force(const kapsel_time &jikan) {
     double *charge = EB(EB_CHARGE), *pot = EB(EB_POT);
@@ -1987,6 +2107,17 @@ int oracle_rescale_solute(void *h, double *factor) {
   return 0;
 }
 int oracle_coulomb_force(void *h, const kapsel_time *t) { NEED_EL O->coulomb_force(*t); return 0; }
+int oracle_free_energy_pb(void *h, const kapsel_time *t, double *fe) { NEED_EL O->free_energy_pb(*t, fe); return 0; }
+int oracle_init_rho_ion_pb(void *h, const kapsel_time *t, int max_iter, int *iters, double *fe) {
+  NEED_EL
+  if (max_iter <= 0) max_iter = 10000;
+  /* Init_rho_ion with Poisson_Boltzmann, salt, several particles
+   * (operate_electrolyte.cxx:820-833): uniform start, steady state, Count_solute_each */
+  O->init_rho_ion();
+  *iters = O->steadystate_ion_density(*t, max_iter, fe);
+  O->count_solute_each(O->total_solute, O->fld(KAPSEL_PHI), O->EB(Oracle::EB_WORK));
+  return 0;
+}
 int oracle_electrolyte_step(void *h, kapsel_time *t) { NEED_EL O->electrolyte_step(*t); return 0; }
 #undef NEED_EL
 #undef O

@@ -60,7 +60,8 @@ def _load():
             "oracle_get_total_solute": [dp], "oracle_set_total_solute": [dp],
             "oracle_make_phi_qq": [i, i], "oracle_make_surface_normal": [i], "oracle_charge_field": [i],
             "oracle_nssolute_euler": [tp], "oracle_rescale_solute": [dp], "oracle_coulomb_force": [tp],
-            "oracle_electrolyte_step": [tp],
+            "oracle_electrolyte_step": [tp], "oracle_free_energy_pb": [tp, dp],
+            "oracle_init_rho_ion_pb": [tp, i, ip, dp],
         }
         for name, args in protos.items():
             fn = getattr(lib, name)
@@ -305,6 +306,17 @@ class Oracle:
 
     def electrolyte_step(self, t):
         self._call("electrolyte_step", ctypes.byref(t))
+
+    def free_energy_pb(self, t):
+        fe = np.zeros(3)
+        self._call("free_energy_pb", ctypes.byref(t), _dp(fe))
+        return fe
+
+    def init_rho_ion_pb(self, t, max_iter=10000):
+        it = ctypes.c_int(0)
+        fe = np.zeros(3)
+        self._call("init_rho_ion_pb", ctypes.byref(t), int(max_iter), ctypes.byref(it), _dp(fe))
+        return it.value, fe
 
 
 class MTRef:

@@ -260,3 +260,36 @@ def test_electrolyte_full_size_invariants_fp32():
     assert abs(rho.sum()) < 1e-4 * q
     v = k.get_particles()[abi.PFIELD["V"]]
     assert np.all(np.isfinite(v)) and v[0].mean() > 0  # positive spheres drift along E
+
+
+def test_free_energy_and_steady_state_ions_match_oracle():
+    """Calc_free_energy_PB of random ion fields at 1e-12; the Poisson-Boltzmann
+    start (uniform salt, then ion relaxation with the fluid at rest) for 7
+    relaxation steps at 1e-12, and run to its free-energy tolerance with the
+    same number of steps and fields within 1e-10."""
+    n, npart = 32, 3
+    print(f"case electrolyte free energy and PB start {n}^3", flush=True)
+    p, st, el = _pair(n, npart, q=30.0, e_ext=(0.0, 0.0, 0.0))
+    conc_k, _ = _state(n, npart, 31)
+    k, o = gpu(p), oracle(p)
+    _load((k, o), st, el, conc_k)
+    t = abi.make_time(workloads.electrolyte_auto_dt(p, el))
+    fk, fo = k.free_energy_pb(t), o.free_energy_pb(t)
+    print("free energy", fk, fo, flush=True)
+    assert np.abs(fk - fo).max() < 1e-12 * np.abs(fo).max()
+    res = {}
+    for x in (k, o):
+        res[id(x)] = x.init_rho_ion_pb(t, 8)
+    (ik, fek), (io, feo) = res[id(k)], res[id(o)]
+    assert ik == io == 7
+    e = {f: rel(k.get_field(f), o.get_field(f)) for f in ("CONC0", "CONC1")}
+    e["free energy"] = float(np.abs(fek - feo).max() / np.abs(feo).max())
+    e["totals"] = float(np.abs(k.get_total_solute() - o.get_total_solute()).max() / o.get_total_solute().max())
+    print("7 relaxation steps", {f: f"{v:.1e}" for f, v in e.items()}, flush=True)
+    assert max(e.values()) < 1e-12, e
+    for x in (k, o):
+        res[id(x)] = x.init_rho_ion_pb(t)
+    (ik, fek), (io, feo) = res[id(k)], res[id(o)]
+    e = {f: rel(k.get_field(f), o.get_field(f)) for f in ("CONC0", "CONC1")}
+    print(f"converged after {ik} (gpu) / {io} (oracle) steps", {f: f"{v:.1e}" for f, v in e.items()}, flush=True)
+    assert ik == io and max(e.values()) < 1e-10

@@ -331,3 +331,72 @@ def test_electrolyte_step_neutral_and_conserving():
     # the field drags the positively charged spheres along +x
     v = o.get_particles()[abi.PFIELD["V"]]
     assert np.all(v[0] > 0)
+
+
+def test_free_energy_pb_numpy():
+    """Calc_free_energy_PB: ideal part kBT dx^3 sum (1-phi) c (ln c - 1) over
+    both species, electrostatic part dx^3 sum rho (psi/2 - E.x) with the
+    numpy Poisson solve."""
+    n = 32
+    e_ext = (0.02, -0.01, 0.03)
+    p, pos, el = _setup(n=n, e_ext=e_ext)
+    el.kbt = 0.7
+    el.dielectric_cst = 1.3
+    o = _oracle(p, el)
+    o.set_particles(workloads.particle_state(pos))
+    rng = np.random.default_rng(8)
+    conc = [_random_conc(n, rng, 0.05, 0.02), _random_conc(n, rng, 0.04, 0.02)]
+    for s in (0, 1):
+        o.set_field(f"CONC{s}", _kspace_conc(conc[s]))
+    fe = o.free_energy_pb(abi.make_time(0.01))
+    phi, _, _ = _brute(pos, n, p.radius, p.xi)
+    ideal = el.kbt * sum(((1 - phi) * c * (np.log(c) - 1)).sum() for c in conc)
+    rho = _numpy_charge(o, n, el, conc)
+    rk = np.fft.rfftn(rho)
+    _, (kx, ky, kz) = sp.wavevectors((n, n, n))
+    k2 = kx * kx + ky * ky + kz * kz
+    psi = np.fft.irfftn(np.where(k2 > 0, rk / (el.dielectric_cst * np.where(k2 > 0, k2, 1.0)), 0.0),
+                        s=(n, n, n), axes=(0, 1, 2))
+    g = np.arange(n, dtype=float)
+    X = np.stack(np.meshgrid(g, g, g, indexing="ij"))
+    ext = sum(e_ext[d] * X[d] for d in range(3))
+    electro = (rho * (0.5 * psi - ext)).sum()
+    assert abs(fe[1] - ideal) < 1e-12 * abs(ideal)
+    assert abs(fe[2] - electro) < 1e-11 * np.abs(rho * (0.5 * psi - ext)).sum()
+    assert fe[0] == fe[1] + fe[2]
+
+
+def test_steady_state_ions_relax_toward_boltzmann():
+    """Set_steadystate_ion_density from the uniform salt start (32^3, three
+    spheres, no field): the free energy falls, the amounts stay, and away
+    from the spheres the electrochemical potential ln c_n + z_n e psi / kBT
+    is flat to a tenth of the variation of z_n e psi / kBT (the relaxation
+    stops at a free-energy change of 1e-4, and the 2/3 cut limits psi)."""
+    n = 32
+    p, pos, el = _setup(n=n, q=30.0)
+    o = _oracle(p, el)
+    o.set_particles(workloads.particle_state(pos))
+    t = abi.make_time(workloads.electrolyte_auto_dt(p, el))
+    o.init_rho_ion()
+    fe0, tot0 = o.free_energy_pb(t), o.get_total_solute()
+    it, fe = o.init_rho_ion_pb(t)
+    assert 1 < it < 10000 and fe[0] < fe0[0]
+    assert np.abs(o.get_total_solute() - tot0).max() < 1e-12 * tot0.max()
+    c = [np.fft.irfftn(sp.to_complex(o.get_field(f"CONC{s}")), s=(n, n, n), axes=(0, 1, 2)) for s in (0, 1)]
+    o.charge_field("WORK0")
+    rho = _real(o, "WORK0", n)
+    rk = np.fft.rfftn(rho)
+    _, (kx, ky, kz) = sp.wavevectors((n, n, n))
+    k2 = kx * kx + ky * ky + kz * kz
+    psi = np.fft.irfftn(np.where(k2 > 0, rk / np.where(k2 > 0, k2, 1.0), 0.0), s=(n, n, n), axes=(0, 1, 2))
+    g = np.arange(n, dtype=float)
+    G = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1)
+    dmin = np.full((n, n, n), 1e9)
+    for xp in pos:
+        d = G - xp
+        d -= n * np.rint(d / n)
+        dmin = np.minimum(dmin, np.sqrt((d * d).sum(-1)))
+    far = dmin > p.radius + p.xi
+    for s, z in ((0, 1.0), (1, -1.0)):
+        mu = np.log(c[s]) + z * psi
+        assert mu[far].std() < 0.15 * (z * psi)[far].std(), s
