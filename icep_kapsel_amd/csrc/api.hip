@@ -953,7 +953,7 @@ int kapsel_init_rho_ion_pb(kapsel_ctx *ctx, const kapsel_time *jikan, int max_it
     throw Error{KAPSEL_ERR_ARG};
   }
   double fe[3];
-  *iters = init_rho_ion_pb(ctx, *jikan, max_iter > 0 ? max_iter : 10000, fe);
+  *iters = init_rho_ion_pb(ctx, *jikan, max_iter, fe);
   if (free_energy)
     for (int c = 0; c < 3; c++) free_energy[c] = fe[c];
   written(ctx, KAPSEL_PHI);

@@ -29,7 +29,7 @@ constexpr unsigned TPB = 256;
 constexpr int NBR = 1024;  // blocks of a reduction (fixed: deterministic sums)
 
 // device scalars
-enum { S_SURF = 0, S_F0, S_F1, S_C0, S_C1, S_VOL, S_FE_IDEAL, S_FE_EL, S_N };
+enum { S_SURF = 0, S_F0, S_F1, S_C0, S_C1, S_VOL, S_FE_IDEAL, S_FE_EL, S_PB0, S_PB1, S_N };
 // scratch fields
 enum { E_N0 = 0, E_N1, E_N2, E_QQ, E_SURF, E_CX0, E_CX1, E_RHOK, E_RHO, E_UX0, E_UX1, E_UX2, E_NF };
 
@@ -51,6 +51,8 @@ __device__ __forceinline__ double block_sum(double v) {
 // OP 2: sum (1 - phi)                    (Set_uniform_ion_charge_density_salt, :581-591)
 // OP 3: sum_n (1 - phi) c_n (log c_n - 1), a = c_0, b = c_1 (Calc_free_energy_PB, :370-381)
 // OP 4: sum rho (psi / 2 - E.(i,j,k) dx), a = rho, b = psi   (Calc_free_energy_PB, :388-411)
+// OP 5: sums (1 - phi) exp(-e0 psi), (1 - phi) exp(-e1 psi), b = psi
+//       (Set_Poisson_Boltzmann_ion_charge_density_salt, :652-662)
 template <typename T, int OP>
 __global__ __launch_bounds__(TPB) void reduce_kernel(const T *__restrict__ a, const T *__restrict__ b,
                                                      const T *__restrict__ phi, double *__restrict__ partial,
@@ -72,6 +74,10 @@ __global__ __launch_bounds__(TPB) void reduce_kernel(const T *__restrict__ a, co
       const double c0 = (double)a[p.im], c1 = (double)b[p.im];
       s0 += w * c0 * (log(c0) - 1.);
       s0 += w * c1 * (log(c1) - 1.);
+    } else if constexpr (OP == 5) {
+      const double w = 1. - (double)phi[p.im], v = (double)b[p.im];
+      s0 += w * exp(-e0 * v);
+      s1 += w * exp(-e1 * v);
     } else {
       double v = (double)b[p.im] * 0.5;
       if (external) v -= ((e0 * (double)p.i + e1 * (double)p.j + e2 * (double)p.k) * g.dx);
@@ -79,7 +85,7 @@ __global__ __launch_bounds__(TPB) void reduce_kernel(const T *__restrict__ a, co
     }
   }
   s0 = block_sum(s0);
-  if constexpr (OP == 1) s1 = block_sum(s1);
+  if constexpr (OP == 1 || OP == 5) s1 = block_sum(s1);
   if (threadIdx.x == 0) {
     partial[blockIdx.x] = s0;
     partial[NBR + blockIdx.x] = s1;
@@ -112,6 +118,9 @@ __global__ __launch_bounds__(TPB) void finish_kernel(const double *__restrict__ 
     scal[S_VOL] = s0;
   } else if (mode == 3) {
     scal[S_FE_IDEAL] = s0 * abs_total;  // abs_total carries kBT DX3 here
+  } else if (mode == 5) {
+    scal[S_PB0] = s0;
+    scal[S_PB1] = s1;
   } else {
     scal[S_FE_EL] = s0 * dx3;
   }
@@ -320,6 +329,51 @@ __global__ void products_kernel(const T *__restrict__ x0, const T *__restrict__ 
   u2[im] = v1 * v2;
   a0[im] = (v2 - v3) * (v2 + v3);
   a1[im] = (v3 - v1) * (v3 + v1);
+}
+
+// Set_Poisson_Boltzmann_ion_charge_density_salt, one fixed-point iteration's
+// pointwise parts (operate_electrolyte.cxx:670-695): the ion densities from
+// the potential, and the charge density they make with the particle's
+// surface charge
+template <typename T>
+__global__ void pb_conc_charge_kernel(const T *__restrict__ ep, T *__restrict__ c0, T *__restrict__ c1,
+                                      const T *__restrict__ qq, const T *__restrict__ surf,
+                                      const double *__restrict__ scal, double rp, double rn, double d0, double d1,
+                                      double nu, double ze0, double ze1, T *__restrict__ rho, Grid g) {
+  size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= g.nr) return;
+  const size_t im = ridx(g, r).im;
+  const double pot = (double)ep[im];
+  const T v0 = (T)(rp * exp(-d0 * pot) * nu), v1 = (T)(rn * exp(-d1 * pot) / nu);
+  c0[im] = v0;
+  c1[im] = v1;
+  const T one_phi = (T)1 - qq[im];
+  T q = surf[im] * (T)scal[S_SURF];
+  q += (T)ze0 * v0 * one_phi;
+  q += (T)ze1 * v1 * one_phi;
+  rho[im] = q;
+}
+
+// the damped update of the potential, with sum |psi| and sum |d psi| per block
+// (operate_electrolyte.cxx:703-716)
+template <typename T>
+__global__ __launch_bounds__(TPB) void pb_update_kernel(T *__restrict__ ep, const T *__restrict__ pot, double alp,
+                                                        double *__restrict__ partial, Grid g) {
+  double s0 = 0.0, s1 = 0.0;
+  for (size_t r = (size_t)blockIdx.x * TPB + threadIdx.x; r < g.nr; r += (size_t)gridDim.x * TPB) {
+    const size_t im = ridx(g, r).im;
+    const T dmy = (T)alp * (pot[im] - ep[im]);
+    const T v = ep[im] + dmy;
+    ep[im] = v;
+    s0 += fabs((double)v);
+    s1 += fabs((double)dmy);
+  }
+  s0 = block_sum(s0);
+  s1 = block_sum(s1);
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = s0;
+    partial[NBR + blockIdx.x] = s1;
+  }
 }
 
 // uniform ion densities on the real points, 0 in the row padding
@@ -601,15 +655,82 @@ template <typename T>
 int steadystate_t(kapsel_ctx *ctx, const kapsel_time &t, int max_iter, double *fe) {
   free_energy_host_t<T>(ctx, t, fe);
   double previous = fe[0];
-  int m;
-  for (m = 1; m < max_iter; m++) {
+  int steps = 0;
+  for (int m = 1; m < max_iter; m++) {
     nssolute_t<T>(ctx, t, false, true);
     rescale_t<T>(ctx, false);
+    steps++;
     free_energy_host_t<T>(ctx, t, fe);
     if (fabs(fe[0] - previous) < 1.e-4) break;
     previous = fe[0];
   }
-  return m;
+  return steps;
+}
+
+// Set_Poisson_Boltzmann_ion_charge_density_salt (operate_electrolyte.cxx:
+// 623-736): one particle, z:z salt; damped fixed point of the potential
+// (alp = 0.01) until sum|d psi| / sum|psi| < TOL = 1e-4, at most max_iter - 1
+// iterations; the ion spectra at the end.  The particle's profile sum and
+// normalised surface charge are the same every iteration (the particle does
+// not move), so they are made once.
+template <typename T>
+int pb_salt_single_t(kapsel_ctx *ctx, int max_iter) {
+  const kapsel_electrolyte &e = ctx->el;
+  const Grid &g = ctx->g;
+  const double rho_inf = e.kbt * e.dielectric_cst / ((e.elementary_charge * e.debye_length) *
+                                                      (e.elementary_charge * e.debye_length));
+  const double rp = rho_inf / (e.valency[0] * (e.valency[0] - e.valency[1]));
+  const double rn = -rho_inf / (e.valency[1] * (e.valency[0] - e.valency[1]));
+  const double alp = 0.01;
+  const double d0 = e.elementary_charge * e.valency[0] / e.kbt, d1 = e.elementary_charge * e.valency[1] / e.kbt;
+  const double surface_ion_number = -e.surface_charge * ctx->P.n_particles;
+  const double dx3 = g.dx * g.dx * g.dx;
+  make_phi_particle(ctx, ctx->fields[KAPSEL_PHI]);
+  phi_qq_t<T>(ctx, ef<T>(ctx, E_QQ), ef<T>(ctx, E_SURF));
+  T *ep = ef<T>(ctx, E_UX0), *rhok = ef<T>(ctx, E_RHOK);
+  T *c0 = fl<T>(ctx, KAPSEL_CONC0), *c1 = fl<T>(ctx, KAPSEL_CONC1);
+  KAPSEL_HIP(hipMemsetAsync(ep, 0, g.f * sizeof(T), ctx->stream));
+  int iters = 0;
+  for (int m = 1; m < max_iter; m++) {
+    iters++;
+    {
+      LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, 2.0 * g.nr * sizeof(T));
+      hipLaunchKernelGGL((reduce_kernel<T, 5>), dim3(NBR), dim3(TPB), 0, ctx->stream, (const T *)nullptr, ep,
+                         fl<T>(ctx, KAPSEL_PHI), ctx->el_partial, g, d0, d1, 0.0, 0);
+      KAPSEL_LAUNCH_CHECK();
+    }
+    finish(ctx, 5);
+    const double sp = read_scal(ctx, S_PB0) * dx3, sn = read_scal(ctx, S_PB1) * dx3;
+    const double a = e.valency[0] * rp * sp, b = -surface_ion_number, c = e.valency[1] * rn * sn;
+    const double nu = (-b + sqrt(b * b - 4. * a * c)) / (2. * a);
+    {
+      LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, 6.0 * g.nr * sizeof(T));
+      hipLaunchKernelGGL(pb_conc_charge_kernel<T>, dim3(blocks_for(g.nr, TPB)), dim3(TPB), 0, ctx->stream, ep, c0,
+                         c1, ef<T>(ctx, E_QQ), ef<T>(ctx, E_SURF), ctx->el_scal, rp, rn, d0, d1, nu,
+                         valency_e(ctx, 0), valency_e(ctx, 1), rhok, g);
+      KAPSEL_LAUNCH_CHECK();
+    }
+    void *one[1] = {rhok};
+    fft_r2c_t<T>(ctx, one, 1);
+    {
+      LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, 2.0 * g.f * sizeof(T));
+      hipLaunchKernelGGL(potential_k_kernel<T>, dim3(blocks_for(g.nc, TPB)), dim3(TPB), 0, ctx->stream, rhok,
+                         1. / e.dielectric_cst, g);
+      KAPSEL_LAUNCH_CHECK();
+    }
+    fft_c2r_t<T>(ctx, one, 1);
+    {
+      LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, 3.0 * g.nr * sizeof(T));
+      hipLaunchKernelGGL(pb_update_kernel<T>, dim3(NBR), dim3(TPB), 0, ctx->stream, ep, rhok, alp, ctx->el_partial,
+                         g);
+      KAPSEL_LAUNCH_CHECK();
+    }
+    finish(ctx, 5);
+    if (read_scal(ctx, S_PB1) / read_scal(ctx, S_PB0) < 1.e-4) break;
+  }
+  void *two[2] = {c0, c1};
+  fft_r2c_t<T>(ctx, two, 2);
+  return iters;
 }
 
 // Init_rho_ion, N_spec = 2, uniform salt (operate_electrolyte.cxx:559-621, 860)
@@ -712,13 +833,25 @@ void init_rho_ion(kapsel_ctx *ctx) {
 
 template <typename T>
 static int init_rho_ion_pb_t(kapsel_ctx *ctx, const kapsel_time &t, int max_iter, double *fe) {
-  init_rho_ion_t<T>(ctx);
-  const int it = steadystate_t<T>(ctx, t, max_iter, fe);
+  int it;
+  if (ctx->P.n_particles == 1) {
+    it = pb_salt_single_t<T>(ctx, max_iter > 0 ? max_iter : 100000);
+    free_energy_host_t<T>(ctx, t, fe);
+  } else {
+    init_rho_ion_t<T>(ctx);
+    it = steadystate_t<T>(ctx, t, max_iter > 0 ? max_iter : 10000, fe);
+  }
   recount_t<T>(ctx);  // Count_solute_each (operate_electrolyte.cxx:860)
   return it;
 }
 int init_rho_ion_pb(kapsel_ctx *ctx, const kapsel_time &t, int max_iter, double *fe) {
   need_el(ctx);
+  if (ctx->P.n_particles == 1 && ctx->el.valency[0] != -ctx->el.valency[1]) {
+    // Set_Poisson_Boltzmann_ion_charge_density_salt (:629-633)
+    set_error("the one-particle Poisson-Boltzmann start needs a z:z salt (valencies %g, %g)", ctx->el.valency[0],
+              ctx->el.valency[1]);
+    throw Error{KAPSEL_ERR_ARG};
+  }
   return EL_DISPATCH(ctx, init_rho_ion_pb_t, ctx, t, max_iter, fe);
 }
 

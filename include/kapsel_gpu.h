@@ -278,14 +278,17 @@ int kapsel_set_electrolyte(kapsel_ctx *ctx, const kapsel_electrolyte *el);
  * (:559-621) fills CONC0/CONC1 (k-space) from the particle positions, then
  * Count_solute_each (solute_rhs.h:143-156) sets the conserved totals. */
 int kapsel_init_rho_ion(kapsel_ctx *ctx);
-/* Init_rho_ion with Poisson_Boltzmann, N_spec = 2, more than one particle
- * (operate_electrolyte.cxx:820-833, 860): the uniform start above, then
+/* Init_rho_ion with Poisson_Boltzmann, N_spec = 2 (operate_electrolyte.cxx:
+ * 820-833, 860).  One particle: Set_Poisson_Boltzmann_ion_charge_density_salt
+ * (:623-736), the damped fixed point of the potential (z:z salt only) until
+ * its relative change falls below 1e-4 (max_iter <= 0: the reference's
+ * 100000).  Several particles: the uniform start above, then
  * Set_steadystate_ion_density (:424-457) -- the ions relax with the fluid at
  * rest (Ion_diffusion_solver_Euler, fluid_solver.cxx:293-316, time step
  * jikan->dt_fluid) until the free energy changes by less than 1e-4
  * (operate_electrolyte.h:22) or max_iter - 1 steps were taken (max_iter <= 0:
- * the reference's 10000) -- then Count_solute_each.  *iters: the steps taken;
- * free_energy (may be NULL): {total, ideal, electrostatic} at the end. */
+ * the reference's 10000).  Then Count_solute_each.  *iters: the iterations
+ * taken; free_energy (may be NULL): {total, ideal, electrostatic} at the end. */
 int kapsel_init_rho_ion_pb(kapsel_ctx *ctx, const kapsel_time *jikan, int max_iter, int *iters,
                            double free_energy[3]);
 /* Calc_free_energy_PB, non-Dielectric branch (operate_electrolyte.cxx:302-421):

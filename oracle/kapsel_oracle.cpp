@@ -1799,16 +1799,92 @@ struct Oracle {
     double fe[3];
     free_energy_pb(jikan, fe);
     double previous = fe[0];
-    int m;
-    for (m = 1; m < max_iter; m++) {
+    int steps = 0;
+    for (int m = 1; m < max_iter; m++) {
       ion_diffusion_euler(jikan);
+      steps++;
       free_energy_pb(jikan, fe);
       double dum = fabs(fe[0] - previous);
       if (dum < 1.e-4) break;
       previous = fe[0];
     }
     if (fe_out) for (int c = 0; c < 3; c++) fe_out[c] = fe[c];
-    return m;
+    return steps;
+  }
+
+  /* Set_Poisson_Boltzmann_ion_charge_density_salt (operate_electrolyte.cxx:
+   * 623-736): one particle, symmetric z:z salt; damped fixed point (alp = 0.01)
+   * of the potential until sum|d psi| / sum|psi| < TOL = 1e-4, at most
+   * max_iter - 1 iterations (STEP = 100000); returns the iterations taken */
+  int pb_salt_single(int max_iter) {
+    const double rho_inf = EL.kbt * EL.dielectric_cst / SQ(EL.elementary_charge * EL.debye_length);
+    const double rp = rho_inf / (EL.valency[0] * (EL.valency[0] - EL.valency[1]));
+    const double rn = -rho_inf / (EL.valency[1] * (EL.valency[0] - EL.valency[1]));
+    const double alp = 0.01;
+    const double dmy0 = EL.elementary_charge * EL.valency[0] / EL.kbt;
+    const double dmy1 = EL.elementary_charge * EL.valency[1] / EL.kbt;
+    const double surface_ion_number = -EL.surface_charge * Np;  /* Init_rho_ion :750-753 */
+    double *phi = fld(KAPSEL_PHI);
+    reset(phi);
+    make_phi_particle(phi);
+    double *ep0 = EB(EB_POT), *v0 = EB(EB_PHIQQ), *v1 = EB(EB_CHARGE);
+    reset(ep0);
+    int iters = 0;
+    for (int m = 1; m < max_iter; m++) {
+      iters++;
+      double sp = 0., sn = 0.;
+#pragma omp parallel for reduction(+ : sp, sn)
+      for (int i = 0; i < NX; i++)
+        for (int j = 0; j < NY; j++)
+          for (int k = 0; k < NZ; k++) {
+            size_t im = ((size_t)i * NY + j) * NZ_ + k;
+            sp += (1. - phi[im]) * exp(-dmy0 * ep0[im]);
+            sn += (1. - phi[im]) * exp(-dmy1 * ep0[im]);
+          }
+      sp *= DX3;
+      sn *= DX3;
+      const double a = EL.valency[0] * rp * sp, b = -surface_ion_number, c = EL.valency[1] * rn * sn;
+      const double nu = (-b + sqrt(b * b - 4. * a * c)) / (2. * a);
+#pragma omp parallel for
+      for (int i = 0; i < NX; i++)
+        for (int j = 0; j < NY; j++)
+          for (int k = 0; k < NZ; k++) {
+            size_t im = ((size_t)i * NY + j) * NZ_ + k;
+            C(0)[im] = rp * exp(-dmy0 * ep0[im]) * nu;
+            C(1)[im] = rn * exp(-dmy1 * ep0[im]) / nu;
+          }
+      reset(v0);
+      reset(v1);
+      make_phi_qq_particle(v0, v1);
+      for (int n = 0; n < 2; n++) {
+        const double ze = valency_e(n);
+#pragma omp parallel for
+        for (int i = 0; i < NX; i++)
+          for (int j = 0; j < NY; j++)
+            for (int k = 0; k < NZ; k++) {
+              size_t im = ((size_t)i * NY + j) * NZ_ + k;
+              v1[im] += ze * C(n)[im] * (1. - v0[im]);
+            }
+      }
+      fft_r2c(v1);
+      coulomb_potential_k(v1);
+      fft_c2r(v1);
+      double sum = 0., error = 0.;
+#pragma omp parallel for reduction(+ : sum, error)
+      for (int i = 0; i < NX; i++)
+        for (int j = 0; j < NY; j++)
+          for (int k = 0; k < NZ; k++) {
+            size_t im = ((size_t)i * NY + j) * NZ_ + k;
+            double dmy = alp * (v1[im] - ep0[im]);
+            ep0[im] += dmy;
+            sum += fabs(ep0[im]);
+            error += fabs(dmy);
+          }
+      if (error / sum < 1.e-4) break;
+    }
+    fft_r2c(C(0));
+    fft_r2c(C(1));
+    return iters;
   }
 
   /* Make_Coulomb_force_x_on_fluid (operate_electrolyte.cxx:90-177), free particles */
@@ -2110,11 +2186,20 @@ int oracle_coulomb_force(void *h, const kapsel_time *t) { NEED_EL O->coulomb_for
 int oracle_free_energy_pb(void *h, const kapsel_time *t, double *fe) { NEED_EL O->free_energy_pb(*t, fe); return 0; }
 int oracle_init_rho_ion_pb(void *h, const kapsel_time *t, int max_iter, int *iters, double *fe) {
   NEED_EL
-  if (max_iter <= 0) max_iter = 10000;
-  /* Init_rho_ion with Poisson_Boltzmann, salt, several particles
-   * (operate_electrolyte.cxx:820-833): uniform start, steady state, Count_solute_each */
-  O->init_rho_ion();
-  *iters = O->steadystate_ion_density(*t, max_iter, fe);
+  /* Init_rho_ion with Poisson_Boltzmann, salt (operate_electrolyte.cxx:
+   * 820-833): one particle -- the Poisson-Boltzmann fixed point; several --
+   * uniform start and steady state; then Count_solute_each */
+  if (O->Np == 1) {
+    if (O->EL.valency[0] != -O->EL.valency[1]) {
+      snprintf(g_err, sizeof g_err, "a z:z salt is needed (valencies %g %g)", O->EL.valency[0], O->EL.valency[1]);
+      return -1;
+    }
+    *iters = O->pb_salt_single(max_iter > 0 ? max_iter : 100000);
+    if (fe) O->free_energy_pb(*t, fe);
+  } else {
+    O->init_rho_ion();
+    *iters = O->steadystate_ion_density(*t, max_iter > 0 ? max_iter : 10000, fe);
+  }
   O->count_solute_each(O->total_solute, O->fld(KAPSEL_PHI), O->EB(Oracle::EB_WORK));
   return 0;
 }

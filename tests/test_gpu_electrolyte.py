@@ -293,3 +293,22 @@ def test_free_energy_and_steady_state_ions_match_oracle():
     e = {f: rel(k.get_field(f), o.get_field(f)) for f in ("CONC0", "CONC1")}
     print(f"converged after {ik} (gpu) / {io} (oracle) steps", {f: f"{v:.1e}" for f, v in e.items()}, flush=True)
     assert ik == io and max(e.values()) < 1e-10
+
+
+def test_pb_one_particle_matches_oracle():
+    """One sphere (Set_Poisson_Boltzmann_ion_charge_density_salt): the same
+    number of fixed-point iterations and ion spectra within 1e-10 (FP64)."""
+    n = 32
+    print(f"case electrolyte one-particle PB start {n}^3", flush=True)
+    p = abi.make_params(n, n_particles=1, radius=4.0, xi=2.0)
+    el = abi.make_electrolyte(surface_charge=40.0, debye_length=4.0)
+    st = workloads.particle_state(np.array([[15.3, 16.1, 14.7]]))
+    k, o = gpu(p), oracle(p)
+    _load((k, o), st, el)
+    t = abi.make_time(0.01)
+    (ik, fek), (io, feo) = k.init_rho_ion_pb(t, 0), o.init_rho_ion_pb(t, 0)
+    e = {f: rel(k.get_field(f), o.get_field(f)) for f in ("CONC0", "CONC1")}
+    e["free energy"] = float(np.abs(fek - feo).max() / np.abs(feo).max())
+    e["totals"] = float(np.abs(k.get_total_solute() - o.get_total_solute()).max() / o.get_total_solute().max())
+    print(f"{ik} (gpu) / {io} (oracle) iterations", {f: f"{v:.1e}" for f, v in e.items()}, flush=True)
+    assert ik == io and max(e.values()) < 1e-10

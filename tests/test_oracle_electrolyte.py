@@ -400,3 +400,33 @@ def test_steady_state_ions_relax_toward_boltzmann():
     for s, z in ((0, 1.0), (1, -1.0)):
         mu = np.log(c[s]) + z * psi
         assert mu[far].std() < 0.15 * (z * psi)[far].std(), s
+
+
+def test_pb_one_particle_fixed_point():
+    """Set_Poisson_Boltzmann_ion_charge_density_salt (one sphere, 1:1 salt):
+    c+ = rho+ exp(-e psi/kBT) nu and c- = rho- exp(e psi/kBT)/nu, so c+ c- is
+    the bulk product everywhere; the box is neutral; and at the fixed point
+    psi is the Coulomb potential of the charge it makes (to the stopping
+    tolerance of the damped iteration)."""
+    n, q, debye = 32, 40.0, 4.0
+    p = abi.make_params(n, n_particles=1, radius=4.0, xi=2.0)
+    el = abi.make_electrolyte(surface_charge=q, debye_length=debye)
+    o = _oracle(p, el)
+    o.set_particles(workloads.particle_state(np.array([[15.3, 16.1, 14.7]])))
+    it, fe = o.init_rho_ion_pb(abi.make_time(0.01), 0)
+    assert 1 < it < 100000
+    c = [np.fft.irfftn(sp.to_complex(o.get_field(f"CONC{s}")), s=(n, n, n), axes=(0, 1, 2)) for s in (0, 1)]
+    rp = rn = 1.0 / debye ** 2 / 2.0
+    assert np.abs(c[0] * c[1] / (rp * rn) - 1.0).max() < 1e-12
+    tot = o.get_total_solute()
+    assert abs(tot[1] - tot[0] - q) < 1e-9 * q  # counter-ions balance the surface charge
+    o.charge_field("WORK0")
+    rho = _real(o, "WORK0", n)
+    assert abs(rho.sum()) < 1e-10 * q
+    rk = np.fft.rfftn(rho)
+    _, (kx, ky, kz) = sp.wavevectors((n, n, n))
+    k2 = kx * kx + ky * ky + kz * kz
+    psi = np.fft.irfftn(np.where(k2 > 0, rk / np.where(k2 > 0, k2, 1.0), 0.0), s=(n, n, n), axes=(0, 1, 2))
+    psi_e = -np.log(c[0] / c[1]) / 2.0  # c+/c- = (rp/rn) nu^2 exp(-2 psi_e): psi_e up to a constant
+    d = (psi - psi.mean()) - (psi_e - psi_e.mean())
+    assert np.abs(d).max() < 0.05 * np.abs(psi - psi.mean()).max()
