@@ -73,12 +73,13 @@ class Electrolyte(ctypes.Structure):
         ("valency", ctypes.c_double * 2), ("onsager_coeff", ctypes.c_double * 2),
         ("debye_length", ctypes.c_double), ("e_ext", ctypes.c_double * 3),
         ("ac", ctypes.c_int), ("angular_frequency", ctypes.c_double),
+        ("dielectric", ctypes.c_int),
     ]
 
 
 def make_electrolyte(kbt=1.0, elementary_charge=1.0, dielectric_cst=1.0, surface_charge=100.0,
                      valency=(1.0, -1.0), onsager_coeff=(7.24, 7.24), debye_length=10.0,
-                     e_ext=(0.0, 0.0, 0.0), ac=0, angular_frequency=0.0):
+                     e_ext=(0.0, 0.0, 0.0), ac=0, angular_frequency=0.0, dielectric=0):
     """Defaults are the salt section of the reference's UDF/input.udf:53-100."""
     e = Electrolyte()
     e.kbt, e.elementary_charge, e.dielectric_cst = kbt, elementary_charge, dielectric_cst
@@ -88,6 +89,7 @@ def make_electrolyte(kbt=1.0, elementary_charge=1.0, dielectric_cst=1.0, surface
     e.debye_length = debye_length
     e.e_ext[:] = list(e_ext)
     e.ac, e.angular_frequency = ac, angular_frequency
+    e.dielectric = dielectric
     return e
 
 
@@ -193,6 +195,9 @@ _PROTOS = {
     "kapsel_rescale_solute": [_DP],
     "kapsel_coulomb_force": [ctypes.POINTER(Time)],
     "kapsel_electrolyte_step": [ctypes.POINTER(Time)],
+    "kapsel_init_rho_ion_dielectric": [],
+    "kapsel_dielectric_potential": [ctypes.POINTER(Time), _D, _I, _I, _IP, _DP],
+    "kapsel_icep_electrolyte_step": [ctypes.POINTER(Time), _D, _I, _I, _IP, _DP],
     "kapsel_pnet_init": [ctypes.POINTER(ctypes.c_long)],
     "kapsel_pnet_set_params": [ctypes.POINTER(ctypes.c_float)],
     "kapsel_pnet_get_params": [ctypes.POINTER(ctypes.c_float)],

@@ -403,6 +403,47 @@ static void electrolyte_step(kapsel_ctx *ctx, kapsel_time &t) {
   ctx->el_reuse_gen = ctx->api_gen + 1;
 }
 
+// the free charge of the ions (and surface charge) in the variable
+// permittivity and its potential, with the uniform background that cancels
+// the charge's total (the ions inside the particles, weight phi_sin, leave one
+// that no periodic potential satisfies) in the right-hand side only
+// (Make_Maxwell_force_x_on_fluid, operate_dielectric.cxx:629-640)
+static void dielectric_potential(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int precond,
+                                 int *iters, double *relres) {
+  dielectric_field(ctx);
+  struct Background {
+    kapsel_ctx *c;
+    ~Background() { c->rho_bg = nullptr; }
+  } bg{ctx};
+  ctx->rho_bg = dielectric_free_charge(ctx);
+  poisson_solve(ctx, e0, tol, maxit, precond, iters, relres);
+}
+
+// Time_evolution_hydro, Electrolyte branch with Dielectric (sp_3d_ns.cxx:
+// 20, 24-186): NSsolute_solver_Euler with the potential's gradient, no
+// Rescale_solute, Make_Maxwell_force_x_on_fluid with the ions' charge
+static void icep_electrolyte_step(kapsel_ctx *ctx, kapsel_time &t, double tol, int maxit, int precond, int *iters,
+                                  double *relres) {
+  nssolute_step(ctx, t, false);               // :20
+  zeta_k2u(ctx, fix_mask(ctx));               // :29-42
+  md_position(ctx, t);                        // :44-50
+  make_phi_particle_sum(ctx);                 // :51-53
+  const double s = ctx->el.ac ? sin(ctx->el.angular_frequency * t.time) : 1.0;
+  const double e0[3] = {ctx->el.e_ext[0] * s, ctx->el.e_ext[1] * s, ctx->el.e_ext[2] * s};
+  dielectric_potential(ctx, e0, tol, maxit, precond, iters, relres);  // :72-74
+  maxwell_force(ctx, e0);
+  add_force_to_u(ctx, t.dt_fluid * ctx->irho);  // :75-84
+  solenoidal_u(ctx, ctx->fields + KAPSEL_U0);   // :85
+  calc_f_hydro(ctx, t);                       // :99-101
+  force(ctx);                                 // :104-110
+  md_velocity_integrate(ctx, t);
+  make_u_particle_sum(ctx);                   // :166-167
+  f_particle_dt_sole_add(ctx);                // :168-169
+  u2zeta_k(ctx);                              // :184
+  t.time += t.dt_fluid;
+  t.ts += 1;
+}
+
 // main, initial rigidity projection (sp_3d_ns.cxx:431-447)
 static void init_rigidity(kapsel_ctx *ctx) {
   make_phi_particle_sum(ctx);
@@ -1061,12 +1102,70 @@ int kapsel_electrolyte_step(kapsel_ctx *ctx, kapsel_time *jikan) {
     set_error("electrolyte parameters not set (kapsel_set_electrolyte)");
     throw Error{KAPSEL_ERR_STATE};
   }
+  if (ctx->el.dielectric) {
+    set_error("the electrolyte was set with Dielectric: kapsel_icep_electrolyte_step");
+    throw Error{KAPSEL_ERR_STATE};
+  }
   if (!(ctx->el_total[0] > 0) || !(ctx->el_total[1] > 0)) {
     set_error("ion totals not set (kapsel_init_rho_ion, kapsel_set_total_solute, or kapsel_load_restart "
               "after kapsel_set_electrolyte)");
     throw Error{KAPSEL_ERR_STATE};
   }
   electrolyte_step(ctx, *jikan);
+  API_END
+}
+
+static void check_solve_args(double tol, int maxit, int precond, const int *iters, const double *relres) {
+  if (!iters || !relres || !(tol > 0) || maxit < 1 ||
+      (precond != KAPSEL_PRECOND_NONE && precond != KAPSEL_PRECOND_SPECTRAL)) {
+    set_error("invalid solve arguments (tol > 0, maxit >= 1, a known preconditioner, non-null outputs)");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+}
+
+int kapsel_init_rho_ion_dielectric(kapsel_ctx *ctx) {
+  API_BEGIN
+  check_ctx(ctx);
+  init_rho_ion_dielectric(ctx);
+  written(ctx, KAPSEL_PHI);
+  API_END
+}
+
+int kapsel_dielectric_potential(kapsel_ctx *ctx, const kapsel_time *jikan, double tol, int maxit, int precond,
+                                int *iters, double *relres) {
+  API_BEGIN
+  check_ctx(ctx);
+  check_time(jikan);
+  check_solve_args(tol, maxit, precond, iters, relres);
+  if (!ctx->el_set || !ctx->el.dielectric) {
+    set_error("kapsel_dielectric_potential needs the electrolyte set with Dielectric");
+    throw Error{KAPSEL_ERR_STATE};
+  }
+  const double s = ctx->el.ac ? sin(ctx->el.angular_frequency * jikan->time) : 1.0;
+  const double e0[3] = {ctx->el.e_ext[0] * s, ctx->el.e_ext[1] * s, ctx->el.e_ext[2] * s};
+  dielectric_potential(ctx, e0, tol, maxit, precond, iters, relres);
+  if (!(*relres < tol)) {
+    set_error("dielectric_potential: Poisson solve stopped at relative residual %.3e", *relres);
+    return KAPSEL_ERR_NOCONV;
+  }
+  API_END
+}
+
+int kapsel_icep_electrolyte_step(kapsel_ctx *ctx, kapsel_time *jikan, double tol, int maxit, int precond,
+                                 int *iters, double *relres) {
+  API_BEGIN
+  check_ctx(ctx);
+  check_time(jikan);
+  check_solve_args(tol, maxit, precond, iters, relres);
+  if (!ctx->el_set || !ctx->el.dielectric) {
+    set_error("kapsel_icep_electrolyte_step needs the electrolyte set with Dielectric");
+    throw Error{KAPSEL_ERR_STATE};
+  }
+  icep_electrolyte_step(ctx, *jikan, tol, maxit, precond, iters, relres);
+  if (!(*relres < tol)) {
+    set_error("icep_electrolyte_step: Poisson solve stopped at relative residual %.3e", *relres);
+    return KAPSEL_ERR_NOCONV;
+  }
   API_END
 }
 

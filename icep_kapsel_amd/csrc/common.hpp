@@ -197,11 +197,7 @@ struct kapsel_ctx {
   const void *kcg_key[11] = {};
   int n_partial = 0;
   kapsel::Pnet *pnet = nullptr;   // learned warm-start network, on first use
-  // restart files (restart.hip), host only
-void restart_write(const char *path, const char *template_path, const kapsel_restart_data *d);
-void restart_read(const char *path, kapsel_restart_data *d);
-
-// electrolyte (electrolyte.hip): parameters, the conserved ion amounts, and
+  // electrolyte (electrolyte.hip): parameters, the conserved ion amounts, and
   // scratch fields allocated by kapsel_set_electrolyte: surface normal (3),
   // profile sum and raw surface charge of Make_phi_qq_particle (2), the two
   // ion concentrations in real space (2), the charge density to transform and
@@ -226,6 +222,10 @@ void restart_read(const char *path, kapsel_restart_data *d);
   unsigned long long api_gen = 0, el_reuse_gen = ~0ull;
   double *el_partial = nullptr;   // block partial sums of the electrolyte reductions
   double *el_scal = nullptr;      // device scalars (electrolyte.hip)
+  // when set, the Poisson right-hand side is built from rho - rho_bg[0]
+  // (a device scalar): the uniform background that neutralises the ions'
+  // free charge in the variable permittivity
+  const double *rho_bg = nullptr;
   // derived physics constants (Set_global_parameters, input.cxx:287-430)
   double nu, irho, hxi, mass, imass, moi, imoi, a_r_cutoff;
   // kernel timing
@@ -371,5 +371,11 @@ void recount_total_solute(kapsel_ctx *ctx);
 int init_rho_ion_pb(kapsel_ctx *ctx, const kapsel_time &t, int max_iter, double *fe);
 // Calc_free_energy_PB, non-Dielectric branch
 void free_energy_pb(kapsel_ctx *ctx, const kapsel_time &t, double *fe);
+// Init_rho_ion_dielectric (operate_electrolyte.cxx:912-979)
+void init_rho_ion_dielectric(kapsel_ctx *ctx);
+// the free charge of the Maxwell force into RHO_FREE (WORK4 holds the
+// compact-sine profile of Make_dielectric_field) and its mean into a device
+// scalar; returns that scalar's address
+const double *dielectric_free_charge(kapsel_ctx *ctx);
 
 }  // namespace kapsel

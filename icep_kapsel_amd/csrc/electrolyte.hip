@@ -29,7 +29,7 @@ constexpr unsigned TPB = 256;
 constexpr int NBR = 1024;  // blocks of a reduction (fixed: deterministic sums)
 
 // device scalars
-enum { S_SURF = 0, S_F0, S_F1, S_C0, S_C1, S_VOL, S_FE_IDEAL, S_FE_EL, S_PB0, S_PB1, S_N };
+enum { S_SURF = 0, S_F0, S_F1, S_C0, S_C1, S_VOL, S_FE_IDEAL, S_FE_EL, S_PB0, S_PB1, S_RHO_MEAN, S_N };
 // scratch fields
 enum { E_N0 = 0, E_N1, E_N2, E_QQ, E_SURF, E_CX0, E_CX1, E_RHOK, E_RHO, E_UX0, E_UX1, E_UX2, E_NF };
 
@@ -53,6 +53,7 @@ __device__ __forceinline__ double block_sum(double v) {
 // OP 4: sum rho (psi / 2 - E.(i,j,k) dx), a = rho, b = psi   (Calc_free_energy_PB, :388-411)
 // OP 5: sums (1 - phi) exp(-e0 psi), (1 - phi) exp(-e1 psi), b = psi
 //       (Set_Poisson_Boltzmann_ion_charge_density_salt, :652-662)
+// OP 6: sum a                            (total of the free charge)
 template <typename T, int OP>
 __global__ __launch_bounds__(TPB) void reduce_kernel(const T *__restrict__ a, const T *__restrict__ b,
                                                      const T *__restrict__ phi, double *__restrict__ partial,
@@ -78,6 +79,8 @@ __global__ __launch_bounds__(TPB) void reduce_kernel(const T *__restrict__ a, co
       const double w = 1. - (double)phi[p.im], v = (double)b[p.im];
       s0 += w * exp(-e0 * v);
       s1 += w * exp(-e1 * v);
+    } else if constexpr (OP == 6) {
+      s0 += (double)a[p.im];
     } else {
       double v = (double)b[p.im] * 0.5;
       if (external) v -= ((e0 * (double)p.i + e1 * (double)p.j + e2 * (double)p.k) * g.dx);
@@ -95,6 +98,7 @@ __global__ __launch_bounds__(TPB) void reduce_kernel(const T *__restrict__ a, co
 // MODE 0: scal[S_SURF] = abs_total / (sum dx^3)     (Make_phi_qq_particle, :230)
 // MODE 1: counts c_n = sum_n dx^3, factors total_n / c_n   (Rescale_solute, :117-118)
 // MODE 2: scal[S_VOL] = sum
+// MODE 6: scal[S_RHO_MEAN] = sum * abs_total (abs_total carries 1 / points)
 __global__ __launch_bounds__(TPB) void finish_kernel(const double *__restrict__ partial, double *__restrict__ scal,
                                                      int mode, double dx3, double abs_total, double t0,
                                                      double t1) {
@@ -121,6 +125,8 @@ __global__ __launch_bounds__(TPB) void finish_kernel(const double *__restrict__ 
   } else if (mode == 5) {
     scal[S_PB0] = s0;
     scal[S_PB1] = s1;
+  } else if (mode == 6) {
+    scal[S_RHO_MEAN] = s0 * abs_total;
   } else {
     scal[S_FE_EL] = s0 * dx3;
   }
@@ -146,7 +152,9 @@ __global__ void rescale_kernel(T *__restrict__ c0k, T *__restrict__ c1k, const d
 }
 
 // Conc_k2charge_field (operate_electrolyte.cxx:45-74): the normalised surface
-// charge plus sum_n Valency_e[n] c_n (1 - phi_qq); with scaled, the ion
+// charge plus sum_n Valency_e[n] c_n (1 - phi_qq); without surf, the ions
+// alone with qq the compact-sine profile (Conc_k2charge_field_no_surfase_charge,
+// operate_dielectric.cxx:135-160); with scaled, the ion
 // fields are multiplied by the rescale factors first (they hold the inverse
 // transforms of the spectra before Rescale_solute scaled them).  The result
 // goes to out (to be transformed) and, when keep is set, to keep as well.
@@ -163,7 +171,7 @@ __global__ void charge_kernel(const T *__restrict__ qq, const T *__restrict__ su
     v0 *= (T)scal[S_F0];
     v1 *= (T)scal[S_F1];
   }
-  T rho = surf[p.im] * (T)scal[S_SURF];
+  T rho = surf ? surf[p.im] * (T)scal[S_SURF] : (T)0;
   rho += (T)ze0 * v0 * one_phi;
   rho += (T)ze1 * v1 * one_phi;
   out[p.im] = rho;
@@ -314,6 +322,51 @@ __global__ void coulomb_force_kernel(T *__restrict__ f0, T *__restrict__ f1, T *
   f2[im] = q * v;
 }
 
+// Rhs_NS_Nernst_Planck, Dielectric branch (fluid_solver.cxx:261-270): the
+// potential's gradient on the staggered faces, i k_d e^{i k_d dx/2} psi_k
+// (A_k2da_k, Shift_vec_fw_imag), the k = 0 mode shifted by the applied field
+// (Add_constant_field_k, :231-244)
+template <typename T>
+__global__ void dielectric_grad_k_kernel(const T *__restrict__ psik, T *__restrict__ g0, T *__restrict__ g1,
+                                         T *__restrict__ g2, const V2<T> *__restrict__ shx,
+                                         const V2<T> *__restrict__ shy, const V2<T> *__restrict__ shz, double dc0,
+                                         double dc1, double dc2, Grid g) {
+  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  const KIdx k = kidx(g, c);
+  const V2<T> x = ld2(psik + 2 * c);
+  const T w0 = (T)(k.kx * g.wave_x), w1 = (T)(k.ky * g.wave_y), w2 = (T)(k.kz * g.wave_z);
+  V2<T> d0 = cmul(V2<T>{-w0 * x.im, w0 * x.re}, shx[k.i]);
+  V2<T> d1 = cmul(V2<T>{-w1 * x.im, w1 * x.re}, shy[k.j]);
+  V2<T> d2 = cmul(V2<T>{-w2 * x.im, w2 * x.re}, shz[k.kz]);
+  if (c == 0) {
+    d0.re -= (T)dc0;
+    d1.re -= (T)dc1;
+    d2.re -= (T)dc2;
+  }
+  st2(g0 + 2 * c, d0);
+  st2(g1 + 2 * c, d1);
+  st2(g2 + 2 * c, d2);
+}
+
+// Interpolate_vec_on_normal_grid (operate_dielectric.cxx:32-55): the face
+// values averaged onto the cells, f_d(i) = g_d(i) / 2 + g_d(i - e_d) / 2
+template <typename T>
+__global__ void normal_grid_kernel(const T *__restrict__ g0, const T *__restrict__ g1, const T *__restrict__ g2,
+                                   T *__restrict__ f0, T *__restrict__ f1, T *__restrict__ f2, Grid g) {
+  size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= g.nr) return;
+  const RI p = ridx(g, r);
+  const int im1 = p.i == 0 ? g.nx - 1 : p.i - 1, jm1 = p.j == 0 ? g.ny - 1 : p.j - 1;
+  const int km1 = p.k == 0 ? g.nz - 1 : p.k - 1;
+  const size_t xm = ((size_t)im1 * g.ny + p.j) * g.nz_ + p.k;
+  const size_t ym = ((size_t)p.i * g.ny + jm1) * g.nz_ + p.k;
+  const size_t zm = ((size_t)p.i * g.ny + p.j) * g.nz_ + km1;
+  f0[p.im] = (T)0.5 * g0[p.im] + (T)0.5 * g0[xm];
+  f1[p.im] = (T)0.5 * g1[p.im] + (T)0.5 * g1[ym];
+  f2[p.im] = (T)0.5 * g2[p.im] + (T)0.5 * g2[zm];
+}
+
 // U2advection_k, real-space part (operate_omega.cxx:30-47), from the kept
 // real velocity: products into U0..U2 and WORK0/WORK1
 template <typename T>
@@ -411,6 +464,31 @@ void need_el(const kapsel_ctx *ctx) {
   }
 }
 
+// the Dielectric electrolyte: one Dielectric_cst for the ions and the
+// permittivity (the reference has one global)
+void need_el_dielectric(const kapsel_ctx *ctx) {
+  need_el(ctx);
+  if (!ctx->el.dielectric) {
+    set_error("the electrolyte was set without Dielectric");
+    throw Error{KAPSEL_ERR_STATE};
+  }
+  if (ctx->el.dielectric_cst != ctx->P.dielectric_cst) {
+    set_error("Dielectric: the electrolyte's Dielectric_cst (%g) differs from the permittivity's (%g, "
+              "kapsel_set_dielectric)",
+              ctx->el.dielectric_cst, ctx->P.dielectric_cst);
+    throw Error{KAPSEL_ERR_ARG};
+  }
+}
+
+// the branches that exist only without Dielectric
+void need_el_plain(const kapsel_ctx *ctx, const char *what) {
+  need_el(ctx);
+  if (ctx->el.dielectric) {
+    set_error("%s: not a Dielectric branch", what);
+    throw Error{KAPSEL_ERR_STATE};
+  }
+}
+
 template <typename T, int OP>
 void reduce(kapsel_ctx *ctx, const void *a, const void *b, const void *phi) {
   const double fields = OP == 1 ? 3.0 : 1.0;
@@ -488,11 +566,14 @@ void coulomb_grad_t(kapsel_ctx *ctx, const T *rhok, T *const *g3, int trunc, con
 // Ion_diffusion_solver_Euler runs for Set_steadystate_ion_density
 // (fluid_solver.cxx:293-316, operate_electrolyte.cxx:436-444; the solvent
 // right-hand side it also computes is discarded there).
+// Under Dielectric (never with reuse) the gradient is the one of the
+// POTENTIAL field, the last solve's answer in the variable permittivity.
 template <typename T>
 void nssolute_t(kapsel_ctx *ctx, const kapsel_time &t, bool reuse, bool ions_only = false) {
   const Grid &g = ctx->g;
+  const bool dielectric = ctx->el.dielectric != 0;
   T *rhok = ef<T>(ctx, E_RHOK);
-  if (!reuse) {
+  if (!reuse && !dielectric) {
     // charge density of the current state (concentrations not yet cut)
     ions_real_t<T>(ctx);
     charge_t<T>(ctx, rhok, nullptr, 0);
@@ -502,8 +583,31 @@ void nssolute_t(kapsel_ctx *ctx, const kapsel_time &t, bool reuse, bool ions_onl
   const double amp = (double)(g.nx * g.ny * g.nz) * ac_factor(ctx, t);
   const double dc[3] = {ctx->el.e_ext[0] * amp, ctx->el.e_ext[1] * amp, ctx->el.e_ext[2] * amp};
   T *grad[3] = {fl<T>(ctx, KAPSEL_F0), fl<T>(ctx, KAPSEL_F1), fl<T>(ctx, KAPSEL_F2)};
-  coulomb_grad_t<T>(ctx, rhok, grad, 1, dc);
-  fft_c2r_t<T>(ctx, (void *const *)grad, 3);
+  if (dielectric) {
+    // E_RHOK: the potential's spectrum; E_UX0..2: the staggered gradient
+    copy_field(ctx, rhok, ctx->fields[KAPSEL_POTENTIAL]);
+    void *one[1] = {rhok};
+    fft_r2c_t<T>(ctx, one, 1);
+    T *face[3] = {ef<T>(ctx, E_UX0), ef<T>(ctx, E_UX1), ef<T>(ctx, E_UX2)};
+    const Twiddles &tw = ctx->tw[prec_of<T>()];
+    {
+      LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, 4.0 * g.f * sizeof(T));
+      hipLaunchKernelGGL(dielectric_grad_k_kernel<T>, dim3(blocks_for(g.nc, TPB)), dim3(TPB), 0, ctx->stream, rhok,
+                         face[0], face[1], face[2], (const V2<T> *)tw.shx, (const V2<T> *)tw.shy,
+                         (const V2<T> *)tw.shz, dc[0], dc[1], dc[2], g);
+      KAPSEL_LAUNCH_CHECK();
+    }
+    fft_c2r_t<T>(ctx, (void *const *)face, 3);
+    {
+      LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, 6.0 * g.nr * sizeof(T));
+      hipLaunchKernelGGL(normal_grid_kernel<T>, dim3(blocks_for(g.nr, TPB)), dim3(TPB), 0, ctx->stream, face[0],
+                         face[1], face[2], grad[0], grad[1], grad[2], g);
+      KAPSEL_LAUNCH_CHECK();
+    }
+  } else {
+    coulomb_grad_t<T>(ctx, rhok, grad, 1, dc);
+    fft_c2r_t<T>(ctx, (void *const *)grad, 3);
+  }
   // u of the truncated zeta (Rhs_NS_solute :203-207)
   T *u[3] = {nullptr, nullptr, nullptr};
   if (ions_only) {
@@ -733,9 +837,13 @@ int pb_salt_single_t(kapsel_ctx *ctx, int max_iter) {
   return iters;
 }
 
-// Init_rho_ion, N_spec = 2, uniform salt (operate_electrolyte.cxx:559-621, 860)
+// Init_rho_ion, N_spec = 2, uniform salt (operate_electrolyte.cxx:559-621,
+// 860); under Dielectric the bulk density carries eps_fluid (:597-600).
+// no_surface: Set_uniform_ion_charge_density_salt_dielectric (:864-910), the
+// bulk density kBT Dielectric_cst eps_fluid / (e lambda_D RADIUS)^2 and no
+// counter-ions
 template <typename T>
-void init_rho_ion_t(kapsel_ctx *ctx) {
+void init_rho_ion_t(kapsel_ctx *ctx, bool no_surface = false) {
   const kapsel_electrolyte &e = ctx->el;
   make_phi_particle(ctx, ctx->fields[KAPSEL_PHI]);
   reduce<T, 2>(ctx, nullptr, nullptr, ctx->fields[KAPSEL_PHI]);
@@ -748,8 +856,14 @@ void init_rho_ion_t(kapsel_ctx *ctx) {
   const double dx3 = ctx->g.dx * ctx->g.dx * ctx->g.dx;
   const double pd = positive_ion_number / (volume_phi * dx3);
   const double nd = negative_ion_number / (volume_phi * dx3);
-  const double rho_inf = e.kbt * e.dielectric_cst / ((e.elementary_charge * e.debye_length) *
-                                                      (e.elementary_charge * e.debye_length));
+  double rho_inf = e.kbt * e.dielectric_cst / ((e.elementary_charge * e.debye_length) *
+                                                (e.elementary_charge * e.debye_length));
+  if (no_surface) {
+    const double l = e.elementary_charge * e.debye_length * ctx->P.radius;
+    rho_inf = e.kbt * e.dielectric_cst * ctx->P.eps_fluid / (l * l);
+  } else if (e.dielectric) {
+    rho_inf *= ctx->P.eps_fluid;
+  }
   const double rp = rho_inf / (e.valency[0] * (e.valency[0] - e.valency[1]));
   const double rn = -rho_inf / (e.valency[1] * (e.valency[0] - e.valency[1]));
   {
@@ -775,16 +889,17 @@ void init_rho_ion_t(kapsel_ctx *ctx) {
 }  // namespace
 
 void electrolyte_set(kapsel_ctx *ctx, const kapsel_electrolyte &el) {
-  if (ctx->P.n_particles < 1) {
+  const bool surface = el.surface_charge != 0.0;
+  if (surface && ctx->P.n_particles < 1) {
     set_error("the electrolyte needs particles: the surface charge is normalised by their total "
               "(Make_phi_qq_particle)");
     throw Error{KAPSEL_ERR_ARG};
   }
-  if (el.surface_charge == 0.0 || !(el.valency[0] > 0.0) || !(el.valency[1] < 0.0) || !(el.kbt > 0.0) ||
+  if ((!surface && !el.dielectric) || !(el.valency[0] > 0.0) || !(el.valency[1] < 0.0) || !(el.kbt > 0.0) ||
       !(el.dielectric_cst > 0.0) || !(el.debye_length > 0.0) || !(el.elementary_charge > 0.0) ||
       !(el.onsager_coeff[0] >= 0.0) || !(el.onsager_coeff[1] >= 0.0)) {
-    set_error("invalid electrolyte parameters (nonzero surface charge, valencies +/-, positive kBT, "
-              "Dielectric_cst, Debye length, elementary charge; Onsager coefficients >= 0)");
+    set_error("invalid electrolyte parameters (nonzero surface charge unless Dielectric, valencies +/-, "
+              "positive kBT, Dielectric_cst, Debye length, elementary charge; Onsager coefficients >= 0)");
     throw Error{KAPSEL_ERR_ARG};
   }
   if (!ctx->el_f[0]) {
@@ -845,7 +960,7 @@ static int init_rho_ion_pb_t(kapsel_ctx *ctx, const kapsel_time &t, int max_iter
   return it;
 }
 int init_rho_ion_pb(kapsel_ctx *ctx, const kapsel_time &t, int max_iter, double *fe) {
-  need_el(ctx);
+  need_el_plain(ctx, "the Poisson-Boltzmann start");
   if (ctx->P.n_particles == 1 && ctx->el.valency[0] != -ctx->el.valency[1]) {
     // Set_Poisson_Boltzmann_ion_charge_density_salt (:629-633)
     set_error("the one-particle Poisson-Boltzmann start needs a z:z salt (valencies %g, %g)", ctx->el.valency[0],
@@ -856,7 +971,7 @@ int init_rho_ion_pb(kapsel_ctx *ctx, const kapsel_time &t, int max_iter, double 
 }
 
 void free_energy_pb(kapsel_ctx *ctx, const kapsel_time &t, double *fe) {
-  need_el(ctx);
+  need_el_plain(ctx, "Calc_free_energy_PB");
   EL_DISPATCH(ctx, free_energy_host_t, ctx, t, fe);
 }
 
@@ -890,7 +1005,7 @@ void nssolute_euler(kapsel_ctx *ctx, const kapsel_time &t) {
 
 void nssolute_step(kapsel_ctx *ctx, const kapsel_time &t, bool reuse) {
   need_el(ctx);
-  EL_DISPATCH(ctx, nssolute_t, ctx, t, reuse);
+  EL_DISPATCH(ctx, nssolute_t, ctx, t, reuse && !ctx->el.dielectric);
 }
 
 void *const *electrolyte_u_keep(kapsel_ctx *ctx) { return ctx->el_f + E_UX0; }
@@ -909,15 +1024,50 @@ void rescale_solute(kapsel_ctx *ctx, double *factor_host) {
 }
 
 void coulomb_force(kapsel_ctx *ctx, const kapsel_time &t) {
-  need_el(ctx);
+  need_el_plain(ctx, "the Coulomb force");
   EL_DISPATCH(ctx, coulomb_force_t, ctx, t, 0);
+}
+
+template <typename T>
+static void init_rho_ion_dielectric_t(kapsel_ctx *ctx) {
+  init_rho_ion_t<T>(ctx, ctx->el.surface_charge == 0.0);
+}
+void init_rho_ion_dielectric(kapsel_ctx *ctx) {
+  need_el_dielectric(ctx);
+  EL_DISPATCH(ctx, init_rho_ion_dielectric_t, ctx);
+}
+
+// Make_Maxwell_force_x_on_fluid's free charge (operate_dielectric.cxx:
+// 629-640): with surface charge Conc_k2charge_field, without it the ions
+// weighted by the compact-sine profile in WORK4; then its mean
+template <typename T>
+static const double *dielectric_free_charge_t(kapsel_ctx *ctx) {
+  const Grid &g = ctx->g;
+  T *rho = fl<T>(ctx, KAPSEL_RHO_FREE);
+  ions_real_t<T>(ctx);
+  if (ctx->el.surface_charge != 0.0) {
+    charge_t<T>(ctx, rho, nullptr, 0);
+  } else {
+    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, 4.0 * g.nr * sizeof(T));
+    hipLaunchKernelGGL(charge_kernel<T>, dim3(blocks_for(g.nr, TPB)), dim3(TPB), 0, ctx->stream,
+                       fl<T>(ctx, KAPSEL_WORK4), (const T *)nullptr, ef<T>(ctx, E_CX0), ef<T>(ctx, E_CX1),
+                       ctx->el_scal, valency_e(ctx, 0), valency_e(ctx, 1), 0, rho, (T *)nullptr, g);
+    KAPSEL_LAUNCH_CHECK();
+  }
+  reduce<T, 6>(ctx, rho, nullptr, nullptr);
+  finish(ctx, 6, 1.0 / (double)g.nr);
+  return ctx->el_scal + S_RHO_MEAN;
+}
+const double *dielectric_free_charge(kapsel_ctx *ctx) {
+  need_el_dielectric(ctx);
+  return EL_DISPATCH(ctx, dielectric_free_charge_t, ctx);
 }
 
 // the Electrolyte branch's middle part (sp_3d_ns.cxx:55-95): Rescale_solute
 // right after the profile scatter (PHI_SUM current), then the Coulomb force
 // from the rescaled ions
 void rescale_and_coulomb_force(kapsel_ctx *ctx, const kapsel_time &t) {
-  need_el(ctx);
+  need_el_plain(ctx, "the Coulomb force");
   if (ctx->prec == KAPSEL_FP64) {
     rescale_t<double>(ctx, true);
     coulomb_force_t<double>(ctx, t, 1);

@@ -187,12 +187,15 @@ __global__ void interp_staggered_kernel(T *__restrict__ v0, T *__restrict__ v1, 
   v2[p.im] = ((T)0.5 * eps[nbr(g, p.i, p.j, p.k + 1)] + sn) * (T)f2;
 }
 
+// a -= b over the real points; with bg, a -= b - bg[0] (a uniform background
+// charge that makes the periodic problem solvable)
 template <typename T>
-__global__ void sub_kernel(T *__restrict__ a, const T *__restrict__ b, Grid g) {
+__global__ void sub_kernel(T *__restrict__ a, const T *__restrict__ b, const double *__restrict__ bg, Grid g) {
   size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= g.nr) return;
   RI p = ridx(g, r);
-  a[p.im] -= b[p.im];
+  if (bg) a[p.im] -= b[p.im] - (T)bg[0];
+  else a[p.im] -= b[p.im];
 }
 
 // z_k <- z_k / (eps_mean |k|^2), 0 at k = 0
@@ -586,7 +589,8 @@ void build_rhs_t(kapsel_ctx *ctx, T *rhs, const T *rho, const T *eps, const doub
   fft_c2r_t<T>(ctx, o1, 1);
   {
     LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 3, sizeof(T)));
-    hipLaunchKernelGGL(sub_kernel<T>, dim3(blocks_for(g.nr, TPB)), dim3(TPB), 0, ctx->stream, rhs, rho, g);
+    hipLaunchKernelGGL(sub_kernel<T>, dim3(blocks_for(g.nr, TPB)), dim3(TPB), 0, ctx->stream, rhs, rho,
+                       ctx->rho_bg, g);
     KAPSEL_LAUNCH_CHECK();
   }
 }

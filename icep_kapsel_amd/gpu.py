@@ -268,6 +268,26 @@ class KapselGPU:
     def init_rho_ion(self):
         self._call("init_rho_ion")
 
+    def init_rho_ion_dielectric(self):
+        self._call("init_rho_ion_dielectric")
+
+    def _solve_call(self, name, t, tol, maxit, precond, check):
+        it = ctypes.c_int(0)
+        rr = ctypes.c_double(0.0)
+        rc = getattr(self.lib, "kapsel_" + name)(self.ctx, ctypes.byref(t), tol, int(maxit), int(precond),
+                                                 ctypes.byref(it), ctypes.byref(rr))
+        if rc != 0 and (check or rc != -4):
+            raise KapselError(f"kapsel_{name}: {rc}: {self.lib.kapsel_last_error().decode()}")
+        return it.value, rr.value
+
+    def dielectric_potential(self, t, tol, maxit, precond=0, check=True):
+        """The ions' (and surface charge's) potential in the permittivity; (iters, relres)."""
+        return self._solve_call("dielectric_potential", t, tol, maxit, precond, check)
+
+    def icep_electrolyte_step(self, t, tol, maxit, precond=0, check=True):
+        """One Electrolyte + Dielectric step; advances t; (iters, relres)."""
+        return self._solve_call("icep_electrolyte_step", t, tol, maxit, precond, check)
+
     def init_rho_ion_pb(self, t, max_iter=10000):
         it = ctypes.c_int(0)
         fe = np.zeros(3)

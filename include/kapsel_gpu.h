@@ -268,6 +268,10 @@ typedef struct kapsel_electrolyte {
   double e_ext[3];           /* E_ext; all zero when External_field is OFF */
   int ac;                    /* AC: the applied field is E_ext sin(angular_frequency * time) */
   double angular_frequency;  /* Angular_Frequency (2 pi Frequency) */
+  int dielectric;            /* Dielectric ON: the ions live in the variable permittivity of
+                                kapsel_params (eps_fluid, eps_particle); surface_charge may be 0,
+                                and then debye_length is in units of RADIUS (Init_rho_ion_dielectric,
+                                operate_electrolyte.cxx:864-979) */
 } kapsel_electrolyte;
 
 /* Sets the electrolyte parameters; checks of Init_rho_ion
@@ -306,8 +310,11 @@ int kapsel_make_surface_normal(kapsel_ctx *ctx, int field0);
  * density of the particles and the ions (CONC0/CONC1) into `field`. */
 int kapsel_charge_field(kapsel_ctx *ctx, int field);
 /* NSsolute_solver_Euler (fluid_solver.cxx:318-338) with Rhs_NS_Nernst_Planck
- * (:246-292, non-Dielectric branch), Rhs_NS_solute (:184-229) and the
- * solute_rhs.cxx fluxes (:42-233): one Euler step of zeta and CONC0/CONC1. */
+ * (:246-292), Rhs_NS_solute (:184-229) and the solute_rhs.cxx fluxes
+ * (:42-233): one Euler step of zeta and CONC0/CONC1.  Under Dielectric the
+ * electric field is the staggered gradient of the POTENTIAL field averaged
+ * onto the cells (:261-270, Interpolate_vec_on_normal_grid,
+ * operate_dielectric.cxx:32-55). */
 int kapsel_nssolute_euler(kapsel_ctx *ctx, const kapsel_time *jikan);
 /* Rescale_solute (solute_rhs.cxx:99-128): PHI <- unclamped profile sum,
  * CONC_n *= Total_solute[n] / Count_single_solute; factors may be NULL. */
@@ -323,6 +330,37 @@ int kapsel_coulomb_force(kapsel_ctx *ctx, const kapsel_time *jikan);
  * time += dt_fluid, ts += 1.  The reference's first Calc_f_hydro of the
  * branch (:60-68) only fills the diagnostic f_hydro1 and is not repeated. */
 int kapsel_electrolyte_step(kapsel_ctx *ctx, kapsel_time *jikan);
+
+/* ---- electrolyte in the variable permittivity (Dielectric ON) ------------ */
+/* Init_rho_ion_dielectric (operate_electrolyte.cxx:912-979): with surface
+ * charge Init_rho_ion (bulk density times eps_fluid, :597-600); without,
+ * Set_uniform_ion_charge_density_salt_dielectric (:864-910), the bulk density
+ * kBT Dielectric_cst eps_fluid / (Elementary_charge Debye_length RADIUS)^2 of
+ * both species and no counter-ions; then Count_solute_each. */
+int kapsel_init_rho_ion_dielectric(kapsel_ctx *ctx);
+/* The potential of the ions (and surface charge) in the permittivity:
+ * Make_dielectric_field, the free charge (Conc_k2charge_field with surface
+ * charge, Conc_k2charge_field_no_surfase_charge without,
+ * operate_dielectric.cxx:135-160, 629-640) into RHO_FREE, the solve into
+ * POTENTIAL warm-started from it, as Init_potential_dielectric
+ * (operate_dielectric.cxx:574-609) and the Maxwell force do.  The applied
+ * field is the electrolyte's E_ext (times sin(omega t) under AC).  The
+ * right-hand side gets a uniform background that cancels the free charge's
+ * total (nonzero once ions sit inside particles), without which the periodic
+ * problem has no solution; RHO_FREE itself keeps the total.  Returns
+ * KAPSEL_ERR_NOCONV when the relative residual is not below tol. */
+int kapsel_dielectric_potential(kapsel_ctx *ctx, const kapsel_time *jikan, double tol, int maxit, int precond,
+                                int *iters, double *relres);
+/* Time_evolution_hydro, Electrolyte branch with Dielectric (sp_3d_ns.cxx:
+ * 20, 24-186): NSsolute_solver_Euler (Dielectric gradient), FIX_CELL,
+ * Zeta_k2u, particle position update, profile scatter, the potential as in
+ * kapsel_dielectric_potential, Make_Maxwell_force_x_on_fluid
+ * (operate_dielectric.cxx:610-760), u += f dt / RHO, Solenoidal_u,
+ * hydrodynamic force, particle velocity update, rigidity constraint,
+ * U2zeta_k; no Rescale_solute.  Needs Dielectric_cst equal in the electrolyte
+ * and kapsel_set_dielectric. */
+int kapsel_icep_electrolyte_step(kapsel_ctx *ctx, kapsel_time *jikan, double tol, int maxit, int precond,
+                                 int *iters, double *relres);
 
 /* ---- thermal fluctuations ------------------------------------------------ */
 /* kBT, alpha_v, alpha_o of the Navier_Stokes equation (input.cxx:280-281,

@@ -1524,7 +1524,8 @@ struct Oracle {
   }
 
   /* Init_rho_ion (operate_electrolyte.cxx:738-861), N_spec = 2 without
-   * Poisson_Boltzmann: Set_uniform_ion_charge_density_salt (:559-621) */
+   * Poisson_Boltzmann: Set_uniform_ion_charge_density_salt (:559-621); under
+   * Dielectric the bulk density carries the fluid's permittivity */
   void init_rho_ion() {
     double *phi = fld(KAPSEL_PHI);
     reset(phi);
@@ -1541,6 +1542,7 @@ struct Oracle {
     double pd = positive_ion_number / (volume_phi * DX3);
     double nd = negative_ion_number / (volume_phi * DX3);
     double rho_inf = EL.kbt * EL.dielectric_cst / SQ(EL.elementary_charge * EL.debye_length);
+    if (EL.dielectric) rho_inf *= P.eps_fluid; /* :598-600 */
     double rp = rho_inf / (EL.valency[0] * (EL.valency[0] - EL.valency[1]));
     double rn = -rho_inf / (EL.valency[1] * (EL.valency[0] - EL.valency[1]));
     reset(C(0));
@@ -1645,9 +1647,41 @@ struct Oracle {
     }
   }
 
+  /* Rhs_NS_Nernst_Planck, Dielectric branch (fluid_solver.cxx:261-270): the
+   * staggered gradient of the potential (A_k2da_k, Shift_vec_fw_imag), its
+   * k = 0 shifted by the applied field (Add_constant_field_k, :231-244),
+   * back on the cell centres (Interpolate_vec_on_normal_grid,
+   * operate_dielectric.cxx:32-55) */
+  void dielectric_grad(const kapsel_time &jikan, double **grad) {
+    double *w = scratch(0);
+    memcpy(w, fld(KAPSEL_POTENTIAL), F * sizeof(double));
+    fft_r2c(w);
+    grad_shift_k(w, grad);
+    double amp = (double)(NX * NY * NZ) * ac_factor(jikan);
+    for (int d = 0; d < 3; d++) grad[d][0] -= EL.e_ext[d] * amp;
+    for (int d = 0; d < 3; d++) fft_c2r(grad[d]);
+    std::vector<double> tmp[3];
+    for (int d = 0; d < 3; d++) tmp[d].assign(grad[d], grad[d] + F);
+#pragma omp parallel for
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++)
+        for (int k = 0; k < NZ; k++) {
+          size_t im = ((size_t)i * NY + j) * NZ_ + k;
+          grad[0][im] = 0.5 * tmp[0][im] + 0.5 * tmp[0][extract_id(i - 1, j, k)];
+          grad[1][im] = 0.5 * tmp[1][im] + 0.5 * tmp[1][extract_id(i, j - 1, k)];
+          grad[2][im] = 0.5 * tmp[2][im] + 0.5 * tmp[2][extract_id(i, j, k - 1)];
+        }
+  }
+
   /* Rhs_NS_Nernst_Planck, non-Dielectric branch (fluid_solver.cxx:246-292)
-   * with Add_constant_field_k (:231-244) */
+   * with Add_constant_field_k (:231-244); the Dielectric branch when EL.dielectric */
   void rhs_ns_nernst_planck(const kapsel_time &jikan, double **rhs_ns, double **rhs_solute) {
+    if (EL.dielectric) {
+      double *grad[3] = {EB(EB_GRAD), EB(EB_GRAD + 1), EB(EB_GRAD + 2)};
+      dielectric_grad(jikan, grad);
+      rhs_ns_solute(grad, rhs_ns, rhs_solute);
+      return;
+    }
     double *pot = EB(EB_CHARGE);
     conc_k2charge_field(pot, EB(EB_PHIQQ), EB(EB_WORK));
     fft_r2c(pot);
@@ -1885,6 +1919,129 @@ struct Oracle {
     fft_r2c(C(0));
     fft_r2c(C(1));
     return iters;
+  }
+
+  /* the free charge of the Maxwell force and the initial potential
+   * (operate_dielectric.cxx:597-607, 631-643): with surface charge
+   * Conc_k2charge_field (smooth profile and surface charge); without,
+   * Conc_k2charge_field_no_surfase_charge (:135-160) with the compact-sine
+   * profile Make_dielectric_field left in phi_sin */
+  void dielectric_free_charge(double *rho, const double *phi_sin) {
+    if (EL.surface_charge != 0.0) {
+      conc_k2charge_field(rho, EB(EB_PHIQQ), EB(EB_WORK));
+      return;
+    }
+    reset(rho);
+    double *work = EB(EB_WORK);
+    for (int n = 0; n < 2; n++) {
+      memcpy(work, C(n), F * sizeof(double));
+      fft_c2r(work);
+      const double ze = valency_e(n);
+#pragma omp parallel for
+      for (int i = 0; i < NX; i++)
+        for (int j = 0; j < NY; j++)
+          for (int k = 0; k < NZ; k++) {
+            size_t im = ((size_t)i * NY + j) * NZ_ + k;
+            rho[im] += ze * work[im] * (1. - phi_sin[im]);
+          }
+    }
+  }
+
+  /* Init_rho_ion_dielectric (operate_electrolyte.cxx:912-979): with surface
+   * charge Init_rho_ion; without, Set_uniform_ion_charge_density_salt_dielectric
+   * (:864-910: bulk density kBT Dielectric_cst eps_fluid / (e lambda_D a)^2, no
+   * counter-ions) and Count_solute_each */
+  void init_rho_ion_dielectric() {
+    if (EL.surface_charge != 0.0) {
+      init_rho_ion();
+      return;
+    }
+    double *phi = fld(KAPSEL_PHI);
+    reset(phi);
+    make_phi_particle(phi);
+    double volume_phi = 0.;
+#pragma omp parallel for reduction(+ : volume_phi)
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++)
+        for (int k = 0; k < NZ; k++) volume_phi += (1. - phi[((size_t)i * NY + j) * NZ_ + k]);
+    const double pd = 0.0 / (volume_phi * DX3), nd = 0.0 / (volume_phi * DX3);
+    const double rho_inf = EL.kbt * EL.dielectric_cst * P.eps_fluid / SQ(EL.elementary_charge * EL.debye_length * RADIUS);
+    const double rp = rho_inf / (EL.valency[0] * (EL.valency[0] - EL.valency[1]));
+    const double rn = -rho_inf / (EL.valency[1] * (EL.valency[0] - EL.valency[1]));
+    reset(C(0));
+    reset(C(1));
+#pragma omp parallel for
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++)
+        for (int k = 0; k < NZ; k++) {
+          size_t im = ((size_t)i * NY + j) * NZ_ + k;
+          C(0)[im] = pd + rp;
+          C(1)[im] = nd + rn;
+        }
+    total_solute[0] = count_single_solute(C(0), phi);
+    total_solute[1] = count_single_solute(C(1), phi);
+    fft_r2c(C(0));
+    fft_r2c(C(1));
+    count_solute_each(total_solute, phi, EB(EB_WORK));
+  }
+
+  /* the potential of the ions (and surface charge) in the variable
+   * permittivity: Make_dielectric_field, the free charge, the solve warm-started
+   * from POTENTIAL (Init_potential_dielectric, operate_dielectric.cxx:574-609;
+   * Make_Maxwell_force_x_on_fluid :629-643) */
+  int dielectric_potential(const kapsel_time &jikan, double tol, int maxit, int solver, double *relres, double *e0) {
+    const double s = ac_factor(jikan);
+    for (int d = 0; d < 3; d++) e0[d] = EL.e_ext[d] * s;
+    make_dielectric_field(fld(KAPSEL_EPSILON), fld(KAPSEL_WORK4));
+    dielectric_free_charge(fld(KAPSEL_RHO_FREE), fld(KAPSEL_WORK4));
+    build_rhs(fld(KAPSEL_RHS), fld(KAPSEL_RHO_FREE), fld(KAPSEL_EPSILON), e0);
+    /* the ions inside the particles (weight phi_sin) leave the free charge
+     * with a nonzero total, which no periodic potential satisfies: a uniform
+     * background cancels it, so the solve sees the consistent part of the
+     * right-hand side (the reference's BiCGSTAB gets the raw total and stops
+     * at MaxIter_potential) */
+    const double mean = dot(fld(KAPSEL_RHO_FREE), scratch_ones()) / ((double)NX * NY * NZ);
+    double *rhs = fld(KAPSEL_RHS);
+#pragma omp parallel for
+    for (int i = 0; i < NX; i++)
+      for (int j = 0; j < NY; j++)
+        for (int k = 0; k < NZ; k++) rhs[((size_t)i * NY + j) * NZ_ + k] += mean;
+    double *x = fld(KAPSEL_POTENTIAL);
+    return solver == 2 ? bicgstab_solve(x, fld(KAPSEL_RHS), tol, maxit, relres)
+                       : cg_solve(x, fld(KAPSEL_RHS), tol, maxit, solver, relres);
+  }
+
+  /* Time_evolution_hydro, Electrolyte branch with Dielectric (sp_3d_ns.cxx:
+   * 20, 24-186): NSsolute_solver_Euler with the potential's gradient, the
+   * particle update, no Rescale_solute, Make_Maxwell_force_x_on_fluid with the
+   * ions' charge, u += f dt / RHO, Solenoidal_u, the hydrodynamic force, the
+   * velocity update, the rigidity constraint, U2zeta_k */
+  int icep_electrolyte_step(kapsel_time &jikan, double tol, int maxit, int solver, double *relres) {
+    nssolute_euler(jikan);
+    for (int d = 0; d < 3; d++)
+      if (P.fix_cell[d]) uk_dc[d] = 0.0;
+    truncate(fld(KAPSEL_ZETA0));
+    truncate(fld(KAPSEL_ZETA1));
+    zeta_k2u();
+    md_position(jikan);
+    reset(fld(KAPSEL_PHI));
+    reset(fld(KAPSEL_PHI_SUM));
+    make_phi_particle_sum(fld(KAPSEL_PHI), fld(KAPSEL_PHI_SUM));
+    double e0[3];
+    int it = dielectric_potential(jikan, tol, maxit, solver, relres, e0);
+    maxwell_force(e0);
+    add_force_to_fluid(jikan);
+    double *u[3] = {U(0), U(1), U(2)};
+    calc_f_hydro(fld(KAPSEL_PHI_SUM), u, jikan);
+    md_velocity(jikan);
+    for (int d = 0; d < 3; d++) reset(UP(d));
+    double *up[3] = {UP(0), UP(1), UP(2)};
+    make_u_particle_sum(up, fld(KAPSEL_PHI_SUM));
+    make_f_particle_dt_sole_add();
+    u2zeta_k();
+    jikan.time += jikan.dt_fluid;
+    jikan.ts += 1;
+    return it;
   }
 
   /* Make_Coulomb_force_x_on_fluid (operate_electrolyte.cxx:90-177), free particles */
@@ -2204,6 +2361,27 @@ int oracle_init_rho_ion_pb(void *h, const kapsel_time *t, int max_iter, int *ite
   return 0;
 }
 int oracle_electrolyte_step(void *h, kapsel_time *t) { NEED_EL O->electrolyte_step(*t); return 0; }
+int oracle_init_rho_ion_dielectric(void *h) { NEED_EL O->init_rho_ion_dielectric(); return 0; }
+/* the gradient the Dielectric branch of Rhs_NS_Nernst_Planck uses, into F0..F2 */
+int oracle_dielectric_grad(void *h, const kapsel_time *t) {
+  NEED_EL
+  double *g[3] = {O->FP(0), O->FP(1), O->FP(2)};
+  O->dielectric_grad(*t, g);
+  return 0;
+}
+int oracle_dielectric_potential(void *h, const kapsel_time *t, double tol, int maxit, int solver, int *iters,
+                                double *relres) {
+  NEED_EL
+  double e0[3];
+  *iters = O->dielectric_potential(*t, tol, maxit, solver, relres, e0);
+  return 0;
+}
+int oracle_icep_electrolyte_step(void *h, kapsel_time *t, double tol, int maxit, int solver, int *iters,
+                                 double *relres) {
+  NEED_EL
+  *iters = O->icep_electrolyte_step(*t, tol, maxit, solver, relres);
+  return 0;
+}
 #undef NEED_EL
 #undef O
 

@@ -61,7 +61,9 @@ def _load():
             "oracle_make_phi_qq": [i, i], "oracle_make_surface_normal": [i], "oracle_charge_field": [i],
             "oracle_nssolute_euler": [tp], "oracle_rescale_solute": [dp], "oracle_coulomb_force": [tp],
             "oracle_electrolyte_step": [tp], "oracle_free_energy_pb": [tp, dp],
-            "oracle_init_rho_ion_pb": [tp, i, ip, dp],
+            "oracle_init_rho_ion_pb": [tp, i, ip, dp], "oracle_init_rho_ion_dielectric": [], "oracle_dielectric_grad": [tp],
+            "oracle_dielectric_potential": [tp, ctypes.c_double, i, i, ip, dp],
+            "oracle_icep_electrolyte_step": [tp, ctypes.c_double, i, i, ip, dp],
         }
         for name, args in protos.items():
             fn = getattr(lib, name)
@@ -306,6 +308,26 @@ class Oracle:
 
     def electrolyte_step(self, t):
         self._call("electrolyte_step", ctypes.byref(t))
+
+    def init_rho_ion_dielectric(self):
+        self._call("init_rho_ion_dielectric")
+
+    def dielectric_grad(self, t):
+        self._call("dielectric_grad", ctypes.byref(t))
+
+    def dielectric_potential(self, t, tol, maxit, solver=0):
+        it = ctypes.c_int(0)
+        rr = ctypes.c_double(0.0)
+        self._call("dielectric_potential", ctypes.byref(t), tol, int(maxit), int(solver), ctypes.byref(it),
+                   ctypes.byref(rr))
+        return it.value, rr.value
+
+    def icep_electrolyte_step(self, t, tol, maxit, solver=0):
+        it = ctypes.c_int(0)
+        rr = ctypes.c_double(0.0)
+        self._call("icep_electrolyte_step", ctypes.byref(t), tol, int(maxit), int(solver), ctypes.byref(it),
+                   ctypes.byref(rr))
+        return it.value, rr.value
 
     def free_energy_pb(self, t):
         fe = np.zeros(3)

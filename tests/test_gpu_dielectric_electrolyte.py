@@ -1,0 +1,196 @@
+"""GPU parity of the electrolyte in the variable permittivity (Electrolyte
+with Dielectric: Nernst-Planck ions driven by the potential of the
+dielectric Poisson solve, Maxwell force) against the CPU oracle.
+
+Tolerances (relative to the field's max magnitude):
+  FP64 pieces: the Nernst-Planck step (CONC0/CONC1, ZETA0/ZETA1) from a given
+        potential 1e-12; permittivity and free charge 1e-12; the potential of
+        the same preconditioned CG stopped at 1e-12, 1e-9 after removing the
+        null-space constant
+  FP64 steps: fields, particles and the potential 1e-9 after 3 steps (the
+        solves stop at 1e-12, not at bit-identical iterates); the ion
+        amounts (k = 0 modes) are invariants of the step, to 1e-13 of
+        themselves; Count_solute_each's totals 1e-12
+  FP32 against FP64 after 5 steps: concentrations, particle velocities and
+        the fluid velocity 1e-4 (bounds written in the test)
+"""
+import numpy as np
+import pytest
+
+from icep_kapsel_amd import abi, workloads
+from tests import spectral as sp
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu(p):
+    from icep_kapsel_amd.gpu import KapselGPU
+    return KapselGPU(p)
+
+
+def oracle(p):
+    from oracle.oracle import Oracle
+    return Oracle(p)
+
+
+def rel(a, b):
+    s = np.abs(b).max()
+    return float(np.abs(a - b).max() / (s if s > 0 else 1.0))
+
+
+def _case(n, npart, q, seed=23, prec=abi.KAPSEL_FP64, e_ext=(0.05, 0.0, 0.02), ac=0):
+    """npart spheres a=6 (eps_particle 0.1 in eps_fluid 1); without surface
+    charge the Debye length is 0.5 RADIUS, with it 4 grid units."""
+    p = abi.make_params(n, n_particles=npart, radius=6.0, xi=2.0, eps_fluid=1.0, eps_particle=0.1,
+                        dielectric_cst=1.0, precision=prec)
+    pos = workloads.random_positions(workloads.Xoshiro256(seed), npart, (n, n, n), 12.0)
+    el = abi.make_electrolyte(surface_charge=q, debye_length=0.5 if q == 0.0 else 4.0, e_ext=e_ext, ac=ac,
+                              angular_frequency=0.7, dielectric=1)
+    el.onsager_coeff[:] = [7.24, 5.0]
+    return p, workloads.particle_state(pos), el
+
+
+def _band(n, rng, base, amp, kmax=5):
+    f = rng.standard_normal((n, n, n))
+    f = np.fft.irfftn(np.fft.rfftn(f) * sp.band_mask((n, n, n), kmax), s=(n, n, n), axes=(0, 1, 2))
+    return base + amp * f / np.abs(f).max()
+
+
+@pytest.mark.parametrize("q", [0.0, 40.0])
+def test_dielectric_electrolyte_pieces_match_oracle(q):
+    n, npart = 64, 6
+    print(f"case dielectric electrolyte pieces {n}^3 q={q}", flush=True)
+    p, st, el = _case(n, npart, q)
+    k, o = gpu(p), oracle(p)
+    rng = np.random.default_rng(31)
+    conc = [sp.from_complex(np.fft.rfftn(_band(n, rng, 0.05, 0.02))),
+            sp.from_complex(np.fft.rfftn(_band(n, rng, 0.04, 0.02)))]
+    u = sp.random_solenoidal((n, n, n), rng, kmax=5, amp=0.01)
+    uk = [sp.from_complex(np.fft.rfftn(u[d])) for d in range(3)]
+    psi = sp.pad_real(_band(n, rng, 0.0, 0.3))
+    for x in (k, o):
+        x.set_particles(st)
+        x.set_electrolyte(el)
+        for s in (0, 1):
+            x.set_field(f"CONC{s}", conc[s])
+        for d in range(3):
+            x.set_field(f"U{d}", uk[d])
+        x.u_k2zeta_k()
+        x.set_field("POTENTIAL", psi)
+    t = abi.make_time(0.01)
+    for x in (k, o):
+        x.nssolute_euler(t)
+    worst = {f: rel(k.get_field(f), o.get_field(f)) for f in ("CONC0", "CONC1", "ZETA0", "ZETA1")}
+    itk, rrk = k.dielectric_potential(t, 1e-12, 5000, 1)
+    ito, rro = o.dielectric_potential(t, 1e-12, 5000, 1)
+    print(f"potential: gpu {itk} it {rrk:.1e}, oracle {ito} it {rro:.1e}", flush=True)
+    assert rrk < 1e-12
+    for f in ("EPSILON", "RHO_FREE"):
+        worst[f] = rel(k.get_field(f)[:, :, :n], o.get_field(f)[:, :, :n])
+    xk, xo = k.get_field("POTENTIAL")[:, :, :n], o.get_field("POTENTIAL")[:, :, :n]
+    print({f: f"{v:.1e}" for f, v in worst.items()}, flush=True)
+    assert max(worst.values()) < 1e-12, worst
+    assert rel(xk - xk.mean(), xo - xo.mean()) < 1e-9
+    # the free charge has a nonzero total here (ions inside the particles)
+    assert abs(o.get_field("RHO_FREE")[:, :, :n].sum()) > 0
+
+
+@pytest.mark.parametrize("q,ac", [(0.0, 0), (40.0, 1)])
+def test_dielectric_electrolyte_step_fp64_matches_oracle(q, ac):
+    n, npart = 64, 6
+    print(f"case dielectric electrolyte step fp64 {n}^3 q={q} ac={ac}", flush=True)
+    p, st, el = _case(n, npart, q, ac=ac)
+    k, o = gpu(p), oracle(p)
+    for x in (k, o):
+        x.set_particles(st)
+        x.set_electrolyte(el)
+        x.init_rho_ion_dielectric()
+        x.init_rigidity()
+    t0 = abi.make_time(0.01)
+    assert k.dielectric_potential(t0, 1e-12, 5000, 1)[1] < 1e-12
+    o.dielectric_potential(t0, 1e-12, 5000, 1)
+    amounts = [sp.to_complex(o.get_field(f"CONC{s}"))[0, 0, 0].real for s in (0, 1)]
+    assert rel(k.get_total_solute(), o.get_total_solute()) < 1e-12  # sums of 64^3 terms, other order
+    dt = workloads.electrolyte_auto_dt(p, el)
+    tk, to = abi.make_time(dt), abi.make_time(dt)
+    for step in range(3):
+        itk, rrk = k.icep_electrolyte_step(tk, 1e-12, 5000, 1)
+        ito, rro = o.icep_electrolyte_step(to, 1e-12, 5000, 1)
+        print(f"step {step}: gpu {itk} it {rrk:.1e}, oracle {ito} it {rro:.1e}", flush=True)
+        assert rrk < 1e-12
+    assert tk.ts == to.ts == 3
+    worst = {}
+    for f in ("ZETA0", "ZETA1", "CONC0", "CONC1"):
+        worst[f] = rel(k.get_field(f), o.get_field(f))
+    for f in ("EPSILON", "RHO_FREE", "F0", "F1", "F2", "PHI_SUM"):
+        worst[f] = rel(k.get_field(f)[:, :, :n], o.get_field(f)[:, :, :n])
+    xk, xo = k.get_field("POTENTIAL")[:, :, :n], o.get_field("POTENTIAL")[:, :, :n]
+    worst["POTENTIAL"] = rel(xk - xk.mean(), xo - xo.mean())
+    pk, po = k.get_particles(), o.get_particles()
+    for f in ("X", "V", "OMEGA"):
+        i = abi.PFIELD[f]
+        worst[f] = rel(pk[i], po[i])
+    print({f: f"{v:.1e}" for f, v in worst.items()}, flush=True)
+    assert max(worst.values()) < 1e-9, worst
+    for s in (0, 1):
+        a = sp.to_complex(k.get_field(f"CONC{s}"))[0, 0, 0].real
+        assert abs(a - amounts[s]) < 1e-13 * abs(amounts[s])
+    assert np.abs(po[abi.PFIELD["V"]]).max() > 0  # the field moves the spheres
+
+
+def test_dielectric_electrolyte_fp32_against_fp64():
+    n, npart = 128, 24
+    print(f"case dielectric electrolyte fp32 vs fp64 {n}^3 {npart} spheres", flush=True)
+    ks = []
+    for prec in (abi.KAPSEL_FP64, abi.KAPSEL_FP32):
+        p, st, el = _case(n, npart, 0.0, seed=41, prec=prec, e_ext=(0.05, 0.0, 0.0))
+        k = gpu(p)
+        k.set_particles(st)
+        k.set_electrolyte(el)
+        k.init_rho_ion_dielectric()
+        k.init_rigidity()
+        t = abi.make_time(workloads.electrolyte_auto_dt(p, el))
+        k.dielectric_potential(t, 1e-10, 5000, 1)
+        for _ in range(5):
+            it, rr = k.icep_electrolyte_step(t, 1e-10, 5000, 1)
+        print(f"prec {prec}: last solve {it} iterations, relres {rr:.1e}", flush=True)
+        ks.append(k)
+    k64, k32 = ks
+    e = {}
+    for f in ("CONC0", "CONC1", "ZETA0", "ZETA1"):
+        e[f] = rel(k32.get_field(f), k64.get_field(f))
+    for f in ("U0", "U1", "U2"):
+        a, b = k32.get_field(f).copy(), k64.get_field(f).copy()
+        a[0, 0, 0] = b[0, 0, 0] = 0.0
+        e[f] = rel(a, b)
+    p64, p32 = k64.get_particles(), k32.get_particles()
+    for f in ("V", "OMEGA", "X"):
+        i = abi.PFIELD[f]
+        e[f] = rel(p32[i], p64[i])
+    print("fp32 vs fp64 after 5 steps", {f: f"{v:.1e}" for f, v in e.items()}, flush=True)
+    assert e["V"] < 1e-4 and max(e["U0"], e["U1"], e["U2"]) < 1e-4
+    assert max(e["CONC0"], e["CONC1"]) < 1e-4
+
+
+def test_dielectric_electrolyte_argument_checks():
+    """The Dielectric entry points refuse an electrolyte set without
+    Dielectric, and the plain step refuses one set with it."""
+    from icep_kapsel_amd.gpu import KapselError
+    n = 32
+    p, st, el = _case(n, 2, 40.0)
+    k = gpu(p)
+    k.set_particles(st)
+    el.dielectric = 0
+    k.set_electrolyte(el)
+    t = abi.make_time(0.01)
+    with pytest.raises(KapselError):
+        k.icep_electrolyte_step(t, 1e-10, 100)
+    with pytest.raises(KapselError):
+        k.init_rho_ion_dielectric()
+    el.dielectric = 1
+    k.set_electrolyte(el)
+    with pytest.raises(KapselError):
+        k.electrolyte_step(t)
+    with pytest.raises(KapselError):
+        k.free_energy_pb(t)
+    assert t.ts == 0

@@ -38,11 +38,12 @@ def _oracle(p, el):
 
 
 def _setup(n=32, npart=3, a=4.0, xi=2.0, q=40.0, seed=11, valency=(1.0, -1.0), e_ext=(0.0, 0.0, 0.0),
-           debye=4.0, **kw):
+           debye=4.0, dielectric=0, **kw):
     rng = workloads.Xoshiro256(seed)
     p = abi.make_params(n, n_particles=npart, radius=a, xi=xi, **kw)
     pos = workloads.random_positions(rng, npart, (n, n, n), 2.0 * a)
-    el = abi.make_electrolyte(surface_charge=q, valency=valency, debye_length=debye, e_ext=e_ext)
+    el = abi.make_electrolyte(surface_charge=q, valency=valency, debye_length=debye, e_ext=e_ext,
+                              dielectric=dielectric)
     return p, pos, el
 
 
@@ -430,3 +431,61 @@ def test_pb_one_particle_fixed_point():
     psi_e = -np.log(c[0] / c[1]) / 2.0  # c+/c- = (rp/rn) nu^2 exp(-2 psi_e): psi_e up to a constant
     d = (psi - psi.mean()) - (psi_e - psi_e.mean())
     assert np.abs(d).max() < 0.05 * np.abs(psi - psi.mean()).max()
+
+
+def test_dielectric_grad_closed_form():
+    """Dielectric branch of Rhs_NS_Nernst_Planck: the potential's gradient at
+    the staggered faces, shifted by the applied field, averaged back onto the
+    cells.  For psi = cos(k x): -k sin(k x) cos(k dx/2) - E_x."""
+    n, m, ex = 32, 3, 0.07
+    p, pos, el = _setup(n=n, e_ext=(ex, 0.0, 0.0), dielectric=1)
+    o = _oracle(p, el)
+    o.set_particles(workloads.particle_state(pos))
+    k = 2 * np.pi * m / n
+    x = np.arange(n, dtype=float)
+    psi = np.broadcast_to(np.cos(k * x)[:, None, None], (n, n, n))
+    o.set_field("POTENTIAL", sp.pad_real(psi))
+    o.dielectric_grad(abi.make_time(0.01))
+    gx = _real(o, "F0", n)
+    expect = (-k * np.sin(k * x) * np.cos(k / 2) - ex)[:, None, None]
+    assert np.abs(gx - expect).max() < 1e-13
+    assert np.abs(_real(o, "F1", n)).max() < 1e-14 and np.abs(_real(o, "F2", n)).max() < 1e-14
+
+
+def test_dielectric_ions_init_potential_and_rest_state():
+    """Init_rho_ion_dielectric without surface charge: bulk density with
+    kappa^2 = e^2 sum z^2 c / (kBT Dielectric_cst eps_f) = 1 / (lambda_D a)^2,
+    a neutral box, so no potential and no flux: with no field the state stays
+    at rest.  With a field, the potential of the ions in the permittivity is
+    the solve's answer, and the ion amounts (k = 0 modes) are exact invariants
+    of the step."""
+    n, debye, epsf = 32, 0.5, 2.0
+    p, pos, el = _setup(n=n, q=0.0, dielectric=1, debye=debye, eps_fluid=epsf, eps_particle=0.2)
+    o = _oracle(p, el)
+    o.set_particles(workloads.particle_state(pos))
+    o.init_rho_ion_dielectric()
+    c = [sp.to_complex(o.get_field(f"CONC{s}"))[0, 0, 0].real / n ** 3 for s in (0, 1)]
+    kappa2 = (c[0] + c[1]) / (el.kbt * el.dielectric_cst * epsf)
+    assert abs(kappa2 * (debye * p.radius) ** 2 - 1.0) < 1e-13
+    t = abi.make_time(0.01)
+    it, rr = o.dielectric_potential(t, 1e-10, 2000)
+    assert np.abs(o.get_field("POTENTIAL")).max() < 1e-12  # neutral everywhere: zero charge
+    o.init_rigidity()
+    o.icep_electrolyte_step(t, 1e-10, 2000)
+    assert np.abs(o.get_particles()[abi.PFIELD["V"]]).max() < 1e-14
+    # with a field: ions move, amounts stay
+    p2, pos2, el2 = _setup(n=n, q=0.0, dielectric=1, debye=debye, eps_fluid=epsf, eps_particle=0.2,
+                           e_ext=(0.05, 0.0, 0.0))
+    o2 = _oracle(p2, el2)
+    o2.set_particles(workloads.particle_state(pos2))
+    o2.init_rho_ion_dielectric()
+    o2.init_rigidity()
+    it, rr = o2.dielectric_potential(t, 1e-10, 2000)
+    assert rr < 1e-10
+    c0 = [sp.to_complex(o2.get_field(f"CONC{s}"))[0, 0, 0] for s in (0, 1)]
+    for _ in range(3):
+        it, rr = o2.icep_electrolyte_step(t, 1e-10, 2000)
+        assert rr < 1e-10
+    for s in (0, 1):
+        assert abs(sp.to_complex(o2.get_field(f"CONC{s}"))[0, 0, 0] - c0[s]) < 1e-13 * abs(c0[s])
+    assert np.abs(o2.get_field("ZETA0")).max() > 0  # the induced charge drives a flow
