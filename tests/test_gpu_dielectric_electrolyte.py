@@ -38,10 +38,11 @@ def rel(a, b):
     return float(np.abs(a - b).max() / (s if s > 0 else 1.0))
 
 
-def _case(n, npart, q, seed=23, prec=abi.KAPSEL_FP64, e_ext=(0.05, 0.0, 0.02), ac=0):
-    """npart spheres a=6 (eps_particle 0.1 in eps_fluid 1); without surface
+def _case(n, npart, q, seed=23, prec=abi.KAPSEL_FP64, e_ext=(0.05, 0.0, 0.02), ac=0, eps_fluid=1.0):
+    """npart spheres a=6 (eps_particle = eps_fluid / 10); without surface
     charge the Debye length is 0.5 RADIUS, with it 4 grid units."""
-    p = abi.make_params(n, n_particles=npart, radius=6.0, xi=2.0, eps_fluid=1.0, eps_particle=0.1,
+    p = abi.make_params(n, n_particles=npart, radius=6.0, xi=2.0, eps_fluid=eps_fluid,
+                        eps_particle=0.1 * eps_fluid,
                         dielectric_cst=1.0, precision=prec)
     pos = workloads.random_positions(workloads.Xoshiro256(seed), npart, (n, n, n), 12.0)
     el = abi.make_electrolyte(surface_charge=q, debye_length=0.5 if q == 0.0 else 4.0, e_ext=e_ext, ac=ac,
@@ -95,11 +96,11 @@ def test_dielectric_electrolyte_pieces_match_oracle(q):
     assert abs(o.get_field("RHO_FREE")[:, :, :n].sum()) > 0
 
 
-@pytest.mark.parametrize("q,ac", [(0.0, 0), (40.0, 1)])
-def test_dielectric_electrolyte_step_fp64_matches_oracle(q, ac):
+@pytest.mark.parametrize("q,ac,eps_fluid", [(0.0, 0, 1.0), (40.0, 1, 2.0)])
+def test_dielectric_electrolyte_step_fp64_matches_oracle(q, ac, eps_fluid):
     n, npart = 64, 6
-    print(f"case dielectric electrolyte step fp64 {n}^3 q={q} ac={ac}", flush=True)
-    p, st, el = _case(n, npart, q, ac=ac)
+    print(f"case dielectric electrolyte step fp64 {n}^3 q={q} ac={ac} eps_fluid={eps_fluid}", flush=True)
+    p, st, el = _case(n, npart, q, ac=ac, eps_fluid=eps_fluid)
     k, o = gpu(p), oracle(p)
     for x in (k, o):
         x.set_particles(st)

@@ -20,6 +20,13 @@ nssolute_euler     -> numpy restatement of the Nernst-Planck equation in
                       on the kept modes; a shear wave decays by 1 - nu k^2 dt
 rescale_solute     -> the (1 - phi)-weighted amounts equal the totals after it
 electrolyte_step   -> electroneutrality and the totals hold over steps
+Dielectric branch  -> the gradient of psi = cos(k x) on the faces averaged onto
+                      the cells is -k sin(k x) cos(k dx / 2) - E_x; the bulk
+                      densities give back the Debye length with eps_fluid
+                      (with and without surface charge); the free charge is
+                      sum z e c (1 - phi_sin) over the brute-force profile;
+                      the solve meets A psi = b + mean(rho); a neutral box
+                      stays at rest; the ion amounts are step invariants
 """
 import numpy as np
 import pytest
@@ -489,3 +496,60 @@ def test_dielectric_ions_init_potential_and_rest_state():
     for s in (0, 1):
         assert abs(sp.to_complex(o2.get_field(f"CONC{s}"))[0, 0, 0] - c0[s]) < 1e-13 * abs(c0[s])
     assert np.abs(o2.get_field("ZETA0")).max() > 0  # the induced charge drives a flow
+
+
+def test_dielectric_free_charge_and_neutralised_solve():
+    """Conc_k2charge_field_no_surfase_charge: rho = sum_n z_n e c_n (1 - phi_sin)
+    with phi_sin the brute-force compact-sine profile sum; ions spread into
+    the particles make its total nonzero, which the periodic problem cannot
+    carry, and the solve meets A psi = b + mean(rho) (the uniform background
+    that cancels it) to its tolerance."""
+    n, a, xi = 32, 4.0, 2.0
+    p, pos, el = _setup(n=n, q=0.0, dielectric=1, debye=0.5, eps_fluid=2.0, eps_particle=0.2,
+                        e_ext=(0.03, 0.01, 0.0))
+    o = _oracle(p, el)
+    o.set_particles(workloads.particle_state(pos))
+    rng = np.random.default_rng(7)
+    c = []
+    for base in (0.05, 0.045):
+        f = rng.standard_normal((n, n, n))
+        f = np.fft.irfftn(np.fft.rfftn(f) * sp.band_mask((n, n, n), 4), s=(n, n, n), axes=(0, 1, 2))
+        c.append(base + 0.02 * f / np.abs(f).max())
+    for s in (0, 1):
+        o.set_field(f"CONC{s}", sp.from_complex(np.fft.rfftn(c[s])))
+    t = abi.make_time(0.01)
+    it, rr = o.dielectric_potential(t, 1e-12, 4000, 1)
+    assert rr < 1e-12
+    phi_sin = np.zeros((n, n, n))
+    for xp in pos:
+        r = grid_vectors(n, xp)
+        phi_sin += _phi_sin(np.sqrt((r * r).sum(-1)), a, xi)
+    rho = (el.valency[0] * c[0] + el.valency[1] * c[1]) * el.elementary_charge * (1.0 - phi_sin)
+    got = o.get_field("RHO_FREE")[:, :, :n]
+    assert np.abs(got - rho).max() < 1e-13 * np.abs(rho).max()
+    mean = rho.mean()
+    psi = o.get_field("POTENTIAL")[:, :, :n]
+    o.poisson_rhs([0.03, 0.01, 0.0])
+    b = o.get_field("RHS")[:, :, :n]
+    ax = o.ax_unpadded(np.ascontiguousarray(psi))
+    assert abs(mean) > 1e-4 * np.abs(rho).max()
+    assert np.linalg.norm(ax - (b + mean)) < 1e-11 * np.linalg.norm(b + mean)
+    assert abs(ax.mean()) < 1e-13 * np.abs(ax).max()  # the operator's range has no mean
+
+
+def test_dielectric_bulk_density_with_surface_charge():
+    """Init_rho_ion under Dielectric: the bulk salt density carries eps_fluid,
+    kappa^2 = e^2 sum z^2 c_bulk / (kBT Dielectric_cst eps_fluid) = 1 / lambda_D^2,
+    and the counter-ions of the surface charge are added uniformly."""
+    n, debye, epsf, q = 32, 4.0, 3.0, 40.0
+    p, pos, el = _setup(n=n, q=q, dielectric=1, debye=debye, eps_fluid=epsf, eps_particle=0.2)
+    o = _oracle(p, el)
+    o.set_particles(workloads.particle_state(pos))
+    o.init_rho_ion_dielectric()
+    c = [sp.to_complex(o.get_field(f"CONC{s}"))[0, 0, 0].real / n ** 3 for s in (0, 1)]
+    # q > 0: the negative species holds the counter-ions, the positive one is the bulk alone
+    assert abs(c[0] * 2.0 * debye ** 2 / (el.kbt * el.dielectric_cst * epsf) - 1.0) < 1e-13
+    o.make_phi_particle("WORK0")
+    vol = (1.0 - o.get_field("WORK0")[:, :, :n]).sum()
+    counter = q * p.n_particles / abs(el.valency[1]) / vol
+    assert abs((c[1] - c[0]) / counter - 1.0) < 1e-12
