@@ -483,6 +483,33 @@ __global__ __launch_bounds__(TPB) void kres_kernel(const T *__restrict__ w0, con
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
+// Build_rhs (operate_dielectric.cxx:524-540) as a half spectrum.  The face
+// average E0_d (eps_i + eps_{i+e_d}) / 2 transforms to E0_d eps_k
+// (1 + e^{i k_d dx}) / 2, and the shifted divergence multiplies it by
+// i k_d e^{-i k_d dx/2}: b_k = i eps_k sum_d k_d E0_d cos(k_d dx / 2) - rho_k,
+// with the background bg (when set) added to the k = 0 mode as nr bg.
+// Partial sums of ||b||^2 (Parseval weights) per block.
+__global__ __launch_bounds__(TPB) void rhs_k_kernel(const double *__restrict__ epsk, const double *__restrict__ rhok,
+                                                    double *__restrict__ b, const V2<double> *__restrict__ shx,
+                                                    const V2<double> *__restrict__ shy,
+                                                    const V2<double> *__restrict__ shz, double e0x, double e0y,
+                                                    double e0z, const double *__restrict__ bg,
+                                                    double *__restrict__ partial, Grid g) {
+  double s = 0.0;
+  for (size_t c = (size_t)blockIdx.x * TPB + threadIdx.x; c < g.nc; c += (size_t)gridDim.x * TPB) {
+    const KIdx k = kidx(g, c);
+    const double f = k.kx * g.wave_x * e0x * shx[k.i].re + k.ky * g.wave_y * e0y * shy[k.j].re +
+                     k.kz * g.wave_z * e0z * shz[k.kz].re;
+    const V2<double> e = ld2(epsk + 2 * c), q = ld2(rhok + 2 * c);
+    V2<double> v{-f * e.im - q.re, f * e.re - q.im};
+    if (c == 0 && bg) v.re += (double)g.nr * bg[0];
+    st2(b + 2 * c, v);
+    s += kweight(g, k.kz) * (v.re * v.re + v.im * v.im);
+  }
+  s = block_sum(s);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
 // over every entry of a field (a half spectrum here): x <- 2 x - prev, prev <- old x
 __global__ void extrapolate_all_kernel(double *__restrict__ x, double *__restrict__ prev, size_t n) {
   size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -638,6 +665,13 @@ double read_scal(kapsel_ctx *ctx, int slot) {
   KAPSEL_HIP(hipMemcpyAsync(&v, ctx->d_scal + slot, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   KAPSEL_HIP(hipStreamSynchronize(ctx->stream));
   return v;
+}
+
+// out[lo..hi] = scalars lo..hi, one copy and one wait
+void read_scals(kapsel_ctx *ctx, double *out, int lo, int hi) {
+  KAPSEL_HIP(hipMemcpyAsync(out + lo, ctx->d_scal + lo, sizeof(double) * (hi - lo + 1), hipMemcpyDeviceToHost,
+                            ctx->stream));
+  KAPSEL_HIP(hipStreamSynchronize(ctx->stream));
 }
 
 // mean of eps over the real points (preconditioner scale)
@@ -849,10 +883,11 @@ void kcg_enqueue(kapsel_ctx *ctx, T *x, const T *eps, int precond, T *const *w, 
 // half spectra (x written, b read); w = {r, p, ap, z} k-space work vectors,
 // s = {g0, g1, g2} flux scratch; sc[S_INVEPS] holds 1 / eps_mean.  Stops
 // when the real-space ||A x - b|| is below tol_abs; returns the iteration
-// count and *rr_out = that ||A x - b||^2.
+// count and *rr_out = that ||A x - b||^2.  rr_known >= 0: ||b||^2 as the
+// caller computed it (x starts at zero), so the start needs no read-back.
 template <typename T>
 int kcg_t(kapsel_ctx *ctx, T *x, const T *b, const T *eps, double tol_abs, int maxit, int precond, T *const *w,
-          T *const *s, T *const *face, double *rr_out) {
+          T *const *s, T *const *face, double *rr_out, double rr_known = -1.0) {
   const Grid &g = ctx->g;
   T *r = w[0], *p = w[1], *z = precond ? w[3] : w[0];
   const double norm = 1.0 / (double)g.nr;  // Parseval factor of the k-space sums
@@ -864,7 +899,8 @@ int kcg_t(kapsel_ctx *ctx, T *x, const T *b, const T *eps, double tol_abs, int m
     hipLaunchKernelGGL(finish2_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->d_partial, ctx->d_scal, 0);
     KAPSEL_LAUNCH_CHECK();
   }
-  double rr = read_scal(ctx, S_RR) * norm;
+  // ||b||^2 when the caller knows it (to rounding), else read back
+  double rr = rr_known >= 0.0 ? rr_known : read_scal(ctx, S_RR) * norm;
   const double tol2 = tol_abs * tol_abs;
   ConvergenceWatch cw(ctx, norm, rr);
   cw.run(tol2, maxit, [&]() { kcg_enqueue<T>(ctx, x, eps, precond, w, s, face); });
@@ -948,13 +984,17 @@ struct RefineScope {
   ~RefineScope() { c->refine64 = false; }
 };
 
-// FP32 context: FP64 iterative refinement around FP32 (P)CG corrections
+// FP32 context: FP64 iterative refinement around FP32 (P)CG corrections.
+// The host waits twice per solve besides the correction's own pipelined
+// checks: once for ||b|| and the first residual, once for each later
+// residual; everything that does not depend on a decision is queued first.
 int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int precond, double *relres) {
   const Grid &g = ctx->g;
   RefineScope refine(ctx);
-  // FP64 shadow: eps and b (real), the potential and the residual as half
-  // spectra, the RHS spectrum, operator scratch, face permittivities
-  double *eps64 = ctx->p64[0], *xk = ctx->p64[1], *b64 = ctx->p64[2], *rk = ctx->p64[3];
+  // FP64 shadow: eps (real, then its spectrum), the potential and the
+  // residual as half spectra (rk holds rho until the RHS is built), the RHS
+  // spectrum, operator scratch, face permittivities
+  double *eps64 = ctx->p64[0], *xk = ctx->p64[1], *rk = ctx->p64[3];
   double *bk = ctx->p64[4];
   double *s64[4] = {ctx->p64[4], ctx->p64[5], ctx->p64[6], ctx->p64[7]};
   double *face64[3] = {ctx->p64[8], ctx->p64[9], ctx->p64[10]};
@@ -965,7 +1005,6 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
   float *d32 = (float *)ctx->pscratch[4], *eps32 = fld<float>(ctx, KAPSEL_EPSILON);
   const unsigned fb = blocks_for(g.f, TPB);
   // FP64 copies of the FP32 inputs: eps and rho enter the residual in FP64
-  // (rk holds rho until the RHS is built)
   hipLaunchKernelGGL((convert_real_kernel<double, float>), dim3(fb), dim3(TPB), 0, ctx->stream, eps64, eps32, g);
   hipLaunchKernelGGL((convert_real_kernel<double, float>), dim3(fb), dim3(TPB), 0, ctx->stream, rk,
                      fld<float>(ctx, KAPSEL_RHO_FREE), g);
@@ -988,22 +1027,7 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
     KAPSEL_HIP(hipMemcpyAsync(ctx->x64_prev, xk, g.f * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     ctx->prev_valid = true;
   }
-  build_rhs_t<double>(ctx, b64, rk, eps64, e0, s64);
-  dot_t<double, double>(ctx, b64, b64, S_BB);
-  const double bb = read_scal(ctx, S_BB);
-  if (bb == 0.0) {
-    KAPSEL_HIP(hipMemsetAsync(ctx->fields[KAPSEL_POTENTIAL], 0, g.f * sizeof(float), ctx->stream));
-    ctx->pot64_valid = false;
-    *relres = 0.0;
-    return 0;
-  }
-  {
-    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 2, sizeof(double)));
-    hipLaunchKernelGGL(copy_real_kernel<double>, dim3(fb), dim3(TPB), 0, ctx->stream, b64, bk, g);
-    KAPSEL_LAUNCH_CHECK();
-  }
-  void *b1[1] = {bk};
-  fft_r2c_t<double>(ctx, b1, 1);
+  // face factors of both operators, then eps64 and rho as spectra
   if (precond) inv_eps_device<float>(ctx, eps32, w32[3]);
   float *face32[3] = {(float *)ctx->pscratch[5], (float *)ctx->pscratch[6], (float *)ctx->pscratch[7]};
   hipLaunchKernelGGL(face_fields_kernel<float>, dim3(fb), dim3(TPB), 0, ctx->stream, eps32, face32[0], face32[1],
@@ -1011,12 +1035,23 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
   hipLaunchKernelGGL(face_fields_kernel<double>, dim3(fb), dim3(TPB), 0, ctx->stream, eps64, face64[0],
                      face64[1], face64[2], g);
   KAPSEL_LAUNCH_CHECK();
+  void *er[2] = {eps64, rk};
+  fft_r2c_t<double>(ctx, er, 2);
+  {
+    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 3, sizeof(double)));
+    hipLaunchKernelGGL(rhs_k_kernel, dim3(NPART), dim3(TPB), 0, ctx->stream, eps64, rk, bk, sh<double>(ctx, 0),
+                       sh<double>(ctx, 1), sh<double>(ctx, 2), e0[0], e0[1], e0[2], ctx->rho_bg, ctx->d_partial, g);
+    KAPSEL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->d_partial, NPART, ctx->d_scal, S_BB,
+                       0);
+    KAPSEL_LAUNCH_CHECK();
+  }
   void *g3[3] = {s64[1], s64[2], s64[3]};
   void *f3[3] = {face64[0], face64[1], face64[2]};
-  const double bnorm = sqrt(bb), norm = 1.0 / (double)g.nr;
+  const double norm = 1.0 / (double)g.nr;  // Parseval factor of the k-space sums
   float *b32 = fld<float>(ctx, KAPSEL_RHS);  // the correction's right-hand side spectrum meanwhile
   int total = 0;
-  double rel = 1.0;
+  double rel = 1.0, bnorm = 0.0;
   for (int outer = 0; outer < 30; outer++) {
     // rk = A xk - bk in k-space, ||r||^2 by Parseval
     {
@@ -1035,7 +1070,20 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
                          S_RR, 0);
       KAPSEL_LAUNCH_CHECK();
     }
-    const double rr = read_scal(ctx, S_RR) * norm;
+    double sc[S_RR + 1];
+    read_scals(ctx, sc, outer == 0 ? S_BB : S_RR, S_RR);
+    if (outer == 0) {
+      const double bb = sc[S_BB] * norm;
+      if (bb == 0.0) {
+        KAPSEL_HIP(hipMemsetAsync(ctx->fields[KAPSEL_POTENTIAL], 0, g.f * sizeof(float), ctx->stream));
+        KAPSEL_HIP(hipMemsetAsync(b32, 0, g.f * sizeof(float), ctx->stream));
+        ctx->pot64_valid = false;
+        *relres = 0.0;
+        return 0;
+      }
+      bnorm = sqrt(bb);
+    }
+    const double rr = sc[S_RR] * norm;
     rel = sqrt(rr) / bnorm;
     if (rel < tol || total >= maxit) break;
     // correction: A d = r in FP32 k-space, then x -= d.  Inner tolerance:
@@ -1046,21 +1094,21 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
     ctx->refine64 = false;  // the FP32 correction solve
     double rr_in = 0.0;
     const double tol_in = fmax(1e-4 * sqrt(rr), 0.5 * tol * bnorm);
-    int it = kcg_t<float>(ctx, d32, b32, eps32, tol_in, maxit - total, precond, w32, s32 + 1, face32, &rr_in);
+    int it = kcg_t<float>(ctx, d32, b32, eps32, tol_in, maxit - total, precond, w32, s32 + 1, face32, &rr_in, rr);
     total += it;
     ctx->refine64 = true;
     hipLaunchKernelGGL(sub_all_kernel, dim3(fb), dim3(TPB), 0, ctx->stream, xk, d32, g.f);
     KAPSEL_LAUNCH_CHECK();
     if (it == 0) break;
   }
-  // outputs: the potential (FP32, real) and the RHS b
+  // outputs: the potential (FP32, real) and the RHS b (FP32, real)
   float *pot = fld<float>(ctx, KAPSEL_POTENTIAL);
   hipLaunchKernelGGL((convert_real_kernel<float, double>), dim3(fb), dim3(TPB), 0, ctx->stream, pot, xk, g);
-  hipLaunchKernelGGL((convert_real_kernel<float, double>), dim3(fb), dim3(TPB), 0, ctx->stream, b32, b64, g);
+  hipLaunchKernelGGL((convert_real_kernel<float, double>), dim3(fb), dim3(TPB), 0, ctx->stream, b32, bk, g);
   KAPSEL_LAUNCH_CHECK();
   ctx->refine64 = false;
-  void *p1[1] = {pot};
-  fft_c2r_t<float>(ctx, p1, 1);
+  void *p2[2] = {pot, b32};
+  fft_c2r_t<float>(ctx, p2, 2);
   ctx->pot64_valid = true;
   *relres = rel;
   return total;
