@@ -258,9 +258,12 @@ __global__ __launch_bounds__(TPB) void finish_kernel(const double *__restrict__ 
 }
 
 // two sums at once: scal[S_RR] = <r,r> (partial[NPART..]); scal[S_RZ] =
-// <r,z>; mode 1 also beta = <r,z> / old scal[S_RZ] (mode 0: beta = 0)
+// <r,z>; mode 1 also beta = <r,z> / old scal[S_RZ] (mode 0: beta = 0).
+// With host (pinned host memory, mapped): mode 0 zeroes the iteration count
+// *cnt, mode 1 counts it up and stores <r,r> in host[count % 2], where the
+// pipelined convergence test reads it once the launch's event completes.
 __global__ __launch_bounds__(TPB) void finish2_kernel(const double *__restrict__ partial, double *__restrict__ scal,
-                                                      int mode) {
+                                                      int mode, double *host = nullptr, unsigned *cnt = nullptr) {
   double a = 0.0, b = 0.0;
   for (int i = threadIdx.x; i < NPART; i += TPB) {
     a += partial[i];
@@ -273,6 +276,16 @@ __global__ __launch_bounds__(TPB) void finish2_kernel(const double *__restrict__
     scal[S_BETA] = (mode == 1 && old != 0.0) ? a / old : 0.0;
     scal[S_RZ] = a;
     scal[S_RR] = b;
+    if (host) {
+      if (mode == 1) {
+        const unsigned k = *cnt + 1u;
+        *cnt = k;
+        host[k & 1u] = b;
+        __threadfence_system();
+      } else {
+        *cnt = 0u;
+      }
+    }
   }
 }
 
@@ -428,13 +441,27 @@ __global__ __launch_bounds__(TPB) void kcg_div_dot_kernel(const T *__restrict__ 
 }
 
 // alpha = <r,z> / <p, S p> with S p = -A p;  x += alpha p, r += alpha ap,
-// z = M^{-1} r; partial[b] = <r, z>, partial[NPART + b] = <r, r>
+// z = M^{-1} r; partial[b] = <r, z>, partial[NPART + b] = <r, r>.  With
+// npap > 0, <p, A p> is the sum of the npap partials in pap_part, which every
+// block forms itself in the same order (no separate reduction launch).
 template <typename T>
 __global__ __launch_bounds__(TPB) void kcg_update_kernel(T *__restrict__ x, T *r, T *z, const T *__restrict__ p,
                                                          const T *__restrict__ ap, const double *__restrict__ sc,
-                                                         int precond, double *__restrict__ partial, Grid g) {
+                                                         int precond, double *__restrict__ partial,
+                                                         const double *__restrict__ pap_part, int npap, Grid g) {
   const double inv_eps = sc[S_INVEPS];
-  const double pap = -sc[S_PAP];
+  double pap;
+  if (npap > 0) {
+    __shared__ double pap_sh;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < npap; i += TPB) s += pap_part[i];
+    s = block_sum(s);
+    if (threadIdx.x == 0) pap_sh = s;
+    __syncthreads();
+    pap = -pap_sh;
+  } else {
+    pap = -sc[S_PAP];
+  }
   const T alpha = (T)(pap != 0.0 ? sc[S_RZ] / pap : 0.0);
   double s_rz = 0.0, s_rr = 0.0;
   for (size_t c = (size_t)blockIdx.x * TPB + threadIdx.x; c < g.nc; c += (size_t)gridDim.x * TPB) {
@@ -624,13 +651,17 @@ void build_rhs_t(kapsel_ctx *ctx, T *rhs, const T *rho, const T *eps, const doub
 
 void ensure_storage(kapsel_ctx *ctx) {
   if (!ctx->d_partial) {
-    KAPSEL_HIP(hipMalloc(&ctx->d_partial, sizeof(double) * 2 * NPART));
+    KAPSEL_HIP(hipMalloc(&ctx->d_partial, sizeof(double) * 3 * NPART));
+    KAPSEL_HIP(hipMalloc(&ctx->d_count, sizeof(unsigned) * 2));
+    KAPSEL_HIP(hipMemsetAsync(ctx->d_count, 0, sizeof(unsigned) * 2, ctx->stream));
     KAPSEL_HIP(hipMalloc(&ctx->d_scal, sizeof(double) * S_N));
     KAPSEL_HIP(hipMemsetAsync(ctx->d_scal, 0, sizeof(double) * S_N, ctx->stream));
     ctx->n_partial = NPART;
   }
   if (!ctx->h_slots) {
     KAPSEL_HIP(hipHostMalloc((void **)&ctx->h_slots, sizeof(double) * 2, hipHostMallocDefault));
+    ctx->h_slots[0] = ctx->h_slots[1] = 0.0;
+    KAPSEL_HIP(hipHostGetDevicePointer((void **)&ctx->h_slots_dev, ctx->h_slots, 0));
     for (hipEvent_t &e : ctx->slot_ev) KAPSEL_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
   if (!ctx->pscratch[0]) {
@@ -693,23 +724,28 @@ __global__ void fill_kernel(T *__restrict__ a, T v, size_t n) {
 // does not drain between iterations.  It stops queueing ahead when the
 // geometric trend of the last two values says the iteration in flight
 // meets the bound, so normally no iteration runs past it.
+// device_slots: the iteration's last kernel stores ||r||^2 into the slot
+// itself (finish2_kernel with host slots), else a copy is queued.
 struct ConvergenceWatch {
   kapsel_ctx *ctx;
   double scale;
   double rr, rr_prev = -1.0;
   int it = 0, known = 0;
-  ConvergenceWatch(kapsel_ctx *c, double s, double rr0) : ctx(c), scale(s), rr(rr0) {}
+  bool device_slots;
+  ConvergenceWatch(kapsel_ctx *c, double s, double rr0, bool dev = false)
+      : ctx(c), scale(s), rr(rr0), device_slots(dev) {}
   void queued() {
     it++;
-    KAPSEL_HIP(hipMemcpyAsync(ctx->h_slots + (it & 1), ctx->d_scal + S_RR, sizeof(double), hipMemcpyDeviceToHost,
-                              ctx->stream));
+    if (!device_slots)
+      KAPSEL_HIP(hipMemcpyAsync(ctx->h_slots + (it & 1), ctx->d_scal + S_RR, sizeof(double),
+                                hipMemcpyDeviceToHost, ctx->stream));
     KAPSEL_HIP(hipEventRecord(ctx->slot_ev[it & 1], ctx->stream));
   }
   void wait_one() {
     known++;
     KAPSEL_HIP(hipEventSynchronize(ctx->slot_ev[known & 1]));
     rr_prev = rr;
-    rr = ctx->h_slots[known & 1] * scale;
+    rr = static_cast<volatile double *>(ctx->h_slots)[known & 1] * scale;
   }
   // runs enqueue() until ||r||^2 < tol2 or maxit iterations
   template <typename F>
@@ -804,6 +840,8 @@ template <typename T>
 void kcg_iteration(kapsel_ctx *ctx, T *x, const T *eps, int precond, T *const *w, T *const *s,
                    T *const *face) {
   const Grid &g = ctx->g;
+  double *pap_part = ctx->d_partial + 2 * NPART;
+  int npap = 0;
   T *r = w[0], *p = w[1], *ap = w[2], *z = precond ? w[3] : w[0];
   void *s3[3] = {s[0], s[1], s[2]};
   void *f3[3] = {face[0], face[1], face[2]};
@@ -811,10 +849,8 @@ void kcg_iteration(kapsel_ctx *ctx, T *x, const T *eps, int precond, T *const *w
     // direction update and gradient in the first pass, divergence and
     // <p, Ap> in the last
     fft_face_mid_t<T>(ctx, s3, f3);
-    const int nb = fft_x_div_t<T>(ctx, s3, p, ap, ctx->d_partial);
-    hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->d_partial, nb, ctx->d_scal, S_PAP,
-                       0);
-    KAPSEL_LAUNCH_CHECK();
+    // <p, A p> partials beside the update's own (summed in the update)
+    npap = fft_x_div_t<T>(ctx, s3, p, ap, pap_part);
   } else {
     {
       LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 6, sizeof(T)));
@@ -836,9 +872,10 @@ void kcg_iteration(kapsel_ctx *ctx, T *x, const T *eps, int precond, T *const *w
   {
     LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, precond ? 7 : 6, sizeof(T)));
     hipLaunchKernelGGL(kcg_update_kernel<T>, dim3(NPART), dim3(TPB), 0, ctx->stream, x, r, z, p, ap, ctx->d_scal,
-                       precond, ctx->d_partial, g);
+                       precond, ctx->d_partial, (const double *)pap_part, npap, g);
     KAPSEL_LAUNCH_CHECK();
-    hipLaunchKernelGGL(finish2_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->d_partial, ctx->d_scal, 1);
+    hipLaunchKernelGGL(finish2_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->d_partial, ctx->d_scal, 1,
+                       ctx->h_slots_dev, ctx->d_count);
     KAPSEL_LAUNCH_CHECK();
   }
 }
@@ -896,13 +933,14 @@ int kcg_t(kapsel_ctx *ctx, T *x, const T *b, const T *eps, double tol_abs, int m
     hipLaunchKernelGGL(kcg_init_kernel<T>, dim3(NPART), dim3(TPB), 0, ctx->stream, b, x, r, z, p, ctx->d_scal,
                        precond, ctx->d_partial, g);
     KAPSEL_LAUNCH_CHECK();
-    hipLaunchKernelGGL(finish2_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->d_partial, ctx->d_scal, 0);
+    hipLaunchKernelGGL(finish2_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->d_partial, ctx->d_scal, 0,
+                       ctx->h_slots_dev, ctx->d_count);
     KAPSEL_LAUNCH_CHECK();
   }
   // ||b||^2 when the caller knows it (to rounding), else read back
   double rr = rr_known >= 0.0 ? rr_known : read_scal(ctx, S_RR) * norm;
   const double tol2 = tol_abs * tol_abs;
-  ConvergenceWatch cw(ctx, norm, rr);
+  ConvergenceWatch cw(ctx, norm, rr, true);
   cw.run(tol2, maxit, [&]() { kcg_enqueue<T>(ctx, x, eps, precond, w, s, face); });
   rr = cw.rr;
   const int it = cw.it;
@@ -1252,6 +1290,8 @@ void poisson_free(kapsel_ctx *ctx) {
   if (ctx->x64_prev) (void)hipFree(ctx->x64_prev), ctx->x64_prev = nullptr;
   if (ctx->d_partial) (void)hipFree(ctx->d_partial), ctx->d_partial = nullptr;
   if (ctx->h_slots) (void)hipHostFree(ctx->h_slots), ctx->h_slots = nullptr;
+  ctx->h_slots_dev = nullptr;
+  if (ctx->d_count) (void)hipFree(ctx->d_count), ctx->d_count = nullptr;
   if (ctx->kcg_exec) (void)hipGraphExecDestroy(ctx->kcg_exec), ctx->kcg_exec = nullptr;
   for (hipEvent_t &e : ctx->slot_ev)
     if (e) (void)hipEventDestroy(e), e = nullptr;
