@@ -362,7 +362,7 @@ static void icep_step(kapsel_ctx *ctx, kapsel_time &t, const double e0[3], doubl
   make_phi_particle_sum(ctx);
   dielectric_field(ctx);
   poisson_solve(ctx, e0, tol, maxit, precond, iters, relres);
-  maxwell_force(ctx, e0);
+  maxwell_force(ctx, e0, true);
   add_force_to_u(ctx, t.dt_fluid * ctx->irho);
   solenoidal_u(ctx, ctx->fields + KAPSEL_U0);
   calc_f_hydro(ctx, t);
@@ -431,7 +431,7 @@ static void icep_electrolyte_step(kapsel_ctx *ctx, kapsel_time &t, double tol, i
   const double s = ctx->el.ac ? sin(ctx->el.angular_frequency * t.time) : 1.0;
   const double e0[3] = {ctx->el.e_ext[0] * s, ctx->el.e_ext[1] * s, ctx->el.e_ext[2] * s};
   dielectric_potential(ctx, e0, tol, maxit, precond, iters, relres);  // :72-74
-  maxwell_force(ctx, e0);
+  maxwell_force(ctx, e0, true);
   add_force_to_u(ctx, t.dt_fluid * ctx->irho);  // :75-84
   solenoidal_u(ctx, ctx->fields + KAPSEL_U0);   // :85
   calc_f_hydro(ctx, t);                       // :99-101

@@ -184,6 +184,9 @@ struct kapsel_ctx {
   // x64_prev holds the solve before it (when prev_valid), so the next start
   // can be the linear extrapolation 2 x_n - x_{n-1} in time
   bool pot64_valid = false;
+  // p64[0] and p64[1] hold the spectra of EPSILON and POTENTIAL as the last
+  // FP32-context solve left them (cleared when a solve starts)
+  bool spec64_fresh = false;
   bool refine64 = false;  // inside the FP64 part of an FP32 solve: launches time as KAPSEL_K_REFINE64
   bool prev_valid = false;
   double *x64_prev = nullptr;
@@ -317,7 +320,9 @@ void poisson_solve(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, i
                    double *relres);
 void poisson_free(kapsel_ctx *ctx);
 double galerkin_scale_potential(kapsel_ctx *ctx, const double e0[3]);
-void maxwell_force(kapsel_ctx *ctx, const double e0[3]);
+// after_solve: called right after poisson_solve, whose FP64 spectra of eps
+// and the potential may then stand in for the transforms of the fields
+void maxwell_force(kapsel_ctx *ctx, const double e0[3], bool after_solve = false);
 void add_force_to_u(kapsel_ctx *ctx, double s);
 
 // learned warm start (pnet.hip)

@@ -126,6 +126,30 @@ __global__ void grad_shift_kernel(const T *__restrict__ xk, T *__restrict__ g0, 
   st2(g2 + 2 * c, cmul(V2<T>{-w2 * x.im, w2 * x.re}, shz[k.kz]));
 }
 
+// g_d = i k_d e^{i k_d dx/2} x_k for two FP64 half spectra a, b into six
+// FP32 spectra (the staggered gradients of eps and of the potential from the
+// spectra the FP64 refinement holds)
+__global__ void grad_shift2_f64_kernel(const double *__restrict__ a, const double *__restrict__ b,
+                                       float *__restrict__ a0, float *__restrict__ a1, float *__restrict__ a2,
+                                       float *__restrict__ b0, float *__restrict__ b1, float *__restrict__ b2,
+                                       const V2<double> *__restrict__ shx, const V2<double> *__restrict__ shy,
+                                       const V2<double> *__restrict__ shz, Grid g) {
+  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  KIdx k = kidx(g, c);
+  const double w[3] = {k.kx * g.wave_x, k.ky * g.wave_y, k.kz * g.wave_z};
+  const V2<double> s[3] = {shx[k.i], shy[k.j], shz[k.kz]};
+  const V2<double> x = ld2(a + 2 * c), y = ld2(b + 2 * c);
+  float *oa[3] = {a0, a1, a2}, *ob[3] = {b0, b1, b2};
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const V2<double> ga = cmul(V2<double>{-w[d] * x.im, w[d] * x.re}, s[d]);
+    const V2<double> gb = cmul(V2<double>{-w[d] * y.im, w[d] * y.re}, s[d]);
+    st2(oa[d] + 2 * c, V2<float>{(float)ga.re, (float)ga.im});
+    st2(ob[d] + 2 * c, V2<float>{(float)gb.re, (float)gb.im});
+  }
+}
+
 // g_d *= 0.5 (eps_i + eps_{i+e_d})   (operate_dielectric.cxx:492-508)
 template <typename T>
 __global__ void face_mul_kernel(T *__restrict__ g0, T *__restrict__ g1, T *__restrict__ g2,
@@ -1029,6 +1053,7 @@ struct RefineScope {
 int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int precond, double *relres) {
   const Grid &g = ctx->g;
   RefineScope refine(ctx);
+  ctx->spec64_fresh = false;
   // FP64 shadow: eps (real, then its spectrum), the potential and the
   // residual as half spectra (rk holds rho until the RHS is built), the RHS
   // spectrum, operator scratch, face permittivities
@@ -1148,6 +1173,7 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
   void *p2[2] = {pot, b32};
   fft_c2r_t<float>(ctx, p2, 2);
   ctx->pot64_valid = true;
+  ctx->spec64_fresh = true;
   *relres = rel;
   return total;
 }
@@ -1173,15 +1199,34 @@ void staggered_grad_t(kapsel_ctx *ctx, const T *a, T *s0, T *const *gout) {
   fft_c2r_t<T>(ctx, g3, 3);
 }
 
+// from_spectra (FP32 context, right after solve_fp32): the staggered
+// gradients come from the FP64 spectra of eps and of the potential the
+// refinement left in p64[0] and p64[1], which saves a copy and a forward
+// transform of each field
 template <typename T>
-void maxwell_t(kapsel_ctx *ctx, const double e0[3]) {
+void maxwell_t(kapsel_ctx *ctx, const double e0[3], bool from_spectra) {
   const Grid &g = ctx->g;
   T *s0 = (T *)ctx->pscratch[0];
   T *ge[3] = {(T *)ctx->pscratch[1], (T *)ctx->pscratch[2], (T *)ctx->pscratch[3]};
   T *gp[3] = {fld<T>(ctx, KAPSEL_WORK1), fld<T>(ctx, KAPSEL_WORK2), fld<T>(ctx, KAPSEL_WORK3)};
   T *e2 = fld<T>(ctx, KAPSEL_WORK0);
-  staggered_grad_t<T>(ctx, fld<T>(ctx, KAPSEL_EPSILON), s0, ge);
-  staggered_grad_t<T>(ctx, fld<T>(ctx, KAPSEL_POTENTIAL), s0, gp);
+  if constexpr (sizeof(T) == 4) {
+    if (from_spectra) {
+      {
+        LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, 2.0 * g.f * sizeof(double) + 6.0 * g.f * sizeof(float));
+        hipLaunchKernelGGL(grad_shift2_f64_kernel, dim3(blocks_for(g.nc, TPB)), dim3(TPB), 0, ctx->stream,
+                           ctx->p64[0], ctx->p64[1], ge[0], ge[1], ge[2], gp[0], gp[1], gp[2], sh<double>(ctx, 0),
+                           sh<double>(ctx, 1), sh<double>(ctx, 2), g);
+        KAPSEL_LAUNCH_CHECK();
+      }
+      fft_c2r_t<T>(ctx, (void *const *)ge, 3);
+      fft_c2r_t<T>(ctx, (void *const *)gp, 3);
+    }
+  }
+  if (!from_spectra || sizeof(T) == 8) {
+    staggered_grad_t<T>(ctx, fld<T>(ctx, KAPSEL_EPSILON), s0, ge);
+    staggered_grad_t<T>(ctx, fld<T>(ctx, KAPSEL_POTENTIAL), s0, gp);
+  }
   LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 15, sizeof(T)));
   hipLaunchKernelGGL(efield2_kernel<T>, dim3(blocks_for(g.nr, TPB)), dim3(TPB), 0, ctx->stream, gp[0], gp[1], gp[2],
                      e0[0], e0[1], e0[2], e2, g);
@@ -1202,10 +1247,10 @@ void add_force_t(kapsel_ctx *ctx, double s) {
 
 }  // namespace
 
-void maxwell_force(kapsel_ctx *ctx, const double e0[3]) {
+void maxwell_force(kapsel_ctx *ctx, const double e0[3], bool after_solve) {
   ensure_storage(ctx);
-  if (ctx->prec == KAPSEL_FP64) maxwell_t<double>(ctx, e0);
-  else maxwell_t<float>(ctx, e0);
+  if (ctx->prec == KAPSEL_FP64) maxwell_t<double>(ctx, e0, false);
+  else maxwell_t<float>(ctx, e0, after_solve && ctx->spec64_fresh);
 }
 
 void add_force_to_u(kapsel_ctx *ctx, double s) {
