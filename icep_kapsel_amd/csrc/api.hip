@@ -326,6 +326,25 @@ static void u2zeta_k(kapsel_ctx *ctx) {
   u_k2zeta_k(ctx);
 }
 
+// The end of an ICEP step in an FP32 context, from Solenoidal_u(u) on.
+// u = c2r(P u_k) and the particle force f = P(u_p - phi u) (f_particle.h)
+// enter U2zeta_k only as their sum, whose spectrum is P u_k + P f_k: the
+// first is kept when u is projected (keep3, three spare fields), the second
+// is never transformed back.  U holds that spectrum afterwards, F the
+// spectrum P f_k; the FP64 context keeps the reference's real-space order.
+static void solenoidal_u_keep(kapsel_ctx *ctx, void *const *keep3) {
+  fft_r2c(ctx, ctx->fields + KAPSEL_U0, 3);
+  solenoidal_uk(ctx, ctx->fields + KAPSEL_U0);
+  for (int d = 0; d < 3; d++) copy_field(ctx, keep3[d], ctx->fields[KAPSEL_U0 + d]);
+  fft_c2r(ctx, ctx->fields + KAPSEL_U0, 3);
+}
+static void f_sole_u2zeta_k(kapsel_ctx *ctx, void *const *keep3) {
+  make_f_nonsole(ctx);
+  fft_r2c(ctx, ctx->fields + KAPSEL_F0, 3);
+  solenoidal_uk(ctx, ctx->fields + KAPSEL_F0);
+  u_k2zeta_sum_k(ctx, keep3, ctx->fields + KAPSEL_F0);
+}
+
 static int fix_mask(const kapsel_ctx *ctx) {
   int m = 0;
   for (int d = 0; d < 3; d++)
@@ -364,13 +383,20 @@ static void icep_step(kapsel_ctx *ctx, kapsel_time &t, const double e0[3], doubl
   poisson_solve(ctx, e0, tol, maxit, precond, iters, relres);
   maxwell_force(ctx, e0, true);
   add_force_to_u(ctx, t.dt_fluid * ctx->irho);
-  solenoidal_u(ctx, ctx->fields + KAPSEL_U0);
+  void *keep3[3] = {ctx->pscratch[5], ctx->pscratch[6], ctx->pscratch[7]};
+  const bool fp32 = ctx->prec == KAPSEL_FP32;
+  if (fp32) solenoidal_u_keep(ctx, keep3);
+  else solenoidal_u(ctx, ctx->fields + KAPSEL_U0);
   calc_f_hydro(ctx, t);
   force(ctx);
   md_velocity_integrate(ctx, t);
   make_u_particle_sum(ctx);
-  f_particle_dt_sole_add(ctx);
-  u2zeta_k(ctx);
+  if (fp32) {
+    f_sole_u2zeta_k(ctx, keep3);
+  } else {
+    f_particle_dt_sole_add(ctx);
+    u2zeta_k(ctx);
+  }
   t.time += t.dt_fluid;
   t.ts += 1;
 }
@@ -433,13 +459,20 @@ static void icep_electrolyte_step(kapsel_ctx *ctx, kapsel_time &t, double tol, i
   dielectric_potential(ctx, e0, tol, maxit, precond, iters, relres);  // :72-74
   maxwell_force(ctx, e0, true);
   add_force_to_u(ctx, t.dt_fluid * ctx->irho);  // :75-84
-  solenoidal_u(ctx, ctx->fields + KAPSEL_U0);   // :85
+  void *keep3[3] = {ctx->pscratch[5], ctx->pscratch[6], ctx->pscratch[7]};
+  const bool fp32 = ctx->prec == KAPSEL_FP32;
+  if (fp32) solenoidal_u_keep(ctx, keep3);    // :85
+  else solenoidal_u(ctx, ctx->fields + KAPSEL_U0);
   calc_f_hydro(ctx, t);                       // :99-101
   force(ctx);                                 // :104-110
   md_velocity_integrate(ctx, t);
   make_u_particle_sum(ctx);                   // :166-167
-  f_particle_dt_sole_add(ctx);                // :168-169
-  u2zeta_k(ctx);                              // :184
+  if (fp32) {
+    f_sole_u2zeta_k(ctx, keep3);              // :168-169, 184
+  } else {
+    f_particle_dt_sole_add(ctx);              // :168-169
+    u2zeta_k(ctx);                            // :184
+  }
   t.time += t.dt_fluid;
   t.ts += 1;
 }

@@ -293,6 +293,8 @@ int fft_x_div_t(kapsel_ctx *ctx, void *const *g3, const T *p, T *ap, double *par
 void truncate(kapsel_ctx *ctx, void *const *fields, int nf);
 void zeta_k2u_k(kapsel_ctx *ctx, int fix_dc);
 void u_k2zeta_k(kapsel_ctx *ctx);
+// U_k2zeta_k of the spectrum a3 + b3, which is also stored in U0..U2
+void u_k2zeta_sum_k(kapsel_ctx *ctx, void *const *a3, void *const *b3);
 void advection_products(kapsel_ctx *ctx);
 void advection_combine_slaved(kapsel_ctx *ctx, double dt_fluid);
 void solenoidal_uk(kapsel_ctx *ctx, void *const *f3);

@@ -82,14 +82,26 @@ __global__ void zeta_k2u_k_kernel(T *__restrict__ z0, T *__restrict__ z1, T *__r
   st2(u2 + 2 * c, e);
 }
 
-// U_k2zeta_k (fft_wrapper.cxx:428-497)
+// U_k2zeta_k (fft_wrapper.cxx:428-497); with p, u is the sum of the two
+// spectra u + p, which is also stored in o
 template <typename T>
 __global__ void u_k2zeta_k_kernel(const T *__restrict__ u0, const T *__restrict__ u1, const T *__restrict__ u2,
-                                  T *__restrict__ z0, T *__restrict__ z1, double *__restrict__ dc, Grid g) {
+                                  T *__restrict__ z0, T *__restrict__ z1, double *__restrict__ dc, Grid g,
+                                  const T *__restrict__ p0 = nullptr, const T *__restrict__ p1 = nullptr,
+                                  const T *__restrict__ p2 = nullptr, T *__restrict__ o0 = nullptr,
+                                  T *__restrict__ o1 = nullptr, T *__restrict__ o2 = nullptr) {
   size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.nc) return;
   KIdx k = kidx(g, c);
   V2<T> a = ld2(u0 + 2 * c), b = ld2(u1 + 2 * c), e = ld2(u2 + 2 * c);
+  if (p0) {
+    a = a + ld2(p0 + 2 * c);
+    b = b + ld2(p1 + 2 * c);
+    e = e + ld2(p2 + 2 * c);
+    st2(o0 + 2 * c, a);
+    st2(o1 + 2 * c, b);
+    st2(o2 + 2 * c, e);
+  }
   if (c == 0) {
     dc[0] = (double)a.re;
     dc[1] = (double)b.re;
@@ -326,6 +338,19 @@ static void u_k2zeta_k_t(kapsel_ctx *ctx) {
   KAPSEL_LAUNCH_CHECK();
 }
 void u_k2zeta_k(kapsel_ctx *ctx) { KAPSEL_DISPATCH(ctx, u_k2zeta_k_t, ctx); }
+
+template <typename T>
+static void u_k2zeta_sum_k_t(kapsel_ctx *ctx, void *const *a3, void *const *b3) {
+  LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, 11.0 * ctx->g.f * sizeof(T));
+  hipLaunchKernelGGL(u_k2zeta_k_kernel<T>, dim3(blocks_for(ctx->g.nc, TPB)), dim3(TPB), 0, ctx->stream,
+                     (const T *)a3[0], (const T *)a3[1], (const T *)a3[2], F<T>(ctx, KAPSEL_ZETA0),
+                     F<T>(ctx, KAPSEL_ZETA1), ctx->d_uk_dc, ctx->g, (const T *)b3[0], (const T *)b3[1],
+                     (const T *)b3[2], F<T>(ctx, KAPSEL_U0), F<T>(ctx, KAPSEL_U1), F<T>(ctx, KAPSEL_U2));
+  KAPSEL_LAUNCH_CHECK();
+}
+void u_k2zeta_sum_k(kapsel_ctx *ctx, void *const *a3, void *const *b3) {
+  KAPSEL_DISPATCH(ctx, u_k2zeta_sum_k_t, ctx, a3, b3);
+}
 
 template <typename T>
 static void advection_products_t(kapsel_ctx *ctx) {
