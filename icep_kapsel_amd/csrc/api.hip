@@ -353,9 +353,19 @@ static int fix_mask(const kapsel_ctx *ctx) {
 }
 
 // Time_evolution_hydro, Navier-Stokes branch (sp_3d_ns.cxx:24-186)
+// (FP32 context: the velocity spectrum of Zeta_k2u is kept in WORK0..WORK2
+// and the step ends in k-space, as f_sole_u2zeta_k describes)
 static void spm_step(kapsel_ctx *ctx, kapsel_time &t) {
   ns_slaved_euler(ctx, t);                    // :27
-  zeta_k2u(ctx, fix_mask(ctx));               // :29-42
+  void *keep3[3] = {fld(ctx, KAPSEL_WORK0), fld(ctx, KAPSEL_WORK1), fld(ctx, KAPSEL_WORK2)};
+  const bool fp32 = ctx->prec == KAPSEL_FP32;
+  if (fp32) {                                 // :29-42
+    zeta_k2u_k(ctx, fix_mask(ctx));
+    for (int d = 0; d < 3; d++) copy_field(ctx, keep3[d], ctx->fields[KAPSEL_U0 + d]);
+    fft_c2r(ctx, ctx->fields + KAPSEL_U0, 3);
+  } else {
+    zeta_k2u(ctx, fix_mask(ctx));
+  }
   md_position(ctx, t);                        // :44-50
   make_phi_particle_sum(ctx);                 // :51-53
   calc_f_hydro(ctx, t);                       // :99-101
@@ -363,8 +373,12 @@ static void spm_step(kapsel_ctx *ctx, kapsel_time &t) {
   md_velocity_integrate(ctx, t);
   thermostat(ctx, t);                         // :155-157, with kBT > 0
   make_u_particle_sum(ctx);                   // :166-167
-  f_particle_dt_sole_add(ctx);                // :168-169
-  u2zeta_k(ctx);                              // :184
+  if (fp32) {
+    f_sole_u2zeta_k(ctx, keep3);              // :168-169, 184
+  } else {
+    f_particle_dt_sole_add(ctx);              // :168-169
+    u2zeta_k(ctx);                            // :184
+  }
   t.time += t.dt_fluid;                       // :525
   t.ts += 1;
 }
