@@ -280,7 +280,8 @@ void fft_face_t(kapsel_ctx *ctx, void *const *g3, void *const *face3);
 template <typename T>
 void fft_face_mid_t(kapsel_ctx *ctx, void *const *g3, void *const *face3);
 // the operator's first pass on a CG direction: p = z + sc[beta_slot] p, then
-// g_d = x-inverse of (i k_d e^{i k_d dx/2} p); false: NX has no fused pass
+// g_d = x-inverse of (i k_d e^{i k_d dx/2} p); beta_slot < 0: the gradient
+// of z, p untouched; false: NX has no fused pass
 template <typename T>
 bool fft_x_grad_t(kapsel_ctx *ctx, const T *z, T *p, const double *sc, int beta_slot, void *const *g3);
 // the operator's last pass: ap = staggered divergence of the x-forward of
@@ -288,6 +289,9 @@ bool fft_x_grad_t(kapsel_ctx *ctx, const T *z, T *p, const double *sc, int beta_
 // number of partials (0: NX has no fused pass)
 template <typename T>
 int fft_x_div_t(kapsel_ctx *ctx, void *const *g3, const T *p, T *ap, double *partial);
+// fft_x_div_t's residual form: r = (divergence) - b, partials of |r|^2
+template <typename T>
+int fft_x_res_t(kapsel_ctx *ctx, void *const *g3, const T *b, T *r, double *partial);
 
 // k-space and real-space pointwise (kspace.hip)
 void truncate(kapsel_ctx *ctx, void *const *fields, int nf);

@@ -765,14 +765,18 @@ __global__ __launch_bounds__(LPB *FS<L>::R2) void fft_x_grad(const T *z, T *p, c
   const bool valid = q < Q;
   const long long base = valid ? q : 0;
   const int j = (int)(base / h), kz = (int)(base % h);
-  const T beta = (T)sc[beta_slot];
+  const bool update = beta_slot >= 0;  // else the gradient of z itself, p untouched
+  const T beta = update ? (T)sc[beta_slot] : (T)0;
   V2<T> pv[R1];
 #pragma unroll
   for (int n1 = 0; n1 < R1; n1++) {
     const long long off = base + (long long)(R2 * n1 + n2) * plane;
     if (valid) {
-      pv[n1] = ld2(z + 2 * off) + scal(beta, ld2(p + 2 * off));
-      st2(p + 2 * off, pv[n1]);
+      pv[n1] = ld2(z + 2 * off);
+      if (update) {
+        pv[n1] = pv[n1] + scal(beta, ld2(p + 2 * off));
+        st2(p + 2 * off, pv[n1]);
+      }
     } else {
       pv[n1] = V2<T>{0, 0};
     }
@@ -814,7 +818,9 @@ __global__ __launch_bounds__(LPB *FS<L>::R2) void fft_x_grad(const T *z, T *p, c
 // w_d = FFT_x(g_d), ap = i sum_d k_d e^{-i k_d dx/2} w_d (stored);
 // partial[tile] = sum over the tile of w(kz) Re(conj(p) ap), w = 1 on the
 // kz = 0 and NZ/2 planes and 2 elsewhere (Parseval, without N^-3)
-template <typename T, int L, int LPB>
+// RES: the residual form -- p is b, ap receives r = (div) - b, and the
+// partials sum |r|^2 (the refinement's A x - b)
+template <typename T, int L, int LPB, bool RES = false>
 __global__ __launch_bounds__(LPB *FS<L>::R2) void fft_x_div(FieldPtrs<T> g3, const T *__restrict__ p,
                                                             T *__restrict__ ap, double *__restrict__ partial,
                                                             long long Q, long long plane, int h, int ny, int nz,
@@ -880,9 +886,15 @@ __global__ __launch_bounds__(LPB *FS<L>::R2) void fft_x_div(FieldPtrs<T> g3, con
       for (int k2 = 0; k2 < R2; k2++) {
         const long long off = base + (long long)(k1 + R1 * k2) * plane;
         const V2<T> av{-acc[c][k2].im, acc[c][k2].re};
-        st2(ap + 2 * off, av);
         const V2<T> pp = ld2(p + 2 * off);
-        s += wk * ((double)pp.re * av.re + (double)pp.im * av.im);
+        if constexpr (RES) {
+          const V2<T> rv = av - pp;
+          st2(ap + 2 * off, rv);
+          s += wk * ((double)rv.re * rv.re + (double)rv.im * rv.im);
+        } else {
+          st2(ap + 2 * off, av);
+          s += wk * ((double)pp.re * av.re + (double)pp.im * av.im);
+        }
       }
     }
   }
@@ -1190,7 +1202,7 @@ void launch_x_grad(kapsel_ctx *ctx, const T *z, T *p, const double *sc, int beta
   KAPSEL_LAUNCH_CHECK();
 }
 
-template <typename T, int L>
+template <typename T, int L, bool RES = false>
 int launch_x_div(kapsel_ctx *ctx, void *const *g3, const T *p, T *ap, double *partial) {
   constexpr int LPB = lines_per_block(sizeof(T));
   const Grid &g = ctx->g;
@@ -1198,7 +1210,7 @@ int launch_x_div(kapsel_ctx *ctx, void *const *g3, const T *p, T *ap, double *pa
   const long long Q = (long long)g.ny * g.h;
   const unsigned nb = (unsigned)((Q + LPB - 1) / LPB);
   LaunchTimer lt(ctx, KAPSEL_K_FFT_X_DIV, 5.0 * g.f * sizeof(T));
-  hipLaunchKernelGGL((fft_x_div<T, L, LPB>), dim3(nb), dim3(LPB * FS<L>::R2), 0, ctx->stream, pack<T>(g3, 3), p,
+  hipLaunchKernelGGL((fft_x_div<T, L, LPB, RES>), dim3(nb), dim3(LPB * FS<L>::R2), 0, ctx->stream, pack<T>(g3, 3), p,
                      ap, partial, Q, Q, g.h, g.ny, g.nz, g.wave_x, g.wave_y, g.wave_z, (const V2<T> *)tw.shx,
                      (const V2<T> *)tw.shy, (const V2<T> *)tw.shz, (const V2<T> *)tw.x);
   KAPSEL_LAUNCH_CHECK();
@@ -1223,7 +1235,17 @@ int fft_x_div_t(kapsel_ctx *ctx, void *const *g3, const T *p, T *ap, double *par
   }
 }
 
+template <typename T>
+int fft_x_res_t(kapsel_ctx *ctx, void *const *g3, const T *b, T *r, double *partial) {
+  switch (ctx->g.nx) {
+    case 64: return launch_x_div<T, 64, true>(ctx, g3, b, r, partial);
+    case 128: return launch_x_div<T, 128, true>(ctx, g3, b, r, partial);
+    default: return 0;
+  }
+}
+
 template void fft_face_mid_t<double>(kapsel_ctx *, void *const *, void *const *);
+template int fft_x_res_t<double>(kapsel_ctx *, void *const *, const double *, double *, double *);
 template void fft_face_mid_t<float>(kapsel_ctx *, void *const *, void *const *);
 template bool fft_x_grad_t<float>(kapsel_ctx *, const float *, float *, const double *, int, void *const *);
 template bool fft_x_grad_t<double>(kapsel_ctx *, const double *, double *, const double *, int, void *const *);

@@ -1116,15 +1116,23 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
   int total = 0;
   double rel = 1.0, bnorm = 0.0;
   for (int outer = 0; outer < 30; outer++) {
-    // rk = A xk - bk in k-space, ||r||^2 by Parseval
-    {
-      LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 4, sizeof(double)));
-      hipLaunchKernelGGL(grad_shift_kernel<double>, dim3(blocks_for(g.nc, TPB)), dim3(TPB), 0, ctx->stream, xk,
-                         s64[1], s64[2], s64[3], sh<double>(ctx, 0), sh<double>(ctx, 1), sh<double>(ctx, 2), g);
+    // rk = A xk - bk in k-space, ||r||^2 by Parseval; where NX has fused x
+    // passes, the gradient symbols ride in the first and the divergence and
+    // the residual's partial sums in the last
+    if (fft_x_grad_t<double>(ctx, xk, xk, ctx->d_scal, -1, g3)) {
+      fft_face_mid_t<double>(ctx, g3, f3);
+      const int nb = fft_x_res_t<double>(ctx, g3, bk, rk, ctx->d_partial);
+      hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->d_partial, nb, ctx->d_scal, S_RR,
+                         0);
       KAPSEL_LAUNCH_CHECK();
-    }
-    fft_face_t<double>(ctx, g3, f3);
-    {
+    } else {
+      {
+        LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 4, sizeof(double)));
+        hipLaunchKernelGGL(grad_shift_kernel<double>, dim3(blocks_for(g.nc, TPB)), dim3(TPB), 0, ctx->stream, xk,
+                           s64[1], s64[2], s64[3], sh<double>(ctx, 0), sh<double>(ctx, 1), sh<double>(ctx, 2), g);
+        KAPSEL_LAUNCH_CHECK();
+      }
+      fft_face_t<double>(ctx, g3, f3);
       LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 5, sizeof(double)));
       hipLaunchKernelGGL(kres_kernel<double>, dim3(NPART), dim3(TPB), 0, ctx->stream, s64[1], s64[2], s64[3], bk,
                          rk, sh<double>(ctx, 0), sh<double>(ctx, 1), sh<double>(ctx, 2), ctx->d_partial, g);
