@@ -249,6 +249,43 @@ __device__ __forceinline__ double block_sum(double v) {
   return s;
 }
 
+// The refinement's set-up in one pass (FP32 context): FP64 copies of eps
+// and rho, the face permittivities 0.5 (eps_i + eps_{i+e_d}) of the FP32
+// correction operator and of the FP64 residual operator (0 in the row
+// padding, which the plane pass multiplies too), and per-block sums of eps,
+// whose mean scales the preconditioner
+__global__ __launch_bounds__(TPB) void refine_setup_kernel(const float *__restrict__ eps, const float *__restrict__ rho,
+                                                           double *__restrict__ eps64, double *__restrict__ rho64,
+                                                           float *__restrict__ a0, float *__restrict__ a1,
+                                                           float *__restrict__ a2, double *__restrict__ b0,
+                                                           double *__restrict__ b1, double *__restrict__ b2,
+                                                           double *__restrict__ partial, Grid g) {
+  double s = 0.0;
+  for (size_t e = (size_t)blockIdx.x * TPB + threadIdx.x; e < g.f; e += (size_t)gridDim.x * TPB) {
+    const int k = (int)(e % (unsigned)g.nz_);
+    if (k >= g.nz) {
+      a0[e] = a1[e] = a2[e] = 0.f;
+      b0[e] = b1[e] = b2[e] = 0.0;
+      continue;
+    }
+    const size_t row = e / (unsigned)g.nz_;
+    const int j = (int)(row % (unsigned)g.ny), i = (int)(row / (unsigned)g.ny);
+    const float en = eps[e];
+    const float ex = eps[nbr(g, i + 1, j, k)], ey = eps[nbr(g, i, j + 1, k)], ez = eps[nbr(g, i, j, k + 1)];
+    eps64[e] = (double)en;
+    rho64[e] = (double)rho[e];
+    a0[e] = 0.5f * (ex + en);
+    a1[e] = 0.5f * (ey + en);
+    a2[e] = 0.5f * (ez + en);
+    b0[e] = 0.5 * ((double)ex + (double)en);
+    b1[e] = 0.5 * ((double)ey + (double)en);
+    b2[e] = 0.5 * ((double)ez + (double)en);
+    s += (double)en;
+  }
+  s = block_sum(s);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
 // partial[b] = sum over block b's real points of a*b (FP64)
 template <typename T, typename U>
 __global__ __launch_bounds__(TPB) void dot_partial_kernel(const T *__restrict__ a, const U *__restrict__ b,
@@ -990,18 +1027,6 @@ double true_relres(kapsel_ctx *ctx, const T *x, const T *b, const T *eps, T *r, 
   return bb > 0 ? sqrt(rr / bb) : 0.0;
 }
 
-// sc[S_INVEPS] = 1 / mean(eps), without a host round trip
-template <typename T>
-void inv_eps_device(kapsel_ctx *ctx, const T *eps, T *scratch) {
-  const Grid &g = ctx->g;
-  hipLaunchKernelGGL(fill_kernel<T>, dim3(blocks_for(g.f, TPB)), dim3(TPB), 0, ctx->stream, scratch, (T)1, g.f);
-  KAPSEL_LAUNCH_CHECK();
-  dot_t<T, T>(ctx, eps, scratch, S_TMP);
-  hipLaunchKernelGGL(inv_mean_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->d_scal, S_TMP, S_INVEPS,
-                     (double)g.nr);
-  KAPSEL_LAUNCH_CHECK();
-}
-
 template <typename T>
 double inv_eps_of(kapsel_ctx *ctx, const T *eps, T *scratch) {
   const Grid &g = ctx->g;
@@ -1067,11 +1092,21 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
                    fld<float>(ctx, KAPSEL_WORK3)};
   float *d32 = (float *)ctx->pscratch[4], *eps32 = fld<float>(ctx, KAPSEL_EPSILON);
   const unsigned fb = blocks_for(g.f, TPB);
-  // FP64 copies of the FP32 inputs: eps and rho enter the residual in FP64
-  hipLaunchKernelGGL((convert_real_kernel<double, float>), dim3(fb), dim3(TPB), 0, ctx->stream, eps64, eps32, g);
-  hipLaunchKernelGGL((convert_real_kernel<double, float>), dim3(fb), dim3(TPB), 0, ctx->stream, rk,
-                     fld<float>(ctx, KAPSEL_RHO_FREE), g);
-  KAPSEL_LAUNCH_CHECK();
+  float *face32[3] = {(float *)ctx->pscratch[5], (float *)ctx->pscratch[6], (float *)ctx->pscratch[7]};
+  // FP64 copies of the FP32 inputs (eps and rho enter the residual in FP64),
+  // the face factors of both operators, 1 / mean(eps) for the preconditioner
+  {
+    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, (4.0 + 4.0 + 8.0 + 8.0 + 12.0 + 24.0) * g.f);
+    hipLaunchKernelGGL(refine_setup_kernel, dim3(NPART), dim3(TPB), 0, ctx->stream, eps32,
+                       fld<float>(ctx, KAPSEL_RHO_FREE), eps64, rk, face32[0], face32[1], face32[2], face64[0],
+                       face64[1], face64[2], ctx->d_partial, g);
+    KAPSEL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->d_partial, NPART, ctx->d_scal, S_TMP,
+                       0);
+    hipLaunchKernelGGL(inv_mean_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->d_scal, S_TMP, S_INVEPS,
+                       (double)g.nr);
+    KAPSEL_LAUNCH_CHECK();
+  }
   // start: the POTENTIAL field if it was written since the last solve; else
   // the last solution, extrapolated linearly in time once two solutions of
   // consecutive solves are known
@@ -1090,14 +1125,7 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
     KAPSEL_HIP(hipMemcpyAsync(ctx->x64_prev, xk, g.f * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     ctx->prev_valid = true;
   }
-  // face factors of both operators, then eps64 and rho as spectra
-  if (precond) inv_eps_device<float>(ctx, eps32, w32[3]);
-  float *face32[3] = {(float *)ctx->pscratch[5], (float *)ctx->pscratch[6], (float *)ctx->pscratch[7]};
-  hipLaunchKernelGGL(face_fields_kernel<float>, dim3(fb), dim3(TPB), 0, ctx->stream, eps32, face32[0], face32[1],
-                     face32[2], g);
-  hipLaunchKernelGGL(face_fields_kernel<double>, dim3(fb), dim3(TPB), 0, ctx->stream, eps64, face64[0],
-                     face64[1], face64[2], g);
-  KAPSEL_LAUNCH_CHECK();
+  // eps64 and rho as spectra
   void *er[2] = {eps64, rk};
   fft_r2c_t<double>(ctx, er, 2);
   {
