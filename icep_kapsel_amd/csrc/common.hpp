@@ -178,7 +178,7 @@ struct kapsel_ctx {
   // context precision, and FP64 shadow fields for the iterative refinement
   // of an FP32 context
   void *pscratch[8] = {};          // [0..3] operator scratch, [4] refinement correction, [5..7] face eps
-  double *p64[11] = {};           // eps, x (k), b, r (k), 4 scratch (b spectrum first), 3 face eps
+  double *p64[10] = {};           // eps (then its spectrum), x (k), r (k), b (k), 3 operator fields, 3 face eps
   // FP32 context: p64[1] holds the FP64 potential of the last solve; it seeds
   // the next solve unless the FP32 POTENTIAL field was written since.
   // x64_prev holds the solve before it (when prev_valid), so the next start

@@ -1082,10 +1082,9 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
   // FP64 shadow: eps (real, then its spectrum), the potential and the
   // residual as half spectra (rk holds rho until the RHS is built), the RHS
   // spectrum, operator scratch, face permittivities
-  double *eps64 = ctx->p64[0], *xk = ctx->p64[1], *rk = ctx->p64[3];
-  double *bk = ctx->p64[4];
-  double *s64[4] = {ctx->p64[4], ctx->p64[5], ctx->p64[6], ctx->p64[7]};
-  double *face64[3] = {ctx->p64[8], ctx->p64[9], ctx->p64[10]};
+  double *eps64 = ctx->p64[0], *xk = ctx->p64[1], *rk = ctx->p64[2], *bk = ctx->p64[3];
+  double *s64[4] = {nullptr, ctx->p64[4], ctx->p64[5], ctx->p64[6]};  // [1..3]: the operator's three fields
+  double *face64[3] = {ctx->p64[7], ctx->p64[8], ctx->p64[9]};
   float *s32[4] = {(float *)ctx->pscratch[0], (float *)ctx->pscratch[1], (float *)ctx->pscratch[2],
                    (float *)ctx->pscratch[3]};
   float *w32[4] = {fld<float>(ctx, KAPSEL_WORK0), fld<float>(ctx, KAPSEL_WORK1), fld<float>(ctx, KAPSEL_WORK2),
