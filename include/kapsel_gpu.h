@@ -179,7 +179,12 @@ int kapsel_make_f_particle_dt_sole(kapsel_ctx *ctx);
 /* Initial rigidity projection of main (sp_3d_ns.cxx:431-447). */
 int kapsel_init_rigidity(kapsel_ctx *ctx);
 /* Time_evolution_hydro, Navier-Stokes branch (sp_3d_ns.cxx:24-186), then
- * jikan->time += dt_fluid and jikan->ts += 1 (sp_3d_ns.cxx:524-525). */
+ * jikan->time += dt_fluid and jikan->ts += 1 (sp_3d_ns.cxx:524-525).
+ * Afterwards U0..U2 hold the spectrum of u (U2zeta_k transforms in place).
+ * An FP32 context ends the step in k-space (the particle force's projected
+ * spectrum is added to the kept velocity spectrum): F0..F2 then hold that
+ * force spectrum, WORK0..WORK2 the velocity spectrum before it; the kapsel_icep_step
+ * and kapsel_icep_electrolyte_step of an FP32 context do the same. */
 int kapsel_spm_step(kapsel_ctx *ctx, kapsel_time *jikan);
 
 /* ---- dielectric Poisson (ICEP) ------------------------------------------- */
