@@ -183,8 +183,9 @@ int kapsel_init_rigidity(kapsel_ctx *ctx);
  * Afterwards U0..U2 hold the spectrum of u (U2zeta_k transforms in place).
  * An FP32 context ends the step in k-space (the particle force's projected
  * spectrum is added to the kept velocity spectrum): F0..F2 then hold that
- * force spectrum, WORK0..WORK2 the velocity spectrum before it; the kapsel_icep_step
- * and kapsel_icep_electrolyte_step of an FP32 context do the same. */
+ * force spectrum and WORK0..WORK2 the velocity spectrum before it.
+ * kapsel_icep_step and kapsel_icep_electrolyte_step of an FP32 context end
+ * the same way (F0..F2 likewise; the kept spectrum in internal scratch). */
 int kapsel_spm_step(kapsel_ctx *ctx, kapsel_time *jikan);
 
 /* ---- dielectric Poisson (ICEP) ------------------------------------------- */
