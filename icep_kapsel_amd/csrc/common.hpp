@@ -192,10 +192,11 @@ struct kapsel_ctx {
   double *x64_prev = nullptr;
   double *d_partial = nullptr;    // block partial sums (reductions)
   double *d_scal = nullptr;       // CG scalars
-  double *h_slots = nullptr;      // pinned host copies of ||r||^2 (pipelined convergence checks)
+  double *h_slots = nullptr;      // pinned host scalars: ||r||^2 of CG iterations [0..1], refinement [2..3]
   double *h_slots_dev = nullptr;  // the same two slots as the device sees them
   unsigned *d_count = nullptr;    // CG iteration count of the device-written slots
   hipEvent_t slot_ev[2] = {};
+  hipEvent_t res_ev = nullptr;    // after a refinement residual (its sums in h_slots[2..3])
   // one k-space PCG iteration captured as a HIP graph, replayed while the
   // buffers it was captured with (kcg_key) stay the same
   hipGraphExec_t kcg_exec = nullptr;

@@ -303,8 +303,10 @@ __global__ __launch_bounds__(TPB) void dot_partial_kernel(const T *__restrict__ 
 // per block to let the last block finish costs more than the launch):
 // scal[slot] = sum(partial) in block order; mode 1 also sets
 // beta = new / scal[S_RZ] and scal[S_RZ] = new (the rz update of CG)
+// with host (mapped pinned host memory): the sum goes to host[hslot] as well
 __global__ __launch_bounds__(TPB) void finish_kernel(const double *__restrict__ partial, int n,
-                                                     double *__restrict__ scal, int slot, int mode) {
+                                                     double *__restrict__ scal, int slot, int mode,
+                                                     double *host = nullptr, int hslot = 0) {
   double s = 0.0;
   for (int i = threadIdx.x; i < n; i += TPB) s += partial[i];
   s = block_sum(s);
@@ -315,6 +317,10 @@ __global__ __launch_bounds__(TPB) void finish_kernel(const double *__restrict__ 
       scal[S_RZ] = s;
     }
     scal[slot] = s;
+    if (host) {
+      host[hslot] = s;
+      __threadfence_system();
+    }
   }
 }
 
@@ -720,8 +726,9 @@ void ensure_storage(kapsel_ctx *ctx) {
     ctx->n_partial = NPART;
   }
   if (!ctx->h_slots) {
-    KAPSEL_HIP(hipHostMalloc((void **)&ctx->h_slots, sizeof(double) * 2, hipHostMallocDefault));
-    ctx->h_slots[0] = ctx->h_slots[1] = 0.0;
+    KAPSEL_HIP(hipHostMalloc((void **)&ctx->h_slots, sizeof(double) * 4, hipHostMallocDefault));
+    for (int i = 0; i < 4; i++) ctx->h_slots[i] = 0.0;
+    KAPSEL_HIP(hipEventCreateWithFlags(&ctx->res_ev, hipEventDisableTiming));
     KAPSEL_HIP(hipHostGetDevicePointer((void **)&ctx->h_slots_dev, ctx->h_slots, 0));
     for (hipEvent_t &e : ctx->slot_ev) KAPSEL_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
@@ -757,13 +764,6 @@ double read_scal(kapsel_ctx *ctx, int slot) {
   KAPSEL_HIP(hipMemcpyAsync(&v, ctx->d_scal + slot, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   KAPSEL_HIP(hipStreamSynchronize(ctx->stream));
   return v;
-}
-
-// out[lo..hi] = scalars lo..hi, one copy and one wait
-void read_scals(kapsel_ctx *ctx, double *out, int lo, int hi) {
-  KAPSEL_HIP(hipMemcpyAsync(out + lo, ctx->d_scal + lo, sizeof(double) * (hi - lo + 1), hipMemcpyDeviceToHost,
-                            ctx->stream));
-  KAPSEL_HIP(hipStreamSynchronize(ctx->stream));
 }
 
 // mean of eps over the real points (preconditioner scale)
@@ -984,24 +984,39 @@ void kcg_enqueue(kapsel_ctx *ctx, T *x, const T *eps, int precond, T *const *w, 
 // count and *rr_out = that ||A x - b||^2.  rr_known >= 0: ||b||^2 as the
 // caller computed it (x starts at zero), so the start needs no read-back.
 template <typename T>
-int kcg_t(kapsel_ctx *ctx, T *x, const T *b, const T *eps, double tol_abs, int maxit, int precond, T *const *w,
-          T *const *s, T *const *face, double *rr_out, double rr_known = -1.0) {
+void kcg_init_t(kapsel_ctx *ctx, T *x, const T *b, int precond, T *const *w) {
   const Grid &g = ctx->g;
   T *r = w[0], *p = w[1], *z = precond ? w[3] : w[0];
+  LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, precond ? 5 : 4, sizeof(T)));
+  hipLaunchKernelGGL(kcg_init_kernel<T>, dim3(NPART), dim3(TPB), 0, ctx->stream, b, x, r, z, p, ctx->d_scal, precond,
+                     ctx->d_partial, g);
+  KAPSEL_LAUNCH_CHECK();
+  hipLaunchKernelGGL(finish2_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->d_partial, ctx->d_scal, 0,
+                     ctx->h_slots_dev, ctx->d_count);
+  KAPSEL_LAUNCH_CHECK();
+}
+
+// Queues the start of a correction and its first iteration before the
+// caller has read whether the correction is needed (kcg_t with begun then
+// continues it; if it is not needed, it only wrote the correction's vectors)
+template <typename T>
+void kcg_begin(kapsel_ctx *ctx, T *x, const T *b, const T *eps, int precond, T *const *w, T *const *s,
+               T *const *face) {
+  kcg_init_t<T>(ctx, x, b, precond, w);
+  kcg_enqueue<T>(ctx, x, eps, precond, w, s, face);
+}
+
+template <typename T>
+int kcg_t(kapsel_ctx *ctx, T *x, const T *b, const T *eps, double tol_abs, int maxit, int precond, T *const *w,
+          T *const *s, T *const *face, double *rr_out, double rr_known = -1.0, bool begun = false) {
+  const Grid &g = ctx->g;
   const double norm = 1.0 / (double)g.nr;  // Parseval factor of the k-space sums
-  {
-    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, precond ? 5 : 4, sizeof(T)));
-    hipLaunchKernelGGL(kcg_init_kernel<T>, dim3(NPART), dim3(TPB), 0, ctx->stream, b, x, r, z, p, ctx->d_scal,
-                       precond, ctx->d_partial, g);
-    KAPSEL_LAUNCH_CHECK();
-    hipLaunchKernelGGL(finish2_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->d_partial, ctx->d_scal, 0,
-                       ctx->h_slots_dev, ctx->d_count);
-    KAPSEL_LAUNCH_CHECK();
-  }
+  if (!begun) kcg_init_t<T>(ctx, x, b, precond, w);
   // ||b||^2 when the caller knows it (to rounding), else read back
   double rr = rr_known >= 0.0 ? rr_known : read_scal(ctx, S_RR) * norm;
   const double tol2 = tol_abs * tol_abs;
   ConvergenceWatch cw(ctx, norm, rr, true);
+  if (begun) cw.queued();  // the first iteration, queued by kcg_begin
   cw.run(tol2, maxit, [&]() { kcg_enqueue<T>(ctx, x, eps, precond, w, s, face); });
   rr = cw.rr;
   const int it = cw.it;
@@ -1133,15 +1148,29 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
                        sh<double>(ctx, 1), sh<double>(ctx, 2), e0[0], e0[1], e0[2], ctx->rho_bg, ctx->d_partial, g);
     KAPSEL_LAUNCH_CHECK();
     hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->d_partial, NPART, ctx->d_scal, S_BB,
-                       0);
+                       0, ctx->h_slots_dev, 2);
     KAPSEL_LAUNCH_CHECK();
   }
   void *g3[3] = {s64[1], s64[2], s64[3]};
   void *f3[3] = {face64[0], face64[1], face64[2]};
   const double norm = 1.0 / (double)g.nr;  // Parseval factor of the k-space sums
   float *b32 = fld<float>(ctx, KAPSEL_RHS);  // the correction's right-hand side spectrum meanwhile
+  float *pot = fld<float>(ctx, KAPSEL_POTENTIAL);
+  // outputs: the potential and the RHS b (FP32, real)
+  auto outputs = [&]() {
+    hipLaunchKernelGGL((convert_real_kernel<float, double>), dim3(fb), dim3(TPB), 0, ctx->stream, pot, xk, g);
+    hipLaunchKernelGGL((convert_real_kernel<float, double>), dim3(fb), dim3(TPB), 0, ctx->stream, b32, bk, g);
+    KAPSEL_LAUNCH_CHECK();
+    void *p2[2] = {pot, b32};
+    fft_c2r_t<float>(ctx, p2, 2);
+  };
+  // While the host waits for a residual, the GPU runs what most likely
+  // follows: after the first residual the correction's start and first
+  // iteration, after a correction the outputs.  Whichever turns out not to
+  // be needed only wrote scratch (or is redone before the solve returns).
   int total = 0;
   double rel = 1.0, bnorm = 0.0;
+  bool outputs_current = false;
   for (int outer = 0; outer < 30; outer++) {
     // rk = A xk - bk in k-space, ||r||^2 by Parseval; where NX has fused x
     // passes, the gradient symbols ride in the first and the divergence and
@@ -1150,7 +1179,7 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
       fft_face_mid_t<double>(ctx, g3, f3);
       const int nb = fft_x_res_t<double>(ctx, g3, bk, rk, ctx->d_partial);
       hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->d_partial, nb, ctx->d_scal, S_RR,
-                         0);
+                         0, ctx->h_slots_dev, 3);
       KAPSEL_LAUNCH_CHECK();
     } else {
       {
@@ -1165,15 +1194,27 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
                          rk, sh<double>(ctx, 0), sh<double>(ctx, 1), sh<double>(ctx, 2), ctx->d_partial, g);
       KAPSEL_LAUNCH_CHECK();
       hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->d_partial, NPART, ctx->d_scal,
-                         S_RR, 0);
+                         S_RR, 0, ctx->h_slots_dev, 3);
       KAPSEL_LAUNCH_CHECK();
     }
-    double sc[S_RR + 1];
-    read_scals(ctx, sc, outer == 0 ? S_BB : S_RR, S_RR);
+    KAPSEL_HIP(hipEventRecord(ctx->res_ev, ctx->stream));
+    const bool begun = outer == 0;
+    if (begun) {
+      // the correction's right-hand side (the residual in FP32) and start
+      hipLaunchKernelGGL((convert_real_kernel<float, double>), dim3(fb), dim3(TPB), 0, ctx->stream, b32, rk, g);
+      KAPSEL_LAUNCH_CHECK();
+      ctx->refine64 = false;
+      kcg_begin<float>(ctx, d32, b32, eps32, precond, w32, s32 + 1, face32);
+      ctx->refine64 = true;
+    } else {
+      outputs();
+    }
+    KAPSEL_HIP(hipEventSynchronize(ctx->res_ev));
+    const volatile double *hs = ctx->h_slots;
     if (outer == 0) {
-      const double bb = sc[S_BB] * norm;
+      const double bb = hs[2] * norm;
       if (bb == 0.0) {
-        KAPSEL_HIP(hipMemsetAsync(ctx->fields[KAPSEL_POTENTIAL], 0, g.f * sizeof(float), ctx->stream));
+        KAPSEL_HIP(hipMemsetAsync(pot, 0, g.f * sizeof(float), ctx->stream));
         KAPSEL_HIP(hipMemsetAsync(b32, 0, g.f * sizeof(float), ctx->stream));
         ctx->pot64_valid = false;
         *relres = 0.0;
@@ -1181,34 +1222,34 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
       }
       bnorm = sqrt(bb);
     }
-    const double rr = sc[S_RR] * norm;
+    const double rr = hs[3] * norm;
     rel = sqrt(rr) / bnorm;
-    if (rel < tol || total >= maxit) break;
+    if (rel < tol || total >= maxit) {
+      outputs_current = !begun;
+      break;
+    }
     // correction: A d = r in FP32 k-space, then x -= d.  Inner tolerance:
     // what the outer bound still needs (80 % of it: the FP32 operator's own
     // rounding leaves the new FP64 residual within a few percent of the
     // inner one), but never below 1e-4 of the current residual, the
     // reduction FP32 certifies
-    hipLaunchKernelGGL((convert_real_kernel<float, double>), dim3(fb), dim3(TPB), 0, ctx->stream, b32, rk, g);
-    KAPSEL_LAUNCH_CHECK();
+    if (!begun) {
+      hipLaunchKernelGGL((convert_real_kernel<float, double>), dim3(fb), dim3(TPB), 0, ctx->stream, b32, rk, g);
+      KAPSEL_LAUNCH_CHECK();
+    }
     ctx->refine64 = false;  // the FP32 correction solve
     double rr_in = 0.0;
     const double tol_in = fmax(1e-4 * sqrt(rr), 0.8 * tol * bnorm);
-    int it = kcg_t<float>(ctx, d32, b32, eps32, tol_in, maxit - total, precond, w32, s32 + 1, face32, &rr_in, rr);
+    int it = kcg_t<float>(ctx, d32, b32, eps32, tol_in, maxit - total, precond, w32, s32 + 1, face32, &rr_in, rr,
+                          begun);
     total += it;
     ctx->refine64 = true;
     hipLaunchKernelGGL(sub_all_kernel, dim3(fb), dim3(TPB), 0, ctx->stream, xk, d32, g.f);
     KAPSEL_LAUNCH_CHECK();
     if (it == 0) break;
   }
-  // outputs: the potential (FP32, real) and the RHS b (FP32, real)
-  float *pot = fld<float>(ctx, KAPSEL_POTENTIAL);
-  hipLaunchKernelGGL((convert_real_kernel<float, double>), dim3(fb), dim3(TPB), 0, ctx->stream, pot, xk, g);
-  hipLaunchKernelGGL((convert_real_kernel<float, double>), dim3(fb), dim3(TPB), 0, ctx->stream, b32, bk, g);
-  KAPSEL_LAUNCH_CHECK();
+  if (!outputs_current) outputs();
   ctx->refine64 = false;
-  void *p2[2] = {pot, b32};
-  fft_c2r_t<float>(ctx, p2, 2);
   ctx->pot64_valid = true;
   ctx->spec64_fresh = true;
   *relres = rel;
@@ -1377,6 +1418,7 @@ void poisson_free(kapsel_ctx *ctx) {
   if (ctx->kcg_exec) (void)hipGraphExecDestroy(ctx->kcg_exec), ctx->kcg_exec = nullptr;
   for (hipEvent_t &e : ctx->slot_ev)
     if (e) (void)hipEventDestroy(e), e = nullptr;
+  if (ctx->res_ev) (void)hipEventDestroy(ctx->res_ev), ctx->res_ev = nullptr;
   if (ctx->d_scal) (void)hipFree(ctx->d_scal), ctx->d_scal = nullptr;
 }
 
