@@ -22,6 +22,8 @@
 // residual is an FP64 residual.  All reductions accumulate in FP64.
 #include <cmath>
 
+#include <functional>
+
 #include "common.hpp"
 
 namespace kapsel {
@@ -808,21 +810,31 @@ struct ConvergenceWatch {
     rr_prev = rr;
     rr = static_cast<volatile double *>(ctx->h_slots)[known & 1] * scale;
   }
-  // runs enqueue() until ||r||^2 < tol2 or maxit iterations
+  // runs enqueue() until ||r||^2 < tol2 or maxit iterations.  With tail:
+  // when the trend predicts that the iteration in flight is the last (or it
+  // is the maxit-th), tail() is queued right behind it before the host reads
+  // it, and the run ends with that iteration (tail_queued) whether or not it
+  // met the bound -- the caller's next step then sees what it left.
+  bool tail_queued = false;
   template <typename F>
-  void run(double tol2, int maxit, F enqueue) {
+  void run(double tol2, int maxit, F enqueue, const std::function<void()> *tail = nullptr) {
     for (;;) {
       if (known == it) {
         if (rr < tol2 || it >= maxit) return;
         enqueue();
         queued();
       } else {
-        const bool ahead = it < maxit && rr_prev > 0.0 && rr * (rr / rr_prev) >= tol2;
+        const bool trend = rr_prev > 0.0;
+        const bool ahead = it < maxit && trend && rr * (rr / rr_prev) >= tol2;
         if (ahead) {
           enqueue();
           queued();
+        } else if (tail && (trend || it >= maxit) && known + 1 == it) {
+          (*tail)();
+          tail_queued = true;
         }
         wait_one();
+        if (tail_queued && known == it) return;
       }
     }
   }
@@ -1008,7 +1020,8 @@ void kcg_begin(kapsel_ctx *ctx, T *x, const T *b, const T *eps, int precond, T *
 
 template <typename T>
 int kcg_t(kapsel_ctx *ctx, T *x, const T *b, const T *eps, double tol_abs, int maxit, int precond, T *const *w,
-          T *const *s, T *const *face, double *rr_out, double rr_known = -1.0, bool begun = false) {
+          T *const *s, T *const *face, double *rr_out, double rr_known = -1.0, bool begun = false,
+          const std::function<void()> *tail = nullptr, bool *tail_queued = nullptr) {
   const Grid &g = ctx->g;
   const double norm = 1.0 / (double)g.nr;  // Parseval factor of the k-space sums
   if (!begun) kcg_init_t<T>(ctx, x, b, precond, w);
@@ -1017,7 +1030,8 @@ int kcg_t(kapsel_ctx *ctx, T *x, const T *b, const T *eps, double tol_abs, int m
   const double tol2 = tol_abs * tol_abs;
   ConvergenceWatch cw(ctx, norm, rr, true);
   if (begun) cw.queued();  // the first iteration, queued by kcg_begin
-  cw.run(tol2, maxit, [&]() { kcg_enqueue<T>(ctx, x, eps, precond, w, s, face); });
+  cw.run(tol2, maxit, [&]() { kcg_enqueue<T>(ctx, x, eps, precond, w, s, face); }, tail);
+  if (tail_queued) *tail_queued = cw.tail_queued;
   rr = cw.rr;
   const int it = cw.it;
   *rr_out = rr;
@@ -1164,17 +1178,10 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
     void *p2[2] = {pot, b32};
     fft_c2r_t<float>(ctx, p2, 2);
   };
-  // While the host waits for a residual, the GPU runs what most likely
-  // follows: after the first residual the correction's start and first
-  // iteration, after a correction the outputs.  Whichever turns out not to
-  // be needed only wrote scratch (or is redone before the solve returns).
-  int total = 0;
-  double rel = 1.0, bnorm = 0.0;
-  bool outputs_current = false;
-  for (int outer = 0; outer < 30; outer++) {
-    // rk = A xk - bk in k-space, ||r||^2 by Parseval; where NX has fused x
-    // passes, the gradient symbols ride in the first and the divergence and
-    // the residual's partial sums in the last
+  // rk = A xk - bk in k-space, ||r||^2 by Parseval (raw sum also into the
+  // host slot 3); where NX has fused x passes, the gradient symbols ride in
+  // the first and the divergence and the residual's partial sums in the last
+  auto residual = [&]() {
     if (fft_x_grad_t<double>(ctx, xk, xk, ctx->d_scal, -1, g3)) {
       fft_face_mid_t<double>(ctx, g3, f3);
       const int nb = fft_x_res_t<double>(ctx, g3, bk, rk, ctx->d_partial);
@@ -1197,6 +1204,27 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
                          S_RR, 0, ctx->h_slots_dev, 3);
       KAPSEL_LAUNCH_CHECK();
     }
+  };
+  // x -= d, then the new residual: queued by the correction itself behind
+  // the iteration its trend predicts to be the last (the next residual read
+  // tells whether it was)
+  const std::function<void()> apply_and_residual = [&]() {
+    const bool was = ctx->refine64;
+    ctx->refine64 = true;
+    hipLaunchKernelGGL(sub_all_kernel, dim3(fb), dim3(TPB), 0, ctx->stream, xk, d32, g.f);
+    KAPSEL_LAUNCH_CHECK();
+    residual();
+    ctx->refine64 = was;
+  };
+  // While the host waits for a residual, the GPU runs what most likely
+  // follows: after the first residual the correction's start and first
+  // iteration, after a correction the outputs.  Whichever turns out not to
+  // be needed only wrote scratch (or is redone before the solve returns).
+  int total = 0;
+  double rel = 1.0, bnorm = 0.0;
+  bool outputs_current = false;
+  residual();
+  for (int outer = 0; outer < 30; outer++) {
     KAPSEL_HIP(hipEventRecord(ctx->res_ev, ctx->stream));
     const bool begun = outer == 0;
     if (begun) {
@@ -1240,13 +1268,15 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
     ctx->refine64 = false;  // the FP32 correction solve
     double rr_in = 0.0;
     const double tol_in = fmax(1e-4 * sqrt(rr), 0.8 * tol * bnorm);
+    bool tail = false;
     int it = kcg_t<float>(ctx, d32, b32, eps32, tol_in, maxit - total, precond, w32, s32 + 1, face32, &rr_in, rr,
-                          begun);
+                          begun, &apply_and_residual, &tail);
     total += it;
     ctx->refine64 = true;
-    hipLaunchKernelGGL(sub_all_kernel, dim3(fb), dim3(TPB), 0, ctx->stream, xk, d32, g.f);
-    KAPSEL_LAUNCH_CHECK();
-    if (it == 0) break;
+    if (!tail) {
+      if (it == 0) break;
+      apply_and_residual();
+    }
   }
   if (!outputs_current) outputs();
   ctx->refine64 = false;
