@@ -207,6 +207,7 @@ _PROTOS = {
     "kapsel_poisson_galerkin_start": [_DP, _DP],
     "kapsel_kernel_timing": [_I],
     "kapsel_kernel_timing_get": [_I, _DP, _IP, _DP],
+    "kapsel_kernel_timing_every": [_I],
 }
 
 _lib = None

@@ -37,7 +37,7 @@ static hipEvent_t take_event(kapsel_ctx *ctx) {
 
 LaunchTimer::LaunchTimer(kapsel_ctx *c, int k, double b)
     : ctx(c), cls(c->refine64 ? KAPSEL_K_REFINE64 : k), bytes(b) {
-  if (ctx->timing_mask & (1 << cls)) {
+  if ((ctx->timing_mask & (1 << cls)) && ctx->timing_seen[cls]++ % (unsigned)ctx->timing_every == 0) {
     start = take_event(ctx);
     stop = take_event(ctx);
     KAPSEL_HIP(hipEventRecord(start, ctx->stream));
@@ -1355,6 +1355,19 @@ int kapsel_kernel_timing(kapsel_ctx *ctx, int mask) {
   check_ctx_ro(ctx);
   harvest_timing(ctx);
   ctx->timing_mask = mask;
+  API_END
+}
+
+int kapsel_kernel_timing_every(kapsel_ctx *ctx, int every) {
+  API_BEGIN
+  check_ctx_ro(ctx);
+  if (every < 1) {
+    set_error("kernel timing: every must be >= 1");
+    throw Error{KAPSEL_ERR_ARG};
+  }
+  harvest_timing(ctx);
+  ctx->timing_every = every;
+  for (unsigned &c : ctx->timing_seen) c = 0;
   API_END
 }
 

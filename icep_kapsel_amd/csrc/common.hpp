@@ -192,11 +192,15 @@ struct kapsel_ctx {
   double *x64_prev = nullptr;
   double *d_partial = nullptr;    // block partial sums (reductions)
   double *d_scal = nullptr;       // CG scalars
-  double *h_slots = nullptr;      // pinned host scalars: ||r||^2 of CG iterations [0..1], refinement [2..3]
-  double *h_slots_dev = nullptr;  // the same two slots as the device sees them
-  unsigned *d_count = nullptr;    // CG iteration count of the device-written slots
-  hipEvent_t slot_ev[2] = {};
-  hipEvent_t res_ev = nullptr;    // after a refinement residual (its sums in h_slots[2..3])
+  // pinned host scalars that kernels store into (mapped): [0..1] ||r||^2 of
+  // the k-space CG iterations by parity, [2] ||b||^2 and [3] ||r||^2 of the
+  // refinement, [4] and [5] the sequence numbers (counts since the context's
+  // start) of the last CG iteration and refinement residual stored
+  double *h_slots = nullptr;
+  double *h_slots_dev = nullptr;  // the same slots as the device sees them
+  unsigned long long *d_count = nullptr;  // [0], [1]: the device side of the two counts
+  unsigned long long seq_cg = 0, seq_res = 0;  // how many of each were queued
+  hipEvent_t slot_ev[2] = {};     // real-space CG (FP64 context): copies and events
   // one k-space PCG iteration captured as a HIP graph, replayed while the
   // buffers it was captured with (kcg_key) stay the same
   hipGraphExec_t kcg_exec = nullptr;
@@ -236,6 +240,8 @@ struct kapsel_ctx {
   double nu, irho, hxi, mass, imass, moi, imoi, a_r_cutoff;
   // kernel timing
   int timing_mask = 0;
+  int timing_every = 1;  // time one launch in timing_every of a selected class
+  unsigned timing_seen[KAPSEL_K_NCLASSES] = {};
   std::vector<kapsel::TimingRecord> timing_pending;
   std::vector<hipEvent_t> event_pool;
   double timing_ms[KAPSEL_K_NCLASSES] = {};

@@ -305,10 +305,13 @@ __global__ __launch_bounds__(TPB) void dot_partial_kernel(const T *__restrict__ 
 // per block to let the last block finish costs more than the launch):
 // scal[slot] = sum(partial) in block order; mode 1 also sets
 // beta = new / scal[S_RZ] and scal[S_RZ] = new (the rz update of CG)
-// with host (mapped pinned host memory): the sum goes to host[hslot] as well
+// with host (mapped pinned host memory): the sum goes to host[hslot] as
+// well; with cnt, the store is counted in cnt[1] and the count stored in
+// host[5] after it (the refinement polls that)
 __global__ __launch_bounds__(TPB) void finish_kernel(const double *__restrict__ partial, int n,
                                                      double *__restrict__ scal, int slot, int mode,
-                                                     double *host = nullptr, int hslot = 0) {
+                                                     double *host = nullptr, int hslot = 0,
+                                                     unsigned long long *cnt = nullptr) {
   double s = 0.0;
   for (int i = threadIdx.x; i < n; i += TPB) s += partial[i];
   s = block_sum(s);
@@ -322,17 +325,24 @@ __global__ __launch_bounds__(TPB) void finish_kernel(const double *__restrict__ 
     if (host) {
       host[hslot] = s;
       __threadfence_system();
+      if (cnt) {
+        const unsigned long long k = cnt[1] + 1ull;
+        cnt[1] = k;
+        host[5] = (double)k;
+        __threadfence_system();
+      }
     }
   }
 }
 
 // two sums at once: scal[S_RR] = <r,r> (partial[NPART..]); scal[S_RZ] =
 // <r,z>; mode 1 also beta = <r,z> / old scal[S_RZ] (mode 0: beta = 0).
-// With host (pinned host memory, mapped): mode 0 zeroes the iteration count
-// *cnt, mode 1 counts it up and stores <r,r> in host[count % 2], where the
-// pipelined convergence test reads it once the launch's event completes.
+// With host (pinned host memory, mapped), mode 1 counts the iteration in
+// cnt[0] and stores <r,r> in host[count % 2], then the count in host[4],
+// which the pipelined convergence test polls.
 __global__ __launch_bounds__(TPB) void finish2_kernel(const double *__restrict__ partial, double *__restrict__ scal,
-                                                      int mode, double *host = nullptr, unsigned *cnt = nullptr) {
+                                                      int mode, double *host = nullptr,
+                                                      unsigned long long *cnt = nullptr) {
   double a = 0.0, b = 0.0;
   for (int i = threadIdx.x; i < NPART; i += TPB) {
     a += partial[i];
@@ -345,15 +355,13 @@ __global__ __launch_bounds__(TPB) void finish2_kernel(const double *__restrict__
     scal[S_BETA] = (mode == 1 && old != 0.0) ? a / old : 0.0;
     scal[S_RZ] = a;
     scal[S_RR] = b;
-    if (host) {
-      if (mode == 1) {
-        const unsigned k = *cnt + 1u;
-        *cnt = k;
-        host[k & 1u] = b;
-        __threadfence_system();
-      } else {
-        *cnt = 0u;
-      }
+    if (host && mode == 1) {
+      const unsigned long long k = cnt[0] + 1ull;
+      cnt[0] = k;
+      host[k & 1ull] = b;
+      __threadfence_system();
+      host[4] = (double)k;
+      __threadfence_system();
     }
   }
 }
@@ -721,16 +729,16 @@ void build_rhs_t(kapsel_ctx *ctx, T *rhs, const T *rho, const T *eps, const doub
 void ensure_storage(kapsel_ctx *ctx) {
   if (!ctx->d_partial) {
     KAPSEL_HIP(hipMalloc(&ctx->d_partial, sizeof(double) * 3 * NPART));
-    KAPSEL_HIP(hipMalloc(&ctx->d_count, sizeof(unsigned) * 2));
-    KAPSEL_HIP(hipMemsetAsync(ctx->d_count, 0, sizeof(unsigned) * 2, ctx->stream));
+    KAPSEL_HIP(hipMalloc(&ctx->d_count, sizeof(unsigned long long) * 2));
+    KAPSEL_HIP(hipMemsetAsync(ctx->d_count, 0, sizeof(unsigned long long) * 2, ctx->stream));
     KAPSEL_HIP(hipMalloc(&ctx->d_scal, sizeof(double) * S_N));
     KAPSEL_HIP(hipMemsetAsync(ctx->d_scal, 0, sizeof(double) * S_N, ctx->stream));
     ctx->n_partial = NPART;
   }
   if (!ctx->h_slots) {
-    KAPSEL_HIP(hipHostMalloc((void **)&ctx->h_slots, sizeof(double) * 4, hipHostMallocDefault));
-    for (int i = 0; i < 4; i++) ctx->h_slots[i] = 0.0;
-    KAPSEL_HIP(hipEventCreateWithFlags(&ctx->res_ev, hipEventDisableTiming));
+    KAPSEL_HIP(hipHostMalloc((void **)&ctx->h_slots, sizeof(double) * 8, hipHostMallocDefault));
+    for (int i = 0; i < 8; i++) ctx->h_slots[i] = 0.0;
+    ctx->seq_cg = ctx->seq_res = 0;
     KAPSEL_HIP(hipHostGetDevicePointer((void **)&ctx->h_slots_dev, ctx->h_slots, 0));
     for (hipEvent_t &e : ctx->slot_ev) KAPSEL_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
@@ -787,27 +795,57 @@ __global__ void fill_kernel(T *__restrict__ a, T v, size_t n) {
 // does not drain between iterations.  It stops queueing ahead when the
 // geometric trend of the last two values says the iteration in flight
 // meets the bound, so normally no iteration runs past it.
-// device_slots: the iteration's last kernel stores ||r||^2 into the slot
-// itself (finish2_kernel with host slots), else a copy is queued.
+// Waits until the sequence number a kernel stores in host slot `slot`
+// reaches target.  The stores are system-scope, so polling sees them; after
+// a bounded spin (seconds) it waits for the stream instead, which ends once
+// the queued work has run.
+void wait_seq(kapsel_ctx *ctx, int slot, unsigned long long target) {
+  const volatile double *h = ctx->h_slots;
+  const double want = (double)target;
+  for (long spin = 0; spin < (1L << 26); spin++) {
+    if (h[slot] >= want) return;
+    __builtin_ia32_pause();
+  }
+  KAPSEL_HIP(hipStreamSynchronize(ctx->stream));
+  if (h[slot] < want) {
+    set_error("a kernel's host-side result (slot %d, number %llu) did not arrive", slot, target);
+    throw Error{KAPSEL_ERR_HIP};
+  }
+}
+
+// device_slots: the iteration's last kernel stores ||r||^2 and its sequence
+// number into the host slots itself (finish2_kernel with host slots), and
+// the host polls the number -- no marker enters the stream between two
+// iterations; else a copy and an event are queued after each iteration.
 struct ConvergenceWatch {
   kapsel_ctx *ctx;
   double scale;
   double rr, rr_prev = -1.0;
   int it = 0, known = 0;
   bool device_slots;
+  unsigned long long seq[2] = {0, 0};
   ConvergenceWatch(kapsel_ctx *c, double s, double rr0, bool dev = false)
       : ctx(c), scale(s), rr(rr0), device_slots(dev) {}
   void queued() {
     it++;
-    if (!device_slots)
-      KAPSEL_HIP(hipMemcpyAsync(ctx->h_slots + (it & 1), ctx->d_scal + S_RR, sizeof(double),
-                                hipMemcpyDeviceToHost, ctx->stream));
+    if (device_slots) {
+      seq[it & 1] = ctx->seq_cg;  // the iteration just queued
+      return;
+    }
+    KAPSEL_HIP(hipMemcpyAsync(ctx->h_slots + (it & 1), ctx->d_scal + S_RR, sizeof(double),
+                              hipMemcpyDeviceToHost, ctx->stream));
     KAPSEL_HIP(hipEventRecord(ctx->slot_ev[it & 1], ctx->stream));
   }
   void wait_one() {
     known++;
-    KAPSEL_HIP(hipEventSynchronize(ctx->slot_ev[known & 1]));
     rr_prev = rr;
+    if (device_slots) {
+      const unsigned long long k = seq[known & 1];
+      wait_seq(ctx, 4, k);
+      rr = static_cast<volatile double *>(ctx->h_slots)[k & 1] * scale;
+      return;
+    }
+    KAPSEL_HIP(hipEventSynchronize(ctx->slot_ev[known & 1]));
     rr = static_cast<volatile double *>(ctx->h_slots)[known & 1] * scale;
   }
   // runs enqueue() until ||r||^2 < tol2 or maxit iterations.  With tail:
@@ -957,6 +995,7 @@ void kcg_iteration(kapsel_ctx *ctx, T *x, const T *eps, int precond, T *const *w
 // the capture (and no kernel timing is on), else captures it anew.
 template <typename T>
 void kcg_enqueue(kapsel_ctx *ctx, T *x, const T *eps, int precond, T *const *w, T *const *s, T *const *face) {
+  ctx->seq_cg++;  // its finish2 stores this number in host slot 4
   if (ctx->timing_mask) {
     kcg_iteration<T>(ctx, x, eps, precond, w, s, face);
     return;
@@ -1186,7 +1225,7 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
       fft_face_mid_t<double>(ctx, g3, f3);
       const int nb = fft_x_res_t<double>(ctx, g3, bk, rk, ctx->d_partial);
       hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->d_partial, nb, ctx->d_scal, S_RR,
-                         0, ctx->h_slots_dev, 3);
+                         0, ctx->h_slots_dev, 3, ctx->d_count);
       KAPSEL_LAUNCH_CHECK();
     } else {
       {
@@ -1201,9 +1240,10 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
                          rk, sh<double>(ctx, 0), sh<double>(ctx, 1), sh<double>(ctx, 2), ctx->d_partial, g);
       KAPSEL_LAUNCH_CHECK();
       hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->d_partial, NPART, ctx->d_scal,
-                         S_RR, 0, ctx->h_slots_dev, 3);
+                         S_RR, 0, ctx->h_slots_dev, 3, ctx->d_count);
       KAPSEL_LAUNCH_CHECK();
     }
+    ctx->seq_res++;  // the finish stores this number in host slot 5
   };
   // x -= d, then the new residual: queued by the correction itself behind
   // the iteration its trend predicts to be the last (the next residual read
@@ -1225,7 +1265,7 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
   bool outputs_current = false;
   residual();
   for (int outer = 0; outer < 30; outer++) {
-    KAPSEL_HIP(hipEventRecord(ctx->res_ev, ctx->stream));
+    const unsigned long long res_seq = ctx->seq_res;
     const bool begun = outer == 0;
     if (begun) {
       // the correction's right-hand side (the residual in FP32) and start
@@ -1237,7 +1277,7 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
     } else {
       outputs();
     }
-    KAPSEL_HIP(hipEventSynchronize(ctx->res_ev));
+    wait_seq(ctx, 5, res_seq);
     const volatile double *hs = ctx->h_slots;
     if (outer == 0) {
       const double bb = hs[2] * norm;
@@ -1448,7 +1488,6 @@ void poisson_free(kapsel_ctx *ctx) {
   if (ctx->kcg_exec) (void)hipGraphExecDestroy(ctx->kcg_exec), ctx->kcg_exec = nullptr;
   for (hipEvent_t &e : ctx->slot_ev)
     if (e) (void)hipEventDestroy(e), e = nullptr;
-  if (ctx->res_ev) (void)hipEventDestroy(ctx->res_ev), ctx->res_ev = nullptr;
   if (ctx->d_scal) (void)hipFree(ctx->d_scal), ctx->d_scal = nullptr;
 }
 

@@ -338,6 +338,10 @@ class KapselGPU:
             mask |= 1 << abi.KCLASS[c]
         self._call("kernel_timing", mask)
 
+    def kernel_timing_every(self, every):
+        """Time one launch in `every` of each selected class."""
+        self._call("kernel_timing_every", int(every))
+
     def kernel_timing_get(self, cls):
         ms = ctypes.c_double(0.0)
         n = ctypes.c_int(0)

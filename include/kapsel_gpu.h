@@ -445,6 +445,11 @@ enum kapsel_kernel_class {
 int kapsel_kernel_timing(kapsel_ctx *ctx, int mask);
 int kapsel_kernel_timing_get(kapsel_ctx *ctx, int kernel_class, double *total_ms, int *launches,
                              double *total_bytes);
+/* Times one launch in `every` (>= 1; 1, the default, times all) of each class
+ * selected by kapsel_kernel_timing: an event pair delays the launch it
+ * brackets by the dispatch of its markers, so a sample perturbs the timed
+ * run less.  No reference counterpart. */
+int kapsel_kernel_timing_every(kapsel_ctx *ctx, int every);
 
 #ifdef __cplusplus
 }
