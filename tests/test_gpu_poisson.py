@@ -76,12 +76,14 @@ def test_poisson_solve_fp64_matches_oracle(precond):
     assert rel(xk - xk.mean(), xo - xo.mean()) < 1e-8
 
 
-@pytest.mark.parametrize("tol", [1e-8, 1e-10])
-def test_poisson_solve_fp32_refined(tol):
-    n = 64
-    print(f"case poisson solve fp32 refined tol={tol}", flush=True)
-    p64, st = _case(n, 6, 6.0, 0.1, 13)
-    p32, _ = _case(n, 6, 6.0, 0.1, 13, precision=abi.KAPSEL_FP32)
+@pytest.mark.parametrize("n,tol", [(64, 1e-8), (64, 1e-10), (32, 1e-10)])
+def test_poisson_solve_fp32_refined(n, tol):
+    """64^3 runs the fused x passes of the correction and the residual, 32^3
+    the separate gradient, face and divergence passes."""
+    print(f"case poisson solve fp32 refined {n}^3 tol={tol}", flush=True)
+    npart, a = (6, 6.0) if n == 64 else (2, 4.0)
+    p64, st = _case(n, npart, a, 0.1, 13)
+    p32, _ = _case(n, npart, a, 0.1, 13, precision=abi.KAPSEL_FP32)
     k64, k32 = gpu(p64), gpu(p32)
     e0 = [0.05, 0.0, 0.1]
     for s in (k64, k32):
@@ -93,6 +95,13 @@ def test_poisson_solve_fp32_refined(tol):
     assert rr32 < tol
     x64, x32 = k64.get_field("POTENTIAL")[:, :, :n], k32.get_field("POTENTIAL")[:, :, :n]
     assert rel(x32 - x32.mean(), x64 - x64.mean()) < 1e-5
+    # solves in a row from the kept FP64 potential: the first residual already
+    # meets the bound (no correction), then a changed field needs one
+    it2, rr2 = k32.poisson_solve(e0, tol, 5000, 1)
+    assert rr2 < tol and it2 == 0
+    it3, rr3 = k32.poisson_solve([0.0, 0.08, 0.1], tol, 5000, 1)
+    print(f"again: {it2} and {it3} inner iterations, relres {rr2:.2e}, {rr3:.2e}", flush=True)
+    assert rr3 < tol and it3 > 0
 
 
 def test_poisson_solve_128_dielectric_spheres():
