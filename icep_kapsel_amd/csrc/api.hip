@@ -329,14 +329,14 @@ static void u2zeta_k(kapsel_ctx *ctx) {
 // The end of an ICEP step in an FP32 context, from Solenoidal_u(u) on.
 // u = c2r(P u_k) and the particle force f = P(u_p - phi u) (f_particle.h)
 // enter U2zeta_k only as their sum, whose spectrum is P u_k + P f_k: the
-// first is kept when u is projected (keep3, three spare fields), the second
-// is never transformed back.  U holds that spectrum afterwards, F the
+// first is kept when u is projected (keep3, three spare fields: the
+// projection writes there and the first inverse pass reads from there), the
+// second is never transformed back.  U holds that spectrum afterwards, F the
 // spectrum P f_k; the FP64 context keeps the reference's real-space order.
 static void solenoidal_u_keep(kapsel_ctx *ctx, void *const *keep3) {
   fft_r2c(ctx, ctx->fields + KAPSEL_U0, 3);
-  solenoidal_uk(ctx, ctx->fields + KAPSEL_U0);
-  for (int d = 0; d < 3; d++) copy_field(ctx, keep3[d], ctx->fields[KAPSEL_U0 + d]);
-  fft_c2r(ctx, ctx->fields + KAPSEL_U0, 3);
+  solenoidal_uk(ctx, ctx->fields + KAPSEL_U0, keep3);
+  fft_c2r(ctx, ctx->fields + KAPSEL_U0, 3, keep3);
 }
 static void f_sole_u2zeta_k(kapsel_ctx *ctx, void *const *keep3) {
   make_f_nonsole(ctx);
@@ -360,9 +360,8 @@ static void spm_step(kapsel_ctx *ctx, kapsel_time &t) {
   void *keep3[3] = {fld(ctx, KAPSEL_WORK0), fld(ctx, KAPSEL_WORK1), fld(ctx, KAPSEL_WORK2)};
   const bool fp32 = ctx->prec == KAPSEL_FP32;
   if (fp32) {                                 // :29-42
-    zeta_k2u_k(ctx, fix_mask(ctx));
-    for (int d = 0; d < 3; d++) copy_field(ctx, keep3[d], ctx->fields[KAPSEL_U0 + d]);
-    fft_c2r(ctx, ctx->fields + KAPSEL_U0, 3);
+    zeta_k2u_k(ctx, fix_mask(ctx), keep3);
+    fft_c2r(ctx, ctx->fields + KAPSEL_U0, 3, keep3);
   } else {
     zeta_k2u(ctx, fix_mask(ctx));
   }

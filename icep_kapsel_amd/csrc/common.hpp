@@ -271,13 +271,14 @@ inline unsigned blocks_for(size_t n, unsigned threads) {
 // FFT (fft.hip)
 void init_twiddles(kapsel_ctx *ctx);
 void free_twiddles(kapsel_ctx *ctx);
-// nf fields transformed together (1..5), of the context precision or of T
+// nf fields transformed together (1..5), of the context precision or of T;
+// c2r with src: the spectra in src (left unchanged) go to fields
 void fft_r2c(kapsel_ctx *ctx, void *const *fields, int nf);
-void fft_c2r(kapsel_ctx *ctx, void *const *fields, int nf);
+void fft_c2r(kapsel_ctx *ctx, void *const *fields, int nf, void *const *src = nullptr);
 template <typename T>
 void fft_r2c_t(kapsel_ctx *ctx, void *const *fields, int nf);
 template <typename T>
-void fft_c2r_t(kapsel_ctx *ctx, void *const *fields, int nf);
+void fft_c2r_t(kapsel_ctx *ctx, void *const *fields, int nf, void *const *src = nullptr);
 // g_d <- r2c(face_d * c2r(g_d)) for the three staggered-gradient spectra
 // (face_d real fields, 0 in the row padding): the inverse and forward z (and,
 // where a plane fits in LDS, y) transforms and the product fused in one pass
@@ -302,13 +303,15 @@ int fft_x_res_t(kapsel_ctx *ctx, void *const *g3, const T *b, T *r, double *part
 
 // k-space and real-space pointwise (kspace.hip)
 void truncate(kapsel_ctx *ctx, void *const *fields, int nf);
-void zeta_k2u_k(kapsel_ctx *ctx, int fix_dc);
+// out3: the velocity spectrum goes there instead of U0..U2
+void zeta_k2u_k(kapsel_ctx *ctx, int fix_dc, void *const *out3 = nullptr);
 void u_k2zeta_k(kapsel_ctx *ctx);
 // U_k2zeta_k of the spectrum a3 + b3, which is also stored in U0..U2
 void u_k2zeta_sum_k(kapsel_ctx *ctx, void *const *a3, void *const *b3);
 void advection_products(kapsel_ctx *ctx);
 void advection_combine_slaved(kapsel_ctx *ctx, double dt_fluid);
-void solenoidal_uk(kapsel_ctx *ctx, void *const *f3);
+// out3: the projected spectrum goes there and f3 is left unchanged
+void solenoidal_uk(kapsel_ctx *ctx, void *const *f3, void *const *out3 = nullptr);
 void make_f_nonsole(kapsel_ctx *ctx);
 // u += f over the real points; keep3: also store the sum there
 void add_f_particle(kapsel_ctx *ctx, void *const *keep3 = nullptr);

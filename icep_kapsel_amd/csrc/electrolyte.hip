@@ -526,10 +526,8 @@ void phi_qq_t(kapsel_ctx *ctx, T *phi, T *surf) {
 // ion concentrations in real space: E_CX0/E_CX1 = c2r(CONC0/CONC1)
 template <typename T>
 void ions_real_t(kapsel_ctx *ctx) {
-  copy_field(ctx, ctx->el_f[E_CX0], ctx->fields[KAPSEL_CONC0]);
-  copy_field(ctx, ctx->el_f[E_CX1], ctx->fields[KAPSEL_CONC1]);
   void *two[2] = {ctx->el_f[E_CX0], ctx->el_f[E_CX1]};
-  fft_c2r_t<T>(ctx, two, 2);
+  fft_c2r_t<T>(ctx, two, 2, ctx->fields + KAPSEL_CONC0);
 }
 
 // Conc_k2charge_field into out (and keep); the ion fields must hold the

@@ -131,16 +131,19 @@ __device__ __forceinline__ V2<T> tw_get(const V2<T> *__restrict__ tw, int m) {
 // ------------------------------------------------------------ c2c line pass
 // Line q (0 <= q < Q) starts at complex offset (q / A) * B + (q % A); its
 // elements are ES complex values apart.  Lines q of one tile are adjacent.
+// out: where the transformed lines go (out = fp transforms in place; each
+// line is read whole before the barrier and written after it)
 template <typename T, int L, int DIR, int LPB>
-__global__ __launch_bounds__(LPB *FS<L>::R2) void fft_c2c_lines(FieldPtrs<T> fp, long long Q, long long A,
-                                                                 long long B, long long ES,
-                                                                 const V2<T> *__restrict__ tw) {
+__global__ __launch_bounds__(LPB *FS<L>::R2) void fft_c2c_lines_io(FieldPtrs<T> fp, FieldPtrs<T> out, long long Q,
+                                                                    long long A, long long B, long long ES,
+                                                                    const V2<T> *__restrict__ tw) {
   constexpr int R1 = FS<L>::R1, R2 = FS<L>::R2, TPL = R2;
   constexpr int LS = R2 * (R1 + 1) + 1;  // LDS line stride (odd)
   __shared__ V2<T> sm[LPB * LS];
   unsigned field;
   const unsigned tile = xcd_tile(field);
-  T *__restrict__ data = fp.p[field];
+  const T *data = fp.p[field];
+  T *dst = out.p[field];
   const int t = threadIdx.x;
   const int l = t % LPB;
   const int n2 = t / LPB;
@@ -169,7 +172,7 @@ __global__ __launch_bounds__(LPB *FS<L>::R2) void fft_c2c_lines(FieldPtrs<T> fp,
     dft<R2, 1, DIR>(w, x);
     if (valid) {
 #pragma unroll
-      for (int k2 = 0; k2 < R2; k2++) st2(data + 2 * (base + (long long)(k1 + R1 * k2) * ES), x[k2]);
+      for (int k2 = 0; k2 < R2; k2++) st2(dst + 2 * (base + (long long)(k1 + R1 * k2) * ES), x[k2]);
     }
   }
 }
@@ -950,25 +953,27 @@ void init_tw_t(kapsel_ctx *ctx) {
 constexpr int lines_per_block(size_t elem) { return elem == 8 ? 16 : 32; }
 
 template <typename T, int L, int DIR>
-void launch_lines(kapsel_ctx *ctx, int cls, const FieldPtrs<T> &fp, int nf, long long Q, long long A,
-                  long long B, long long ES, const void *tw) {
+void launch_lines(kapsel_ctx *ctx, int cls, const FieldPtrs<T> &fp, const FieldPtrs<T> &out, int nf, long long Q,
+                  long long A, long long B, long long ES, const void *tw) {
   constexpr int LPB = lines_per_block(sizeof(T));
   dim3 grid((unsigned)((Q + LPB - 1) / LPB), nf);
   LaunchTimer lt(ctx, cls, 2.0 * nf * ctx->g.f * sizeof(T));
-  hipLaunchKernelGGL((fft_c2c_lines<T, L, DIR, LPB>), grid, dim3(LPB * FS<L>::R2), 0, ctx->stream, fp, Q, A,
-                     B, ES, (const V2<T> *)tw);
+  hipLaunchKernelGGL((fft_c2c_lines_io<T, L, DIR, LPB>), grid, dim3(LPB * FS<L>::R2), 0, ctx->stream, fp, out, Q,
+                     A, B, ES, (const V2<T> *)tw);
   KAPSEL_LAUNCH_CHECK();
 }
 
+// out: nullptr transforms in place
 template <typename T, int DIR>
 void lines_dispatch(kapsel_ctx *ctx, int cls, int L, const FieldPtrs<T> &fp, int nf, long long Q, long long A,
-                    long long B, long long ES, const void *tw) {
+                    long long B, long long ES, const void *tw, const FieldPtrs<T> *out = nullptr) {
+  const FieldPtrs<T> &o = out ? *out : fp;
   switch (L) {
-    case 16: launch_lines<T, 16, DIR>(ctx, cls, fp, nf, Q, A, B, ES, tw); break;
-    case 32: launch_lines<T, 32, DIR>(ctx, cls, fp, nf, Q, A, B, ES, tw); break;
-    case 64: launch_lines<T, 64, DIR>(ctx, cls, fp, nf, Q, A, B, ES, tw); break;
-    case 128: launch_lines<T, 128, DIR>(ctx, cls, fp, nf, Q, A, B, ES, tw); break;
-    case 256: launch_lines<T, 256, DIR>(ctx, cls, fp, nf, Q, A, B, ES, tw); break;
+    case 16: launch_lines<T, 16, DIR>(ctx, cls, fp, o, nf, Q, A, B, ES, tw); break;
+    case 32: launch_lines<T, 32, DIR>(ctx, cls, fp, o, nf, Q, A, B, ES, tw); break;
+    case 64: launch_lines<T, 64, DIR>(ctx, cls, fp, o, nf, Q, A, B, ES, tw); break;
+    case 128: launch_lines<T, 128, DIR>(ctx, cls, fp, o, nf, Q, A, B, ES, tw); break;
+    case 256: launch_lines<T, 256, DIR>(ctx, cls, fp, o, nf, Q, A, B, ES, tw); break;
     default: set_error("unsupported FFT length %d", L); throw Error{KAPSEL_ERR_ARG};
   }
 }
@@ -1117,13 +1122,20 @@ void fft_r2c_t(kapsel_ctx *ctx, void *const *fields, int nf) {
   lines_dispatch<T, -1>(ctx, KAPSEL_K_FFT_LINES_FWD, g.nx, fp, nf, plane, plane, 0, plane, tw.x);
 }
 
+// src: nullptr transforms fields in place; otherwise the spectra in src are
+// transformed into fields and src is left as it was
 template <typename T>
-void fft_c2r_t(kapsel_ctx *ctx, void *const *fields, int nf) {
+void fft_c2r_t(kapsel_ctx *ctx, void *const *fields, int nf, void *const *src) {
   const Grid &g = ctx->g;
   const Twiddles &tw = ctx->tw[prec_of<T>()];
   FieldPtrs<T> fp = pack<T>(fields, nf);
   long long plane = (long long)g.ny * g.h;
-  lines_dispatch<T, +1>(ctx, KAPSEL_K_FFT_LINES_INV, g.nx, fp, nf, plane, plane, 0, plane, tw.x);
+  if (src) {
+    FieldPtrs<T> sp = pack<T>(src, nf);
+    lines_dispatch<T, +1>(ctx, KAPSEL_K_FFT_LINES_INV, g.nx, sp, nf, plane, plane, 0, plane, tw.x, &fp);
+  } else {
+    lines_dispatch<T, +1>(ctx, KAPSEL_K_FFT_LINES_INV, g.nx, fp, nf, plane, plane, 0, plane, tw.x);
+  }
   if (!plane_dispatch<T>(ctx, fp, nf, false)) {
     lines_dispatch<T, +1>(ctx, KAPSEL_K_FFT_LINES_INV, g.ny, fp, nf, (long long)g.nx * g.h, g.h,
                           (long long)g.ny * g.h, g.h, tw.y);
@@ -1255,8 +1267,8 @@ template void fft_face_t<double>(kapsel_ctx *, void *const *, void *const *);
 template void fft_face_t<float>(kapsel_ctx *, void *const *, void *const *);
 template void fft_r2c_t<double>(kapsel_ctx *, void *const *, int);
 template void fft_r2c_t<float>(kapsel_ctx *, void *const *, int);
-template void fft_c2r_t<double>(kapsel_ctx *, void *const *, int);
-template void fft_c2r_t<float>(kapsel_ctx *, void *const *, int);
+template void fft_c2r_t<double>(kapsel_ctx *, void *const *, int, void *const *);
+template void fft_c2r_t<float>(kapsel_ctx *, void *const *, int, void *const *);
 
 // tables of both precisions: an FP32 context refines its Poisson solve in FP64
 void init_twiddles(kapsel_ctx *ctx) {
@@ -1277,9 +1289,9 @@ void fft_r2c(kapsel_ctx *ctx, void *const *fields, int nf) {
   else fft_r2c_t<float>(ctx, fields, nf);
 }
 
-void fft_c2r(kapsel_ctx *ctx, void *const *fields, int nf) {
-  if (ctx->prec == KAPSEL_FP64) fft_c2r_t<double>(ctx, fields, nf);
-  else fft_c2r_t<float>(ctx, fields, nf);
+void fft_c2r(kapsel_ctx *ctx, void *const *fields, int nf, void *const *src) {
+  if (ctx->prec == KAPSEL_FP64) fft_c2r_t<double>(ctx, fields, nf, src);
+  else fft_c2r_t<float>(ctx, fields, nf, src);
 }
 
 }  // namespace kapsel

@@ -222,27 +222,26 @@ __global__ void advection_combine_slaved_kernel(const T *__restrict__ u0, const 
 }
 
 // Solenoidal_uk (operate_omega.cxx:187-220), optionally after the 2/3 cut
-// (Solenoidal_u, operate_omega.h:109-116)
+// (Solenoidal_u, operate_omega.h:109-116); in place when o = u (each point
+// is read before it is written)
 template <typename T>
-__global__ void solenoidal_uk_kernel(T *__restrict__ u0, T *__restrict__ u1, T *__restrict__ u2, Grid g,
-                                     int trunc) {
+__global__ void solenoidal_uk_kernel(P5<T> u, P5<T> o, Grid g, int trunc) {
   size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.nc) return;
   KIdx k = kidx(g, c);
   if (trunc && is_cut(g, k)) {
-    st2(u0 + 2 * c, V2<T>{0, 0});
-    st2(u1 + 2 * c, V2<T>{0, 0});
-    st2(u2 + 2 * c, V2<T>{0, 0});
+#pragma unroll
+    for (int d = 0; d < 3; d++) st2(o.p[d] + 2 * c, V2<T>{0, 0});
     return;
   }
   const T kx = (T)(k.kx * g.wave_x), ky = (T)(k.ky * g.wave_y), kz = (T)(k.kz * g.wave_z);
   const T k2 = kx * kx + ky * ky + kz * kz;
   const T ik2 = k2 > (T)0 ? (T)1 / k2 : (T)0;
-  V2<T> a = ld2(u0 + 2 * c), b = ld2(u1 + 2 * c), e = ld2(u2 + 2 * c);
+  V2<T> a = ld2(u.p[0] + 2 * c), b = ld2(u.p[1] + 2 * c), e = ld2(u.p[2] + 2 * c);
   V2<T> dmy = {ik2 * (a.re * kx + b.re * ky + e.re * kz), ik2 * (a.im * kx + b.im * ky + e.im * kz)};
-  st2(u0 + 2 * c, a - scal(kx, dmy));
-  st2(u1 + 2 * c, b - scal(ky, dmy));
-  st2(u2 + 2 * c, e - scal(kz, dmy));
+  st2(o.p[0] + 2 * c, a - scal(kx, dmy));
+  st2(o.p[1] + 2 * c, b - scal(ky, dmy));
+  st2(o.p[2] + 2 * c, e - scal(kz, dmy));
 }
 
 // Make_f_particle_dt_nonsole (f_particle.h:37-60): f = up - phi u
@@ -313,20 +312,21 @@ static void truncate_t(kapsel_ctx *ctx, void *const *fields, int nf) {
 void truncate(kapsel_ctx *ctx, void *const *fields, int nf) { KAPSEL_DISPATCH(ctx, truncate_t, ctx, fields, nf); }
 
 template <typename T>
-static void zeta_k2u_k_t(kapsel_ctx *ctx, int trunc, int fix_mask) {
+static void zeta_k2u_k_t(kapsel_ctx *ctx, int trunc, int fix_mask, void *const *out3) {
   LaunchTimer lt(ctx, KAPSEL_K_POINTWISE,
                  (5.0 * ctx->g.f + (trunc ? 4.0 * cut_count(ctx->g) : 0.0)) * sizeof(T));
+  void *const *u = out3 ? out3 : ctx->fields + KAPSEL_U0;
   hipLaunchKernelGGL(zeta_k2u_k_kernel<T>, dim3(blocks_for(ctx->g.nc, TPB)), dim3(TPB), 0, ctx->stream,
-                     F<T>(ctx, KAPSEL_ZETA0), F<T>(ctx, KAPSEL_ZETA1), F<T>(ctx, KAPSEL_U0),
-                     F<T>(ctx, KAPSEL_U1), F<T>(ctx, KAPSEL_U2), ctx->d_uk_dc, ctx->g, trunc, fix_mask);
+                     F<T>(ctx, KAPSEL_ZETA0), F<T>(ctx, KAPSEL_ZETA1), (T *)u[0], (T *)u[1], (T *)u[2],
+                     ctx->d_uk_dc, ctx->g, trunc, fix_mask);
   KAPSEL_LAUNCH_CHECK();
 }
 // fix_dc < 0: plain Zeta_k2u_k; otherwise truncate zeta first and apply the
 // FIX_CELL mask fix_dc.
-void zeta_k2u_k(kapsel_ctx *ctx, int fix_dc) {
+void zeta_k2u_k(kapsel_ctx *ctx, int fix_dc, void *const *out3) {
   int trunc = fix_dc >= 0;
   int mask = fix_dc >= 0 ? fix_dc : 0;
-  KAPSEL_DISPATCH(ctx, zeta_k2u_k_t, ctx, trunc, mask);
+  KAPSEL_DISPATCH(ctx, zeta_k2u_k_t, ctx, trunc, mask, out3);
 }
 
 template <typename T>
@@ -391,14 +391,16 @@ void advection_combine_euler(kapsel_ctx *ctx, double dt_fluid) {
 }
 
 template <typename T>
-static void solenoidal_uk_t(kapsel_ctx *ctx, void *const *f3) {
+static void solenoidal_uk_t(kapsel_ctx *ctx, void *const *f3, void *const *out3) {
   LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, (3.0 * ctx->g.f + 6.0 * kept_count(ctx->g)) * sizeof(T));
   hipLaunchKernelGGL(solenoidal_uk_kernel<T>, dim3(blocks_for(ctx->g.nc, TPB)), dim3(TPB), 0, ctx->stream,
-                     (T *)f3[0], (T *)f3[1], (T *)f3[2], ctx->g, 1);
+                     p5<T>(f3, 3), p5<T>(out3 ? out3 : f3, 3), ctx->g, 1);
   KAPSEL_LAUNCH_CHECK();
 }
 // truncation followed by the projection, as Solenoidal_u does in k-space
-void solenoidal_uk(kapsel_ctx *ctx, void *const *f3) { KAPSEL_DISPATCH(ctx, solenoidal_uk_t, ctx, f3); }
+void solenoidal_uk(kapsel_ctx *ctx, void *const *f3, void *const *out3) {
+  KAPSEL_DISPATCH(ctx, solenoidal_uk_t, ctx, f3, out3);
+}
 
 template <typename T>
 static void make_f_nonsole_t(kapsel_ctx *ctx) {
