@@ -306,9 +306,12 @@ __global__ __launch_bounds__(RPB *FS<NZ / 2>::R2) void fft_z_c2r(FieldPtrs<T> fp
 // ------------------------------------------------------------ plane passes
 // The z and y transforms of one x-plane in one workgroup.  Plane i is the
 // NY rows of H = NZ/2+1 complex values at complex offset i*NY*H, contiguous
-// in memory; it is read once into LDS, the row (z) and column (y)
-// transforms run out of LDS, and it is written once: one read and one write
-// of the field for two of the three axes.  Each line's four-step exchange
+// in memory; it is read once, the row (z) and column (y) transforms run out
+// of LDS, and it is written once: one read and one write of the field for
+// two of the three axes.  Where the column transform comes first its first
+// stage reads the plane from global memory, where it comes last its last
+// stage writes it there; a row transform first or last goes through a
+// coalesced plane copy to or from LDS.  Each line's four-step exchange
 // goes through the line's own LDS slots (its inputs are all in registers by
 // then), so the plane is the only LDS a workgroup holds.  Used where the
 // plane fits (gfx950: 160 KB of LDS per workgroup).
@@ -373,22 +376,34 @@ __device__ __forceinline__ void plane_tw_load(PlaneTw<T, NY, NZ> &tw, const V2<T
   for (int e = threadIdx.x; e < NY; e += THREADS) tw.twy[e] = twy[e];
 }
 
-// y transform of every column of the plane in P (in place), direction DIR
-template <typename T, int NY, int NZ, int THREADS, int DIR>
-__device__ __forceinline__ void plane_columns(V2<T> *P, const V2<T> *__restrict__ twy) {
+// y transform of every column of the plane in P (in place), direction DIR.
+// GIN: the columns are read from the plane in global memory at g instead of
+// from P; GOUT: the results are written to g instead of P.  Consecutive
+// lanes take consecutive columns, so the lanes of a wavefront touch
+// consecutive LDS words (the column stride H would alias banks) and, in
+// global memory, one contiguous run: the plane needs no pass through LDS of
+// its own before the first column stage or after the last.
+template <typename T, int NY, int NZ, int THREADS, int DIR, bool GIN = false, bool GOUT = false>
+__device__ __forceinline__ void plane_columns(V2<T> *P, const V2<T> *__restrict__ twy, T *g = nullptr) {
   using C = PlaneCfg<T, NY, NZ, THREADS>;
   constexpr int H = C::H, R1 = C::YR1, R2 = C::YR2;
-  // consecutive lanes take consecutive columns: the lanes of a wavefront
-  // touch consecutive LDS words (the column stride H would alias banks)
   const int t = threadIdx.x;
   const int cl = t % C::YCOLS, n2 = t / C::YCOLS;
   for (int c0 = 0; c0 < H; c0 += C::YCOLS) {
     const int kz = c0 + cl;
     const bool valid = kz < H;
     V2<T> *col = P + (valid ? kz : 0);
+    T *gcol = g + 2 * (valid ? kz : 0);
     V2<T> v[R1], o[R1];
 #pragma unroll
-    for (int n1 = 0; n1 < R1; n1++) v[n1] = valid ? col[(R2 * n1 + n2) * H] : V2<T>{0, 0};
+    for (int n1 = 0; n1 < R1; n1++) {
+      if constexpr (GIN) {
+        const V2<T> a = ld2(gcol + 2 * (R2 * n1 + n2) * H);
+        v[n1] = valid ? a : V2<T>{0, 0};
+      } else {
+        v[n1] = valid ? col[(R2 * n1 + n2) * H] : V2<T>{0, 0};
+      }
+    }
     dft<R1, 1, DIR>(v, o);
 #pragma unroll
     for (int k1 = 1; k1 < R1; k1++) o[k1] = cmul(o[k1], tw_get<DIR>(twy, n2 * k1));
@@ -408,7 +423,10 @@ __device__ __forceinline__ void plane_columns(V2<T> *P, const V2<T> *__restrict_
         for (int m = 0; m < R2; m++) w[m] = col[(m * R1 + k1) * H];
         dft<R2, 1, DIR>(w, x);
 #pragma unroll
-        for (int k2 = 0; k2 < R2; k2++) col[(k1 + R1 * k2) * H] = x[k2];
+        for (int k2 = 0; k2 < R2; k2++) {
+          if constexpr (GOUT) st2(gcol + 2 * (k1 + R1 * k2) * H, x[k2]);
+          else col[(k1 + R1 * k2) * H] = x[k2];
+        }
       }
     }
     __syncthreads();
@@ -547,8 +565,7 @@ __global__ __launch_bounds__(THREADS) void fft_plane_r2c(FieldPtrs<T> fp, const 
   plane_tw_load<T, NY, NZ, THREADS>(tw, twm, zrot, twy);
   __syncthreads();
   plane_rows_r2c<T, NY, NZ, THREADS>(P, tw);
-  plane_columns<T, NY, NZ, THREADS, -1>(P, tw.twy);
-  plane_store<T, C::PLANE, THREADS>(gp, P);
+  plane_columns<T, NY, NZ, THREADS, -1, false, true>(P, tw.twy, gp);
 }
 
 template <typename T, int NY, int NZ, int THREADS>
@@ -561,10 +578,9 @@ __global__ __launch_bounds__(THREADS) void fft_plane_c2r(FieldPtrs<T> fp, const 
   unsigned field;
   const unsigned plane = xcd_tile(field);
   T *__restrict__ gp = fp.p[field] + (size_t)plane * 2 * C::PLANE;
-  plane_load<T, C::PLANE, THREADS>(gp, P);
   plane_tw_load<T, NY, NZ, THREADS>(tw, twm, zrot, twy);
   __syncthreads();
-  plane_columns<T, NY, NZ, THREADS, +1>(P, tw.twy);
+  plane_columns<T, NY, NZ, THREADS, +1, true, false>(P, tw.twy, gp);
   plane_rows_c2r<T, NY, NZ, THREADS>(P, tw, scale);
   plane_store<T, C::PLANE, THREADS>(gp, P);
 }
@@ -585,10 +601,9 @@ __global__ __launch_bounds__(THREADS) void fft_plane_face(FieldPtrs<T> fp, const
   unsigned field;
   const unsigned plane = xcd_tile(field);
   T *__restrict__ gp = fp.p[field] + (size_t)plane * 2 * C::PLANE;
-  plane_load<T, C::PLANE, THREADS>(gp, P);
   plane_tw_load<T, NY, NZ, THREADS>(tw, twm, zrot, twy);
   __syncthreads();
-  plane_columns<T, NY, NZ, THREADS, +1>(P, tw.twy);
+  plane_columns<T, NY, NZ, THREADS, +1, true, false>(P, tw.twy, gp);
   // the face permittivity of this plane, in flight during the row transforms
   constexpr int NV = C::PLANE * (int)sizeof(V2<T>) / 16;
   constexpr int IT = (NV + THREADS - 1) / THREADS;
@@ -613,8 +628,7 @@ __global__ __launch_bounds__(THREADS) void fft_plane_face(FieldPtrs<T> fp, const
   }
   __syncthreads();
   plane_rows_r2c<T, NY, NZ, THREADS>(P, tw);
-  plane_columns<T, NY, NZ, THREADS, -1>(P, tw.twy);
-  plane_store<T, C::PLANE, THREADS>(gp, P);
+  plane_columns<T, NY, NZ, THREADS, -1, false, true>(P, tw.twy, gp);
 }
 
 // One z row of each of the three staggered-gradient spectra (after the x
