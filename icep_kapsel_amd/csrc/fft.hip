@@ -308,60 +308,40 @@ __global__ __launch_bounds__(RPB *FS<NZ / 2>::R2) void fft_z_c2r(FieldPtrs<T> fp
 // NY rows of H = NZ/2+1 complex values at complex offset i*NY*H, contiguous
 // in memory; it is read once, the row (z) and column (y) transforms run out
 // of LDS, and it is written once: one read and one write of the field for
-// two of the three axes.  Where the column transform comes first its first
-// stage reads the plane from global memory, where it comes last its last
-// stage writes it there; a row transform first or last goes through a
-// coalesced plane copy to or from LDS.  Each line's four-step exchange
-// goes through the line's own LDS slots (its inputs are all in registers by
-// then), so the plane is the only LDS a workgroup holds.  Used where the
-// plane fits (gfx950: 160 KB of LDS per workgroup).
+// two of the three axes.  A transform that comes first reads the plane from
+// global memory in its first stage, one that comes last writes it there in
+// its last stage; only the hand-over between the row and the column
+// transforms goes through the LDS plane.  Used where the plane fits
+// (gfx950: 160 KB of LDS per CU; at 128^3 FP32 two workgroups share a CU).
+//
+// Rows: the ZR2 lanes of a row sit in one wavefront (lane = row * ZR2 + n2;
+// RPW rows per wavefront), so a row's four-step exchange is ordered by the
+// wavefront itself and needs no workgroup barrier.  LDS rows are HP complex
+// values apart: room for the exchange's padded slots, and rows 2 ZR2 words
+// apart in the banks, so the 16 lanes of a quarter-wavefront (16 / ZR2 rows)
+// meet 32 different banks.
+constexpr int row_pitch(int need, int r2) {
+  int h = need;
+  while (h % 16 != r2 % 16) h++;
+  return h;
+}
+
 template <typename T, int NY, int NZ, int THREADS>
 struct PlaneCfg {
   static constexpr int M = NZ / 2, H = NZ / 2 + 1;
   static constexpr int ZR1 = FS<M>::R1, ZR2 = FS<M>::R2;
-  static constexpr int ZROWS = THREADS / ZR2;  // rows per round
+  static constexpr int RPW = 64 / ZR2;     // rows per wavefront
+  static constexpr int NW = THREADS / 64;  // wavefronts
   static constexpr int YR1 = FS<NY>::R1, YR2 = FS<NY>::R2;
   static constexpr int YCOLS = THREADS / YR2;  // columns per round
-  static constexpr int PLANE = NY * H;
-  static constexpr size_t LDS_BYTES = sizeof(V2<T>) * (PLANE + NZ + 1 + NY);
+  static constexpr int XS = ZR2 * (ZR1 + 1);  // a row's exchange slots
+  static constexpr int HP = row_pitch(H > XS ? H : XS, ZR2);
+  static constexpr int PLANE = NY * H;    // complex values of a plane in global memory
+  static constexpr int LPLANE = NY * HP;  // and in LDS
+  static constexpr size_t LDS_BYTES = sizeof(V2<T>) * (LPLANE + NZ + 1 + NY);
+  static_assert(64 % ZR2 == 0 && THREADS % 64 == 0 && NY % RPW == 0, "row tiling");
+  static_assert(THREADS % YR2 == 0, "column tiling");
 };
-
-// Plane <-> LDS copies with 16-byte accesses, every load of a thread issued
-// before the first LDS store (the copy is latency-bound otherwise)
-template <typename T, int PLANE, int THREADS>
-__device__ __forceinline__ void plane_load(const T *__restrict__ gp, V2<T> *P) {
-  constexpr int NV = PLANE * (int)sizeof(V2<T>) / 16;
-  static_assert(PLANE * sizeof(V2<T>) % 16 == 0, "plane size");
-  constexpr int IT = (NV + THREADS - 1) / THREADS;
-  const int4 *src = reinterpret_cast<const int4 *>(gp);
-  int4 *dst = reinterpret_cast<int4 *>(P);
-  int4 r[IT];
-#pragma unroll
-  for (int i = 0; i < IT; i++) {
-    // unconditional (clamped) loads: a conditionally assigned array is
-    // placed in scratch memory and its loads serialise
-    const int idx = threadIdx.x + i * THREADS;
-    r[i] = src[idx < NV ? idx : NV - 1];
-  }
-#pragma unroll
-  for (int i = 0; i < IT; i++) {
-    const int idx = threadIdx.x + i * THREADS;
-    if (idx < NV) dst[idx] = r[i];
-  }
-}
-
-template <typename T, int PLANE, int THREADS>
-__device__ __forceinline__ void plane_store(T *__restrict__ gp, const V2<T> *P) {
-  constexpr int NV = PLANE * (int)sizeof(V2<T>) / 16;
-  constexpr int IT = (NV + THREADS - 1) / THREADS;
-  const int4 *src = reinterpret_cast<const int4 *>(P);
-  int4 *dst = reinterpret_cast<int4 *>(gp);
-#pragma unroll
-  for (int i = 0; i < IT; i++) {
-    const int idx = threadIdx.x + i * THREADS;
-    if (idx < NV) dst[idx] = src[idx];
-  }
-}
 
 // the twiddle tables of a plane pass, copied to LDS
 template <typename T, int NY, int NZ>
@@ -376,19 +356,110 @@ __device__ __forceinline__ void plane_tw_load(PlaneTw<T, NY, NZ> &tw, const V2<T
   for (int e = threadIdx.x; e < NY; e += THREADS) tw.twy[e] = twy[e];
 }
 
+// Every lane's LDS accesses before this point precede every lane's after
+// it: the LDS operations of one wavefront are performed in issue order, and
+// the fences keep the compiler from moving accesses across.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Four-step DFT of one row of length R1*R2 held by its R2 lanes: lane n2
+// holds v[n1] = z[n2 + R2 n1]; out(n1, Z) receives Z[n2 + R2 n1] of the
+// transform.  The exchange uses the row's padded slots 0 .. R2 (R1+1) - 1,
+// and every lane has read its slots before out() is called.
+template <typename T, int R1, int R2, int DIR, typename Out>
+__device__ __forceinline__ void row_four_step(V2<T> *row, V2<T> (&v)[R1], const V2<T> *tw, int n2, Out out) {
+  V2<T> o[R1];
+  dft<R1, 1, DIR>(v, o);
+#pragma unroll
+  for (int k1 = 1; k1 < R1; k1++) o[k1] = cmul(o[k1], tw_get<DIR>(tw, n2 * k1));
+  wave_lds_sync();  // every lane's earlier reads of the row are done
+#pragma unroll
+  for (int k1 = 0; k1 < R1; k1++) row[n2 * (R1 + 1) + k1] = o[k1];
+  wave_lds_sync();
+  V2<T> res[R1];
+#pragma unroll
+  for (int c = 0; c < R1 / R2; c++) {
+    const int k1 = n2 + R2 * c;
+    V2<T> w[R2], x[R2];
+#pragma unroll
+    for (int m = 0; m < R2; m++) w[m] = row[m * (R1 + 1) + k1];
+    dft<R2, 1, DIR>(w, x);
+    // output k1 + R1 k2 = n2 + R2 (c + (R1/R2) k2)
+#pragma unroll
+    for (int k2 = 0; k2 < R2; k2++) res[c + (R1 / R2) * k2] = x[k2];
+  }
+  wave_lds_sync();  // every lane's exchange reads are done
+#pragma unroll
+  for (int n1 = 0; n1 < R1; n1++) out(n1, res[n1]);
+}
+
+// z r2c of one row: v[n1] = z[n2 + R2 n1] = a[2n] + i a[2n+1] (the packed
+// reals); a length-M complex FFT, then X[k] = E + W^k O with E = (Z[k] +
+// conj Z[M-k]) / 2, O = (Z[k] - conj Z[M-k]) / 2i, k = 0..M, into row[k]
+template <typename T, int M, int R1, int R2>
+__device__ __forceinline__ void row_r2c(V2<T> *row, V2<T> (&v)[R1], const V2<T> *twm, const V2<T> *zrot, int n2) {
+  row_four_step<T, R1, R2, -1>(row, v, twm, n2, [&](int n1, V2<T> z) { row[n2 + R2 * n1] = z; });
+  wave_lds_sync();
+  constexpr int KPT = M / R2 + 1;  // split outputs per lane (at most)
+  V2<T> zk[KPT], zm[KPT];
+#pragma unroll
+  for (int q = 0; q < KPT; q++) {
+    const int k = n2 + R2 * q;
+    if (k <= M) {
+      zk[q] = row[k & (M - 1)];
+      zm[q] = conjv(row[(M - k) & (M - 1)]);
+    }
+  }
+  wave_lds_sync();
+#pragma unroll
+  for (int q = 0; q < KPT; q++) {
+    const int k = n2 + R2 * q;
+    if (k <= M) {
+      const V2<T> e = scal((T)0.5, zk[q] + zm[q]);
+      const V2<T> d = zk[q] - zm[q];
+      const V2<T> od = {(T)0.5 * d.im, (T)-0.5 * d.re};
+      row[k] = e + cmul(zrot[k], od);
+    }
+  }
+}
+
+// z c2r of one row (scaled): X[k] = row[k], k = 0..M, with X[0] and X[M]
+// taken as real; out(n1, z) receives z[n2 + R2 n1] = (a[2n], a[2n+1])
+template <typename T, int M, int R1, int R2, typename Out>
+__device__ __forceinline__ void row_c2r(V2<T> *row, const V2<T> *twm, const V2<T> *zrot, T scale, int n2, Out out) {
+  V2<T> v[R1];
+#pragma unroll
+  for (int n1 = 0; n1 < R1; n1++) {
+    const int n = R2 * n1 + n2;
+    V2<T> xn = row[n];
+    V2<T> xm = row[M - n];
+    if (n == 0) {
+      xn.im = 0;
+      xm.im = 0;  // M - n = M
+    }
+    xm = conjv(xm);
+    const V2<T> e = xn + xm;                         // 2 E[n]
+    const V2<T> od = cmul(xn - xm, conjv(zrot[n]));  // 2 O[n]
+    v[n1] = V2<T>{e.re - od.im, e.im + od.re};       // E + i O
+  }
+  row_four_step<T, R1, R2, +1>(row, v, twm, n2, [&](int n1, V2<T> z) { out(n1, scal(scale, z)); });
+}
+
 // y transform of every column of the plane in P (in place), direction DIR.
 // GIN: the columns are read from the plane in global memory at g instead of
 // from P; GOUT: the results are written to g instead of P.  Consecutive
 // lanes take consecutive columns, so the lanes of a wavefront touch
-// consecutive LDS words (the column stride H would alias banks) and, in
-// global memory, one contiguous run: the plane needs no pass through LDS of
-// its own before the first column stage or after the last.
+// consecutive LDS words and, in global memory, one contiguous run.
 template <typename T, int NY, int NZ, int THREADS, int DIR, bool GIN = false, bool GOUT = false>
 __device__ __forceinline__ void plane_columns(V2<T> *P, const V2<T> *__restrict__ twy, T *g = nullptr) {
   using C = PlaneCfg<T, NY, NZ, THREADS>;
-  constexpr int H = C::H, R1 = C::YR1, R2 = C::YR2;
+  constexpr int H = C::H, HP = C::HP, R1 = C::YR1, R2 = C::YR2;
   const int t = threadIdx.x;
   const int cl = t % C::YCOLS, n2 = t / C::YCOLS;
+#pragma unroll 1  // one round's registers at a time
   for (int c0 = 0; c0 < H; c0 += C::YCOLS) {
     const int kz = c0 + cl;
     const bool valid = kz < H;
@@ -401,7 +472,7 @@ __device__ __forceinline__ void plane_columns(V2<T> *P, const V2<T> *__restrict_
         const V2<T> a = ld2(gcol + 2 * (R2 * n1 + n2) * H);
         v[n1] = valid ? a : V2<T>{0, 0};
       } else {
-        v[n1] = valid ? col[(R2 * n1 + n2) * H] : V2<T>{0, 0};
+        v[n1] = valid ? col[(R2 * n1 + n2) * HP] : V2<T>{0, 0};
       }
     }
     dft<R1, 1, DIR>(v, o);
@@ -410,7 +481,7 @@ __device__ __forceinline__ void plane_columns(V2<T> *P, const V2<T> *__restrict_
     __syncthreads();  // every column of the round is in registers
     if (valid) {
 #pragma unroll
-      for (int k1 = 0; k1 < R1; k1++) col[(n2 * R1 + k1) * H] = o[k1];
+      for (int k1 = 0; k1 < R1; k1++) col[(n2 * R1 + k1) * HP] = o[k1];
     }
     __syncthreads();
     if (valid) {
@@ -420,12 +491,12 @@ __device__ __forceinline__ void plane_columns(V2<T> *P, const V2<T> *__restrict_
         const int k1 = n2 + R2 * c;
         V2<T> w[R2], x[R2];
 #pragma unroll
-        for (int m = 0; m < R2; m++) w[m] = col[(m * R1 + k1) * H];
+        for (int m = 0; m < R2; m++) w[m] = col[(m * R1 + k1) * HP];
         dft<R2, 1, DIR>(w, x);
 #pragma unroll
         for (int k2 = 0; k2 < R2; k2++) {
           if constexpr (GOUT) st2(gcol + 2 * (k1 + R1 * k2) * H, x[k2]);
-          else col[(k1 + R1 * k2) * H] = x[k2];
+          else col[(k1 + R1 * k2) * HP] = x[k2];
         }
       }
     }
@@ -433,147 +504,46 @@ __device__ __forceinline__ void plane_columns(V2<T> *P, const V2<T> *__restrict_
   }
 }
 
-// z: r2c of every row of the plane in P (in place) -- a length-M complex FFT
-// of the packed reals, then X[k] = E + W^k O with E = (Z[k] + conj Z[M-k]) / 2,
-// O = (Z[k] - conj Z[M-k]) / 2i.  Consecutive lanes take consecutive rows.
-template <typename T, int NY, int NZ, int THREADS>
-__device__ __forceinline__ void plane_rows_r2c(V2<T> *P, const PlaneTw<T, NY, NZ> &tw) {
-  using C = PlaneCfg<T, NY, NZ, THREADS>;
-  constexpr int M = C::M, H = C::H, R1 = C::ZR1, R2 = C::ZR2;
-  constexpr int KPT = M / R2 + 1;  // split outputs per thread (at most)
-  const int t = threadIdx.x;
-  const int rl = t % C::ZROWS, n2 = t / C::ZROWS;
-  for (int r0 = 0; r0 < NY; r0 += C::ZROWS) {
-    const int j = r0 + rl;
-    const bool valid = j < NY;
-    V2<T> *row = P + (valid ? j : 0) * H;
-    V2<T> v[R1], o[R1];
-#pragma unroll
-    for (int n1 = 0; n1 < R1; n1++) v[n1] = valid ? row[R2 * n1 + n2] : V2<T>{0, 0};
-    dft<R1, 1, -1>(v, o);
-#pragma unroll
-    for (int k1 = 1; k1 < R1; k1++) o[k1] = cmul(o[k1], tw_get<-1>(tw.twm, n2 * k1));
-    __syncthreads();
-    if (valid) {
-#pragma unroll
-      for (int k1 = 0; k1 < R1; k1++) row[n2 * R1 + k1] = o[k1];
-    }
-    __syncthreads();
-    if (valid) {
-#pragma unroll
-      for (int c = 0; c < R1 / R2; c++) {
-        const int k1 = n2 + R2 * c;
-        V2<T> w[R2], x[R2];
-#pragma unroll
-        for (int m = 0; m < R2; m++) w[m] = row[m * R1 + k1];
-        dft<R2, 1, -1>(w, x);
-#pragma unroll
-        for (int k2 = 0; k2 < R2; k2++) row[k1 + R1 * k2] = x[k2];  // Z[k], natural order
-      }
-    }
-    __syncthreads();
-    V2<T> zk[KPT], zm[KPT];
-#pragma unroll
-    for (int q = 0; q < KPT; q++) {
-      const int k = n2 + R2 * q;
-      if (valid && k <= M) {
-        zk[q] = row[k & (M - 1)];
-        zm[q] = conjv(row[(M - k) & (M - 1)]);
-      }
-    }
-    __syncthreads();
-    if (valid) {
-#pragma unroll
-      for (int q = 0; q < KPT; q++) {
-        const int k = n2 + R2 * q;
-        if (k <= M) {
-          const V2<T> e = scal((T)0.5, zk[q] + zm[q]);
-          const V2<T> d = zk[q] - zm[q];
-          const V2<T> od = {(T)0.5 * d.im, (T)-0.5 * d.re};
-          row[k] = e + cmul(tw.zrot[k], od);
-        }
-      }
-    }
-    __syncthreads();
-  }
-}
+// The rows of lane t: (t / 64) RPW + (t % 64) / ZR2, then every NW RPW rows
+// further.  NY is a multiple of RPW, so all lanes of a wavefront run the
+// loop alike.
+#define KAPSEL_PLANE_ROWS(C)                        \
+  const int n2 = threadIdx.x % 64 % C::ZR2;         \
+  for (int j = threadIdx.x / 64 * C::RPW + threadIdx.x % 64 / C::ZR2; j < NY; j += C::NW * C::RPW)
 
-// z: c2r of every row of the plane in P (in place), scaled; the kz = 0 and
-// kz = NZ/2 entries are taken as real and the row padding is zeroed
-template <typename T, int NY, int NZ, int THREADS>
-__device__ __forceinline__ void plane_rows_c2r(V2<T> *P, const PlaneTw<T, NY, NZ> &tw, T scale) {
-  using C = PlaneCfg<T, NY, NZ, THREADS>;
-  constexpr int M = C::M, H = C::H, R1 = C::ZR1, R2 = C::ZR2;
-  const int t = threadIdx.x;
-  const int rl = t % C::ZROWS, n2 = t / C::ZROWS;
-  for (int r0 = 0; r0 < NY; r0 += C::ZROWS) {
-    const int j = r0 + rl;
-    const bool valid = j < NY;
-    V2<T> *row = P + (valid ? j : 0) * H;
-    V2<T> v[R1], o[R1];
-#pragma unroll
-    for (int n1 = 0; n1 < R1; n1++) {
-      const int n = R2 * n1 + n2;
-      V2<T> xn = valid ? row[n] : V2<T>{0, 0};
-      V2<T> xm = valid ? row[M - n] : V2<T>{0, 0};
-      if (n == 0) {
-        xn.im = 0;
-        xm.im = 0;  // M - n = M
-      }
-      xm = conjv(xm);
-      const V2<T> e = xn + xm;                            // 2 E[n]
-      const V2<T> od = cmul(xn - xm, conjv(tw.zrot[n]));  // 2 O[n]
-      v[n1] = V2<T>{e.re - od.im, e.im + od.re};          // E + i O
-    }
-    dft<R1, 1, +1>(v, o);
-#pragma unroll
-    for (int k1 = 1; k1 < R1; k1++) o[k1] = cmul(o[k1], tw_get<+1>(tw.twm, n2 * k1));
-    __syncthreads();  // every row of the round is in registers
-    if (valid) {
-#pragma unroll
-      for (int k1 = 0; k1 < R1; k1++) row[n2 * R1 + k1] = o[k1];
-    }
-    __syncthreads();
-    if (valid) {
-#pragma unroll
-      for (int c = 0; c < R1 / R2; c++) {
-        const int k1 = n2 + R2 * c;
-        V2<T> w[R2], x[R2];
-#pragma unroll
-        for (int m = 0; m < R2; m++) w[m] = row[m * R1 + k1];
-        dft<R2, 1, +1>(w, x);
-#pragma unroll
-        for (int k2 = 0; k2 < R2; k2++) row[k1 + R1 * k2] = scal(scale, x[k2]);
-      }
-      if (n2 == 0) row[M] = V2<T>{0, 0};
-    }
-    __syncthreads();
-  }
-}
+// the plane pass kernels take two workgroups per CU at 128^3 FP32: five
+// wavefronts per SIMD (at most 96 VGPRs)
+#define KAPSEL_PLANE_BOUNDS(THREADS) __launch_bounds__(THREADS, (2 * (THREADS) / 64 + 3) / 4)
 
 template <typename T, int NY, int NZ, int THREADS>
-__global__ __launch_bounds__(THREADS) void fft_plane_r2c(FieldPtrs<T> fp, const V2<T> *__restrict__ twm,
-                                                         const V2<T> *__restrict__ zrot,
-                                                         const V2<T> *__restrict__ twy) {
+__global__ KAPSEL_PLANE_BOUNDS(THREADS) void fft_plane_r2c(FieldPtrs<T> fp, const V2<T> *__restrict__ twm,
+                                                           const V2<T> *__restrict__ zrot,
+                                                           const V2<T> *__restrict__ twy) {
   using C = PlaneCfg<T, NY, NZ, THREADS>;
-  __shared__ __align__(16) V2<T> P[C::PLANE];
+  __shared__ __align__(16) V2<T> P[C::LPLANE];
   __shared__ PlaneTw<T, NY, NZ> tw;
   unsigned field;
   const unsigned plane = xcd_tile(field);
   T *__restrict__ gp = fp.p[field] + (size_t)plane * 2 * C::PLANE;
-  plane_load<T, C::PLANE, THREADS>(gp, P);
   plane_tw_load<T, NY, NZ, THREADS>(tw, twm, zrot, twy);
   __syncthreads();
-  plane_rows_r2c<T, NY, NZ, THREADS>(P, tw);
+  KAPSEL_PLANE_ROWS(C) {
+    const T *grow = gp + (size_t)j * 2 * C::H;
+    V2<T> v[C::ZR1];
+#pragma unroll
+    for (int n1 = 0; n1 < C::ZR1; n1++) v[n1] = ld2(grow + 2 * (C::ZR2 * n1 + n2));
+    row_r2c<T, C::M, C::ZR1, C::ZR2>(P + j * C::HP, v, tw.twm, tw.zrot, n2);
+  }
+  __syncthreads();
   plane_columns<T, NY, NZ, THREADS, -1, false, true>(P, tw.twy, gp);
 }
 
 template <typename T, int NY, int NZ, int THREADS>
-__global__ __launch_bounds__(THREADS) void fft_plane_c2r(FieldPtrs<T> fp, const V2<T> *__restrict__ twm,
-                                                         const V2<T> *__restrict__ zrot,
-                                                         const V2<T> *__restrict__ twy, T scale) {
+__global__ KAPSEL_PLANE_BOUNDS(THREADS) void fft_plane_c2r(FieldPtrs<T> fp, const V2<T> *__restrict__ twm,
+                                                           const V2<T> *__restrict__ zrot,
+                                                           const V2<T> *__restrict__ twy, T scale) {
   using C = PlaneCfg<T, NY, NZ, THREADS>;
-  __shared__ __align__(16) V2<T> P[C::PLANE];
+  __shared__ __align__(16) V2<T> P[C::LPLANE];
   __shared__ PlaneTw<T, NY, NZ> tw;
   unsigned field;
   const unsigned plane = xcd_tile(field);
@@ -581,53 +551,48 @@ __global__ __launch_bounds__(THREADS) void fft_plane_c2r(FieldPtrs<T> fp, const 
   plane_tw_load<T, NY, NZ, THREADS>(tw, twm, zrot, twy);
   __syncthreads();
   plane_columns<T, NY, NZ, THREADS, +1, true, false>(P, tw.twy, gp);
-  plane_rows_c2r<T, NY, NZ, THREADS>(P, tw, scale);
-  plane_store<T, C::PLANE, THREADS>(gp, P);
+  KAPSEL_PLANE_ROWS(C) {
+    T *grow = gp + (size_t)j * 2 * C::H;
+    row_c2r<T, C::M, C::ZR1, C::ZR2>(P + j * C::HP, tw.twm, tw.zrot, scale, n2,
+                                     [&](int n1, V2<T> z) { st2(grow + 2 * (C::ZR2 * n1 + n2), z); });
+    if (n2 == 0) st2(grow + 2 * C::M, V2<T>{0, 0});  // the row padding
+  }
 }
 
 // The middle of the dielectric operator on one plane of the staggered
 // gradient field d (fields 0, 1, 2 = x, y, z): inverse z and y transforms,
 // times the face permittivity (a real field of the same layout), forward z
 // and y transforms -- one read and one write where two plane passes and a
-// pointwise pass would make three of each.
+// pointwise pass would make three of each.  A row goes from its inverse z
+// transform through the face product into its forward z transform in the
+// registers of its own lanes.
 template <typename T, int NY, int NZ, int THREADS>
-__global__ __launch_bounds__(THREADS) void fft_plane_face(FieldPtrs<T> fp, const V2<T> *__restrict__ twm,
-                                                          const V2<T> *__restrict__ zrot,
-                                                          const V2<T> *__restrict__ twy, T scale,
-                                                          FieldPtrs<T> face) {
+__global__ KAPSEL_PLANE_BOUNDS(THREADS) void fft_plane_face(FieldPtrs<T> fp, const V2<T> *__restrict__ twm,
+                                                            const V2<T> *__restrict__ zrot,
+                                                            const V2<T> *__restrict__ twy, T scale,
+                                                            FieldPtrs<T> face) {
   using C = PlaneCfg<T, NY, NZ, THREADS>;
-  __shared__ __align__(16) V2<T> P[C::PLANE];
+  __shared__ __align__(16) V2<T> P[C::LPLANE];
   __shared__ PlaneTw<T, NY, NZ> tw;
   unsigned field;
   const unsigned plane = xcd_tile(field);
   T *__restrict__ gp = fp.p[field] + (size_t)plane * 2 * C::PLANE;
+  const T *fpl = face.p[field] + (size_t)plane * 2 * C::PLANE;
   plane_tw_load<T, NY, NZ, THREADS>(tw, twm, zrot, twy);
   __syncthreads();
   plane_columns<T, NY, NZ, THREADS, +1, true, false>(P, tw.twy, gp);
-  // the face permittivity of this plane, in flight during the row transforms
-  constexpr int NV = C::PLANE * (int)sizeof(V2<T>) / 16;
-  constexpr int IT = (NV + THREADS - 1) / THREADS;
-  const int4 *fsrc = reinterpret_cast<const int4 *>(face.p[field] + (size_t)plane * 2 * C::PLANE);
-  int4 fr[IT];
+  KAPSEL_PLANE_ROWS(C) {
+    const T *frow = fpl + (size_t)j * 2 * C::H;
+    V2<T> fv[C::ZR1], v[C::ZR1];
 #pragma unroll
-  for (int i = 0; i < IT; i++) {
-    const int idx = threadIdx.x + i * THREADS;
-    fr[i] = fsrc[idx < NV ? idx : NV - 1];
-  }
-  plane_rows_c2r<T, NY, NZ, THREADS>(P, tw, scale);
-  constexpr int PER = 16 / (int)sizeof(T);
-#pragma unroll
-  for (int i = 0; i < IT; i++) {
-    const int idx = threadIdx.x + i * THREADS;
-    if (idx < NV) {
-      T *pv = reinterpret_cast<T *>(P) + (size_t)idx * PER;
-      const T *fv = reinterpret_cast<const T *>(&fr[i]);
-#pragma unroll
-      for (int e = 0; e < PER; e++) pv[e] *= fv[e];
-    }
+    for (int n1 = 0; n1 < C::ZR1; n1++) fv[n1] = ld2(frow + 2 * (C::ZR2 * n1 + n2));
+    V2<T> *row = P + j * C::HP;
+    row_c2r<T, C::M, C::ZR1, C::ZR2>(row, tw.twm, tw.zrot, scale, n2, [&](int n1, V2<T> z) {
+      v[n1] = V2<T>{z.re * fv[n1].re, z.im * fv[n1].im};
+    });
+    row_r2c<T, C::M, C::ZR1, C::ZR2>(row, v, tw.twm, tw.zrot, n2);
   }
   __syncthreads();
-  plane_rows_r2c<T, NY, NZ, THREADS>(P, tw);
   plane_columns<T, NY, NZ, THREADS, -1, false, true>(P, tw.twy, gp);
 }
 
@@ -1067,7 +1032,7 @@ bool plane_dispatch(kapsel_ctx *ctx, const FieldPtrs<T> &fp, int nf, bool fwd) {
       // an FP64 128 x 65 plane (133 KB) leaves one workgroup per CU, slower
       // than the separate z and y passes
       if constexpr (sizeof(T) == 8) return false;
-      launch_plane<T, 128, 576>(ctx, fp, nf, fwd);
+      launch_plane<T, 128, 512>(ctx, fp, nf, fwd);
       return true;
     default: return false;
   }
@@ -1171,7 +1136,7 @@ void fft_face_t(kapsel_ctx *ctx, void *const *g3, void *const *face3) {
       case 64: launch_plane_face<T, 64, 320>(ctx, fp, fc); done = true; break;
       case 128:
         if constexpr (sizeof(T) == 4) {
-          launch_plane_face<T, 128, 576>(ctx, fp, fc);
+          launch_plane_face<T, 128, 512>(ctx, fp, fc);
           done = true;
         }
         break;
@@ -1200,7 +1165,7 @@ void fft_face_mid_t(kapsel_ctx *ctx, void *const *g3, void *const *face3) {
       case 64: launch_plane_face<T, 64, 320>(ctx, fp, fc); return;
       case 128:
         if constexpr (sizeof(T) == 4) {
-          launch_plane_face<T, 128, 576>(ctx, fp, fc);
+          launch_plane_face<T, 128, 512>(ctx, fp, fc);
           return;
         }
         break;
