@@ -46,8 +46,8 @@ SINGLE_KERNEL_CLASSES = ["FFT_Z_R2C", "FFT_Z_C2R", "FFT_LINES_FWD", "FFT_LINES_I
 KERNEL_OF_CLASS = {  # the kernel symbol of each single-kernel class in an FP32 128^3 context
     "FFT_Z_R2C": "fft_plane_r2c<float, 128, 128, 512>",  # fused z + y plane pass
     "FFT_Z_C2R": "fft_plane_c2r<float, 128, 128, 512>",
-    "FFT_LINES_FWD": "fft_c2c_lines<float, 128, -1, 32>",
-    "FFT_LINES_INV": "fft_c2c_lines<float, 128, 1, 32>",
+    "FFT_LINES_FWD": "fft_c2c_lines_io<float, 128, -1, 32>",
+    "FFT_LINES_INV": "fft_c2c_lines_io<float, 128, 1, 32>",
     "FFT_PLANE_FACE": "fft_plane_face<float, 128, 128, 512>",  # inverse z+y, face eps, forward z+y
 }
 
