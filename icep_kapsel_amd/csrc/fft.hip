@@ -1015,7 +1015,7 @@ void launch_plane(kapsel_ctx *ctx, const FieldPtrs<T> &fp, int nf, bool fwd) {
 template <typename T>
 bool plane_ok(const Grid &g) {
   if (g.ny != g.nz) return false;
-  return g.nz == 32 || g.nz == 64 || (g.nz == 128 && sizeof(T) == 4);
+  return g.nz == 32 || g.nz == 64 || g.nz == 128;
 }
 
 // the z and y transforms as one plane pass where plane_ok; false: use the z
@@ -1029,9 +1029,7 @@ bool plane_dispatch(kapsel_ctx *ctx, const FieldPtrs<T> &fp, int nf, bool fwd) {
     case 32: launch_plane<T, 32, 128>(ctx, fp, nf, fwd); return true;
     case 64: launch_plane<T, 64, 320>(ctx, fp, nf, fwd); return true;
     case 128:
-      // an FP64 128 x 65 plane (133 KB) leaves one workgroup per CU, slower
-      // than the separate z and y passes
-      if constexpr (sizeof(T) == 8) return false;
+      // FP64: the padded plane (147 KB) leaves one workgroup per CU
       launch_plane<T, 128, 512>(ctx, fp, nf, fwd);
       return true;
     default: return false;
@@ -1134,12 +1132,7 @@ void fft_face_t(kapsel_ctx *ctx, void *const *g3, void *const *face3) {
     switch (g.nz) {
       case 32: launch_plane_face<T, 32, 128>(ctx, fp, fc); done = true; break;
       case 64: launch_plane_face<T, 64, 320>(ctx, fp, fc); done = true; break;
-      case 128:
-        if constexpr (sizeof(T) == 4) {
-          launch_plane_face<T, 128, 512>(ctx, fp, fc);
-          done = true;
-        }
-        break;
+      case 128: launch_plane_face<T, 128, 512>(ctx, fp, fc); done = true; break;
       default: break;
     }
   }
@@ -1163,12 +1156,7 @@ void fft_face_mid_t(kapsel_ctx *ctx, void *const *g3, void *const *face3) {
     switch (g.nz) {
       case 32: launch_plane_face<T, 32, 128>(ctx, fp, fc); return;
       case 64: launch_plane_face<T, 64, 320>(ctx, fp, fc); return;
-      case 128:
-        if constexpr (sizeof(T) == 4) {
-          launch_plane_face<T, 128, 512>(ctx, fp, fc);
-          return;
-        }
-        break;
+      case 128: launch_plane_face<T, 128, 512>(ctx, fp, fc); return;
       default: break;
     }
   }
