@@ -828,13 +828,21 @@ __global__ __launch_bounds__(LPB *FS<L>::R2) void fft_x_div(FieldPtrs<T> g3, con
 #pragma unroll
     for (int k2 = 0; k2 < R2; k2++) acc[c][k2] = V2<T>{0, 0};
   V2<T> *sl = sm + l * LS;
+  // direction d + 1's loads are in flight while direction d is transformed
+  // (base is 0 for an invalid line, so every load is in bounds)
+  V2<T> nxt[R1];
+#pragma unroll
+  for (int n1 = 0; n1 < R1; n1++) nxt[n1] = ld2(g3.p[0] + 2 * (base + (long long)(R2 * n1 + n2) * plane));
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    const T *__restrict__ gd = g3.p[d];
     V2<T> v[R1], o[R1];
 #pragma unroll
-    for (int n1 = 0; n1 < R1; n1++)
-      v[n1] = valid ? ld2(gd + 2 * (base + (long long)(R2 * n1 + n2) * plane)) : V2<T>{0, 0};
+    for (int n1 = 0; n1 < R1; n1++) v[n1] = valid ? nxt[n1] : V2<T>{0, 0};
+    if (d < 2) {
+#pragma unroll
+      for (int n1 = 0; n1 < R1; n1++)
+        nxt[n1] = ld2(g3.p[d + 1] + 2 * (base + (long long)(R2 * n1 + n2) * plane));
+    }
     dft<R1, 1, -1>(v, o);
 #pragma unroll
     for (int k1 = 1; k1 < R1; k1++) o[k1] = cmul(o[k1], tw_get<-1>(tw, n2 * k1));
