@@ -339,6 +339,8 @@ struct PlaneCfg {
   static constexpr int PLANE = NY * H;    // complex values of a plane in global memory
   static constexpr int LPLANE = NY * HP;  // and in LDS
   static constexpr size_t LDS_BYTES = sizeof(V2<T>) * (LPLANE + NZ + 1 + NY);
+  static constexpr int WGS_PER_CU = 2 * LDS_BYTES <= 160 * 1024 ? 2 : 1;
+  static constexpr int WAVES_PER_SIMD = (WGS_PER_CU * NW + 3) / 4;
   static_assert(64 % ZR2 == 0 && THREADS % 64 == 0 && NY % RPW == 0, "row tiling");
   static_assert(THREADS % YR2 == 0, "column tiling");
 };
@@ -511,12 +513,16 @@ __device__ __forceinline__ void plane_columns(V2<T> *P, const V2<T> *__restrict_
   const int n2 = threadIdx.x % 64 % C::ZR2;         \
   for (int j = threadIdx.x / 64 * C::RPW + threadIdx.x % 64 / C::ZR2; j < NY; j += C::NW * C::RPW)
 
-// the plane pass kernels take two workgroups per CU at 128^3 FP32: five
-// wavefronts per SIMD (at most 96 VGPRs)
-#define KAPSEL_PLANE_BOUNDS(THREADS) __launch_bounds__(THREADS, (2 * (THREADS) / 64 + 3) / 4)
+// the register budget of a plane pass: as many wavefronts per SIMD as the
+// workgroups its LDS lets share a CU (at most two) bring
+template <typename T, int NY, int NZ, int THREADS>
+constexpr int plane_waves_per_simd() {
+  return PlaneCfg<T, NY, NZ, THREADS>::WAVES_PER_SIMD;
+}
+#define KAPSEL_PLANE_BOUNDS __launch_bounds__(THREADS, (plane_waves_per_simd<T, NY, NZ, THREADS>()))
 
 template <typename T, int NY, int NZ, int THREADS>
-__global__ KAPSEL_PLANE_BOUNDS(THREADS) void fft_plane_r2c(FieldPtrs<T> fp, const V2<T> *__restrict__ twm,
+__global__ KAPSEL_PLANE_BOUNDS void fft_plane_r2c(FieldPtrs<T> fp, const V2<T> *__restrict__ twm,
                                                            const V2<T> *__restrict__ zrot,
                                                            const V2<T> *__restrict__ twy) {
   using C = PlaneCfg<T, NY, NZ, THREADS>;
@@ -539,7 +545,7 @@ __global__ KAPSEL_PLANE_BOUNDS(THREADS) void fft_plane_r2c(FieldPtrs<T> fp, cons
 }
 
 template <typename T, int NY, int NZ, int THREADS>
-__global__ KAPSEL_PLANE_BOUNDS(THREADS) void fft_plane_c2r(FieldPtrs<T> fp, const V2<T> *__restrict__ twm,
+__global__ KAPSEL_PLANE_BOUNDS void fft_plane_c2r(FieldPtrs<T> fp, const V2<T> *__restrict__ twm,
                                                            const V2<T> *__restrict__ zrot,
                                                            const V2<T> *__restrict__ twy, T scale) {
   using C = PlaneCfg<T, NY, NZ, THREADS>;
@@ -567,7 +573,7 @@ __global__ KAPSEL_PLANE_BOUNDS(THREADS) void fft_plane_c2r(FieldPtrs<T> fp, cons
 // transform through the face product into its forward z transform in the
 // registers of its own lanes.
 template <typename T, int NY, int NZ, int THREADS>
-__global__ KAPSEL_PLANE_BOUNDS(THREADS) void fft_plane_face(FieldPtrs<T> fp, const V2<T> *__restrict__ twm,
+__global__ KAPSEL_PLANE_BOUNDS void fft_plane_face(FieldPtrs<T> fp, const V2<T> *__restrict__ twm,
                                                             const V2<T> *__restrict__ zrot,
                                                             const V2<T> *__restrict__ twy, T scale,
                                                             FieldPtrs<T> face) {
