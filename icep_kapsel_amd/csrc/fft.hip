@@ -5,7 +5,8 @@
 // NZ_ = NZ+2 reals.  Inverse c2r (A_k2a, fft_wrapper.h:704-723) scales by
 // 1/(NX NY NZ).
 //
-// Three passes, each one read and one write of the field:
+// Three passes, each one read and one write of the field (the z and y
+// passes are one plane pass where a plane fits in LDS, below):
 //   z pass  r2c of each row: a length-NZ/2 complex FFT of the even/odd
 //           interleaved row, then the real split (c2r: the split first);
 //   y pass  complex FFT along y, lines at stride h = NZ/2+1 complex values;
@@ -1023,9 +1024,9 @@ void launch_plane(kapsel_ctx *ctx, const FieldPtrs<T> &fp, int nf, bool fwd) {
   KAPSEL_LAUNCH_CHECK();
 }
 
-// whether the z and y transforms run as one plane pass: NY = NZ = 32, 64;
-// 128 in FP32 (an FP64 128 x 65 plane, 133 KB, leaves one workgroup per CU
-// and is slower than the separate z and y passes)
+// whether the z and y transforms run as one plane pass: NY = NZ = 32, 64,
+// 128 (at 128 the padded FP64 plane, 147 KB, leaves one workgroup per CU and
+// still beats the separate z and y passes)
 template <typename T>
 bool plane_ok(const Grid &g) {
   if (g.ny != g.nz) return false;
