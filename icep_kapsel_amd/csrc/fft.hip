@@ -461,26 +461,29 @@ __device__ __forceinline__ void plane_columns(V2<T> *P, const V2<T> *__restrict_
   using C = PlaneCfg<T, NY, NZ, THREADS>;
   constexpr int H = C::H, HP = C::HP, R1 = C::YR1, R2 = C::YR2;
   const int t = threadIdx.x;
-  const int cl = t % C::YCOLS, n2 = t / C::YCOLS;
 #pragma unroll 1  // one round's registers at a time
   for (int c0 = 0; c0 < H; c0 += C::YCOLS) {
+    // a last round of at most 64 / R2 columns (the kz = NZ/2 column when
+    // YCOLS = NZ/2) gives each column R2 adjacent lanes of wavefront 0, so
+    // that the other wavefronts skip its arithmetic
+    const bool few = H - c0 < C::YCOLS && (H - c0) * R2 <= 64;
+    const int cl = few ? t / R2 : t % C::YCOLS, n2 = few ? t % R2 : t / C::YCOLS;
     const int kz = c0 + cl;
     const bool valid = kz < H;
     V2<T> *col = P + (valid ? kz : 0);
     T *gcol = g + 2 * (valid ? kz : 0);
-    V2<T> v[R1], o[R1];
+    V2<T> o[R1];
+    if (valid) {
+      V2<T> v[R1];
 #pragma unroll
-    for (int n1 = 0; n1 < R1; n1++) {
-      if constexpr (GIN) {
-        const V2<T> a = ld2(gcol + 2 * (R2 * n1 + n2) * H);
-        v[n1] = valid ? a : V2<T>{0, 0};
-      } else {
-        v[n1] = valid ? col[(R2 * n1 + n2) * HP] : V2<T>{0, 0};
+      for (int n1 = 0; n1 < R1; n1++) {
+        if constexpr (GIN) v[n1] = ld2(gcol + 2 * (R2 * n1 + n2) * H);
+        else v[n1] = col[(R2 * n1 + n2) * HP];
       }
-    }
-    dft<R1, 1, DIR>(v, o);
+      dft<R1, 1, DIR>(v, o);
 #pragma unroll
-    for (int k1 = 1; k1 < R1; k1++) o[k1] = cmul(o[k1], tw_get<DIR>(twy, n2 * k1));
+      for (int k1 = 1; k1 < R1; k1++) o[k1] = cmul(o[k1], tw_get<DIR>(twy, n2 * k1));
+    }
     __syncthreads();  // every column of the round is in registers
     if (valid) {
 #pragma unroll
