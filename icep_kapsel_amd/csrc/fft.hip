@@ -774,6 +774,11 @@ __global__ __launch_bounds__(LPB *FS<L>::R2) void fft_x_grad(const T *z, T *p, c
     }
   }
   V2<T> *sl = sm + l * LS;
+  // the twiddles of this thread's first stage, loaded once for the three
+  // directions (the stores in between could alias the table otherwise)
+  V2<T> twr[R1];
+#pragma unroll
+  for (int k1 = 1; k1 < R1; k1++) twr[k1] = tw_get<+1>(tw, n2 * k1);
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     V2<T> v[R1], o[R1];
@@ -786,7 +791,7 @@ __global__ __launch_bounds__(LPB *FS<L>::R2) void fft_x_grad(const T *z, T *p, c
     }
     dft<R1, 1, +1>(v, o);
 #pragma unroll
-    for (int k1 = 1; k1 < R1; k1++) o[k1] = cmul(o[k1], tw_get<+1>(tw, n2 * k1));
+    for (int k1 = 1; k1 < R1; k1++) o[k1] = cmul(o[k1], twr[k1]);
     if (d > 0) __syncthreads();  // the previous direction's LDS reads are done
 #pragma unroll
     for (int k1 = 0; k1 < R1; k1++) sl[n2 * (R1 + 1) + k1] = o[k1];
@@ -843,6 +848,10 @@ __global__ __launch_bounds__(LPB *FS<L>::R2) void fft_x_div(FieldPtrs<T> g3, con
   V2<T> nxt[R1];
 #pragma unroll
   for (int n1 = 0; n1 < R1; n1++) nxt[n1] = ld2(g3.p[0] + 2 * (base + (long long)(R2 * n1 + n2) * plane));
+  // the first stage's twiddles, loaded once for the three directions
+  V2<T> twr[R1];
+#pragma unroll
+  for (int k1 = 1; k1 < R1; k1++) twr[k1] = tw_get<-1>(tw, n2 * k1);
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     V2<T> v[R1], o[R1];
@@ -855,7 +864,7 @@ __global__ __launch_bounds__(LPB *FS<L>::R2) void fft_x_div(FieldPtrs<T> g3, con
     }
     dft<R1, 1, -1>(v, o);
 #pragma unroll
-    for (int k1 = 1; k1 < R1; k1++) o[k1] = cmul(o[k1], tw_get<-1>(tw, n2 * k1));
+    for (int k1 = 1; k1 < R1; k1++) o[k1] = cmul(o[k1], twr[k1]);
     if (d > 0) __syncthreads();
 #pragma unroll
     for (int k1 = 0; k1 < R1; k1++) sl[n2 * (R1 + 1) + k1] = o[k1];
