@@ -351,6 +351,8 @@ def main():
     ap.add_argument("--el-steps", type=int, default=50, help="timed steps of the 256^3 electrolyte workload")
     ap.add_argument("--no-electrolyte", action="store_true", help="skip the 256^3 electrolyte workload")
     ap.add_argument("--only-electrolyte", action="store_true", help="print the electrolyte record only")
+    ap.add_argument("--only-spm", action="store_true",
+                    help="print the SPM-only record (128^3 x 200 spheres, FP32 and FP64) only")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="K",
                     help="only run K plain field copies (128^3 FP32, 8.52 MB each way) to calibrate "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE, then exit")
@@ -364,6 +366,12 @@ def main():
         sys.exit(2)
     from icep_kapsel_amd import abi
 
+    if args.only_spm:
+        log(f"bench: SPM step (128^3 x 200 spheres, no Poisson) FP32 / FP64, warmup 5, steps {args.steps}")
+        print(json.dumps({"spm_ns_step_128_200_spheres": {
+            "fp32_steps_per_s": run_spm(abi.KAPSEL_FP32, args.steps, 5),
+            "fp64_steps_per_s": run_spm(abi.KAPSEL_FP64, args.steps, 5)}}), flush=True)
+        return
     electrolyte = None
     if not args.no_electrolyte:
         electrolyte = electrolyte_record(args)

@@ -27,6 +27,9 @@
  *                         md_force.cxx:173-259)
  *   particle update       closed-form Euler/AB2 increments
  *   dielectric operator   constant eps: symbol -eps|k|^2; symmetry; residual of the solve
+ *   init_rigidity, f_particle  closed-form centre velocity of one sphere on a grid node
+ *   spm/icep/electrolyte step  exact per-step momentum budget (tests/test_oracle_step.py);
+ *                         `plant` switches in known mistakes that the checks must catch
  */
 #include <algorithm>
 #include <cmath>
@@ -111,6 +114,12 @@ struct Oracle {
   std::vector<double> qs;   /* orientations q (c < 4) and q_old (c = 4..7) at [c*Np + n] */
   int NP_domain;
   std::vector<int> sekibun; /* 3 ints per cell point */
+  /* Planted mistake (0 = none), for tests that show a check fails on it:
+   * 1 Add_f_particle subtracts f; 2 the hydrodynamic gather runs after the
+   * velocity update; 3 f is built from phi_sum instead of phi; 4 the
+   * position update runs after the profile scatter; 5 the dielectric or
+   * Coulomb force is subtracted from u. */
+  int plant = 0;
 
   double *fld(int f) { return fields[f]; }
   double *pf(int f, int d) { return ps.data() + ((size_t)f * 3 + d) * Np; }
@@ -875,7 +884,7 @@ struct Oracle {
     double *f[3] = {FP(0), FP(1), FP(2)};
     double *u[3] = {U(0), U(1), U(2)};
     double *up[3] = {UP(0), UP(1), UP(2)};
-    double *phi = fld(KAPSEL_PHI);
+    double *phi = fld(plant == 3 ? KAPSEL_PHI_SUM : KAPSEL_PHI);
 #pragma omp parallel for
     for (int i = 0; i < NX; i++)
       for (int j = 0; j < NY; j++)
@@ -884,12 +893,13 @@ struct Oracle {
           for (int d = 0; d < 3; d++) f[d][im] = up[d][im] - phi[im] * u[d][im];
         }
     solenoidal_u(f);
+    const double sgn = plant == 1 ? -1.0 : 1.0;
 #pragma omp parallel for
     for (int i = 0; i < NX; i++)
       for (int j = 0; j < NY; j++)
         for (int k = 0; k < NZ; k++) {
           size_t im = ((size_t)i * NY + j) * NZ_ + k;
-          for (int d = 0; d < 3; d++) u[d][im] += f[d][im];
+          for (int d = 0; d < 3; d++) u[d][im] += sgn * f[d][im];
         }
   }
 
@@ -901,14 +911,16 @@ struct Oracle {
     truncate(fld(KAPSEL_ZETA0));
     truncate(fld(KAPSEL_ZETA1));
     zeta_k2u();
-    md_position(jikan);
+    if (plant != 4) md_position(jikan);
     reset(fld(KAPSEL_PHI));
     reset(fld(KAPSEL_PHI_SUM));
     make_phi_particle_sum(fld(KAPSEL_PHI), fld(KAPSEL_PHI_SUM));
+    if (plant == 4) md_position(jikan);
     for (int d = 0; d < 3; d++) reset(UP(d));
     double *u[3] = {U(0), U(1), U(2)};
+    if (plant == 2) md_velocity(jikan);
     calc_f_hydro(fld(KAPSEL_PHI_SUM), u, jikan);
-    md_velocity(jikan);
+    if (plant != 2) md_velocity(jikan);
     if (thermo_on && thermo_kbt > 0.) thermostat(jikan);  /* sp_3d_ns.cxx:155-157 */
     for (int d = 0; d < 3; d++) reset(UP(d));
     double *up[3] = {UP(0), UP(1), UP(2)};
@@ -1247,7 +1259,7 @@ struct Oracle {
   }
   /* u += f dt / rho, then Solenoidal_u(u) (sp_3d_ns.cxx:83-94) */
   void add_force_to_fluid(const kapsel_time &jikan) {
-    double dmy = jikan.dt_fluid * IRHO;
+    double dmy = (plant == 5 ? -1.0 : 1.0) * jikan.dt_fluid * IRHO;
 #pragma omp parallel for
     for (int i = 0; i < NX; i++)
       for (int j = 0; j < NY; j++)
@@ -2276,6 +2288,7 @@ int oracle_icep_step(void *h, kapsel_time *t, const double *e0, double tol, int 
   return 0;
 }
 int oracle_init_rigidity(void *h) { O->init_rigidity(); return 0; }
+int oracle_set_plant(void *h, int code) { O->plant = code; return 0; }
 int oracle_set_thermostat(void *h, double kbt, double alpha_v, double alpha_o, unsigned long seed) {
   O->thermo_on = true;
   O->thermo_kbt = kbt;

@@ -46,7 +46,7 @@ def _load():
             "oracle_make_phi_particle": [i], "oracle_make_phi_sin": [i],
             "oracle_calc_f_hydro": [tp], "oracle_force": [],
             "oracle_md_position": [tp], "oracle_md_velocity": [tp],
-            "oracle_make_f_particle_dt_sole": [], "oracle_init_rigidity": [],
+            "oracle_make_f_particle_dt_sole": [], "oracle_init_rigidity": [], "oracle_set_plant": [i],
             "oracle_spm_step": [tp],
             "oracle_make_dielectric_field": [], "oracle_poisson_apply": [i, i],
             "oracle_ax_unpadded": [dp, dp], "oracle_poisson_rhs": [dp],
@@ -225,6 +225,10 @@ class Oracle:
 
     def init_rigidity(self):
         self._call("init_rigidity")
+
+    def set_plant(self, code):
+        """Plants a known mistake in the step (0 removes it); see Oracle::plant."""
+        self._call("set_plant", int(code))
 
     def spm_step(self, t):
         self._call("spm_step", ctypes.byref(t))
