@@ -69,21 +69,17 @@ static void harvest_timing(kapsel_ctx *ctx) {
 // Particle_domain (profile.cxx:25-74): every integer offset within
 // RADIUS + XI/2 + 1.7321 DX, in the reference's i, j, k loop order
 static void build_sekibun(kapsel_ctx *ctx) {
-  const double dx = ctx->g.dx;
-  const int A = (int)(ctx->P.radius / dx + 1e-9), A_XI = (int)(ctx->P.xi / dx + 1e-9);
-  const int nh = A + A_XI + 2;
-  const double max_radius = ctx->P.radius + ctx->hxi + 1.7321 * dx;
+  const SekibunGeom sg = sekibun_geom(ctx->P.radius, ctx->P.xi, ctx->g.dx);
+  const int nh = sg.nh;
   ctx->sekibun.clear();
   for (int i = -nh; i <= nh; i++)
     for (int j = -nh; j <= nh; j++)
-      for (int k = -nh; k <= nh; k++) {
-        double x0 = (double)i * dx, x1 = (double)j * dx, x2 = (double)k * dx;
-        if (sqrt(x0 * x0 + x1 * x1 + x2 * x2) < max_radius) {
+      for (int k = -nh; k <= nh; k++)
+        if (sekibun_inside(sg, i, j, k)) {
           ctx->sekibun.push_back(i);
           ctx->sekibun.push_back(j);
           ctx->sekibun.push_back(k);
         }
-      }
   ctx->np_domain = (int)ctx->sekibun.size() / 3;
   if (ctx->P.n_particles > 0 && (2 * nh + 1 > ctx->g.nx || 2 * nh + 1 > ctx->g.ny || 2 * nh + 1 > ctx->g.nz)) {
     set_error("Sekibun cell (%d points across) wider than the grid", 2 * nh + 1);
@@ -117,6 +113,7 @@ static void destroy(kapsel_ctx *ctx) {
   if (ctx->d_ps) (void)hipFree(ctx->d_ps);
   if (ctx->d_q) (void)hipFree(ctx->d_q);
   if (ctx->d_sekibun) (void)hipFree(ctx->d_sekibun);
+  if (ctx->d_gpart) (void)hipFree(ctx->d_gpart);
   free_twiddles(ctx);
   poisson_free(ctx);
   pnet_free(ctx);
@@ -227,6 +224,8 @@ static void create(const kapsel_params *p, kapsel_ctx **out) {
       KAPSEL_HIP(hipStreamSynchronize(ctx->stream));
     }
     build_sekibun(ctx);
+    KAPSEL_HIP(hipMalloc(&ctx->d_gpart, sizeof(double) * 12 * np *
+                                            (size_t)gather_slices((int)np, ctx->np_domain)));
     init_twiddles(ctx);
     KAPSEL_HIP(hipStreamSynchronize(ctx->stream));
   } catch (...) {

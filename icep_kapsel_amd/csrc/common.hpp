@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstddef>
 #include <cstdint>
 #include <random>
@@ -172,6 +173,7 @@ struct kapsel_ctx {
   double *d_q = nullptr;          // orientations q, q_old (device): [c*np + n], c < 8
   int *d_sekibun = nullptr;       // np_domain * 3 ints
   int np_domain = 0;
+  double *d_gpart = nullptr;      // gather partial sums: 12 doubles per (particle, slice)
   std::vector<int> sekibun;       // host copy
   kapsel::Twiddles tw[2];          // [KAPSEL_FP64], [KAPSEL_FP32]
   // Poisson solver storage, allocated on first use: operator scratch of the
@@ -265,6 +267,32 @@ struct LaunchTimer {
 
 inline unsigned blocks_for(size_t n, unsigned threads) {
   return (unsigned)((n + threads - 1) / threads);
+}
+
+// Particle_domain (profile.cxx:25-74): offsets (i, j, k), |i|,|j|,|k| <= nh,
+// in i-j-k order, kept when sqrt((i dx)^2 + (j dx)^2 + (k dx)^2) < max_radius
+// = RADIUS + XI/2 + 1.7321 DX, nh = A + A_XI + 2.
+struct SekibunGeom {
+  int nh;
+  double dx, max_radius;
+};
+inline SekibunGeom sekibun_geom(double radius, double xi, double dx) {
+  const int A = (int)(radius / dx + 1e-9), A_XI = (int)(xi / dx + 1e-9);
+  return SekibunGeom{A + A_XI + 2, dx, radius + 0.5 * xi + 1.7321 * dx};
+}
+inline bool sekibun_inside(const SekibunGeom &s, int i, int j, int k) {
+  const double x0 = (double)i * s.dx, x1 = (double)j * s.dx, x2 = (double)k * s.dx;
+  return sqrt(x0 * x0 + x1 * x1 + x2 * x2) < s.max_radius;
+}
+
+// Slices of the Sekibun table per particle in the hydrodynamic gather: about
+// 1024 wavefronts in all (four per CU), at least 128 table points each.
+inline int gather_slices(int np, int np_domain) {
+  if (np <= 0) return 1;
+  int s = (1024 + np - 1) / np;
+  const int smax = (np_domain + 127) / 128;
+  if (s > smax) s = smax;
+  return s < 1 ? 1 : s;
 }
 
 // ---- launchers (defined in the .hip files) ----------------------------------

@@ -12,9 +12,10 @@
 // i outside the profile support has phi = 0 exactly, and every point with
 // phi > 0 lies in the cell (|o| <= r + |residue| < a + xi/2 + sqrt(3) dx).
 //
-// Gather (Calc_f_hydro_correct_precision, md_force.cxx:340-449): one
-// workgroup per particle walks the Sekibun table in reference order, FP64
-// compensated (TwoSum) partial sums per lane, then a wavefront reduction.
+// Gather (Calc_f_hydro_correct_precision, md_force.cxx:340-449): each
+// particle's Sekibun table is cut into slices, one wavefront per (particle,
+// slice) sums its slice with FP64 compensated (TwoSum) partial sums per lane
+// and a wavefront reduction; a second kernel merges the slices in order.
 #include <cfloat>
 #include <type_traits>
 
@@ -284,13 +285,21 @@ __device__ __forceinline__ CSum cmerge(CSum a, CSum b) {
   return r;
 }
 
+// Gather, first stage: unit u = (particle n, slice s) is one wavefront that
+// sums the Sekibun points [s*chunk, min((s+1)*chunk, np_domain)) of particle
+// n (lane-strided, TwoSum-compensated FP64), reduces across the wavefront and
+// stores its 6 compensated sums (force, torque) as 12 doubles at
+// partial[12*u].  Slicing the table puts nslices wavefronts on each particle,
+// so a few dozen particles still fill the chip.
 template <typename T>
-__global__ __launch_bounds__(256) void hydro_gather_kernel(double *__restrict__ ps, const int *__restrict__ cells,
-                                                           int np_domain, const T *__restrict__ u0,
-                                                           const T *__restrict__ u1, const T *__restrict__ u2,
-                                                           const T *__restrict__ phi_sum, Grid g, Phys p,
-                                                           double dmy) {
-  const int n = blockIdx.x;
+__global__ __launch_bounds__(64) void hydro_gather_part_kernel(const double *__restrict__ ps,
+                                                               const int *__restrict__ cells, int np_domain,
+                                                               int nslices, int chunk, const T *__restrict__ u0,
+                                                               const T *__restrict__ u1, const T *__restrict__ u2,
+                                                               const T *__restrict__ phi_sum, Grid g, Phys p,
+                                                               double *__restrict__ partial) {
+  const int unit = blockIdx.x;
+  const int n = unit / nslices, s = unit - n * nslices;
   const int t = threadIdx.x;
   double xp[3], vp[3], om[3];
   int x_int[3];
@@ -312,7 +321,9 @@ __global__ __launch_bounds__(256) void hydro_gather_kernel(double *__restrict__ 
   CSum acc[6];
 #pragma unroll
   for (int c = 0; c < 6; c++) acc[c] = CSum{0.0, 0.0};
-  for (int mesh = t; mesh < np_domain; mesh += 256) {
+  const int m0 = s * chunk;
+  const int m1 = min(np_domain, m0 + chunk);
+  for (int mesh = m0 + t; mesh < m1; mesh += 64) {
     int rm[3];
     double r[3], x[3];
 #pragma unroll
@@ -344,7 +355,6 @@ __global__ __launch_bounds__(256) void hydro_gather_kernel(double *__restrict__ 
     cadd(acc[4], r[2] * fp[0] - r[0] * fp[2]);
     cadd(acc[5], r[0] * fp[1] - r[1] * fp[0]);
   }
-  // wavefront reduction, then across the 4 wavefronts
 #pragma unroll
   for (int c = 0; c < 6; c++) {
     for (int off = 32; off > 0; off >>= 1) {
@@ -352,26 +362,31 @@ __global__ __launch_bounds__(256) void hydro_gather_kernel(double *__restrict__ 
       acc[c] = cmerge(acc[c], o);
     }
   }
-  __shared__ double red[4][6][2];
-  const int wid = t / 64, lane = t % 64;
-  if (lane == 0) {
-#pragma unroll
-    for (int c = 0; c < 6; c++) {
-      red[wid][c][0] = acc[c].s;
-      red[wid][c][1] = acc[c].c;
-    }
-  }
-  __syncthreads();
   if (t == 0) {
+    double *dst = partial + (size_t)12 * unit;
 #pragma unroll
     for (int c = 0; c < 6; c++) {
-      CSum s = CSum{red[0][c][0], red[0][c][1]};
-      for (int w = 1; w < 4; w++) s = cmerge(s, CSum{red[w][c][0], red[w][c][1]});
-      const double v = s.s + s.c;
-      if (c < 3) pref(ps, KAPSEL_P_F_HYDRO, c, n, p.np) = dmy * v;
-      else if (p.rotation) pref(ps, KAPSEL_P_TORQUE_HYDRO, c - 3, n, p.np) = dmy * v;
+      dst[2 * c] = acc[c].s;
+      dst[2 * c + 1] = acc[c].c;
     }
   }
+}
+
+// Gather, second stage: one thread per (particle, component) merges the
+// slices' compensated sums in slice order (a fixed order: the result does not
+// depend on scheduling) and writes f_hydro / torque_hydro
+// (md_force.cxx:440-447).
+__global__ void hydro_gather_final_kernel(double *__restrict__ ps, const double *__restrict__ partial, int nslices,
+                                          Phys p, double dmy) {
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= 6 * p.np) return;
+  const int n = id / 6, c = id - 6 * n;
+  const double *src = partial + (size_t)12 * n * nslices + 2 * c;
+  CSum acc{src[0], src[1]};
+  for (int s = 1; s < nslices; s++) acc = cmerge(acc, CSum{src[12 * s], src[12 * s + 1]});
+  const double v = acc.s + acc.c;
+  if (c < 3) pref(ps, KAPSEL_P_F_HYDRO, c, n, p.np) = dmy * v;
+  else if (p.rotation) pref(ps, KAPSEL_P_TORQUE_HYDRO, c - 3, n, p.np) = dmy * v;
 }
 
 struct LJ {
@@ -618,14 +633,27 @@ void launch_points(kapsel_ctx *ctx, void *o0, void *o1, void *o2, const void *ph
   KAPSEL_LAUNCH_CHECK();
 }
 
+// both stages of the gather on the given memory; partial holds 12 doubles per
+// (particle, slice) unit
+template <typename T>
+void launch_gather(hipStream_t stream, double *ps, const int *cells, int np_domain, const T *u0, const T *u1,
+                   const T *u2, const T *phi_sum, const Grid &g, const Phys &p, double dmy, double *partial) {
+  const int ns = gather_slices(p.np, np_domain);
+  const int chunk = (np_domain + ns - 1) / ns;
+  hipLaunchKernelGGL(hydro_gather_part_kernel<T>, dim3((unsigned)(p.np * ns)), dim3(64), 0, stream, ps, cells,
+                     np_domain, ns, chunk, u0, u1, u2, phi_sum, g, p, partial);
+  KAPSEL_LAUNCH_CHECK();
+  hipLaunchKernelGGL(hydro_gather_final_kernel, dim3(blocks_for((size_t)6 * p.np, 64)), dim3(64), 0, stream, ps,
+                     partial, ns, p, dmy);
+  KAPSEL_LAUNCH_CHECK();
+}
+
 template <typename T>
 void gather_t(kapsel_ctx *ctx, double dmy) {
   LaunchTimer lt(ctx, KAPSEL_K_PARTICLE, 4.0 * ctx->P.n_particles * ctx->np_domain * sizeof(T));
-  hipLaunchKernelGGL(hydro_gather_kernel<T>, dim3(ctx->P.n_particles), dim3(256), 0, ctx->stream, ctx->d_ps,
-                     ctx->d_sekibun, ctx->np_domain, (const T *)ctx->fields[KAPSEL_U0],
-                     (const T *)ctx->fields[KAPSEL_U1], (const T *)ctx->fields[KAPSEL_U2],
-                     (const T *)ctx->fields[KAPSEL_PHI_SUM], ctx->g, phys(ctx), dmy);
-  KAPSEL_LAUNCH_CHECK();
+  launch_gather<T>(ctx->stream, ctx->d_ps, ctx->d_sekibun, ctx->np_domain, (const T *)ctx->fields[KAPSEL_U0],
+                   (const T *)ctx->fields[KAPSEL_U1], (const T *)ctx->fields[KAPSEL_U2],
+                   (const T *)ctx->fields[KAPSEL_PHI_SUM], ctx->g, phys(ctx), dmy, ctx->d_gpart);
 }
 
 }  // namespace
@@ -785,3 +813,4 @@ void md_velocity_integrate(kapsel_ctx *ctx, const kapsel_time &t) {
 }
 
 }  // namespace kapsel
+

@@ -74,7 +74,7 @@ enum kapsel_precond {
 
 /* Values of the reference's globals that the hot path reads (input.h). */
 typedef struct kapsel_params {
-  int nx, ny, nz;            /* NX, NY, NZ: powers of two, 16..512 */
+  int nx, ny, nz;            /* NX, NY, NZ: powers of two, 16..256 */
   double dx;                 /* DX */
   double rho, eta;           /* RHO, ETA (NU = ETA/RHO) */
   int n_particles;           /* Particle_Number (one species) */
