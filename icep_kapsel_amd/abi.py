@@ -235,6 +235,10 @@ def load_library(path=None):
     lib.kapsel_restart_write.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(RestartData)]
     lib.kapsel_restart_read.restype = _I
     lib.kapsel_restart_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(RestartData)]
+    # device-pointer entry point with no context (allocates nothing)
+    lib.kapsel_calc_f_hydro_noalloc.restype = _I
+    lib.kapsel_calc_f_hydro_noalloc.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(Time), _VP, _VP, _VP, _VP,
+                                                _VP, _VP]
     if path == LIB_PATH:
         _lib = lib
     return lib

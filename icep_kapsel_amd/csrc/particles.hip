@@ -17,6 +17,7 @@
 // slice) sums its slice with FP64 compensated (TwoSum) partial sums per lane
 // and a wavefront reduction; a second kernel merges the slices in order.
 #include <cfloat>
+#include <mutex>
 #include <type_traits>
 
 #include "common.hpp"
@@ -814,3 +815,177 @@ void md_velocity_integrate(kapsel_ctx *ctx, const kapsel_time &t) {
 
 }  // namespace kapsel
 
+// ---------------------------------------------------------------------------
+// kapsel_calc_f_hydro_noalloc: the gather on caller-owned device memory with
+// its scratch (the Sekibun table and the slices' partial sums) in a static
+// device array, so that a call allocates nothing.
+namespace kapsel {
+namespace {
+
+constexpr size_t kNoallocTableInts = (size_t)3 * KAPSEL_NOALLOC_MAX_CELL_POINTS;
+constexpr size_t kNoallocTableDoubles = (kNoallocTableInts * sizeof(int) + sizeof(double) - 1) / sizeof(double);
+constexpr size_t kNoallocDoubles = KAPSEL_NOALLOC_SCRATCH_BYTES / sizeof(double);
+static_assert(kNoallocTableDoubles < kNoallocDoubles, "scratch too small for the Sekibun table");
+constexpr size_t kNoallocUnits = (kNoallocDoubles - kNoallocTableDoubles) / 12;
+
+// The array as each device holds it.  In host code `scratch` names a host-side
+// stand-in, not device memory: host code only ever uses the per-device address
+// that hipGetSymbolAddress returns (scratch_ptr below).
+static __device__ __align__(256) double scratch[kNoallocDoubles];
+
+// Particle_domain (profile.cxx:25-74) on the device, one workgroup: the cube
+// of offsets in i-j-k order, 256 at a time; a kept offset goes to slot
+// base + (kept offsets before it in this round), so the table has the
+// reference order.  The test is sekibun_inside's, with each rounding explicit.
+__global__ __launch_bounds__(256) void sekibun_build_kernel(int *__restrict__ cells, int nh, double dx,
+                                                            double max_radius) {
+  __shared__ int wave_count[4];
+  const int side = 2 * nh + 1, total = side * side * side;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int base = 0;
+  for (int c0 = 0; c0 < total; c0 += 256) {
+    const int idx = c0 + (int)threadIdx.x;
+    int i = 0, j = 0, k = 0;
+    bool in = false;
+    if (idx < total) {
+      i = idx / (side * side) - nh;
+      j = (idx / side) % side - nh;
+      k = idx % side - nh;
+      const double x0 = __dmul_rn((double)i, dx), x1 = __dmul_rn((double)j, dx), x2 = __dmul_rn((double)k, dx);
+      const double r2 = __dadd_rn(__dadd_rn(__dmul_rn(x0, x0), __dmul_rn(x1, x1)), __dmul_rn(x2, x2));
+      in = __dsqrt_rn(r2) < max_radius;
+    }
+    const unsigned long long m = __ballot(in);
+    const int before = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) wave_count[w] = __popcll(m);
+    __syncthreads();
+    int off = base;
+    for (int q = 0; q < w; q++) off += wave_count[q];
+    if (in) {
+      cells[3 * (off + before)] = i;
+      cells[3 * (off + before) + 1] = j;
+      cells[3 * (off + before) + 2] = k;
+    }
+    base += wave_count[0] + wave_count[1] + wave_count[2] + wave_count[3];
+    __syncthreads();
+  }
+}
+
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[KAPSEL_NOALLOC_MAX_DEVICES] = {};
+
+// The device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept.
+double *scratch_ptr() {
+  int dev = 0;
+  KAPSEL_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= KAPSEL_NOALLOC_MAX_DEVICES) {
+    set_error("device %d beyond the %d devices the allocation-free entry points serve", dev,
+              KAPSEL_NOALLOC_MAX_DEVICES);
+    throw Error{KAPSEL_ERR_ARG};
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    KAPSEL_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) {
+      set_error("hipGetSymbolAddress returned no address for the scratch array");
+      throw Error{KAPSEL_ERR_HIP};
+    }
+    g_scratch_ptr[dev] = ptr;
+  }
+  return (double *)g_scratch_ptr[dev];
+}
+
+template <typename T>
+void noalloc_gather(const kapsel_params &P, const Phys &p, const Grid &g, double dmy, double *ps, const void *u0,
+                    const void *u1, const void *u2, const void *phi_sum, int np_domain, const SekibunGeom &sg,
+                    hipStream_t stream, double *base) {
+  int *cells = (int *)base;
+  double *partial = base + kNoallocTableDoubles;
+  hipLaunchKernelGGL(sekibun_build_kernel, dim3(1), dim3(256), 0, stream, cells, sg.nh, sg.dx, sg.max_radius);
+  KAPSEL_LAUNCH_CHECK();
+  launch_gather<T>(stream, ps, cells, np_domain, (const T *)u0, (const T *)u1, (const T *)u2, (const T *)phi_sum,
+                   g, p, dmy, partial);
+}
+
+}  // namespace
+
+}  // namespace kapsel
+
+extern "C" int kapsel_calc_f_hydro_noalloc(const kapsel_params *params, const kapsel_time *jikan, double *ps,
+                                           const void *u0, const void *u1, const void *u2, const void *phi_sum,
+                                           void *stream) {
+  using namespace kapsel;
+  try {
+    if (!params || !jikan || !ps || !u0 || !u1 || !u2 || !phi_sum) {
+      set_error("kapsel_calc_f_hydro_noalloc: null argument");
+      return KAPSEL_ERR_ARG;
+    }
+    const kapsel_params &P = *params;
+    if (!(jikan->dt_fluid > 0) || !(P.dx > 0) || !(P.rho > 0) || P.n_particles < 0 || !(P.radius > 0) ||
+        !(P.xi > 0) || (P.precision != KAPSEL_FP64 && P.precision != KAPSEL_FP32)) {
+      set_error("kapsel_calc_f_hydro_noalloc: invalid parameters");
+      return KAPSEL_ERR_ARG;
+    }
+    for (int n : {P.nx, P.ny, P.nz})
+      if (n < 1) {
+        set_error("kapsel_calc_f_hydro_noalloc: grid size %d", n);
+        return KAPSEL_ERR_ARG;
+      }
+    if (P.n_particles == 0) return KAPSEL_OK;
+    const SekibunGeom sg = sekibun_geom(P.radius, P.xi, P.dx);
+    if (2 * sg.nh + 1 > P.nx || 2 * sg.nh + 1 > P.ny || 2 * sg.nh + 1 > P.nz) {
+      set_error("kapsel_calc_f_hydro_noalloc: Sekibun cell (%d points across) wider than the grid", 2 * sg.nh + 1);
+      return KAPSEL_ERR_ARG;
+    }
+    int np_domain = 0;
+    for (int i = -sg.nh; i <= sg.nh; i++)
+      for (int j = -sg.nh; j <= sg.nh; j++)
+        for (int k = -sg.nh; k <= sg.nh; k++) np_domain += sekibun_inside(sg, i, j, k) ? 1 : 0;
+    if (np_domain > KAPSEL_NOALLOC_MAX_CELL_POINTS) {
+      set_error("kapsel_calc_f_hydro_noalloc: Sekibun cell of %d points exceeds the scratch's %d", np_domain,
+                KAPSEL_NOALLOC_MAX_CELL_POINTS);
+      return KAPSEL_ERR_ARG;
+    }
+    const size_t units = (size_t)P.n_particles * gather_slices(P.n_particles, np_domain);
+    if (units > kNoallocUnits) {
+      set_error("kapsel_calc_f_hydro_noalloc: %d particles need %zu partial-sum slots, the scratch holds %zu",
+                P.n_particles, units, kNoallocUnits);
+      return KAPSEL_ERR_ARG;
+    }
+    Grid g{};
+    g.nx = P.nx;
+    g.ny = P.ny;
+    g.nz = P.nz;
+    g.h = g.nz / 2 + 1;
+    g.nz_ = 2 * g.h;
+    g.dx = P.dx;
+    Phys p{};
+    p.dx = P.dx;
+    p.idx = 1. / P.dx;
+    p.radius = P.radius;
+    p.xi = P.xi;
+    p.hxi = 0.5 * P.xi;
+    p.l[0] = P.nx * P.dx;
+    p.l[1] = P.ny * P.dx;
+    p.l[2] = P.nz * P.dx;
+    p.n[0] = P.nx;
+    p.n[1] = P.ny;
+    p.n[2] = P.nz;
+    p.np = P.n_particles;
+    p.rotation = P.rotation;
+    const double dmy = -(P.dx * P.dx * P.dx) * P.rho / jikan->dt_fluid;  // md_force.cxx:342-343
+    double *base = scratch_ptr();
+    if (P.precision == KAPSEL_FP64)
+      noalloc_gather<double>(P, p, g, dmy, ps, u0, u1, u2, phi_sum, np_domain, sg, (hipStream_t)stream, base);
+    else
+      noalloc_gather<float>(P, p, g, dmy, ps, u0, u1, u2, phi_sum, np_domain, sg, (hipStream_t)stream, base);
+    return KAPSEL_OK;
+  } catch (const Error &e) {
+    return e.code;
+  } catch (const std::exception &e) {
+    set_error("%s", e.what());
+    return KAPSEL_ERR_STATE;
+  }
+}

@@ -421,6 +421,34 @@ int kapsel_save_restart(kapsel_ctx *ctx, const char *path, const kapsel_time *ji
  * (Count_solute_each, solute_rhs.h:143-156). */
 int kapsel_load_restart(kapsel_ctx *ctx, const char *path, kapsel_time *jikan);
 
+/* ---- allocation-free hydrodynamic force ------------------------------------ */
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): Calc_f_hydro_correct_precision as kapsel_calc_f_hydro
+ * computes it (same kernels, same summation order), on caller-owned device
+ * memory, with no context.  Its scratch (the Sekibun table and the per-slice
+ * partial sums) is a static device array of KAPSEL_NOALLOC_SCRATCH_BYTES that
+ * each device holds until the process ends; a call allocates and frees
+ * nothing.  Every call on one device shares that device's array: calls that
+ * overlap on one device race.  Limits: a Sekibun cell of at most
+ * KAPSEL_NOALLOC_MAX_CELL_POINTS points (a + xi/2 up to about 23 dx) and
+ * 35,498 partial-sum slots, one per particle and slice of its table (the
+ * library takes about 1024 / n_particles slices, at least one, so up to 35,498
+ * particles); beyond them the call returns KAPSEL_ERR_ARG.  Devices 0 ..
+ * KAPSEL_NOALLOC_MAX_DEVICES-1, the calling thread's current device.
+ *   params: grid, dx, rho, radius, xi, rotation, n_particles; precision
+ *           gives the field type (double for KAPSEL_FP64, float for FP32)
+ *   ps:     device particle state, layout of kapsel_get_particles (doubles);
+ *           reads X, V, OMEGA, writes F_HYDRO and (rotation) TORQUE_HYDRO
+ *   u0..u2, phi_sum: device fields in the padded layout
+ *   stream: a hipStream_t, NULL for the null stream; the call only queues work */
+#define KAPSEL_NOALLOC_SCRATCH_BYTES (4u * 1024u * 1024u)
+#define KAPSEL_NOALLOC_MAX_CELL_POINTS 65536
+#define KAPSEL_NOALLOC_MAX_DEVICES 64
+int kapsel_calc_f_hydro_noalloc(const kapsel_params *params, const kapsel_time *jikan, double *ps,
+                                const void *u0, const void *u1, const void *u2, const void *phi_sum,
+                                void *stream);
+
 /* ---- kernel timing (HIP events on the context stream) --------------------- */
 enum kapsel_kernel_class {
   KAPSEL_K_FFT_Z_R2C = 0, /* z-direction r2c pass (one kernel) */
