@@ -267,8 +267,8 @@ def run_electrolyte(precision, steps, warmup, snapshot=False):
             "v": v, "snapshot": snap}
 
 
-def run_cpu_electrolyte(snap):
-    """One FP64 oracle electrolyte step from the GPU FP64 state after the warm-up."""
+def run_cpu_electrolyte(snap, steps):
+    """FP64 oracle electrolyte steps from the GPU FP64 state after the warm-up."""
     from icep_kapsel_amd import abi, workloads
     from oracle import oracle as omod
 
@@ -282,13 +282,14 @@ def run_cpu_electrolyte(snap):
     o.set_total_solute(snap["totals"])
     t = abi.make_time(dt, ts=snap["ts"], time=snap["time"])
     t0 = time.perf_counter()
-    o.electrolyte_step(t)
+    for _ in range(steps):
+        o.electrolyte_step(t)
     t1 = time.perf_counter()
-    return 1.0 / (t1 - t0), omod.num_threads()
+    return steps / (t1 - t0), omod.num_threads()
 
 
-def run_cpu(snap):
-    """One FP64 oracle ICEP step (reference BiCGSTAB, identity preconditioner)
+def run_cpu(snap, steps):
+    """FP64 oracle ICEP steps (reference BiCGSTAB, identity preconditioner)
     from the GPU FP64 state after the warm-up steps."""
     from icep_kapsel_amd import abi, workloads
     from oracle import oracle as omod
@@ -301,10 +302,68 @@ def run_cpu(snap):
     o.set_uk_dc(snap["uk_dc"])
     o.set_field("POTENTIAL", snap["POTENTIAL"])
     t = abi.make_time(dt, ts=snap["ts"], time=snap["time"])
+    its = []
     t0 = time.perf_counter()
-    it, rr = o.icep_step(t, e0, TOL, MAXIT, 2)
+    for _ in range(steps):
+        it, rr = o.icep_step(t, e0, TOL, MAXIT, 2)
+        its.append(it)
     t1 = time.perf_counter()
-    return 1.0 / (t1 - t0), omod.num_threads(), it
+    return steps / (t1 - t0), omod.num_threads(), float(sum(its)) / len(its)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def fft_calibration(n=128, reps=5):
+    """The oracle's radix-2 FFT against the reference's own Ooura rdft3d +
+    rdft3dsort (built from the reference sources into oracle/_ref when they
+    are present), one forward and one inverse 3-D transform of the same
+    n^3 field, one thread each (fftsg3d.c is built without its thread
+    option).  Returns None when the reference build is absent."""
+    import numpy as np
+    from icep_kapsel_amd import abi
+    from oracle import oracle as omod
+
+    try:
+        ref = omod.OouraRef()
+    except OSError:
+        return None
+    a = np.zeros((n, n, n + 2))
+    a[:, :, :n] = np.random.default_rng(1).standard_normal((n, n, n))
+    threads = omod.num_threads()
+    omod.set_threads(1)
+    o = omod.Oracle(abi.make_params(n))
+    try:
+        best_o = best_r = float("inf")
+        for _ in range(reps):
+            o.set_field("WORK0", a)
+            t0 = time.perf_counter()
+            o.fft_r2c("WORK0")
+            o.fft_c2r("WORK0")
+            best_o = min(best_o, time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            ref.a_k2a(ref.a2a_k(a))
+            best_r = min(best_r, time.perf_counter() - t0)
+    finally:
+        omod.set_threads(threads)
+    best_all = float("inf")
+    for _ in range(reps):
+        o.set_field("WORK0", a)
+        t0 = time.perf_counter()
+        o.fft_r2c("WORK0")
+        o.fft_c2r("WORK0")
+        best_all = min(best_all, time.perf_counter() - t0)
+    return {"grid": n, "threads": 1, "oracle_fft_pair_ms": 1e3 * best_o, "ooura_rdft3d_pair_ms": 1e3 * best_r,
+            "oracle_over_ooura": best_o / best_r, "oracle_fft_pair_ms_all_threads": 1e3 * best_all,
+            "all_threads": threads}
 
 
 def pmc_traffic(kernel_class):
@@ -346,6 +405,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cold", type=int, default=3, help="held-out cold solves (zero vs network warm start)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=8, help="FP64 oracle ICEP steps of the CPU baseline")
+    ap.add_argument("--cpu-el-steps", type=int, default=4, help="FP64 oracle electrolyte steps")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 plain run")
     ap.add_argument("--no-spm", action="store_true", help="skip the SPM-only step timings")
     ap.add_argument("--el-steps", type=int, default=50, help="timed steps of the 256^3 electrolyte workload")
@@ -383,12 +444,38 @@ def main():
     log(f"bench: fp32 {r32['steps_per_s']:.2f} steps/s, {r32['iters_mean']:.1f} inner it/step, "
         f"warm-up iterations {r32['warmup_iters']}, dominant {r32['dominant']}")
     r64 = None
+    split = None
     if not args.no_fp64:
         log("bench: FP64 plain CG ICEP step")
         r64 = run_icep(abi.KAPSEL_FP64, abi.PRECOND_NONE, args.steps, args.warmup, learned=False,
                        timed_class=r32["dominant"], snapshot=not args.no_cpu)
         log(f"bench: fp64 plain {r64['steps_per_s']:.2f} steps/s, {r64['iters_mean']:.1f} it/step, "
             f"warm-up iterations {r64['warmup_iters']}")
+        # the same step with one ingredient of the chosen form at a time: the
+        # spectral preconditioner in FP64, then FP32 + refinement without the
+        # network's cold start
+        log("bench: FP64 spectral PCG ICEP step")
+        r64s = run_icep(abi.KAPSEL_FP64, abi.PRECOND_SPECTRAL, args.steps, args.warmup, learned=False,
+                        timed_class=r32["dominant"])
+        log(f"bench: fp64 spectral {r64s['steps_per_s']:.2f} steps/s, {r64s['iters_mean']:.1f} it/step")
+        log("bench: FP32 spectral PCG ICEP step without the network")
+        r32n = run_icep(abi.KAPSEL_FP32, abi.PRECOND_SPECTRAL, args.steps, args.warmup, learned=False)
+        log(f"bench: fp32 without network {r32n['steps_per_s']:.2f} steps/s, {r32n['iters_mean']:.1f} it/step")
+        split = {
+            "fp64_plain_cg_steps_per_s": r64["steps_per_s"],
+            "fp64_spectral_pcg_steps_per_s": r64s["steps_per_s"],
+            "fp64_spectral_pcg_iters_per_step": r64s["iters_mean"],
+            "fp32_spectral_pcg_refined_no_network_steps_per_s": r32n["steps_per_s"],
+            "fp32_spectral_pcg_refined_no_network_iters_per_step": r32n["iters_mean"],
+            "fp32_spectral_pcg_refined_network_cold_start_steps_per_s": r32["steps_per_s"],
+            "factor_preconditioner": r64s["steps_per_s"] / r64["steps_per_s"],
+            "factor_precision": r32n["steps_per_s"] / r64s["steps_per_s"],
+            "factor_network": r32["steps_per_s"] / r32n["steps_per_s"],
+            "first_step_s_network": r32["first_step_s"],
+            "first_step_s_no_network": r32n["first_step_s"],
+            "first_step_iters_network": r32["warmup_iters"][0] if r32["warmup_iters"] else None,
+            "first_step_iters_no_network": r32n["warmup_iters"][0] if r32n["warmup_iters"] else None,
+        }
     cold = cold_solves(args.cold) if args.cold else None
     spm = None
     if not args.no_spm:
@@ -397,13 +484,18 @@ def main():
                "fp64_steps_per_s": run_spm(abi.KAPSEL_FP64, 50, 5)}
     cpu = None
     if not args.no_cpu and r64 is not None:
-        log("bench: CPU oracle, one ICEP step from the GPU FP64 state after the warm-up")
-        cps, cores, cit = run_cpu(r64["snapshot"])
+        log(f"bench: CPU oracle, {args.cpu_steps} ICEP steps from the GPU FP64 state after the warm-up")
+        cps, cores, cit = run_cpu(r64["snapshot"], args.cpu_steps)
         cpu = {"value": cps, "unit": "steps/s", "cores": cores, "kind": "port",
-               "sample": f"1 FP64 ICEP step of the same 128^3 x 64-sphere workload from the state after "
-                         f"{args.warmup} warm-up steps, reference BiCGSTAB to 1e-8 ({cit} iterations), "
-                         "OpenMP C++ oracle"}
-        log(f"bench: cpu {cps:.4f} steps/s on {cores} threads ({cit} BiCGSTAB iterations)")
+               "sample": f"{args.cpu_steps} FP64 ICEP steps of the same 128^3 x 64-sphere workload from the "
+                         f"state after {args.warmup} warm-up steps, reference BiCGSTAB to 1e-8 ({cit:.1f} "
+                         "iterations per step), OpenMP C++ oracle",
+               "cpu_model": cpu_model(), "nproc": os.cpu_count()}
+        log(f"bench: cpu {cps:.4f} steps/s on {cores} threads ({cit:.1f} BiCGSTAB iterations per step)")
+        cal = fft_calibration()
+        if cal:
+            cpu["fft_calibration"] = cal
+            log(f"bench: oracle FFT / reference Ooura rdft3d at 128^3, 1 thread: {cal['oracle_over_ooura']:.3f}")
     avg_ms = r32["dominant_ms"] / max(1, r32["dominant_launches"])
     achieved = (r32["dominant_bytes"] / 1e9) / (r32["dominant_ms"] / 1e3) if r32["dominant_ms"] > 0 else 0.0
     traffic = pmc_traffic(r32["dominant"])
@@ -423,7 +515,7 @@ def main():
         "config": {"workload": "icep_step_128^3_64_dielectric_spheres_a6_xi2_eps0.1_E0.1", "grid": 128,
                    "particles": 64, "radius": 6.0, "xi": 2.0, "eps_ratio": 0.1, "e0_magnitude": 0.1,
                    "dt": 0.05, "poisson_tol": TOL, "seed": 3003,
-                   "solver": "fp32 spectral PCG in fp64 refinement, learned cold start",
+                   "solver": "fp32 spectral PCG in fp64 refinement, learned cold start (first solve only)",
                    "baseline_solver": "fp64 plain CG", "parallelism": "single_gpu"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
@@ -451,6 +543,8 @@ def main():
             "vorticity_rel_err_vs_fp64": rel_diff(r32["zeta"], r64["zeta"]),
             "kernel_class_ms_per_step_fp64": r64["shares_ms"],
         })
+    if split:
+        out["solver_split"] = split
     if cold:
         out["cold_solve_1e-8"] = cold["mean"]
     if spm:
@@ -479,11 +573,12 @@ def electrolyte_record(args):
            "kernel_class_launches_per_step_fp32": e32["shares_launches"],
            "kernel_class_ms_per_step_fp64": e64["shares_ms"]}
     if not args.no_cpu:
-        log("bench: CPU oracle, one electrolyte step from the GPU FP64 state after the warm-up")
-        cps, cores = run_cpu_electrolyte(e64["snapshot"])
+        log(f"bench: CPU oracle, {args.cpu_el_steps} electrolyte steps from the GPU FP64 state after the warm-up")
+        cps, cores = run_cpu_electrolyte(e64["snapshot"], args.cpu_el_steps)
         rec["cpu_baseline"] = {"value": cps, "unit": "steps/s", "cores": cores, "kind": "port",
-                               "sample": "1 FP64 electrolyte step of the same 256^3 x 2000-sphere workload "
-                                         "from the state after 3 warm-up steps, OpenMP C++ oracle"}
+                               "sample": f"{args.cpu_el_steps} FP64 electrolyte steps of the same 256^3 x "
+                                         "2000-sphere workload from the state after 3 warm-up steps, OpenMP C++ "
+                                         "oracle"}
         log(f"bench: electrolyte cpu {cps:.4f} steps/s on {cores} threads")
     return rec
 
