@@ -72,42 +72,116 @@ def test_fp32_per_field_error_c2():
         assert max(e[f] for f in FIELDS if f not in SCRATCH) < 1e-4
 
 
-def test_icep_fp32_velocity_error():
-    """ICEP box, 5 steps: FP32 context (spectral PCG, FP64 refinement, tol 1e-8)
-    against the FP64 context (plain CG, tol 1e-8)."""
+def _icep_run(prec, precond, tol, steps=5):
     from icep_kapsel_amd.gpu import KapselGPU
-    print("case precision icep fp32 vs fp64", flush=True)
-    ks = []
-    for prec, pc in ((abi.KAPSEL_FP64, abi.PRECOND_NONE), (abi.KAPSEL_FP32, abi.PRECOND_SPECTRAL)):
-        p, st, e0, dt = workloads.icep_config(precision=prec)
-        k = KapselGPU(p)
-        k.set_particles(st)
-        k.init_rigidity()
-        t = abi.make_time(dt)
-        for _ in range(5):
-            it, rr = k.icep_step(t, e0, 1e-8, 20000, pc)
-            assert rr < 1e-8
-        ks.append(k)
-    k64, k32 = ks
-    e = _errors(k64, k32)
+    p, st, e0, dt = workloads.icep_config(precision=prec)
+    k = KapselGPU(p)
+    k.set_particles(st)
+    k.init_rigidity()
+    t = abi.make_time(dt)
+    for _ in range(steps):
+        it, rr = k.icep_step(t, e0, tol, 20000, precond)
+        assert rr < tol
+    return k
+
+
+def _icep_errors(k, ref):
+    e = _errors(ref, k)
     for f in SCRATCH:
         del e[f]
-    e["POTENTIAL"] = rel(k32.get_field("POTENTIAL") - k32.get_field("POTENTIAL")[:, :, :128].mean(),
-                         k64.get_field("POTENTIAL") - k64.get_field("POTENTIAL")[:, :, :128].mean())
-    # rigid-body velocity: surface velocity |v| + a |omega| of each sphere
-    # (the angular velocities alone are small here: spheres in a uniform
-    # field carry almost no torque, so their own relative error is larger)
+    n = k.nz
+    e["POTENTIAL"] = rel(k.get_field("POTENTIAL") - k.get_field("POTENTIAL")[:, :, :n].mean(),
+                         ref.get_field("POTENTIAL") - ref.get_field("POTENTIAL")[:, :, :n].mean())
+    # a |omega - omega_64| against the largest particle speed: the spheres'
+    # rotation is small beside their translation (a dielectric sphere in a
+    # uniform field carries no torque; here only the neighbours' fields turn it)
     a = 6.0
-    p64, p32 = k64.get_particles(), k32.get_particles()
-    v64, v32 = p64[abi.PFIELD["V"]], p32[abi.PFIELD["V"]]
-    w64, w32 = p64[abi.PFIELD["OMEGA"]], p32[abi.PFIELD["OMEGA"]]
-    surf = np.linalg.norm(v64, axis=0) + a * np.linalg.norm(w64, axis=0)
-    dsurf = np.linalg.norm(v32 - v64, axis=0) + a * np.linalg.norm(w32 - w64, axis=0)
-    e["rigid_body_velocity"] = float(dsurf.max() / surf.max())
-    print("icep after 5 steps:", {f: f"{v:.1e}" for f, v in e.items()}, flush=True)
-    assert e["V"] < 1e-5 and e["rigid_body_velocity"] < 1e-5
-    assert max(e["U0"], e["U1"], e["U2"]) < 1e-5
-    assert e["OMEGA"] < 1e-4 and max(e["ZETA0"], e["ZETA1"]) < 1e-4
+    pk, pr = k.get_particles(), ref.get_particles()
+    e["a_domega_over_vmax"] = float(a * np.abs(pk[abi.PFIELD["OMEGA"]] - pr[abi.PFIELD["OMEGA"]]).max() /
+                                    np.abs(pr[abi.PFIELD["V"]]).max())
+    e["a_omega_over_vmax"] = float(a * np.abs(pr[abi.PFIELD["OMEGA"]]).max() / np.abs(pr[abi.PFIELD["V"]]).max())
+    return e
+
+
+def test_icep_fp32_velocity_error():
+    """ICEP box, 5 steps, FP32 context (spectral PCG, FP64 refinement) against
+    the FP64 context (plain CG).
+
+    Two sources of difference: the solve's own tolerance (two different
+    iterations stopped at the same relative residual leave different
+    potentials) and FP32 rounding of the fields.  The first is measured by two
+    FP64 runs at 1e-8 (plain CG against spectral PCG) and by repeating the
+    comparison at 1e-11: it is below 1e-7 everywhere, so the FP32-FP64
+    difference is FP32 rounding.  Bounds: particle and fluid velocities and
+    zeta 1e-5 (the target).  omega and the hydrodynamic torque 1e-4 relative:
+    a dielectric sphere in a uniform field carries almost no torque, a |omega|
+    is about 2 % of the largest particle speed, and the torque -- a moment of
+    (u_p - u) phi over the sphere -- turns the ~1e-6 FP32 error of u into a
+    relative error some 50 times larger; in absolute terms a |d omega| stays
+    within 1e-5 of the largest particle speed (asserted)."""
+    print("case precision icep fp32 vs fp64", flush=True)
+    k64 = _icep_run(abi.KAPSEL_FP64, abi.PRECOND_NONE, 1e-8)
+    k64s = _icep_run(abi.KAPSEL_FP64, abi.PRECOND_SPECTRAL, 1e-8)
+    k32 = _icep_run(abi.KAPSEL_FP32, abi.PRECOND_SPECTRAL, 1e-8)
+    e32 = _icep_errors(k32, k64)
+    esol = _icep_errors(k64s, k64)
+    for k in (k64, k64s, k32):
+        k.close()
+    print("tol 1e-8, fp32 vs fp64:", {f: f"{v:.1e}" for f, v in e32.items()}, flush=True)
+    print("tol 1e-8, fp64 spectral vs fp64 plain:", {f: f"{v:.1e}" for f, v in esol.items()}, flush=True)
+    k64 = _icep_run(abi.KAPSEL_FP64, abi.PRECOND_SPECTRAL, 1e-11)
+    k32 = _icep_run(abi.KAPSEL_FP32, abi.PRECOND_SPECTRAL, 1e-11)
+    etight = _icep_errors(k32, k64)
+    k64.close()
+    k32.close()
+    print("tol 1e-11, fp32 vs fp64:", {f: f"{v:.1e}" for f, v in etight.items()}, flush=True)
+    solver_fields = [f for f in esol if f != "a_omega_over_vmax"]
+    assert max(esol[f] for f in solver_fields) < 1e-7, esol
+    for e in (e32, etight):
+        assert e["V"] < 1e-5 and max(e["U0"], e["U1"], e["U2"]) < 1e-5
+        assert max(e["ZETA0"], e["ZETA1"]) < 1e-5
+        assert e["a_domega_over_vmax"] < 1e-5
+        assert e["OMEGA"] < 1e-4 and e["TORQUE_HYDRO_PREVIOUS"] < 1e-4
+
+
+def test_electrolyte_fp32_drift_500_steps_256():
+    """The scale + electrolyte workload at its stated length: 256^3, 2000
+    charged spheres, 500 steps, FP32 against FP64 from the same start.
+    Reported: the relative difference of particle velocities, the ion
+    spectra, zeta and positions at steps 1, 50, 100, 250, 500.  The velocity
+    difference grows about linearly, 2e-7 per step (measured: 1.1e-5 at step
+    50, 9.2e-5 at step 500): FP32 rounding of the fluid fields enters every
+    step's gather and accumulates in the particle momenta.  Bounds: particle
+    velocity 2e-4, ion spectra 1e-5, positions 1e-6 of the box."""
+    from icep_kapsel_amd.gpu import KapselGPU
+    print("case electrolyte 256^3 x 2000 fp32 vs fp64, 500 steps", flush=True)
+    ks = []
+    for prec in (abi.KAPSEL_FP64, abi.KAPSEL_FP32):
+        p, st, el, dt = workloads.electrolyte_config(precision=prec)
+        k = KapselGPU(p)
+        k.set_particles(st)
+        k.set_electrolyte(el)
+        k.init_rigidity()
+        k.init_rho_ion()
+        ks.append((k, abi.make_time(dt)))
+    (k64, t64), (k32, t32) = ks
+    worst = {}
+    for step in range(1, 501):
+        k64.electrolyte_step(t64)
+        k32.electrolyte_step(t32)
+        if step in (1, 50, 100, 250, 500):
+            p64, p32 = k64.get_particles(), k32.get_particles()
+            e = {f: rel(p32[abi.PFIELD[f]], p64[abi.PFIELD[f]]) for f in ("V", "OMEGA")}
+            e["X_over_box"] = float(np.abs(p32[abi.PFIELD["X_NOPBC"]] - p64[abi.PFIELD["X_NOPBC"]]).max() / 256.0)
+            for f in ("CONC0", "CONC1", "ZETA0", "ZETA1"):
+                e[f] = rel(k32.get_field(f), k64.get_field(f))
+            worst = e
+            print(f"step {step}: " + ", ".join(f"{a} {b:.2e}" for a, b in e.items()), flush=True)
+    k64.close()
+    k32.close()
+    assert worst["V"] < 2e-4
+    assert max(worst["CONC0"], worst["CONC1"]) < 1e-5
+    assert worst["X_over_box"] < 1e-6
 
 
 def test_fp64_drift_1000_steps_c1():
