@@ -104,6 +104,46 @@ def test_poisson_solve_fp32_refined(n, tol):
     assert rr3 < tol and it3 > 0
 
 
+def test_poisson_solve_non_cubic_grid():
+    """128 x 64 x 32 grid: the fused x passes of the correction and of the FP64
+    residual (chosen by NX) meet the separate y line and z face passes (no
+    plane pass for NY != NZ).  FP64 solves (plain and spectral) against the
+    oracle; the FP32 refined solve against the FP64 solution."""
+    shape = (128, 64, 32)
+    print(f"case poisson non-cubic {shape}", flush=True)
+    pos = workloads.random_positions(workloads.Xoshiro256(17), 5, shape, 8.0)
+    e0 = [0.03, -0.04, 0.1]
+    sols = {}
+    for prec in (abi.KAPSEL_FP64, abi.KAPSEL_FP32):
+        p = abi.make_params(shape, n_particles=5, radius=4.0, xi=2.0, eps_fluid=1.0, eps_particle=0.1,
+                            precision=prec)
+        k = gpu(p)
+        k.set_particles(workloads.particle_state(pos))
+        k.make_dielectric_field()
+        if prec == abi.KAPSEL_FP64:
+            o = oracle(p)
+            o.set_particles(workloads.particle_state(pos))
+            o.make_dielectric_field()
+            for precond in (0, 1):
+                for s in (k, o):
+                    s.set_field("POTENTIAL", np.zeros(k.shape))
+                itk, rrk = k.poisson_solve(e0, 1e-10, 5000, precond)
+                ito, rro = o.poisson_solve(e0, 1e-10, 5000, precond)
+                print(f"fp64 precond={precond}: gpu {itk} it ({rrk:.1e}), oracle {ito} it ({rro:.1e})", flush=True)
+                assert rrk < 1e-10 and abs(itk - ito) <= max(3, ito // 50)
+                xk, xo = k.get_field("POTENTIAL")[:, :, :32], o.get_field("POTENTIAL")[:, :, :32]
+                assert rel(xk - xk.mean(), xo - xo.mean()) < 1e-8
+            sols[prec] = xk
+        else:
+            it, rr = k.poisson_solve(e0, 1e-10, 5000, 1)
+            print(f"fp32 refined: {it} inner iterations, fp64 relres {rr:.1e}", flush=True)
+            assert rr < 1e-10
+            x = k.get_field("POTENTIAL")[:, :, :32]
+            x64 = sols[abi.KAPSEL_FP64]
+            assert rel(x - x.mean(), x64 - x64.mean()) < 1e-5
+        k.close()
+
+
 def test_poisson_solve_128_dielectric_spheres():
     """128^3, 64 spheres a=6, eps ratio 0.1 (the ICEP configuration): FP64
     plain CG and spectral PCG both reach 1e-10 and agree."""
