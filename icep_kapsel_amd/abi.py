@@ -178,6 +178,7 @@ _PROTOS = {
     "kapsel_set_dielectric": [_D, _D, _D],
     "kapsel_maxwell_force": [_DP],
     "kapsel_icep_step": [ctypes.POINTER(Time), _DP, _D, _I, _I, _IP, _DP],
+    "kapsel_set_concurrency": [_I],
     "kapsel_set_thermostat": [_D, _D, _D, ctypes.c_ulong],
     "kapsel_thermostat": [ctypes.POINTER(Time)],
     "kapsel_save_restart": [ctypes.c_char_p, ctypes.POINTER(Time), ctypes.c_char_p],

@@ -53,6 +53,7 @@ LaunchTimer::~LaunchTimer() noexcept(false) {
 
 static void harvest_timing(kapsel_ctx *ctx) {
   if (ctx->timing_pending.empty()) return;
+  KAPSEL_HIP(hipStreamSynchronize(ctx->side));
   KAPSEL_HIP(hipStreamSynchronize(ctx->stream));
   for (const TimingRecord &r : ctx->timing_pending) {
     float ms = 0.f;
@@ -107,8 +108,14 @@ static void written(kapsel_ctx *ctx, int f) {
 static void destroy(kapsel_ctx *ctx) {
   if (!ctx) return;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->side) (void)hipStreamSynchronize(ctx->side);
   for (void *&p : ctx->fields)
     if (p) (void)hipFree(p);
+  for (void *&p : ctx->ns_work)
+    if (p) (void)hipFree(p);
+  if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+  if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+  if (ctx->side) (void)hipStreamDestroy(ctx->side);
   if (ctx->d_uk_dc) (void)hipFree(ctx->d_uk_dc);
   if (ctx->d_ps) (void)hipFree(ctx->d_ps);
   if (ctx->d_q) (void)hipFree(ctx->d_q);
@@ -206,10 +213,15 @@ static void create(const kapsel_params *p, kapsel_ctx **out) {
       ctx->a_r_cutoff = 0.;
     }
     KAPSEL_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    KAPSEL_HIP(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+    KAPSEL_HIP(hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+    KAPSEL_HIP(hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
     for (int f = 0; f < KAPSEL_NFIELDS; f++) {
       KAPSEL_HIP(hipMalloc(&ctx->fields[f], g.f * ctx->elem));
       KAPSEL_HIP(hipMemsetAsync(ctx->fields[f], 0, g.f * ctx->elem, ctx->stream));
     }
+    ctx->adv_w[0] = ctx->fields[KAPSEL_WORK0];
+    ctx->adv_w[1] = ctx->fields[KAPSEL_WORK1];
     KAPSEL_HIP(hipMalloc(&ctx->d_uk_dc, 3 * sizeof(double)));
     KAPSEL_HIP(hipMemsetAsync(ctx->d_uk_dc, 0, 3 * sizeof(double), ctx->stream));
     size_t np = p->n_particles > 0 ? (size_t)p->n_particles : 1;
@@ -247,6 +259,9 @@ static void check_ctx_ro(kapsel_ctx *ctx) {
 static void check_ctx(kapsel_ctx *ctx) {
   check_ctx_ro(ctx);
   ctx->api_gen++;
+  // the FP32 POTENTIAL/RHS an ICEP step left to be made: every entry point
+  // but the ICEP step itself may read or overwrite them
+  materialize_outputs(ctx);
 }
 
 static void check_time(const kapsel_time *t) {
@@ -298,8 +313,7 @@ static void zeta_k2u(kapsel_ctx *ctx, int fix_dc) {
 static void ns_slaved_euler(kapsel_ctx *ctx, const kapsel_time &t) {
   zeta_k2u(ctx, 0);
   advection_products(ctx);
-  void *five[5] = {fld(ctx, KAPSEL_U0), fld(ctx, KAPSEL_U1), fld(ctx, KAPSEL_U2), fld(ctx, KAPSEL_WORK0),
-                   fld(ctx, KAPSEL_WORK1)};
+  void *five[5] = {fld(ctx, KAPSEL_U0), fld(ctx, KAPSEL_U1), fld(ctx, KAPSEL_U2), ctx->adv_w[0], ctx->adv_w[1]};
   fft_r2c(ctx, five, 5);
   advection_combine_slaved(ctx, t.dt_fluid);
 }
@@ -385,15 +399,80 @@ static void spm_step(kapsel_ctx *ctx, kapsel_time &t) {
 // dielectric force of the Electrolyte+Dielectric branch inserted after the
 // profile scatter (sp_3d_ns.cxx:51-95): eps, Poisson solve warm-started from
 // the previous potential, Maxwell force, u += f dt / rho, projection
+//
+// The Navier-Stokes part (the slaved step and Zeta_k2u, which read and write
+// zeta, uk_dc, U and the advection's two extra product fields) does not touch
+// what the particle and Poisson part reads or writes (particles, PHI,
+// EPSILON, the potential, the solver's vectors WORK0..WORK3 and scratch),
+// and only u += f dt / rho joins them.  With concurrency on, the NS part is
+// queued on the context's second stream with its own two product fields
+// while the Poisson part, whose CG and refinement leave the device partly
+// idle (single-workgroup reductions, plane passes on 1.5 planes per CU, host
+// read-backs), runs on the context stream; an event joins them.  Same
+// kernels, same inputs: the results are those of the one-stream order.
+struct SideStream {
+  kapsel_ctx *c;
+  hipStream_t main;
+  bool refine;
+  explicit SideStream(kapsel_ctx *x) : c(x), main(x->stream), refine(x->refine64) {
+    c->stream = c->side;
+    c->refine64 = false;  // queued from inside the solve: not refinement work
+    c->adv_w[0] = c->ns_work[0];
+    c->adv_w[1] = c->ns_work[1];
+  }
+  ~SideStream() {
+    c->stream = main;
+    c->refine64 = refine;
+    c->adv_w[0] = c->fields[KAPSEL_WORK0];
+    c->adv_w[1] = c->fields[KAPSEL_WORK1];
+  }
+};
+
 static void icep_step(kapsel_ctx *ctx, kapsel_time &t, const double e0[3], double tol, int maxit, int precond,
                       int *iters, double *relres) {
-  ns_slaved_euler(ctx, t);
-  zeta_k2u(ctx, fix_mask(ctx));
+  // (kernel timing records each launch's events on the stream it goes to)
+  const bool split = ctx->concurrent;
+  // joins the context stream to the NS chain on every way out (an error in the
+  // Poisson part included), so nothing later on the context stream races it
+  struct Join {
+    kapsel_ctx *c;
+    bool armed = false;
+    void now() {
+      if (armed) {
+        armed = false;
+        KAPSEL_HIP(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+      }
+    }
+    ~Join() {
+      if (armed) (void)hipStreamWaitEvent(c->stream, c->ev_join, 0);
+    }
+  } join{ctx};
+  // the NS chain runs beside the start of the Poisson part (profile,
+  // permittivity, the refinement's set-up and first FP64 residual, whose FP64
+  // plane pass runs a second round on half the CUs) and is done before the
+  // FP32 CG starts
+  if (split) {
+    for (void *&w : ctx->ns_work)
+      if (!w) KAPSEL_HIP(hipMalloc(&w, ctx->g.f * ctx->elem));
+    KAPSEL_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
+    KAPSEL_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+    {
+      SideStream on_side(ctx);
+      ns_slaved_euler(ctx, t);
+      zeta_k2u(ctx, fix_mask(ctx));
+    }
+    KAPSEL_HIP(hipEventRecord(ctx->ev_join, ctx->side));
+    join.armed = true;
+  } else {
+    ns_slaved_euler(ctx, t);
+    zeta_k2u(ctx, fix_mask(ctx));
+  }
   md_position(ctx, t);
   make_phi_particle_sum(ctx);
   dielectric_field(ctx);
   poisson_solve(ctx, e0, tol, maxit, precond, iters, relres);
   maxwell_force(ctx, e0, true);
+  join.now();
   add_force_to_u(ctx, t.dt_fluid * ctx->irho);
   void *keep3[3] = {ctx->pscratch[5], ctx->pscratch[6], ctx->pscratch[7]};
   const bool fp32 = ctx->prec == KAPSEL_FP32;
@@ -538,7 +617,15 @@ int kapsel_destroy(kapsel_ctx *ctx) {
 int kapsel_synchronize(kapsel_ctx *ctx) {
   API_BEGIN
   check_ctx_ro(ctx);
+  KAPSEL_HIP(hipStreamSynchronize(ctx->side));
   KAPSEL_HIP(hipStreamSynchronize(ctx->stream));
+  API_END
+}
+
+int kapsel_set_concurrency(kapsel_ctx *ctx, int on) {
+  API_BEGIN
+  check_ctx(ctx);
+  ctx->concurrent = on != 0;
   API_END
 }
 
@@ -575,6 +662,7 @@ int kapsel_get_field(kapsel_ctx *ctx, int field, double *host) {
     set_error("null host buffer");
     throw Error{KAPSEL_ERR_ARG};
   }
+  if (field == KAPSEL_POTENTIAL || field == KAPSEL_RHS) materialize_outputs(ctx);
   download(ctx, host, ctx->fields[field]);
   API_END
 }
@@ -995,14 +1083,24 @@ int kapsel_maxwell_force(kapsel_ctx *ctx, const double e0[3]) {
 int kapsel_icep_step(kapsel_ctx *ctx, kapsel_time *jikan, const double e0[3], double tol, int maxit, int precond,
                      int *iters, double *relres) {
   API_BEGIN
-  check_ctx(ctx);
+  // no materialize_outputs: the step's solve reads the FP64 shadow, not the
+  // FP32 POTENTIAL, and its outputs replace the pending ones
+  check_ctx_ro(ctx);
+  ctx->api_gen++;
   check_time(jikan);
   if (!e0 || !iters || !relres || !(tol > 0) || maxit < 1 ||
       (precond != KAPSEL_PRECOND_NONE && precond != KAPSEL_PRECOND_SPECTRAL)) {
     set_error("invalid icep_step arguments");
     throw Error{KAPSEL_ERR_ARG};
   }
-  icep_step(ctx, *jikan, e0, tol, maxit, precond, iters, relres);
+  {
+    struct Lazy {
+      kapsel_ctx *c;
+      explicit Lazy(kapsel_ctx *x) : c(x) { c->lazy_outputs = true; }
+      ~Lazy() { c->lazy_outputs = false; }
+    } lazy(ctx);
+    icep_step(ctx, *jikan, e0, tol, maxit, precond, iters, relres);
+  }
   if (!(*relres < tol)) {
     set_error("icep_step: Poisson solve stopped at relative residual %.3e", *relres);
     return KAPSEL_ERR_NOCONV;

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <random>
 #include <string>
 #include <vector>
@@ -166,6 +167,16 @@ struct kapsel_ctx {
   int prec;  // KAPSEL_FP64 or KAPSEL_FP32
   size_t elem;  // sizeof(T)
   hipStream_t stream = nullptr;
+  // a second stream (default priority) for the ICEP step's Navier-Stokes
+  // chain, which runs beside the Poisson chain on `stream`; forked and joined
+  // with the two events
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  bool concurrent = true;         // kapsel_set_concurrency
+  void *adv_w[2] = {};            // the advection's two extra product fields (WORK0, WORK1 by default)
+  void *ns_work[2] = {};          // their stand-ins while the NS chain runs beside the solve
+  bool lazy_outputs = false;      // an FP32 solve leaves POTENTIAL/RHS to materialize_outputs
+  bool outputs_stale = false;     // POTENTIAL/RHS wait for materialize_outputs
   int device = 0;
   void *fields[KAPSEL_NFIELDS] = {};
   double *d_uk_dc = nullptr;      // 3 doubles (device)
@@ -363,6 +374,7 @@ void poisson_rhs(kapsel_ctx *ctx, const double e0[3]);
 void poisson_solve(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int precond, int *iters,
                    double *relres);
 void poisson_free(kapsel_ctx *ctx);
+void materialize_outputs(kapsel_ctx *ctx);
 double galerkin_scale_potential(kapsel_ctx *ctx, const double e0[3]);
 // after_solve: called right after poisson_solve, whose FP64 spectra of eps
 // and the potential may then stand in for the transforms of the fields

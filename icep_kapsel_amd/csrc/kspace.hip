@@ -356,8 +356,8 @@ template <typename T>
 static void advection_products_t(kapsel_ctx *ctx) {
   LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, 8.0 * ctx->g.nr * sizeof(T));
   hipLaunchKernelGGL(advection_products_kernel<T>, dim3(blocks_for(ctx->g.nr, TPB)), dim3(TPB), 0, ctx->stream,
-                     F<T>(ctx, KAPSEL_U0), F<T>(ctx, KAPSEL_U1), F<T>(ctx, KAPSEL_U2), F<T>(ctx, KAPSEL_WORK0),
-                     F<T>(ctx, KAPSEL_WORK1), ctx->g);
+                     F<T>(ctx, KAPSEL_U0), F<T>(ctx, KAPSEL_U1), F<T>(ctx, KAPSEL_U2), (T *)ctx->adv_w[0],
+                     (T *)ctx->adv_w[1], ctx->g);
   KAPSEL_LAUNCH_CHECK();
 }
 void advection_products(kapsel_ctx *ctx) { KAPSEL_DISPATCH(ctx, advection_products_t, ctx); }
@@ -369,7 +369,7 @@ static void advection_combine_slaved_t(kapsel_ctx *ctx, double dt_fluid) {
   LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, (7.0 * ctx->g.f + 8.0 * kept_count(ctx->g)) * sizeof(T));
   hipLaunchKernelGGL((advection_combine_slaved_kernel<T, 0>), dim3(blocks_for(ctx->g.nc, TPB)), dim3(TPB), 0,
                      ctx->stream, F<T>(ctx, KAPSEL_U0), F<T>(ctx, KAPSEL_U1), F<T>(ctx, KAPSEL_U2),
-                     F<T>(ctx, KAPSEL_WORK0), F<T>(ctx, KAPSEL_WORK1), F<T>(ctx, KAPSEL_ZETA0),
+                     (T *)ctx->adv_w[0], (T *)ctx->adv_w[1], F<T>(ctx, KAPSEL_ZETA0),
                      F<T>(ctx, KAPSEL_ZETA1), ctx->g, dmy0, dmy1);
   KAPSEL_LAUNCH_CHECK();
 }
@@ -382,7 +382,7 @@ static void advection_combine_euler_t(kapsel_ctx *ctx, double dt_fluid) {
   LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, (7.0 * ctx->g.f + 4.0 * kept_count(ctx->g)) * sizeof(T));
   hipLaunchKernelGGL((advection_combine_slaved_kernel<T, 1>), dim3(blocks_for(ctx->g.nc, TPB)), dim3(TPB), 0,
                      ctx->stream, F<T>(ctx, KAPSEL_U0), F<T>(ctx, KAPSEL_U1), F<T>(ctx, KAPSEL_U2),
-                     F<T>(ctx, KAPSEL_WORK0), F<T>(ctx, KAPSEL_WORK1), F<T>(ctx, KAPSEL_ZETA0),
+                     (T *)ctx->adv_w[0], (T *)ctx->adv_w[1], F<T>(ctx, KAPSEL_ZETA0),
                      F<T>(ctx, KAPSEL_ZETA1), ctx->g, ctx->nu, dt_fluid);
   KAPSEL_LAUNCH_CHECK();
 }

@@ -1139,6 +1139,20 @@ struct RefineScope {
   ~RefineScope() { c->refine64 = false; }
 };
 
+// The FP32 outputs of an FP32-context solve: POTENTIAL and RHS as real
+// fields, from the FP64 spectra of the solution (p64[1]) and of the
+// right-hand side (p64[3]).
+void solve_outputs_impl(kapsel_ctx *ctx) {
+  const Grid &g = ctx->g;
+  const unsigned fb = blocks_for(g.f, TPB);
+  float *pot = fld<float>(ctx, KAPSEL_POTENTIAL), *b32 = fld<float>(ctx, KAPSEL_RHS);
+  hipLaunchKernelGGL((convert_real_kernel<float, double>), dim3(fb), dim3(TPB), 0, ctx->stream, pot, ctx->p64[1], g);
+  hipLaunchKernelGGL((convert_real_kernel<float, double>), dim3(fb), dim3(TPB), 0, ctx->stream, b32, ctx->p64[3], g);
+  KAPSEL_LAUNCH_CHECK();
+  void *p2[2] = {pot, b32};
+  fft_c2r_t<float>(ctx, p2, 2);
+}
+
 // FP32 context: FP64 iterative refinement around FP32 (P)CG corrections.
 // The host waits twice per solve besides the correction's own pipelined
 // checks: once for ||b|| and the first residual, once for each later
@@ -1209,14 +1223,17 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
   const double norm = 1.0 / (double)g.nr;  // Parseval factor of the k-space sums
   float *b32 = fld<float>(ctx, KAPSEL_RHS);  // the correction's right-hand side spectrum meanwhile
   float *pot = fld<float>(ctx, KAPSEL_POTENTIAL);
-  // outputs: the potential and the RHS b (FP32, real)
+  // outputs: the potential and the RHS b (FP32, real); with lazy_outputs
+  // (inside kapsel_icep_step, whose later parts read the FP64 spectra) they
+  // are only marked stale and made when the API next reads or uses them
   auto outputs = [&]() {
-    hipLaunchKernelGGL((convert_real_kernel<float, double>), dim3(fb), dim3(TPB), 0, ctx->stream, pot, xk, g);
-    hipLaunchKernelGGL((convert_real_kernel<float, double>), dim3(fb), dim3(TPB), 0, ctx->stream, b32, bk, g);
-    KAPSEL_LAUNCH_CHECK();
-    void *p2[2] = {pot, b32};
-    fft_c2r_t<float>(ctx, p2, 2);
+    if (ctx->lazy_outputs) {
+      ctx->outputs_stale = true;
+      return;
+    }
+    solve_outputs_impl(ctx);
   };
+  ctx->outputs_stale = false;
   // rk = A xk - bk in k-space, ||r||^2 by Parseval (raw sum also into the
   // host slot 3); where NX has fused x passes, the gradient symbols ride in
   // the first and the divergence and the residual's partial sums in the last
@@ -1473,6 +1490,12 @@ void poisson_solve(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, i
   ensure_storage(ctx);
   if (ctx->prec == KAPSEL_FP64) *iters = solve_fp64(ctx, e0, tol, maxit, precond, relres);
   else *iters = solve_fp32(ctx, e0, tol, maxit, precond, relres);
+}
+
+void materialize_outputs(kapsel_ctx *ctx) {
+  if (!ctx->outputs_stale) return;
+  ctx->outputs_stale = false;
+  solve_outputs_impl(ctx);
 }
 
 void poisson_free(kapsel_ctx *ctx) {

@@ -194,6 +194,9 @@ class KapselGPU:
         e = np.ascontiguousarray(e0, dtype=np.float64)
         self._call("maxwell_force", _dp(e))
 
+    def set_concurrency(self, on):
+        self._call("set_concurrency", 1 if on else 0)
+
     def icep_step(self, t, e0, tol, maxit, precond=0, check=True):
         e = np.ascontiguousarray(e0, dtype=np.float64)
         it = ctypes.c_int(0)

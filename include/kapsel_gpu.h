@@ -229,6 +229,13 @@ int kapsel_maxwell_force(kapsel_ctx *ctx, const double e0[3]);
 int kapsel_icep_step(kapsel_ctx *ctx, kapsel_time *jikan, const double e0[3], double tol, int maxit, int precond,
                      int *iters, double *relres);
 
+/* Whether kapsel_icep_step queues its Navier-Stokes part (slaved step and
+ * Zeta_k2u) on the context's second stream beside the particle and Poisson
+ * part (1, the default) or runs everything in order on one stream (0).  The
+ * results are the same; no reference counterpart (the reference runs its
+ * OpenMP loops in order). */
+int kapsel_set_concurrency(kapsel_ctx *ctx, int on);
+
 /* ---- learned warm start (no counterpart in the reference) ----------------- */
 /* A 3-level periodic convolutional encoder-decoder (182,083 parameters) maps
  * (phi_sin, beta phi_sin) -- the compact-sine sum left in WORK4 by

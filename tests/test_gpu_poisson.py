@@ -213,6 +213,74 @@ def test_icep_step_fp64_matches_oracle():
         assert rel(pk[i], po[i]) < 1e-9, f
 
 
+@pytest.mark.parametrize("prec,n,npart", [(abi.KAPSEL_FP64, 64, 6), (abi.KAPSEL_FP32, 64, 6),
+                                           (abi.KAPSEL_FP32, 128, 64)])
+def test_icep_step_two_streams_bit_identical(prec, n, npart):
+    """kapsel_icep_step with its Navier-Stokes part on the second stream
+    (default) against the one-stream order: four steps, every field and the
+    particle state bit-identical (same kernels on the same inputs; a race
+    between the two chains would show as a difference)."""
+    print(f"case icep two streams vs one, {n}^3 prec={prec}", flush=True)
+    runs = []
+    for concurrent in (True, False):
+        p, st = _case(n, npart, 6.0, 0.1, 23, precision=prec)
+        k = gpu(p)
+        k.set_concurrency(concurrent)
+        k.set_particles(st)
+        k.init_rigidity()
+        t = abi.make_time(0.05)
+        its = []
+        for _ in range(4):
+            it, rr = k.icep_step(t, [0.02, 0.05, 0.1], 1e-8, 20000, 1)
+            its.append((it, rr))
+        runs.append((its, {f: k.get_field(f) for f in ("ZETA0", "ZETA1", "U0", "U1", "U2", "POTENTIAL", "PHI")},
+                     k.get_particles(), k.get_uk_dc()))
+        k.close()
+    (i1, f1, p1, u1), (i2, f2, p2, u2) = runs
+    print("iterations / residuals", i1, i2, flush=True)
+    assert i1 == i2
+    for f in f1:
+        assert np.array_equal(f1[f], f2[f]), f
+    assert np.array_equal(p1, p2) and np.array_equal(u1, u2)
+
+
+def test_icep_step_fp32_outputs_made_on_read():
+    """An FP32 ICEP step leaves the real POTENTIAL and RHS to be made when the
+    API next reads or uses them (its own later parts read the FP64 spectra):
+    after three steps the RHS read back equals Build_rhs recomputed in FP32
+    from the same permittivity, and the potential read back is the solution:
+    the FP64 operator leaves a residual of 1e-5 relative on it, what rounding
+    the FP64 solution to FP32 (6e-8 relative) does under A, whose symbol grows
+    as |k|^2 up to the grid scale (a wrong or stale potential would leave
+    order 1)."""
+    n = 64
+    print("case icep fp32 outputs on read", flush=True)
+    p, st = _case(n, 6, 6.0, 0.1, 24, precision=abi.KAPSEL_FP32)
+    k = gpu(p)
+    k.set_particles(st)
+    k.init_rigidity()
+    t = abi.make_time(0.05)
+    e0 = [0.02, 0.05, 0.1]
+    for _ in range(3):
+        k.icep_step(t, e0, 1e-8, 20000, 1)
+    rhs_made = k.get_field("RHS")[:, :, :n]
+    pot = k.get_field("POTENTIAL")
+    eps = k.get_field("EPSILON")
+    k.poisson_rhs(e0)
+    rhs_fresh = k.get_field("RHS")[:, :, :n]
+    assert rel(rhs_made, rhs_fresh) < 1e-5
+    p64, _ = _case(n, 6, 6.0, 0.1, 24)
+    k64 = gpu(p64)
+    k64.set_field("EPSILON", eps)
+    k64.set_field("POTENTIAL", pot)
+    k64.poisson_apply("POTENTIAL", "WORK0")
+    k64.poisson_rhs(e0)
+    r = k64.get_field("WORK0")[:, :, :n] - k64.get_field("RHS")[:, :, :n]
+    relres = np.linalg.norm(r) / np.linalg.norm(k64.get_field("RHS")[:, :, :n])
+    print(f"FP32 potential, FP64 residual {relres:.2e}", flush=True)
+    assert relres < 1e-4
+
+
 def test_fp32_solve_graph_replay_matches_eager():
     """The FP32 correction solve replays one captured k-space CG iteration as
     a HIP graph; with kernel timing on, the same iteration is launched
