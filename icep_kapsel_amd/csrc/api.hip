@@ -346,14 +346,26 @@ static void u2zeta_k(kapsel_ctx *ctx) {
 // projection writes there and the first inverse pass reads from there), the
 // second is never transformed back.  U holds that spectrum afterwards, F the
 // spectrum P f_k; the FP64 context keeps the reference's real-space order.
-static void solenoidal_u_keep(kapsel_ctx *ctx, void *const *keep3) {
-  fft_r2c(ctx, ctx->fields + KAPSEL_U0, 3);
-  solenoidal_uk(ctx, ctx->fields + KAPSEL_U0, keep3);
-  fft_c2r(ctx, ctx->fields + KAPSEL_U0, 3, keep3);
+// add_s > 0: u += f add_s first (the dielectric force, sp_3d_ns.cxx:77-84),
+// formed inside the forward transform where a plane pass applies
+static void solenoidal_u_keep(kapsel_ctx *ctx, void *const *keep3, double add_s = 0.0) {
+  void **u = ctx->fields + KAPSEL_U0;
+  if (!(add_s != 0.0 && fft_r2c_pro_t<float>(ctx, u, u, ctx->fields + KAPSEL_F0, nullptr, add_s, 3))) {
+    if (add_s != 0.0) add_force_to_u(ctx, add_s);
+    fft_r2c(ctx, u, 3);
+  }
+  solenoidal_uk(ctx, u, keep3);
+  fft_c2r(ctx, u, 3, keep3);
 }
+// f = u_p - phi u (Make_f_particle_dt_nonsole) formed inside the forward
+// transform of f where a plane pass applies
 static void f_sole_u2zeta_k(kapsel_ctx *ctx, void *const *keep3) {
-  make_f_nonsole(ctx);
-  fft_r2c(ctx, ctx->fields + KAPSEL_F0, 3);
+  void *phi3[3] = {fld(ctx, KAPSEL_PHI), fld(ctx, KAPSEL_PHI), fld(ctx, KAPSEL_PHI)};
+  if (!fft_r2c_pro_t<float>(ctx, ctx->fields + KAPSEL_F0, ctx->fields + KAPSEL_UP0, phi3, ctx->fields + KAPSEL_U0,
+                            -1.0, 3)) {
+    make_f_nonsole(ctx);
+    fft_r2c(ctx, ctx->fields + KAPSEL_F0, 3);
+  }
   solenoidal_uk(ctx, ctx->fields + KAPSEL_F0);
   u_k2zeta_sum_k(ctx, keep3, ctx->fields + KAPSEL_F0);
 }
@@ -473,11 +485,14 @@ static void icep_step(kapsel_ctx *ctx, kapsel_time &t, const double e0[3], doubl
   poisson_solve(ctx, e0, tol, maxit, precond, iters, relres);
   maxwell_force(ctx, e0, true);
   join.now();
-  add_force_to_u(ctx, t.dt_fluid * ctx->irho);
   void *keep3[3] = {ctx->pscratch[5], ctx->pscratch[6], ctx->pscratch[7]};
   const bool fp32 = ctx->prec == KAPSEL_FP32;
-  if (fp32) solenoidal_u_keep(ctx, keep3);
-  else solenoidal_u(ctx, ctx->fields + KAPSEL_U0);
+  if (fp32) {
+    solenoidal_u_keep(ctx, keep3, t.dt_fluid * ctx->irho);  // u += f dt / rho inside
+  } else {
+    add_force_to_u(ctx, t.dt_fluid * ctx->irho);
+    solenoidal_u(ctx, ctx->fields + KAPSEL_U0);
+  }
   calc_f_hydro(ctx, t);
   force(ctx);
   md_velocity_integrate(ctx, t);
@@ -549,11 +564,14 @@ static void icep_electrolyte_step(kapsel_ctx *ctx, kapsel_time &t, double tol, i
   const double e0[3] = {ctx->el.e_ext[0] * s, ctx->el.e_ext[1] * s, ctx->el.e_ext[2] * s};
   dielectric_potential(ctx, e0, tol, maxit, precond, iters, relres);  // :72-74
   maxwell_force(ctx, e0, true);
-  add_force_to_u(ctx, t.dt_fluid * ctx->irho);  // :75-84
   void *keep3[3] = {ctx->pscratch[5], ctx->pscratch[6], ctx->pscratch[7]};
   const bool fp32 = ctx->prec == KAPSEL_FP32;
-  if (fp32) solenoidal_u_keep(ctx, keep3);    // :85
-  else solenoidal_u(ctx, ctx->fields + KAPSEL_U0);
+  if (fp32) {
+    solenoidal_u_keep(ctx, keep3, t.dt_fluid * ctx->irho);  // :75-85, u += f dt / rho inside
+  } else {
+    add_force_to_u(ctx, t.dt_fluid * ctx->irho);  // :75-84
+    solenoidal_u(ctx, ctx->fields + KAPSEL_U0);   // :85
+  }
   calc_f_hydro(ctx, t);                       // :99-101
   force(ctx);                                 // :104-110
   md_velocity_integrate(ctx, t);

@@ -322,6 +322,9 @@ void fft_c2r_t(kapsel_ctx *ctx, void *const *fields, int nf, void *const *src = 
 // (face_d real fields, 0 in the row padding): the inverse and forward z (and,
 // where a plane fits in LDS, y) transforms and the product fused in one pass
 template <typename T>
+bool fft_r2c_pro_t(kapsel_ctx *ctx, void *const *out, void *const *a, void *const *b, void *const *c, double s,
+                   int nf);
+template <typename T>
 void fft_face_t(kapsel_ctx *ctx, void *const *g3, void *const *face3);
 // fft_face_t without its first (x inverse) and last (x forward) passes
 template <typename T>

@@ -548,6 +548,47 @@ __global__ KAPSEL_PLANE_BOUNDS void fft_plane_r2c(FieldPtrs<T> fp, const V2<T> *
   plane_columns<T, NY, NZ, THREADS, -1, false, true>(P, tw.twy, gp);
 }
 
+// fft_plane_r2c of a field formed on the fly: out_f = a_f + s b_f c_f (c
+// NULL: out_f = a_f + s b_f), read pairwise where the row stage reads its
+// input -- the pointwise pass before a forward transform (f = u_p - phi u of
+// Make_f_particle_dt_nonsole, u + f dt/rho of the dielectric force) without
+// its own read and write of the fields.
+template <typename T, int NY, int NZ, int THREADS>
+__global__ KAPSEL_PLANE_BOUNDS void fft_plane_r2c_pro(FieldPtrs<T> out, FieldPtrs<T> a, FieldPtrs<T> b,
+                                                               FieldPtrs<T> c, T s, const V2<T> *__restrict__ twm,
+                                                               const V2<T> *__restrict__ zrot,
+                                                               const V2<T> *__restrict__ twy) {
+  using C = PlaneCfg<T, NY, NZ, THREADS>;
+  __shared__ __align__(16) V2<T> P[C::LPLANE];
+  __shared__ PlaneTw<T, NY, NZ> tw;
+  unsigned field;
+  const unsigned plane = xcd_tile(field);
+  const size_t base = (size_t)plane * 2 * C::PLANE;
+  T *__restrict__ gp = out.p[field] + base;
+  const T *ga = a.p[field] + base, *gb = b.p[field] + base;
+  const T *gc = c.p[field] ? c.p[field] + base : nullptr;
+  plane_tw_load<T, NY, NZ, THREADS>(tw, twm, zrot, twy);
+  __syncthreads();
+  KAPSEL_PLANE_ROWS(C) {
+    const size_t ro = (size_t)j * 2 * C::H;
+    V2<T> v[C::ZR1];
+#pragma unroll
+    for (int n1 = 0; n1 < C::ZR1; n1++) {
+      const size_t o = ro + 2 * (C::ZR2 * n1 + n2);
+      const V2<T> va = ld2(ga + o);
+      V2<T> vb = ld2(gb + o);
+      if (gc) {
+        const V2<T> vc = ld2(gc + o);
+        vb = V2<T>{vb.re * vc.re, vb.im * vc.im};
+      }
+      v[n1] = V2<T>{va.re + s * vb.re, va.im + s * vb.im};
+    }
+    row_r2c<T, C::M, C::ZR1, C::ZR2>(P + j * C::HP, v, tw.twm, tw.zrot, n2);
+  }
+  __syncthreads();
+  plane_columns<T, NY, NZ, THREADS, -1, false, true>(P, tw.twy, gp);
+}
+
 template <typename T, int NY, int NZ, int THREADS>
 __global__ KAPSEL_PLANE_BOUNDS void fft_plane_c2r(FieldPtrs<T> fp, const V2<T> *__restrict__ twm,
                                                            const V2<T> *__restrict__ zrot,
@@ -1128,6 +1169,42 @@ void fft_r2c_t(kapsel_ctx *ctx, void *const *fields, int nf) {
 
 // src: nullptr transforms fields in place; otherwise the spectra in src are
 // transformed into fields and src is left as it was
+// out = a + s b c (c may be nullptr) transformed forward, where a plane pass
+// applies; returns false (nothing queued) otherwise
+template <typename T>
+bool fft_r2c_pro_t(kapsel_ctx *ctx, void *const *out, void *const *a, void *const *b, void *const *c, double s,
+                   int nf) {
+  const Grid &g = ctx->g;
+  if (!plane_ok<T>(g)) return false;
+  const Twiddles &tw = ctx->tw[prec_of<T>()];
+  FieldPtrs<T> fo = pack<T>(out, nf), fa = pack<T>(a, nf), fb = pack<T>(b, nf);
+  FieldPtrs<T> fc = c ? pack<T>(c, nf) : FieldPtrs<T>{};
+  dim3 grid((unsigned)g.nx, nf);
+  {
+    LaunchTimer lt(ctx, KAPSEL_K_FFT_Z_R2C, (double)(c ? 4 : 3) * nf * g.f * sizeof(T));
+    switch (g.nz) {
+#define KAPSEL_PRO(N, TH)                                                                                        \
+  case N:                                                                                                        \
+    hipLaunchKernelGGL((fft_plane_r2c_pro<T, N, N, TH>), grid, dim3(TH), 0, ctx->stream, fo, fa, fb, fc, (T)s, \
+                       (const V2<T> *)tw.zh, (const V2<T> *)tw.zrot, (const V2<T> *)tw.y);                     \
+    break;
+      KAPSEL_PRO(32, 128)
+      KAPSEL_PRO(64, 320)
+      KAPSEL_PRO(128, 512)
+#undef KAPSEL_PRO
+      default: return false;
+    }
+    KAPSEL_LAUNCH_CHECK();
+  }
+  long long plane = (long long)g.ny * g.h;
+  lines_dispatch<T, -1>(ctx, KAPSEL_K_FFT_LINES_FWD, g.nx, fo, nf, plane, plane, 0, plane, tw.x);
+  return true;
+}
+template bool fft_r2c_pro_t<float>(kapsel_ctx *, void *const *, void *const *, void *const *, void *const *, double,
+                                   int);
+template bool fft_r2c_pro_t<double>(kapsel_ctx *, void *const *, void *const *, void *const *, void *const *,
+                                    double, int);
+
 template <typename T>
 void fft_c2r_t(kapsel_ctx *ctx, void *const *fields, int nf, void *const *src) {
   const Grid &g = ctx->g;
