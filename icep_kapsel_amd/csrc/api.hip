@@ -113,6 +113,8 @@ static void destroy(kapsel_ctx *ctx) {
     if (p) (void)hipFree(p);
   for (void *&p : ctx->ns_work)
     if (p) (void)hipFree(p);
+  for (void *&p : ctx->adv_x)
+    if (p) (void)hipFree(p);
   if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
   if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
   if (ctx->side) (void)hipStreamDestroy(ctx->side);
@@ -303,19 +305,39 @@ static void download(kapsel_ctx *ctx, double *host, const void *src) {
 static void *fld(kapsel_ctx *ctx, int f) { return ctx->fields[f]; }
 
 // Zeta_k2u (operate_omega.h:138-141), optionally after the 2/3 cut and FIX_CELL
-static void zeta_k2u(kapsel_ctx *ctx, int fix_dc) {
-  zeta_k2u_k(ctx, fix_dc);
-  fft_c2r(ctx, ctx->fields + KAPSEL_U0, 3);
+// (fix_dc as zeta_k2u_k); keep3: the velocity spectrum is kept there (the
+// inverse transform reads it from there)
+static void zeta_k2u(kapsel_ctx *ctx, int fix_dc, void *const *keep3 = nullptr) {
+  zeta_k2u_k(ctx, fix_dc, keep3);
+  fft_c2r(ctx, ctx->fields + KAPSEL_U0, 3, keep3);
 }
 
 // NS_solver_slavedEuler (fluid_solver.cxx:340-366) with Zeta_k2advection_k
-// (operate_omega.cxx:100-111) and U2advection_k (operate_omega.cxx:12-98)
-static void ns_slaved_euler(kapsel_ctx *ctx, const kapsel_time &t) {
+// (operate_omega.cxx:100-111) and U2advection_k (operate_omega.cxx:12-98).
+// Where a plane pass applies, the five products are formed inside it (into
+// adv_x and adv_w, U left as it is) instead of by the products kernel.
+// keep_adv false: the advection term (the reference's f_ns0) is not stored.
+template <typename T>
+static void ns_slaved_euler_t(kapsel_ctx *ctx, const kapsel_time &t, bool keep_adv) {
   zeta_k2u(ctx, 0);
+  void **u = ctx->fields + KAPSEL_U0;
+  if (plane_ok_grid(ctx->g)) {
+    for (void *&p : ctx->adv_x)
+      if (!p) KAPSEL_HIP(hipMalloc(&p, ctx->g.f * sizeof(T)));
+    void *prod[5] = {ctx->adv_x[0], ctx->adv_x[1], ctx->adv_x[2], ctx->adv_w[0], ctx->adv_w[1]};
+    void *b[5] = {u[1], u[2], u[0], u[1], u[2]}, *c[5] = {u[2], u[0], u[1], u[2], u[0]};
+    (void)fft_r2c_pro_t<T>(ctx, prod, nullptr, b, c, 1.0, 5, 0x18u);
+    advection_combine_slaved(ctx, t.dt_fluid, prod, keep_adv);
+    return;
+  }
   advection_products(ctx);
-  void *five[5] = {fld(ctx, KAPSEL_U0), fld(ctx, KAPSEL_U1), fld(ctx, KAPSEL_U2), ctx->adv_w[0], ctx->adv_w[1]};
+  void *five[5] = {u[0], u[1], u[2], ctx->adv_w[0], ctx->adv_w[1]};
   fft_r2c(ctx, five, 5);
-  advection_combine_slaved(ctx, t.dt_fluid);
+  advection_combine_slaved(ctx, t.dt_fluid, nullptr, keep_adv);
+}
+static void ns_slaved_euler(kapsel_ctx *ctx, const kapsel_time &t, bool keep_adv = true) {
+  if (ctx->prec == KAPSEL_FP64) ns_slaved_euler_t<double>(ctx, t, keep_adv);
+  else ns_slaved_euler_t<float>(ctx, t, keep_adv);
 }
 
 // Solenoidal_u (operate_omega.h:109-116)
@@ -342,12 +364,12 @@ static void u2zeta_k(kapsel_ctx *ctx) {
 // The end of an ICEP step in an FP32 context, from Solenoidal_u(u) on.
 // u = c2r(P u_k) and the particle force f = P(u_p - phi u) (f_particle.h)
 // enter U2zeta_k only as their sum, whose spectrum is P u_k + P f_k: the
-// first is kept when u is projected (keep3, three spare fields: the
-// projection writes there and the first inverse pass reads from there), the
-// second is never transformed back.  U holds that spectrum afterwards, F the
-// spectrum P f_k; the FP64 context keeps the reference's real-space order.
-// add_s > 0: u += f add_s first (the dielectric force, sp_3d_ns.cxx:77-84),
-// formed inside the forward transform where a plane pass applies
+// first is kept when u is projected (keep3, three spare fields), the second
+// is never transformed back.  U holds that spectrum afterwards and F the
+// unprojected spectrum of f; the FP64 context keeps the reference's
+// real-space order.  add_s > 0: u += f add_s first (the dielectric force,
+// sp_3d_ns.cxx:77-84), formed inside the forward transform where a plane
+// pass applies.
 static void solenoidal_u_keep(kapsel_ctx *ctx, void *const *keep3, double add_s = 0.0) {
   void **u = ctx->fields + KAPSEL_U0;
   if (!(add_s != 0.0 && fft_r2c_pro_t<float>(ctx, u, u, ctx->fields + KAPSEL_F0, nullptr, add_s, 3))) {
@@ -358,16 +380,16 @@ static void solenoidal_u_keep(kapsel_ctx *ctx, void *const *keep3, double add_s 
   fft_c2r(ctx, u, 3, keep3);
 }
 // f = u_p - phi u (Make_f_particle_dt_nonsole) formed inside the forward
-// transform of f where a plane pass applies
+// transform of f where a plane pass applies; the projection of f, the sum
+// and U_k2zeta_k in one pointwise pass
 static void f_sole_u2zeta_k(kapsel_ctx *ctx, void *const *keep3) {
   void *phi3[3] = {fld(ctx, KAPSEL_PHI), fld(ctx, KAPSEL_PHI), fld(ctx, KAPSEL_PHI)};
-  if (!fft_r2c_pro_t<float>(ctx, ctx->fields + KAPSEL_F0, ctx->fields + KAPSEL_UP0, phi3, ctx->fields + KAPSEL_U0,
-                            -1.0, 3)) {
+  void **f = ctx->fields + KAPSEL_F0;
+  if (!fft_r2c_pro_t<float>(ctx, f, ctx->fields + KAPSEL_UP0, phi3, ctx->fields + KAPSEL_U0, -1.0, 3)) {
     make_f_nonsole(ctx);
-    fft_r2c(ctx, ctx->fields + KAPSEL_F0, 3);
+    fft_r2c(ctx, f, 3);
   }
-  solenoidal_uk(ctx, ctx->fields + KAPSEL_F0);
-  u_k2zeta_sum_k(ctx, keep3, ctx->fields + KAPSEL_F0);
+  sole_u2zeta_sum(ctx, keep3, f);
 }
 
 static int fix_mask(const kapsel_ctx *ctx) {
@@ -381,15 +403,10 @@ static int fix_mask(const kapsel_ctx *ctx) {
 // (FP32 context: the velocity spectrum of Zeta_k2u is kept in WORK0..WORK2
 // and the step ends in k-space, as f_sole_u2zeta_k describes)
 static void spm_step(kapsel_ctx *ctx, kapsel_time &t) {
-  ns_slaved_euler(ctx, t);                    // :27
+  ns_slaved_euler(ctx, t, false);             // :27
   void *keep3[3] = {fld(ctx, KAPSEL_WORK0), fld(ctx, KAPSEL_WORK1), fld(ctx, KAPSEL_WORK2)};
   const bool fp32 = ctx->prec == KAPSEL_FP32;
-  if (fp32) {                                 // :29-42
-    zeta_k2u_k(ctx, fix_mask(ctx), keep3);
-    fft_c2r(ctx, ctx->fields + KAPSEL_U0, 3, keep3);
-  } else {
-    zeta_k2u(ctx, fix_mask(ctx));
-  }
+  zeta_k2u(ctx, fix_mask(ctx), fp32 ? keep3 : nullptr);  // :29-42
   md_position(ctx, t);                        // :44-50
   make_phi_particle_sum(ctx);                 // :51-53
   calc_f_hydro(ctx, t);                       // :99-101
@@ -470,13 +487,13 @@ static void icep_step(kapsel_ctx *ctx, kapsel_time &t, const double e0[3], doubl
     KAPSEL_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
     {
       SideStream on_side(ctx);
-      ns_slaved_euler(ctx, t);
+      ns_slaved_euler(ctx, t, false);
       zeta_k2u(ctx, fix_mask(ctx));
     }
     KAPSEL_HIP(hipEventRecord(ctx->ev_join, ctx->side));
     join.armed = true;
   } else {
-    ns_slaved_euler(ctx, t);
+    ns_slaved_euler(ctx, t, false);
     zeta_k2u(ctx, fix_mask(ctx));
   }
   md_position(ctx, t);

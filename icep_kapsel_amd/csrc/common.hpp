@@ -175,6 +175,8 @@ struct kapsel_ctx {
   bool concurrent = true;         // kapsel_set_concurrency
   void *adv_w[2] = {};            // the advection's two extra product fields (WORK0, WORK1 by default)
   void *ns_work[2] = {};          // their stand-ins while the NS chain runs beside the solve
+  void *adv_x[3] = {};            // the three other product fields where the products are formed
+                                  // inside the forward transform (allocated on first use)
   bool lazy_outputs = false;      // an FP32 solve leaves POTENTIAL/RHS to materialize_outputs
   bool outputs_stale = false;     // POTENTIAL/RHS wait for materialize_outputs
   int device = 0;
@@ -318,12 +320,17 @@ template <typename T>
 void fft_r2c_t(kapsel_ctx *ctx, void *const *fields, int nf);
 template <typename T>
 void fft_c2r_t(kapsel_ctx *ctx, void *const *fields, int nf, void *const *src = nullptr);
+// out_f = a_f + s b_f c_f transformed forward where a plane pass applies (a
+// or c NULL: left out; bit f of dsq: (b_f - c_f)(b_f + c_f) instead of the
+// product); false: nothing queued
+template <typename T>
+bool fft_r2c_pro_t(kapsel_ctx *ctx, void *const *out, void *const *a, void *const *b, void *const *c, double s,
+                   int nf, unsigned dsq = 0);
+// whether the z and y transforms of the context's grid run as one plane pass
+inline bool plane_ok_grid(const Grid &g) { return g.ny == g.nz && (g.nz == 32 || g.nz == 64 || g.nz == 128); }
 // g_d <- r2c(face_d * c2r(g_d)) for the three staggered-gradient spectra
 // (face_d real fields, 0 in the row padding): the inverse and forward z (and,
 // where a plane fits in LDS, y) transforms and the product fused in one pass
-template <typename T>
-bool fft_r2c_pro_t(kapsel_ctx *ctx, void *const *out, void *const *a, void *const *b, void *const *c, double s,
-                   int nf);
 template <typename T>
 void fft_face_t(kapsel_ctx *ctx, void *const *g3, void *const *face3);
 // fft_face_t without its first (x inverse) and last (x forward) passes
@@ -348,10 +355,13 @@ void truncate(kapsel_ctx *ctx, void *const *fields, int nf);
 // out3: the velocity spectrum goes there instead of U0..U2
 void zeta_k2u_k(kapsel_ctx *ctx, int fix_dc, void *const *out3 = nullptr);
 void u_k2zeta_k(kapsel_ctx *ctx);
-// U_k2zeta_k of the spectrum a3 + b3, which is also stored in U0..U2
-void u_k2zeta_sum_k(kapsel_ctx *ctx, void *const *a3, void *const *b3);
+// the 2/3 cut and Solenoidal_uk of the spectrum f3, then U_k2zeta_k of
+// a3 + P f3, which is stored in U0..U2 (a3 and f3 unchanged)
+void sole_u2zeta_sum(kapsel_ctx *ctx, void *const *a3, void *const *f3);
 void advection_products(kapsel_ctx *ctx);
-void advection_combine_slaved(kapsel_ctx *ctx, double dt_fluid);
+// in5: the five product spectra (default U0..U2 and adv_w); the advection
+// term is stored over the last two unless store_a is 0
+void advection_combine_slaved(kapsel_ctx *ctx, double dt_fluid, void *const *in5 = nullptr, int store_a = 1);
 // out3: the projected spectrum goes there and f3 is left unchanged
 void solenoidal_uk(kapsel_ctx *ctx, void *const *f3, void *const *out3 = nullptr);
 void make_f_nonsole(kapsel_ctx *ctx);

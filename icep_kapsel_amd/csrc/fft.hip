@@ -18,163 +18,41 @@
 // lines are adjacent in memory (lines of a tile differ in the fastest
 // index), so each load/store instruction of a wavefront touches few 64-B
 // segments.  Up to five fields are transformed by one launch (grid.y).
-#include <utility>
-
-#include "common.hpp"
+#include "fft_prims.hpp"
 
 namespace kapsel {
-
-// ---------------------------------------------------------------- constants
-// cos(2 pi m / 16)
-__host__ __device__ constexpr double cos16(int m) {
-  m &= 15;
-  return m == 0    ? 1.0
-         : m == 1  ? 0.92387953251128675613
-         : m == 2  ? 0.70710678118654752440
-         : m == 3  ? 0.38268343236508977173
-         : m == 4  ? 0.0
-         : m == 5  ? -0.38268343236508977173
-         : m == 6  ? -0.70710678118654752440
-         : m == 7  ? -0.92387953251128675613
-         : m == 8  ? -1.0
-         : m == 9  ? -0.92387953251128675613
-         : m == 10 ? -0.70710678118654752440
-         : m == 11 ? -0.38268343236508977173
-         : m == 12 ? 0.0
-         : m == 13 ? 0.38268343236508977173
-         : m == 14 ? 0.70710678118654752440
-                   : 0.92387953251128675613;
-}
-__host__ __device__ constexpr double sin16(int m) { return cos16(m - 4); }
-
-// b * W_16^M with W_16 = exp(DIR 2 pi i / 16); exact for the axis multiples
-template <int DIR, int M, typename T>
-__device__ __forceinline__ V2<T> mulw16(V2<T> b) {
-  constexpr int m = M & 15;
-  if constexpr (m == 0) {
-    return b;
-  } else if constexpr (m == 4) {
-    if constexpr (DIR < 0) return V2<T>{b.im, -b.re};
-    else return V2<T>{-b.im, b.re};
-  } else if constexpr (m == 8) {
-    return V2<T>{-b.re, -b.im};
-  } else if constexpr (m == 12) {
-    if constexpr (DIR < 0) return V2<T>{-b.im, b.re};
-    else return V2<T>{b.im, -b.re};
-  } else {
-    constexpr T c = (T)cos16(m);
-    constexpr T s = (T)(DIR * sin16(m));
-    return V2<T>{b.re * c - b.im * s, b.re * s + b.im * c};
-  }
-}
-
-template <int R, int DIR, typename T, int... K>
-__device__ __forceinline__ void dit_combine(const V2<T> *e, const V2<T> *o, V2<T> *out,
-                                            std::integer_sequence<int, K...>) {
-  ((out[K] = e[K] + mulw16<DIR, K *(16 / R)>(o[K]),
-    out[K + R / 2] = e[K] - mulw16<DIR, K *(16 / R)>(o[K])),
-   ...);
-}
-
-// out[k] = sum_n in[S n] W_R^{nk}, k < R, radix-2 decimation in time in
-// registers (all indices compile-time)
-template <int R, int S, int DIR, typename T>
-__device__ __forceinline__ void dft(const V2<T> *in, V2<T> *out) {
-  if constexpr (R == 1) {
-    out[0] = in[0];
-  } else {
-    V2<T> e[R / 2], o[R / 2];
-    dft<R / 2, 2 * S, DIR>(in, e);
-    dft<R / 2, 2 * S, DIR>(in + S, o);
-    dit_combine<R, DIR>(e, o, out, std::make_integer_sequence<int, R / 2>{});
-  }
-}
-
-// factorisation L = R1 * R2 (R1 >= R2)
-template <int L>
-struct FS;
-template <> struct FS<8> { static constexpr int R1 = 4, R2 = 2; };
-template <> struct FS<16> { static constexpr int R1 = 4, R2 = 4; };
-template <> struct FS<32> { static constexpr int R1 = 8, R2 = 4; };
-template <> struct FS<64> { static constexpr int R1 = 8, R2 = 8; };
-template <> struct FS<128> { static constexpr int R1 = 16, R2 = 8; };
-template <> struct FS<256> { static constexpr int R1 = 16, R2 = 16; };
-
-template <typename T>
-struct FieldPtrs {
-  T *p[5];
-};
-
-// XCD-aware tile order.  Workgroups are dispatched round-robin over the 8
-// XCDs by linear id (blockIdx.y * gridDim.x + blockIdx.x), and each XCD has
-// its own L2; remapping gives each XCD one contiguous range of (field, tile)
-// ids, so the cache lines that two neighbouring tiles share are fetched once,
-// by one L2.  Returns the remapped linear id; field = id / gridDim.x.
-__device__ __forceinline__ unsigned xcd_tile(unsigned &field) {
-  constexpr unsigned NXCD = 8;
-  const unsigned total = gridDim.x * gridDim.y;
-  const unsigned lin = blockIdx.y * gridDim.x + blockIdx.x;
-  const unsigned q = total / NXCD, r = total % NXCD;
-  const unsigned x = lin % NXCD, k = lin / NXCD;
-  const unsigned id = x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
-  field = id / gridDim.x;
-  return id % gridDim.x;
-}
-
-// twiddle W_L^m, table holds the forward sign
-template <int DIR, typename T>
-__device__ __forceinline__ V2<T> tw_get(const V2<T> *__restrict__ tw, int m) {
-  V2<T> w = tw[m];
-  if constexpr (DIR > 0) w.im = -w.im;
-  return w;
-}
 
 // ------------------------------------------------------------ c2c line pass
 // Line q (0 <= q < Q) starts at complex offset (q / A) * B + (q % A); its
 // elements are ES complex values apart.  Lines q of one tile are adjacent.
-// out: where the transformed lines go (out = fp transforms in place; each
-// line is read whole before the barrier and written after it)
+// Thread t of a tile works on line t % LPB as its thread s = t / LPB (so s
+// and s ^ 1 are LPB lanes apart).  out: where the transformed lines go (out
+// = fp transforms in place; each line is read whole before the barrier and
+// written after it)
 template <typename T, int L, int DIR, int LPB>
-__global__ __launch_bounds__(LPB *FS<L>::R2) void fft_c2c_lines_io(FieldPtrs<T> fp, FieldPtrs<T> out, long long Q,
-                                                                    long long A, long long B, long long ES,
-                                                                    const V2<T> *__restrict__ tw) {
-  constexpr int R1 = FS<L>::R1, R2 = FS<L>::R2, TPL = R2;
-  constexpr int LS = R2 * (R1 + 1) + 1;  // LDS line stride (odd)
-  __shared__ V2<T> sm[LPB * LS];
+__global__ __launch_bounds__((LPB * LineFFT<T, L>::S)) void fft_c2c_lines_io(FieldPtrs<T> fp, FieldPtrs<T> out,
+                                                                           long long Q, long long A, long long B,
+                                                                           long long ES, const V2<T> *__restrict__ tw) {
+  using F = LineFFT<T, L>;
+  __shared__ V2<T> sm[LPB * F::LS];
   unsigned field;
   const unsigned tile = xcd_tile(field);
   const T *data = fp.p[field];
   T *dst = out.p[field];
   const int t = threadIdx.x;
   const int l = t % LPB;
-  const int n2 = t / LPB;
+  const int s = t / LPB;
   const long long q = (long long)tile * LPB + l;
   const bool valid = q < Q;
   const long long base = valid ? (q / A) * B + (q % A) : 0;
-
-  V2<T> v[R1], o[R1];
+  V2<T> v[F::P], twr[F::P];
 #pragma unroll
-  for (int n1 = 0; n1 < R1; n1++) {
-    v[n1] = valid ? ld2(data + 2 * (base + (long long)(R2 * n1 + n2) * ES)) : V2<T>{0, 0};
-  }
-  dft<R1, 1, DIR>(v, o);
+  for (int n = 0; n < F::P; n++) v[n] = ld2(data + 2 * (base + (long long)F::in_i(s, n) * ES));
+  line_twiddles<F>(twr, tw, s);
+  F::template run<DIR>(v, sm + l * F::LS, twr, s, LPB);
+  if (valid) {
 #pragma unroll
-  for (int k1 = 1; k1 < R1; k1++) o[k1] = cmul(o[k1], tw_get<DIR>(tw, n2 * k1));
-  V2<T> *sl = sm + l * LS;
-#pragma unroll
-  for (int k1 = 0; k1 < R1; k1++) sl[n2 * (R1 + 1) + k1] = o[k1];
-  __syncthreads();
-#pragma unroll
-  for (int c = 0; c < R1 / TPL; c++) {
-    const int k1 = n2 + TPL * c;
-    V2<T> w[R2], x[R2];
-#pragma unroll
-    for (int m = 0; m < R2; m++) w[m] = sl[m * (R1 + 1) + k1];
-    dft<R2, 1, DIR>(w, x);
-    if (valid) {
-#pragma unroll
-      for (int k2 = 0; k2 < R2; k2++) st2(dst + 2 * (base + (long long)(k1 + R1 * k2) * ES), x[k2]);
-    }
+    for (int m = 0; m < F::P; m++) st2(dst + 2 * (base + (long long)F::out_i(s, m) * ES), v[m]);
   }
 }
 
@@ -548,14 +426,17 @@ __global__ KAPSEL_PLANE_BOUNDS void fft_plane_r2c(FieldPtrs<T> fp, const V2<T> *
   plane_columns<T, NY, NZ, THREADS, -1, false, true>(P, tw.twy, gp);
 }
 
-// fft_plane_r2c of a field formed on the fly: out_f = a_f + s b_f c_f (c
-// NULL: out_f = a_f + s b_f), read pairwise where the row stage reads its
-// input -- the pointwise pass before a forward transform (f = u_p - phi u of
-// Make_f_particle_dt_nonsole, u + f dt/rho of the dielectric force) without
-// its own read and write of the fields.
+// fft_plane_r2c of a field formed on the fly: out_f = a_f + s b_f c_f (a_f
+// NULL: s b_f c_f; c_f NULL: a_f + s b_f; bit f of dsq: (b_f - c_f)(b_f +
+// c_f)), read pairwise where the row stage reads its input -- the pointwise
+// pass before a forward transform (f = u_p - phi u of
+// Make_f_particle_dt_nonsole, u + f dt/rho of the dielectric force, the
+// products of U2advection_k) without its own read and write of the fields.
+// out_f must not be an input of another field of the launch.
 template <typename T, int NY, int NZ, int THREADS>
 __global__ KAPSEL_PLANE_BOUNDS void fft_plane_r2c_pro(FieldPtrs<T> out, FieldPtrs<T> a, FieldPtrs<T> b,
-                                                               FieldPtrs<T> c, T s, const V2<T> *__restrict__ twm,
+                                                               FieldPtrs<T> c, T s, unsigned dsq,
+                                                               const V2<T> *__restrict__ twm,
                                                                const V2<T> *__restrict__ zrot,
                                                                const V2<T> *__restrict__ twy) {
   using C = PlaneCfg<T, NY, NZ, THREADS>;
@@ -565,8 +446,9 @@ __global__ KAPSEL_PLANE_BOUNDS void fft_plane_r2c_pro(FieldPtrs<T> out, FieldPtr
   const unsigned plane = xcd_tile(field);
   const size_t base = (size_t)plane * 2 * C::PLANE;
   T *__restrict__ gp = out.p[field] + base;
-  const T *ga = a.p[field] + base, *gb = b.p[field] + base;
+  const T *ga = a.p[field] ? a.p[field] + base : nullptr, *gb = b.p[field] + base;
   const T *gc = c.p[field] ? c.p[field] + base : nullptr;
+  const bool diff = (dsq >> field) & 1;
   plane_tw_load<T, NY, NZ, THREADS>(tw, twm, zrot, twy);
   __syncthreads();
   KAPSEL_PLANE_ROWS(C) {
@@ -575,13 +457,18 @@ __global__ KAPSEL_PLANE_BOUNDS void fft_plane_r2c_pro(FieldPtrs<T> out, FieldPtr
 #pragma unroll
     for (int n1 = 0; n1 < C::ZR1; n1++) {
       const size_t o = ro + 2 * (C::ZR2 * n1 + n2);
-      const V2<T> va = ld2(ga + o);
       V2<T> vb = ld2(gb + o);
       if (gc) {
         const V2<T> vc = ld2(gc + o);
-        vb = V2<T>{vb.re * vc.re, vb.im * vc.im};
+        vb = diff ? V2<T>{(vb.re - vc.re) * (vb.re + vc.re), (vb.im - vc.im) * (vb.im + vc.im)}
+                  : V2<T>{vb.re * vc.re, vb.im * vc.im};
       }
-      v[n1] = V2<T>{va.re + s * vb.re, va.im + s * vb.im};
+      if (ga) {
+        const V2<T> va = ld2(ga + o);
+        v[n1] = V2<T>{va.re + s * vb.re, va.im + s * vb.im};
+      } else {
+        v[n1] = V2<T>{s * vb.re, s * vb.im};
+      }
     }
     row_r2c<T, C::M, C::ZR1, C::ZR2>(P + j * C::HP, v, tw.twm, tw.zrot, n2);
   }
@@ -763,8 +650,6 @@ __global__ __launch_bounds__(RPB *FS<NZ / 2>::R2) void fft_z_face(FieldPtrs<T> f
 // symbols; the last pass (x, forward) also forms the staggered divergence and
 // <p, A p>.  x line q = j*h + kz: elements q + i*plane, i = 0..NX-1.
 
-__device__ __forceinline__ int signed_k(int i, int n) { return i > n / 2 ? i - n : i; }
-
 template <int THREADS>
 __device__ __forceinline__ double block_sum_fft(double v) {
   __shared__ double red[THREADS / 64];
@@ -779,76 +664,57 @@ __device__ __forceinline__ double block_sum_fft(double v) {
 }
 
 // p = z + sc[beta] p (stored), then g_d = IFFT_x(i k_d e^{i k_d dx/2} p), d = x, y, z
+// (thread t of a tile: line t % LPB, its thread s = t / LPB)
 template <typename T, int L, int LPB>
-__global__ __launch_bounds__(LPB *FS<L>::R2) void fft_x_grad(const T *z, T *p, const double *__restrict__ sc,
-                                                             int beta_slot, FieldPtrs<T> g3, long long Q,
-                                                             long long plane, int h, int ny, double wx, double wy,
-                                                             double wz, const V2<T> *__restrict__ shx,
-                                                             const V2<T> *__restrict__ shy,
-                                                             const V2<T> *__restrict__ shz,
-                                                             const V2<T> *__restrict__ tw) {
-  constexpr int R1 = FS<L>::R1, R2 = FS<L>::R2;
-  constexpr int LS = R2 * (R1 + 1) + 1;
-  __shared__ V2<T> sm[LPB * LS];
+__global__ __launch_bounds__((LPB * LineFFT<T, L>::S)) void fft_x_grad(
+    const T *z, T *p, const double *__restrict__ sc, int beta_slot, FieldPtrs<T> g3, long long Q, long long plane,
+    int h, int ny, double wx, double wy, double wz, const V2<T> *__restrict__ shx, const V2<T> *__restrict__ shy,
+    const V2<T> *__restrict__ shz, const V2<T> *__restrict__ tw) {
+  using F = LineFFT<T, L>;
+  constexpr int P = F::P;
+  __shared__ V2<T> sm[LPB * F::LS];
   unsigned field;
   const unsigned tile = xcd_tile(field);
   const int t = threadIdx.x;
-  const int l = t % LPB, n2 = t / LPB;
+  const int l = t % LPB, s = t / LPB;
   const long long q = (long long)tile * LPB + l;
   const bool valid = q < Q;
   const long long base = valid ? q : 0;
   const int j = (int)(base / h), kz = (int)(base % h);
   const bool update = beta_slot >= 0;  // else the gradient of z itself, p untouched
   const T beta = update ? (T)sc[beta_slot] : (T)0;
-  V2<T> pv[R1];
+  V2<T> pv[P];
 #pragma unroll
-  for (int n1 = 0; n1 < R1; n1++) {
-    const long long off = base + (long long)(R2 * n1 + n2) * plane;
+  for (int n = 0; n < P; n++) {
+    const long long off = base + (long long)F::in_i(s, n) * plane;
     if (valid) {
-      pv[n1] = ld2(z + 2 * off);
+      pv[n] = ld2(z + 2 * off);
       if (update) {
-        pv[n1] = pv[n1] + scal(beta, ld2(p + 2 * off));
-        st2(p + 2 * off, pv[n1]);
+        pv[n] = pv[n] + scal(beta, ld2(p + 2 * off));
+        st2(p + 2 * off, pv[n]);
       }
     } else {
-      pv[n1] = V2<T>{0, 0};
+      pv[n] = V2<T>{0, 0};
     }
   }
-  V2<T> *sl = sm + l * LS;
-  // the twiddles of this thread's first stage, loaded once for the three
-  // directions (the stores in between could alias the table otherwise)
-  V2<T> twr[R1];
-#pragma unroll
-  for (int k1 = 1; k1 < R1; k1++) twr[k1] = tw_get<+1>(tw, n2 * k1);
+  V2<T> *sl = sm + l * F::LS;
+  V2<T> twr[P];
+  line_twiddles<F>(twr, tw, s);
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    V2<T> v[R1], o[R1];
+    V2<T> v[P];
 #pragma unroll
-    for (int n1 = 0; n1 < R1; n1++) {
-      const int i = R2 * n1 + n2;
+    for (int n = 0; n < P; n++) {
+      const int i = F::in_i(s, n);
       const T w = d == 0 ? (T)(signed_k(i, L) * wx) : d == 1 ? (T)(signed_k(j, ny) * wy) : (T)(kz * wz);
       const V2<T> sft = d == 0 ? shx[i] : d == 1 ? shy[j] : shz[kz];
-      v[n1] = cmul(V2<T>{-w * pv[n1].im, w * pv[n1].re}, sft);
+      v[n] = cmul(V2<T>{-w * pv[n].im, w * pv[n].re}, sft);
     }
-    dft<R1, 1, +1>(v, o);
+    F::template run<+1>(v, sl, twr, s, LPB);
+    if (valid) {
+      T *__restrict__ gd = g3.p[d];
 #pragma unroll
-    for (int k1 = 1; k1 < R1; k1++) o[k1] = cmul(o[k1], twr[k1]);
-    if (d > 0) __syncthreads();  // the previous direction's LDS reads are done
-#pragma unroll
-    for (int k1 = 0; k1 < R1; k1++) sl[n2 * (R1 + 1) + k1] = o[k1];
-    __syncthreads();
-    T *__restrict__ gd = g3.p[d];
-#pragma unroll
-    for (int c = 0; c < R1 / R2; c++) {
-      const int k1 = n2 + R2 * c;
-      V2<T> w[R2], x[R2];
-#pragma unroll
-      for (int m = 0; m < R2; m++) w[m] = sl[m * (R1 + 1) + k1];
-      dft<R2, 1, +1>(w, x);
-      if (valid) {
-#pragma unroll
-        for (int k2 = 0; k2 < R2; k2++) st2(gd + 2 * (base + (long long)(k1 + R1 * k2) * plane), x[k2]);
-      }
+      for (int m = 0; m < P; m++) st2(gd + 2 * (base + (long long)F::out_i(s, m) * plane), v[m]);
     }
   }
 }
@@ -858,98 +724,75 @@ __global__ __launch_bounds__(LPB *FS<L>::R2) void fft_x_grad(const T *z, T *p, c
 // kz = 0 and NZ/2 planes and 2 elsewhere (Parseval, without N^-3)
 // RES: the residual form -- p is b, ap receives r = (div) - b, and the
 // partials sum |r|^2 (the refinement's A x - b)
+// x_div's line transforms: the pair split at 128 points in FP64 only (in
+// FP32 the four-step 16 x 8 measured faster for this kernel)
+template <typename T, int L>
+using DivFFT = LineFFT<T, L, L == 128 && sizeof(T) == 8>;
 template <typename T, int L, int LPB, bool RES = false>
-__global__ __launch_bounds__(LPB *FS<L>::R2) void fft_x_div(FieldPtrs<T> g3, const T *__restrict__ p,
-                                                            T *__restrict__ ap, double *__restrict__ partial,
-                                                            long long Q, long long plane, int h, int ny, int nz,
-                                                            double wx, double wy, double wz,
-                                                            const V2<T> *__restrict__ shx,
-                                                            const V2<T> *__restrict__ shy,
-                                                            const V2<T> *__restrict__ shz,
-                                                            const V2<T> *__restrict__ tw) {
-  constexpr int R1 = FS<L>::R1, R2 = FS<L>::R2;
-  constexpr int LS = R2 * (R1 + 1) + 1;
-  __shared__ V2<T> sm[LPB * LS];
+__global__ __launch_bounds__((LPB * DivFFT<T, L>::S)) void fft_x_div(
+    FieldPtrs<T> g3, const T *__restrict__ p, T *__restrict__ ap, double *__restrict__ partial, long long Q,
+    long long plane, int h, int ny, int nz, double wx, double wy, double wz, const V2<T> *__restrict__ shx,
+    const V2<T> *__restrict__ shy, const V2<T> *__restrict__ shz, const V2<T> *__restrict__ tw) {
+  using F = DivFFT<T, L>;
+  constexpr int P = F::P;
+  __shared__ V2<T> sm[LPB * F::LS];
   unsigned field;
   const unsigned tile = xcd_tile(field);
   const int t = threadIdx.x;
-  const int l = t % LPB, n2 = t / LPB;
+  const int l = t % LPB, s = t / LPB;
   const long long q = (long long)tile * LPB + l;
   const bool valid = q < Q;
   const long long base = valid ? q : 0;
   const int j = (int)(base / h), kz = (int)(base % h);
-  V2<T> acc[R1 / R2][R2];
+  V2<T> acc[P];
 #pragma unroll
-  for (int c = 0; c < R1 / R2; c++)
-#pragma unroll
-    for (int k2 = 0; k2 < R2; k2++) acc[c][k2] = V2<T>{0, 0};
-  V2<T> *sl = sm + l * LS;
+  for (int m = 0; m < P; m++) acc[m] = V2<T>{0, 0};
+  V2<T> *sl = sm + l * F::LS;
   // direction d + 1's loads are in flight while direction d is transformed
   // (base is 0 for an invalid line, so every load is in bounds)
-  V2<T> nxt[R1];
+  V2<T> nxt[P];
 #pragma unroll
-  for (int n1 = 0; n1 < R1; n1++) nxt[n1] = ld2(g3.p[0] + 2 * (base + (long long)(R2 * n1 + n2) * plane));
-  // the first stage's twiddles, loaded once for the three directions
-  V2<T> twr[R1];
-#pragma unroll
-  for (int k1 = 1; k1 < R1; k1++) twr[k1] = tw_get<-1>(tw, n2 * k1);
+  for (int n = 0; n < P; n++) nxt[n] = ld2(g3.p[0] + 2 * (base + (long long)F::in_i(s, n) * plane));
+  V2<T> twr[P];
+  line_twiddles<F>(twr, tw, s);
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    V2<T> v[R1], o[R1];
+    V2<T> v[P];
 #pragma unroll
-    for (int n1 = 0; n1 < R1; n1++) v[n1] = valid ? nxt[n1] : V2<T>{0, 0};
+    for (int n = 0; n < P; n++) v[n] = valid ? nxt[n] : V2<T>{0, 0};
     if (d < 2) {
 #pragma unroll
-      for (int n1 = 0; n1 < R1; n1++)
-        nxt[n1] = ld2(g3.p[d + 1] + 2 * (base + (long long)(R2 * n1 + n2) * plane));
+      for (int n = 0; n < P; n++) nxt[n] = ld2(g3.p[d + 1] + 2 * (base + (long long)F::in_i(s, n) * plane));
     }
-    dft<R1, 1, -1>(v, o);
+    F::template run<-1>(v, sl, twr, s, LPB);
 #pragma unroll
-    for (int k1 = 1; k1 < R1; k1++) o[k1] = cmul(o[k1], twr[k1]);
-    if (d > 0) __syncthreads();
-#pragma unroll
-    for (int k1 = 0; k1 < R1; k1++) sl[n2 * (R1 + 1) + k1] = o[k1];
-    __syncthreads();
-#pragma unroll
-    for (int c = 0; c < R1 / R2; c++) {
-      const int k1 = n2 + R2 * c;
-      V2<T> w[R2], x[R2];
-#pragma unroll
-      for (int m = 0; m < R2; m++) w[m] = sl[m * (R1 + 1) + k1];
-      dft<R2, 1, -1>(w, x);
-#pragma unroll
-      for (int k2 = 0; k2 < R2; k2++) {
-        const int i = k1 + R1 * k2;
-        const T qd = d == 0 ? (T)(signed_k(i, L) * wx) : d == 1 ? (T)(signed_k(j, ny) * wy) : (T)(kz * wz);
-        const V2<T> sft = d == 0 ? shx[i] : d == 1 ? shy[j] : shz[kz];
-        acc[c][k2] = acc[c][k2] + cmul(x[k2], scal(qd, conjv(sft)));
-      }
+    for (int m = 0; m < P; m++) {
+      const int i = F::out_i(s, m);
+      const T qd = d == 0 ? (T)(signed_k(i, L) * wx) : d == 1 ? (T)(signed_k(j, ny) * wy) : (T)(kz * wz);
+      const V2<T> sft = d == 0 ? shx[i] : d == 1 ? shy[j] : shz[kz];
+      acc[m] = acc[m] + cmul(v[m], scal(qd, conjv(sft)));
     }
   }
   const double wk = (kz == 0 || 2 * kz == nz) ? 1.0 : 2.0;
-  double s = 0.0;
+  double sum = 0.0;
   if (valid) {
 #pragma unroll
-    for (int c = 0; c < R1 / R2; c++) {
-      const int k1 = n2 + R2 * c;
-#pragma unroll
-      for (int k2 = 0; k2 < R2; k2++) {
-        const long long off = base + (long long)(k1 + R1 * k2) * plane;
-        const V2<T> av{-acc[c][k2].im, acc[c][k2].re};
-        const V2<T> pp = ld2(p + 2 * off);
-        if constexpr (RES) {
-          const V2<T> rv = av - pp;
-          st2(ap + 2 * off, rv);
-          s += wk * ((double)rv.re * rv.re + (double)rv.im * rv.im);
-        } else {
-          st2(ap + 2 * off, av);
-          s += wk * ((double)pp.re * av.re + (double)pp.im * av.im);
-        }
+    for (int m = 0; m < P; m++) {
+      const long long off = base + (long long)F::out_i(s, m) * plane;
+      const V2<T> av{-acc[m].im, acc[m].re};
+      const V2<T> pp = ld2(p + 2 * off);
+      if constexpr (RES) {
+        const V2<T> rv = av - pp;
+        st2(ap + 2 * off, rv);
+        sum += wk * ((double)rv.re * rv.re + (double)rv.im * rv.im);
+      } else {
+        st2(ap + 2 * off, av);
+        sum += wk * ((double)pp.re * av.re + (double)pp.im * av.im);
       }
     }
   }
-  s = block_sum_fft<LPB * FS<L>::R2>(s);
-  if (threadIdx.x == 0) partial[tile] = s;
+  sum = block_sum_fft<LPB * F::S>(sum);
+  if (threadIdx.x == 0) partial[tile] = sum;
 }
 
 // ------------------------------------------------------------------- host
@@ -1005,8 +848,8 @@ void launch_lines(kapsel_ctx *ctx, int cls, const FieldPtrs<T> &fp, const FieldP
   constexpr int LPB = lines_per_block(sizeof(T));
   dim3 grid((unsigned)((Q + LPB - 1) / LPB), nf);
   LaunchTimer lt(ctx, cls, 2.0 * nf * ctx->g.f * sizeof(T));
-  hipLaunchKernelGGL((fft_c2c_lines_io<T, L, DIR, LPB>), grid, dim3(LPB * FS<L>::R2), 0, ctx->stream, fp, out, Q,
-                     A, B, ES, (const V2<T> *)tw);
+  hipLaunchKernelGGL((fft_c2c_lines_io<T, L, DIR, LPB>), grid, dim3(LPB * LineFFT<T, L>::S), 0, ctx->stream, fp,
+                     out, Q, A, B, ES, (const V2<T> *)tw);
   KAPSEL_LAUNCH_CHECK();
 }
 
@@ -1082,8 +925,7 @@ void launch_plane(kapsel_ctx *ctx, const FieldPtrs<T> &fp, int nf, bool fwd) {
 // still beats the separate z and y passes)
 template <typename T>
 bool plane_ok(const Grid &g) {
-  if (g.ny != g.nz) return false;
-  return g.nz == 32 || g.nz == 64 || g.nz == 128;
+  return plane_ok_grid(g);
 }
 
 // the z and y transforms as one plane pass where plane_ok; false: use the z
@@ -1167,26 +1009,25 @@ void fft_r2c_t(kapsel_ctx *ctx, void *const *fields, int nf) {
   lines_dispatch<T, -1>(ctx, KAPSEL_K_FFT_LINES_FWD, g.nx, fp, nf, plane, plane, 0, plane, tw.x);
 }
 
-// src: nullptr transforms fields in place; otherwise the spectra in src are
-// transformed into fields and src is left as it was
-// out = a + s b c (c may be nullptr) transformed forward, where a plane pass
-// applies; returns false (nothing queued) otherwise
+// out = a + s b c (as fft_plane_r2c_pro) transformed forward, where a plane
+// pass applies; returns false (nothing queued) otherwise
 template <typename T>
 bool fft_r2c_pro_t(kapsel_ctx *ctx, void *const *out, void *const *a, void *const *b, void *const *c, double s,
-                   int nf) {
+                   int nf, unsigned dsq) {
   const Grid &g = ctx->g;
   if (!plane_ok<T>(g)) return false;
   const Twiddles &tw = ctx->tw[prec_of<T>()];
-  FieldPtrs<T> fo = pack<T>(out, nf), fa = pack<T>(a, nf), fb = pack<T>(b, nf);
+  FieldPtrs<T> fo = pack<T>(out, nf), fb = pack<T>(b, nf);
+  FieldPtrs<T> fa = a ? pack<T>(a, nf) : FieldPtrs<T>{};
   FieldPtrs<T> fc = c ? pack<T>(c, nf) : FieldPtrs<T>{};
   dim3 grid((unsigned)g.nx, nf);
   {
-    LaunchTimer lt(ctx, KAPSEL_K_FFT_Z_R2C, (double)(c ? 4 : 3) * nf * g.f * sizeof(T));
+    LaunchTimer lt(ctx, KAPSEL_K_FFT_Z_R2C, (double)((a ? 1 : 0) + (c ? 3 : 2)) * nf * g.f * sizeof(T));
     switch (g.nz) {
-#define KAPSEL_PRO(N, TH)                                                                                        \
-  case N:                                                                                                        \
+#define KAPSEL_PRO(N, TH)                                                                                      \
+  case N:                                                                                                      \
     hipLaunchKernelGGL((fft_plane_r2c_pro<T, N, N, TH>), grid, dim3(TH), 0, ctx->stream, fo, fa, fb, fc, (T)s, \
-                       (const V2<T> *)tw.zh, (const V2<T> *)tw.zrot, (const V2<T> *)tw.y);                     \
+                       dsq, (const V2<T> *)tw.zh, (const V2<T> *)tw.zrot, (const V2<T> *)tw.y);                \
     break;
       KAPSEL_PRO(32, 128)
       KAPSEL_PRO(64, 320)
@@ -1201,9 +1042,10 @@ bool fft_r2c_pro_t(kapsel_ctx *ctx, void *const *out, void *const *a, void *cons
   return true;
 }
 template bool fft_r2c_pro_t<float>(kapsel_ctx *, void *const *, void *const *, void *const *, void *const *, double,
-                                   int);
+                                   int, unsigned);
 template bool fft_r2c_pro_t<double>(kapsel_ctx *, void *const *, void *const *, void *const *, void *const *,
-                                    double, int);
+                                    double, int, unsigned);
+
 
 template <typename T>
 void fft_c2r_t(kapsel_ctx *ctx, void *const *fields, int nf, void *const *src) {
@@ -1278,7 +1120,7 @@ void launch_x_grad(kapsel_ctx *ctx, const T *z, T *p, const double *sc, int beta
   const Twiddles &tw = ctx->tw[prec_of<T>()];
   const long long Q = (long long)g.ny * g.h;
   LaunchTimer lt(ctx, KAPSEL_K_FFT_X_GRAD, 6.0 * g.f * sizeof(T));
-  hipLaunchKernelGGL((fft_x_grad<T, L, LPB>), dim3((unsigned)((Q + LPB - 1) / LPB)), dim3(LPB * FS<L>::R2), 0,
+  hipLaunchKernelGGL((fft_x_grad<T, L, LPB>), dim3((unsigned)((Q + LPB - 1) / LPB)), dim3(LPB * LineFFT<T, L>::S), 0,
                      ctx->stream, z, p, sc, beta_slot, pack<T>(g3, 3), Q, Q, g.h, g.ny, g.wave_x, g.wave_y,
                      g.wave_z, (const V2<T> *)tw.shx, (const V2<T> *)tw.shy, (const V2<T> *)tw.shz,
                      (const V2<T> *)tw.x);
@@ -1293,7 +1135,7 @@ int launch_x_div(kapsel_ctx *ctx, void *const *g3, const T *p, T *ap, double *pa
   const long long Q = (long long)g.ny * g.h;
   const unsigned nb = (unsigned)((Q + LPB - 1) / LPB);
   LaunchTimer lt(ctx, KAPSEL_K_FFT_X_DIV, 5.0 * g.f * sizeof(T));
-  hipLaunchKernelGGL((fft_x_div<T, L, LPB, RES>), dim3(nb), dim3(LPB * FS<L>::R2), 0, ctx->stream, pack<T>(g3, 3), p,
+  hipLaunchKernelGGL((fft_x_div<T, L, LPB, RES>), dim3(nb), dim3(LPB * DivFFT<T, L>::S), 0, ctx->stream, pack<T>(g3, 3), p,
                      ap, partial, Q, Q, g.h, g.ny, g.nz, g.wave_x, g.wave_y, g.wave_z, (const V2<T> *)tw.shx,
                      (const V2<T> *)tw.shy, (const V2<T> *)tw.shz, (const V2<T> *)tw.x);
   KAPSEL_LAUNCH_CHECK();

@@ -6,7 +6,7 @@
 // wavevector is computed from the index (Calc_K*_Ooura, fft_wrapper.h:
 // 761-773) instead of being read from the KX/KY/KZ/K2/IK2 tables of Init_K
 // (fft_wrapper.cxx:47-87): the same values, five fewer streams from HBM.
-#include "common.hpp"
+#include "kspace_ops.hpp"
 
 namespace kapsel {
 
@@ -42,31 +42,8 @@ __global__ void zeta_k2u_k_kernel(T *__restrict__ z0, T *__restrict__ z1, T *__r
     st2(z0 + 2 * c, d1);
     st2(z1 + 2 * c, d2);
   }
-  T kx = (T)(g.wave_x * k.kx), ky = (T)(g.wave_y * k.ky), kz = (T)(g.wave_z * k.kz);
-  T k2 = kx * kx + ky * ky + kz * kz;
-  T ik2 = k2 > (T)0 ? (T)1 / k2 : (T)0;
-  V2<T> o0, o1, o2;
-  if (k.kx != 0) {
-    T ikx = (T)1 / kx;
-    o0 = V2<T>{-ikx * (ky * d1.re + kz * d2.re), -ikx * (ky * d1.im + kz * d2.im)};
-    o1 = d1;
-    o2 = d2;
-  } else if (k.ky != 0) {
-    T dmy = -(kz / ky);
-    o0 = d2;
-    o1 = scal(dmy, d1);
-    o2 = d1;
-  } else {
-    o0 = d1;
-    o1 = d2;
-    o2 = V2<T>{0, 0};
-  }
-  kx *= ik2;
-  ky *= ik2;
-  kz *= ik2;
-  V2<T> a = {-(ky * o2.im - kz * o1.im), (ky * o2.re - kz * o1.re)};
-  V2<T> b = {-(kz * o0.im - kx * o2.im), (kz * o0.re - kx * o2.re)};
-  V2<T> e = {-(kx * o1.im - ky * o0.im), (kx * o1.re - ky * o0.re)};
+  V2<T> a, b, e;
+  zeta2u_mode(g, k, d1, d2, a, b, e);
   if (c == 0) {
     double dd[3];
     for (int d = 0; d < 3; d++) {
@@ -82,46 +59,52 @@ __global__ void zeta_k2u_k_kernel(T *__restrict__ z0, T *__restrict__ z1, T *__r
   st2(u2 + 2 * c, e);
 }
 
-// U_k2zeta_k (fft_wrapper.cxx:428-497); with p, u is the sum of the two
-// spectra u + p, which is also stored in o
+// U_k2zeta_k (fft_wrapper.cxx:428-497)
 template <typename T>
 __global__ void u_k2zeta_k_kernel(const T *__restrict__ u0, const T *__restrict__ u1, const T *__restrict__ u2,
-                                  T *__restrict__ z0, T *__restrict__ z1, double *__restrict__ dc, Grid g,
-                                  const T *__restrict__ p0 = nullptr, const T *__restrict__ p1 = nullptr,
-                                  const T *__restrict__ p2 = nullptr, T *__restrict__ o0 = nullptr,
-                                  T *__restrict__ o1 = nullptr, T *__restrict__ o2 = nullptr) {
+                                  T *__restrict__ z0, T *__restrict__ z1, double *__restrict__ dc, Grid g) {
   size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.nc) return;
   KIdx k = kidx(g, c);
   V2<T> a = ld2(u0 + 2 * c), b = ld2(u1 + 2 * c), e = ld2(u2 + 2 * c);
-  if (p0) {
-    a = a + ld2(p0 + 2 * c);
-    b = b + ld2(p1 + 2 * c);
-    e = e + ld2(p2 + 2 * c);
-    st2(o0 + 2 * c, a);
-    st2(o1 + 2 * c, b);
-    st2(o2 + 2 * c, e);
-  }
   if (c == 0) {
     dc[0] = (double)a.re;
     dc[1] = (double)b.re;
     dc[2] = (double)e.re;
   }
-  T ks0 = (T)(k.kx * g.wave_x), ks1 = (T)(k.ky * g.wave_y), ks2 = (T)(k.kz * g.wave_z);
-  V2<T> w0 = {-(ks1 * e.im - ks2 * b.im), (ks1 * e.re - ks2 * b.re)};
-  V2<T> w1 = {-(ks2 * a.im - ks0 * e.im), (ks2 * a.re - ks0 * e.re)};
-  V2<T> w2 = {-(ks0 * b.im - ks1 * a.im), (ks0 * b.re - ks1 * a.re)};
   V2<T> za, zb;
-  if (k.kx != 0) {
-    za = w1;
-    zb = w2;
-  } else if (k.ky != 0) {
-    za = w2;
-    zb = w0;
-  } else {
-    za = w0;
-    zb = w1;
+  u2zeta_mode(g, k, a, b, e, za, zb);
+  st2(z0 + 2 * c, za);
+  st2(z1 + 2 * c, zb);
+}
+
+// Solenoidal_uk after the 2/3 cut of the spectrum f, then U_k2zeta_k of
+// u = a + P f, which is stored in o (a and f are left unchanged): the
+// solenoidal_uk and u_k2zeta_k kernels of the sum in one pass
+template <typename T>
+__global__ void sole_u2zeta_sum_kernel(P5<T> a3, P5<T> f3, P5<T> o3, T *__restrict__ z0, T *__restrict__ z1,
+                                       double *__restrict__ dc, Grid g) {
+  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  KIdx k = kidx(g, c);
+  V2<T> fa{0, 0}, fb{0, 0}, fe{0, 0};
+  if (!is_cut(g, k)) {
+    fa = ld2(f3.p[0] + 2 * c);
+    fb = ld2(f3.p[1] + 2 * c);
+    fe = ld2(f3.p[2] + 2 * c);
+    solenoidal_mode(g, k, fa, fb, fe);
   }
+  const V2<T> a = ld2(a3.p[0] + 2 * c) + fa, b = ld2(a3.p[1] + 2 * c) + fb, e = ld2(a3.p[2] + 2 * c) + fe;
+  st2(o3.p[0] + 2 * c, a);
+  st2(o3.p[1] + 2 * c, b);
+  st2(o3.p[2] + 2 * c, e);
+  if (c == 0) {
+    dc[0] = (double)a.re;
+    dc[1] = (double)b.re;
+    dc[2] = (double)e.re;
+  }
+  V2<T> za, zb;
+  u2zeta_mode(g, k, a, b, e, za, zb);
   st2(z0 + 2 * c, za);
   st2(z1 + 2 * c, zb);
 }
@@ -150,51 +133,27 @@ __global__ void advection_products_kernel(T *__restrict__ u0, T *__restrict__ u1
 //          Euler step Field_solver_Euler (fluid_solver.cxx:121-136) of
 //          NSsolute_solver_Euler, dmy0 = NU, dmy1 = dt.
 // The advection term is left in a0/a1 as the reference leaves it in its
-// global f_ns0 (slaved); the Euler variant leaves the full right-hand side
-// there, as Rhs_NS_solute leaves it in f_ns1.
+// global f_ns0 (slaved; store_a 0: not stored); the Euler variant leaves the
+// full right-hand side there, as Rhs_NS_solute leaves it in f_ns1.
 template <typename T, int EULER>
 __global__ void advection_combine_slaved_kernel(const T *__restrict__ u0, const T *__restrict__ u1,
                                                 const T *__restrict__ u2, T *__restrict__ a0,
                                                 T *__restrict__ a1, T *__restrict__ z0, T *__restrict__ z1,
-                                                Grid g, double dmy0, double dmy1) {
+                                                Grid g, double dmy0, double dmy1, int store_a = 1) {
   size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.nc) return;
   KIdx k = kidx(g, c);
-  const T k1 = (T)(k.kx * g.wave_x), k2 = (T)(k.ky * g.wave_y), k3 = (T)(k.kz * g.wave_z);
-  V2<T> u2u3 = ld2(u0 + 2 * c), u3u1 = ld2(u1 + 2 * c), u1u2 = ld2(u2 + 2 * c);
-  V2<T> u22_u32 = ld2(a0 + 2 * c), u32_u12 = ld2(a1 + 2 * c);
-  V2<T> u12_u22 = V2<T>{-u32_u12.re - u22_u32.re, -u32_u12.im - u22_u32.im};
-  const T k1k2 = k1 * k2, k2k3 = k2 * k3, k3k1 = k3 * k1;
-  const T c0 = (k3 - k2) * (k3 + k2), c1 = (k1 - k3) * (k1 + k3), c2 = (k2 - k1) * (k2 + k1);
   V2<T> w[3];
-#define KAPSEL_OMEGA(out, p)                                                             \
-  out[0].p = -(k2k3 * u22_u32.p + c0 * u2u3.p + k3k1 * u1u2.p - k1k2 * u3u1.p);          \
-  out[1].p = -(k3k1 * u32_u12.p + c1 * u3u1.p + k1k2 * u2u3.p - k2k3 * u1u2.p);          \
-  out[2].p = -(k1k2 * u12_u22.p + c2 * u1u2.p + k2k3 * u3u1.p - k3k1 * u2u3.p);
-  KAPSEL_OMEGA(w, re)
-  KAPSEL_OMEGA(w, im)
-#undef KAPSEL_OMEGA
+  advection_omega(AdvCoef<T>(g, k), ld2(u0 + 2 * c), ld2(u1 + 2 * c), ld2(u2 + 2 * c), ld2(a0 + 2 * c),
+                  ld2(a1 + 2 * c), w);
   V2<T> f0, f1;
-  if (k.kx != 0) {
-    f0 = w[1];
-    f1 = w[2];
-  } else if (k.ky != 0) {
-    f0 = w[2];
-    f1 = w[0];
-  } else {
-    f0 = w[0];
-    f1 = w[1];
-  }
+  omega_pick(k, w, f0, f1);
   if constexpr (EULER) {
     if (!is_cut(g, k)) {
-      const double q1 = k.kx * g.wave_x, q2 = k.ky * g.wave_y, q3 = k.kz * g.wave_z;
-      const T nk2 = (T)(dmy0 * (q1 * q1 + q2 * q2 + q3 * q3));  // NU * K2[im]
-      const T dt = (T)dmy1;
       V2<T> za = ld2(z0 + 2 * c), zb = ld2(z1 + 2 * c);
-      f0 = V2<T>{f0.re + -(nk2 * za.re), f0.im + -(nk2 * za.im)};
-      f1 = V2<T>{f1.re + -(nk2 * zb.re), f1.im + -(nk2 * zb.im)};
-      st2(z0 + 2 * c, V2<T>{za.re + dt * f0.re, za.im + dt * f0.im});
-      st2(z1 + 2 * c, V2<T>{zb.re + dt * f1.re, zb.im + dt * f1.im});
+      euler_update(g, k, dmy0, dmy1, f0, f1, za, zb);
+      st2(z0 + 2 * c, za);
+      st2(z1 + 2 * c, zb);
     } else {
       // Rhs_NS_solute cut zeta before (fluid_solver.cxx:201-203)
       st2(z0 + 2 * c, V2<T>{0, 0});
@@ -204,18 +163,13 @@ __global__ void advection_combine_slaved_kernel(const T *__restrict__ u0, const 
     st2(a1 + 2 * c, f1);
     return;
   }
-  st2(a0 + 2 * c, f0);
-  st2(a1 + 2 * c, f1);
+  if (store_a) {
+    st2(a0 + 2 * c, f0);
+    st2(a1 + 2 * c, f1);
+  }
   if (!is_cut(g, k)) {
-    const double q1 = k.kx * g.wave_x, q2 = k.ky * g.wave_y, q3 = k.kz * g.wave_z;
-    const double K2 = q1 * q1 + q2 * q2 + q3 * q3;  // K2 of Init_K (fft_wrapper.cxx:71-73)
-    const double ik2 = K2 > 0.0 ? 1. / K2 : 0.0;
-    // the reference exponent and factors are double; the update is done in T
-    const T dmy = (T)(exp(dmy0 * K2) - 1.);
-    const T s = (T)(dmy1 * ik2);
     V2<T> za = ld2(z0 + 2 * c), zb = ld2(z1 + 2 * c);
-    za = V2<T>{za.re + dmy * (za.re - s * f0.re), za.im + dmy * (za.im - s * f0.im)};
-    zb = V2<T>{zb.re + dmy * (zb.re - s * f1.re), zb.im + dmy * (zb.im - s * f1.im)};
+    slaved_update(g, k, dmy0, dmy1, f0, f1, za, zb);
     st2(z0 + 2 * c, za);
     st2(z1 + 2 * c, zb);
   }
@@ -234,14 +188,11 @@ __global__ void solenoidal_uk_kernel(P5<T> u, P5<T> o, Grid g, int trunc) {
     for (int d = 0; d < 3; d++) st2(o.p[d] + 2 * c, V2<T>{0, 0});
     return;
   }
-  const T kx = (T)(k.kx * g.wave_x), ky = (T)(k.ky * g.wave_y), kz = (T)(k.kz * g.wave_z);
-  const T k2 = kx * kx + ky * ky + kz * kz;
-  const T ik2 = k2 > (T)0 ? (T)1 / k2 : (T)0;
   V2<T> a = ld2(u.p[0] + 2 * c), b = ld2(u.p[1] + 2 * c), e = ld2(u.p[2] + 2 * c);
-  V2<T> dmy = {ik2 * (a.re * kx + b.re * ky + e.re * kz), ik2 * (a.im * kx + b.im * ky + e.im * kz)};
-  st2(o.p[0] + 2 * c, a - scal(kx, dmy));
-  st2(o.p[1] + 2 * c, b - scal(ky, dmy));
-  st2(o.p[2] + 2 * c, e - scal(kz, dmy));
+  solenoidal_mode(g, k, a, b, e);
+  st2(o.p[0] + 2 * c, a);
+  st2(o.p[1] + 2 * c, b);
+  st2(o.p[2] + 2 * c, e);
 }
 
 // Make_f_particle_dt_nonsole (f_particle.h:37-60): f = up - phi u
@@ -340,16 +291,15 @@ static void u_k2zeta_k_t(kapsel_ctx *ctx) {
 void u_k2zeta_k(kapsel_ctx *ctx) { KAPSEL_DISPATCH(ctx, u_k2zeta_k_t, ctx); }
 
 template <typename T>
-static void u_k2zeta_sum_k_t(kapsel_ctx *ctx, void *const *a3, void *const *b3) {
-  LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, 11.0 * ctx->g.f * sizeof(T));
-  hipLaunchKernelGGL(u_k2zeta_k_kernel<T>, dim3(blocks_for(ctx->g.nc, TPB)), dim3(TPB), 0, ctx->stream,
-                     (const T *)a3[0], (const T *)a3[1], (const T *)a3[2], F<T>(ctx, KAPSEL_ZETA0),
-                     F<T>(ctx, KAPSEL_ZETA1), ctx->d_uk_dc, ctx->g, (const T *)b3[0], (const T *)b3[1],
-                     (const T *)b3[2], F<T>(ctx, KAPSEL_U0), F<T>(ctx, KAPSEL_U1), F<T>(ctx, KAPSEL_U2));
+static void sole_u2zeta_sum_t(kapsel_ctx *ctx, void *const *a3, void *const *f3) {
+  LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, (8.0 * ctx->g.f + 3.0 * 2 * kept_count(ctx->g)) * sizeof(T));
+  hipLaunchKernelGGL(sole_u2zeta_sum_kernel<T>, dim3(blocks_for(ctx->g.nc, TPB)), dim3(TPB), 0, ctx->stream,
+                     p5<T>(a3, 3), p5<T>(f3, 3), p5<T>(ctx->fields + KAPSEL_U0, 3), F<T>(ctx, KAPSEL_ZETA0),
+                     F<T>(ctx, KAPSEL_ZETA1), ctx->d_uk_dc, ctx->g);
   KAPSEL_LAUNCH_CHECK();
 }
-void u_k2zeta_sum_k(kapsel_ctx *ctx, void *const *a3, void *const *b3) {
-  KAPSEL_DISPATCH(ctx, u_k2zeta_sum_k_t, ctx, a3, b3);
+void sole_u2zeta_sum(kapsel_ctx *ctx, void *const *a3, void *const *f3) {
+  KAPSEL_DISPATCH(ctx, sole_u2zeta_sum_t, ctx, a3, f3);
 }
 
 template <typename T>
@@ -363,18 +313,20 @@ static void advection_products_t(kapsel_ctx *ctx) {
 void advection_products(kapsel_ctx *ctx) { KAPSEL_DISPATCH(ctx, advection_products_t, ctx); }
 
 template <typename T>
-static void advection_combine_slaved_t(kapsel_ctx *ctx, double dt_fluid) {
+static void advection_combine_slaved_t(kapsel_ctx *ctx, double dt_fluid, void *const *in5, int store_a) {
   double dmy0 = -ctx->nu * dt_fluid;
   double dmy1 = 1. / ctx->nu;
-  LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, (7.0 * ctx->g.f + 8.0 * kept_count(ctx->g)) * sizeof(T));
+  void *const def[5] = {ctx->fields[KAPSEL_U0], ctx->fields[KAPSEL_U1], ctx->fields[KAPSEL_U2], ctx->adv_w[0],
+                        ctx->adv_w[1]};
+  void *const *p = in5 ? in5 : def;
+  LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, ((store_a ? 7.0 : 5.0) * ctx->g.f + 8.0 * kept_count(ctx->g)) * sizeof(T));
   hipLaunchKernelGGL((advection_combine_slaved_kernel<T, 0>), dim3(blocks_for(ctx->g.nc, TPB)), dim3(TPB), 0,
-                     ctx->stream, F<T>(ctx, KAPSEL_U0), F<T>(ctx, KAPSEL_U1), F<T>(ctx, KAPSEL_U2),
-                     (T *)ctx->adv_w[0], (T *)ctx->adv_w[1], F<T>(ctx, KAPSEL_ZETA0),
-                     F<T>(ctx, KAPSEL_ZETA1), ctx->g, dmy0, dmy1);
+                     ctx->stream, (const T *)p[0], (const T *)p[1], (const T *)p[2], (T *)p[3], (T *)p[4],
+                     F<T>(ctx, KAPSEL_ZETA0), F<T>(ctx, KAPSEL_ZETA1), ctx->g, dmy0, dmy1, store_a);
   KAPSEL_LAUNCH_CHECK();
 }
-void advection_combine_slaved(kapsel_ctx *ctx, double dt_fluid) {
-  KAPSEL_DISPATCH(ctx, advection_combine_slaved_t, ctx, dt_fluid);
+void advection_combine_slaved(kapsel_ctx *ctx, double dt_fluid, void *const *in5, int store_a) {
+  KAPSEL_DISPATCH(ctx, advection_combine_slaved_t, ctx, dt_fluid, in5, store_a);
 }
 
 template <typename T>
