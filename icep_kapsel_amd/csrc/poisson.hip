@@ -152,21 +152,9 @@ __global__ void grad_shift2_f64_kernel(const double *__restrict__ a, const doubl
   }
 }
 
-// g_d *= 0.5 (eps_i + eps_{i+e_d})   (operate_dielectric.cxx:492-508)
-template <typename T>
-__global__ void face_mul_kernel(T *__restrict__ g0, T *__restrict__ g1, T *__restrict__ g2,
-                                const T *__restrict__ eps, Grid g) {
-  size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= g.nr) return;
-  RI p = ridx(g, r);
-  const T en = eps[p.im];
-  g0[p.im] *= (T)0.5 * (eps[nbr(g, p.i + 1, p.j, p.k)] + en);
-  g1[p.im] *= (T)0.5 * (eps[nbr(g, p.i, p.j + 1, p.k)] + en);
-  g2[p.im] *= (T)0.5 * (eps[nbr(g, p.i, p.j, p.k + 1)] + en);
-}
-
 // f_d = 0.5 (eps_i + eps_{i+e_d}) at the real points, 0 in the row padding:
-// the factors face_mul_kernel applies, kept for the fused plane pass
+// the face permittivities of the operator (operate_dielectric.cxx:492-508),
+// which the plane pass multiplies in
 template <typename T>
 __global__ void face_fields_kernel(const T *__restrict__ eps, T *__restrict__ f0, T *__restrict__ f1,
                                    T *__restrict__ f2, Grid g) {
@@ -222,19 +210,6 @@ __global__ void sub_kernel(T *__restrict__ a, const T *__restrict__ b, const dou
   RI p = ridx(g, r);
   if (bg) a[p.im] -= b[p.im] - (T)bg[0];
   else a[p.im] -= b[p.im];
-}
-
-// z_k <- z_k / (eps_mean |k|^2), 0 at k = 0
-template <typename T>
-__global__ void precond_scale_kernel(T *__restrict__ zk, double inv_eps, Grid g) {
-  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= g.nc) return;
-  KIdx k = kidx(g, c);
-  const double q0 = k.kx * g.wave_x, q1 = k.ky * g.wave_y, q2 = k.kz * g.wave_z;
-  const double k2 = q0 * q0 + q1 * q1 + q2 * q2;
-  const T s = k2 > 0 ? (T)(inv_eps / k2) : (T)0;
-  V2<T> z = ld2(zk + 2 * c);
-  st2(zk + 2 * c, scal(s, z));
 }
 
 __device__ __forceinline__ double block_sum(double v) {
@@ -364,44 +339,6 @@ __global__ __launch_bounds__(TPB) void finish2_kernel(const double *__restrict__
       __threadfence_system();
     }
   }
-}
-
-// CG step on S = -A with ap = A p:  alpha = rz / (p.S p) = rz / (-p.ap),
-// x += alpha p, r = A x - b  ->  r += alpha ap; partial sums of r.r
-template <typename T>
-__global__ __launch_bounds__(TPB) void cg_xr_kernel(T *__restrict__ x, T *__restrict__ r, const T *__restrict__ p,
-                                                    const T *__restrict__ ap, const double *__restrict__ scal,
-                                                    double *__restrict__ partial, Grid g) {
-  const double pap = -scal[S_PAP];
-  const double alpha = pap != 0.0 ? scal[S_RZ] / pap : 0.0;
-  const T a = (T)alpha;
-  double s = 0.0;
-  for (size_t q = (size_t)blockIdx.x * TPB + threadIdx.x; q < g.nr; q += (size_t)gridDim.x * TPB) {
-    RI pt = ridx(g, q);
-    x[pt.im] += a * p[pt.im];
-    const T rn = r[pt.im] + a * ap[pt.im];
-    r[pt.im] = rn;
-    s += (double)rn * (double)rn;
-  }
-  s = block_sum(s);
-  if (threadIdx.x == 0) partial[blockIdx.x] = s;
-}
-
-template <typename T>
-__global__ void cg_p_kernel(T *__restrict__ p, const T *__restrict__ z, const double *__restrict__ scal, Grid g) {
-  size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= g.nr) return;
-  RI pt = ridx(g, q);
-  p[pt.im] = z[pt.im] + (T)scal[S_BETA] * p[pt.im];
-}
-
-// y = x - y over real points (r = A x - b with y holding A x is done as b - Ax via negation)
-template <typename T>
-__global__ void resid_kernel(const T *__restrict__ ax, const T *__restrict__ b, T *__restrict__ r, Grid g) {
-  size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= g.nr) return;
-  RI pt = ridx(g, q);
-  r[pt.im] = ax[pt.im] - b[pt.im];
 }
 
 template <typename D, typename S>
@@ -624,6 +561,11 @@ __global__ void extrapolate_all_kernel(double *__restrict__ x, double *__restric
 }
 
 // over every entry: x64 -= (double) d   (r = A x - b: the correction enters with a minus sign)
+__global__ void sub_f64_kernel(double *__restrict__ x, const double *__restrict__ d, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] -= d[i];
+}
+
 __global__ void sub_all_kernel(double *__restrict__ x, const float *__restrict__ d, size_t n) {
   size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e < n) x[e] -= (double)d[e];
@@ -746,6 +688,13 @@ void ensure_storage(kapsel_ctx *ctx) {
     for (void *&p : ctx->pscratch) {
       KAPSEL_HIP(hipMalloc(&p, ctx->g.f * ctx->elem));
       KAPSEL_HIP(hipMemsetAsync(p, 0, ctx->g.f * ctx->elem, ctx->stream));
+    }
+  }
+  if (ctx->prec == KAPSEL_FP64 && !ctx->p64[2]) {
+    // the residual and right-hand side spectra of the FP64 solve
+    for (int i = 2; i < 4; i++) {
+      KAPSEL_HIP(hipMalloc(&ctx->p64[i], ctx->g.f * sizeof(double)));
+      KAPSEL_HIP(hipMemsetAsync(ctx->p64[i], 0, ctx->g.f * sizeof(double), ctx->stream));
     }
   }
   if (ctx->prec == KAPSEL_FP32 && !ctx->p64[0]) {
@@ -877,71 +826,6 @@ struct ConvergenceWatch {
     }
   }
 };
-
-// z = M^{-1} r
-template <typename T>
-void apply_precond(kapsel_ctx *ctx, const T *r, T *z, double inv_eps) {
-  const Grid &g = ctx->g;
-  {
-    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 2, sizeof(T)));
-    hipLaunchKernelGGL(copy_real_kernel<T>, dim3(blocks_for(g.f, TPB)), dim3(TPB), 0, ctx->stream, r, z, g);
-    KAPSEL_LAUNCH_CHECK();
-  }
-  void *z1[1] = {z};
-  fft_r2c_t<T>(ctx, z1, 1);
-  {
-    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 2, sizeof(T)));
-    hipLaunchKernelGGL(precond_scale_kernel<T>, dim3(blocks_for(g.nc, TPB)), dim3(TPB), 0, ctx->stream, z, inv_eps,
-                       g);
-    KAPSEL_LAUNCH_CHECK();
-  }
-  fft_c2r_t<T>(ctx, z1, 1);
-}
-
-// Preconditioned CG on S = -A, x warm-started; w = {r, p, ap, z}; s = operator
-// scratch.  Stops when ||A x - b|| < tol_abs (checked every iteration).
-// Returns the iteration count; *rr_out = final ||r||^2.
-template <typename T>
-int cg_t(kapsel_ctx *ctx, T *x, const T *b, const T *eps, double tol_abs, int maxit, int precond, double inv_eps,
-         T *const *w, T *const *s, double *rr_out) {
-  const Grid &g = ctx->g;
-  T *r = w[0], *p = w[1], *ap = w[2], *z = precond ? w[3] : w[0];
-  ax_t<T>(ctx, x, ap, eps, s);
-  hipLaunchKernelGGL(resid_kernel<T>, dim3(blocks_for(g.nr, TPB)), dim3(TPB), 0, ctx->stream, ap, b, r, g);
-  KAPSEL_LAUNCH_CHECK();
-  if (precond) apply_precond<T>(ctx, r, z, inv_eps);
-  KAPSEL_HIP(hipMemcpyAsync(p, z, g.f * sizeof(T), hipMemcpyDeviceToDevice, ctx->stream));
-  dot_t<T, T>(ctx, r, z, S_RZ);
-  dot_t<T, T>(ctx, r, r, S_RR);
-  double rr = read_scal(ctx, S_RR);
-  const double tol2 = tol_abs * tol_abs;
-  ConvergenceWatch cw(ctx, 1.0, rr);
-  cw.run(tol2, maxit, [&]() {
-    ax_t<T>(ctx, p, ap, eps, s);
-    dot_t<T, T>(ctx, p, ap, S_PAP);
-    {
-      LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 6, sizeof(T)));
-      hipLaunchKernelGGL(cg_xr_kernel<T>, dim3(NPART), dim3(TPB), 0, ctx->stream, x, r, p, ap, ctx->d_scal,
-                         ctx->d_partial, g);
-      KAPSEL_LAUNCH_CHECK();
-      hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->d_partial, NPART, ctx->d_scal,
-                         S_RR, 0);
-      KAPSEL_LAUNCH_CHECK();
-    }
-    if (precond) apply_precond<T>(ctx, r, z, inv_eps);
-    dot_t<T, T>(ctx, r, z, S_RZN, 1);
-    {
-      LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 3, sizeof(T)));
-      hipLaunchKernelGGL(cg_p_kernel<T>, dim3(blocks_for(g.nr, TPB)), dim3(TPB), 0, ctx->stream, p, z,
-                         ctx->d_scal, g);
-      KAPSEL_LAUNCH_CHECK();
-    }
-  });
-  rr = cw.rr;
-  const int it = cw.it;
-  *rr_out = rr;
-  return it;
-}
 
 // One k-space PCG iteration: p = z + beta p, ap = A p (3 inverse and 3
 // forward transforms), alpha, x += alpha p, r += alpha ap, z = M^{-1} r,
@@ -1082,19 +966,6 @@ T *fld(kapsel_ctx *ctx, int f) {
   return (T *)ctx->fields[f];
 }
 
-// true relative residual ||b - A x|| / ||b|| of the context-precision fields
-template <typename T>
-double true_relres(kapsel_ctx *ctx, const T *x, const T *b, const T *eps, T *r, T *const *s) {
-  const Grid &g = ctx->g;
-  ax_t<T>(ctx, x, r, eps, s);
-  hipLaunchKernelGGL(resid_kernel<T>, dim3(blocks_for(g.nr, TPB)), dim3(TPB), 0, ctx->stream, r, b, r, g);
-  KAPSEL_LAUNCH_CHECK();
-  dot_t<T, T>(ctx, r, r, S_RR);
-  dot_t<T, T>(ctx, b, b, S_BB);
-  double rr = read_scal(ctx, S_RR), bb = read_scal(ctx, S_BB);
-  return bb > 0 ? sqrt(rr / bb) : 0.0;
-}
-
 template <typename T>
 double inv_eps_of(kapsel_ctx *ctx, const T *eps, T *scratch) {
   const Grid &g = ctx->g;
@@ -1104,31 +975,98 @@ double inv_eps_of(kapsel_ctx *ctx, const T *eps, T *scratch) {
   return m > 0 ? 1.0 / m : 1.0;
 }
 
-// FP64 context: CG in FP64 with the reference's restart-from-zero fallback
+// FP64 context: the reference's CG (operate_dielectric.cxx:541-578: start
+// from the current potential, stop at ||A x - b|| < tol ||b||, solve again
+// from zero after maxit iterations) on the half spectra.  With x0 the
+// start, r0 = A x0 - b once, then CG on A d = r0 from d = 0 -- the iterates
+// of CG on A x = b from x0, x = x0 - d -- with the k-space iteration of the
+// FP32 correction solves (fused x passes, one plane pass, one update kernel)
+// in FP64.  relres: the residual of the final x, evaluated anew.
 int solve_fp64(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int precond, double *relres) {
+  const Grid &g = ctx->g;
   double *eps = fld<double>(ctx, KAPSEL_EPSILON), *x = fld<double>(ctx, KAPSEL_POTENTIAL);
   double *b = fld<double>(ctx, KAPSEL_RHS);
   double *s[4] = {(double *)ctx->pscratch[0], (double *)ctx->pscratch[1], (double *)ctx->pscratch[2],
                   (double *)ctx->pscratch[3]};
   double *w[4] = {fld<double>(ctx, KAPSEL_WORK0), fld<double>(ctx, KAPSEL_WORK1), fld<double>(ctx, KAPSEL_WORK2),
                   fld<double>(ctx, KAPSEL_WORK3)};
+  double *face[3] = {(double *)ctx->pscratch[5], (double *)ctx->pscratch[6], (double *)ctx->pscratch[7]};
+  double *xk = s[0], *d = (double *)ctx->pscratch[4], *rk = ctx->p64[2], *bk = ctx->p64[3];
+  const unsigned fb = blocks_for(g.f, TPB);
   build_rhs_t<double>(ctx, b, fld<double>(ctx, KAPSEL_RHO_FREE), eps, e0, s);
   dot_t<double, double>(ctx, b, b, S_BB);
   const double bb = read_scal(ctx, S_BB);
   if (bb == 0.0) {
-    KAPSEL_HIP(hipMemsetAsync(x, 0, ctx->g.f * sizeof(double), ctx->stream));
+    KAPSEL_HIP(hipMemsetAsync(x, 0, g.f * sizeof(double), ctx->stream));
     *relres = 0.0;
     return 0;
   }
   const double inv_eps = precond ? inv_eps_of<double>(ctx, eps, w[3]) : 1.0;
-  double rr = 0.0;
-  int it = cg_t<double>(ctx, x, b, eps, tol * sqrt(bb), maxit, precond, inv_eps, w, s, &rr);
+  {
+    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 4, sizeof(double)));
+    hipLaunchKernelGGL(fill_kernel<double>, dim3(1), dim3(64), 0, ctx->stream, ctx->d_scal + S_INVEPS, inv_eps,
+                       (size_t)1);
+    hipLaunchKernelGGL(face_fields_kernel<double>, dim3(fb), dim3(TPB), 0, ctx->stream, eps, face[0], face[1], face[2],
+                       g);
+    KAPSEL_LAUNCH_CHECK();
+  }
+  // the start and the right-hand side as spectra
+  {
+    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 4, sizeof(double)));
+    hipLaunchKernelGGL(copy_real_kernel<double>, dim3(fb), dim3(TPB), 0, ctx->stream, x, xk, g);
+    hipLaunchKernelGGL(copy_real_kernel<double>, dim3(fb), dim3(TPB), 0, ctx->stream, b, bk, g);
+    KAPSEL_LAUNCH_CHECK();
+  }
+  void *xb[2] = {xk, bk};
+  fft_r2c_t<double>(ctx, xb, 2);
+  void *g3[3] = {s[1], s[2], s[3]};
+  void *f3[3] = {face[0], face[1], face[2]};
+  // rk = A xk - bk and ||rk||^2 (Parseval) into S_RR
+  auto residual = [&]() {
+    if (fft_x_grad_t<double>(ctx, xk, xk, ctx->d_scal, -1, g3)) {
+      fft_face_mid_t<double>(ctx, g3, f3);
+      const int nb = fft_x_res_t<double>(ctx, g3, bk, rk, ctx->d_partial);
+      hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->d_partial, nb, ctx->d_scal, S_RR,
+                         0);
+    } else {
+      {
+        LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 4, sizeof(double)));
+        hipLaunchKernelGGL(grad_shift_kernel<double>, dim3(blocks_for(g.nc, TPB)), dim3(TPB), 0, ctx->stream, xk,
+                           s[1], s[2], s[3], sh<double>(ctx, 0), sh<double>(ctx, 1), sh<double>(ctx, 2), g);
+        KAPSEL_LAUNCH_CHECK();
+      }
+      fft_face_t<double>(ctx, g3, f3);
+      LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 5, sizeof(double)));
+      hipLaunchKernelGGL(kres_kernel<double>, dim3(NPART), dim3(TPB), 0, ctx->stream, s[1], s[2], s[3], bk, rk,
+                         sh<double>(ctx, 0), sh<double>(ctx, 1), sh<double>(ctx, 2), ctx->d_partial, g);
+      hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->d_partial, NPART, ctx->d_scal,
+                         S_RR, 0);
+    }
+    KAPSEL_LAUNCH_CHECK();
+    return read_scal(ctx, S_RR) / (double)g.nr;
+  };
+  const double tol_abs = tol * sqrt(bb);
+  double rr0 = residual(), rr = 0.0;
+  int it = kcg_t<double>(ctx, d, rk, eps, tol_abs, maxit, precond, w, s + 1, face, &rr, rr0);
   if (it >= maxit) {
     // operate_dielectric.cxx:563-570: solve again without the initial guess
-    KAPSEL_HIP(hipMemsetAsync(x, 0, ctx->g.f * sizeof(double), ctx->stream));
-    it = cg_t<double>(ctx, x, b, eps, tol * sqrt(bb), maxit, precond, inv_eps, w, s, &rr);
+    KAPSEL_HIP(hipMemsetAsync(xk, 0, g.f * sizeof(double), ctx->stream));
+    rr0 = residual();
+    it = kcg_t<double>(ctx, d, rk, eps, tol_abs, maxit, precond, w, s + 1, face, &rr, rr0);
   }
-  *relres = true_relres<double>(ctx, x, b, eps, w[0], s);
+  {
+    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 3, sizeof(double)));
+    hipLaunchKernelGGL(sub_f64_kernel, dim3(fb), dim3(TPB), 0, ctx->stream, xk, d, g.f);
+    KAPSEL_LAUNCH_CHECK();
+  }
+  *relres = sqrt(residual()) / sqrt(bb);
+  void *x1[1] = {xk};
+  fft_c2r_t<double>(ctx, x1, 1);
+  {
+    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 2, sizeof(double)));
+    hipLaunchKernelGGL(copy_real_kernel<double>, dim3(fb), dim3(TPB), 0, ctx->stream, xk, x, g);
+    KAPSEL_LAUNCH_CHECK();
+  }
   return it;
 }
 
