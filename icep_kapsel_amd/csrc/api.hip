@@ -317,27 +317,40 @@ static void zeta_k2u(kapsel_ctx *ctx, int fix_dc, void *const *keep3 = nullptr) 
 // Where a plane pass applies, the five products are formed inside it (into
 // adv_x and adv_w, U left as it is) instead of by the products kernel.
 // keep_adv false: the advection term (the reference's f_ns0) is not stored.
+// fix_dc >= 0: followed by the step's Zeta_k2u after the 2/3 cut with the
+// FIX_CELL mask fix_dc (sp_3d_ns.cxx:29-42), its Zeta_k2u_k in the same
+// pass as the zeta update (keep3 as zeta_k2u; the advection term is then
+// not stored).
 template <typename T>
-static void ns_slaved_euler_t(kapsel_ctx *ctx, const kapsel_time &t, bool keep_adv) {
+static void ns_slaved_euler_t(kapsel_ctx *ctx, const kapsel_time &t, bool keep_adv, int fix_dc,
+                              void *const *keep3) {
   zeta_k2u(ctx, 0);
   void **u = ctx->fields + KAPSEL_U0;
+  void *prod[5] = {ctx->adv_x[0], ctx->adv_x[1], ctx->adv_x[2], ctx->adv_w[0], ctx->adv_w[1]};
+  void *five[5] = {u[0], u[1], u[2], ctx->adv_w[0], ctx->adv_w[1]};
+  void *const *in5 = five;
   if (plane_ok_grid(ctx->g)) {
-    for (void *&p : ctx->adv_x)
-      if (!p) KAPSEL_HIP(hipMalloc(&p, ctx->g.f * sizeof(T)));
-    void *prod[5] = {ctx->adv_x[0], ctx->adv_x[1], ctx->adv_x[2], ctx->adv_w[0], ctx->adv_w[1]};
+    for (int i = 0; i < 3; i++)
+      if (!ctx->adv_x[i]) KAPSEL_HIP(hipMalloc(&ctx->adv_x[i], ctx->g.f * sizeof(T)));
+    for (int i = 0; i < 3; i++) prod[i] = ctx->adv_x[i];
     void *b[5] = {u[1], u[2], u[0], u[1], u[2]}, *c[5] = {u[2], u[0], u[1], u[2], u[0]};
     (void)fft_r2c_pro_t<T>(ctx, prod, nullptr, b, c, 1.0, 5, 0x18u);
-    advection_combine_slaved(ctx, t.dt_fluid, prod, keep_adv);
-    return;
+    in5 = prod;
+  } else {
+    advection_products(ctx);
+    fft_r2c(ctx, five, 5);
   }
-  advection_products(ctx);
-  void *five[5] = {u[0], u[1], u[2], ctx->adv_w[0], ctx->adv_w[1]};
-  fft_r2c(ctx, five, 5);
-  advection_combine_slaved(ctx, t.dt_fluid, nullptr, keep_adv);
+  if (fix_dc >= 0) {
+    slaved_zeta2u(ctx, t.dt_fluid, in5, fix_dc, keep3);
+    fft_c2r(ctx, u, 3, keep3);
+  } else {
+    advection_combine_slaved(ctx, t.dt_fluid, in5, keep_adv);
+  }
 }
-static void ns_slaved_euler(kapsel_ctx *ctx, const kapsel_time &t, bool keep_adv = true) {
-  if (ctx->prec == KAPSEL_FP64) ns_slaved_euler_t<double>(ctx, t, keep_adv);
-  else ns_slaved_euler_t<float>(ctx, t, keep_adv);
+static void ns_slaved_euler(kapsel_ctx *ctx, const kapsel_time &t, bool keep_adv = true, int fix_dc = -1,
+                            void *const *keep3 = nullptr) {
+  if (ctx->prec == KAPSEL_FP64) ns_slaved_euler_t<double>(ctx, t, keep_adv, fix_dc, keep3);
+  else ns_slaved_euler_t<float>(ctx, t, keep_adv, fix_dc, keep3);
 }
 
 // Solenoidal_u (operate_omega.h:109-116)
@@ -403,15 +416,12 @@ static int fix_mask(const kapsel_ctx *ctx) {
 // (FP32 context: the velocity spectrum of Zeta_k2u is kept in WORK0..WORK2
 // and the step ends in k-space, as f_sole_u2zeta_k describes)
 static void spm_step(kapsel_ctx *ctx, kapsel_time &t) {
-  ns_slaved_euler(ctx, t, false);             // :27
   void *keep3[3] = {fld(ctx, KAPSEL_WORK0), fld(ctx, KAPSEL_WORK1), fld(ctx, KAPSEL_WORK2)};
   const bool fp32 = ctx->prec == KAPSEL_FP32;
-  zeta_k2u(ctx, fix_mask(ctx), fp32 ? keep3 : nullptr);  // :29-42
+  ns_slaved_euler(ctx, t, false, fix_mask(ctx), fp32 ? keep3 : nullptr);  // :27, 29-42
   md_position(ctx, t);                        // :44-50
   make_phi_particle_sum(ctx);                 // :51-53
-  calc_f_hydro(ctx, t);                       // :99-101
-  force(ctx);                                 // :104-110
-  md_velocity_integrate(ctx, t);
+  hydro_force_velocity(ctx, t);               // :99-110, Calc_f_hydro, Force, velocity
   thermostat(ctx, t);                         // :155-157, with kBT > 0
   make_u_particle_sum(ctx);                   // :166-167
   if (fp32) {
@@ -487,14 +497,12 @@ static void icep_step(kapsel_ctx *ctx, kapsel_time &t, const double e0[3], doubl
     KAPSEL_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
     {
       SideStream on_side(ctx);
-      ns_slaved_euler(ctx, t, false);
-      zeta_k2u(ctx, fix_mask(ctx));
+      ns_slaved_euler(ctx, t, false, fix_mask(ctx));
     }
     KAPSEL_HIP(hipEventRecord(ctx->ev_join, ctx->side));
     join.armed = true;
   } else {
-    ns_slaved_euler(ctx, t, false);
-    zeta_k2u(ctx, fix_mask(ctx));
+    ns_slaved_euler(ctx, t, false, fix_mask(ctx));
   }
   md_position(ctx, t);
   make_phi_particle_sum(ctx);
@@ -510,9 +518,7 @@ static void icep_step(kapsel_ctx *ctx, kapsel_time &t, const double e0[3], doubl
     add_force_to_u(ctx, t.dt_fluid * ctx->irho);
     solenoidal_u(ctx, ctx->fields + KAPSEL_U0);
   }
-  calc_f_hydro(ctx, t);
-  force(ctx);
-  md_velocity_integrate(ctx, t);
+  hydro_force_velocity(ctx, t);
   make_u_particle_sum(ctx);
   if (fp32) {
     f_sole_u2zeta_k(ctx, keep3);
@@ -539,9 +545,7 @@ static void electrolyte_step(kapsel_ctx *ctx, kapsel_time &t) {
   rescale_and_coulomb_force(ctx, t);          // :55-72
   add_force_to_u(ctx, t.dt_fluid * ctx->irho);  // :75-84
   solenoidal_u(ctx, ctx->fields + KAPSEL_U0);   // :85
-  calc_f_hydro(ctx, t);                       // :99-101
-  force(ctx);                                 // :104-110
-  md_velocity_integrate(ctx, t);
+  hydro_force_velocity(ctx, t);               // :99-110, Calc_f_hydro, Force, velocity
   make_u_particle_sum(ctx);                   // :166-167
   f_particle_dt_sole_add(ctx, electrolyte_u_keep(ctx));  // :168-169
   u2zeta_k(ctx);                              // :184
@@ -589,9 +593,7 @@ static void icep_electrolyte_step(kapsel_ctx *ctx, kapsel_time &t, double tol, i
     add_force_to_u(ctx, t.dt_fluid * ctx->irho);  // :75-84
     solenoidal_u(ctx, ctx->fields + KAPSEL_U0);   // :85
   }
-  calc_f_hydro(ctx, t);                       // :99-101
-  force(ctx);                                 // :104-110
-  md_velocity_integrate(ctx, t);
+  hydro_force_velocity(ctx, t);               // :99-110, Calc_f_hydro, Force, velocity
   make_u_particle_sum(ctx);                   // :166-167
   if (fp32) {
     f_sole_u2zeta_k(ctx, keep3);              // :168-169, 184

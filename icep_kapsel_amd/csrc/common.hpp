@@ -362,6 +362,10 @@ void advection_products(kapsel_ctx *ctx);
 // in5: the five product spectra (default U0..U2 and adv_w); the advection
 // term is stored over the last two unless store_a is 0
 void advection_combine_slaved(kapsel_ctx *ctx, double dt_fluid, void *const *in5 = nullptr, int store_a = 1);
+// advection_combine_slaved (the term not stored) then zeta_k2u_k after the
+// 2/3 cut with the FIX_CELL mask fix_mask, in one pass; out3: where the
+// velocity spectrum goes (default U0..U2)
+void slaved_zeta2u(kapsel_ctx *ctx, double dt_fluid, void *const *in5, int fix_mask, void *const *out3 = nullptr);
 // out3: the projected spectrum goes there and f3 is left unchanged
 void solenoidal_uk(kapsel_ctx *ctx, void *const *f3, void *const *out3 = nullptr);
 void make_f_nonsole(kapsel_ctx *ctx);
@@ -406,6 +410,9 @@ void pnet_warm_start(kapsel_ctx *ctx, double beta, const double e0[3], double sc
 void calc_f_hydro(kapsel_ctx *ctx, const kapsel_time &t);
 void force(kapsel_ctx *ctx);
 void md_velocity_integrate(kapsel_ctx *ctx, const kapsel_time &t);
+// calc_f_hydro, force and md_velocity_integrate in two launches (the gather's
+// slices, then one workgroup per particle for the rest)
+void hydro_force_velocity(kapsel_ctx *ctx, const kapsel_time &t);
 // Add_random_force_thermostat (fluct.cxx:9-40)
 void thermostat(kapsel_ctx *ctx, const kapsel_time &t);
 void thermostat_free(kapsel_ctx *ctx);

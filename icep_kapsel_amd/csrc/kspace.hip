@@ -175,6 +175,49 @@ __global__ void advection_combine_slaved_kernel(const T *__restrict__ u0, const 
   }
 }
 
+// The slaved step's combination and zeta update (advection_combine_slaved
+// kernel, EULER 0, the advection term not stored) followed by Zeta_k2u_k
+// after the 2/3 cut with the FIX_CELL mask (zeta_k2u_k kernel): one pass
+// over the modes.  A cut mode's zeta is zeroed as the cut zeroes it (its
+// products are not read); the velocity spectrum goes to o.
+template <typename T>
+__global__ void slaved_zeta2u_kernel(const T *__restrict__ u0, const T *__restrict__ u1, const T *__restrict__ u2,
+                                     const T *__restrict__ a0, const T *__restrict__ a1, T *__restrict__ z0,
+                                     T *__restrict__ z1, P5<T> o, double *__restrict__ dc, Grid g, double dmy0,
+                                     double dmy1, int fix_mask) {
+  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  KIdx k = kidx(g, c);
+  V2<T> za{0, 0}, zb{0, 0};
+  if (!is_cut(g, k)) {
+    V2<T> w[3];
+    advection_omega(AdvCoef<T>(g, k), ld2(u0 + 2 * c), ld2(u1 + 2 * c), ld2(u2 + 2 * c), ld2(a0 + 2 * c),
+                    ld2(a1 + 2 * c), w);
+    V2<T> f0, f1;
+    omega_pick(k, w, f0, f1);
+    za = ld2(z0 + 2 * c);
+    zb = ld2(z1 + 2 * c);
+    slaved_update(g, k, dmy0, dmy1, f0, f1, za, zb);
+  }
+  st2(z0 + 2 * c, za);
+  st2(z1 + 2 * c, zb);
+  V2<T> a, b, e;
+  zeta2u_mode(g, k, za, zb, a, b, e);
+  if (c == 0) {
+    double dd[3];
+    for (int d = 0; d < 3; d++) {
+      dd[d] = ((fix_mask >> d) & 1) ? 0.0 : dc[d];
+      if ((fix_mask >> d) & 1) dc[d] = 0.0;
+    }
+    a.re = (T)dd[0];
+    b.re = (T)dd[1];
+    e.re = (T)dd[2];
+  }
+  st2(o.p[0] + 2 * c, a);
+  st2(o.p[1] + 2 * c, b);
+  st2(o.p[2] + 2 * c, e);
+}
+
 // Solenoidal_uk (operate_omega.cxx:187-220), optionally after the 2/3 cut
 // (Solenoidal_u, operate_omega.h:109-116); in place when o = u (each point
 // is read before it is written)
@@ -327,6 +370,20 @@ static void advection_combine_slaved_t(kapsel_ctx *ctx, double dt_fluid, void *c
 }
 void advection_combine_slaved(kapsel_ctx *ctx, double dt_fluid, void *const *in5, int store_a) {
   KAPSEL_DISPATCH(ctx, advection_combine_slaved_t, ctx, dt_fluid, in5, store_a);
+}
+
+template <typename T>
+static void slaved_zeta2u_t(kapsel_ctx *ctx, double dt_fluid, void *const *in5, int fix_mask, void *const *out3) {
+  void *const *u = out3 ? out3 : ctx->fields + KAPSEL_U0;
+  LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, (5.0 * ctx->g.f + 7.0 * 2.0 * kept_count(ctx->g)) * sizeof(T));
+  hipLaunchKernelGGL(slaved_zeta2u_kernel<T>, dim3(blocks_for(ctx->g.nc, TPB)), dim3(TPB), 0, ctx->stream,
+                     (const T *)in5[0], (const T *)in5[1], (const T *)in5[2], (const T *)in5[3], (const T *)in5[4],
+                     F<T>(ctx, KAPSEL_ZETA0), F<T>(ctx, KAPSEL_ZETA1), p5<T>(u, 3), ctx->d_uk_dc, ctx->g,
+                     -ctx->nu * dt_fluid, 1. / ctx->nu, fix_mask);
+  KAPSEL_LAUNCH_CHECK();
+}
+void slaved_zeta2u(kapsel_ctx *ctx, double dt_fluid, void *const *in5, int fix_mask, void *const *out3) {
+  KAPSEL_DISPATCH(ctx, slaved_zeta2u_t, ctx, dt_fluid, in5, fix_mask, out3);
 }
 
 template <typename T>

@@ -597,6 +597,94 @@ __global__ void md_velocity_kernel(double *__restrict__ ps, Phys p, int euler, d
   }
 }
 
+// The end of Calc_f_hydro (the slices merged as hydro_gather_final_kernel
+// merges them), Force() (as force_kernel) and the velocity update with
+// reset_Forces (as md_velocity_kernel) of particle blockIdx.x in one
+// workgroup: each reads and writes the particle's own state only (the LJ
+// sum reads positions, which none of them writes), so the three launches
+// of a step are one, with the same arithmetic.
+__global__ __launch_bounds__(256) void hydro_force_velocity_kernel(double *__restrict__ ps,
+                                                                   const double *__restrict__ partial, int nslices,
+                                                                   Phys p, double dmy_f, LJ lj, int euler,
+                                                                   double dmy, double dmy_rot) {
+  const int i = blockIdx.x;
+  __shared__ double fh[6], fr[3];
+  __shared__ double red[3][4];
+  if (threadIdx.x < 6) {
+    const int c = threadIdx.x;
+    const double *src = partial + (size_t)12 * i * nslices + 2 * c;
+    CSum acc{src[0], src[1]};
+    for (int sl = 1; sl < nslices; sl++) acc = cmerge(acc, CSum{src[12 * sl], src[12 * sl + 1]});
+    const double v = acc.s + acc.c;
+    fh[c] = c < 3 || p.rotation ? dmy_f * v : pval(ps, KAPSEL_P_TORQUE_HYDRO, c - 3, i, p.np);
+  }
+  double fi[3] = {0, 0, 0};
+  if (lj.on) {
+    double xi[3];
+    int ci[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      xi[d] = pval(ps, KAPSEL_P_X, d, i, p.np);
+      ci[d] = int(xi[d] / lj.lc_r);
+    }
+    for (int j = threadIdx.x; j < p.np; j += blockDim.x) {
+      if (j == i) continue;
+      double xj[3];
+      int mult = 1;
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        xj[d] = pval(ps, KAPSEL_P_X, d, j, p.np);
+        mult *= nb_count(ci[d], int(xj[d] / lj.lc_r), lj.lc[d]);
+      }
+      if (!mult) continue;
+      double x12[3];
+      const double r = sqrt(min_image(xi, xj, p, x12));  // x12 = x_j - x_i
+      if (r < lj.cutoff) {
+        const double f = lj_f(r, lj);
+#pragma unroll
+        for (int d = 0; d < 3; d++) fi[d] += mult * (f * (-x12[d]));
+      }
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    for (int off = 32; off > 0; off >>= 1) fi[d] += __shfl_down(fi[d], off);
+    if ((threadIdx.x & 63) == 0) red[d][threadIdx.x >> 6] = fi[d];
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int d = threadIdx.x;
+    double s = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += red[d][w];
+    fr[d] = pval(ps, KAPSEL_P_FR, d, i, p.np) + (s + lj.body[d]);
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int d = threadIdx.x;
+    double &v = pref(ps, KAPSEL_P_V, d, i, p.np);
+    double &om = pref(ps, KAPSEL_P_OMEGA, d, i, p.np);
+    pref(ps, KAPSEL_P_V_OLD, d, i, p.np) = v;
+    pref(ps, KAPSEL_P_OMEGA_OLD, d, i, p.np) = om;
+    const double f_h = fh[d], f_r = fr[d], th = fh[3 + d];
+    const double tr = pval(ps, KAPSEL_P_TORQUE_R, d, i, p.np);
+    if (euler) {
+      v += dmy * (f_h + f_r);
+      om += dmy_rot * (th + tr);
+    } else {
+      v += dmy * (2.0 * f_h + f_r + pval(ps, KAPSEL_P_FR_PREVIOUS, d, i, p.np));
+      om += dmy_rot * (2.0 * th + tr + pval(ps, KAPSEL_P_TORQUE_R_PREVIOUS, d, i, p.np));
+    }
+    pref(ps, KAPSEL_P_FR_PREVIOUS, d, i, p.np) = f_r;
+    pref(ps, KAPSEL_P_FR, d, i, p.np) = 0.0;
+    pref(ps, KAPSEL_P_F_HYDRO_PREVIOUS, d, i, p.np) = f_h;
+    pref(ps, KAPSEL_P_F_HYDRO, d, i, p.np) = 0.0;
+    pref(ps, KAPSEL_P_TORQUE_R_PREVIOUS, d, i, p.np) = tr;
+    pref(ps, KAPSEL_P_TORQUE_R, d, i, p.np) = 0.0;
+    pref(ps, KAPSEL_P_TORQUE_HYDRO_PREVIOUS, d, i, p.np) = th;
+    pref(ps, KAPSEL_P_TORQUE_HYDRO, d, i, p.np) = 0.0;
+  }
+}
+
 Phys phys(kapsel_ctx *ctx) {
   Phys p;
   const Grid &g = ctx->g;
@@ -716,8 +804,8 @@ void calc_f_hydro(kapsel_ctx *ctx, const kapsel_time &t) {
   else gather_t<float>(ctx, dmy);
 }
 
-void force(kapsel_ctx *ctx) {
-  if (ctx->P.n_particles <= 0) return;
+namespace {
+LJ lj_of(kapsel_ctx *ctx) {
   LJ lj;
   lj.on = ctx->P.lj_truncate >= 0;
   lj.cutoff = ctx->a_r_cutoff * ctx->P.lj_sigma;
@@ -730,10 +818,49 @@ void force(kapsel_ctx *ctx) {
   lj.lc[2] = ctx->g.nz / cell_length;
   lj.lc_r = cell_length * ctx->g.dx;
   for (int d = 0; d < 3; d++) lj.body[d] = ctx->P.body_force[d];
+  return lj;
+}
+unsigned lj_threads(int np) { return np > 128 ? 256 : (np > 64 ? 128 : 64); }
+}  // namespace
+
+void force(kapsel_ctx *ctx) {
+  if (ctx->P.n_particles <= 0) return;
   LaunchTimer lt(ctx, KAPSEL_K_PARTICLE, 8.0 * KAPSEL_P_NFIELDS * 3 * ctx->P.n_particles);
-  const unsigned threads = ctx->P.n_particles > 128 ? 256 : (ctx->P.n_particles > 64 ? 128 : 64);
-  hipLaunchKernelGGL(force_kernel, dim3(ctx->P.n_particles), dim3(threads), 0, ctx->stream, ctx->d_ps, phys(ctx),
-                     lj);
+  hipLaunchKernelGGL(force_kernel, dim3(ctx->P.n_particles), dim3(lj_threads(ctx->P.n_particles)), 0, ctx->stream,
+                     ctx->d_ps, phys(ctx), lj_of(ctx));
+  KAPSEL_LAUNCH_CHECK();
+}
+
+void hydro_force_velocity(kapsel_ctx *ctx, const kapsel_time &t) {
+  const int np = ctx->P.n_particles;
+  if (np <= 0) return;
+  const Grid &g = ctx->g;
+  const Phys p = phys(ctx);
+  const int ns = gather_slices(np, ctx->np_domain);
+  const int chunk = (ctx->np_domain + ns - 1) / ns;
+  const size_t elem = ctx->prec == KAPSEL_FP64 ? sizeof(double) : sizeof(float);
+  {
+    LaunchTimer lt(ctx, KAPSEL_K_PARTICLE, 4.0 * np * ctx->np_domain * elem);
+    void *const *f = ctx->fields;
+    if (ctx->prec == KAPSEL_FP64)
+      hipLaunchKernelGGL(hydro_gather_part_kernel<double>, dim3((unsigned)(np * ns)), dim3(64), 0, ctx->stream,
+                         ctx->d_ps, ctx->d_sekibun, ctx->np_domain, ns, chunk, (const double *)f[KAPSEL_U0],
+                         (const double *)f[KAPSEL_U1], (const double *)f[KAPSEL_U2],
+                         (const double *)f[KAPSEL_PHI_SUM], g, p, ctx->d_gpart);
+    else
+      hipLaunchKernelGGL(hydro_gather_part_kernel<float>, dim3((unsigned)(np * ns)), dim3(64), 0, ctx->stream,
+                         ctx->d_ps, ctx->d_sekibun, ctx->np_domain, ns, chunk, (const float *)f[KAPSEL_U0],
+                         (const float *)f[KAPSEL_U1], (const float *)f[KAPSEL_U2],
+                         (const float *)f[KAPSEL_PHI_SUM], g, p, ctx->d_gpart);
+    KAPSEL_LAUNCH_CHECK();
+  }
+  const double dmy_f = -(g.dx * g.dx * g.dx) * ctx->P.rho / t.dt_fluid;  // md_force.cxx:342-343
+  const int euler = t.ts == 0;
+  const double dtm = euler ? t.dt_md : t.hdt_md;
+  LaunchTimer lt(ctx, KAPSEL_K_PARTICLE, 8.0 * KAPSEL_P_NFIELDS * 3 * np);
+  hipLaunchKernelGGL(hydro_force_velocity_kernel, dim3(np), dim3(lj_threads(np)), 0, ctx->stream, ctx->d_ps,
+                     (const double *)ctx->d_gpart, ns, p, dmy_f, lj_of(ctx), euler, dtm * ctx->imass,
+                     dtm * ctx->imoi);
   KAPSEL_LAUNCH_CHECK();
 }
 
