@@ -320,12 +320,15 @@ static void zeta_k2u(kapsel_ctx *ctx, int fix_dc, void *const *keep3 = nullptr) 
 // fix_dc >= 0: followed by the step's Zeta_k2u after the 2/3 cut with the
 // FIX_CELL mask fix_dc (sp_3d_ns.cxx:29-42), its Zeta_k2u_k in the same
 // pass as the zeta update (keep3 as zeta_k2u; the advection term is then
-// not stored).
+// not stored).  In an FP32 context right after a step that left U as the
+// spectrum zeta was formed from (u_spec_gen), the first Zeta_k2u transforms
+// U back instead of forming the spectrum from zeta again.
 template <typename T>
 static void ns_slaved_euler_t(kapsel_ctx *ctx, const kapsel_time &t, bool keep_adv, int fix_dc,
                               void *const *keep3) {
-  zeta_k2u(ctx, 0);
   void **u = ctx->fields + KAPSEL_U0;
+  if (ctx->prec == KAPSEL_FP32 && ctx->u_spec_gen == ctx->api_gen) fft_c2r(ctx, u, 3);
+  else zeta_k2u(ctx, 0);
   void *prod[5] = {ctx->adv_x[0], ctx->adv_x[1], ctx->adv_x[2], ctx->adv_w[0], ctx->adv_w[1]};
   void *five[5] = {u[0], u[1], u[2], ctx->adv_w[0], ctx->adv_w[1]};
   void *const *in5 = five;
@@ -403,6 +406,7 @@ static void f_sole_u2zeta_k(kapsel_ctx *ctx, void *const *keep3) {
     fft_r2c(ctx, f, 3);
   }
   sole_u2zeta_sum(ctx, keep3, f);
+  ctx->u_spec_gen = ctx->api_gen + 1;  // what the next entry point sees if it comes right after
 }
 
 static int fix_mask(const kapsel_ctx *ctx) {

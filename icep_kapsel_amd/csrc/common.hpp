@@ -245,6 +245,11 @@ struct kapsel_ctx {
   // if nothing runs in between; that successor then reuses what the step
   // left: the charge spectrum, the real ion fields and the real velocity.
   unsigned long long api_gen = 0, el_reuse_gen = ~0ull;
+  // An FP32 step that ends with U holding the cut, solenoidal velocity
+  // spectrum whose curl it stored as zeta sets u_spec_gen the same way; a
+  // slaved step that comes right after takes that spectrum for Zeta_k2u_k
+  // of zeta (equal but for rounding) instead of forming it again.
+  unsigned long long u_spec_gen = ~0ull;
   double *el_partial = nullptr;   // block partial sums of the electrolyte reductions
   double *el_scal = nullptr;      // device scalars (electrolyte.hip)
   // when set, the Poisson right-hand side is built from rho - rho_bg[0]
