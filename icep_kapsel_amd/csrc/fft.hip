@@ -432,7 +432,10 @@ __global__ KAPSEL_PLANE_BOUNDS void fft_plane_r2c(FieldPtrs<T> fp, const V2<T> *
 // pass before a forward transform (f = u_p - phi u of
 // Make_f_particle_dt_nonsole, u + f dt/rho of the dielectric force, the
 // products of U2advection_k) without its own read and write of the fields.
-// out_f must not be an input of another field of the launch.
+// out_f must not be an input of another field of the launch.  The fields of
+// one plane run at consecutive ids (xcd_tile_by_tile), so the planes they
+// read in common (phi; the velocity components of the products) come from
+// HBM once.
 template <typename T, int NY, int NZ, int THREADS>
 __global__ KAPSEL_PLANE_BOUNDS void fft_plane_r2c_pro(FieldPtrs<T> out, FieldPtrs<T> a, FieldPtrs<T> b,
                                                                FieldPtrs<T> c, T s, unsigned dsq,
@@ -443,7 +446,7 @@ __global__ KAPSEL_PLANE_BOUNDS void fft_plane_r2c_pro(FieldPtrs<T> out, FieldPtr
   __shared__ __align__(16) V2<T> P[C::LPLANE];
   __shared__ PlaneTw<T, NY, NZ> tw;
   unsigned field;
-  const unsigned plane = xcd_tile(field);
+  const unsigned plane = xcd_tile_by_tile(field);
   const size_t base = (size_t)plane * 2 * C::PLANE;
   T *__restrict__ gp = out.p[field] + base;
   const T *ga = a.p[field] ? a.p[field] + base : nullptr, *gb = b.p[field] + base;

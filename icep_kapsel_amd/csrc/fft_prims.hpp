@@ -107,6 +107,21 @@ __device__ __forceinline__ unsigned xcd_tile(unsigned &field) {
   return id % gridDim.x;
 }
 
+// xcd_tile with the other order of the ids: tile-major, the fields of one
+// tile at consecutive ids, so that they run on one XCD close together and
+// what they read in common (the inputs of a prologue) is fetched once, by
+// its L2.  Returns the tile; field = id % gridDim.y.
+__device__ __forceinline__ unsigned xcd_tile_by_tile(unsigned &field) {
+  constexpr unsigned NXCD = 8;
+  const unsigned total = gridDim.x * gridDim.y;
+  const unsigned lin = blockIdx.y * gridDim.x + blockIdx.x;
+  const unsigned q = total / NXCD, r = total % NXCD;
+  const unsigned x = lin % NXCD, k = lin / NXCD;
+  const unsigned id = x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+  field = id % gridDim.y;
+  return id / gridDim.y;
+}
+
 // twiddle W_L^m, table holds the forward sign
 template <int DIR, typename T>
 __device__ __forceinline__ V2<T> tw_get(const V2<T> *__restrict__ tw, int m) {

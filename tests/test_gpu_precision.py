@@ -72,6 +72,38 @@ def test_fp32_per_field_error_c2():
         assert max(e[f] for f in FIELDS if f not in SCRATCH) < 1e-4
 
 
+def test_fp32_step_spectrum_reuse_matches_recompute():
+    """An FP32 slaved step right after a step starts from the velocity
+    spectrum that step left in U; with an entry point in between it forms the
+    spectrum from zeta again (Zeta_k2u_k).  The two are equal but for FP32
+    rounding: 10 steps of c1 both ways agree to 1e-5 (relative, velocities
+    and vorticity) and each stays as close to the FP64 run as the other."""
+    print("case fp32 spectrum reuse c1", flush=True)
+    from icep_kapsel_amd.gpu import KapselGPU
+    p32, st, dt = workloads.spm_config("c1", precision=abi.KAPSEL_FP32)
+    p64, _, _ = workloads.spm_config("c1", precision=abi.KAPSEL_FP64)
+    ks = [KapselGPU(p) for p in (p32, p32, p64)]
+    for k in ks:
+        k.set_particles(st)
+        k.init_rigidity()
+    ts = [abi.make_time(dt) for _ in ks]
+    for _ in range(10):
+        for i, (k, t) in enumerate(zip(ks, ts)):
+            if i == 1:
+                k.set_concurrency(True)  # an entry point between the steps: no reuse
+            k.spm_step(t)
+    reuse, again, k64 = ks
+    e = _errors(again, reuse)
+    print("reuse vs recompute:", {f: f"{v:.1e}" for f, v in e.items() if f not in SCRATCH}, flush=True)
+    assert e["V"] < 1e-5 and e["OMEGA"] < 1e-5
+    assert max(e["U0"], e["U1"], e["U2"], e["ZETA0"], e["ZETA1"]) < 1e-5
+    er, ea = _errors(k64, reuse), _errors(k64, again)
+    for f in ("V", "OMEGA", "ZETA0", "ZETA1"):
+        assert er[f] < 2 * ea[f] + 1e-7 and ea[f] < 2 * er[f] + 1e-7, f
+    for k in ks:
+        k.close()
+
+
 def _icep_run(prec, precond, tol, steps=5):
     from icep_kapsel_amd.gpu import KapselGPU
     p, st, e0, dt = workloads.icep_config(precision=prec)
