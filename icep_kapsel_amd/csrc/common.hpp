@@ -317,7 +317,7 @@ inline int gather_slices(int np, int np_domain) {
 // FFT (fft.hip)
 void init_twiddles(kapsel_ctx *ctx);
 void free_twiddles(kapsel_ctx *ctx);
-// nf fields transformed together (1..5), of the context precision or of T;
+// nf fields transformed together (1..6), of the context precision or of T;
 // c2r with src: the spectra in src (left unchanged) go to fields
 void fft_r2c(kapsel_ctx *ctx, void *const *fields, int nf);
 void fft_c2r(kapsel_ctx *ctx, void *const *fields, int nf, void *const *src = nullptr);
@@ -346,6 +346,10 @@ void fft_face_mid_t(kapsel_ctx *ctx, void *const *g3, void *const *face3);
 // of z, p untouched; false: NX has no fused pass
 template <typename T>
 bool fft_x_grad_t(kapsel_ctx *ctx, const T *z, T *p, const double *sc, int beta_slot, void *const *g3);
+// the staggered gradients of two FP64 half spectra a, b as six real FP32
+// fields (a's three, then b's); false where NX has no fused x pass (nothing
+// queued)
+bool fft_grad2_f64_to_f32(kapsel_ctx *ctx, const double *a, const double *b, void *const *out6);
 // the operator's last pass: ap = staggered divergence of the x-forward of
 // g_d; one Parseval-weighted <p, ap> partial per workgroup; returns the
 // number of partials (0: NX has no fused pass)

@@ -17,7 +17,7 @@
 // R2-point DFTs finish in registers and store straight to HBM.  Adjacent
 // lines are adjacent in memory (lines of a tile differ in the fastest
 // index), so each load/store instruction of a wavefront touches few 64-B
-// segments.  Up to five fields are transformed by one launch (grid.y).
+// segments.  Up to six fields are transformed by one launch (grid.y).
 #include "fft_prims.hpp"
 
 namespace kapsel {
@@ -722,6 +722,54 @@ __global__ __launch_bounds__((LPB * LineFFT<T, L>::S)) void fft_x_grad(
   }
 }
 
+// The staggered gradients of two FP64 half spectra a, b (grid.y: 0 = a,
+// 1 = b) as six FP32 fields after their inverse x transform:
+// out[3f + d] = IFFT_x(i k_d e^{i k_d dx/2} in_f).  The symbols are formed in
+// FP64 and rounded once to FP32 (the values grad_shift2_f64_kernel stores),
+// the transforms run in FP32.
+template <int L, int LPB>
+__global__ __launch_bounds__((LPB * LineFFT<float, L>::S)) void fft_x_grad2_f64(
+    const double *__restrict__ a, const double *__restrict__ b, FieldPtrs<float> out, long long Q, long long plane,
+    int h, int ny, double wx, double wy, double wz, const V2<double> *__restrict__ shx,
+    const V2<double> *__restrict__ shy, const V2<double> *__restrict__ shz, const V2<float> *__restrict__ tw) {
+  using F = LineFFT<float, L>;
+  constexpr int P = F::P;
+  __shared__ V2<float> sm[LPB * F::LS];
+  unsigned field;
+  const unsigned tile = xcd_tile(field);
+  const double *in = field == 0 ? a : b;
+  const int t = threadIdx.x;
+  const int l = t % LPB, s = t / LPB;
+  const long long q = (long long)tile * LPB + l;
+  const bool valid = q < Q;
+  const long long base = valid ? q : 0;
+  const int j = (int)(base / h), kz = (int)(base % h);
+  V2<double> xv[P];
+#pragma unroll
+  for (int n = 0; n < P; n++) xv[n] = ld2(in + 2 * (base + (long long)F::in_i(s, n) * plane));
+  V2<float> *sl = sm + l * F::LS;
+  V2<float> twr[P];
+  line_twiddles<F>(twr, tw, s);
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    V2<float> v[P];
+#pragma unroll
+    for (int n = 0; n < P; n++) {
+      const int i = F::in_i(s, n);
+      const double w = d == 0 ? signed_k(i, L) * wx : d == 1 ? signed_k(j, ny) * wy : kz * wz;
+      const V2<double> sft = d == 0 ? shx[i] : d == 1 ? shy[j] : shz[kz];
+      const V2<double> gv = cmul(V2<double>{-w * xv[n].im, w * xv[n].re}, sft);
+      v[n] = V2<float>{(float)gv.re, (float)gv.im};
+    }
+    F::template run<+1>(v, sl, twr, s, LPB);
+    if (valid) {
+      float *__restrict__ gd = out.p[3 * field + d];
+#pragma unroll
+      for (int m = 0; m < P; m++) st2(gd + 2 * (base + (long long)F::out_i(s, m) * plane), v[m]);
+    }
+  }
+}
+
 // w_d = FFT_x(g_d), ap = i sum_d k_d e^{-i k_d dx/2} w_d (stored);
 // partial[tile] = sum over the tile of w(kz) Re(conj(p) ap), w = 1 on the
 // kz = 0 and NZ/2 planes and 2 elsewhere (Parseval, without N^-3)
@@ -1154,6 +1202,20 @@ bool fft_x_grad_t(kapsel_ctx *ctx, const T *z, T *p, const double *sc, int beta_
   }
 }
 
+template <int L>
+void launch_x_grad2_f64(kapsel_ctx *ctx, const double *a, const double *b, void *const *out6) {
+  constexpr int LPB = lines_per_block(sizeof(float));
+  const Grid &g = ctx->g;
+  const Twiddles &t64 = ctx->tw[prec_of<double>()];
+  const long long Q = (long long)g.ny * g.h;
+  LaunchTimer lt(ctx, KAPSEL_K_FFT_X_GRAD, 2.0 * g.f * sizeof(double) + 6.0 * g.f * sizeof(float));
+  hipLaunchKernelGGL((fft_x_grad2_f64<L, LPB>), dim3((unsigned)((Q + LPB - 1) / LPB), 2),
+                     dim3(LPB * LineFFT<float, L>::S), 0, ctx->stream, a, b, pack<float>(out6, 6), Q, Q, g.h, g.ny,
+                     g.wave_x, g.wave_y, g.wave_z, (const V2<double> *)t64.shx, (const V2<double> *)t64.shy,
+                     (const V2<double> *)t64.shz, (const V2<float> *)ctx->tw[prec_of<float>()].x);
+  KAPSEL_LAUNCH_CHECK();
+}
+
 template <typename T>
 int fft_x_div_t(kapsel_ctx *ctx, void *const *g3, const T *p, T *ap, double *partial) {
   switch (ctx->g.nx) {
@@ -1170,6 +1232,26 @@ int fft_x_res_t(kapsel_ctx *ctx, void *const *g3, const T *b, T *r, double *part
     case 128: return launch_x_div<T, 128, true>(ctx, g3, b, r, partial);
     default: return 0;
   }
+}
+
+// the staggered gradients of the FP64 spectra a and b as six real FP32
+// fields (out6: a's three, then b's): the x pass fused with the gradient
+// symbols where NX has one (64, 128), then the y and z passes of all six in
+// one launch; false: nothing queued
+bool fft_grad2_f64_to_f32(kapsel_ctx *ctx, const double *a, const double *b, void *const *out6) {
+  const Grid &g = ctx->g;
+  switch (g.nx) {
+    case 64: launch_x_grad2_f64<64>(ctx, a, b, out6); break;
+    case 128: launch_x_grad2_f64<128>(ctx, a, b, out6); break;
+    default: return false;
+  }
+  FieldPtrs<float> fp = pack<float>(out6, 6);
+  if (!plane_dispatch<float>(ctx, fp, 6, false)) {
+    lines_dispatch<float, +1>(ctx, KAPSEL_K_FFT_LINES_INV, g.ny, fp, 6, (long long)g.nx * g.h, g.h,
+                              (long long)g.ny * g.h, g.h, ctx->tw[prec_of<float>()].y);
+    z_dispatch<float>(ctx, fp, 6, false);
+  }
+  return true;
 }
 
 template void fft_face_mid_t<double>(kapsel_ctx *, void *const *, void *const *);

@@ -88,7 +88,7 @@ template <> struct FS<256> { static constexpr int R1 = 16, R2 = 16; };
 
 template <typename T>
 struct FieldPtrs {
-  T *p[5];
+  T *p[6];
 };
 
 // XCD-aware tile order.  Workgroups are dispatched round-robin over the 8

@@ -1314,7 +1314,8 @@ void maxwell_t(kapsel_ctx *ctx, const double e0[3], bool from_spectra) {
   T *gp[3] = {fld<T>(ctx, KAPSEL_WORK1), fld<T>(ctx, KAPSEL_WORK2), fld<T>(ctx, KAPSEL_WORK3)};
   T *e2 = fld<T>(ctx, KAPSEL_WORK0);
   if constexpr (sizeof(T) == 4) {
-    if (from_spectra) {
+    void *six[6] = {ge[0], ge[1], ge[2], gp[0], gp[1], gp[2]};
+    if (from_spectra && !fft_grad2_f64_to_f32(ctx, ctx->p64[0], ctx->p64[1], six)) {
       {
         LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, 2.0 * g.f * sizeof(double) + 6.0 * g.f * sizeof(float));
         hipLaunchKernelGGL(grad_shift2_f64_kernel, dim3(blocks_for(g.nc, TPB)), dim3(TPB), 0, ctx->stream,
@@ -1322,8 +1323,7 @@ void maxwell_t(kapsel_ctx *ctx, const double e0[3], bool from_spectra) {
                            sh<double>(ctx, 1), sh<double>(ctx, 2), g);
         KAPSEL_LAUNCH_CHECK();
       }
-      fft_c2r_t<T>(ctx, (void *const *)ge, 3);
-      fft_c2r_t<T>(ctx, (void *const *)gp, 3);
+      fft_c2r_t<T>(ctx, six, 6);
     }
   }
   if (!from_spectra || sizeof(T) == 8) {
