@@ -320,12 +320,18 @@ static void zeta_k2u(kapsel_ctx *ctx, int fix_dc, void *const *keep3 = nullptr) 
 // fix_dc >= 0: followed by the step's Zeta_k2u after the 2/3 cut with the
 // FIX_CELL mask fix_dc (sp_3d_ns.cxx:29-42), its Zeta_k2u_k in the same
 // pass as the zeta update (keep3 as zeta_k2u; the advection term is then
-// not stored).  In an FP32 context right after a step that left U as the
+// not stored; spectrum_only: keep3 receives the spectrum and U is not
+// transformed).  In an FP32 context right after a step that left U as the
 // spectrum zeta was formed from (u_spec_gen), the first Zeta_k2u transforms
 // U back instead of forming the spectrum from zeta again.
 template <typename T>
+static void ensure_adv_x(kapsel_ctx *ctx) {
+  for (int i = 0; i < 3; i++)
+    if (!ctx->adv_x[i]) KAPSEL_HIP(hipMalloc(&ctx->adv_x[i], ctx->g.f * sizeof(T)));
+}
+template <typename T>
 static void ns_slaved_euler_t(kapsel_ctx *ctx, const kapsel_time &t, bool keep_adv, int fix_dc,
-                              void *const *keep3) {
+                              void *const *keep3, bool spectrum_only) {
   void **u = ctx->fields + KAPSEL_U0;
   if (ctx->prec == KAPSEL_FP32 && ctx->u_spec_gen == ctx->api_gen) fft_c2r(ctx, u, 3);
   else zeta_k2u(ctx, 0);
@@ -333,8 +339,7 @@ static void ns_slaved_euler_t(kapsel_ctx *ctx, const kapsel_time &t, bool keep_a
   void *five[5] = {u[0], u[1], u[2], ctx->adv_w[0], ctx->adv_w[1]};
   void *const *in5 = five;
   if (plane_ok_grid(ctx->g)) {
-    for (int i = 0; i < 3; i++)
-      if (!ctx->adv_x[i]) KAPSEL_HIP(hipMalloc(&ctx->adv_x[i], ctx->g.f * sizeof(T)));
+    ensure_adv_x<T>(ctx);
     for (int i = 0; i < 3; i++) prod[i] = ctx->adv_x[i];
     void *b[5] = {u[1], u[2], u[0], u[1], u[2]}, *c[5] = {u[2], u[0], u[1], u[2], u[0]};
     (void)fft_r2c_pro_t<T>(ctx, prod, nullptr, b, c, 1.0, 5, 0x18u);
@@ -345,15 +350,15 @@ static void ns_slaved_euler_t(kapsel_ctx *ctx, const kapsel_time &t, bool keep_a
   }
   if (fix_dc >= 0) {
     slaved_zeta2u(ctx, t.dt_fluid, in5, fix_dc, keep3);
-    fft_c2r(ctx, u, 3, keep3);
+    if (!spectrum_only) fft_c2r(ctx, u, 3, keep3);
   } else {
     advection_combine_slaved(ctx, t.dt_fluid, in5, keep_adv);
   }
 }
 static void ns_slaved_euler(kapsel_ctx *ctx, const kapsel_time &t, bool keep_adv = true, int fix_dc = -1,
-                            void *const *keep3 = nullptr) {
-  if (ctx->prec == KAPSEL_FP64) ns_slaved_euler_t<double>(ctx, t, keep_adv, fix_dc, keep3);
-  else ns_slaved_euler_t<float>(ctx, t, keep_adv, fix_dc, keep3);
+                            void *const *keep3 = nullptr, bool spectrum_only = false) {
+  if (ctx->prec == KAPSEL_FP64) ns_slaved_euler_t<double>(ctx, t, keep_adv, fix_dc, keep3, spectrum_only);
+  else ns_slaved_euler_t<float>(ctx, t, keep_adv, fix_dc, keep3, spectrum_only);
 }
 
 // Solenoidal_u (operate_omega.h:109-116)
@@ -394,6 +399,17 @@ static void solenoidal_u_keep(kapsel_ctx *ctx, void *const *keep3, double add_s 
   }
   solenoidal_uk(ctx, u, keep3);
   fft_c2r(ctx, u, 3, keep3);
+}
+// The same end of the step from the velocity spectrum ns_keep the
+// Navier-Stokes part left (cut and solenoidal, so that P(u + f s) = u + P f s):
+// keep3 = ns_keep + s P f_k from the forward transform of the force alone,
+// then U = c2r(keep3) -- the velocity is not transformed back after the
+// Navier-Stokes part and forward again with the force
+static void force_sole_keep(kapsel_ctx *ctx, void *const *ns_keep, void *const *keep3, double s) {
+  void **f = ctx->fields + KAPSEL_F0;
+  fft_r2c(ctx, f, 3);
+  sole_add(ctx, ns_keep, f, keep3, s);
+  fft_c2r(ctx, ctx->fields + KAPSEL_U0, 3, keep3);
 }
 // f = u_p - phi u (Make_f_particle_dt_nonsole) formed inside the forward
 // transform of f where a plane pass applies; the projection of f, the sum
@@ -493,7 +509,12 @@ static void icep_step(kapsel_ctx *ctx, kapsel_time &t, const double e0[3], doubl
   // the NS chain runs beside the start of the Poisson part (profile,
   // permittivity, the refinement's set-up and first FP64 residual, whose FP64
   // plane pass runs a second round on half the CUs) and is done before the
-  // FP32 CG starts
+  // FP32 CG starts.  FP32 with a plane pass: it ends with its velocity
+  // spectrum in adv_x, and the step's end adds the projected force to that
+  const bool fp32 = ctx->prec == KAPSEL_FP32;
+  const bool spec = fp32 && plane_ok_grid(ctx->g);
+  if (spec) ensure_adv_x<float>(ctx);
+  void *ns_keep[3] = {ctx->adv_x[0], ctx->adv_x[1], ctx->adv_x[2]};
   if (split) {
     for (void *&w : ctx->ns_work)
       if (!w) KAPSEL_HIP(hipMalloc(&w, ctx->g.f * ctx->elem));
@@ -501,12 +522,12 @@ static void icep_step(kapsel_ctx *ctx, kapsel_time &t, const double e0[3], doubl
     KAPSEL_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
     {
       SideStream on_side(ctx);
-      ns_slaved_euler(ctx, t, false, fix_mask(ctx));
+      ns_slaved_euler(ctx, t, false, fix_mask(ctx), spec ? ns_keep : nullptr, spec);
     }
     KAPSEL_HIP(hipEventRecord(ctx->ev_join, ctx->side));
     join.armed = true;
   } else {
-    ns_slaved_euler(ctx, t, false, fix_mask(ctx));
+    ns_slaved_euler(ctx, t, false, fix_mask(ctx), spec ? ns_keep : nullptr, spec);
   }
   md_position(ctx, t);
   make_phi_particle_sum(ctx);
@@ -515,8 +536,9 @@ static void icep_step(kapsel_ctx *ctx, kapsel_time &t, const double e0[3], doubl
   maxwell_force(ctx, e0, true);
   join.now();
   void *keep3[3] = {ctx->pscratch[5], ctx->pscratch[6], ctx->pscratch[7]};
-  const bool fp32 = ctx->prec == KAPSEL_FP32;
-  if (fp32) {
+  if (spec) {
+    force_sole_keep(ctx, ns_keep, keep3, t.dt_fluid * ctx->irho);
+  } else if (fp32) {
     solenoidal_u_keep(ctx, keep3, t.dt_fluid * ctx->irho);  // u += f dt / rho inside
   } else {
     add_force_to_u(ctx, t.dt_fluid * ctx->irho);

@@ -367,6 +367,8 @@ void u_k2zeta_k(kapsel_ctx *ctx);
 // the 2/3 cut and Solenoidal_uk of the spectrum f3, then U_k2zeta_k of
 // a3 + P f3, which is stored in U0..U2 (a3 and f3 unchanged)
 void sole_u2zeta_sum(kapsel_ctx *ctx, void *const *a3, void *const *f3);
+// o3 = a3 + s P f3 (the projection after the 2/3 cut), a3 and f3 unchanged
+void sole_add(kapsel_ctx *ctx, void *const *a3, void *const *f3, void *const *o3, double s);
 void advection_products(kapsel_ctx *ctx);
 // in5: the five product spectra (default U0..U2 and adv_w); the advection
 // term is stored over the last two unless store_a is 0

@@ -218,6 +218,25 @@ __global__ void slaved_zeta2u_kernel(const T *__restrict__ u0, const T *__restri
   st2(o.p[2] + 2 * c, e);
 }
 
+// o = a + s P f: the spectrum a plus s times the projection of f after the
+// 2/3 cut (Solenoidal_uk with the cut), a and f left unchanged
+template <typename T>
+__global__ void sole_add_kernel(P5<T> a3, P5<T> f3, P5<T> o3, T s, Grid g) {
+  size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.nc) return;
+  KIdx k = kidx(g, c);
+  V2<T> fa{0, 0}, fb{0, 0}, fe{0, 0};
+  if (!is_cut(g, k)) {
+    fa = ld2(f3.p[0] + 2 * c);
+    fb = ld2(f3.p[1] + 2 * c);
+    fe = ld2(f3.p[2] + 2 * c);
+    solenoidal_mode(g, k, fa, fb, fe);
+  }
+  st2(o3.p[0] + 2 * c, ld2(a3.p[0] + 2 * c) + scal(s, fa));
+  st2(o3.p[1] + 2 * c, ld2(a3.p[1] + 2 * c) + scal(s, fb));
+  st2(o3.p[2] + 2 * c, ld2(a3.p[2] + 2 * c) + scal(s, fe));
+}
+
 // Solenoidal_uk (operate_omega.cxx:187-220), optionally after the 2/3 cut
 // (Solenoidal_u, operate_omega.h:109-116); in place when o = u (each point
 // is read before it is written)
@@ -343,6 +362,17 @@ static void sole_u2zeta_sum_t(kapsel_ctx *ctx, void *const *a3, void *const *f3)
 }
 void sole_u2zeta_sum(kapsel_ctx *ctx, void *const *a3, void *const *f3) {
   KAPSEL_DISPATCH(ctx, sole_u2zeta_sum_t, ctx, a3, f3);
+}
+
+template <typename T>
+static void sole_add_t(kapsel_ctx *ctx, void *const *a3, void *const *f3, void *const *o3, double s) {
+  LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, (6.0 * ctx->g.f + 6.0 * kept_count(ctx->g)) * sizeof(T));
+  hipLaunchKernelGGL(sole_add_kernel<T>, dim3(blocks_for(ctx->g.nc, TPB)), dim3(TPB), 0, ctx->stream,
+                     p5<T>(a3, 3), p5<T>(f3, 3), p5<T>(o3, 3), (T)s, ctx->g);
+  KAPSEL_LAUNCH_CHECK();
+}
+void sole_add(kapsel_ctx *ctx, void *const *a3, void *const *f3, void *const *o3, double s) {
+  KAPSEL_DISPATCH(ctx, sole_add_t, ctx, a3, f3, o3, s);
 }
 
 template <typename T>
