@@ -532,8 +532,19 @@ static void icep_step(kapsel_ctx *ctx, kapsel_time &t, const double e0[3], doubl
   md_position(ctx, t);
   make_phi_particle_sum(ctx);
   dielectric_field(ctx);
-  poisson_solve(ctx, e0, tol, maxit, precond, iters, relres);
-  maxwell_force(ctx, e0, true);
+  {
+    // FP32: the Maxwell force is queued with each potential the solve is
+    // about to check, so the GPU computes it while the host waits for the
+    // check (the solve says whether it was queued for its final potential)
+    const std::function<void()> mx = [&]() { maxwell_force_from_spectra(ctx, e0); };
+    struct Hook {
+      kapsel_ctx *c;
+      ~Hook() { c->solve_hook = nullptr; }
+    } hook{ctx};
+    if (fp32) ctx->solve_hook = &mx;
+    poisson_solve(ctx, e0, tol, maxit, precond, iters, relres);
+  }
+  if (!(fp32 && ctx->solve_hook_done)) maxwell_force(ctx, e0, true);
   join.now();
   void *keep3[3] = {ctx->pscratch[5], ctx->pscratch[6], ctx->pscratch[7]};
   if (spec) {

@@ -179,6 +179,11 @@ struct kapsel_ctx {
                                   // inside the forward transform (allocated on first use)
   bool lazy_outputs = false;      // an FP32 solve leaves POTENTIAL/RHS to materialize_outputs
   bool outputs_stale = false;     // POTENTIAL/RHS wait for materialize_outputs
+  // queued by an FP32 solve with each potential it is about to check (while
+  // the host waits for the check); solve_hook_done: queued for the potential
+  // the solve returned
+  const std::function<void()> *solve_hook = nullptr;
+  bool solve_hook_done = false;
   int device = 0;
   void *fields[KAPSEL_NFIELDS] = {};
   double *d_uk_dc = nullptr;      // 3 doubles (device)
@@ -407,6 +412,9 @@ double galerkin_scale_potential(kapsel_ctx *ctx, const double e0[3]);
 // after_solve: called right after poisson_solve, whose FP64 spectra of eps
 // and the potential may then stand in for the transforms of the fields
 void maxwell_force(kapsel_ctx *ctx, const double e0[3], bool after_solve = false);
+// the Maxwell force of an FP32 context from the FP64 spectra its solve holds
+// (the after_solve form), also while the solve runs
+void maxwell_force_from_spectra(kapsel_ctx *ctx, const double e0[3]);
 void add_force_to_u(kapsel_ctx *ctx, double s);
 
 // learned warm start (pnet.hip)

@@ -1203,13 +1203,26 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
   // x -= d, then the new residual: queued by the correction itself behind
   // the iteration its trend predicts to be the last (the next residual read
   // tells whether it was)
+  bool hook_current = false;  // the solve hook is queued for the current xk
+  ctx->solve_hook_done = false;
   const std::function<void()> apply_and_residual = [&]() {
     const bool was = ctx->refine64;
     ctx->refine64 = true;
     hipLaunchKernelGGL(sub_all_kernel, dim3(fb), dim3(TPB), 0, ctx->stream, xk, d32, g.f);
     KAPSEL_LAUNCH_CHECK();
+    hook_current = false;
     residual();
     ctx->refine64 = was;
+  };
+  // the caller's work on the potential about to be checked (it only writes
+  // what a further correction rewrites before reading)
+  auto hook = [&]() {
+    if (!ctx->solve_hook) return;
+    const bool was = ctx->refine64;
+    ctx->refine64 = false;
+    (*ctx->solve_hook)();
+    ctx->refine64 = was;
+    hook_current = true;
   };
   // While the host waits for a residual, the GPU runs what most likely
   // follows: after the first residual the correction's start and first
@@ -1231,6 +1244,7 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
       ctx->refine64 = true;
     } else {
       outputs();
+      hook();
     }
     wait_seq(ctx, 5, res_seq);
     const volatile double *hs = ctx->h_slots;
@@ -1277,6 +1291,7 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
   ctx->refine64 = false;
   ctx->pot64_valid = true;
   ctx->spec64_fresh = true;
+  ctx->solve_hook_done = hook_current;
   *relres = rel;
   return total;
 }
@@ -1354,6 +1369,11 @@ void maxwell_force(kapsel_ctx *ctx, const double e0[3], bool after_solve) {
   ensure_storage(ctx);
   if (ctx->prec == KAPSEL_FP64) maxwell_t<double>(ctx, e0, false);
   else maxwell_t<float>(ctx, e0, after_solve && ctx->spec64_fresh);
+}
+
+void maxwell_force_from_spectra(kapsel_ctx *ctx, const double e0[3]) {
+  ensure_storage(ctx);
+  maxwell_t<float>(ctx, e0, true);
 }
 
 void add_force_to_u(kapsel_ctx *ctx, double s) {
