@@ -775,10 +775,11 @@ struct ConvergenceWatch {
   unsigned long long seq[2] = {0, 0};
   ConvergenceWatch(kapsel_ctx *c, double s, double rr0, bool dev = false)
       : ctx(c), scale(s), rr(rr0), device_slots(dev) {}
-  void queued() {
+  // back: how many iterations were queued after this one (0: it is the last)
+  void queued(unsigned long long back = 0) {
     it++;
     if (device_slots) {
-      seq[it & 1] = ctx->seq_cg;  // the iteration just queued
+      seq[it & 1] = ctx->seq_cg - back;  // the iteration's sequence number
       return;
     }
     KAPSEL_HIP(hipMemcpyAsync(ctx->h_slots + (it & 1), ctx->d_scal + S_RR, sizeof(double),
@@ -811,8 +812,12 @@ struct ConvergenceWatch {
         enqueue();
         queued();
       } else {
+        // before a trend exists, up to two iterations in flight while the
+        // known residual is far above the bound (one iteration does not take
+        // it 100x down), so the GPU does not wait for the host
         const bool trend = rr_prev > 0.0;
-        const bool ahead = it < maxit && trend && rr * (rr / rr_prev) >= tol2;
+        const bool ahead =
+            it < maxit && (trend ? rr * (rr / rr_prev) >= tol2 : (it - known < 2 && rr >= 1e4 * tol2));
         if (ahead) {
           enqueue();
           queued();
@@ -931,19 +936,19 @@ void kcg_init_t(kapsel_ctx *ctx, T *x, const T *b, int precond, T *const *w) {
   KAPSEL_LAUNCH_CHECK();
 }
 
-// Queues the start of a correction and its first iteration before the
-// caller has read whether the correction is needed (kcg_t with begun then
-// continues it; if it is not needed, it only wrote the correction's vectors)
+// Queues the start of a correction and its first n iterations before the
+// caller has read whether the correction is needed (kcg_t with begun = n then
+// continues it; if it is not needed, they only wrote the correction's vectors)
 template <typename T>
 void kcg_begin(kapsel_ctx *ctx, T *x, const T *b, const T *eps, int precond, T *const *w, T *const *s,
-               T *const *face) {
+               T *const *face, int n) {
   kcg_init_t<T>(ctx, x, b, precond, w);
-  kcg_enqueue<T>(ctx, x, eps, precond, w, s, face);
+  for (int i = 0; i < n; i++) kcg_enqueue<T>(ctx, x, eps, precond, w, s, face);
 }
 
 template <typename T>
 int kcg_t(kapsel_ctx *ctx, T *x, const T *b, const T *eps, double tol_abs, int maxit, int precond, T *const *w,
-          T *const *s, T *const *face, double *rr_out, double rr_known = -1.0, bool begun = false,
+          T *const *s, T *const *face, double *rr_out, double rr_known = -1.0, int begun = 0,
           const std::function<void()> *tail = nullptr, bool *tail_queued = nullptr) {
   const Grid &g = ctx->g;
   const double norm = 1.0 / (double)g.nr;  // Parseval factor of the k-space sums
@@ -952,7 +957,7 @@ int kcg_t(kapsel_ctx *ctx, T *x, const T *b, const T *eps, double tol_abs, int m
   double rr = rr_known >= 0.0 ? rr_known : read_scal(ctx, S_RR) * norm;
   const double tol2 = tol_abs * tol_abs;
   ConvergenceWatch cw(ctx, norm, rr, true);
-  if (begun) cw.queued();  // the first iteration, queued by kcg_begin
+  for (int i = 0; i < begun; i++) cw.queued(begun - 1 - i);  // the iterations kcg_begin queued
   cw.run(tol2, maxit, [&]() { kcg_enqueue<T>(ctx, x, eps, precond, w, s, face); }, tail);
   if (tail_queued) *tail_queued = cw.tail_queued;
   rr = cw.rr;
@@ -1230,6 +1235,7 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
   // be needed only wrote scratch (or is redone before the solve returns).
   int total = 0;
   double rel = 1.0, bnorm = 0.0;
+  const int nbegin = maxit < 2 ? maxit : 2;  // iterations queued before the first read-back
   bool outputs_current = false;
   residual();
   for (int outer = 0; outer < 30; outer++) {
@@ -1240,7 +1246,7 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
       hipLaunchKernelGGL((convert_real_kernel<float, double>), dim3(fb), dim3(TPB), 0, ctx->stream, b32, rk, g);
       KAPSEL_LAUNCH_CHECK();
       ctx->refine64 = false;
-      kcg_begin<float>(ctx, d32, b32, eps32, precond, w32, s32 + 1, face32);
+      kcg_begin<float>(ctx, d32, b32, eps32, precond, w32, s32 + 1, face32, nbegin);
       ctx->refine64 = true;
     } else {
       outputs();
@@ -1279,7 +1285,7 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
     const double tol_in = fmax(1e-4 * sqrt(rr), 0.8 * tol * bnorm);
     bool tail = false;
     int it = kcg_t<float>(ctx, d32, b32, eps32, tol_in, maxit - total, precond, w32, s32 + 1, face32, &rr_in, rr,
-                          begun, &apply_and_residual, &tail);
+                          begun ? nbegin : 0, &apply_and_residual, &tail);
     total += it;
     ctx->refine64 = true;
     if (!tail) {
