@@ -180,11 +180,12 @@ __global__ void advection_combine_slaved_kernel(const T *__restrict__ u0, const 
 // after the 2/3 cut with the FIX_CELL mask (zeta_k2u_k kernel): one pass
 // over the modes.  A cut mode's zeta is zeroed as the cut zeroes it (its
 // products are not read); the velocity spectrum goes to o.
+// (o may be the product fields themselves: each mode is read before it is
+// written, by the same thread)
 template <typename T>
-__global__ void slaved_zeta2u_kernel(const T *__restrict__ u0, const T *__restrict__ u1, const T *__restrict__ u2,
-                                     const T *__restrict__ a0, const T *__restrict__ a1, T *__restrict__ z0,
-                                     T *__restrict__ z1, P5<T> o, double *__restrict__ dc, Grid g, double dmy0,
-                                     double dmy1, int fix_mask) {
+__global__ void slaved_zeta2u_kernel(const T *u0, const T *u1, const T *u2, const T *a0, const T *a1,
+                                     T *__restrict__ z0, T *__restrict__ z1, P5<T> o, double *__restrict__ dc,
+                                     Grid g, double dmy0, double dmy1, int fix_mask) {
   size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= g.nc) return;
   KIdx k = kidx(g, c);
