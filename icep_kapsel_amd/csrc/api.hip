@@ -404,10 +404,12 @@ static void solenoidal_u_keep(kapsel_ctx *ctx, void *const *keep3, double add_s 
 // Navier-Stokes part left (cut and solenoidal, so that P(u + f s) = u + P f s):
 // keep3 = ns_keep + s P f_k from the forward transform of the force alone,
 // then U = c2r(keep3) -- the velocity is not transformed back after the
-// Navier-Stokes part and forward again with the force
-static void force_sole_keep(kapsel_ctx *ctx, void *const *ns_keep, void *const *keep3, double s) {
+// Navier-Stokes part and forward again with the force.  f_spectrum: F holds
+// the force's spectrum already
+static void force_sole_keep(kapsel_ctx *ctx, void *const *ns_keep, void *const *keep3, double s,
+                            bool f_spectrum = false) {
   void **f = ctx->fields + KAPSEL_F0;
-  fft_r2c(ctx, f, 3);
+  if (!f_spectrum) fft_r2c(ctx, f, 3);
   sole_add(ctx, ns_keep, f, keep3, s);
   fft_c2r(ctx, ctx->fields + KAPSEL_U0, 3, keep3);
 }
@@ -573,19 +575,36 @@ static void icep_step(kapsel_ctx *ctx, kapsel_time &t, const double e0[3], doubl
 // applied to u before the hydrodynamic force.  The branch's first
 // Calc_f_hydro (:60-68) only fills the diagnostic f_hydro1; it is skipped
 // (the second call overwrites f_hydro and torque_hydro before any use).
+// FP32 context: the velocity of Zeta_k2u stays a spectrum (cut and
+// solenoidal) until the Coulomb force is added in k-space, and the step ends
+// in k-space (force_sole_keep, f_sole_u2zeta_k), leaving U as the velocity
+// spectrum; a step that reuses it transforms that back first.
 static void electrolyte_step(kapsel_ctx *ctx, kapsel_time &t) {
   const bool reuse = ctx->el_reuse_gen == ctx->api_gen;
+  const bool fp32 = ctx->prec == KAPSEL_FP32;
+  void **u = ctx->fields + KAPSEL_U0;
+  if (reuse && fp32) fft_c2r(ctx, electrolyte_u_keep(ctx), 3, u);
   nssolute_step(ctx, t, reuse);               // :20
-  zeta_k2u(ctx, fix_mask(ctx));               // :29-42
+  if (fp32) zeta_k2u_k(ctx, fix_mask(ctx));  // :29-42, U: the spectrum
+  else zeta_k2u(ctx, fix_mask(ctx));
   md_position(ctx, t);                        // :44-50
   make_phi_particle_sum(ctx);                 // :51-53
-  rescale_and_coulomb_force(ctx, t);          // :55-72
-  add_force_to_u(ctx, t.dt_fluid * ctx->irho);  // :75-84
-  solenoidal_u(ctx, ctx->fields + KAPSEL_U0);   // :85
+  rescale_and_coulomb_force(ctx, t, fp32);    // :55-72 (FP32: F receives the spectrum)
+  void *keep3[3] = {fld(ctx, KAPSEL_WORK0), fld(ctx, KAPSEL_WORK1), fld(ctx, KAPSEL_WORK2)};
+  if (fp32) {
+    force_sole_keep(ctx, u, keep3, t.dt_fluid * ctx->irho, true);  // :75-85
+  } else {
+    add_force_to_u(ctx, t.dt_fluid * ctx->irho);  // :75-84
+    solenoidal_u(ctx, u);                         // :85
+  }
   hydro_force_velocity(ctx, t);               // :99-110, Calc_f_hydro, Force, velocity
   make_u_particle_sum(ctx);                   // :166-167
-  f_particle_dt_sole_add(ctx, electrolyte_u_keep(ctx));  // :168-169
-  u2zeta_k(ctx);                              // :184
+  if (fp32) {
+    f_sole_u2zeta_k(ctx, keep3);              // :168-169, 184
+  } else {
+    f_particle_dt_sole_add(ctx, electrolyte_u_keep(ctx));  // :168-169
+    u2zeta_k(ctx);                            // :184
+  }
   t.time += t.dt_fluid;
   t.ts += 1;
   // the next entry point sees api_gen + 1; if it is this step again, what
@@ -614,8 +633,11 @@ static void dielectric_potential(kapsel_ctx *ctx, const double e0[3], double tol
 // Rescale_solute, Make_Maxwell_force_x_on_fluid with the ions' charge
 static void icep_electrolyte_step(kapsel_ctx *ctx, kapsel_time &t, double tol, int maxit, int precond, int *iters,
                                   double *relres) {
+  const bool fp32 = ctx->prec == KAPSEL_FP32;
+  void **u = ctx->fields + KAPSEL_U0;
   nssolute_step(ctx, t, false);               // :20
-  zeta_k2u(ctx, fix_mask(ctx));               // :29-42
+  if (fp32) zeta_k2u_k(ctx, fix_mask(ctx));  // :29-42, U: the spectrum (as electrolyte_step)
+  else zeta_k2u(ctx, fix_mask(ctx));
   md_position(ctx, t);                        // :44-50
   make_phi_particle_sum(ctx);                 // :51-53
   const double s = ctx->el.ac ? sin(ctx->el.angular_frequency * t.time) : 1.0;
@@ -623,9 +645,8 @@ static void icep_electrolyte_step(kapsel_ctx *ctx, kapsel_time &t, double tol, i
   dielectric_potential(ctx, e0, tol, maxit, precond, iters, relres);  // :72-74
   maxwell_force(ctx, e0, true);
   void *keep3[3] = {ctx->pscratch[5], ctx->pscratch[6], ctx->pscratch[7]};
-  const bool fp32 = ctx->prec == KAPSEL_FP32;
   if (fp32) {
-    solenoidal_u_keep(ctx, keep3, t.dt_fluid * ctx->irho);  // :75-85, u += f dt / rho inside
+    force_sole_keep(ctx, u, keep3, t.dt_fluid * ctx->irho);  // :75-85
   } else {
     add_force_to_u(ctx, t.dt_fluid * ctx->irho);  // :75-84
     solenoidal_u(ctx, ctx->fields + KAPSEL_U0);   // :85

@@ -330,12 +330,13 @@ template <typename T>
 void fft_r2c_t(kapsel_ctx *ctx, void *const *fields, int nf);
 template <typename T>
 void fft_c2r_t(kapsel_ctx *ctx, void *const *fields, int nf, void *const *src = nullptr);
-// out_f = a_f + s b_f c_f transformed forward where a plane pass applies (a
-// or c NULL: left out; bit f of dsq: (b_f - c_f)(b_f + c_f) instead of the
-// product); false: nothing queued
+// out_f = a_f + s (b_f + e_f) c_f transformed forward, the product formed in
+// the first pass (a or c NULL: left out; bit f of dsq: (b_f - c_f)(b_f + c_f)
+// instead of the product; e NULL: e_f = 0).  out_f may be one of field f's
+// own inputs, no other field's.  false: nothing queued (no pass for NZ)
 template <typename T>
 bool fft_r2c_pro_t(kapsel_ctx *ctx, void *const *out, void *const *a, void *const *b, void *const *c, double s,
-                   int nf, unsigned dsq = 0);
+                   int nf, unsigned dsq = 0, const double *e = nullptr);
 // whether the z and y transforms of the context's grid run as one plane pass
 inline bool plane_ok_grid(const Grid &g) { return g.ny == g.nz && (g.nz == 32 || g.nz == 64 || g.nz == 128); }
 // g_d <- r2c(face_d * c2r(g_d)) for the three staggered-gradient spectra
@@ -464,8 +465,8 @@ void *const *electrolyte_u_keep(kapsel_ctx *ctx);
 void rescale_solute(kapsel_ctx *ctx, double *factor_host);
 void coulomb_force(kapsel_ctx *ctx, const kapsel_time &t);
 // Rescale_solute right after the profile scatter (PHI_SUM current) and the
-// Coulomb force of the rescaled ions into F0..F2
-void rescale_and_coulomb_force(kapsel_ctx *ctx, const kapsel_time &t);
+// Coulomb force of the rescaled ions into F0..F2 (spectrum: its spectrum)
+void rescale_and_coulomb_force(kapsel_ctx *ctx, const kapsel_time &t, bool spectrum = false);
 // Count_solute_each (solute_rhs.h:143-156) into the conserved totals
 void recount_total_solute(kapsel_ctx *ctx);
 // Init_rho_ion with Poisson_Boltzmann, salt, several particles; returns the

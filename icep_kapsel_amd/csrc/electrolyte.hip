@@ -20,6 +20,7 @@
 #include <cmath>
 
 #include "common.hpp"
+#include "fft_prims.hpp"
 
 namespace kapsel {
 
@@ -229,44 +230,58 @@ __global__ void flux_k_kernel(T *__restrict__ ck, T *__restrict__ f0, T *__restr
   if (cx) st2(cx + 2 * c, a);
 }
 
-// Per species, real-space half of the flux: Solute_solver_rhs_nonlinear_x_single
+// Real-space half of the flux of both species in one pass (the gradient,
+// normals and velocity are read once): Solute_solver_rhs_nonlinear_x_single
 // (solute_rhs.cxx:150-188), Solute_impermeability (:42-97) on every point
 // whose surface normal is nonzero (exactly the points the reference's
 // particle loop projects; the projection is idempotent, so points of two
-// particles' cells need it once), Add_advection_flux (:190-210).  With fac,
-// the real ion field is the one kept from before the last rescale and is
-// multiplied by its factor.
+// particles' cells need it once), Add_advection_flux (:190-210).  f: the
+// fluxes of species 0 (p[0..2]) and 1 (p[3..5]), the diffusive part in,
+// the whole flux out.  With fac, the real ion fields are the ones kept from
+// before the last rescale and are multiplied by their factors fac[0..1].
 template <typename T>
-__global__ void flux_x_kernel(T *__restrict__ f0, T *__restrict__ f1, T *__restrict__ f2, const T *__restrict__ cx,
-                              const double *__restrict__ fac, const T *__restrict__ g0, const T *__restrict__ g1,
-                              const T *__restrict__ g2, const T *__restrict__ n0, const T *__restrict__ n1,
-                              const T *__restrict__ n2, const T *__restrict__ u0, const T *__restrict__ u1,
-                              const T *__restrict__ u2, double onsager, double ze, int advect, Grid g) {
+__global__ void flux_x2_kernel(FieldPtrs<T> f, const T *__restrict__ cx0, const T *__restrict__ cx1,
+                               const double *__restrict__ fac, const T *__restrict__ g0, const T *__restrict__ g1,
+                               const T *__restrict__ g2, const T *__restrict__ n0, const T *__restrict__ n1,
+                               const T *__restrict__ n2, const T *__restrict__ u0, const T *__restrict__ u1,
+                               const T *__restrict__ u2, double on0, double on1, double ze0, double ze1, int advect,
+                               Grid g) {
   size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= g.nr) return;
   const size_t im = ridx(g, r).im;
-  T c = cx[im];
-  if (fac) c *= (T)*fac;
-  const T L = (T)onsager, z = (T)ze;
-  T a0 = f0[im] + c * (L * (g0[im] * z));
-  T a1 = f1[im] + c * (L * (g1[im] * z));
-  T a2 = f2[im] + c * (L * (g2[im] * z));
+  const T gr0 = g0[im], gr1 = g1[im], gr2 = g2[im];
   const T m0 = n0[im], m1 = n1[im], m2 = n2[im];
   const T norm2 = m0 * m0 + m1 * m1 + m2 * m2;
-  if (norm2 > (T)0) {
-    const T dmy = (a0 * m0 + a1 * m1 + a2 * m2) / norm2;
-    a0 -= m0 * dmy;
-    a1 -= m1 * dmy;
-    a2 -= m2 * dmy;
-  }
+  T v0 = 0, v1 = 0, v2 = 0;
   if (advect) {
-    a0 += c * -u0[im];
-    a1 += c * -u1[im];
-    a2 += c * -u2[im];
+    v0 = u0[im];
+    v1 = u1[im];
+    v2 = u2[im];
   }
-  f0[im] = a0;
-  f1[im] = a1;
-  f2[im] = a2;
+#pragma unroll
+  for (int n = 0; n < 2; n++) {
+    T *__restrict__ f0 = f.p[3 * n], *__restrict__ f1 = f.p[3 * n + 1], *__restrict__ f2 = f.p[3 * n + 2];
+    T c = (n ? cx1 : cx0)[im];
+    if (fac) c *= (T)fac[n];
+    const T L = (T)(n ? on1 : on0), z = (T)(n ? ze1 : ze0);
+    T a0 = f0[im] + c * (L * (gr0 * z));
+    T a1 = f1[im] + c * (L * (gr1 * z));
+    T a2 = f2[im] + c * (L * (gr2 * z));
+    if (norm2 > (T)0) {
+      const T dmy = (a0 * m0 + a1 * m1 + a2 * m2) / norm2;
+      a0 -= m0 * dmy;
+      a1 -= m1 * dmy;
+      a2 -= m2 * dmy;
+    }
+    if (advect) {
+      a0 += c * -v0;
+      a1 += c * -v1;
+      a2 += c * -v2;
+    }
+    f0[im] = a0;
+    f1[im] = a1;
+    f2[im] = a2;
+  }
 }
 
 // psi_k = rho_k IK2 / Dielectric_cst (Charge_field_k2Coulomb_potential_k_PBC,
@@ -552,8 +567,9 @@ void coulomb_grad_t(kapsel_ctx *ctx, const T *rhok, T *const *g3, int trunc, con
 
 // Rhs_NS_Nernst_Planck + Rhs_NS_solute + Field_solver_Euler
 // (fluid_solver.cxx:184-338), non-Dielectric branch.  Buffers: grad_potential
-// in F0..F2 and solute_flux in UP0..UP2 as in the reference's call
-// (:328), the real c_n in WORK2, the right-hand side of zeta in WORK0/WORK1.
+// in F0..F2 and solute_flux in UP0..UP2 as in the reference's call (:328),
+// the second species' flux in WORK0..WORK2, the real c_n in E_CX0/E_CX1,
+// the right-hand side of zeta in WORK0/WORK1 after the fluxes.
 // reuse: the previous electrolyte step ended right before, so E_RHOK holds
 // the charge spectrum of the current ions and particles (its Coulomb force),
 // E_CX0/E_CX1 times the rescale factors are the real ion fields (the spectra
@@ -618,50 +634,61 @@ void nssolute_t(kapsel_ctx *ctx, const kapsel_time &t, bool reuse, bool ions_onl
     for (int d = 0; d < 3; d++) u[d] = fl<T>(ctx, KAPSEL_U0 + d);
   }
   make_surface_normal(ctx, ctx->el_f + E_N0);
-  T *flux[3] = {fl<T>(ctx, KAPSEL_UP0), fl<T>(ctx, KAPSEL_UP1), fl<T>(ctx, KAPSEL_UP2)};
+  // the fluxes of species 0 in UP0..UP2, of species 1 in WORK0..WORK2; the
+  // real (cut) ion fields are formed in E_CX0/E_CX1 unless reused
+  FieldPtrs<T> flux{{fl<T>(ctx, KAPSEL_UP0), fl<T>(ctx, KAPSEL_UP1), fl<T>(ctx, KAPSEL_UP2), fl<T>(ctx, KAPSEL_WORK0),
+                     fl<T>(ctx, KAPSEL_WORK1), fl<T>(ctx, KAPSEL_WORK2)}};
   for (int n = 0; n < 2; n++) {
     T *ck = fl<T>(ctx, KAPSEL_CONC0 + n);
-    T *cx = reuse ? ef<T>(ctx, E_CX0 + n) : fl<T>(ctx, KAPSEL_WORK2);
-    const double *fac = reuse ? ctx->el_scal + S_F0 + n : nullptr;
+    T *cx = ef<T>(ctx, E_CX0 + n);
     {
       LaunchTimer lt(ctx, KAPSEL_K_POINTWISE,
                      (2.0 * g.f + 2.0 * cut_count_el(g) + (reuse ? 3.0 : 4.0) * g.f) * sizeof(T));
-      hipLaunchKernelGGL(flux_k_kernel<T>, dim3(blocks_for(g.nc, TPB)), dim3(TPB), 0, ctx->stream, ck, flux[0],
-                         flux[1], flux[2], reuse ? nullptr : cx, ctx->el.kbt * ctx->el.onsager_coeff[n], g);
+      hipLaunchKernelGGL(flux_k_kernel<T>, dim3(blocks_for(g.nc, TPB)), dim3(TPB), 0, ctx->stream, ck,
+                         flux.p[3 * n], flux.p[3 * n + 1], flux.p[3 * n + 2], reuse ? nullptr : cx,
+                         ctx->el.kbt * ctx->el.onsager_coeff[n], g);
       KAPSEL_LAUNCH_CHECK();
     }
-    void *four[4] = {flux[0], flux[1], flux[2], cx};
+    void *four[4] = {flux.p[3 * n], flux.p[3 * n + 1], flux.p[3 * n + 2], cx};
     fft_c2r_t<T>(ctx, four, reuse ? 3 : 4);
-    {
-      LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, 16.0 * g.nr * sizeof(T));
-      hipLaunchKernelGGL(flux_x_kernel<T>, dim3(blocks_for(g.nr, TPB)), dim3(TPB), 0, ctx->stream, flux[0],
-                         flux[1], flux[2], cx, fac, grad[0], grad[1], grad[2], ef<T>(ctx, E_N0), ef<T>(ctx, E_N1),
-                         ef<T>(ctx, E_N2), u[0], u[1], u[2], ctx->el.onsager_coeff[n], valency_e(ctx, n),
-                         ions_only ? 0 : 1, g);
-      KAPSEL_LAUNCH_CHECK();
-    }
-    fft_r2c_t<T>(ctx, (void *const *)flux, 3);
-    {
-      LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, 5.0 * 2.0 * kept_count_el(g) * sizeof(T));
-      hipLaunchKernelGGL(div_euler_kernel<T>, dim3(blocks_for(g.nc, TPB)), dim3(TPB), 0, ctx->stream, flux[0],
-                         flux[1], flux[2], ck, t.dt_fluid, g);
-      KAPSEL_LAUNCH_CHECK();
-    }
+  }
+  {
+    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, (ions_only ? 20.0 : 23.0) * g.nr * sizeof(T));
+    hipLaunchKernelGGL(flux_x2_kernel<T>, dim3(blocks_for(g.nr, TPB)), dim3(TPB), 0, ctx->stream, flux,
+                       ef<T>(ctx, E_CX0), ef<T>(ctx, E_CX1), reuse ? ctx->el_scal + S_F0 : nullptr, grad[0],
+                       grad[1], grad[2], ef<T>(ctx, E_N0), ef<T>(ctx, E_N1), ef<T>(ctx, E_N2), u[0], u[1], u[2],
+                       ctx->el.onsager_coeff[0], ctx->el.onsager_coeff[1], valency_e(ctx, 0), valency_e(ctx, 1),
+                       ions_only ? 0 : 1, g);
+    KAPSEL_LAUNCH_CHECK();
+  }
+  for (int n = 0; n < 2; n++) {
+    void *three[3] = {flux.p[3 * n], flux.p[3 * n + 1], flux.p[3 * n + 2]};
+    fft_r2c_t<T>(ctx, three, 3);
+    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, 5.0 * 2.0 * kept_count_el(g) * sizeof(T));
+    hipLaunchKernelGGL(div_euler_kernel<T>, dim3(blocks_for(g.nc, TPB)), dim3(TPB), 0, ctx->stream, flux.p[3 * n],
+                       flux.p[3 * n + 1], flux.p[3 * n + 2], fl<T>(ctx, KAPSEL_CONC0 + n), t.dt_fluid, g);
+    KAPSEL_LAUNCH_CHECK();
   }
   if (ions_only) return;
-  // Rhs_solvent (U2advection_k) + viscous term + Euler step of zeta
-  if (reuse) {
-    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, 8.0 * g.nr * sizeof(T));
-    hipLaunchKernelGGL(products_kernel<T>, dim3(blocks_for(g.nr, TPB)), dim3(TPB), 0, ctx->stream, u[0], u[1],
-                       u[2], fl<T>(ctx, KAPSEL_U0), fl<T>(ctx, KAPSEL_U1), fl<T>(ctx, KAPSEL_U2),
-                       fl<T>(ctx, KAPSEL_WORK0), fl<T>(ctx, KAPSEL_WORK1), g);
-    KAPSEL_LAUNCH_CHECK();
-  } else {
-    advection_products(ctx);
+  // Rhs_solvent (U2advection_k) + viscous term + Euler step of zeta; with
+  // reuse in an FP32 context the five products of the real velocity are
+  // formed inside the first pass of their forward transform (same
+  // arithmetic as products_kernel)
+  void *five[5] = {ctx->fields[KAPSEL_U0], ctx->fields[KAPSEL_U1], ctx->fields[KAPSEL_U2], ctx->adv_w[0],
+                   ctx->adv_w[1]};
+  void *b5[5] = {u[1], u[2], u[0], u[1], u[2]}, *c5[5] = {u[2], u[0], u[1], u[2], u[0]};
+  if (!(reuse && sizeof(T) == 4 && fft_r2c_pro_t<T>(ctx, five, nullptr, b5, c5, 1.0, 5, 0x18u))) {
+    if (reuse) {
+      LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, 8.0 * g.nr * sizeof(T));
+      hipLaunchKernelGGL(products_kernel<T>, dim3(blocks_for(g.nr, TPB)), dim3(TPB), 0, ctx->stream, u[0], u[1],
+                         u[2], fl<T>(ctx, KAPSEL_U0), fl<T>(ctx, KAPSEL_U1), fl<T>(ctx, KAPSEL_U2),
+                         (T *)ctx->adv_w[0], (T *)ctx->adv_w[1], g);
+      KAPSEL_LAUNCH_CHECK();
+    } else {
+      advection_products(ctx);
+    }
+    fft_r2c_t<T>(ctx, five, 5);
   }
-  void *five[5] = {ctx->fields[KAPSEL_U0], ctx->fields[KAPSEL_U1], ctx->fields[KAPSEL_U2],
-                   ctx->fields[KAPSEL_WORK0], ctx->fields[KAPSEL_WORK1]};
-  fft_r2c_t<T>(ctx, five, 5);
   advection_combine_euler(ctx, t.dt_fluid);
 }
 
@@ -686,8 +713,10 @@ void rescale_t(kapsel_ctx *ctx, bool phi_sum_current) {
 // Make_Coulomb_force_x_on_fluid (operate_electrolyte.cxx:90-177), free
 // particles; scaled: the ion fields hold the transforms from before the last
 // rescale (see rescale_t), otherwise they are recomputed
+// spectrum: F0..F2 receive the force's spectrum, formed inside the first pass
+// of its forward transform instead of by the pointwise pass (same arithmetic)
 template <typename T>
-void coulomb_force_t(kapsel_ctx *ctx, const kapsel_time &t, int scaled) {
+void coulomb_force_t(kapsel_ctx *ctx, const kapsel_time &t, int scaled, bool spectrum = false) {
   if (!scaled) ions_real_t<T>(ctx);
   T *rhok = ef<T>(ctx, E_RHOK), *rho = ef<T>(ctx, E_RHO);
   charge_t<T>(ctx, rhok, rho, scaled);
@@ -698,10 +727,19 @@ void coulomb_force_t(kapsel_ctx *ctx, const kapsel_time &t, int scaled) {
   coulomb_grad_t<T>(ctx, rhok, f, 0, zero);
   fft_c2r_t<T>(ctx, (void *const *)f, 3);
   const double s = ac_factor(ctx, t);
-  LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, 7.0 * ctx->g.nr * sizeof(T));
-  hipLaunchKernelGGL(coulomb_force_kernel<T>, dim3(blocks_for(ctx->g.nr, TPB)), dim3(TPB), 0, ctx->stream, f[0],
-                     f[1], f[2], rho, ctx->el.e_ext[0] * s, ctx->el.e_ext[1] * s, ctx->el.e_ext[2] * s, ctx->g);
-  KAPSEL_LAUNCH_CHECK();
+  if (spectrum) {
+    // f_d = -((grad_d psi - E_d) rho) = rho (E_d - grad_d psi)
+    const double me[3] = {-ctx->el.e_ext[0] * s, -ctx->el.e_ext[1] * s, -ctx->el.e_ext[2] * s};
+    void *r3[3] = {rho, rho, rho};
+    if (fft_r2c_pro_t<T>(ctx, (void *const *)f, nullptr, (void *const *)f, r3, -1.0, 3, 0, me)) return;
+  }
+  {
+    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, 7.0 * ctx->g.nr * sizeof(T));
+    hipLaunchKernelGGL(coulomb_force_kernel<T>, dim3(blocks_for(ctx->g.nr, TPB)), dim3(TPB), 0, ctx->stream, f[0],
+                       f[1], f[2], rho, ctx->el.e_ext[0] * s, ctx->el.e_ext[1] * s, ctx->el.e_ext[2] * s, ctx->g);
+    KAPSEL_LAUNCH_CHECK();
+  }
+  if (spectrum) fft_r2c_t<T>(ctx, (void *const *)f, 3);
 }
 
 // Calc_free_energy_PB, non-Dielectric branch (operate_electrolyte.cxx:302-421):
@@ -1064,14 +1102,14 @@ const double *dielectric_free_charge(kapsel_ctx *ctx) {
 // the Electrolyte branch's middle part (sp_3d_ns.cxx:55-95): Rescale_solute
 // right after the profile scatter (PHI_SUM current), then the Coulomb force
 // from the rescaled ions
-void rescale_and_coulomb_force(kapsel_ctx *ctx, const kapsel_time &t) {
+void rescale_and_coulomb_force(kapsel_ctx *ctx, const kapsel_time &t, bool spectrum) {
   need_el_plain(ctx, "the Coulomb force");
   if (ctx->prec == KAPSEL_FP64) {
     rescale_t<double>(ctx, true);
-    coulomb_force_t<double>(ctx, t, 1);
+    coulomb_force_t<double>(ctx, t, 1, spectrum);
   } else {
     rescale_t<float>(ctx, true);
-    coulomb_force_t<float>(ctx, t, 1);
+    coulomb_force_t<float>(ctx, t, 1, spectrum);
   }
 }
 

@@ -58,32 +58,35 @@ __global__ __launch_bounds__((LPB * LineFFT<T, L>::S)) void fft_c2c_lines_io(Fie
 
 // --------------------------------------------------------------- z passes
 // Row r (NX*NY rows, pitch nz_ reals).  z[n] = a[2n] + i a[2n+1], n < M.
-template <typename T, int NZ, int RPB>
-__global__ __launch_bounds__(RPB *FS<NZ / 2>::R2) void fft_z_r2c(FieldPtrs<T> fp, long long rows, int nz_,
-                                                                  const V2<T> *__restrict__ twm,
-                                                                  const V2<T> *__restrict__ zrot) {
+// The plain passes read and write each row in the transform's order (TPL
+// threads per row, 8 bytes each); the r2c pass that forms its input from
+// several fields (fft_z_r2c_pro) instead reads and writes its block's tile
+// of rows 16 bytes per lane in address order, through the rows' LDS slots.
+// (Measured at 256^3 FP32: staging costs the one-input passes 10-15 %, and
+// saves the three-input prologue about 30 %.)
+
+// the row stride (complex values) of a row's LDS slot in the z passes
+template <int NZ>
+constexpr int z_row_stride() {
+  return FS<NZ / 2>::R2 * (FS<NZ / 2>::R1 + 1) + 1;
+}
+
+// The r2c of a row: thread n2 of the row's TPL = R2 threads holds v[n1] =
+// z[R2 n1 + n2]; the M-point complex transform (R1 in registers, R2 across
+// the row's threads through its LDS slot sr), then the split step, whose
+// outputs X[k], k = n2 + TPL j (and k = M for n2 = 0) go to out(k, X[k])
+// while the slot is still read.  Starts with a barrier.
+template <typename T, int NZ, typename Out>
+__device__ __forceinline__ void z_r2c_core(V2<T> (&v)[FS<NZ / 2>::R1], V2<T> *__restrict__ sr, int n2,
+                                           const V2<T> *__restrict__ twm, const V2<T> *__restrict__ zrot,
+                                           Out out) {
   constexpr int M = NZ / 2;
   constexpr int R1 = FS<M>::R1, R2 = FS<M>::R2, TPL = R2;
-  constexpr int RS = R2 * (R1 + 1) + 1;
-  static_assert(RS >= M + 1, "row stride");
-  __shared__ V2<T> sm[RPB * RS];
-  unsigned field;
-  const unsigned tile = xcd_tile(field);
-  T *__restrict__ data = fp.p[field];
-  const int t = threadIdx.x;
-  const int n2 = t % TPL;
-  const int r = t / TPL;
-  const long long row = (long long)tile * RPB + r;
-  const bool valid = row < rows;
-  T *__restrict__ rp = data + (valid ? row : 0) * nz_;
-  V2<T> *sr = sm + r * RS;
-
-  V2<T> v[R1], o[R1];
-#pragma unroll
-  for (int n1 = 0; n1 < R1; n1++) v[n1] = valid ? ld2(rp + 2 * (R2 * n1 + n2)) : V2<T>{0, 0};
+  V2<T> o[R1];
   dft<R1, 1, -1>(v, o);
 #pragma unroll
   for (int k1 = 1; k1 < R1; k1++) o[k1] = cmul(o[k1], tw_get<-1>(twm, n2 * k1));
+  __syncthreads();
 #pragma unroll
   for (int k1 = 0; k1 < R1; k1++) sr[n2 * (R1 + 1) + k1] = o[k1];
   __syncthreads();
@@ -104,17 +107,138 @@ __global__ __launch_bounds__(RPB *FS<NZ / 2>::R2) void fft_z_r2c(FieldPtrs<T> fp
     for (int k2 = 0; k2 < R2; k2++) sr[k1 + R1 * k2] = zz[c][k2];
   }
   __syncthreads();
-  if (valid) {
-    // X[k] = E + W^k O,  E = (Z[k] + conj Z[M-k]) / 2,  O = (Z[k] - conj Z[M-k]) / 2i
-    for (int k = n2; k <= M; k += TPL) {
-      V2<T> zk = sr[k & (M - 1)];
-      V2<T> zm = conjv(sr[(M - k) & (M - 1)]);
-      V2<T> e = scal((T)0.5, zk + zm);
-      V2<T> d = zk - zm;
-      V2<T> od = {(T)0.5 * d.im, (T)-0.5 * d.re};
-      st2(rp + 2 * k, e + cmul(zrot[k], od));
-    }
+  // X[k] = E + W^k O,  E = (Z[k] + conj Z[M-k]) / 2,  O = (Z[k] - conj Z[M-k]) / 2i
+  auto split = [&](int k) {
+    V2<T> zk = sr[k & (M - 1)];
+    V2<T> zm = conjv(sr[(M - k) & (M - 1)]);
+    V2<T> e = scal((T)0.5, zk + zm);
+    V2<T> d = zk - zm;
+    V2<T> od = {(T)0.5 * d.im, (T)-0.5 * d.re};
+    return e + cmul(zrot[k], od);
+  };
+#pragma unroll
+  for (int j = 0; j < M / TPL; j++) out(n2 + TPL * j, split(n2 + TPL * j));
+  if (n2 == 0) out(M, split(M));
+}
+
+template <typename T, int NZ, int RPB>
+__global__ __launch_bounds__(RPB *FS<NZ / 2>::R2) void fft_z_r2c(FieldPtrs<T> fp, long long rows, int nz_,
+                                                                  const V2<T> *__restrict__ twm,
+                                                                  const V2<T> *__restrict__ zrot) {
+  constexpr int M = NZ / 2;
+  constexpr int R1 = FS<M>::R1, R2 = FS<M>::R2, TPL = R2;
+  constexpr int RS = z_row_stride<NZ>();
+  static_assert(RS >= M + 1, "row stride");
+  __shared__ V2<T> sm[RPB * RS];
+  unsigned field;
+  const unsigned tile = xcd_tile(field);
+  const int t = threadIdx.x;
+  const int n2 = t % TPL;
+  const int r = t / TPL;
+  const long long row = (long long)tile * RPB + r;
+  const bool valid = row < rows;
+  // row 0 for an invalid row: every load is in bounds and none is
+  // predicated; such a row's results are never stored
+  T *__restrict__ rp = fp.p[field] + (valid ? row : 0) * nz_;
+  V2<T> v[R1];
+#pragma unroll
+  for (int n1 = 0; n1 < R1; n1++) v[n1] = ld2(rp + 2 * (R2 * n1 + n2));
+  z_r2c_core<T, NZ>(v, sm + r * RS, n2, twm, zrot, [&](int k, V2<T> x) {
+    if (valid) st2(rp + 2 * k, x);
+  });
+}
+
+// 16 bytes of complex values
+template <typename T>
+struct alignas(16) Q16 {
+  V2<T> c[16 / (2 * sizeof(T))];
+};
+
+// Tile access in address order: complex value q of the block's tile of
+// nrow rows (row q / H, column q % H, H = nz_ / 2) has the LDS slot place
+// sm[row * RS + column].  f(q) is called for the first value q of every
+// 16-byte group (a group may span two rows).
+template <typename T, int NZ, int RPB, int THREADS, typename F>
+__device__ __forceinline__ void z_tile_groups(int nrow, F f) {
+  constexpr int CPG = 16 / (2 * (int)sizeof(T));  // complex values per group
+  constexpr int H = NZ / 2 + 1;
+  constexpr int NG = RPB * H / CPG;                 // groups in a full tile
+  static_assert(RPB * H % CPG == 0, "tile of whole groups");
+  const int ng = nrow * H / CPG;
+#pragma unroll
+  for (int i = 0; i < (NG + THREADS - 1) / THREADS; i++) {
+    const int gi = (int)threadIdx.x + THREADS * i;
+    if (gi < ng) f(gi * CPG);
   }
+}
+
+// fft_z_r2c of a field formed on the fly as the tile is read: out_f =
+// a_f + s (b_f + e_f) c_f (a_f NULL: no a term; c_f NULL: a_f + s (b_f +
+// e_f); bit f of dsq: (b_f - c_f)(b_f + c_f) in place of the product), the
+// prologue of fft_plane_r2c_pro for grids whose planes do not fit in LDS.
+// out_f may be one of field f's own inputs (a block reads its tile before
+// it stores it), but no other field's input.  The fields of one tile run
+// at consecutive ids (xcd_tile_by_tile), so tiles they read in common come
+// from HBM once.
+template <typename T, int NZ, int RPB>
+__global__ __launch_bounds__(RPB *FS<NZ / 2>::R2) void fft_z_r2c_pro(FieldPtrs<T> out, FieldPtrs<T> a,
+                                                                      FieldPtrs<T> b, FieldPtrs<T> c,
+                                                                      FieldConsts<T> e, T s, unsigned dsq,
+                                                                      long long rows, int nz_,
+                                                                      const V2<T> *__restrict__ twm,
+                                                                      const V2<T> *__restrict__ zrot) {
+  constexpr int M = NZ / 2, R1 = FS<M>::R1, R2 = FS<M>::R2, TPL = R2, THREADS = RPB * TPL;
+  constexpr int RS = z_row_stride<NZ>(), H = M + 1, CPG = 16 / (2 * (int)sizeof(T));
+  __shared__ V2<T> sm[RPB * RS];
+  unsigned field;
+  const unsigned tile = xcd_tile_by_tile(field);
+  const long long row0 = (long long)tile * RPB;
+  const int nrow = rows - row0 < RPB ? (int)(rows - row0) : RPB;
+  const size_t to = (size_t)row0 * nz_;
+  const T *ga = a.p[field] ? a.p[field] + to : nullptr, *gb = b.p[field] + to;
+  const T *gc = c.p[field] ? c.p[field] + to : nullptr;
+  const bool diff = (dsq >> field) & 1;
+  const T ef = e.v[field];
+  z_tile_groups<T, NZ, RPB, THREADS>(nrow, [&](int q) {
+    Q16<T> vb = *reinterpret_cast<const Q16<T> *>(gb + 2 * (size_t)q);
+    Q16<T> vc{}, va{};
+    if (gc) vc = *reinterpret_cast<const Q16<T> *>(gc + 2 * (size_t)q);
+    if (ga) va = *reinterpret_cast<const Q16<T> *>(ga + 2 * (size_t)q);
+#pragma unroll
+    for (int h = 0; h < CPG; h++) {
+      V2<T> w = V2<T>{vb.c[h].re + ef, vb.c[h].im + ef};
+      if (gc) {
+        const V2<T> y = vc.c[h];
+        w = diff ? V2<T>{(w.re - y.re) * (w.re + y.re), (w.im - y.im) * (w.im + y.im)}
+                 : V2<T>{w.re * y.re, w.im * y.im};
+      }
+      w = ga ? V2<T>{va.c[h].re + s * w.re, va.c[h].im + s * w.im} : V2<T>{s * w.re, s * w.im};
+      const int row = (q + h) / H;
+      sm[row * RS + (q + h - row * H)] = w;
+    }
+  });
+  __syncthreads();
+  const int t = threadIdx.x, n2 = t % TPL;
+  V2<T> *sr = sm + (t / TPL) * RS;
+  V2<T> v[R1], x[M / TPL + 1];
+#pragma unroll
+  for (int n1 = 0; n1 < R1; n1++) v[n1] = sr[R2 * n1 + n2];
+  z_r2c_core<T, NZ>(v, sr, n2, twm, zrot, [&](int k, V2<T> y) { x[k == M ? M / TPL : (k - n2) / TPL] = y; });
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < M / TPL; j++) sr[n2 + TPL * j] = x[j];
+  if (n2 == 0) sr[M] = x[M / TPL];
+  __syncthreads();
+  T *__restrict__ dst = out.p[field] + to;
+  z_tile_groups<T, NZ, RPB, THREADS>(nrow, [&](int q) {
+    Q16<T> g;
+#pragma unroll
+    for (int h = 0; h < CPG; h++) {
+      const int row = (q + h) / H;
+      g.c[h] = sm[row * RS + (q + h - row * H)];
+    }
+    *reinterpret_cast<Q16<T> *>(dst + 2 * (size_t)q) = g;
+  });
 }
 
 template <typename T, int NZ, int RPB>
@@ -123,7 +247,7 @@ __global__ __launch_bounds__(RPB *FS<NZ / 2>::R2) void fft_z_c2r(FieldPtrs<T> fp
                                                                   const V2<T> *__restrict__ zrot, T scale) {
   constexpr int M = NZ / 2;
   constexpr int R1 = FS<M>::R1, R2 = FS<M>::R2, TPL = R2;
-  constexpr int RS = R2 * (R1 + 1) + 1;
+  constexpr int RS = z_row_stride<NZ>();
   __shared__ V2<T> sm[RPB * RS];
   unsigned field;
   const unsigned tile = xcd_tile(field);
@@ -426,19 +550,20 @@ __global__ KAPSEL_PLANE_BOUNDS void fft_plane_r2c(FieldPtrs<T> fp, const V2<T> *
   plane_columns<T, NY, NZ, THREADS, -1, false, true>(P, tw.twy, gp);
 }
 
-// fft_plane_r2c of a field formed on the fly: out_f = a_f + s b_f c_f (a_f
-// NULL: s b_f c_f; c_f NULL: a_f + s b_f; bit f of dsq: (b_f - c_f)(b_f +
-// c_f)), read pairwise where the row stage reads its input -- the pointwise
-// pass before a forward transform (f = u_p - phi u of
-// Make_f_particle_dt_nonsole, u + f dt/rho of the dielectric force, the
-// products of U2advection_k) without its own read and write of the fields.
-// out_f must not be an input of another field of the launch.  The fields of
-// one plane run at consecutive ids (xcd_tile_by_tile), so the planes they
-// read in common (phi; the velocity components of the products) come from
-// HBM once.
+// fft_plane_r2c of a field formed on the fly: out_f = a_f + s (b_f + e_f)
+// c_f (a_f NULL: no a term; c_f NULL: a_f + s (b_f + e_f); bit f of dsq:
+// (b_f - c_f)(b_f + c_f) in place of the product), read pairwise where the
+// row stage reads its input -- the pointwise pass before a forward transform
+// (f = u_p - phi u of Make_f_particle_dt_nonsole, u + f dt/rho of the
+// dielectric force, the products of U2advection_k, the Coulomb force
+// rho (E - grad psi)) without its own read and write of the fields.  out_f
+// may be one of field f's own inputs (the plane is read before it is
+// written), but no other field's input.  The fields of one plane
+// run at consecutive ids (xcd_tile_by_tile), so the planes they read in
+// common (phi; the velocity components of the products) come from HBM once.
 template <typename T, int NY, int NZ, int THREADS>
 __global__ KAPSEL_PLANE_BOUNDS void fft_plane_r2c_pro(FieldPtrs<T> out, FieldPtrs<T> a, FieldPtrs<T> b,
-                                                               FieldPtrs<T> c, T s, unsigned dsq,
+                                                               FieldPtrs<T> c, FieldConsts<T> e, T s, unsigned dsq,
                                                                const V2<T> *__restrict__ twm,
                                                                const V2<T> *__restrict__ zrot,
                                                                const V2<T> *__restrict__ twy) {
@@ -452,6 +577,7 @@ __global__ KAPSEL_PLANE_BOUNDS void fft_plane_r2c_pro(FieldPtrs<T> out, FieldPtr
   const T *ga = a.p[field] ? a.p[field] + base : nullptr, *gb = b.p[field] + base;
   const T *gc = c.p[field] ? c.p[field] + base : nullptr;
   const bool diff = (dsq >> field) & 1;
+  const T ef = e.v[field];
   plane_tw_load<T, NY, NZ, THREADS>(tw, twm, zrot, twy);
   __syncthreads();
   KAPSEL_PLANE_ROWS(C) {
@@ -461,6 +587,7 @@ __global__ KAPSEL_PLANE_BOUNDS void fft_plane_r2c_pro(FieldPtrs<T> out, FieldPtr
     for (int n1 = 0; n1 < C::ZR1; n1++) {
       const size_t o = ro + 2 * (C::ZR2 * n1 + n2);
       V2<T> vb = ld2(gb + o);
+      vb = V2<T>{vb.re + ef, vb.im + ef};
       if (gc) {
         const V2<T> vc = ld2(gc + o);
         vb = diff ? V2<T>{(vb.re - vc.re) * (vb.re + vc.re), (vb.im - vc.im) * (vb.im + vc.im)}
@@ -921,6 +1048,16 @@ void lines_dispatch(kapsel_ctx *ctx, int cls, int L, const FieldPtrs<T> &fp, int
 
 constexpr int rows_per_block(size_t elem) { return elem == 8 ? 16 : 32; }
 
+// fft_z_r2c_pro reads and writes a block's rows 16 bytes at a time
+template <typename T>
+void check_z_align(const FieldPtrs<T> &fp, int nf) {
+  for (int i = 0; i < nf; i++)
+    if (fp.p[i] && ((uintptr_t)fp.p[i] & 15u)) {
+      set_error("field pointer %p is not 16-byte aligned", (void *)fp.p[i]);
+      throw Error{KAPSEL_ERR_ARG};
+    }
+}
+
 template <typename T, int NZ>
 void launch_z(kapsel_ctx *ctx, const FieldPtrs<T> &fp, int nf, bool fwd) {
   constexpr int RPB = rows_per_block(sizeof(T));
@@ -1060,42 +1197,73 @@ void fft_r2c_t(kapsel_ctx *ctx, void *const *fields, int nf) {
   lines_dispatch<T, -1>(ctx, KAPSEL_K_FFT_LINES_FWD, g.nx, fp, nf, plane, plane, 0, plane, tw.x);
 }
 
-// out = a + s b c (as fft_plane_r2c_pro) transformed forward, where a plane
-// pass applies; returns false (nothing queued) otherwise
+// out = a + s (b + e) c (as fft_plane_r2c_pro / fft_z_r2c_pro) transformed
+// forward: the product formed in the first pass (the plane pass, or the z
+// pass followed by the y lines), then the x lines.  e NULL: all e_f = 0.
+// Returns false (nothing queued) for an NZ without a z pass.
 template <typename T>
 bool fft_r2c_pro_t(kapsel_ctx *ctx, void *const *out, void *const *a, void *const *b, void *const *c, double s,
-                   int nf, unsigned dsq) {
+                   int nf, unsigned dsq, const double *e) {
   const Grid &g = ctx->g;
-  if (!plane_ok<T>(g)) return false;
+  const bool plane = plane_ok<T>(g);
+  if (!plane && !(g.nz == 16 || g.nz == 32 || g.nz == 64 || g.nz == 128 || g.nz == 256)) return false;
   const Twiddles &tw = ctx->tw[prec_of<T>()];
   FieldPtrs<T> fo = pack<T>(out, nf), fb = pack<T>(b, nf);
   FieldPtrs<T> fa = a ? pack<T>(a, nf) : FieldPtrs<T>{};
   FieldPtrs<T> fc = c ? pack<T>(c, nf) : FieldPtrs<T>{};
-  dim3 grid((unsigned)g.nx, nf);
+  FieldConsts<T> fe{};
+  for (int i = 0; i < nf; i++) fe.v[i] = e ? (T)e[i] : (T)0;
   {
     LaunchTimer lt(ctx, KAPSEL_K_FFT_Z_R2C, (double)((a ? 1 : 0) + (c ? 3 : 2)) * nf * g.f * sizeof(T));
-    switch (g.nz) {
-#define KAPSEL_PRO(N, TH)                                                                                      \
-  case N:                                                                                                      \
-    hipLaunchKernelGGL((fft_plane_r2c_pro<T, N, N, TH>), grid, dim3(TH), 0, ctx->stream, fo, fa, fb, fc, (T)s, \
-                       dsq, (const V2<T> *)tw.zh, (const V2<T> *)tw.zrot, (const V2<T> *)tw.y);                \
+    if (plane) {
+      dim3 grid((unsigned)g.nx, nf);
+      switch (g.nz) {
+#define KAPSEL_PRO(N, TH)                                                                                 \
+  case N:                                                                                                 \
+    hipLaunchKernelGGL((fft_plane_r2c_pro<T, N, N, TH>), grid, dim3(TH), 0, ctx->stream, fo, fa, fb, fc, fe, \
+                       (T)s, dsq, (const V2<T> *)tw.zh, (const V2<T> *)tw.zrot, (const V2<T> *)tw.y);     \
     break;
-      KAPSEL_PRO(32, 128)
-      KAPSEL_PRO(64, 320)
-      KAPSEL_PRO(128, 512)
+        KAPSEL_PRO(32, 128)
+        KAPSEL_PRO(64, 320)
+        KAPSEL_PRO(128, 512)
 #undef KAPSEL_PRO
-      default: return false;
+        default: return false;
+      }
+    } else {
+      check_z_align(fo, nf);
+      check_z_align(fa, nf);
+      check_z_align(fb, nf);
+      check_z_align(fc, nf);
+      constexpr int RPB = rows_per_block(sizeof(T));
+      const long long rows = (long long)g.nx * g.ny;
+      dim3 grid((unsigned)((rows + RPB - 1) / RPB), nf);
+      switch (g.nz) {
+#define KAPSEL_PRO(N)                                                                                           \
+  case N:                                                                                                       \
+    hipLaunchKernelGGL((fft_z_r2c_pro<T, N, RPB>), grid, dim3(RPB * FS<N / 2>::R2), 0, ctx->stream, fo, fa, fb, \
+                       fc, fe, (T)s, dsq, rows, g.nz_, (const V2<T> *)tw.zh, (const V2<T> *)tw.zrot);          \
+    break;
+        KAPSEL_PRO(16)
+        KAPSEL_PRO(32)
+        KAPSEL_PRO(64)
+        KAPSEL_PRO(128)
+        KAPSEL_PRO(256)
+#undef KAPSEL_PRO
+      }
     }
     KAPSEL_LAUNCH_CHECK();
   }
-  long long plane = (long long)g.ny * g.h;
-  lines_dispatch<T, -1>(ctx, KAPSEL_K_FFT_LINES_FWD, g.nx, fo, nf, plane, plane, 0, plane, tw.x);
+  if (!plane)
+    lines_dispatch<T, -1>(ctx, KAPSEL_K_FFT_LINES_FWD, g.ny, fo, nf, (long long)g.nx * g.h, g.h,
+                          (long long)g.ny * g.h, g.h, tw.y);
+  const long long pl = (long long)g.ny * g.h;
+  lines_dispatch<T, -1>(ctx, KAPSEL_K_FFT_LINES_FWD, g.nx, fo, nf, pl, pl, 0, pl, tw.x);
   return true;
 }
 template bool fft_r2c_pro_t<float>(kapsel_ctx *, void *const *, void *const *, void *const *, void *const *, double,
-                                   int, unsigned);
+                                   int, unsigned, const double *);
 template bool fft_r2c_pro_t<double>(kapsel_ctx *, void *const *, void *const *, void *const *, void *const *,
-                                    double, int, unsigned);
+                                    double, int, unsigned, const double *);
 
 
 template <typename T>

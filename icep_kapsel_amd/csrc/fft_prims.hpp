@@ -90,6 +90,11 @@ template <typename T>
 struct FieldPtrs {
   T *p[6];
 };
+// one constant per field of a launch
+template <typename T>
+struct FieldConsts {
+  T v[6];
+};
 
 // XCD-aware tile order.  Workgroups are dispatched round-robin over the 8
 // XCDs by linear id (blockIdx.y * gridDim.x + blockIdx.x), and each XCD has

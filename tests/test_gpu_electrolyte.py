@@ -174,13 +174,15 @@ def test_electrolyte_step_fp64_matches_oracle(ac):
     assert max(m1.values()) < 1e-8 and max(m10.values()) < 1e-8
 
 
-def test_electrolyte_step_reuse_matches_recompute():
+@pytest.mark.parametrize("prec,bound", [(abi.KAPSEL_FP64, 1e-12), (abi.KAPSEL_FP32, 1e-5)])
+def test_electrolyte_step_reuse_matches_recompute(prec, bound):
     """Consecutive steps reuse what the previous step left (charge spectrum,
-    real ion fields, real velocity); a state-changing call in between makes
-    the next step recompute them.  Both ways agree to rounding."""
+    real ion fields, the velocity: real in FP64, its spectrum in FP32); a
+    state-changing call in between makes the next step recompute them.  Both
+    ways agree to rounding: 1e-12 in FP64, 1e-5 in FP32 after 6 steps."""
     n, npart = 64, 8
-    print(f"case electrolyte reuse vs recompute {n}^3", flush=True)
-    p, st, el = _pair(n, npart, q=60.0, e_ext=(0.05, 0.0, 0.02))
+    print(f"case electrolyte reuse vs recompute {n}^3 precision {prec}", flush=True)
+    p, st, el = _pair(n, npart, q=60.0, e_ext=(0.05, 0.0, 0.02), prec=prec)
     ks = [gpu(p), gpu(p)]
     _load(ks, st, el)
     for x in ks:
@@ -200,7 +202,7 @@ def test_electrolyte_step_reuse_matches_recompute():
     for f in ("X", "V", "OMEGA"):
         e[f] = rel(pa[abi.PFIELD[f]], pb[abi.PFIELD[f]])
     print({f: f"{v:.1e}" for f, v in e.items()}, flush=True)
-    assert max(e.values()) < 1e-12, e
+    assert max(e.values()) < bound, e
 
 
 def test_electrolyte_fp32_against_fp64():
