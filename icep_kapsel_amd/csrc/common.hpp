@@ -330,6 +330,9 @@ template <typename T>
 void fft_r2c_t(kapsel_ctx *ctx, void *const *fields, int nf);
 template <typename T>
 void fft_c2r_t(kapsel_ctx *ctx, void *const *fields, int nf, void *const *src = nullptr);
+// fft_c2r_t after its first (x, inverse) pass
+template <typename T>
+void fft_c2r_yz_t(kapsel_ctx *ctx, void *const *fields, int nf);
 // out_f = a_f + s (b_f + e_f) c_f transformed forward, the product formed in
 // the first pass (a or c NULL: left out; bit f of dsq: (b_f - c_f)(b_f + c_f)
 // instead of the product; e NULL: e_f = 0).  out_f may be one of field f's

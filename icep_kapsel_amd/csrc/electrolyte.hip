@@ -219,8 +219,9 @@ __global__ void flux_k_kernel(T *__restrict__ ck, T *__restrict__ f0, T *__restr
   const KIdx k = kidx(g, c);
   V2<T> a = ld2(ck + 2 * c);
   if (is_cut(g, k)) {
+    // (a spectrum the previous Euler step left is zero there already)
+    if (a.re != (T)0 || a.im != (T)0) st2(ck + 2 * c, V2<T>{0, 0});
     a = V2<T>{0, 0};
-    st2(ck + 2 * c, a);
   }
   const V2<T> w{(T)kl * a.re, (T)kl * a.im};
   const T wx = (T)(k.kx * g.wave_x), wy = (T)(k.ky * g.wave_y), wz = (T)(k.kz * g.wave_z);
@@ -282,6 +283,89 @@ __global__ void flux_x2_kernel(FieldPtrs<T> f, const T *__restrict__ cx0, const 
     f1[im] = a1;
     f2[im] = a2;
   }
+}
+
+// The first (x, inverse) pass of the inverse transforms of the Coulomb
+// potential's gradient (coulomb_grad_kernel), the symbol fused in: workgroup
+// field d (grid.y) forms out_d = IFFT_x(i k_d b) - dc_d on the line of k = 0,
+// b = a IK2 p0 (cut modes zeroed if trunc), as it reads the line of a (x line
+// q = j*h + kz: elements q + i*plane), operation for operation as the
+// pointwise kernel.  The fields of one tile run at consecutive ids
+// (xcd_tile_by_tile), so a's lines come from HBM once.  FP32 only (the FP64
+// context keeps the separate passes).  Against the separate kernel and pass
+// (43 + 76 us at 256^3): 79 us with a just written, 120 us with a cold.
+template <typename T, int L, int LPB>
+__global__ __launch_bounds__((LPB * LineFFT<T, L>::S)) void el_x_grad_kernel(
+    const T *__restrict__ a, FieldPtrs<T> out, long long Q, long long plane, Grid g, double p0, int trunc,
+    double dc0, double dc1, double dc2, const V2<T> *__restrict__ tw) {
+  using F = LineFFT<T, L>;
+  constexpr int P = F::P;
+  __shared__ V2<T> sm[LPB * F::LS];
+  unsigned d;
+  const unsigned tile = xcd_tile_by_tile(d);
+  const int t = threadIdx.x;
+  const int l = t % LPB, s = t / LPB;
+  const long long q = (long long)tile * LPB + l;
+  const bool valid = q < Q;
+  const long long base = valid ? q : 0;
+  KIdx k;
+  k.j = (int)(base / g.h);
+  k.kz = (int)(base - (long long)k.j * g.h);
+  k.ky = k.j > g.hny ? k.j - g.ny : k.j;
+  V2<T> v[P], twr[P];
+#pragma unroll
+  for (int n = 0; n < P; n++) {
+    k.i = F::in_i(s, n);
+    k.kx = k.i > g.hnx ? k.i - g.nx : k.i;
+    V2<T> x = ld2(a + 2 * (base + (long long)k.i * plane));
+    const double wx = k.kx * g.wave_x, wy = k.ky * g.wave_y, wz = k.kz * g.wave_z;
+    const double k2 = wx * wx + wy * wy + wz * wz;
+    const double ik2 = k2 > 0.0 ? 1. / k2 : 0.0;
+    const T sc = (T)(ik2 * p0);
+    x = V2<T>{x.re * sc, x.im * sc};
+    if (trunc && is_cut(g, k)) x = V2<T>{0, 0};
+    const T w = d == 0 ? (T)wx : d == 1 ? (T)wy : (T)wz;
+    x = V2<T>{-w * x.im, w * x.re};
+    if (base == 0 && k.i == 0) x.re -= (T)(d == 0 ? dc0 : d == 1 ? dc1 : dc2);
+    v[n] = x;
+  }
+  line_twiddles<F>(twr, tw, s);
+  F::template run<+1>(v, sm + l * F::LS, twr, s, LPB);
+  if (valid) {
+    T *__restrict__ od = out.p[d];
+#pragma unroll
+    for (int m = 0; m < F::P; m++) st2(od + 2 * (base + (long long)F::out_i(s, m) * plane), v[m]);
+  }
+}
+
+// coulomb_grad_t's kernel and the first pass of the inverse transform of g3
+template <typename T>
+void el_x_grad(kapsel_ctx *ctx, const T *rhok, void *const *g3, int trunc, const double dc[3]) {
+  static_assert(sizeof(T) == 4, "FP32 only");
+  const Grid &g = ctx->g;
+  constexpr int LPB = 32;
+  const long long plane = (long long)g.ny * g.h;
+  FieldPtrs<T> fo{};
+  for (int i = 0; i < 3; i++) fo.p[i] = (T *)g3[i];
+  const V2<T> *tw = (const V2<T> *)ctx->tw[prec_of<T>()].x;
+  const dim3 grid((unsigned)((plane + LPB - 1) / LPB), 3);
+  const double p0 = 1. / ctx->el.dielectric_cst;
+  LaunchTimer lt(ctx, KAPSEL_K_FFT_LINES_INV, 4.0 * g.f * sizeof(T));
+  switch (g.nx) {
+#define KAPSEL_XGRAD(L)                                                                                     \
+  case L:                                                                                                   \
+    hipLaunchKernelGGL((el_x_grad_kernel<T, L, LPB>), grid, dim3(LPB * LineFFT<T, L>::S), 0, ctx->stream,     \
+                       rhok, fo, plane, plane, g, p0, trunc, dc[0], dc[1], dc[2], tw);                      \
+    break;
+    KAPSEL_XGRAD(16)
+    KAPSEL_XGRAD(32)
+    KAPSEL_XGRAD(64)
+    KAPSEL_XGRAD(128)
+    KAPSEL_XGRAD(256)
+#undef KAPSEL_XGRAD
+    default: set_error("unsupported NX %d", g.nx); throw Error{KAPSEL_ERR_ARG};
+  }
+  KAPSEL_LAUNCH_CHECK();
 }
 
 // psi_k = rho_k IK2 / Dielectric_cst (Charge_field_k2Coulomb_potential_k_PBC,
@@ -618,6 +702,10 @@ void nssolute_t(kapsel_ctx *ctx, const kapsel_time &t, bool reuse, bool ions_onl
                          face[1], face[2], grad[0], grad[1], grad[2], g);
       KAPSEL_LAUNCH_CHECK();
     }
+  } else if constexpr (sizeof(T) == 4) {
+    // coulomb_grad_t with its inverse transform, the first pass fused
+    el_x_grad<T>(ctx, rhok, (void *const *)grad, 1, dc);
+    fft_c2r_yz_t<T>(ctx, (void *const *)grad, 3);
   } else {
     coulomb_grad_t<T>(ctx, rhok, grad, 1, dc);
     fft_c2r_t<T>(ctx, (void *const *)grad, 3);
@@ -641,6 +729,9 @@ void nssolute_t(kapsel_ctx *ctx, const kapsel_time &t, bool reuse, bool ions_onl
   for (int n = 0; n < 2; n++) {
     T *ck = fl<T>(ctx, KAPSEL_CONC0 + n);
     T *cx = ef<T>(ctx, E_CX0 + n);
+    void *four[4] = {flux.p[3 * n], flux.p[3 * n + 1], flux.p[3 * n + 2], cx};
+    // (fused into the first inverse pass, with the line read once per output
+    // field, this measured slower at 256^3 FP32: 150 us against 60 + 76 us)
     {
       LaunchTimer lt(ctx, KAPSEL_K_POINTWISE,
                      (2.0 * g.f + 2.0 * cut_count_el(g) + (reuse ? 3.0 : 4.0) * g.f) * sizeof(T));
@@ -649,7 +740,6 @@ void nssolute_t(kapsel_ctx *ctx, const kapsel_time &t, bool reuse, bool ions_onl
                          ctx->el.kbt * ctx->el.onsager_coeff[n], g);
       KAPSEL_LAUNCH_CHECK();
     }
-    void *four[4] = {flux.p[3 * n], flux.p[3 * n + 1], flux.p[3 * n + 2], cx};
     fft_c2r_t<T>(ctx, four, reuse ? 3 : 4);
   }
   {
@@ -724,8 +814,13 @@ void coulomb_force_t(kapsel_ctx *ctx, const kapsel_time &t, int scaled, bool spe
   fft_r2c_t<T>(ctx, one, 1);
   T *f[3] = {fl<T>(ctx, KAPSEL_F0), fl<T>(ctx, KAPSEL_F1), fl<T>(ctx, KAPSEL_F2)};
   const double zero[3] = {0.0, 0.0, 0.0};
-  coulomb_grad_t<T>(ctx, rhok, f, 0, zero);
-  fft_c2r_t<T>(ctx, (void *const *)f, 3);
+  if constexpr (sizeof(T) == 4) {
+    el_x_grad<T>(ctx, rhok, (void *const *)f, 0, zero);
+    fft_c2r_yz_t<T>(ctx, (void *const *)f, 3);
+  } else {
+    coulomb_grad_t<T>(ctx, rhok, f, 0, zero);
+    fft_c2r_t<T>(ctx, (void *const *)f, 3);
+  }
   const double s = ac_factor(ctx, t);
   if (spectrum) {
     // f_d = -((grad_d psi - E_d) rho) = rho (E_d - grad_d psi)

@@ -1285,6 +1285,21 @@ void fft_c2r_t(kapsel_ctx *ctx, void *const *fields, int nf, void *const *src) {
   }
 }
 
+// fft_c2r_t after its first (x, inverse) pass: the y and z inverse passes
+template <typename T>
+void fft_c2r_yz_t(kapsel_ctx *ctx, void *const *fields, int nf) {
+  const Grid &g = ctx->g;
+  const Twiddles &tw = ctx->tw[prec_of<T>()];
+  FieldPtrs<T> fp = pack<T>(fields, nf);
+  if (!plane_dispatch<T>(ctx, fp, nf, false)) {
+    lines_dispatch<T, +1>(ctx, KAPSEL_K_FFT_LINES_INV, g.ny, fp, nf, (long long)g.nx * g.h, g.h,
+                          (long long)g.ny * g.h, g.h, tw.y);
+    z_dispatch<T>(ctx, fp, nf, false);
+  }
+}
+template void fft_c2r_yz_t<float>(kapsel_ctx *, void *const *, int);
+template void fft_c2r_yz_t<double>(kapsel_ctx *, void *const *, int);
+
 template <typename T>
 void fft_face_t(kapsel_ctx *ctx, void *const *g3, void *const *face3) {
   const Grid &g = ctx->g;
