@@ -100,9 +100,11 @@ static void check_field(int f) {
   }
 }
 
-// a write to the FP32 potential through the API supersedes its FP64 shadow
+// a write to the FP32 potential through the API supersedes its FP64 shadow;
+// the free charge is no longer known to be zero once written
 static void written(kapsel_ctx *ctx, int f) {
   if (f == KAPSEL_POTENTIAL) ctx->pot64_valid = false;
+  if (f == KAPSEL_RHO_FREE) ctx->rho_free_set = true;
 }
 
 static void destroy(kapsel_ctx *ctx) {
@@ -624,7 +626,8 @@ static void dielectric_potential(kapsel_ctx *ctx, const double e0[3], double tol
     kapsel_ctx *c;
     ~Background() { c->rho_bg = nullptr; }
   } bg{ctx};
-  ctx->rho_bg = dielectric_free_charge(ctx);
+  ctx->rho_bg = dielectric_free_charge(ctx);  // writes RHO_FREE
+  ctx->rho_free_set = true;
   poisson_solve(ctx, e0, tol, maxit, precond, iters, relres);
 }
 
@@ -908,6 +911,7 @@ int kapsel_solenoidal_u(kapsel_ctx *ctx, int field0) {
   check_ctx(ctx);
   check_field(field0);
   check_field(field0 + 2);
+  for (int d = 0; d < 3; d++) written(ctx, field0 + d);
   solenoidal_u(ctx, ctx->fields + field0);
   API_END
 }
@@ -938,6 +942,7 @@ int kapsel_make_phi_particle(kapsel_ctx *ctx, int field) {
   API_BEGIN
   check_ctx(ctx);
   check_field(field);
+  written(ctx, field);
   make_phi_particle(ctx, ctx->fields[field]);
   API_END
 }

@@ -209,6 +209,10 @@ struct kapsel_ctx {
   bool spec64_fresh = false;
   bool refine64 = false;  // inside the FP64 part of an FP32 solve: launches time as KAPSEL_K_REFINE64
   bool prev_valid = false;
+  // RHO_FREE has been written (through the API or by the dielectric
+  // electrolyte's free charge) since the context was created zeroed; until
+  // then an FP32-context solve takes rho = 0 without transforming it
+  bool rho_free_set = false;
   double *x64_prev = nullptr;
   double *d_partial = nullptr;    // block partial sums (reductions)
   double *d_scal = nullptr;       // CG scalars

@@ -227,7 +227,7 @@ __device__ __forceinline__ double block_sum(double v) {
 }
 
 // The refinement's set-up in one pass (FP32 context): FP64 copies of eps
-// and rho, the face permittivities 0.5 (eps_i + eps_{i+e_d}) of the FP32
+// and rho (rho NULL: known zero, not copied), the face permittivities 0.5 (eps_i + eps_{i+e_d}) of the FP32
 // correction operator and of the FP64 residual operator (0 in the row
 // padding, which the plane pass multiplies too), and per-block sums of eps,
 // whose mean scales the preconditioner
@@ -250,7 +250,7 @@ __global__ __launch_bounds__(TPB) void refine_setup_kernel(const float *__restri
     const float en = eps[e];
     const float ex = eps[nbr(g, i + 1, j, k)], ey = eps[nbr(g, i, j + 1, k)], ez = eps[nbr(g, i, j, k + 1)];
     eps64[e] = (double)en;
-    rho64[e] = (double)rho[e];
+    if (rho) rho64[e] = (double)rho[e];
     a0[e] = 0.5f * (ex + en);
     a1[e] = 0.5f * (ey + en);
     a2[e] = 0.5f * (ez + en);
@@ -528,7 +528,8 @@ __global__ __launch_bounds__(TPB) void kres_kernel(const T *__restrict__ w0, con
 // average E0_d (eps_i + eps_{i+e_d}) / 2 transforms to E0_d eps_k
 // (1 + e^{i k_d dx}) / 2, and the shifted divergence multiplies it by
 // i k_d e^{-i k_d dx/2}: b_k = i eps_k sum_d k_d E0_d cos(k_d dx / 2) - rho_k,
-// with the background bg (when set) added to the k = 0 mode as nr bg.
+// with the background bg (when set) added to the k = 0 mode as nr bg
+// (rhok NULL: rho = 0).
 // Partial sums of ||b||^2 (Parseval weights) per block.
 __global__ __launch_bounds__(TPB) void rhs_k_kernel(const double *__restrict__ epsk, const double *__restrict__ rhok,
                                                     double *__restrict__ b, const V2<double> *__restrict__ shx,
@@ -541,7 +542,7 @@ __global__ __launch_bounds__(TPB) void rhs_k_kernel(const double *__restrict__ e
     const KIdx k = kidx(g, c);
     const double f = k.kx * g.wave_x * e0x * shx[k.i].re + k.ky * g.wave_y * e0y * shy[k.j].re +
                      k.kz * g.wave_z * e0z * shz[k.kz].re;
-    const V2<double> e = ld2(epsk + 2 * c), q = ld2(rhok + 2 * c);
+    const V2<double> e = ld2(epsk + 2 * c), q = rhok ? ld2(rhok + 2 * c) : V2<double>{0, 0};
     V2<double> v{-f * e.im - q.re, f * e.re - q.im};
     if (c == 0 && bg) v.re += (double)g.nr * bg[0];
     st2(b + 2 * c, v);
@@ -1117,13 +1118,15 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
   float *d32 = (float *)ctx->pscratch[4], *eps32 = fld<float>(ctx, KAPSEL_EPSILON);
   const unsigned fb = blocks_for(g.f, TPB);
   float *face32[3] = {(float *)ctx->pscratch[5], (float *)ctx->pscratch[6], (float *)ctx->pscratch[7]};
+  // a free charge never written is zero: no FP64 copy, no transform
+  const bool rho_zero = !ctx->rho_free_set && !ctx->rho_bg;
   // FP64 copies of the FP32 inputs (eps and rho enter the residual in FP64),
   // the face factors of both operators, 1 / mean(eps) for the preconditioner
   {
     LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, (4.0 + 4.0 + 8.0 + 8.0 + 12.0 + 24.0) * g.f);
-    hipLaunchKernelGGL(refine_setup_kernel, dim3(NPART), dim3(TPB), 0, ctx->stream, eps32,
-                       fld<float>(ctx, KAPSEL_RHO_FREE), eps64, rk, face32[0], face32[1], face32[2], face64[0],
-                       face64[1], face64[2], ctx->d_partial, g);
+    const float *rho32 = rho_zero ? nullptr : fld<float>(ctx, KAPSEL_RHO_FREE);
+    hipLaunchKernelGGL(refine_setup_kernel, dim3(NPART), dim3(TPB), 0, ctx->stream, eps32, rho32, eps64, rk,
+                       face32[0], face32[1], face32[2], face64[0], face64[1], face64[2], ctx->d_partial, g);
     KAPSEL_LAUNCH_CHECK();
     hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->d_partial, NPART, ctx->d_scal, S_TMP,
                        0);
@@ -1151,10 +1154,11 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
   }
   // eps64 and rho as spectra
   void *er[2] = {eps64, rk};
-  fft_r2c_t<double>(ctx, er, 2);
+  fft_r2c_t<double>(ctx, er, rho_zero ? 1 : 2);
   {
     LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 3, sizeof(double)));
-    hipLaunchKernelGGL(rhs_k_kernel, dim3(NPART), dim3(TPB), 0, ctx->stream, eps64, rk, bk, sh<double>(ctx, 0),
+    const double *rho_k = rho_zero ? nullptr : rk;
+    hipLaunchKernelGGL(rhs_k_kernel, dim3(NPART), dim3(TPB), 0, ctx->stream, eps64, rho_k, bk, sh<double>(ctx, 0),
                        sh<double>(ctx, 1), sh<double>(ctx, 2), e0[0], e0[1], e0[2], ctx->rho_bg, ctx->d_partial, g);
     KAPSEL_LAUNCH_CHECK();
     hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->d_partial, NPART, ctx->d_scal, S_BB,

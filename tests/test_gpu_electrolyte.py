@@ -42,9 +42,10 @@ def _random_conc(n, rng, base, amp, kmax=5):
 
 
 def _pair(n, npart, q=40.0, e_ext=(0.02, -0.01, 0.005), seed=17, prec=abi.KAPSEL_FP64, ac=0):
+    shape = n if isinstance(n, tuple) else (n, n, n)
     rng = workloads.Xoshiro256(seed)
-    p = abi.make_params(n, n_particles=npart, radius=4.0, xi=2.0, precision=prec)
-    pos = workloads.random_positions(rng, npart, (n, n, n), 8.0)
+    p = abi.make_params(shape, n_particles=npart, radius=4.0, xi=2.0, precision=prec)
+    pos = workloads.random_positions(rng, npart, shape, 8.0)
     el = abi.make_electrolyte(surface_charge=q, debye_length=4.0, e_ext=e_ext, ac=ac, angular_frequency=0.7)
     el.onsager_coeff[:] = [7.24, 5.0]
     st = workloads.particle_state(pos)
@@ -227,6 +228,42 @@ def test_electrolyte_fp32_against_fp64():
         # spectrum of u; its k = 0 mode (mean flow) separately, relative to itself
         a, b = k32.get_field(f).copy(), k64.get_field(f).copy()
         e[f + " k=0"] = float(abs(a[0, 0, 0] - b[0, 0, 0]) / abs(b[0, 0, 0]))
+        a[0, 0, 0] = b[0, 0, 0] = 0.0
+        e[f] = rel(a, b)
+    p64, p32 = k64.get_particles(), k32.get_particles()
+    for f in ("V", "OMEGA", "X"):
+        i = abi.PFIELD[f]
+        e[f] = rel(p32[i], p64[i])
+    print("fp32 vs fp64 after 10 steps", {f: f"{v:.1e}" for f, v in e.items()}, flush=True)
+    assert e["V"] < 1e-5 and max(e["U0"], e["U1"], e["U2"]) < 1e-5
+    assert max(e["CONC0"], e["CONC1"]) < 1e-5
+
+
+def test_electrolyte_fp32_against_fp64_noncubic():
+    """64 x 32 x 64 (NY != NZ: no plane pass, so the FP32 step runs the z
+    passes with their prologues and the fused x pass of the Coulomb
+    gradient at these lengths): FP32 against FP64 after 10 steps, the same
+    bounds as at 128^3 (particle and fluid velocity 1e-5, ion spectra
+    1e-5)."""
+    shape, npart = (64, 32, 64), 6
+    print(f"case electrolyte fp32 vs fp64 {shape} {npart} spheres", flush=True)
+    ks = []
+    for prec in (abi.KAPSEL_FP64, abi.KAPSEL_FP32):
+        p, st, el = _pair(shape, npart, q=60.0, e_ext=(0.02, 0.01, 0.0), prec=prec)
+        k = gpu(p)
+        _load((k,), st, el)
+        k.init_rigidity()
+        k.init_rho_ion()
+        t = abi.make_time(workloads.electrolyte_auto_dt(p, el))
+        for _ in range(10):
+            k.electrolyte_step(t)
+        ks.append(k)
+    k64, k32 = ks
+    e = {}
+    for f in ("CONC0", "CONC1", "ZETA0", "ZETA1"):
+        e[f] = rel(k32.get_field(f), k64.get_field(f))
+    for f in ("U0", "U1", "U2"):
+        a, b = k32.get_field(f).copy(), k64.get_field(f).copy()
         a[0, 0, 0] = b[0, 0, 0] = 0.0
         e[f] = rel(a, b)
     p64, p32 = k64.get_particles(), k32.get_particles()
