@@ -794,12 +794,13 @@ __device__ __forceinline__ double block_sum_fft(double v) {
 }
 
 // p = z + sc[beta] p (stored), then g_d = IFFT_x(i k_d e^{i k_d dx/2} p), d = x, y, z
-// (thread t of a tile: line t % LPB, its thread s = t / LPB)
+// (thread t of a tile: line t % LPB, its thread s = t / LPB); with sub (and
+// no beta): p = z - sub (stored), the refinement's x -= d before its residual
 template <typename T, int L, int LPB>
 __global__ __launch_bounds__((LPB * LineFFT<T, L>::S)) void fft_x_grad(
     const T *z, T *p, const double *__restrict__ sc, int beta_slot, FieldPtrs<T> g3, long long Q, long long plane,
     int h, int ny, double wx, double wy, double wz, const V2<T> *__restrict__ shx, const V2<T> *__restrict__ shy,
-    const V2<T> *__restrict__ shz, const V2<T> *__restrict__ tw) {
+    const V2<T> *__restrict__ shz, const V2<T> *__restrict__ tw, const float *__restrict__ sub) {
   using F = LineFFT<T, L>;
   constexpr int P = F::P;
   __shared__ V2<T> sm[LPB * F::LS];
@@ -821,6 +822,10 @@ __global__ __launch_bounds__((LPB * LineFFT<T, L>::S)) void fft_x_grad(
       pv[n] = ld2(z + 2 * off);
       if (update) {
         pv[n] = pv[n] + scal(beta, ld2(p + 2 * off));
+        st2(p + 2 * off, pv[n]);
+      } else if (sub) {
+        const V2<float> d = ld2(sub + 2 * off);
+        pv[n] = V2<T>{pv[n].re - (T)d.re, pv[n].im - (T)d.im};
         st2(p + 2 * off, pv[n]);
       }
     } else {
@@ -1348,7 +1353,8 @@ void fft_face_mid_t(kapsel_ctx *ctx, void *const *g3, void *const *face3) {
 }
 
 template <typename T, int L>
-void launch_x_grad(kapsel_ctx *ctx, const T *z, T *p, const double *sc, int beta_slot, void *const *g3) {
+void launch_x_grad(kapsel_ctx *ctx, const T *z, T *p, const double *sc, int beta_slot, void *const *g3,
+                   const float *sub) {
   constexpr int LPB = lines_per_block(sizeof(T));
   const Grid &g = ctx->g;
   const Twiddles &tw = ctx->tw[prec_of<T>()];
@@ -1357,7 +1363,7 @@ void launch_x_grad(kapsel_ctx *ctx, const T *z, T *p, const double *sc, int beta
   hipLaunchKernelGGL((fft_x_grad<T, L, LPB>), dim3((unsigned)((Q + LPB - 1) / LPB)), dim3(LPB * LineFFT<T, L>::S), 0,
                      ctx->stream, z, p, sc, beta_slot, pack<T>(g3, 3), Q, Q, g.h, g.ny, g.wave_x, g.wave_y,
                      g.wave_z, (const V2<T> *)tw.shx, (const V2<T> *)tw.shy, (const V2<T> *)tw.shz,
-                     (const V2<T> *)tw.x);
+                     (const V2<T> *)tw.x, sub);
   KAPSEL_LAUNCH_CHECK();
 }
 
@@ -1377,10 +1383,11 @@ int launch_x_div(kapsel_ctx *ctx, void *const *g3, const T *p, T *ap, double *pa
 }
 
 template <typename T>
-bool fft_x_grad_t(kapsel_ctx *ctx, const T *z, T *p, const double *sc, int beta_slot, void *const *g3) {
+bool fft_x_grad_t(kapsel_ctx *ctx, const T *z, T *p, const double *sc, int beta_slot, void *const *g3,
+                  const float *sub) {
   switch (ctx->g.nx) {
-    case 64: launch_x_grad<T, 64>(ctx, z, p, sc, beta_slot, g3); return true;
-    case 128: launch_x_grad<T, 128>(ctx, z, p, sc, beta_slot, g3); return true;
+    case 64: launch_x_grad<T, 64>(ctx, z, p, sc, beta_slot, g3, sub); return true;
+    case 128: launch_x_grad<T, 128>(ctx, z, p, sc, beta_slot, g3, sub); return true;
     default: return false;
   }
 }
@@ -1440,8 +1447,10 @@ bool fft_grad2_f64_to_f32(kapsel_ctx *ctx, const double *a, const double *b, voi
 template void fft_face_mid_t<double>(kapsel_ctx *, void *const *, void *const *);
 template int fft_x_res_t<double>(kapsel_ctx *, void *const *, const double *, double *, double *);
 template void fft_face_mid_t<float>(kapsel_ctx *, void *const *, void *const *);
-template bool fft_x_grad_t<float>(kapsel_ctx *, const float *, float *, const double *, int, void *const *);
-template bool fft_x_grad_t<double>(kapsel_ctx *, const double *, double *, const double *, int, void *const *);
+template bool fft_x_grad_t<float>(kapsel_ctx *, const float *, float *, const double *, int, void *const *,
+                                  const float *);
+template bool fft_x_grad_t<double>(kapsel_ctx *, const double *, double *, const double *, int, void *const *,
+                                   const float *);
 template int fft_x_div_t<float>(kapsel_ctx *, void *const *, const float *, float *, double *);
 template int fft_x_div_t<double>(kapsel_ctx *, void *const *, const double *, double *, double *);
 template void fft_face_t<double>(kapsel_ctx *, void *const *, void *const *);

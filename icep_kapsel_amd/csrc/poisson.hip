@@ -21,8 +21,8 @@
 // each correction is an FP32 (preconditioned) CG solve, so the reported
 // residual is an FP64 residual.  All reductions accumulate in FP64.
 #include <cmath>
-
 #include <functional>
+#include <utility>
 
 #include "common.hpp"
 
@@ -286,7 +286,7 @@ __global__ __launch_bounds__(TPB) void dot_partial_kernel(const T *__restrict__ 
 __global__ __launch_bounds__(TPB) void finish_kernel(const double *__restrict__ partial, int n,
                                                      double *__restrict__ scal, int slot, int mode,
                                                      double *host = nullptr, int hslot = 0,
-                                                     unsigned long long *cnt = nullptr) {
+                                                     unsigned long long *cnt = nullptr, double count = 0.0) {
   double s = 0.0;
   for (int i = threadIdx.x; i < n; i += TPB) s += partial[i];
   s = block_sum(s);
@@ -296,6 +296,7 @@ __global__ __launch_bounds__(TPB) void finish_kernel(const double *__restrict__ 
       scal[S_BETA] = old != 0.0 ? s / old : 0.0;
       scal[S_RZ] = s;
     }
+    if (mode == 2) scal[S_INVEPS] = s > 0.0 ? count / s : 1.0;  // 1 / mean of a sum over count points
     scal[slot] = s;
     if (host) {
       host[hslot] = s;
@@ -552,13 +553,13 @@ __global__ __launch_bounds__(TPB) void rhs_k_kernel(const double *__restrict__ e
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// over every entry of a field (a half spectrum here): x <- 2 x - prev, prev <- old x
-__global__ void extrapolate_all_kernel(double *__restrict__ x, double *__restrict__ prev, size_t n) {
+// over every entry of a field (a half spectrum here): prev <- 2 x - prev (the
+// caller then swaps the two buffers, so x holds the extrapolation and prev
+// the old x without a copy)
+__global__ void extrapolate_all_kernel(const double *__restrict__ x, double *__restrict__ prev, size_t n) {
   size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
-  const double xn = x[e];
-  x[e] = 2.0 * xn - prev[e];
-  prev[e] = xn;
+  prev[e] = 2.0 * x[e] - prev[e];
 }
 
 // over every entry: x64 -= (double) d   (r = A x - b: the correction enters with a minus sign)
@@ -572,10 +573,6 @@ __global__ void sub_all_kernel(double *__restrict__ x, const float *__restrict__
   if (e < n) x[e] -= (double)d[e];
 }
 
-// sc[out] = count / sc[sum] (1 if the sum is not positive)
-__global__ void inv_mean_kernel(double *__restrict__ sc, int sum, int out, double count) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) sc[out] = sc[sum] > 0.0 ? count / sc[sum] : 1.0;
-}
 
 // x <- 2 x - prev, prev <- old x  (linear extrapolation of the potential in time)
 __global__ void extrapolate_kernel(double *__restrict__ x, double *__restrict__ prev, Grid g) {
@@ -1129,9 +1126,7 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
                        face32[0], face32[1], face32[2], face64[0], face64[1], face64[2], ctx->d_partial, g);
     KAPSEL_LAUNCH_CHECK();
     hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->d_partial, NPART, ctx->d_scal, S_TMP,
-                       0);
-    hipLaunchKernelGGL(inv_mean_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->d_scal, S_TMP, S_INVEPS,
-                       (double)g.nr);
+                       2, nullptr, 0, nullptr, (double)g.nr);
     KAPSEL_LAUNCH_CHECK();
   }
   // start: the POTENTIAL field if it was written since the last solve; else
@@ -1145,9 +1140,11 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
     fft_r2c_t<double>(ctx, x1, 1);
     ctx->prev_valid = false;
   } else if (ctx->prev_valid) {
-    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 4, sizeof(double)));
+    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 3, sizeof(double)));
     hipLaunchKernelGGL(extrapolate_all_kernel, dim3(fb), dim3(TPB), 0, ctx->stream, xk, ctx->x64_prev, g.f);
     KAPSEL_LAUNCH_CHECK();
+    std::swap(ctx->p64[1], ctx->x64_prev);
+    xk = ctx->p64[1];
   } else {
     KAPSEL_HIP(hipMemcpyAsync(ctx->x64_prev, xk, g.f * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     ctx->prev_valid = true;
@@ -1183,9 +1180,15 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
   ctx->outputs_stale = false;
   // rk = A xk - bk in k-space, ||r||^2 by Parseval (raw sum also into the
   // host slot 3); where NX has fused x passes, the gradient symbols ride in
-  // the first and the divergence and the residual's partial sums in the last
-  auto residual = [&]() {
-    if (fft_x_grad_t<double>(ctx, xk, xk, ctx->d_scal, -1, g3)) {
+  // the first and the divergence and the residual's partial sums in the
+  // last.  sub: xk -= sub first (in the first pass where it is fused)
+  auto residual = [&](const float *sub) {
+    if (sub && !(ctx->g.nx == 64 || ctx->g.nx == 128)) {
+      hipLaunchKernelGGL(sub_all_kernel, dim3(fb), dim3(TPB), 0, ctx->stream, xk, sub, g.f);
+      KAPSEL_LAUNCH_CHECK();
+      sub = nullptr;
+    }
+    if (fft_x_grad_t<double>(ctx, xk, xk, ctx->d_scal, -1, g3, sub)) {
       fft_face_mid_t<double>(ctx, g3, f3);
       const int nb = fft_x_res_t<double>(ctx, g3, bk, rk, ctx->d_partial);
       hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->d_partial, nb, ctx->d_scal, S_RR,
@@ -1217,10 +1220,8 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
   const std::function<void()> apply_and_residual = [&]() {
     const bool was = ctx->refine64;
     ctx->refine64 = true;
-    hipLaunchKernelGGL(sub_all_kernel, dim3(fb), dim3(TPB), 0, ctx->stream, xk, d32, g.f);
-    KAPSEL_LAUNCH_CHECK();
     hook_current = false;
-    residual();
+    residual(d32);
     ctx->refine64 = was;
   };
   // the caller's work on the potential about to be checked (it only writes
@@ -1241,7 +1242,7 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
   double rel = 1.0, bnorm = 0.0;
   const int nbegin = maxit < 2 ? maxit : 2;  // iterations queued before the first read-back
   bool outputs_current = false;
-  residual();
+  residual(nullptr);
   for (int outer = 0; outer < 30; outer++) {
     const unsigned long long res_seq = ctx->seq_res;
     const bool begun = outer == 0;
