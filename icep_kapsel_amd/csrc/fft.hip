@@ -403,36 +403,38 @@ __device__ __forceinline__ void row_four_step(V2<T> *row, V2<T> (&v)[R1], const 
 
 // z r2c of one row: v[n1] = z[n2 + R2 n1] = a[2n] + i a[2n+1] (the packed
 // reals); a length-M complex FFT, then X[k] = E + W^k O with E = (Z[k] +
-// conj Z[M-k]) / 2, O = (Z[k] - conj Z[M-k]) / 2i, k = 0..M, into row[k]
+// conj Z[M-k]) / 2, O = (Z[k] - conj Z[M-k]) / 2i, k = 0..M-1, into row[k],
+// except that row[0] holds the two real values (X[0], X[M]) = (Re Z[0] +
+// Im Z[0], Re Z[0] - Im Z[0]): the kz = 0 and kz = M columns of the plane
+// travel as one complex column
 template <typename T, int M, int R1, int R2>
 __device__ __forceinline__ void row_r2c(V2<T> *row, V2<T> (&v)[R1], const V2<T> *twm, const V2<T> *zrot, int n2) {
   row_four_step<T, R1, R2, -1>(row, v, twm, n2, [&](int n1, V2<T> z) { row[n2 + R2 * n1] = z; });
   wave_lds_sync();
-  constexpr int KPT = M / R2 + 1;  // split outputs per lane (at most)
+  constexpr int KPT = M / R2;  // split outputs per lane
   V2<T> zk[KPT], zm[KPT];
 #pragma unroll
   for (int q = 0; q < KPT; q++) {
     const int k = n2 + R2 * q;
-    if (k <= M) {
-      zk[q] = row[k & (M - 1)];
-      zm[q] = conjv(row[(M - k) & (M - 1)]);
-    }
+    zk[q] = row[k];
+    zm[q] = conjv(row[(M - k) & (M - 1)]);
   }
   wave_lds_sync();
 #pragma unroll
   for (int q = 0; q < KPT; q++) {
     const int k = n2 + R2 * q;
-    if (k <= M) {
-      const V2<T> e = scal((T)0.5, zk[q] + zm[q]);
-      const V2<T> d = zk[q] - zm[q];
-      const V2<T> od = {(T)0.5 * d.im, (T)-0.5 * d.re};
-      row[k] = e + cmul(zrot[k], od);
-    }
+    const V2<T> e = scal((T)0.5, zk[q] + zm[q]);
+    const V2<T> d = zk[q] - zm[q];
+    const V2<T> od = {(T)0.5 * d.im, (T)-0.5 * d.re};
+    V2<T> x = e + cmul(zrot[k], od);
+    if (q == 0 && n2 == 0) x = V2<T>{zk[0].re + zk[0].im, zk[0].re - zk[0].im};
+    row[k] = x;
   }
 }
 
-// z c2r of one row (scaled): X[k] = row[k], k = 0..M, with X[0] and X[M]
-// taken as real; out(n1, z) receives z[n2 + R2 n1] = (a[2n], a[2n+1])
+// z c2r of one row (scaled): X[k] = row[k], k = 1..M-1, and the real X[0],
+// X[M] packed in row[0] (as row_r2c leaves them); out(n1, z) receives
+// z[n2 + R2 n1] = (a[2n], a[2n+1])
 template <typename T, int M, int R1, int R2, typename Out>
 __device__ __forceinline__ void row_c2r(V2<T> *row, const V2<T> *twm, const V2<T> *zrot, T scale, int n2, Out out) {
   V2<T> v[R1];
@@ -440,10 +442,10 @@ __device__ __forceinline__ void row_c2r(V2<T> *row, const V2<T> *twm, const V2<T
   for (int n1 = 0; n1 < R1; n1++) {
     const int n = R2 * n1 + n2;
     V2<T> xn = row[n];
-    V2<T> xm = row[M - n];
-    if (n == 0) {
+    V2<T> xm = row[(M - n) & (M - 1)];
+    if (n1 == 0 && n2 == 0) {  // n = 0: X[0] = row[0].re, X[M] = row[0].im
+      xm = V2<T>{xn.im, 0};
       xn.im = 0;
-      xm.im = 0;  // M - n = M
     }
     xm = conjv(xm);
     const V2<T> e = xn + xm;                         // 2 E[n]
@@ -453,34 +455,78 @@ __device__ __forceinline__ void row_c2r(V2<T> *row, const V2<T> *twm, const V2<T
   row_four_step<T, R1, R2, +1>(row, v, twm, n2, [&](int n1, V2<T> z) { out(n1, scal(scale, z)); });
 }
 
+// The kz = 0 and kz = M columns of a plane in global memory (at g, the
+// plane's first real) folded into one complex column in the column-0 slots
+// of P: c[j] = S(X_0)[j] + i S(X_M)[j], S(X)[j] = (X[j] + conj X[-j]) / 2.
+// For a real field both columns are Hermitian in y (S(X) = X), their inverse
+// transforms are real, and the inverse of c carries them as its real and
+// imaginary parts; S makes that exact for any input, with the real parts of
+// the two inverse transforms as the result (what dropping their imaginary
+// parts gives).  Wavefront 0 does it (slots 1 and 2 of each row hold the
+// loaded columns meanwhile); a workgroup barrier must follow.
+template <typename T, int NY, int NZ, int THREADS>
+__device__ __forceinline__ void plane_pack_in(V2<T> *P, const T *__restrict__ g) {
+  using C = PlaneCfg<T, NY, NZ, THREADS>;
+  constexpr int H = C::H, HP = C::HP, M = C::M, NJ = (NY + 63) / 64;
+  if (threadIdx.x >= 64) return;
+  V2<T> a[NJ], b[NJ];
+#pragma unroll
+  for (int q = 0; q < NJ; q++) {
+    const int j = threadIdx.x + 64 * q;
+    if (j < NY) {
+      a[q] = ld2(g + 2 * (size_t)j * H);
+      b[q] = ld2(g + 2 * ((size_t)j * H + M));
+      P[j * HP + 1] = a[q];
+      P[j * HP + 2] = b[q];
+    }
+  }
+  wave_lds_sync();
+#pragma unroll
+  for (int q = 0; q < NJ; q++) {
+    const int j = threadIdx.x + 64 * q;
+    if (j < NY) {
+      const int jm = (NY - j) & (NY - 1);
+      const V2<T> sa = scal((T)0.5, a[q] + conjv(P[jm * HP + 1]));
+      const V2<T> sb = scal((T)0.5, b[q] + conjv(P[jm * HP + 2]));
+      P[j * HP] = V2<T>{sa.re - sb.im, sa.im + sb.re};
+    }
+  }
+}
+
 // y transform of every column of the plane in P (in place), direction DIR.
-// GIN: the columns are read from the plane in global memory at g instead of
-// from P; GOUT: the results are written to g instead of P.  Consecutive
+// Column 0 is the packed pair of the kz = 0 and kz = M columns (M columns
+// in all).  GIN: the columns are read from the plane in global memory at g
+// instead of from P (column 0 from P, where plane_pack_in put it); GOUT: the
+// results are written to g instead of P, and the packed column, left in P,
+// is parted into the columns 0 and M of g after the last round: X_0 =
+// (C[j] + conj C[-j]) / 2, X_M = (C[j] - conj C[-j]) / 2i.  Consecutive
 // lanes take consecutive columns, so the lanes of a wavefront touch
 // consecutive LDS words and, in global memory, one contiguous run.
 template <typename T, int NY, int NZ, int THREADS, int DIR, bool GIN = false, bool GOUT = false>
 __device__ __forceinline__ void plane_columns(V2<T> *P, const V2<T> *__restrict__ twy, T *g = nullptr) {
   using C = PlaneCfg<T, NY, NZ, THREADS>;
-  constexpr int H = C::H, HP = C::HP, R1 = C::YR1, R2 = C::YR2;
+  constexpr int H = C::H, HP = C::HP, M = C::M, R1 = C::YR1, R2 = C::YR2;
   const int t = threadIdx.x;
 #pragma unroll 1  // one round's registers at a time
-  for (int c0 = 0; c0 < H; c0 += C::YCOLS) {
-    // a last round of at most 64 / R2 columns (the kz = NZ/2 column when
-    // YCOLS = NZ/2) gives each column R2 adjacent lanes of wavefront 0, so
-    // that the other wavefronts skip its arithmetic
-    const bool few = H - c0 < C::YCOLS && (H - c0) * R2 <= 64;
+  for (int c0 = 0; c0 < M; c0 += C::YCOLS) {
+    // a last round of at most 64 / R2 columns gives each column R2 adjacent
+    // lanes of wavefront 0, so that the other wavefronts skip its arithmetic
+    const bool few = M - c0 < C::YCOLS && (M - c0) * R2 <= 64;
     const int cl = few ? t / R2 : t % C::YCOLS, n2 = few ? t % R2 : t / C::YCOLS;
     const int kz = c0 + cl;
-    const bool valid = kz < H;
+    const bool valid = kz < M;
+    const bool packed = kz == 0;
     V2<T> *col = P + (valid ? kz : 0);
     T *gcol = g + 2 * (valid ? kz : 0);
     V2<T> o[R1];
     if (valid) {
       V2<T> v[R1];
+      if (GIN && !packed) {
 #pragma unroll
-      for (int n1 = 0; n1 < R1; n1++) {
-        if constexpr (GIN) v[n1] = ld2(gcol + 2 * (R2 * n1 + n2) * H);
-        else v[n1] = col[(R2 * n1 + n2) * HP];
+        for (int n1 = 0; n1 < R1; n1++) v[n1] = ld2(gcol + 2 * (R2 * n1 + n2) * H);
+      } else {
+#pragma unroll
+        for (int n1 = 0; n1 < R1; n1++) v[n1] = col[(R2 * n1 + n2) * HP];
       }
       dft<R1, 1, DIR>(v, o);
 #pragma unroll
@@ -501,14 +547,27 @@ __device__ __forceinline__ void plane_columns(V2<T> *P, const V2<T> *__restrict_
 #pragma unroll
         for (int m = 0; m < R2; m++) w[m] = col[(m * R1 + k1) * HP];
         dft<R2, 1, DIR>(w, x);
+        if (GOUT && !packed) {
 #pragma unroll
-        for (int k2 = 0; k2 < R2; k2++) {
-          if constexpr (GOUT) st2(gcol + 2 * (k1 + R1 * k2) * H, x[k2]);
-          else col[(k1 + R1 * k2) * HP] = x[k2];
+          for (int k2 = 0; k2 < R2; k2++) st2(gcol + 2 * (k1 + R1 * k2) * H, x[k2]);
+        } else {
+#pragma unroll
+          for (int k2 = 0; k2 < R2; k2++) col[(k1 + R1 * k2) * HP] = x[k2];
         }
       }
     }
     __syncthreads();
+  }
+  if constexpr (GOUT) {
+    if (t < 64) {
+      for (int j = t; j < NY; j += 64) {
+        const int jm = (NY - j) & (NY - 1);
+        const V2<T> c = P[j * HP], d = conjv(P[jm * HP]);
+        const V2<T> df = c - d;
+        st2(g + 2 * (size_t)j * H, scal((T)0.5, c + d));
+        st2(g + 2 * ((size_t)j * H + M), V2<T>{(T)0.5 * df.im, (T)-0.5 * df.re});
+      }
+    }
   }
 }
 
@@ -616,6 +675,7 @@ __global__ KAPSEL_PLANE_BOUNDS void fft_plane_c2r(FieldPtrs<T> fp, const V2<T> *
   unsigned field;
   const unsigned plane = xcd_tile(field);
   T *__restrict__ gp = fp.p[field] + (size_t)plane * 2 * C::PLANE;
+  plane_pack_in<T, NY, NZ, THREADS>(P, gp);
   plane_tw_load<T, NY, NZ, THREADS>(tw, twm, zrot, twy);
   __syncthreads();
   plane_columns<T, NY, NZ, THREADS, +1, true, false>(P, tw.twy, gp);
@@ -646,6 +706,7 @@ __global__ KAPSEL_PLANE_BOUNDS void fft_plane_face(FieldPtrs<T> fp, const V2<T> 
   const unsigned plane = xcd_tile(field);
   T *__restrict__ gp = fp.p[field] + (size_t)plane * 2 * C::PLANE;
   const T *fpl = face.p[field] + (size_t)plane * 2 * C::PLANE;
+  plane_pack_in<T, NY, NZ, THREADS>(P, gp);
   plane_tw_load<T, NY, NZ, THREADS>(tw, twm, zrot, twy);
   __syncthreads();
   plane_columns<T, NY, NZ, THREADS, +1, true, false>(P, tw.twy, gp);
@@ -1130,7 +1191,7 @@ bool plane_dispatch(kapsel_ctx *ctx, const FieldPtrs<T> &fp, int nf, bool fwd) {
   switch (g.nz) {
     // workgroup sizes: every column of the plane in one round
     case 32: launch_plane<T, 32, 128>(ctx, fp, nf, fwd); return true;
-    case 64: launch_plane<T, 64, 320>(ctx, fp, nf, fwd); return true;
+    case 64: launch_plane<T, 64, 256>(ctx, fp, nf, fwd); return true;
     case 128:
       // FP64: the padded plane (147 KB) leaves one workgroup per CU
       launch_plane<T, 128, 512>(ctx, fp, nf, fwd);
@@ -1229,7 +1290,7 @@ bool fft_r2c_pro_t(kapsel_ctx *ctx, void *const *out, void *const *a, void *cons
                        (T)s, dsq, (const V2<T> *)tw.zh, (const V2<T> *)tw.zrot, (const V2<T> *)tw.y);     \
     break;
         KAPSEL_PRO(32, 128)
-        KAPSEL_PRO(64, 320)
+        KAPSEL_PRO(64, 256)
         KAPSEL_PRO(128, 512)
 #undef KAPSEL_PRO
         default: return false;
@@ -1316,7 +1377,7 @@ void fft_face_t(kapsel_ctx *ctx, void *const *g3, void *const *face3) {
   if (plane_ok<T>(g)) {
     switch (g.nz) {
       case 32: launch_plane_face<T, 32, 128>(ctx, fp, fc); done = true; break;
-      case 64: launch_plane_face<T, 64, 320>(ctx, fp, fc); done = true; break;
+      case 64: launch_plane_face<T, 64, 256>(ctx, fp, fc); done = true; break;
       case 128: launch_plane_face<T, 128, 512>(ctx, fp, fc); done = true; break;
       default: break;
     }
@@ -1340,7 +1401,7 @@ void fft_face_mid_t(kapsel_ctx *ctx, void *const *g3, void *const *face3) {
   if (plane_ok<T>(g)) {
     switch (g.nz) {
       case 32: launch_plane_face<T, 32, 128>(ctx, fp, fc); return;
-      case 64: launch_plane_face<T, 64, 320>(ctx, fp, fc); return;
+      case 64: launch_plane_face<T, 64, 256>(ctx, fp, fc); return;
       case 128: launch_plane_face<T, 128, 512>(ctx, fp, fc); return;
       default: break;
     }
