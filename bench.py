@@ -48,7 +48,7 @@ KERNEL_OF_CLASS = {  # the kernel symbol of each single-kernel class in an FP32 
     "FFT_Z_C2R": "fft_plane_c2r<float, 128, 128, 512>",
     "FFT_LINES_FWD": "fft_c2c_lines_io<float, 128, -1, 32>",
     "FFT_LINES_INV": "fft_c2c_lines_io<float, 128, 1, 32>",
-    "FFT_PLANE_FACE": "fft_plane_face<float, 128, 128, 512>",  # inverse z+y, face eps, forward z+y
+    "FFT_PLANE_FACE": "fft_half_face<float, 128, 128, 512>",  # inverse z+y, face eps, forward z+y (half planes)
 }
 
 

@@ -353,12 +353,13 @@ template <typename T, int NY, int NZ>
 struct PlaneTw {
   V2<T> twm[NZ / 2], zrot[NZ / 2 + 1], twy[NY];
 };
-template <typename T, int NY, int NZ, int THREADS>
+// (YS: the y table holds YS NY points, every YS-th is taken)
+template <typename T, int NY, int NZ, int THREADS, int YS = 1>
 __device__ __forceinline__ void plane_tw_load(PlaneTw<T, NY, NZ> &tw, const V2<T> *__restrict__ twm,
                                               const V2<T> *__restrict__ zrot, const V2<T> *__restrict__ twy) {
   for (int e = threadIdx.x; e < NZ / 2; e += THREADS) tw.twm[e] = twm[e];
   for (int e = threadIdx.x; e < NZ / 2 + 1; e += THREADS) tw.zrot[e] = zrot[e];
-  for (int e = threadIdx.x; e < NY; e += THREADS) tw.twy[e] = twy[e];
+  for (int e = threadIdx.x; e < NY; e += THREADS) tw.twy[e] = twy[YS * e];
 }
 
 // Every lane's LDS accesses before this point precede every lane's after
@@ -725,6 +726,55 @@ __global__ KAPSEL_PLANE_BOUNDS void fft_plane_face(FieldPtrs<T> fp, const V2<T> 
   plane_columns<T, NY, NZ, THREADS, -1, false, true>(P, tw.twy, gp);
 }
 
+// fft_plane_face on the y-split gradient fields that fft_x_grad_dec
+// writes: half h of plane i (rows hJ .. hJ + J - 1, J = NY/2, contiguous)
+// holds the J-point spectra whose inverse y transforms are the real rows
+// 2m + h.  One workgroup per (field, plane, half): inverse z and J-point y
+// transforms, times the face permittivity of row 2m + h, forward z and
+// J-point y transforms, the result (E for the even rows, O for the odd)
+// back into the half; fft_x_div_dec combines E(ky) + W^ky O(ky) and
+// E(ky) - W^ky O(ky) into rows ky and ky + J.  Twice the units of
+// fft_plane_face at half the size each (768 at 128^3: three per CU), the
+// two halves of a plane at consecutive ids.
+template <typename T, int NY, int NZ, int THREADS>
+constexpr int half_waves_per_simd() {
+  using C = PlaneCfg<T, NY / 2, NZ, THREADS>;
+  constexpr int wgs = C::LDS_BYTES * 3 <= 160 * 1024 ? 3 : C::LDS_BYTES * 2 <= 160 * 1024 ? 2 : 1;
+  return (wgs * C::NW + 3) / 4;
+}
+template <typename T, int NY, int NZ, int THREADS>
+__global__ __launch_bounds__(THREADS, (half_waves_per_simd<T, NY, NZ, THREADS>())) void fft_half_face(
+    FieldPtrs<T> fp, const V2<T> *__restrict__ twm, const V2<T> *__restrict__ zrot, const V2<T> *__restrict__ twy,
+    T scale, FieldPtrs<T> face) {
+  constexpr int J = NY / 2;
+  using C = PlaneCfg<T, J, NZ, THREADS>;
+  __shared__ __align__(16) V2<T> P[C::LPLANE];
+  __shared__ PlaneTw<T, J, NZ> tw;
+  unsigned field;
+  const unsigned u = xcd_tile(field);
+  const unsigned plane = u >> 1, hf = u & 1;
+  T *__restrict__ gp = fp.p[field] + (size_t)plane * 2 * (NY * C::H) + (size_t)hf * 2 * C::PLANE;
+  const T *fpl = face.p[field] + (size_t)plane * 2 * (NY * C::H);
+  plane_pack_in<T, J, NZ, THREADS>(P, gp);
+  plane_tw_load<T, J, NZ, THREADS, 2>(tw, twm, zrot, twy);
+  __syncthreads();
+  plane_columns<T, J, NZ, THREADS, +1, true, false>(P, tw.twy, gp);
+  const int n2 = threadIdx.x % 64 % C::ZR2;
+  for (int m = threadIdx.x / 64 * C::RPW + threadIdx.x % 64 / C::ZR2; m < J; m += C::NW * C::RPW) {
+    const T *frow = fpl + (size_t)(2 * m + hf) * 2 * C::H;
+    V2<T> fv[C::ZR1], v[C::ZR1];
+#pragma unroll
+    for (int n1 = 0; n1 < C::ZR1; n1++) fv[n1] = ld2(frow + 2 * (C::ZR2 * n1 + n2));
+    V2<T> *row = P + m * C::HP;
+    row_c2r<T, C::M, C::ZR1, C::ZR2>(row, tw.twm, tw.zrot, scale, n2, [&](int n1, V2<T> z) {
+      v[n1] = V2<T>{z.re * fv[n1].re, z.im * fv[n1].im};
+    });
+    row_r2c<T, C::M, C::ZR1, C::ZR2>(row, v, tw.twm, tw.zrot, n2);
+  }
+  __syncthreads();
+  plane_columns<T, J, NZ, THREADS, -1, false, true>(P, tw.twy, gp);
+}
+
 // One z row of each of the three staggered-gradient spectra (after the x
 // and y inverse passes): c2r along z, times the face permittivity row, r2c
 // along z -- the middle of the dielectric operator where no plane pass
@@ -915,6 +965,104 @@ __global__ __launch_bounds__((LPB * LineFFT<T, L>::S)) void fft_x_grad(
   }
 }
 
+// fft_x_grad with the gradients left y-split for fft_half_face: with G_d(j)
+// the gradient spectrum of row j (ky index) and J = NY/2, row j' < J
+// receives G_d(j') + G_d(j' + J) and row J + j' receives (G_d(j') -
+// G_d(j' + J)) W^{-j'} (W = e^{-2 pi i / NY}): the inputs of the J-point
+// inverse y transforms of the even and of the odd real rows (one radix-2
+// step of the y transform, decimation in frequency), formed before the x
+// transforms (they act on different axes).  A tile's LPB line slots are
+// LPB/2 row pairs (j', kz), (j' + J, kz): slot l < LPB/2 takes row j', slot
+// l + LPB/2 row j' + J (Q = J h pairs).  Each line loads, updates and stores
+// its own row and leaves it in its LDS slot, where the line of the other
+// row reads it: the two lines' threads with the same s are LPB/2 lanes
+// apart in one wavefront, so a wavefront-ordered exchange suffices.
+// 32-bit offsets (a field holds fewer than 2^31 values).
+template <typename T, int L, int LPB>
+__global__ __launch_bounds__((LPB * LineFFT<T, L>::S)) void fft_x_grad_dec(
+    const T *z, T *p, const double *__restrict__ sc, int beta_slot, FieldPtrs<T> g3, unsigned Q, unsigned plane,
+    int h, int ny, double wx, double wy, double wz, const V2<T> *__restrict__ shx, const V2<T> *__restrict__ shy,
+    const V2<T> *__restrict__ shz, const V2<T> *__restrict__ tw, const float *__restrict__ sub,
+    const V2<T> *__restrict__ twy) {
+  using F = LineFFT<T, L>;
+  constexpr int P = F::P, LPT = LPB / 2;
+  static_assert(64 % LPB == 0, "the two lines of a pair share a wavefront");
+  __shared__ V2<T> sm[LPB * F::LS];
+  unsigned field;
+  const unsigned tile = xcd_tile(field);
+  const int t = threadIdx.x;
+  const int l = t % LPB, s = t / LPB;
+  const int mb = l / LPT;  // 0: row j' (the sum), 1: row j' + J (the twiddled difference)
+  const unsigned q = tile * LPT + l % LPT;
+  const bool valid = q < Q;
+  const unsigned base = valid ? q : 0, half = (unsigned)(ny / 2) * h, own = base + mb * half;
+  const int j = (int)(base / h), kz = (int)(base % h);
+  const bool update = beta_slot >= 0;
+  const T beta = update ? (T)sc[beta_slot] : (T)0;
+  V2<T> *sl = sm + l * F::LS;
+  V2<T> pv[P], vy[P];
+  {
+    V2<T> pa[P];
+#pragma unroll
+    for (int n = 0; n < P; n++) {
+      const unsigned off = own + F::in_i(s, n) * plane;
+      if (valid) {
+        pa[n] = ld2(z + 2 * off);
+        if (update) {
+          pa[n] = pa[n] + scal(beta, ld2(p + 2 * off));
+          st2(p + 2 * off, pa[n]);
+        } else if (sub) {
+          const V2<float> d = ld2(sub + 2 * off);
+          pa[n] = V2<T>{pa[n].re - (T)d.re, pa[n].im - (T)d.im};
+          st2(p + 2 * off, pa[n]);
+        }
+      } else {
+        pa[n] = V2<T>{0, 0};
+      }
+      sl[s * P + n] = pa[n];
+    }
+    wave_lds_sync();
+    const V2<T> *so = sm + (l ^ LPT) * F::LS;  // the other row's slot (LPT is a power of two)
+    V2<T> wj = tw_get<+1>(twy, j);
+    if (!mb) wj = V2<T>{1, 0};
+    const T sg = mb ? (T)-1 : (T)1;
+    const T k0 = (T)(signed_k(j, ny) * wy), k1 = (T)(signed_k(j + ny / 2, ny) * wy);
+    const V2<T> s0 = shy[j], s1 = shy[j + ny / 2];
+#pragma unroll
+    for (int n = 0; n < P; n++) {
+      const V2<T> po = so[s * P + n];
+      const V2<T> p0 = mb ? po : pa[n], p1 = mb ? pa[n] : po;
+      const V2<T> g0 = cmul(V2<T>{-k0 * p0.im, k0 * p0.re}, s0);
+      const V2<T> g1 = cmul(V2<T>{-k1 * p1.im, k1 * p1.re}, s1);
+      pv[n] = cmul(p0 + scal(sg, p1), wj);
+      vy[n] = cmul(g0 + scal(sg, g1), wj);
+    }
+  }
+  V2<T> twr[P];
+  line_twiddles<F>(twr, tw, s);
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    V2<T> v[P];
+#pragma unroll
+    for (int n = 0; n < P; n++) {
+      if (d == 1) {
+        v[n] = vy[n];
+      } else {
+        const int i = F::in_i(s, n);
+        const T w = d == 0 ? (T)(signed_k(i, L) * wx) : (T)(kz * wz);
+        const V2<T> sft = d == 0 ? shx[i] : shz[kz];
+        v[n] = cmul(V2<T>{-w * pv[n].im, w * pv[n].re}, sft);
+      }
+    }
+    F::template run<+1>(v, sl, twr, s, LPB);
+    if (valid) {
+      T *__restrict__ gd = g3.p[d];
+#pragma unroll
+      for (int m = 0; m < P; m++) st2(gd + 2 * (own + F::out_i(s, m) * plane), v[m]);
+    }
+  }
+}
+
 // The staggered gradients of two FP64 half spectra a, b (grid.y: 0 = a,
 // 1 = b) as six FP32 fields after their inverse x transform:
 // out[3f + d] = IFFT_x(i k_d e^{i k_d dx/2} in_f).  The symbols are formed in
@@ -1023,6 +1171,94 @@ __global__ __launch_bounds__((LPB * DivFFT<T, L>::S)) void fft_x_div(
 #pragma unroll
     for (int m = 0; m < P; m++) {
       const long long off = base + (long long)F::out_i(s, m) * plane;
+      const V2<T> av{-acc[m].im, acc[m].re};
+      const V2<T> pp = ld2(p + 2 * off);
+      if constexpr (RES) {
+        const V2<T> rv = av - pp;
+        st2(ap + 2 * off, rv);
+        sum += wk * ((double)rv.re * rv.re + (double)rv.im * rv.im);
+      } else {
+        st2(ap + 2 * off, av);
+        sum += wk * ((double)pp.re * av.re + (double)pp.im * av.im);
+      }
+    }
+  }
+  sum = block_sum_fft<LPB * F::S>(sum);
+  if (threadIdx.x == 0) partial[tile] = sum;
+}
+
+// fft_x_div on the y-split output of fft_half_face (rows j' < J = NY/2: E,
+// rows J + j': O): a tile's line slots are row pairs as in fft_x_grad_dec;
+// the line of row j' loads E, the line of row j' + J loads O, they swap
+// them through their LDS slots (one wavefront), and form E + W^{j'} O and
+// E - W^{j'} O (W = e^{-2 pi i / NY}), the spectra of rows j' and j' + J,
+// before their forward x transforms.  32-bit offsets.
+template <typename T, int L, int LPB, bool RES = false>
+__global__ __launch_bounds__((LPB * DivFFT<T, L>::S), 2) void fft_x_div_dec(
+    FieldPtrs<T> g3, const T *__restrict__ p, T *__restrict__ ap, double *__restrict__ partial, unsigned Q,
+    unsigned plane, int h, int ny, int nz, double wx, double wy, double wz, const V2<T> *__restrict__ shx,
+    const V2<T> *__restrict__ shy, const V2<T> *__restrict__ shz, const V2<T> *__restrict__ tw,
+    const V2<T> *__restrict__ twy) {
+  using F = DivFFT<T, L>;
+  constexpr int P = F::P, LPT = LPB / 2;
+  static_assert(64 % LPB == 0, "the two lines of a pair share a wavefront");
+  __shared__ V2<T> sm[LPB * F::LS];
+  unsigned field;
+  const unsigned tile = xcd_tile(field);
+  const int t = threadIdx.x;
+  const int l = t % LPB, s = t / LPB;
+  const int mb = l / LPT;
+  const unsigned q = tile * LPT + l % LPT;
+  const bool valid = q < Q;
+  const unsigned base = valid ? q : 0, half = (unsigned)(ny / 2) * h, own = base + mb * half;
+  const int jp = (int)(base / h), kz = (int)(base % h);
+  const int j = jp + mb * (ny / 2);  // this line's row (ky index)
+  const V2<T> wj = twy[jp];
+  const T sg = mb ? (T)-1 : (T)1;
+  V2<T> acc[P];
+#pragma unroll
+  for (int m = 0; m < P; m++) acc[m] = V2<T>{0, 0};
+  V2<T> *sl = sm + l * F::LS;
+  const V2<T> *so = sm + (l ^ LPT) * F::LS;
+  // direction d + 1's loads are in flight while direction d is transformed
+  // (base is 0 for an invalid line, so every load is in bounds)
+  V2<T> nxt[P];
+#pragma unroll
+  for (int n = 0; n < P; n++) nxt[n] = ld2(g3.p[0] + 2 * (own + F::in_i(s, n) * plane));
+  V2<T> twr[P];
+  line_twiddles<F>(twr, tw, s);
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    V2<T> v[P];
+    if (d > 0) __syncthreads();  // the slots' last reads (the previous transform) are done
+#pragma unroll
+    for (int n = 0; n < P; n++) sl[s * P + n] = nxt[n];
+    wave_lds_sync();
+#pragma unroll
+    for (int n = 0; n < P; n++) {
+      const V2<T> o = so[s * P + n];
+      const V2<T> e = mb ? o : nxt[n], od = mb ? nxt[n] : o;
+      v[n] = valid ? e + scal(sg, cmul(od, wj)) : V2<T>{0, 0};
+    }
+    if (d < 2) {
+#pragma unroll
+      for (int n = 0; n < P; n++) nxt[n] = ld2(g3.p[d + 1] + 2 * (own + F::in_i(s, n) * plane));
+    }
+    F::template run<-1>(v, sl, twr, s, LPB);
+#pragma unroll
+    for (int m = 0; m < P; m++) {
+      const int i = F::out_i(s, m);
+      const T qd = d == 0 ? (T)(signed_k(i, L) * wx) : d == 1 ? (T)(signed_k(j, ny) * wy) : (T)(kz * wz);
+      const V2<T> sft = d == 0 ? shx[i] : d == 1 ? shy[j] : shz[kz];
+      acc[m] = acc[m] + cmul(v[m], scal(qd, conjv(sft)));
+    }
+  }
+  const double wk = (kz == 0 || 2 * kz == nz) ? 1.0 : 2.0;
+  double sum = 0.0;
+  if (valid) {
+#pragma unroll
+    for (int m = 0; m < P; m++) {
+      const unsigned off = own + F::out_i(s, m) * plane;
       const V2<T> av{-acc[m].im, acc[m].re};
       const V2<T> pp = ld2(p + 2 * off);
       if constexpr (RES) {
@@ -1392,12 +1628,42 @@ void fft_face_t(kapsel_ctx *ctx, void *const *g3, void *const *face3) {
   lines_dispatch<T, -1>(ctx, KAPSEL_K_FFT_LINES_FWD, g.nx, fp, 3, plane, plane, 0, plane, tw.x);
 }
 
-// the middle of fft_face_t: everything but the two x passes
+template <typename T, int N, int THREADS>
+void launch_half_face(kapsel_ctx *ctx, const FieldPtrs<T> &fp, const FieldPtrs<T> &face) {
+  const Grid &g = ctx->g;
+  static_assert(PlaneCfg<T, N / 2, N, THREADS>::LDS_BYTES <= 160 * 1024, "half plane does not fit LDS");
+  dim3 grid(2u * (unsigned)g.nx, 3);
+  LaunchTimer lt(ctx, KAPSEL_K_FFT_PLANE_FACE, 2.0 * 3 * g.f * sizeof(T) + 3.0 * g.f * sizeof(T));
+  const Twiddles &tw = ctx->tw[prec_of<T>()];
+  T scale = (T)(1.0 / ((double)g.nx * g.ny * g.nz));
+  hipLaunchKernelGGL((fft_half_face<T, N, N, THREADS>), grid, dim3(THREADS), 0, ctx->stream, fp,
+                     (const V2<T> *)tw.zh, (const V2<T> *)tw.zrot, (const V2<T> *)tw.y, scale, face);
+  KAPSEL_LAUNCH_CHECK();
+}
+
+// whether the operator's x passes and plane round trip use the y-split
+// gradient fields (fft_x_grad_dec, fft_half_face, fft_x_div_dec): wherever the
+// fused x passes meet a plane pass
+template <typename T>
+bool dec_ok(const Grid &g) {
+  return plane_ok<T>(g) && (g.nx == 64 || g.nx == 128);
+}
+
+// the middle of fft_face_t: everything but the two x passes (on the y-split
+// fields where dec_ok)
 template <typename T>
 void fft_face_mid_t(kapsel_ctx *ctx, void *const *g3, void *const *face3) {
   const Grid &g = ctx->g;
   const Twiddles &tw = ctx->tw[prec_of<T>()];
   FieldPtrs<T> fp = pack<T>(g3, 3), fc = pack<T>(face3, 3);
+  if (dec_ok<T>(g)) {
+    switch (g.nz) {
+      case 32: launch_half_face<T, 32, 64>(ctx, fp, fc); return;
+      case 64: launch_half_face<T, 64, 128>(ctx, fp, fc); return;
+      case 128: launch_half_face<T, 128, 512>(ctx, fp, fc); return;
+      default: break;
+    }
+  }
   if (plane_ok<T>(g)) {
     switch (g.nz) {
       case 32: launch_plane_face<T, 32, 128>(ctx, fp, fc); return;
@@ -1419,26 +1685,46 @@ void launch_x_grad(kapsel_ctx *ctx, const T *z, T *p, const double *sc, int beta
   constexpr int LPB = lines_per_block(sizeof(T));
   const Grid &g = ctx->g;
   const Twiddles &tw = ctx->tw[prec_of<T>()];
-  const long long Q = (long long)g.ny * g.h;
   LaunchTimer lt(ctx, KAPSEL_K_FFT_X_GRAD, 6.0 * g.f * sizeof(T));
-  hipLaunchKernelGGL((fft_x_grad<T, L, LPB>), dim3((unsigned)((Q + LPB - 1) / LPB)), dim3(LPB * LineFFT<T, L>::S), 0,
-                     ctx->stream, z, p, sc, beta_slot, pack<T>(g3, 3), Q, Q, g.h, g.ny, g.wave_x, g.wave_y,
-                     g.wave_z, (const V2<T> *)tw.shx, (const V2<T> *)tw.shy, (const V2<T> *)tw.shz,
-                     (const V2<T> *)tw.x, sub);
+  if (dec_ok<T>(g)) {
+    constexpr int LPT = LPB / 2;  // row pairs per tile
+    const unsigned Q = (unsigned)(g.ny / 2) * g.h;
+    hipLaunchKernelGGL((fft_x_grad_dec<T, L, LPB>), dim3((Q + LPT - 1) / LPT), dim3(LPB * LineFFT<T, L>::S), 0,
+                       ctx->stream, z, p, sc, beta_slot, pack<T>(g3, 3), Q, (unsigned)(g.ny * g.h), g.h, g.ny,
+                       g.wave_x, g.wave_y, g.wave_z, (const V2<T> *)tw.shx, (const V2<T> *)tw.shy,
+                       (const V2<T> *)tw.shz, (const V2<T> *)tw.x, sub, (const V2<T> *)tw.y);
+  } else {
+    const long long Q = (long long)g.ny * g.h;
+    hipLaunchKernelGGL((fft_x_grad<T, L, LPB>), dim3((unsigned)((Q + LPB - 1) / LPB)), dim3(LPB * LineFFT<T, L>::S),
+                       0, ctx->stream, z, p, sc, beta_slot, pack<T>(g3, 3), Q, Q, g.h, g.ny, g.wave_x, g.wave_y,
+                       g.wave_z, (const V2<T> *)tw.shx, (const V2<T> *)tw.shy, (const V2<T> *)tw.shz,
+                       (const V2<T> *)tw.x, sub);
+  }
   KAPSEL_LAUNCH_CHECK();
 }
 
-template <typename T, int L, bool RES = false>
+template <typename T, int L, bool RES>
 int launch_x_div(kapsel_ctx *ctx, void *const *g3, const T *p, T *ap, double *partial) {
   constexpr int LPB = lines_per_block(sizeof(T));
   const Grid &g = ctx->g;
   const Twiddles &tw = ctx->tw[prec_of<T>()];
-  const long long Q = (long long)g.ny * g.h;
-  const unsigned nb = (unsigned)((Q + LPB - 1) / LPB);
   LaunchTimer lt(ctx, KAPSEL_K_FFT_X_DIV, 5.0 * g.f * sizeof(T));
-  hipLaunchKernelGGL((fft_x_div<T, L, LPB, RES>), dim3(nb), dim3(LPB * DivFFT<T, L>::S), 0, ctx->stream, pack<T>(g3, 3), p,
-                     ap, partial, Q, Q, g.h, g.ny, g.nz, g.wave_x, g.wave_y, g.wave_z, (const V2<T> *)tw.shx,
-                     (const V2<T> *)tw.shy, (const V2<T> *)tw.shz, (const V2<T> *)tw.x);
+  unsigned nb;
+  if (dec_ok<T>(g)) {
+    const unsigned Q = (unsigned)(g.ny / 2) * g.h;
+    constexpr int LPT = LPB / 2;
+    nb = (Q + LPT - 1) / LPT;
+    hipLaunchKernelGGL((fft_x_div_dec<T, L, LPB, RES>), dim3(nb), dim3(LPB * DivFFT<T, L>::S), 0, ctx->stream,
+                       pack<T>(g3, 3), p, ap, partial, Q, (unsigned)(g.ny * g.h), g.h, g.ny, g.nz, g.wave_x,
+                       g.wave_y, g.wave_z, (const V2<T> *)tw.shx, (const V2<T> *)tw.shy, (const V2<T> *)tw.shz,
+                       (const V2<T> *)tw.x, (const V2<T> *)tw.y);
+  } else {
+    const long long Q = (long long)g.ny * g.h;
+    nb = (unsigned)((Q + LPB - 1) / LPB);
+    hipLaunchKernelGGL((fft_x_div<T, L, LPB, RES>), dim3(nb), dim3(LPB * DivFFT<T, L>::S), 0, ctx->stream,
+                       pack<T>(g3, 3), p, ap, partial, Q, Q, g.h, g.ny, g.nz, g.wave_x, g.wave_y, g.wave_z,
+                       (const V2<T> *)tw.shx, (const V2<T> *)tw.shy, (const V2<T> *)tw.shz, (const V2<T> *)tw.x);
+  }
   KAPSEL_LAUNCH_CHECK();
   return (int)nb;
 }
@@ -1470,8 +1756,8 @@ void launch_x_grad2_f64(kapsel_ctx *ctx, const double *a, const double *b, void 
 template <typename T>
 int fft_x_div_t(kapsel_ctx *ctx, void *const *g3, const T *p, T *ap, double *partial) {
   switch (ctx->g.nx) {
-    case 64: return launch_x_div<T, 64>(ctx, g3, p, ap, partial);
-    case 128: return launch_x_div<T, 128>(ctx, g3, p, ap, partial);
+    case 64: return launch_x_div<T, 64, false>(ctx, g3, p, ap, partial);
+    case 128: return launch_x_div<T, 128, false>(ctx, g3, p, ap, partial);
     default: return 0;
   }
 }

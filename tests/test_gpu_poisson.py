@@ -104,13 +104,16 @@ def test_poisson_solve_fp32_refined(n, tol):
     assert rr3 < tol and it3 > 0
 
 
-def test_poisson_solve_non_cubic_grid():
+@pytest.mark.parametrize("shape", [(128, 64, 32), (64, 32, 32)])
+def test_poisson_solve_non_cubic_grid(shape):
     """128 x 64 x 32 grid: the fused x passes of the correction and of the FP64
     residual (chosen by NX) meet the separate y line and z face passes (no
-    plane pass for NY != NZ).  FP64 solves (plain and spectral) against the
-    oracle; the FP32 refined solve against the FP64 solution."""
-    shape = (128, 64, 32)
+    plane pass for NY != NZ).  64 x 32 x 32: the fused x passes with the
+    y-split gradient fields and the half-plane round trip (NX != NY).  FP64
+    solves (plain and spectral) against the oracle; the FP32 refined solve
+    against the FP64 solution."""
     print(f"case poisson non-cubic {shape}", flush=True)
+    nz = shape[2]
     pos = workloads.random_positions(workloads.Xoshiro256(17), 5, shape, 8.0)
     e0 = [0.03, -0.04, 0.1]
     sols = {}
@@ -131,14 +134,14 @@ def test_poisson_solve_non_cubic_grid():
                 ito, rro = o.poisson_solve(e0, 1e-10, 5000, precond)
                 print(f"fp64 precond={precond}: gpu {itk} it ({rrk:.1e}), oracle {ito} it ({rro:.1e})", flush=True)
                 assert rrk < 1e-10 and abs(itk - ito) <= max(3, ito // 50)
-                xk, xo = k.get_field("POTENTIAL")[:, :, :32], o.get_field("POTENTIAL")[:, :, :32]
+                xk, xo = k.get_field("POTENTIAL")[:, :, :nz], o.get_field("POTENTIAL")[:, :, :nz]
                 assert rel(xk - xk.mean(), xo - xo.mean()) < 1e-8
             sols[prec] = xk
         else:
             it, rr = k.poisson_solve(e0, 1e-10, 5000, 1)
             print(f"fp32 refined: {it} inner iterations, fp64 relres {rr:.1e}", flush=True)
             assert rr < 1e-10
-            x = k.get_field("POTENTIAL")[:, :, :32]
+            x = k.get_field("POTENTIAL")[:, :, :nz]
             x64 = sols[abi.KAPSEL_FP64]
             assert rel(x - x.mean(), x64 - x64.mean()) < 1e-5
         k.close()
