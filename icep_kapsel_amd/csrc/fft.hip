@@ -1000,6 +1000,22 @@ __global__ __launch_bounds__((LPB * LineFFT<T, L>::S)) void fft_x_grad_dec(
   const bool update = beta_slot >= 0;
   const T beta = update ? (T)sc[beta_slot] : (T)0;
   V2<T> *sl = sm + l * F::LS;
+  // the x and z gradient symbols i k_d e^{i k_d dx/2}, loaded here, where
+  // their latency hides behind the field loads (x: per input point)
+  V2<T> gx[P];
+#pragma unroll
+  for (int n = 0; n < P; n++) {
+    const int i = F::in_i(s, n);
+    const V2<T> e = shx[i];
+    const T w = (T)(signed_k(i, L) * wx);
+    gx[n] = V2<T>{-w * e.im, w * e.re};
+  }
+  V2<T> gz;
+  {
+    const V2<T> e = shz[kz];
+    const T w = (T)(kz * wz);
+    gz = V2<T>{-w * e.im, w * e.re};
+  }
   V2<T> pv[P], vy[P];
   {
     V2<T> pa[P];
@@ -1044,16 +1060,7 @@ __global__ __launch_bounds__((LPB * LineFFT<T, L>::S)) void fft_x_grad_dec(
   for (int d = 0; d < 3; d++) {
     V2<T> v[P];
 #pragma unroll
-    for (int n = 0; n < P; n++) {
-      if (d == 1) {
-        v[n] = vy[n];
-      } else {
-        const int i = F::in_i(s, n);
-        const T w = d == 0 ? (T)(signed_k(i, L) * wx) : (T)(kz * wz);
-        const V2<T> sft = d == 0 ? shx[i] : shz[kz];
-        v[n] = cmul(V2<T>{-w * pv[n].im, w * pv[n].re}, sft);
-      }
-    }
+    for (int n = 0; n < P; n++) v[n] = d == 1 ? vy[n] : cmul(pv[n], d == 0 ? gx[n] : gz);
     F::template run<+1>(v, sl, twr, s, LPB);
     if (valid) {
       T *__restrict__ gd = g3.p[d];
@@ -1227,6 +1234,16 @@ __global__ __launch_bounds__((LPB * DivFFT<T, L>::S), 2) void fft_x_div_dec(
   for (int n = 0; n < P; n++) nxt[n] = ld2(g3.p[0] + 2 * (own + F::in_i(s, n) * plane));
   V2<T> twr[P];
   line_twiddles<F>(twr, tw, s);
+  // the divergence symbols q_d conj(e^{i k_d dx/2}), loaded here, where
+  // their latency hides behind the field loads (x: per output point)
+  V2<T> fx[P];
+#pragma unroll
+  for (int m = 0; m < P; m++) {
+    const int i = F::out_i(s, m);
+    fx[m] = scal((T)(signed_k(i, L) * wx), conjv(shx[i]));
+  }
+  const V2<T> fy = scal((T)(signed_k(j, ny) * wy), conjv(shy[j]));
+  const V2<T> fz = scal((T)(kz * wz), conjv(shz[kz]));
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     V2<T> v[P];
@@ -1246,12 +1263,7 @@ __global__ __launch_bounds__((LPB * DivFFT<T, L>::S), 2) void fft_x_div_dec(
     }
     F::template run<-1>(v, sl, twr, s, LPB);
 #pragma unroll
-    for (int m = 0; m < P; m++) {
-      const int i = F::out_i(s, m);
-      const T qd = d == 0 ? (T)(signed_k(i, L) * wx) : d == 1 ? (T)(signed_k(j, ny) * wy) : (T)(kz * wz);
-      const V2<T> sft = d == 0 ? shx[i] : d == 1 ? shy[j] : shz[kz];
-      acc[m] = acc[m] + cmul(v[m], scal(qd, conjv(sft)));
-    }
+    for (int m = 0; m < P; m++) acc[m] = acc[m] + cmul(v[m], d == 0 ? fx[m] : d == 1 ? fy : fz);
   }
   const double wk = (kz == 0 || 2 * kz == nz) ? 1.0 : 2.0;
   double sum = 0.0;
