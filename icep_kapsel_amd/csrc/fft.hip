@@ -341,7 +341,8 @@ struct PlaneCfg {
   static constexpr int HP = row_pitch(H > XS ? H : XS, ZR2);
   static constexpr int PLANE = NY * H;    // complex values of a plane in global memory
   static constexpr int LPLANE = NY * HP;  // and in LDS
-  static constexpr size_t LDS_BYTES = sizeof(V2<T>) * (LPLANE + NZ + 1 + NY);
+  static constexpr int PK = 2 * NY;         // the packed column's staging slots
+  static constexpr size_t LDS_BYTES = sizeof(V2<T>) * (LPLANE + NZ + 1 + NY + PK);
   static constexpr int WGS_PER_CU = 2 * LDS_BYTES <= 160 * 1024 ? 2 : 1;
   static constexpr int WAVES_PER_SIMD = (WGS_PER_CU * NW + 3) / 4;
   static_assert(64 % ZR2 == 0 && THREADS % 64 == 0 && NY % RPW == 0, "row tiling");
@@ -457,18 +458,18 @@ __device__ __forceinline__ void row_c2r(V2<T> *row, const V2<T> *twm, const V2<T
 }
 
 // The kz = 0 and kz = M columns of a plane in global memory (at g, the
-// plane's first real) folded into one complex column in the column-0 slots
-// of P: c[j] = S(X_0)[j] + i S(X_M)[j], S(X)[j] = (X[j] + conj X[-j]) / 2.
-// For a real field both columns are Hermitian in y (S(X) = X), their inverse
+// plane's first real) folded into one complex column, left in pk[0, NY):
+// c[j] = S(X_0)[j] + i S(X_M)[j], S(X)[j] = (X[j] + conj X[-j]) / 2.  For a
+// real field both columns are Hermitian in y (S(X) = X), their inverse
 // transforms are real, and the inverse of c carries them as its real and
 // imaginary parts; S makes that exact for any input, with the real parts of
 // the two inverse transforms as the result (what dropping their imaginary
-// parts gives).  Wavefront 0 does it (slots 1 and 2 of each row hold the
-// loaded columns meanwhile); a workgroup barrier must follow.
+// parts gives).  Wavefront 0 does it through pk (2 NY contiguous slots, so
+// its 64 lanes meet distinct banks); a workgroup barrier must follow.
 template <typename T, int NY, int NZ, int THREADS>
-__device__ __forceinline__ void plane_pack_in(V2<T> *P, const T *__restrict__ g) {
+__device__ __forceinline__ void plane_pack_in(V2<T> *pk, const T *__restrict__ g) {
   using C = PlaneCfg<T, NY, NZ, THREADS>;
-  constexpr int H = C::H, HP = C::HP, M = C::M, NJ = (NY + 63) / 64;
+  constexpr int H = C::H, M = C::M, NJ = (NY + 63) / 64;
   if (threadIdx.x >= 64) return;
   V2<T> a[NJ], b[NJ];
 #pragma unroll
@@ -477,34 +478,41 @@ __device__ __forceinline__ void plane_pack_in(V2<T> *P, const T *__restrict__ g)
     if (j < NY) {
       a[q] = ld2(g + 2 * (size_t)j * H);
       b[q] = ld2(g + 2 * ((size_t)j * H + M));
-      P[j * HP + 1] = a[q];
-      P[j * HP + 2] = b[q];
+      pk[j] = a[q];
+      pk[NY + j] = b[q];
+    }
+  }
+  wave_lds_sync();
+  V2<T> c[NJ];
+#pragma unroll
+  for (int q = 0; q < NJ; q++) {
+    const int j = threadIdx.x + 64 * q;
+    if (j < NY) {
+      const int jm = (NY - j) & (NY - 1);
+      const V2<T> sa = scal((T)0.5, a[q] + conjv(pk[jm]));
+      const V2<T> sb = scal((T)0.5, b[q] + conjv(pk[NY + jm]));
+      c[q] = V2<T>{sa.re - sb.im, sa.im + sb.re};
     }
   }
   wave_lds_sync();
 #pragma unroll
   for (int q = 0; q < NJ; q++) {
     const int j = threadIdx.x + 64 * q;
-    if (j < NY) {
-      const int jm = (NY - j) & (NY - 1);
-      const V2<T> sa = scal((T)0.5, a[q] + conjv(P[jm * HP + 1]));
-      const V2<T> sb = scal((T)0.5, b[q] + conjv(P[jm * HP + 2]));
-      P[j * HP] = V2<T>{sa.re - sb.im, sa.im + sb.re};
-    }
+    if (j < NY) pk[j] = c[q];
   }
 }
 
 // y transform of every column of the plane in P (in place), direction DIR.
 // Column 0 is the packed pair of the kz = 0 and kz = M columns (M columns
 // in all).  GIN: the columns are read from the plane in global memory at g
-// instead of from P (column 0 from P, where plane_pack_in put it); GOUT: the
-// results are written to g instead of P, and the packed column, left in P,
-// is parted into the columns 0 and M of g after the last round: X_0 =
+// instead of from P (column 0 from pk, where plane_pack_in put it); GOUT:
+// the results are written to g instead of P, and the packed column, left in
+// pk, is parted into the columns 0 and M of g after the last round: X_0 =
 // (C[j] + conj C[-j]) / 2, X_M = (C[j] - conj C[-j]) / 2i.  Consecutive
 // lanes take consecutive columns, so the lanes of a wavefront touch
 // consecutive LDS words and, in global memory, one contiguous run.
 template <typename T, int NY, int NZ, int THREADS, int DIR, bool GIN = false, bool GOUT = false>
-__device__ __forceinline__ void plane_columns(V2<T> *P, const V2<T> *__restrict__ twy, T *g = nullptr) {
+__device__ __forceinline__ void plane_columns(V2<T> *P, V2<T> *pk, const V2<T> *__restrict__ twy, T *g = nullptr) {
   using C = PlaneCfg<T, NY, NZ, THREADS>;
   constexpr int H = C::H, HP = C::HP, M = C::M, R1 = C::YR1, R2 = C::YR2;
   const int t = threadIdx.x;
@@ -522,7 +530,10 @@ __device__ __forceinline__ void plane_columns(V2<T> *P, const V2<T> *__restrict_
     V2<T> o[R1];
     if (valid) {
       V2<T> v[R1];
-      if (GIN && !packed) {
+      if (GIN && packed) {
+#pragma unroll
+        for (int n1 = 0; n1 < R1; n1++) v[n1] = pk[R2 * n1 + n2];
+      } else if (GIN) {
 #pragma unroll
         for (int n1 = 0; n1 < R1; n1++) v[n1] = ld2(gcol + 2 * (R2 * n1 + n2) * H);
       } else {
@@ -548,7 +559,10 @@ __device__ __forceinline__ void plane_columns(V2<T> *P, const V2<T> *__restrict_
 #pragma unroll
         for (int m = 0; m < R2; m++) w[m] = col[(m * R1 + k1) * HP];
         dft<R2, 1, DIR>(w, x);
-        if (GOUT && !packed) {
+        if (GOUT && packed) {
+#pragma unroll
+          for (int k2 = 0; k2 < R2; k2++) pk[k1 + R1 * k2] = x[k2];
+        } else if (GOUT) {
 #pragma unroll
           for (int k2 = 0; k2 < R2; k2++) st2(gcol + 2 * (k1 + R1 * k2) * H, x[k2]);
         } else {
@@ -563,7 +577,7 @@ __device__ __forceinline__ void plane_columns(V2<T> *P, const V2<T> *__restrict_
     if (t < 64) {
       for (int j = t; j < NY; j += 64) {
         const int jm = (NY - j) & (NY - 1);
-        const V2<T> c = P[j * HP], d = conjv(P[jm * HP]);
+        const V2<T> c = pk[j], d = conjv(pk[jm]);
         const V2<T> df = c - d;
         st2(g + 2 * (size_t)j * H, scal((T)0.5, c + d));
         st2(g + 2 * ((size_t)j * H + M), V2<T>{(T)0.5 * df.im, (T)-0.5 * df.re});
@@ -593,6 +607,7 @@ __global__ KAPSEL_PLANE_BOUNDS void fft_plane_r2c(FieldPtrs<T> fp, const V2<T> *
                                                            const V2<T> *__restrict__ twy) {
   using C = PlaneCfg<T, NY, NZ, THREADS>;
   __shared__ __align__(16) V2<T> P[C::LPLANE];
+  __shared__ __align__(16) V2<T> pk[C::PK];
   __shared__ PlaneTw<T, NY, NZ> tw;
   unsigned field;
   const unsigned plane = xcd_tile(field);
@@ -607,7 +622,7 @@ __global__ KAPSEL_PLANE_BOUNDS void fft_plane_r2c(FieldPtrs<T> fp, const V2<T> *
     row_r2c<T, C::M, C::ZR1, C::ZR2>(P + j * C::HP, v, tw.twm, tw.zrot, n2);
   }
   __syncthreads();
-  plane_columns<T, NY, NZ, THREADS, -1, false, true>(P, tw.twy, gp);
+  plane_columns<T, NY, NZ, THREADS, -1, false, true>(P, pk, tw.twy, gp);
 }
 
 // fft_plane_r2c of a field formed on the fly: out_f = a_f + s (b_f + e_f)
@@ -629,6 +644,7 @@ __global__ KAPSEL_PLANE_BOUNDS void fft_plane_r2c_pro(FieldPtrs<T> out, FieldPtr
                                                                const V2<T> *__restrict__ twy) {
   using C = PlaneCfg<T, NY, NZ, THREADS>;
   __shared__ __align__(16) V2<T> P[C::LPLANE];
+  __shared__ __align__(16) V2<T> pk[C::PK];
   __shared__ PlaneTw<T, NY, NZ> tw;
   unsigned field;
   const unsigned plane = xcd_tile_by_tile(field);
@@ -663,7 +679,7 @@ __global__ KAPSEL_PLANE_BOUNDS void fft_plane_r2c_pro(FieldPtrs<T> out, FieldPtr
     row_r2c<T, C::M, C::ZR1, C::ZR2>(P + j * C::HP, v, tw.twm, tw.zrot, n2);
   }
   __syncthreads();
-  plane_columns<T, NY, NZ, THREADS, -1, false, true>(P, tw.twy, gp);
+  plane_columns<T, NY, NZ, THREADS, -1, false, true>(P, pk, tw.twy, gp);
 }
 
 template <typename T, int NY, int NZ, int THREADS>
@@ -672,14 +688,15 @@ __global__ KAPSEL_PLANE_BOUNDS void fft_plane_c2r(FieldPtrs<T> fp, const V2<T> *
                                                            const V2<T> *__restrict__ twy, T scale) {
   using C = PlaneCfg<T, NY, NZ, THREADS>;
   __shared__ __align__(16) V2<T> P[C::LPLANE];
+  __shared__ __align__(16) V2<T> pk[C::PK];
   __shared__ PlaneTw<T, NY, NZ> tw;
   unsigned field;
   const unsigned plane = xcd_tile(field);
   T *__restrict__ gp = fp.p[field] + (size_t)plane * 2 * C::PLANE;
-  plane_pack_in<T, NY, NZ, THREADS>(P, gp);
+  plane_pack_in<T, NY, NZ, THREADS>(pk, gp);
   plane_tw_load<T, NY, NZ, THREADS>(tw, twm, zrot, twy);
   __syncthreads();
-  plane_columns<T, NY, NZ, THREADS, +1, true, false>(P, tw.twy, gp);
+  plane_columns<T, NY, NZ, THREADS, +1, true, false>(P, pk, tw.twy, gp);
   KAPSEL_PLANE_ROWS(C) {
     T *grow = gp + (size_t)j * 2 * C::H;
     row_c2r<T, C::M, C::ZR1, C::ZR2>(P + j * C::HP, tw.twm, tw.zrot, scale, n2,
@@ -702,15 +719,16 @@ __global__ KAPSEL_PLANE_BOUNDS void fft_plane_face(FieldPtrs<T> fp, const V2<T> 
                                                             FieldPtrs<T> face) {
   using C = PlaneCfg<T, NY, NZ, THREADS>;
   __shared__ __align__(16) V2<T> P[C::LPLANE];
+  __shared__ __align__(16) V2<T> pk[C::PK];
   __shared__ PlaneTw<T, NY, NZ> tw;
   unsigned field;
   const unsigned plane = xcd_tile(field);
   T *__restrict__ gp = fp.p[field] + (size_t)plane * 2 * C::PLANE;
   const T *fpl = face.p[field] + (size_t)plane * 2 * C::PLANE;
-  plane_pack_in<T, NY, NZ, THREADS>(P, gp);
+  plane_pack_in<T, NY, NZ, THREADS>(pk, gp);
   plane_tw_load<T, NY, NZ, THREADS>(tw, twm, zrot, twy);
   __syncthreads();
-  plane_columns<T, NY, NZ, THREADS, +1, true, false>(P, tw.twy, gp);
+  plane_columns<T, NY, NZ, THREADS, +1, true, false>(P, pk, tw.twy, gp);
   KAPSEL_PLANE_ROWS(C) {
     const T *frow = fpl + (size_t)j * 2 * C::H;
     V2<T> fv[C::ZR1], v[C::ZR1];
@@ -723,7 +741,7 @@ __global__ KAPSEL_PLANE_BOUNDS void fft_plane_face(FieldPtrs<T> fp, const V2<T> 
     row_r2c<T, C::M, C::ZR1, C::ZR2>(row, v, tw.twm, tw.zrot, n2);
   }
   __syncthreads();
-  plane_columns<T, NY, NZ, THREADS, -1, false, true>(P, tw.twy, gp);
+  plane_columns<T, NY, NZ, THREADS, -1, false, true>(P, pk, tw.twy, gp);
 }
 
 // fft_plane_face on the y-split gradient fields that fft_x_grad_dec
@@ -749,16 +767,17 @@ __global__ __launch_bounds__(THREADS, (half_waves_per_simd<T, NY, NZ, THREADS>()
   constexpr int J = NY / 2;
   using C = PlaneCfg<T, J, NZ, THREADS>;
   __shared__ __align__(16) V2<T> P[C::LPLANE];
+  __shared__ __align__(16) V2<T> pk[C::PK];
   __shared__ PlaneTw<T, J, NZ> tw;
   unsigned field;
   const unsigned u = xcd_tile(field);
   const unsigned plane = u >> 1, hf = u & 1;
   T *__restrict__ gp = fp.p[field] + (size_t)plane * 2 * (NY * C::H) + (size_t)hf * 2 * C::PLANE;
   const T *fpl = face.p[field] + (size_t)plane * 2 * (NY * C::H);
-  plane_pack_in<T, J, NZ, THREADS>(P, gp);
+  plane_pack_in<T, J, NZ, THREADS>(pk, gp);
   plane_tw_load<T, J, NZ, THREADS, 2>(tw, twm, zrot, twy);
   __syncthreads();
-  plane_columns<T, J, NZ, THREADS, +1, true, false>(P, tw.twy, gp);
+  plane_columns<T, J, NZ, THREADS, +1, true, false>(P, pk, tw.twy, gp);
   const int n2 = threadIdx.x % 64 % C::ZR2;
   for (int m = threadIdx.x / 64 * C::RPW + threadIdx.x % 64 / C::ZR2; m < J; m += C::NW * C::RPW) {
     const T *frow = fpl + (size_t)(2 * m + hf) * 2 * C::H;
@@ -772,7 +791,7 @@ __global__ __launch_bounds__(THREADS, (half_waves_per_simd<T, NY, NZ, THREADS>()
     row_r2c<T, C::M, C::ZR1, C::ZR2>(row, v, tw.twm, tw.zrot, n2);
   }
   __syncthreads();
-  plane_columns<T, J, NZ, THREADS, -1, false, true>(P, tw.twy, gp);
+  plane_columns<T, J, NZ, THREADS, -1, false, true>(P, pk, tw.twy, gp);
 }
 
 // One z row of each of the three staggered-gradient spectra (after the x
