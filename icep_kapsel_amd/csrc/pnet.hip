@@ -25,6 +25,7 @@
 // reduces over voxels into per-chunk FP32 partials, summed in a fixed order.
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 #include <vector>
 
 #include "common.hpp"
@@ -108,6 +109,222 @@ __global__ __launch_bounds__(256) void conv_mfma_kernel(const bf16 *__restrict__
       if (out32) out32[o] = x;
     }
   }
+}
+
+// The same convolution (CIN, COUT multiples of 16; n >= 16) as an
+// LDS-tiled implicit GEMM.  A workgroup of 256 threads computes the voxels
+// (i0 + a, j0 + b, k0 .. k0 + TK - 1), a, b in {0, 1}: wavefront w = 2a + b
+// owns one z run of TK voxels (MT = TK/16 M-tiles) and all COUT outputs.
+// Input channels go in chunks of KC = min(CIN, 32): the chunk's input tile
+// with its halo, 4 x 4 x (TK + 2) voxels x KC channels, is staged in LDS
+// once (periodic wrap, 16-byte loads), and for each of the 27 taps a
+// wavefront reads its MT A-fragments from LDS (one contiguous 1 KB block
+// each) and its NT B-fragments (weights, [tap][co][ci]) from global memory,
+// where the CU's wavefronts share them through L1, and issues MT x NT
+// MFMAs: v_mfma_f32_16x16x32_bf16 (lane l: A[l%16][8(l/16)+e], B[8(l/16)+e]
+// [l%16], e < 8) for a 32-channel chunk, 16x16x16 for a 16-channel one.
+// Each B-fragment feeds MT MFMAs and each A-fragment NT.
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+
+// One chunk of KC input channels from channel c0: stage the tile, then the
+// 27 taps' MFMAs into acc.  Ends with the chunk's reads of the tile done by
+// its own wavefront only (the caller syncs before the next chunk's staging).
+// The chunk's channels are read from src, whose voxels hold cs channels,
+// from channel sc0; up: src is a grid of n/2 points per axis, sampled at
+// the parent voxel (the nearest-neighbour upsampling of a decoder concat).
+// LDS voxel pitch (bf16 elements) of a staged chunk: 16-channel chunks get
+// 24 (48 bytes: the A-fragment reads of 16 voxels x 2 channel groups then
+// meet 16 distinct groups of four banks), 32-channel chunks 32
+template <int KC>
+constexpr int tile_pitch() {
+  return KC == 16 ? 24 : KC;
+}
+
+template <int CIN, int COUT, int TK, int KC>
+__device__ __forceinline__ void conv_tile_chunk(const bf16 *__restrict__ src, int cs, int sc0, bool up, int n,
+                                                const bf16 *__restrict__ K, int sign, int c0, int i0, int j0,
+                                                int k0, bf16 *tile, f4v (&acc)[TK / 16][COUT / 16]) {
+  constexpr int NT = COUT / 16, MT = TK / 16, TR = TK + 2, PT = tile_pitch<KC>();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wa = wave >> 1, wb = wave & 1;
+  const int m = lane & 15, kq = lane >> 4;
+  // stage: item e = (row r = a*4 + b, position c, 8-channel part q), in
+  // batches of SB items per thread whose loads are all issued before their
+  // LDS stores, so that their latencies overlap; 32-bit offsets (a tensor
+  // holds fewer than 2^31 values)
+  constexpr int Q = KC / 8, TRQ = TR * Q, ITEMS = 16 * TRQ, SB = 6;
+  const int sh = up ? 1 : 0;
+  const unsigned ns = (unsigned)n >> sh;
+#pragma unroll 1
+  for (int e0 = 0; e0 < ITEMS; e0 += 256 * SB) {
+    uint4 st[SB];
+#pragma unroll
+    for (int p = 0; p < SB; p++) {
+      // past the end: the last item again (loaded, not stored)
+      const int e = min(e0 + (int)threadIdx.x + 256 * p, ITEMS - 1);
+      const int r = e / TRQ, u = e - r * TRQ, c = u / Q, q = u % Q;
+      const unsigned gi = (unsigned)wrapi(i0 - 1 + (r >> 2), n) >> sh;
+      const unsigned gj = (unsigned)wrapi(j0 - 1 + (r & 3), n) >> sh;
+      const unsigned gk = (unsigned)wrapi(k0 - 1 + c, n) >> sh;
+      const unsigned off = ((gi * ns + gj) * ns + gk) * (unsigned)cs + (unsigned)(sc0 + 8 * q);
+      st[p] = *reinterpret_cast<const uint4 *>(src + off);
+    }
+#pragma unroll
+    for (int p = 0; p < SB; p++) {
+      const int e = e0 + (int)threadIdx.x + 256 * p;
+      if (e < ITEMS) {
+        const int r = e / TRQ, u = e - r * TRQ, c = u / Q, q = u % Q;
+        *reinterpret_cast<uint4 *>(tile + (r * TR + c) * PT + 8 * q) = st[p];
+      }
+    }
+  }
+  __syncthreads();
+  // B-fragments (weights): the tap loop is unrolled, so the scheduler
+  // issues each tap's weight loads ahead of the MFMAs that consume them
+  // (a tap's load used by its own MFMAs would expose its L2 latency 27
+  // times per chunk)
+  if constexpr (KC == 16) {
+    // two taps per v_mfma_f32_16x16x32_bf16: lanes kq < 2 take tap t
+    // (channels 8 kq ..), lanes kq >= 2 tap t + 1 (channels 8 (kq - 2) ..);
+    // the 27th tap alone on the 16x16x16 form
+    const int hi = kq >> 1, ch = 8 * (kq & 1);
+    const bf16 *kb2 = K + (size_t)m * CIN + c0 + ch;
+    const bf16 *ab2 = tile + ((wa * 4 + wb) * TR + m) * PT + ch;
+    auto tap_off = [&](int tap) {
+      const int di = (tap / 9 - 1) * sign, dj = ((tap / 3) % 3 - 1) * sign, dk = (tap % 3 - 1) * sign;
+      return (((1 + di) * 4 + (1 + dj)) * TR + 1 + dk) * PT;
+    };
+#pragma unroll
+    for (int t0 = 0; t0 < 26; t0 += 2) {
+      const int tap = t0 + hi;
+      const bf16 *arow = ab2 + (hi ? tap_off(t0 + 1) : tap_off(t0));
+      bf8v b[NT];
+#pragma unroll
+      for (int t = 0; t < NT; t++) b[t] = *reinterpret_cast<const bf8v *>(kb2 + ((size_t)tap * COUT + t * 16) * CIN);
+#pragma unroll
+      for (int a = 0; a < MT; a++) {
+        const bf8v av = *reinterpret_cast<const bf8v *>(arow + a * 16 * PT);
+#pragma unroll
+        for (int t = 0; t < NT; t++) acc[a][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b[t], acc[a][t], 0, 0, 0);
+      }
+    }
+    {
+      const bf16 *kb = K + (size_t)m * CIN + c0 + 4 * kq;
+      const bf16 *arow = tile + ((wa * 4 + wb) * TR + m) * PT + 4 * kq + tap_off(26);
+      s4v b[NT];
+#pragma unroll
+      for (int t = 0; t < NT; t++) b[t] = *reinterpret_cast<const s4v *>(kb + ((size_t)26 * COUT + t * 16) * CIN);
+#pragma unroll
+      for (int a = 0; a < MT; a++) {
+        const s4v av = *reinterpret_cast<const s4v *>(arow + a * 16 * PT);
+#pragma unroll
+        for (int t = 0; t < NT; t++) acc[a][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, b[t], acc[a][t], 0, 0, 0);
+      }
+    }
+    return;
+  }
+  using BV = bf8v;
+  constexpr int BE = 8;  // elements per lane
+  const bf16 *kb = K + (size_t)m * CIN + c0 + BE * kq;
+  const bf16 *abase = tile + ((wa * 4 + wb) * TR + m) * PT + BE * kq;
+#pragma unroll
+  for (int tap = 0; tap < 27; tap++) {
+    const int di = (tap / 9 - 1) * sign, dj = ((tap / 3) % 3 - 1) * sign, dk = (tap % 3 - 1) * sign;
+    const bf16 *arow = abase + (((1 + di) * 4 + (1 + dj)) * TR + 1 + dk) * PT;
+    BV b[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) b[t] = *reinterpret_cast<const BV *>(kb + ((size_t)tap * COUT + t * 16) * CIN);
+#pragma unroll
+    for (int a = 0; a < MT; a++) {
+      const BV av = *reinterpret_cast<const BV *>(arow + a * 16 * PT);
+#pragma unroll
+      for (int t = 0; t < NT; t++) acc[a][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b[t], acc[a][t], 0, 0, 0);
+    }
+  }
+}
+
+template <int CIN, int COUT, int TK>
+__global__ __launch_bounds__(256) void conv_tile_kernel(const bf16 *__restrict__ in, int n,
+                                                        const bf16 *__restrict__ K, const float *__restrict__ bias,
+                                                        int relu, int sign, const bf16 *__restrict__ mask,
+                                                        bf16 *__restrict__ out, float *__restrict__ out32,
+                                                        const bf16 *__restrict__ upin, int cu) {
+  constexpr int NT = COUT / 16, MT = TK / 16, TR = TK + 2;
+  // LDS elements per voxel of the widest chunk
+  constexpr int KMAX = CIN < 32 ? tile_pitch<16>() : tile_pitch<32>();
+  static_assert(tile_pitch<16>() <= tile_pitch<32>(), "a 16-channel chunk fits the 32-channel tile");
+  static_assert(CIN % 16 == 0 && COUT % 16 == 0 && TK % 16 == 0, "tile shape");
+  __shared__ __align__(16) bf16 tile[16 * TR * KMAX];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wa = wave >> 1, wb = wave & 1;
+  const int m = lane & 15, kq = lane >> 4;
+  const int nk = n / TK;
+  int bid = blockIdx.x;
+  const int k0 = (bid % nk) * TK;
+  bid /= nk;
+  const int j0 = (bid % (n / 2)) * 2, i0 = (bid / (n / 2)) * 2;
+  f4v acc[MT][NT];
+#pragma unroll
+  for (int a = 0; a < MT; a++)
+#pragma unroll
+    for (int t = 0; t < NT; t++) acc[a][t] = f4v{0.f, 0.f, 0.f, 0.f};
+  // chunks of 32 channels, and one of 16 where CIN is an odd multiple of 16
+  const int cin_in = upin ? CIN - cu : CIN;  // channels of a voxel of in
+  auto source = [&](int c0, const bf16 *&src, int &cs, int &sc0, bool &up) {
+    up = upin && c0 < cu;
+    src = up ? upin : in;
+    cs = up ? cu : cin_in;
+    sc0 = up ? c0 : c0 - (CIN - cin_in);
+  };
+#pragma unroll 1
+  for (int c0 = 0; c0 + 32 <= CIN; c0 += 32) {
+    if (c0 > 0) __syncthreads();  // every wavefront is done with the previous chunk
+    const bf16 *src;
+    int cs, sc0;
+    bool up;
+    source(c0, src, cs, sc0, up);
+    conv_tile_chunk<CIN, COUT, TK, 32>(src, cs, sc0, up, n, K, sign, c0, i0, j0, k0, tile, acc);
+  }
+  if constexpr (CIN % 32 != 0) {
+    if (CIN > 16) __syncthreads();
+    const bf16 *src;
+    int cs, sc0;
+    bool up;
+    source(CIN - 16, src, cs, sc0, up);
+    conv_tile_chunk<CIN, COUT, TK, 16>(src, cs, sc0, up, n, K, sign, CIN - 16, i0, j0, k0, tile, acc);
+  }
+  // lane holds rows 4 kq + r (voxels) of column m (output channel) per
+  // tile; the wavefront's TK x COUT outputs are contiguous in memory, so
+  // they go out through LDS as 16-byte rows (the tile is free once every
+  // wavefront's last MFMA operands are read)
+  const size_t vrow = ((size_t)(i0 + wa) * n + (j0 + wb)) * n + k0;
+  static_assert(4 * TK * COUT <= 16 * TR * KMAX, "the output staging fits the input tile");
+  __syncthreads();
+  bf16 *ot = tile + wave * TK * COUT;
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    const int co = t * 16 + m;
+    const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+    for (int a = 0; a < MT; a++) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int vl = a * 16 + 4 * kq + r;
+        float x = acc[a][t][r] + bv;
+        if (relu && !(x > 0.f)) x *= LEAK;
+        const size_t o = (vrow + vl) * COUT + co;
+        if (mask && !(bf2f(mask[o]) > 0.f)) x *= LEAK;
+        ot[vl * COUT + co] = f2bf(x);
+        if (out32) out32[o] = x;
+      }
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  constexpr int ROWS = TK * COUT / 8;  // 16-byte rows of the wavefront's outputs
+  for (int e = lane; e < ROWS; e += 64)
+    *reinterpret_cast<uint4 *>(out + vrow * COUT + 8 * e) = *reinterpret_cast<const uint4 *>(ot + 8 * e);
 }
 
 // partial[chunk][tap][ci][co] = sum over the chunk's voxels v of
@@ -320,16 +537,17 @@ __global__ void unpack_grad_kernel(const float *__restrict__ dw, const float *__
 }
 
 // network input from the compact-sine sum phi (padded field layout, type T):
-// channel 0 = phi, channel 1 = beta phi, channels 2..15 = 0
+// channel 0 = phi, channel 1 = beta phi, channels 2..15 = 0; one thread per
+// voxel, two 16-byte stores
 template <typename T>
 __global__ void pnet_input_kernel(const T *__restrict__ phi, Grid g, float beta, bf16 *__restrict__ x0) {
-  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= g.nr * 16) return;
-  const int ch = (int)(e & 15);
-  const size_t v = e >> 4;
+  const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= g.nr) return;
   const size_t row = v / (unsigned)g.nz;
   const float ph = (float)phi[row * g.nz_ + (v - row * g.nz)];
-  x0[e] = f2bf(ch == 0 ? ph : ch == 1 ? beta * ph : 0.f);
+  uint4 lo{(uint32_t)f2bf(ph) | ((uint32_t)f2bf(beta * ph) << 16), 0u, 0u, 0u};
+  *reinterpret_cast<uint4 *>(x0 + v * 16) = lo;
+  *reinterpret_cast<uint4 *>(x0 + v * 16 + 8) = uint4{0u, 0u, 0u, 0u};
 }
 
 // x[im] = scale * sum_d e_d y[v][d]  (the warm start)
@@ -449,19 +667,37 @@ size_t nvox(int n) { return (size_t)n * n * n; }
 
 template <int CIN, int COUT>
 void conv_launch(kapsel_ctx *ctx, const bf16 *in, int n, const bf16 *K, const float *bias, int relu, int sign,
-                 const bf16 *mask, bf16 *out, float *out32) {
+                 const bf16 *mask, bf16 *out, float *out32, const bf16 *upin = nullptr, int cu = 0) {
   const size_t nv = nvox(n);
   LaunchTimer lt(ctx, KAPSEL_K_NETWORK, (double)nv * 2 * (CIN + COUT));
-  hipLaunchKernelGGL((conv_mfma_kernel<CIN, COUT>), dim3((unsigned)((nv + 63) / 64)), dim3(256), 0, ctx->stream,
-                     in, n, K, bias, relu, sign, mask, out, out32);
+  // the LDS-tiled kernel: z runs of 64 voxels (32, 16 on smaller grids)
+  // and two rows by two per workgroup; grids below 16 use the direct form
+  const unsigned nb = (unsigned)(nv / (4 * (n < 64 ? n : 64)));
+  if (n >= 64) {
+    hipLaunchKernelGGL((conv_tile_kernel<CIN, COUT, 64>), dim3(nb), dim3(256), 0, ctx->stream, in, n, K, bias, relu,
+                       sign, mask, out, out32, upin, cu);
+  } else if (n == 32) {
+    hipLaunchKernelGGL((conv_tile_kernel<CIN, COUT, 32>), dim3(nb), dim3(256), 0, ctx->stream, in, n, K, bias, relu,
+                       sign, mask, out, out32, upin, cu);
+  } else if (n == 16) {
+    hipLaunchKernelGGL((conv_tile_kernel<CIN, COUT, 16>), dim3(nb), dim3(256), 0, ctx->stream, in, n, K, bias, relu,
+                       sign, mask, out, out32, upin, cu);
+  } else {
+    if (upin) {
+      set_error("a concat input needs the tiled convolution (grid of 16 or more points per axis)");
+      throw Error{KAPSEL_ERR_STATE};
+    }
+    hipLaunchKernelGGL((conv_mfma_kernel<CIN, COUT>), dim3((unsigned)((nv + 63) / 64)), dim3(256), 0, ctx->stream,
+                       in, n, K, bias, relu, sign, mask, out, out32);
+  }
   KAPSEL_LAUNCH_CHECK();
 }
 
 // dispatch on (cin, cout) of the convolution as executed
 void conv(kapsel_ctx *ctx, int cin, int cout, const bf16 *in, int n, const bf16 *K, const float *bias, int relu,
-          int sign, const bf16 *mask, bf16 *out, float *out32) {
+          int sign, const bf16 *mask, bf16 *out, float *out32, const bf16 *upin = nullptr, int cu = 0) {
 #define KAPSEL_CONV_CASE(A, B) \
-  if (cin == A && cout == B) return conv_launch<A, B>(ctx, in, n, K, bias, relu, sign, mask, out, out32);
+  if (cin == A && cout == B) return conv_launch<A, B>(ctx, in, n, K, bias, relu, sign, mask, out, out32, upin, cu);
   KAPSEL_CONV_CASE(16, 16)
   KAPSEL_CONV_CASE(16, 32)
   KAPSEL_CONV_CASE(32, 32)
@@ -525,7 +761,11 @@ void pack_params(kapsel_ctx *ctx, Pnet *pn) {
 
 unsigned eb(size_t n) { return blocks_for(n, 256); }
 
-void forward(kapsel_ctx *ctx, Pnet *pn) {
+// concat: write the decoder's concatenated inputs (c2, c1; the weight
+// gradients read them), else the convolutions read [up(.) | skip] directly
+// where the tiled kernel runs (from 32 points per axis: the decoder's
+// convolutions then run on 16 or more)
+void forward(kapsel_ctx *ctx, Pnet *pn, bool concat = true) {
   const int n0 = pn->n, n1 = n0 / 2, n2 = n0 / 4;
   const PL *L = pn->L;
   conv(ctx, 16, 16, pn->x0, n0, pn->kf[0], pn->bias[0], 1, 1, nullptr, pn->a1, nullptr);
@@ -536,12 +776,18 @@ void forward(kapsel_ctx *ctx, Pnet *pn) {
   hipLaunchKernelGGL(pool_kernel, dim3(eb(nvox(n2) * 32)), dim3(256), 0, ctx->stream, pn->s2, n1, 32, pn->p2);
   conv(ctx, 32, 32, pn->p2, n2, pn->kf[4], pn->bias[4], 1, 1, nullptr, pn->a3, nullptr);
   conv(ctx, 32, 32, pn->a3, n2, pn->kf[5], pn->bias[5], 1, 1, nullptr, pn->a4, nullptr);
-  hipLaunchKernelGGL(up_concat_kernel, dim3(eb(nvox(n1) * 64)), dim3(256), 0, ctx->stream, pn->a4, n2, 32, pn->s2,
-                     32, pn->c2);
-  conv(ctx, 64, 32, pn->c2, n1, pn->kf[6], pn->bias[6], 1, 1, nullptr, pn->d2, nullptr);
-  hipLaunchKernelGGL(up_concat_kernel, dim3(eb(nvox(n0) * 48)), dim3(256), 0, ctx->stream, pn->d2, n1, 32, pn->s1,
-                     16, pn->c1);
-  conv(ctx, 48, 16, pn->c1, n0, pn->kf[7], pn->bias[7], 1, 1, nullptr, pn->d1, nullptr);
+  const bool fuse = !concat && n1 >= 16;
+  if (fuse) {
+    conv(ctx, 64, 32, pn->s2, n1, pn->kf[6], pn->bias[6], 1, 1, nullptr, pn->d2, nullptr, pn->a4, 32);
+    conv(ctx, 48, 16, pn->s1, n0, pn->kf[7], pn->bias[7], 1, 1, nullptr, pn->d1, nullptr, pn->d2, 32);
+  } else {
+    hipLaunchKernelGGL(up_concat_kernel, dim3(eb(nvox(n1) * 64)), dim3(256), 0, ctx->stream, pn->a4, n2, 32,
+                       pn->s2, 32, pn->c2);
+    conv(ctx, 64, 32, pn->c2, n1, pn->kf[6], pn->bias[6], 1, 1, nullptr, pn->d2, nullptr);
+    hipLaunchKernelGGL(up_concat_kernel, dim3(eb(nvox(n0) * 48)), dim3(256), 0, ctx->stream, pn->d2, n1, 32,
+                       pn->s1, 16, pn->c1);
+    conv(ctx, 48, 16, pn->c1, n0, pn->kf[7], pn->bias[7], 1, 1, nullptr, pn->d1, nullptr);
+  }
   conv(ctx, 16, 16, pn->d1, n0, pn->kf[8], pn->bias[8], L[8].relu, 1, nullptr, pn->scratch, pn->y32);
   KAPSEL_LAUNCH_CHECK();
 }
@@ -596,7 +842,7 @@ Pnet *get_pnet(kapsel_ctx *ctx) {
 
 template <typename T>
 void make_input(kapsel_ctx *ctx, Pnet *pn, double beta) {
-  hipLaunchKernelGGL(pnet_input_kernel<T>, dim3(eb(ctx->g.nr * 16)), dim3(256), 0, ctx->stream,
+  hipLaunchKernelGGL(pnet_input_kernel<T>, dim3(eb(ctx->g.nr)), dim3(256), 0, ctx->stream,
                      (const T *)ctx->fields[KAPSEL_WORK4], ctx->g, (float)beta, pn->x0);
   KAPSEL_LAUNCH_CHECK();
 }
@@ -637,7 +883,7 @@ double loss_grad_t(kapsel_ctx *ctx, Pnet *pn, double beta, const double e[3], do
 template <typename T>
 void warm_start_t(kapsel_ctx *ctx, Pnet *pn, double beta, const double e0[3], double scale) {
   make_input<T>(ctx, pn, beta);
-  forward(ctx, pn);
+  forward(ctx, pn, false);
   hipLaunchKernelGGL(pnet_output_kernel<T>, dim3(eb(ctx->g.nr)), dim3(256), 0, ctx->stream, pn->y32, ctx->g,
                      (float)e0[0], (float)e0[1], (float)e0[2], scale, (T *)ctx->fields[KAPSEL_POTENTIAL]);
   KAPSEL_LAUNCH_CHECK();

@@ -81,10 +81,14 @@ def test_pnet_gradient_matches_oracle(n):
         assert r < 0.2 and c > 0.98, (cin, cout, r, c)
 
 
-def test_pnet_forward_matches_oracle_and_training_reduces_loss():
+@pytest.mark.parametrize("n", [16, 32])
+def test_pnet_forward_matches_oracle_and_training_reduces_loss(n):
+    """16^3: the decoder's convolutions read the concatenated inputs; 32^3:
+    they read [up(coarse) | skip] straight from the two tensors (the warm
+    start's fused form)."""
     from oracle import oracle as om
-    n, beta, scale = 16, -0.4285714, 0.5
-    print("case pnet forward + training 16^3", flush=True)
+    beta, scale = -0.4285714, 0.5
+    print(f"case pnet forward + training {n}^3", flush=True)
     k, rng, t = _setup(n, seed=1)
     P = k.pnet_init()
     params = _init(rng).astype(np.float32)
