@@ -23,6 +23,7 @@
 // wavefront reads one contiguous 512-byte block.  The input gradient is the
 // same kernel with the taps mirrored and W transposed; the weight gradient
 // reduces over voxels into per-chunk FP32 partials, summed in a fixed order.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <type_traits>
@@ -382,44 +383,240 @@ __global__ __launch_bounds__(256) void conv_dw_kernel(const bf16 *__restrict__ i
   }
 }
 
-// g[e] = sum_c partial[c][e] (fixed order), e < count; optionally only the
-// first cin_real input channels are kept (padded input layer)
-__global__ void reduce_partials_kernel(const float *__restrict__ partial, int nchunk, size_t stride, size_t count,
-                                       float *__restrict__ g) {
-  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= count) return;
-  float s = 0.f;
-  for (int c = 0; c < nchunk; c++) s += partial[(size_t)c * stride + e];
-  g[e] = s;
+// The weight gradient as an LDS-tiled GEMM over voxels: dW[tap][ci][co] =
+// sum_v X[v + o_tap][ci] DY[v][co], K = the voxels.  A workgroup of 512
+// threads (8 wavefronts) walks a fixed list of tiles (2 x 2 z runs of TK
+// voxels, the forward kernel's tiles): per tile and 32-channel input chunk
+// it stages the input tile with its halo channel-major (tileT[ci][row][pos],
+// rows padded to TK + 8 positions) and DY channel-major (dyT[co][run][v]),
+// so that a lane's 8 consecutive voxels are 16 contiguous bytes; wavefront w
+// accumulates the taps w, w + 8, ... over all tiles of the workgroup.  A
+// tap's z offset dk shifts the voxel window by s = 1 + dk positions; the
+// lane reads the aligned window (five dwords) and takes elements s .. s + 7
+// (alignbyte for s = 1).  The workgroup's partial sums go to
+// partial[block][tap][ci][co], reduced over the workgroups in a fixed order.
+// one chunk of KCH input channels from c0: stage the tile channel-major,
+// then every K-step's MFMAs of the wavefront's taps into acc
+template <int CIN, int COUT, int TK, int KCH>
+__device__ __forceinline__ void dw_tile_chunk(const bf16 *__restrict__ in, int n, int c0, int i0, int j0, int k0,
+                                              bf16 *tileT, const bf16 *dyT,
+                                              f4v (&acc)[4][CIN / 16][COUT / 16]) {
+  constexpr int NT = COUT / 16, TRp = TK + 8, TR = TK + 2, TPW = 4;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m = lane & 15, kq = lane >> 4;
+  // the input tile with halo, channel-major: item (row, part q, position pair)
+  constexpr int Q = KCH / 8, PP = TR / 2, ITEMS = 16 * Q * PP;
+  for (int e = threadIdx.x; e < ITEMS; e += 512) {
+    const int pp = e % PP, q = (e / PP) % Q, r = e / (PP * Q);
+    const int gi = wrapi(i0 - 1 + (r >> 2), n), gj = wrapi(j0 - 1 + (r & 3), n);
+    const size_t rowv = ((size_t)gi * n + gj) * n;
+    const int ga = wrapi(k0 - 1 + 2 * pp, n), gb = wrapi(k0 + 2 * pp, n);
+    const uint4 a = *reinterpret_cast<const uint4 *>(in + (rowv + ga) * CIN + c0 + 8 * q);
+    const uint4 b = *reinterpret_cast<const uint4 *>(in + (rowv + gb) * CIN + c0 + 8 * q);
+    const uint32_t av[4] = {a.x, a.y, a.z, a.w}, bv[4] = {b.x, b.y, b.z, b.w};
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      const uint32_t lo = (av[c >> 1] >> (16 * (c & 1))) & 0xffffu, hi = (bv[c >> 1] >> (16 * (c & 1))) & 0xffffu;
+      *reinterpret_cast<uint32_t *>(tileT + ((8 * q + c) * 16 + r) * TRp + 2 * pp) = lo | (hi << 16);
+    }
+  }
+  __syncthreads();
+  // K-steps: run rw, voxels kv .. kv + 31
+#pragma unroll 1
+  for (int rw = 0; rw < 4; rw++) {
+#pragma unroll
+    for (int kv = 0; kv < TK; kv += 32) {
+      bf8v b[NT];
+#pragma unroll
+      for (int t = 0; t < NT; t++)
+        b[t] = *reinterpret_cast<const bf8v *>(dyT + ((t * 16 + m) * 4 + rw) * TK + kv + 8 * kq);
+#pragma unroll
+      for (int j = 0; j < TPW; j++) {
+        const int tap = wave + 8 * j;
+        if (tap < 27) {
+          const int di = tap / 9 - 1, dj = (tap / 3) % 3 - 1, sft = tap % 3;  // sft = 1 + dk
+          const int row = ((rw >> 1) + 1 + di) * 4 + ((rw & 1) + 1 + dj);
+#pragma unroll
+          for (int i = 0; i < KCH / 16; i++) {
+            const bf16 *w = tileT + ((i * 16 + m) * 16 + row) * TRp + kv + 8 * kq;
+            const uint4 w0 = *reinterpret_cast<const uint4 *>(w);
+            const uint32_t w4 = *reinterpret_cast<const uint32_t *>(w + 8);
+            uint4 x;
+            if (sft == 0) {
+              x = w0;
+            } else if (sft == 2) {
+              x = uint4{w0.y, w0.z, w0.w, w4};
+            } else {
+              x = uint4{__builtin_amdgcn_alignbyte(w0.y, w0.x, 2), __builtin_amdgcn_alignbyte(w0.z, w0.y, 2),
+                        __builtin_amdgcn_alignbyte(w0.w, w0.z, 2), __builtin_amdgcn_alignbyte(w4, w0.w, 2)};
+            }
+            const bf8v av = __builtin_bit_cast(bf8v, x);
+            const int ii = c0 / 16 + i;
+#pragma unroll
+            for (int t = 0; t < NT; t++)
+              acc[j][ii][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, b[t], acc[j][ii][t], 0, 0, 0);
+          }
+        }
+      }
+    }
+  }
 }
 
-// db[co] = sum_v dy[v][co]
-__global__ void bias_grad_kernel(const bf16 *__restrict__ dy, int nv, int cout, float *__restrict__ db) {
-  __shared__ float red[256];
-  const int co = blockIdx.x;
-  float s = 0.f;
-  for (int v = threadIdx.x; v < nv; v += 256) s += bf2f(dy[(size_t)v * cout + co]);
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
+template <int CIN, int COUT, int TK>
+__global__ __launch_bounds__(512) void conv_dw_tile_kernel(const bf16 *__restrict__ in, const bf16 *__restrict__ dy,
+                                                          int n, int ntile, int tiles_per_block,
+                                                          float *__restrict__ partial) {
+  constexpr int KC = CIN < 32 ? CIN : 32, NI = CIN / 16, NT = COUT / 16, TRp = TK + 8;
+  constexpr int TPW = 4;  // taps per wavefront (8 wavefronts, 27 taps)
+  static_assert(CIN % 16 == 0 && COUT % 16 == 0, "channel counts");
+  __shared__ __align__(16) bf16 tileT[KC * 16 * TRp];
+  __shared__ __align__(16) bf16 dyT[COUT * 4 * TK];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m = lane & 15, kq = lane >> 4;
+  const int nk = n / TK;
+  f4v acc[TPW][NI][NT];
+#pragma unroll
+  for (int j = 0; j < TPW; j++)
+#pragma unroll
+    for (int i = 0; i < NI; i++)
+#pragma unroll
+      for (int t = 0; t < NT; t++) acc[j][i][t] = f4v{0.f, 0.f, 0.f, 0.f};
+  for (int it = 0; it < tiles_per_block; it++) {
+    const int tile = blockIdx.x + gridDim.x * it;
+    if (tile >= ntile) break;
+    int bid = tile;
+    const int k0 = (bid % nk) * TK;
+    bid /= nk;
+    const int j0 = (bid % (n / 2)) * 2, i0 = (bid / (n / 2)) * 2;
+    // DY of the tile's four runs, channel-major: item (run, part q, voxel pair)
+    __syncthreads();  // the previous tile's reads are done
+    {
+      constexpr int QO = COUT / 8, ITEMS = 4 * QO * (TK / 2);
+      for (int e = threadIdx.x; e < ITEMS; e += 512) {
+        const int vp = e % (TK / 2), q = (e / (TK / 2)) % QO, rw = e / ((TK / 2) * QO);
+        const size_t v = ((size_t)(i0 + (rw >> 1)) * n + (j0 + (rw & 1))) * n + k0 + 2 * vp;
+        const uint4 a = *reinterpret_cast<const uint4 *>(dy + v * COUT + 8 * q);
+        const uint4 b = *reinterpret_cast<const uint4 *>(dy + (v + 1) * COUT + 8 * q);
+        const uint32_t av[4] = {a.x, a.y, a.z, a.w}, bv[4] = {b.x, b.y, b.z, b.w};
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+          const uint32_t lo = (av[c >> 1] >> (16 * (c & 1))) & 0xffffu, hi = (bv[c >> 1] >> (16 * (c & 1))) & 0xffffu;
+          *reinterpret_cast<uint32_t *>(dyT + ((8 * q + c) * 4 + rw) * TK + 2 * vp) = lo | (hi << 16);
+        }
+      }
+    }
+#pragma unroll  // (a compile-time chunk offset keeps acc in registers)
+    for (int c0 = 0; c0 + 32 <= CIN; c0 += 32) {
+      if (c0 > 0) __syncthreads();
+      dw_tile_chunk<CIN, COUT, TK, 32>(in, n, c0, i0, j0, k0, tileT, dyT, acc);
+    }
+    if constexpr (CIN % 32 != 0) {
+      if (CIN > 16) __syncthreads();
+      dw_tile_chunk<CIN, COUT, TK, 16>(in, n, CIN - 16, i0, j0, k0, tileT, dyT, acc);
+    }
   }
-  if (threadIdx.x == 0) db[co] = red[0];
+  // lane holds C[4 kq + r][m] = dW[tap][ci = i 16 + 4 kq + r][co = t 16 + m]
+#pragma unroll
+  for (int j = 0; j < TPW; j++) {
+    const int tap = wave + 8 * j;
+    if (tap < 27) {
+      float *dst = partial + ((size_t)blockIdx.x * 27 + tap) * CIN * COUT;
+#pragma unroll
+      for (int i = 0; i < NI; i++)
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+#pragma unroll
+          for (int r = 0; r < 4; r++) dst[(size_t)(i * 16 + 4 * kq + r) * COUT + t * 16 + m] = acc[j][i][t][r];
+    }
+  }
 }
+
+// partial[block][co] = sum over the block's voxels of dy[v][co] (FP32); one
+// thread per (voxel, 8-channel part), 16-byte loads, then a block reduction
+// per channel through LDS
+template <int COUT>
+__global__ __launch_bounds__(256) void bias_partial_kernel(const bf16 *__restrict__ dy, int nv,
+                                                           float *__restrict__ partial) {
+  constexpr int Q = COUT / 8, VPB = 256 / Q;  // voxels per pass of the block
+  __shared__ float red[256 * 8];
+  const int q = threadIdx.x % Q, vl = threadIdx.x / Q;
+  float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (vl < VPB) {
+    for (int v = blockIdx.x * VPB + vl; v < nv; v += gridDim.x * VPB) {
+      const uint4 a = *reinterpret_cast<const uint4 *>(dy + (size_t)v * COUT + 8 * q);
+      const uint32_t av[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+      for (int c = 0; c < 8; c++) s[c] += bf2f((bf16)((av[c >> 1] >> (16 * (c & 1))) & 0xffffu));
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 8; c++) red[threadIdx.x * 8 + c] = s[c];
+  __syncthreads();
+  if (threadIdx.x < COUT) {
+    const int co = threadIdx.x, qq = co / 8, c = co % 8;
+    float t = 0.f;
+    for (int v = 0; v < VPB; v++) t += red[(v * Q + qq) * 8 + c];
+    partial[(size_t)blockIdx.x * COUT + co] = t;
+  }
+}
+
+// g[e] = sum_c partial[c][e], e < count: a workgroup takes 16 elements;
+// its 16 thread groups sum every 16th chunk from their own first one, then
+// the 16 group sums are added in group order (a fixed order)
+__global__ __launch_bounds__(256) void reduce_partials_kernel(const float *__restrict__ partial, int nchunk,
+                                                              size_t stride, size_t count, float *__restrict__ g) {
+  __shared__ float red[16][17];
+  const int el = threadIdx.x & 15, grp = threadIdx.x >> 4;
+  const size_t e = (size_t)blockIdx.x * 16 + el;
+  float s = 0.f;
+  if (e < count)
+    for (int c = grp; c < nchunk; c += 16) s += partial[(size_t)c * stride + e];
+  red[grp][el] = s;
+  __syncthreads();
+  if (grp == 0 && e < count) {
+    float t = 0.f;
+    for (int q = 0; q < 16; q++) t += red[q][el];
+    g[e] = t;
+  }
+}
+
+// 8 BF16 values (16 bytes) to floats and back
+__device__ __forceinline__ void unpack8(uint4 a, float (&f)[8]) {
+  const uint32_t w[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+  for (int c = 0; c < 8; c++) f[c] = bf2f((bf16)((w[c >> 1] >> (16 * (c & 1))) & 0xffffu));
+}
+__device__ __forceinline__ uint4 pack8(const float (&f)[8]) {
+  uint32_t w[4];
+#pragma unroll
+  for (int c = 0; c < 4; c++) w[c] = (uint32_t)f2bf(f[2 * c]) | ((uint32_t)f2bf(f[2 * c + 1]) << 16);
+  return uint4{w[0], w[1], w[2], w[3]};
+}
+__device__ __forceinline__ uint4 ld8(const bf16 *p) { return *reinterpret_cast<const uint4 *>(p); }
+__device__ __forceinline__ void st8(bf16 *p, uint4 v) { *reinterpret_cast<uint4 *>(p) = v; }
+
+// The elementwise network kernels below take one thread per (voxel, group
+// of 8 channels): 16-byte loads and stores (every channel count is a
+// multiple of 8).
 
 // average pool 2x2x2: in (n^3 x c) -> out ((n/2)^3 x c)
 __global__ void pool_kernel(const bf16 *__restrict__ in, int n, int c, bf16 *__restrict__ out) {
-  const int m = n / 2;
-  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (size_t)m * m * m * c) return;
-  const int ch = (int)(e % c);
-  const int v = (int)(e / c);
+  const int m = n / 2, c8 = c / 8;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)m * m * m * c8) return;
+  const int ch = (int)(e % c8) * 8;
+  const int v = (int)(e / c8);
   const int k = v % m, j = (v / m) % m, i = v / (m * m);
-  float s = 0.f;
-  for (int a = 0; a < 8; a++)
-    s += bf2f(in[((size_t)((2 * i + (a >> 2)) * n + 2 * j + ((a >> 1) & 1)) * n + 2 * k + (a & 1)) * c + ch]);
-  out[e] = f2bf(0.125f * s);
+  float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int a = 0; a < 8; a++) {
+    float f[8];
+    unpack8(ld8(in + ((size_t)((2 * i + (a >> 2)) * n + 2 * j + ((a >> 1) & 1)) * n + 2 * k + (a & 1)) * c + ch), f);
+#pragma unroll
+    for (int q = 0; q < 8; q++) s[q] += f[q];
+  }
+#pragma unroll
+  for (int q = 0; q < 8; q++) s[q] *= 0.125f;
+  st8(out + (size_t)v * c + ch, pack8(s));
 }
 
 // din (n^3 x c) += act'(act) / 8 dout(parent), act = the pooled tensor's
@@ -427,52 +624,67 @@ __global__ void pool_kernel(const bf16 *__restrict__ in, int n, int c, bf16 *__r
 // the concat's input-gradient convolution has already passed through act'
 __global__ void pool_bwd_kernel(const bf16 *__restrict__ dout, int n, int c, const bf16 *__restrict__ act,
                                 bf16 *__restrict__ din) {
-  const int m = n / 2;
-  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (size_t)n * n * n * c) return;
-  const int ch = (int)(e % c);
-  const int v = (int)(e / c);
+  const int m = n / 2, c8 = c / 8;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)n * n * n * c8) return;
+  const int ch = (int)(e % c8) * 8;
+  const int v = (int)(e / c8);
   const int k = v % n, j = (v / n) % n, i = v / (n * n);
-  const float slope = bf2f(act[e]) > 0.f ? 0.125f : 0.125f * LEAK;
-  din[e] = f2bf(bf2f(din[e]) + slope * bf2f(dout[((size_t)((i / 2) * m + j / 2) * m + k / 2) * c + ch]));
+  const size_t o = (size_t)v * c + ch;
+  float fa[8], fd[8], fo[8];
+  unpack8(ld8(act + o), fa);
+  unpack8(ld8(din + o), fd);
+  unpack8(ld8(dout + ((size_t)((i / 2) * m + j / 2) * m + k / 2) * c + ch), fo);
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const float slope = fa[q] > 0.f ? 0.125f : 0.125f * LEAK;
+    fd[q] = fd[q] + slope * fo[q];
+  }
+  st8(din + o, pack8(fd));
 }
 
 // concat [up(src_m) | skip]: out ((2m)^3 x (cu + cs))
 __global__ void up_concat_kernel(const bf16 *__restrict__ src, int m, int cu, const bf16 *__restrict__ skip, int cs,
                                  bf16 *__restrict__ out) {
-  const int n = 2 * m, c = cu + cs;
-  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (size_t)n * n * n * c) return;
-  const int ch = (int)(e % c);
-  const int v = (int)(e / c);
+  const int n = 2 * m, c = cu + cs, c8 = c / 8;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)n * n * n * c8) return;
+  const int ch = (int)(e % c8) * 8;
+  const int v = (int)(e / c8);
+  uint4 x;
   if (ch < cu) {
     const int k = v % n, j = (v / n) % n, i = v / (n * n);
-    out[e] = src[((size_t)((i / 2) * m + j / 2) * m + k / 2) * cu + ch];
+    x = ld8(src + ((size_t)((i / 2) * m + j / 2) * m + k / 2) * cu + ch);
   } else {
-    out[e] = skip[(size_t)v * cs + ch - cu];
+    x = ld8(skip + (size_t)v * cs + ch - cu);
   }
+  st8(out + (size_t)v * c + ch, x);
 }
 
 // split the gradient of a concat: gup (m^3 x cu) = sum of the 8 children of
 // channels [0, cu); gskip (n^3 x cs) = channels [cu, cu + cs)
 __global__ void up_concat_bwd_kernel(const bf16 *__restrict__ g, int m, int cu, int cs, bf16 *__restrict__ gup,
                                      bf16 *__restrict__ gskip) {
-  const int n = 2 * m, c = cu + cs;
-  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t nup = (size_t)m * m * m * cu, nsk = (size_t)n * n * n * cs;
+  const int n = 2 * m, c = cu + cs, cu8 = cu / 8, cs8 = cs / 8;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t nup = (size_t)m * m * m * cu8, nsk = (size_t)n * n * n * cs8;
   if (e < nup) {
-    const int ch = (int)(e % cu);
-    const int v = (int)(e / cu);
+    const int ch = (int)(e % cu8) * 8;
+    const int v = (int)(e / cu8);
     const int k = v % m, j = (v / m) % m, i = v / (m * m);
-    float s = 0.f;
-    for (int a = 0; a < 8; a++)
-      s += bf2f(g[((size_t)((2 * i + (a >> 2)) * n + 2 * j + ((a >> 1) & 1)) * n + 2 * k + (a & 1)) * c + ch]);
-    gup[e] = f2bf(s);
+    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int a = 0; a < 8; a++) {
+      float f[8];
+      unpack8(ld8(g + ((size_t)((2 * i + (a >> 2)) * n + 2 * j + ((a >> 1) & 1)) * n + 2 * k + (a & 1)) * c + ch), f);
+#pragma unroll
+      for (int q = 0; q < 8; q++) s[q] += f[q];
+    }
+    st8(gup + (size_t)v * cu + ch, pack8(s));
   } else if (e < nup + nsk) {
     const size_t q = e - nup;
-    const int ch = (int)(q % cs);
-    const size_t v = q / cs;
-    gskip[q] = g[v * c + cu + ch];
+    const int ch = (int)(q % cs8) * 8;
+    const size_t v = q / cs8;
+    st8(gskip + v * cs + ch, ld8(g + v * c + cu + ch));
   }
 }
 
@@ -611,6 +823,15 @@ struct PL {
 constexpr int NLAYER = 9;
 constexpr int RED_BLOCKS = 256;
 constexpr int DW_CHUNK = 4096;
+constexpr int BIAS_BLOCKS = 512;
+
+// workgroups of the tiled weight gradient on an n^3 grid (n >= 32): at most
+// 256, each walking a fixed list of tiles
+int dw_blocks(int n) {
+  const int tk = n >= 64 ? 64 : 32;
+  const int ntile = (int)((size_t)n * n * n / (4 * tk));
+  return ntile < 256 ? ntile : 256;
+}
 
 }  // namespace
 
@@ -714,15 +935,43 @@ void conv(kapsel_ctx *ctx, int cin, int cout, const bf16 *in, int n, const bf16 
 template <int CIN, int COUT>
 void dw_launch(kapsel_ctx *ctx, Pnet *pn, const bf16 *in, const bf16 *dy, int n) {
   const size_t nv = nvox(n);
-  const int nchunk = (int)((nv + DW_CHUNK - 1) / DW_CHUNK);
-  const int chunk = (int)((nv + nchunk - 1) / nchunk + 15) / 16 * 16;
   LaunchTimer lt(ctx, KAPSEL_K_NETWORK, (double)nv * 2 * (CIN + COUT) * 27);
-  hipLaunchKernelGGL((conv_dw_kernel<CIN, COUT>), dim3(27, nchunk), dim3(256), 0, ctx->stream, in, dy, n, chunk,
-                     pn->partial);
+  int nchunk;
+  if (n >= 32) {
+    // the tiled form: tiles of 2 x 2 runs of TK voxels, a fixed list per workgroup
+    const int tk = n >= 64 ? 64 : 32;
+    const int ntile = (int)(nv / (4 * tk));
+    nchunk = dw_blocks(n);
+    const int per = (ntile + nchunk - 1) / nchunk;
+    if (tk == 64)
+      hipLaunchKernelGGL((conv_dw_tile_kernel<CIN, COUT, 64>), dim3(nchunk), dim3(512), 0, ctx->stream, in, dy, n,
+                         ntile, per, pn->partial);
+    else
+      hipLaunchKernelGGL((conv_dw_tile_kernel<CIN, COUT, 32>), dim3(nchunk), dim3(512), 0, ctx->stream, in, dy, n,
+                         ntile, per, pn->partial);
+  } else {
+    nchunk = (int)((nv + DW_CHUNK - 1) / DW_CHUNK);
+    const int chunk = (int)((nv + nchunk - 1) / nchunk + 15) / 16 * 16;
+    hipLaunchKernelGGL((conv_dw_kernel<CIN, COUT>), dim3(27, nchunk), dim3(256), 0, ctx->stream, in, dy, n, chunk,
+                       pn->partial);
+  }
   KAPSEL_LAUNCH_CHECK();
   const size_t cnt = (size_t)27 * CIN * COUT;
-  hipLaunchKernelGGL(reduce_partials_kernel, dim3(blocks_for(cnt, 256)), dim3(256), 0, ctx->stream, pn->partial,
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(blocks_for(cnt, 16)), dim3(256), 0, ctx->stream, pn->partial,
                      nchunk, cnt, cnt, pn->dw);
+  KAPSEL_LAUNCH_CHECK();
+}
+
+// db[co] = sum_v dy[v][co]: per-workgroup FP32 partials, summed in a fixed order
+template <int COUT>
+void bias_grad(kapsel_ctx *ctx, Pnet *pn, const bf16 *dy, int n) {
+  const int nv = (int)nvox(n);
+  constexpr int VPB = 256 / (COUT / 8);
+  const int nb = std::min(BIAS_BLOCKS, (nv + VPB - 1) / VPB);
+  hipLaunchKernelGGL((bias_partial_kernel<COUT>), dim3(nb), dim3(256), 0, ctx->stream, dy, nv, pn->partial);
+  KAPSEL_LAUNCH_CHECK();
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(blocks_for(COUT, 16)), dim3(256), 0, ctx->stream, pn->partial, nb, (size_t)COUT,
+                     (size_t)COUT, pn->db);
   KAPSEL_LAUNCH_CHECK();
 }
 
@@ -740,9 +989,13 @@ void layer_grad(kapsel_ctx *ctx, Pnet *pn, int l, const bf16 *in, const bf16 *dy
     throw Error{KAPSEL_ERR_STATE};
   }
 #undef KAPSEL_DW_CASE
-  hipLaunchKernelGGL(bias_grad_kernel, dim3(q.coutp), dim3(256), 0, ctx->stream, dy, (int)nvox(n), q.coutp,
-                     pn->db);
-  KAPSEL_LAUNCH_CHECK();
+  switch (q.coutp) {
+    case 16: bias_grad<16>(ctx, pn, dy, n); break;
+    case 32: bias_grad<32>(ctx, pn, dy, n); break;
+    default:
+      set_error("no bias-gradient kernel for %d channels", q.coutp);
+      throw Error{KAPSEL_ERR_STATE};
+  }
   const size_t cnt = (size_t)27 * q.cin * q.cout;
   hipLaunchKernelGGL(unpack_grad_kernel, dim3(blocks_for(cnt > (size_t)q.cout ? cnt : q.cout, 256)), dim3(256), 0,
                      ctx->stream, pn->dw, pn->db, q.woff, q.boff, q.cin, q.cout, q.cinp, q.coutp, pn->grad);
@@ -770,10 +1023,10 @@ void forward(kapsel_ctx *ctx, Pnet *pn, bool concat = true) {
   const PL *L = pn->L;
   conv(ctx, 16, 16, pn->x0, n0, pn->kf[0], pn->bias[0], 1, 1, nullptr, pn->a1, nullptr);
   conv(ctx, 16, 16, pn->a1, n0, pn->kf[1], pn->bias[1], 1, 1, nullptr, pn->s1, nullptr);
-  hipLaunchKernelGGL(pool_kernel, dim3(eb(nvox(n1) * 16)), dim3(256), 0, ctx->stream, pn->s1, n0, 16, pn->p1);
+  hipLaunchKernelGGL(pool_kernel, dim3(eb(nvox(n1) * 2)), dim3(256), 0, ctx->stream, pn->s1, n0, 16, pn->p1);
   conv(ctx, 16, 32, pn->p1, n1, pn->kf[2], pn->bias[2], 1, 1, nullptr, pn->a2, nullptr);
   conv(ctx, 32, 32, pn->a2, n1, pn->kf[3], pn->bias[3], 1, 1, nullptr, pn->s2, nullptr);
-  hipLaunchKernelGGL(pool_kernel, dim3(eb(nvox(n2) * 32)), dim3(256), 0, ctx->stream, pn->s2, n1, 32, pn->p2);
+  hipLaunchKernelGGL(pool_kernel, dim3(eb(nvox(n2) * 4)), dim3(256), 0, ctx->stream, pn->s2, n1, 32, pn->p2);
   conv(ctx, 32, 32, pn->p2, n2, pn->kf[4], pn->bias[4], 1, 1, nullptr, pn->a3, nullptr);
   conv(ctx, 32, 32, pn->a3, n2, pn->kf[5], pn->bias[5], 1, 1, nullptr, pn->a4, nullptr);
   const bool fuse = !concat && n1 >= 16;
@@ -781,10 +1034,10 @@ void forward(kapsel_ctx *ctx, Pnet *pn, bool concat = true) {
     conv(ctx, 64, 32, pn->s2, n1, pn->kf[6], pn->bias[6], 1, 1, nullptr, pn->d2, nullptr, pn->a4, 32);
     conv(ctx, 48, 16, pn->s1, n0, pn->kf[7], pn->bias[7], 1, 1, nullptr, pn->d1, nullptr, pn->d2, 32);
   } else {
-    hipLaunchKernelGGL(up_concat_kernel, dim3(eb(nvox(n1) * 64)), dim3(256), 0, ctx->stream, pn->a4, n2, 32,
+    hipLaunchKernelGGL(up_concat_kernel, dim3(eb(nvox(n1) * 8)), dim3(256), 0, ctx->stream, pn->a4, n2, 32,
                        pn->s2, 32, pn->c2);
     conv(ctx, 64, 32, pn->c2, n1, pn->kf[6], pn->bias[6], 1, 1, nullptr, pn->d2, nullptr);
-    hipLaunchKernelGGL(up_concat_kernel, dim3(eb(nvox(n0) * 48)), dim3(256), 0, ctx->stream, pn->d2, n1, 32,
+    hipLaunchKernelGGL(up_concat_kernel, dim3(eb(nvox(n0) * 6)), dim3(256), 0, ctx->stream, pn->d2, n1, 32,
                        pn->s1, 16, pn->c1);
     conv(ctx, 48, 16, pn->c1, n0, pn->kf[7], pn->bias[7], 1, 1, nullptr, pn->d1, nullptr);
   }
@@ -801,12 +1054,12 @@ void backward(kapsel_ctx *ctx, Pnet *pn) {
   // d1 (7): input c1 = [up(d2) | s1]
   layer_grad(ctx, pn, 7, pn->c1, pn->gd1, n0);
   conv(ctx, 16, 48, pn->gd1, n0, pn->kb[7], nullptr, 0, -1, pn->c1, pn->gc1, nullptr);
-  hipLaunchKernelGGL(up_concat_bwd_kernel, dim3(eb(nvox(n1) * 32 + nvox(n0) * 16)), dim3(256), 0, ctx->stream,
+  hipLaunchKernelGGL(up_concat_bwd_kernel, dim3(eb(nvox(n1) * 4 + nvox(n0) * 2)), dim3(256), 0, ctx->stream,
                      pn->gc1, n1, 32, 16, pn->gd2, pn->gs1);
   // d2 (6): input c2 = [up(a4) | s2]
   layer_grad(ctx, pn, 6, pn->c2, pn->gd2, n1);
   conv(ctx, 32, 64, pn->gd2, n1, pn->kb[6], nullptr, 0, -1, pn->c2, pn->gc2, nullptr);
-  hipLaunchKernelGGL(up_concat_bwd_kernel, dim3(eb(nvox(n2) * 32 + nvox(n1) * 32)), dim3(256), 0, ctx->stream,
+  hipLaunchKernelGGL(up_concat_bwd_kernel, dim3(eb(nvox(n2) * 4 + nvox(n1) * 4)), dim3(256), 0, ctx->stream,
                      pn->gc2, n2, 32, 32, pn->ga4, pn->gs2);
   // e3b (5): input a3
   layer_grad(ctx, pn, 5, pn->a3, pn->ga4, n2);
@@ -814,7 +1067,7 @@ void backward(kapsel_ctx *ctx, Pnet *pn) {
   // e3a (4): input p2 (pooled, not a ReLU output: no mask)
   layer_grad(ctx, pn, 4, pn->p2, pn->ga3, n2);
   conv(ctx, 32, 32, pn->ga3, n2, pn->kb[4], nullptr, 0, -1, nullptr, pn->gp2, nullptr);
-  hipLaunchKernelGGL(pool_bwd_kernel, dim3(eb(nvox(n1) * 32)), dim3(256), 0, ctx->stream, pn->gp2, n1, 32, pn->s2,
+  hipLaunchKernelGGL(pool_bwd_kernel, dim3(eb(nvox(n1) * 4)), dim3(256), 0, ctx->stream, pn->gp2, n1, 32, pn->s2,
                      pn->gs2);
   // e2b (3): input a2
   layer_grad(ctx, pn, 3, pn->a2, pn->gs2, n1);
@@ -822,7 +1075,7 @@ void backward(kapsel_ctx *ctx, Pnet *pn) {
   // e2a (2): input p1
   layer_grad(ctx, pn, 2, pn->p1, pn->ga2, n1);
   conv(ctx, 32, 16, pn->ga2, n1, pn->kb[2], nullptr, 0, -1, nullptr, pn->gp1, nullptr);
-  hipLaunchKernelGGL(pool_bwd_kernel, dim3(eb(nvox(n0) * 16)), dim3(256), 0, ctx->stream, pn->gp1, n0, 16, pn->s1,
+  hipLaunchKernelGGL(pool_bwd_kernel, dim3(eb(nvox(n0) * 2)), dim3(256), 0, ctx->stream, pn->gp1, n0, 16, pn->s1,
                      pn->gs1);
   // e1b (1): input a1
   layer_grad(ctx, pn, 1, pn->a1, pn->gs1, n0);
@@ -934,9 +1187,11 @@ void pnet_init(kapsel_ctx *ctx) {
   for (int l = 0; l < NLAYER; l++) {
     const PL &q = pn->L[l];
     const size_t nv = nvox(pn->n >> q.level);
-    const size_t nchunk = (nv + DW_CHUNK - 1) / DW_CHUNK;
+    const int nl = pn->n >> q.level;
+    const size_t nchunk = nl >= 32 ? (size_t)dw_blocks(nl) : (nv + DW_CHUNK - 1) / DW_CHUNK;
     const size_t need = nchunk * 27 * q.cinp * q.coutp;
     maxpart = need > maxpart ? need : maxpart;
+    maxpart = (size_t)BIAS_BLOCKS * q.coutp > maxpart ? (size_t)BIAS_BLOCKS * q.coutp : maxpart;
   }
   pn->partial = palloc<float>(pn, maxpart * 4);
   pn->dw = palloc<float>(pn, maxw * 4);
