@@ -362,6 +362,47 @@ __device__ __forceinline__ void plane_tw_load(PlaneTw<T, NY, NZ> &tw, const V2<T
   for (int e = threadIdx.x; e < NZ / 2 + 1; e += THREADS) tw.zrot[e] = zrot[e];
   for (int e = threadIdx.x; e < NY; e += THREADS) tw.twy[e] = twy[YS * e];
 }
+// the same in two halves, for a workgroup of at least NZ/2 + 1 and NY
+// threads (one entry of each table per thread): the loads into registers,
+// then, after the kernel's other loads are issued, the stores to LDS
+template <typename T>
+struct PlaneTwRegs {
+  V2<T> m, z, y;
+};
+template <typename T, int NY, int NZ, int THREADS, int YS = 1>
+__device__ __forceinline__ PlaneTwRegs<T> plane_tw_fetch(const V2<T> *__restrict__ twm,
+                                                         const V2<T> *__restrict__ zrot,
+                                                         const V2<T> *__restrict__ twy) {
+  static_assert(THREADS >= NZ / 2 + 1 && THREADS >= NY, "one table entry per thread");
+  const int e = threadIdx.x;
+  PlaneTwRegs<T> r;
+  if (e < NZ / 2) r.m = twm[e];
+  if (e < NZ / 2 + 1) r.z = zrot[e];
+  if (e < NY) r.y = twy[YS * e];
+  return r;
+}
+template <typename T, int NY, int NZ>
+__device__ __forceinline__ void plane_tw_put(PlaneTw<T, NY, NZ> &tw, const PlaneTwRegs<T> &r) {
+  const int e = threadIdx.x;
+  if (e < NZ / 2) tw.twm[e] = r.m;
+  if (e < NZ / 2 + 1) tw.zrot[e] = r.z;
+  if (e < NY) tw.twy[e] = r.y;
+}
+// plane_tw_load with the loads of issue() issued between the table loads
+// and their LDS stores (where one entry per thread covers the tables)
+template <typename T, int NY, int NZ, int THREADS, typename Issue>
+__device__ __forceinline__ void plane_tw_load_around(PlaneTw<T, NY, NZ> &tw, const V2<T> *__restrict__ twm,
+                                                     const V2<T> *__restrict__ zrot, const V2<T> *__restrict__ twy,
+                                                     Issue issue) {
+  if constexpr (THREADS >= NZ / 2 + 1 && THREADS >= NY) {
+    const PlaneTwRegs<T> r = plane_tw_fetch<T, NY, NZ, THREADS>(twm, zrot, twy);
+    issue();
+    plane_tw_put<T, NY, NZ>(tw, r);
+  } else {
+    issue();
+    plane_tw_load<T, NY, NZ, THREADS>(tw, twm, zrot, twy);
+  }
+}
 
 // Every lane's LDS accesses before this point precede every lane's after
 // it: the LDS operations of one wavefront are performed in issue order, and
@@ -466,18 +507,36 @@ __device__ __forceinline__ void row_c2r(V2<T> *row, const V2<T> *twm, const V2<T
 // the two inverse transforms as the result (what dropping their imaginary
 // parts gives).  Wavefront 0 does it through pk (2 NY contiguous slots, so
 // its 64 lanes meet distinct banks); a workgroup barrier must follow.
-template <typename T, int NY, int NZ, int THREADS>
-__device__ __forceinline__ void plane_pack_in(V2<T> *pk, const T *__restrict__ g) {
-  using C = PlaneCfg<T, NY, NZ, THREADS>;
-  constexpr int H = C::H, M = C::M, NJ = (NY + 63) / 64;
-  if (threadIdx.x >= 64) return;
+// (plane_pack_fetch: the loads of wavefront 0; plane_pack_fold: the rest,
+// so that a kernel can issue its other loads in between)
+template <typename T, int NY>
+struct PlanePackRegs {
+  static constexpr int NJ = (NY + 63) / 64;
   V2<T> a[NJ], b[NJ];
+};
+template <typename T, int NY, int NZ, int THREADS>
+__device__ __forceinline__ void plane_pack_fetch(PlanePackRegs<T, NY> &r, const T *__restrict__ g) {
+  using C = PlaneCfg<T, NY, NZ, THREADS>;
+  constexpr int H = C::H, M = C::M, NJ = PlanePackRegs<T, NY>::NJ;
+  if (threadIdx.x >= 64) return;
 #pragma unroll
   for (int q = 0; q < NJ; q++) {
     const int j = threadIdx.x + 64 * q;
     if (j < NY) {
-      a[q] = ld2(g + 2 * (size_t)j * H);
-      b[q] = ld2(g + 2 * ((size_t)j * H + M));
+      r.a[q] = ld2(g + 2 * (size_t)j * H);
+      r.b[q] = ld2(g + 2 * ((size_t)j * H + M));
+    }
+  }
+}
+template <typename T, int NY, int NZ, int THREADS>
+__device__ __forceinline__ void plane_pack_fold(V2<T> *pk, const PlanePackRegs<T, NY> &r) {
+  constexpr int NJ = PlanePackRegs<T, NY>::NJ;
+  if (threadIdx.x >= 64) return;
+  const V2<T> *a = r.a, *b = r.b;
+#pragma unroll
+  for (int q = 0; q < NJ; q++) {
+    const int j = threadIdx.x + 64 * q;
+    if (j < NY) {
       pk[j] = a[q];
       pk[NY + j] = b[q];
     }
@@ -501,6 +560,12 @@ __device__ __forceinline__ void plane_pack_in(V2<T> *pk, const T *__restrict__ g
     if (j < NY) pk[j] = c[q];
   }
 }
+template <typename T, int NY, int NZ, int THREADS>
+__device__ __forceinline__ void plane_pack_in(V2<T> *pk, const T *__restrict__ g) {
+  PlanePackRegs<T, NY> r;
+  plane_pack_fetch<T, NY, NZ, THREADS>(r, g);
+  plane_pack_fold<T, NY, NZ, THREADS>(pk, r);
+}
 
 // y transform of every column of the plane in P (in place), direction DIR.
 // Column 0 is the packed pair of the kz = 0 and kz = M columns (M columns
@@ -511,8 +576,17 @@ __device__ __forceinline__ void plane_pack_in(V2<T> *pk, const T *__restrict__ g
 // (C[j] + conj C[-j]) / 2, X_M = (C[j] - conj C[-j]) / 2i.  Consecutive
 // lanes take consecutive columns, so the lanes of a wavefront touch
 // consecutive LDS words and, in global memory, one contiguous run.
-template <typename T, int NY, int NZ, int THREADS, int DIR, bool GIN = false, bool GOUT = false>
-__device__ __forceinline__ void plane_columns(V2<T> *P, V2<T> *pk, const V2<T> *__restrict__ twy, T *g = nullptr) {
+// first(): called by every thread once, in the first round after the
+// loads from global memory (GIN) are issued and before anything is read
+// from LDS -- for a kernel's set-up (LDS stores, a barrier) to wait for
+// those loads together with its own.
+struct PlaneNoSetup {
+  __device__ void operator()() const {}
+};
+template <typename T, int NY, int NZ, int THREADS, int DIR, bool GIN = false, bool GOUT = false,
+          typename First = PlaneNoSetup>
+__device__ __forceinline__ void plane_columns(V2<T> *P, V2<T> *pk, const V2<T> *__restrict__ twy, T *g = nullptr,
+                                              First first = First()) {
   using C = PlaneCfg<T, NY, NZ, THREADS>;
   constexpr int H = C::H, HP = C::HP, M = C::M, R1 = C::YR1, R2 = C::YR2;
   const int t = threadIdx.x;
@@ -527,16 +601,17 @@ __device__ __forceinline__ void plane_columns(V2<T> *P, V2<T> *pk, const V2<T> *
     const bool packed = kz == 0;
     V2<T> *col = P + (valid ? kz : 0);
     T *gcol = g + 2 * (valid ? kz : 0);
-    V2<T> o[R1];
+    V2<T> o[R1], v[R1];
+    if (GIN && valid && !packed) {
+#pragma unroll
+      for (int n1 = 0; n1 < R1; n1++) v[n1] = ld2(gcol + 2 * (R2 * n1 + n2) * H);
+    }
+    if (c0 == 0) first();
     if (valid) {
-      V2<T> v[R1];
       if (GIN && packed) {
 #pragma unroll
         for (int n1 = 0; n1 < R1; n1++) v[n1] = pk[R2 * n1 + n2];
-      } else if (GIN) {
-#pragma unroll
-        for (int n1 = 0; n1 < R1; n1++) v[n1] = ld2(gcol + 2 * (R2 * n1 + n2) * H);
-      } else {
+      } else if (!GIN) {
 #pragma unroll
         for (int n1 = 0; n1 < R1; n1++) v[n1] = col[(R2 * n1 + n2) * HP];
       }
@@ -601,6 +676,33 @@ constexpr int plane_waves_per_simd() {
 }
 #define KAPSEL_PLANE_BOUNDS __launch_bounds__(THREADS, (plane_waves_per_simd<T, NY, NZ, THREADS>()))
 
+// The set-up of a plane pass that starts with the y transforms of a plane
+// in global memory (at g), and those transforms.  All units of a launch
+// start together, so loads waited for one after the other would cost a
+// memory latency each: the loads of the packed columns and of the
+// twiddles, whose LDS stores wait for them, and those of the first round
+// of columns are all issued before the first wait.
+template <typename T, int NY, int NZ, int THREADS, int DIR, int YS = 1>
+__device__ __forceinline__ void plane_prologue_columns(V2<T> *P, V2<T> *pk, PlaneTw<T, NY, NZ> &tw,
+                                                       const V2<T> *__restrict__ twm, const V2<T> *__restrict__ zrot,
+                                                       const V2<T> *__restrict__ twy, T *g) {
+  if constexpr (THREADS >= NZ / 2 + 1 && THREADS >= NY) {
+    PlanePackRegs<T, NY> pr;
+    plane_pack_fetch<T, NY, NZ, THREADS>(pr, g);
+    const PlaneTwRegs<T> twr = plane_tw_fetch<T, NY, NZ, THREADS, YS>(twm, zrot, twy);
+    plane_columns<T, NY, NZ, THREADS, DIR, true, false>(P, pk, tw.twy, g, [&] {
+      plane_pack_fold<T, NY, NZ, THREADS>(pk, pr);
+      plane_tw_put<T, NY, NZ>(tw, twr);
+      __syncthreads();
+    });
+  } else {
+    plane_pack_in<T, NY, NZ, THREADS>(pk, g);
+    plane_tw_load<T, NY, NZ, THREADS, YS>(tw, twm, zrot, twy);
+    __syncthreads();
+    plane_columns<T, NY, NZ, THREADS, DIR, true, false>(P, pk, tw.twy, g);
+  }
+}
+
 template <typename T, int NY, int NZ, int THREADS>
 __global__ KAPSEL_PLANE_BOUNDS void fft_plane_r2c(FieldPtrs<T> fp, const V2<T> *__restrict__ twm,
                                                            const V2<T> *__restrict__ zrot,
@@ -612,13 +714,20 @@ __global__ KAPSEL_PLANE_BOUNDS void fft_plane_r2c(FieldPtrs<T> fp, const V2<T> *
   unsigned field;
   const unsigned plane = xcd_tile(field);
   T *__restrict__ gp = fp.p[field] + (size_t)plane * 2 * C::PLANE;
-  plane_tw_load<T, NY, NZ, THREADS>(tw, twm, zrot, twy);
-  __syncthreads();
-  KAPSEL_PLANE_ROWS(C) {
+  const int n2 = threadIdx.x % 64 % C::ZR2, j0 = threadIdx.x / 64 * C::RPW + threadIdx.x % 64 / C::ZR2;
+  V2<T> v[C::ZR1];
+  auto row_in = [&](int j) {
     const T *grow = gp + (size_t)j * 2 * C::H;
-    V2<T> v[C::ZR1];
 #pragma unroll
     for (int n1 = 0; n1 < C::ZR1; n1++) v[n1] = ld2(grow + 2 * (C::ZR2 * n1 + n2));
+  };
+  // the first row's loads go out with the twiddles'
+  plane_tw_load_around<T, NY, NZ, THREADS>(tw, twm, zrot, twy, [&] {
+    if (j0 < NY) row_in(j0);
+  });
+  __syncthreads();
+  for (int j = j0; j < NY; j += C::NW * C::RPW) {
+    if (j != j0) row_in(j);
     row_r2c<T, C::M, C::ZR1, C::ZR2>(P + j * C::HP, v, tw.twm, tw.zrot, n2);
   }
   __syncthreads();
@@ -654,11 +763,10 @@ __global__ KAPSEL_PLANE_BOUNDS void fft_plane_r2c_pro(FieldPtrs<T> out, FieldPtr
   const T *gc = c.p[field] ? c.p[field] + base : nullptr;
   const bool diff = (dsq >> field) & 1;
   const T ef = e.v[field];
-  plane_tw_load<T, NY, NZ, THREADS>(tw, twm, zrot, twy);
-  __syncthreads();
-  KAPSEL_PLANE_ROWS(C) {
+  const int n2 = threadIdx.x % 64 % C::ZR2, j0 = threadIdx.x / 64 * C::RPW + threadIdx.x % 64 / C::ZR2;
+  V2<T> v[C::ZR1];
+  auto row_in = [&](int j) {
     const size_t ro = (size_t)j * 2 * C::H;
-    V2<T> v[C::ZR1];
 #pragma unroll
     for (int n1 = 0; n1 < C::ZR1; n1++) {
       const size_t o = ro + 2 * (C::ZR2 * n1 + n2);
@@ -676,6 +784,14 @@ __global__ KAPSEL_PLANE_BOUNDS void fft_plane_r2c_pro(FieldPtrs<T> out, FieldPtr
         v[n1] = V2<T>{s * vb.re, s * vb.im};
       }
     }
+  };
+  // the first row's loads go out with the twiddles'
+  plane_tw_load_around<T, NY, NZ, THREADS>(tw, twm, zrot, twy, [&] {
+    if (j0 < NY) row_in(j0);
+  });
+  __syncthreads();
+  for (int j = j0; j < NY; j += C::NW * C::RPW) {
+    if (j != j0) row_in(j);
     row_r2c<T, C::M, C::ZR1, C::ZR2>(P + j * C::HP, v, tw.twm, tw.zrot, n2);
   }
   __syncthreads();
@@ -693,10 +809,7 @@ __global__ KAPSEL_PLANE_BOUNDS void fft_plane_c2r(FieldPtrs<T> fp, const V2<T> *
   unsigned field;
   const unsigned plane = xcd_tile(field);
   T *__restrict__ gp = fp.p[field] + (size_t)plane * 2 * C::PLANE;
-  plane_pack_in<T, NY, NZ, THREADS>(pk, gp);
-  plane_tw_load<T, NY, NZ, THREADS>(tw, twm, zrot, twy);
-  __syncthreads();
-  plane_columns<T, NY, NZ, THREADS, +1, true, false>(P, pk, tw.twy, gp);
+  plane_prologue_columns<T, NY, NZ, THREADS, +1>(P, pk, tw, twm, zrot, twy, gp);
   KAPSEL_PLANE_ROWS(C) {
     T *grow = gp + (size_t)j * 2 * C::H;
     row_c2r<T, C::M, C::ZR1, C::ZR2>(P + j * C::HP, tw.twm, tw.zrot, scale, n2,
@@ -725,10 +838,7 @@ __global__ KAPSEL_PLANE_BOUNDS void fft_plane_face(FieldPtrs<T> fp, const V2<T> 
   const unsigned plane = xcd_tile(field);
   T *__restrict__ gp = fp.p[field] + (size_t)plane * 2 * C::PLANE;
   const T *fpl = face.p[field] + (size_t)plane * 2 * C::PLANE;
-  plane_pack_in<T, NY, NZ, THREADS>(pk, gp);
-  plane_tw_load<T, NY, NZ, THREADS>(tw, twm, zrot, twy);
-  __syncthreads();
-  plane_columns<T, NY, NZ, THREADS, +1, true, false>(P, pk, tw.twy, gp);
+  plane_prologue_columns<T, NY, NZ, THREADS, +1>(P, pk, tw, twm, zrot, twy, gp);
   KAPSEL_PLANE_ROWS(C) {
     const T *frow = fpl + (size_t)j * 2 * C::H;
     V2<T> fv[C::ZR1], v[C::ZR1];
@@ -774,12 +884,11 @@ __global__ __launch_bounds__(THREADS, (half_waves_per_simd<T, NY, NZ, THREADS>()
   const unsigned plane = u >> 1, hf = u & 1;
   T *__restrict__ gp = fp.p[field] + (size_t)plane * 2 * (NY * C::H) + (size_t)hf * 2 * C::PLANE;
   const T *fpl = face.p[field] + (size_t)plane * 2 * (NY * C::H);
-  plane_pack_in<T, J, NZ, THREADS>(pk, gp);
-  plane_tw_load<T, J, NZ, THREADS, 2>(tw, twm, zrot, twy);
-  __syncthreads();
-  plane_columns<T, J, NZ, THREADS, +1, true, false>(P, pk, tw.twy, gp);
+  plane_prologue_columns<T, J, NZ, THREADS, +1, 2>(P, pk, tw, twm, zrot, twy, gp);
   const int n2 = threadIdx.x % 64 % C::ZR2;
   for (int m = threadIdx.x / 64 * C::RPW + threadIdx.x % 64 / C::ZR2; m < J; m += C::NW * C::RPW) {
+    // the face row is loaded here: issued with the columns it would slow
+    // their arrival, which the first transforms wait for
     const T *frow = fpl + (size_t)(2 * m + hf) * 2 * C::H;
     V2<T> fv[C::ZR1], v[C::ZR1];
 #pragma unroll
