@@ -1032,6 +1032,45 @@ __device__ __forceinline__ double block_sum_fft(double v) {
   return s;
 }
 
+// The P points of a line's row of p = z + beta p (update; stored), of p =
+// z - sub (sub; stored) or of z (neither), at the offsets off(n): every
+// load is issued before the first store (a store to p may alias a later
+// point's load for all the compiler knows, which would otherwise hold each
+// point's loads until the store before it is issued).  Zero for an invalid
+// line.
+template <typename T, int P, typename Off>
+__device__ __forceinline__ void p_update_row(V2<T> (&x)[P], const T *z, T *p, const float *sub, bool update, T beta,
+                                             bool valid, Off off) {
+  if (!valid) {
+#pragma unroll
+    for (int n = 0; n < P; n++) x[n] = V2<T>{0, 0};
+    return;
+  }
+  if (update) {
+    V2<T> b[P];
+#pragma unroll
+    for (int n = 0; n < P; n++) x[n] = ld2(z + 2 * off(n));
+#pragma unroll
+    for (int n = 0; n < P; n++) b[n] = ld2(p + 2 * off(n));
+#pragma unroll
+    for (int n = 0; n < P; n++) x[n] = x[n] + scal(beta, b[n]);
+  } else if (sub) {
+    V2<float> d[P];
+#pragma unroll
+    for (int n = 0; n < P; n++) x[n] = ld2(z + 2 * off(n));
+#pragma unroll
+    for (int n = 0; n < P; n++) d[n] = ld2(sub + 2 * off(n));
+#pragma unroll
+    for (int n = 0; n < P; n++) x[n] = V2<T>{x[n].re - (T)d[n].re, x[n].im - (T)d[n].im};
+  } else {
+#pragma unroll
+    for (int n = 0; n < P; n++) x[n] = ld2(z + 2 * off(n));
+    return;
+  }
+#pragma unroll
+  for (int n = 0; n < P; n++) st2(p + 2 * off(n), x[n]);
+}
+
 // p = z + sc[beta] p (stored), then g_d = IFFT_x(i k_d e^{i k_d dx/2} p), d = x, y, z
 // (thread t of a tile: line t % LPB, its thread s = t / LPB); with sub (and
 // no beta): p = z - sub (stored), the refinement's x -= d before its residual
@@ -1054,58 +1093,63 @@ __global__ __launch_bounds__((LPB * LineFFT<T, L>::S)) void fft_x_grad(
   const bool update = beta_slot >= 0;  // else the gradient of z itself, p untouched
   const T beta = update ? (T)sc[beta_slot] : (T)0;
   V2<T> pv[P];
-#pragma unroll
-  for (int n = 0; n < P; n++) {
-    const long long off = base + (long long)F::in_i(s, n) * plane;
-    if (valid) {
-      pv[n] = ld2(z + 2 * off);
-      if (update) {
-        pv[n] = pv[n] + scal(beta, ld2(p + 2 * off));
-        st2(p + 2 * off, pv[n]);
-      } else if (sub) {
-        const V2<float> d = ld2(sub + 2 * off);
-        pv[n] = V2<T>{pv[n].re - (T)d.re, pv[n].im - (T)d.im};
-        st2(p + 2 * off, pv[n]);
-      }
-    } else {
-      pv[n] = V2<T>{0, 0};
-    }
-  }
+  p_update_row<T, P>(pv, z, p, sub, update, beta, valid,
+                     [&](int n) { return base + (long long)F::in_i(s, n) * plane; });
   V2<T> *sl = sm + l * F::LS;
   V2<T> twr[P];
   line_twiddles<F>(twr, tw, s);
+  // G_x = IFFT(i k_x e^{i k_x dx/2} p); the y and z symbols are constant
+  // along the line: G_y = c_y IFFT(p), G_z = c_z IFFT(p) (two transforms)
+  V2<T> v[P];
 #pragma unroll
-  for (int d = 0; d < 3; d++) {
-    V2<T> v[P];
+  for (int n = 0; n < P; n++) {
+    const int i = F::in_i(s, n);
+    const T w = (T)(signed_k(i, L) * wx);
+    const V2<T> e = shx[i];
+    v[n] = cmul(pv[n], V2<T>{-w * e.im, w * e.re});
+  }
+  F::template run<+1>(v, sl, twr, s, LPB);
+  if (valid) {  // (in flight while IFFT(p) runs)
 #pragma unroll
-    for (int n = 0; n < P; n++) {
-      const int i = F::in_i(s, n);
-      const T w = d == 0 ? (T)(signed_k(i, L) * wx) : d == 1 ? (T)(signed_k(j, ny) * wy) : (T)(kz * wz);
-      const V2<T> sft = d == 0 ? shx[i] : d == 1 ? shy[j] : shz[kz];
-      v[n] = cmul(V2<T>{-w * pv[n].im, w * pv[n].re}, sft);
-    }
-    F::template run<+1>(v, sl, twr, s, LPB);
-    if (valid) {
-      T *__restrict__ gd = g3.p[d];
+    for (int m = 0; m < P; m++) st2(g3.p[0] + 2 * (base + (long long)F::out_i(s, m) * plane), v[m]);
+  }
+  F::template run<+1>(pv, sl, twr, s, LPB);
+  const T ky = (T)(signed_k(j, ny) * wy), kzw = (T)(kz * wz);
+  const V2<T> ey = shy[j], ez = shz[kz];
+  const V2<T> cy{-ky * ey.im, ky * ey.re}, cz{-kzw * ez.im, kzw * ez.re};
+  if (valid) {
 #pragma unroll
-      for (int m = 0; m < P; m++) st2(gd + 2 * (base + (long long)F::out_i(s, m) * plane), v[m]);
+    for (int m = 0; m < P; m++) {
+      const long long off = 2 * (base + (long long)F::out_i(s, m) * plane);
+      st2(g3.p[1] + off, cmul(pv[m], cy));
+      st2(g3.p[2] + off, cmul(pv[m], cz));
     }
   }
 }
+
+// the line transforms of fft_x_div and fft_x_div_dec: the pair split at 128
+// points in FP64 only (in FP32 the four-step 16 x 8 measured faster for
+// these kernels; fft_x_grad_dec is faster with the split in both)
+template <typename T, int L>
+using DivFFT = LineFFT<T, L, L == 128 && sizeof(T) == 8>;
 
 // fft_x_grad with the gradients left y-split for fft_half_face: with G_d(j)
 // the gradient spectrum of row j (ky index) and J = NY/2, row j' < J
 // receives G_d(j') + G_d(j' + J) and row J + j' receives (G_d(j') -
 // G_d(j' + J)) W^{-j'} (W = e^{-2 pi i / NY}): the inputs of the J-point
 // inverse y transforms of the even and of the odd real rows (one radix-2
-// step of the y transform, decimation in frequency), formed before the x
-// transforms (they act on different axes).  A tile's LPB line slots are
-// LPB/2 row pairs (j', kz), (j' + J, kz): slot l < LPB/2 takes row j', slot
-// l + LPB/2 row j' + J (Q = J h pairs).  Each line loads, updates and stores
-// its own row and leaves it in its LDS slot, where the line of the other
-// row reads it: the two lines' threads with the same s are LPB/2 lanes
-// apart in one wavefront, so a wavefront-ordered exchange suffices.
-// 32-bit offsets (a field holds fewer than 2^31 values).
+// step of the y transform, decimation in frequency).  A tile's LPB line
+// slots are LPB/2 row pairs (j', kz), (j' + J, kz): slot l < LPB/2 takes
+// row j', slot l + LPB/2 row j' + J (Q = J h pairs).  Each line loads,
+// updates and stores its own row p_r and transforms it twice: X_r =
+// IFFT_x(p_r) and Y_r = IFFT_x(i k_x e^{i k_x dx/2} p_r).  The y and z
+// symbols are constant along a line, so by linearity G_x(r) = Y_r, G_y(r) =
+// c_y(r) X_r and G_z(r) = c_z X_r; the two lines of a pair (their threads
+// with the same s are LPB/2 lanes apart, in one 32-lane half of a
+// wavefront) swap X and Y by a lane swizzle and form the sum or the
+// twiddled difference -- two transforms per line where forming the inputs
+// first would take three.  32-bit offsets (a field holds fewer than 2^31
+// values).
 template <typename T, int L, int LPB>
 __global__ __launch_bounds__((LPB * LineFFT<T, L>::S)) void fft_x_grad_dec(
     const T *z, T *p, const double *__restrict__ sc, int beta_slot, FieldPtrs<T> g3, unsigned Q, unsigned plane,
@@ -1114,7 +1158,7 @@ __global__ __launch_bounds__((LPB * LineFFT<T, L>::S)) void fft_x_grad_dec(
     const V2<T> *__restrict__ twy) {
   using F = LineFFT<T, L>;
   constexpr int P = F::P, LPT = LPB / 2;
-  static_assert(64 % LPB == 0, "the two lines of a pair share a wavefront");
+  static_assert(64 % LPB == 0 && LPT < 32, "the two lines of a pair share a 32-lane half of a wavefront");
   __shared__ V2<T> sm[LPB * F::LS];
   unsigned field;
   const unsigned tile = xcd_tile(field);
@@ -1128,81 +1172,57 @@ __global__ __launch_bounds__((LPB * LineFFT<T, L>::S)) void fft_x_grad_dec(
   const bool update = beta_slot >= 0;
   const T beta = update ? (T)sc[beta_slot] : (T)0;
   V2<T> *sl = sm + l * F::LS;
-  // the x and z gradient symbols i k_d e^{i k_d dx/2}, loaded here, where
-  // their latency hides behind the field loads (x: per input point)
-  V2<T> gx[P];
+  V2<T> x[P], y[P];
+  p_update_row<T, P>(x, z, p, sub, update, beta, valid, [&](int n) { return own + F::in_i(s, n) * plane; });
+  // the x gradient symbol i k_x e^{i k_x dx/2} per input point
 #pragma unroll
   for (int n = 0; n < P; n++) {
     const int i = F::in_i(s, n);
     const V2<T> e = shx[i];
     const T w = (T)(signed_k(i, L) * wx);
-    gx[n] = V2<T>{-w * e.im, w * e.re};
-  }
-  V2<T> gz;
-  {
-    const V2<T> e = shz[kz];
-    const T w = (T)(kz * wz);
-    gz = V2<T>{-w * e.im, w * e.re};
-  }
-  V2<T> pv[P], vy[P];
-  {
-    V2<T> pa[P];
-#pragma unroll
-    for (int n = 0; n < P; n++) {
-      const unsigned off = own + F::in_i(s, n) * plane;
-      if (valid) {
-        pa[n] = ld2(z + 2 * off);
-        if (update) {
-          pa[n] = pa[n] + scal(beta, ld2(p + 2 * off));
-          st2(p + 2 * off, pa[n]);
-        } else if (sub) {
-          const V2<float> d = ld2(sub + 2 * off);
-          pa[n] = V2<T>{pa[n].re - (T)d.re, pa[n].im - (T)d.im};
-          st2(p + 2 * off, pa[n]);
-        }
-      } else {
-        pa[n] = V2<T>{0, 0};
-      }
-      sl[s * P + n] = pa[n];
-    }
-    wave_lds_sync();
-    const V2<T> *so = sm + (l ^ LPT) * F::LS;  // the other row's slot (LPT is a power of two)
-    V2<T> wj = tw_get<+1>(twy, j);
-    if (!mb) wj = V2<T>{1, 0};
-    const T sg = mb ? (T)-1 : (T)1;
-    const T k0 = (T)(signed_k(j, ny) * wy), k1 = (T)(signed_k(j + ny / 2, ny) * wy);
-    const V2<T> s0 = shy[j], s1 = shy[j + ny / 2];
-#pragma unroll
-    for (int n = 0; n < P; n++) {
-      const V2<T> po = so[s * P + n];
-      const V2<T> p0 = mb ? po : pa[n], p1 = mb ? pa[n] : po;
-      const V2<T> g0 = cmul(V2<T>{-k0 * p0.im, k0 * p0.re}, s0);
-      const V2<T> g1 = cmul(V2<T>{-k1 * p1.im, k1 * p1.re}, s1);
-      pv[n] = cmul(p0 + scal(sg, p1), wj);
-      vy[n] = cmul(g0 + scal(sg, g1), wj);
-    }
+    y[n] = cmul(x[n], V2<T>{-w * e.im, w * e.re});
   }
   V2<T> twr[P];
   line_twiddles<F>(twr, tw, s);
+  V2<T> wj = tw_get<+1>(twy, j);
+  if (!mb) wj = V2<T>{1, 0};
+  const T sg = mb ? (T)-1 : (T)1;
+  // G_x first: its stores are in flight while X is transformed
+  F::template run<+1>(y, sl, twr, s, LPB);
+  T *__restrict__ gx = g3.p[0], *__restrict__ gy = g3.p[1], *__restrict__ gz = g3.p[2];
 #pragma unroll
-  for (int d = 0; d < 3; d++) {
-    V2<T> v[P];
+  for (int m = 0; m < P; m++) {
+    const V2<T> yo = swz_xor<LPT>(y[m]);
+    const V2<T> y0 = mb ? yo : y[m], y1 = mb ? y[m] : yo;
+    if (valid) st2(gx + 2 * (own + F::out_i(s, m) * plane), cmul(y0 + scal(sg, y1), wj));
+  }
+  F::template run<+1>(x, sl, twr, s, LPB);
+  // the y symbols of the pair's rows j' and j' + J, the z symbol
+  const T k0 = (T)(signed_k(j, ny) * wy), k1 = (T)(signed_k(j + ny / 2, ny) * wy);
+  const V2<T> e0 = shy[j], e1 = shy[j + ny / 2], ez = shz[kz];
+  const T kzw = (T)(kz * wz);
+  const V2<T> c0{-k0 * e0.im, k0 * e0.re}, c1{-k1 * e1.im, k1 * e1.re}, cz{-kzw * ez.im, kzw * ez.re};
 #pragma unroll
-    for (int n = 0; n < P; n++) v[n] = d == 1 ? vy[n] : cmul(pv[n], d == 0 ? gx[n] : gz);
-    F::template run<+1>(v, sl, twr, s, LPB);
+  for (int m = 0; m < P; m++) {
+    const V2<T> xo = swz_xor<LPT>(x[m]);
+    const V2<T> x0 = mb ? xo : x[m], x1 = mb ? x[m] : xo;
+    const V2<T> vy = cmul(cmul(x0, c0) + scal(sg, cmul(x1, c1)), wj);
+    const V2<T> vz = cmul(cmul(x0 + scal(sg, x1), wj), cz);
     if (valid) {
-      T *__restrict__ gd = g3.p[d];
-#pragma unroll
-      for (int m = 0; m < P; m++) st2(gd + 2 * (own + F::out_i(s, m) * plane), v[m]);
+      const unsigned off = 2 * (own + F::out_i(s, m) * plane);
+      st2(gy + off, vy);
+      st2(gz + off, vz);
     }
   }
 }
 
 // The staggered gradients of two FP64 half spectra a, b (grid.y: 0 = a,
 // 1 = b) as six FP32 fields after their inverse x transform:
-// out[3f + d] = IFFT_x(i k_d e^{i k_d dx/2} in_f).  The symbols are formed in
-// FP64 and rounded once to FP32 (the values grad_shift2_f64_kernel stores),
-// the transforms run in FP32.
+// out[3f + d] = IFFT_x(i k_d e^{i k_d dx/2} in_f).  The x product is formed
+// in FP64 and rounded once to FP32 (the value grad_shift2_f64_kernel
+// stores); the y and z symbols are constant along the line, so those two
+// gradients are the FP32 transform of the rounded input times the rounded
+// symbol (two transforms per line); the transforms run in FP32.
 template <int L, int LPB>
 __global__ __launch_bounds__((LPB * LineFFT<float, L>::S)) void fft_x_grad2_f64(
     const double *__restrict__ a, const double *__restrict__ b, FieldPtrs<float> out, long long Q, long long plane,
@@ -1226,22 +1246,30 @@ __global__ __launch_bounds__((LPB * LineFFT<float, L>::S)) void fft_x_grad2_f64(
   V2<float> *sl = sm + l * F::LS;
   V2<float> twr[P];
   line_twiddles<F>(twr, tw, s);
+  V2<float> v[P], x[P];
 #pragma unroll
-  for (int d = 0; d < 3; d++) {
-    V2<float> v[P];
+  for (int n = 0; n < P; n++) {
+    const int i = F::in_i(s, n);
+    const double w = signed_k(i, L) * wx;
+    const V2<double> gv = cmul(V2<double>{-w * xv[n].im, w * xv[n].re}, shx[i]);
+    v[n] = V2<float>{(float)gv.re, (float)gv.im};
+    x[n] = V2<float>{(float)xv[n].re, (float)xv[n].im};
+  }
+  F::template run<+1>(v, sl, twr, s, LPB);
+  if (valid) {  // (in flight while the second transform runs)
 #pragma unroll
-    for (int n = 0; n < P; n++) {
-      const int i = F::in_i(s, n);
-      const double w = d == 0 ? signed_k(i, L) * wx : d == 1 ? signed_k(j, ny) * wy : kz * wz;
-      const V2<double> sft = d == 0 ? shx[i] : d == 1 ? shy[j] : shz[kz];
-      const V2<double> gv = cmul(V2<double>{-w * xv[n].im, w * xv[n].re}, sft);
-      v[n] = V2<float>{(float)gv.re, (float)gv.im};
-    }
-    F::template run<+1>(v, sl, twr, s, LPB);
-    if (valid) {
-      float *__restrict__ gd = out.p[3 * field + d];
+    for (int m = 0; m < P; m++) st2(out.p[3 * field] + 2 * (base + (long long)F::out_i(s, m) * plane), v[m]);
+  }
+  F::template run<+1>(x, sl, twr, s, LPB);
+  const double ky = signed_k(j, ny) * wy, kzw = kz * wz;
+  const V2<double> ey = shy[j], ez = shz[kz];
+  const V2<float> cy{(float)(-ky * ey.im), (float)(ky * ey.re)}, cz{(float)(-kzw * ez.im), (float)(kzw * ez.re)};
+  if (valid) {
 #pragma unroll
-      for (int m = 0; m < P; m++) st2(gd + 2 * (base + (long long)F::out_i(s, m) * plane), v[m]);
+    for (int m = 0; m < P; m++) {
+      const long long off = 2 * (base + (long long)F::out_i(s, m) * plane);
+      st2(out.p[3 * field + 1] + off, cmul(x[m], cy));
+      st2(out.p[3 * field + 2] + off, cmul(x[m], cz));
     }
   }
 }
@@ -1251,10 +1279,7 @@ __global__ __launch_bounds__((LPB * LineFFT<float, L>::S)) void fft_x_grad2_f64(
 // kz = 0 and NZ/2 planes and 2 elsewhere (Parseval, without N^-3)
 // RES: the residual form -- p is b, ap receives r = (div) - b, and the
 // partials sum |r|^2 (the refinement's A x - b)
-// x_div's line transforms: the pair split at 128 points in FP64 only (in
-// FP32 the four-step 16 x 8 measured faster for this kernel)
-template <typename T, int L>
-using DivFFT = LineFFT<T, L, L == 128 && sizeof(T) == 8>;
+// (x_div's line transforms: DivFFT, above)
 template <typename T, int L, int LPB, bool RES = false>
 __global__ __launch_bounds__((LPB * DivFFT<T, L>::S)) void fft_x_div(
     FieldPtrs<T> g3, const T *__restrict__ p, T *__restrict__ ap, double *__restrict__ partial, long long Q,
@@ -1271,34 +1296,34 @@ __global__ __launch_bounds__((LPB * DivFFT<T, L>::S)) void fft_x_div(
   const bool valid = q < Q;
   const long long base = valid ? q : 0;
   const int j = (int)(base / h), kz = (int)(base % h);
-  V2<T> acc[P];
-#pragma unroll
-  for (int m = 0; m < P; m++) acc[m] = V2<T>{0, 0};
   V2<T> *sl = sm + l * F::LS;
-  // direction d + 1's loads are in flight while direction d is transformed
-  // (base is 0 for an invalid line, so every load is in bounds)
-  V2<T> nxt[P];
+  // The y and z divergence symbols q_d conj(e^{i k_d dx/2}) are constant
+  // along the line, so their terms are summed before one transform, which
+  // runs while the x input loads (base is 0 for an invalid line, so every
+  // load is in bounds).
+  V2<T> vy[P], vz[P];
 #pragma unroll
-  for (int n = 0; n < P; n++) nxt[n] = ld2(g3.p[0] + 2 * (base + (long long)F::in_i(s, n) * plane));
+  for (int n = 0; n < P; n++) vy[n] = ld2(g3.p[1] + 2 * (base + (long long)F::in_i(s, n) * plane));
+#pragma unroll
+  for (int n = 0; n < P; n++) vz[n] = ld2(g3.p[2] + 2 * (base + (long long)F::in_i(s, n) * plane));
   V2<T> twr[P];
   line_twiddles<F>(twr, tw, s);
+  const V2<T> fy = scal((T)(signed_k(j, ny) * wy), conjv(shy[j]));
+  const V2<T> fz = scal((T)(kz * wz), conjv(shz[kz]));
+  V2<T> acc[P];
 #pragma unroll
-  for (int d = 0; d < 3; d++) {
-    V2<T> v[P];
+  for (int n = 0; n < P; n++) acc[n] = valid ? cmul(vy[n], fy) + cmul(vz[n], fz) : V2<T>{0, 0};
+  V2<T> vx[P];
 #pragma unroll
-    for (int n = 0; n < P; n++) v[n] = valid ? nxt[n] : V2<T>{0, 0};
-    if (d < 2) {
+  for (int n = 0; n < P; n++) vx[n] = ld2(g3.p[0] + 2 * (base + (long long)F::in_i(s, n) * plane));
+  F::template run<-1>(acc, sl, twr, s, LPB);
 #pragma unroll
-      for (int n = 0; n < P; n++) nxt[n] = ld2(g3.p[d + 1] + 2 * (base + (long long)F::in_i(s, n) * plane));
-    }
-    F::template run<-1>(v, sl, twr, s, LPB);
+  for (int n = 0; n < P; n++) vx[n] = valid ? vx[n] : V2<T>{0, 0};
+  F::template run<-1>(vx, sl, twr, s, LPB);
 #pragma unroll
-    for (int m = 0; m < P; m++) {
-      const int i = F::out_i(s, m);
-      const T qd = d == 0 ? (T)(signed_k(i, L) * wx) : d == 1 ? (T)(signed_k(j, ny) * wy) : (T)(kz * wz);
-      const V2<T> sft = d == 0 ? shx[i] : d == 1 ? shy[j] : shz[kz];
-      acc[m] = acc[m] + cmul(v[m], scal(qd, conjv(sft)));
-    }
+  for (int m = 0; m < P; m++) {
+    const int i = F::out_i(s, m);
+    acc[m] = acc[m] + cmul(vx[m], scal((T)(signed_k(i, L) * wx), conjv(shx[i])));
   }
   const double wk = (kz == 0 || 2 * kz == nz) ? 1.0 : 2.0;
   double sum = 0.0;
@@ -1325,9 +1350,12 @@ __global__ __launch_bounds__((LPB * DivFFT<T, L>::S)) void fft_x_div(
 // fft_x_div on the y-split output of fft_half_face (rows j' < J = NY/2: E,
 // rows J + j': O): a tile's line slots are row pairs as in fft_x_grad_dec;
 // the line of row j' loads E, the line of row j' + J loads O, they swap
-// them through their LDS slots (one wavefront), and form E + W^{j'} O and
-// E - W^{j'} O (W = e^{-2 pi i / NY}), the spectra of rows j' and j' + J,
-// before their forward x transforms.  32-bit offsets.
+// them by a lane swizzle and form E + W^{j'} O and E - W^{j'} O (W =
+// e^{-2 pi i / NY}), the spectra of rows j' and j' + J, before their
+// forward x transforms.  The y and z divergence symbols are constant along
+// a line, so their terms are summed before one transform (FFT(v_y) f_y +
+// FFT(v_z) f_z = FFT(f_y v_y + f_z v_z)): two transforms per line, the y-z
+// one running while the x input loads.  32-bit offsets.
 template <typename T, int L, int LPB, bool RES = false>
 __global__ __launch_bounds__((LPB * DivFFT<T, L>::S), 2) void fft_x_div_dec(
     FieldPtrs<T> g3, const T *__restrict__ p, T *__restrict__ ap, double *__restrict__ partial, unsigned Q,
@@ -1336,7 +1364,7 @@ __global__ __launch_bounds__((LPB * DivFFT<T, L>::S), 2) void fft_x_div_dec(
     const V2<T> *__restrict__ twy) {
   using F = DivFFT<T, L>;
   constexpr int P = F::P, LPT = LPB / 2;
-  static_assert(64 % LPB == 0, "the two lines of a pair share a wavefront");
+  static_assert(64 % LPB == 0 && LPT < 32, "the two lines of a pair share a 32-lane half of a wavefront");
   __shared__ V2<T> sm[LPB * F::LS];
   unsigned field;
   const unsigned tile = xcd_tile(field);
@@ -1344,54 +1372,44 @@ __global__ __launch_bounds__((LPB * DivFFT<T, L>::S), 2) void fft_x_div_dec(
   const int l = t % LPB, s = t / LPB;
   const int mb = l / LPT;
   const unsigned q = tile * LPT + l % LPT;
-  const bool valid = q < Q;
+  const bool valid = q < Q;  // the same for both lines of a pair
   const unsigned base = valid ? q : 0, half = (unsigned)(ny / 2) * h, own = base + mb * half;
   const int jp = (int)(base / h), kz = (int)(base % h);
   const int j = jp + mb * (ny / 2);  // this line's row (ky index)
   const V2<T> wj = twy[jp];
   const T sg = mb ? (T)-1 : (T)1;
-  V2<T> acc[P];
-#pragma unroll
-  for (int m = 0; m < P; m++) acc[m] = V2<T>{0, 0};
   V2<T> *sl = sm + l * F::LS;
-  const V2<T> *so = sm + (l ^ LPT) * F::LS;
-  // direction d + 1's loads are in flight while direction d is transformed
-  // (base is 0 for an invalid line, so every load is in bounds)
-  V2<T> nxt[P];
+  // this line's row from its own half a and the other line's (base is 0
+  // for an invalid pair, so every load is in bounds)
+  auto row_of = [&](const V2<T> a) {
+    const V2<T> o = swz_xor<LPT>(a);
+    const V2<T> e = mb ? o : a, od = mb ? a : o;
+    return valid ? e + scal(sg, cmul(od, wj)) : V2<T>{0, 0};
+  };
+  V2<T> vy[P], vz[P];
 #pragma unroll
-  for (int n = 0; n < P; n++) nxt[n] = ld2(g3.p[0] + 2 * (own + F::in_i(s, n) * plane));
+  for (int n = 0; n < P; n++) vy[n] = ld2(g3.p[1] + 2 * (own + F::in_i(s, n) * plane));
+#pragma unroll
+  for (int n = 0; n < P; n++) vz[n] = ld2(g3.p[2] + 2 * (own + F::in_i(s, n) * plane));
   V2<T> twr[P];
   line_twiddles<F>(twr, tw, s);
-  // the divergence symbols q_d conj(e^{i k_d dx/2}), loaded here, where
-  // their latency hides behind the field loads (x: per output point)
-  V2<T> fx[P];
+  // the divergence symbols q_d conj(e^{i k_d dx/2})
+  const V2<T> fy = scal((T)(signed_k(j, ny) * wy), conjv(shy[j]));
+  const V2<T> fz = scal((T)(kz * wz), conjv(shz[kz]));
+  V2<T> acc[P];
+#pragma unroll
+  for (int n = 0; n < P; n++) acc[n] = cmul(row_of(vy[n]), fy) + cmul(row_of(vz[n]), fz);
+  V2<T> vx[P];
+#pragma unroll
+  for (int n = 0; n < P; n++) vx[n] = ld2(g3.p[0] + 2 * (own + F::in_i(s, n) * plane));
+  F::template run<-1>(acc, sl, twr, s, LPB);
+#pragma unroll
+  for (int n = 0; n < P; n++) vx[n] = row_of(vx[n]);
+  F::template run<-1>(vx, sl, twr, s, LPB);
 #pragma unroll
   for (int m = 0; m < P; m++) {
     const int i = F::out_i(s, m);
-    fx[m] = scal((T)(signed_k(i, L) * wx), conjv(shx[i]));
-  }
-  const V2<T> fy = scal((T)(signed_k(j, ny) * wy), conjv(shy[j]));
-  const V2<T> fz = scal((T)(kz * wz), conjv(shz[kz]));
-#pragma unroll
-  for (int d = 0; d < 3; d++) {
-    V2<T> v[P];
-    if (d > 0) __syncthreads();  // the slots' last reads (the previous transform) are done
-#pragma unroll
-    for (int n = 0; n < P; n++) sl[s * P + n] = nxt[n];
-    wave_lds_sync();
-#pragma unroll
-    for (int n = 0; n < P; n++) {
-      const V2<T> o = so[s * P + n];
-      const V2<T> e = mb ? o : nxt[n], od = mb ? nxt[n] : o;
-      v[n] = valid ? e + scal(sg, cmul(od, wj)) : V2<T>{0, 0};
-    }
-    if (d < 2) {
-#pragma unroll
-      for (int n = 0; n < P; n++) nxt[n] = ld2(g3.p[d + 1] + 2 * (own + F::in_i(s, n) * plane));
-    }
-    F::template run<-1>(v, sl, twr, s, LPB);
-#pragma unroll
-    for (int m = 0; m < P; m++) acc[m] = acc[m] + cmul(v[m], d == 0 ? fx[m] : d == 1 ? fy : fz);
+    acc[m] = acc[m] + cmul(vx[m], scal((T)(signed_k(i, L) * wx), conjv(shx[i])));
   }
   const double wk = (kz == 0 || 2 * kz == nz) ? 1.0 : 2.0;
   double sum = 0.0;

@@ -196,6 +196,27 @@ __device__ __forceinline__ T shfl_xor_t(T v, int m) {
   return __shfl_xor(v, m);
 }
 
+// The value of lane (lane ^ MASK), MASK < 32, for every lane of a wavefront
+// (all lanes active): ds_swizzle in bit mode (and-mask 31, or-mask 0,
+// xor-mask MASK), one per 32-bit word, no LDS storage.
+template <int MASK>
+__device__ __forceinline__ float swz_xor(float x) {
+  static_assert(MASK > 0 && MASK < 32, "swizzle within 32 lanes");
+  return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(x), 0x1F | (MASK << 10)));
+}
+template <int MASK>
+__device__ __forceinline__ double swz_xor(double x) {
+  static_assert(MASK > 0 && MASK < 32, "swizzle within 32 lanes");
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_ds_swizzle((int)(unsigned)b, 0x1F | (MASK << 10));
+  const unsigned hi = (unsigned)__builtin_amdgcn_ds_swizzle((int)(unsigned)(b >> 32), 0x1F | (MASK << 10));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+template <int MASK, typename T>
+__device__ __forceinline__ V2<T> swz_xor(V2<T> v) {
+  return V2<T>{swz_xor<MASK>(v.re), swz_xor<MASK>(v.im)};
+}
+
 // L = 128 as 8 x 16 with the radix-16 stage split over lane pairs (SPLIT,
 // the default at this length): S = 16, P = 8.  Thread s loads the points s + 16 n, a radix-8 DFT and the twiddles
 // W_128^{s k1}, the exchange through LDS; the 16-point DFT of k1 is shared by
