@@ -354,13 +354,13 @@ void fft_face_t(kapsel_ctx *ctx, void *const *g3, void *const *face3);
 // fft_face_t without its first (x inverse) and last (x forward) passes
 template <typename T>
 void fft_face_mid_t(kapsel_ctx *ctx, void *const *g3, void *const *face3);
-// the operator's first pass on a CG direction: p = z + sc[beta_slot] p, then
-// g_d = x-inverse of (i k_d e^{i k_d dx/2} p); beta_slot < 0: the gradient
-// of z, p untouched, or with sub of p = z - sub (stored); false: NX has no
-// fused pass
+// the operator's first pass on a CG direction: p = zscale z + sc[beta_slot]
+// p, then g_d = x-inverse of (i k_d e^{i k_d dx/2} p); beta_slot < 0: the
+// gradient of z, p untouched, or with sub of p = z - sub (stored); false: NX
+// has no fused pass
 template <typename T>
 bool fft_x_grad_t(kapsel_ctx *ctx, const T *z, T *p, const double *sc, int beta_slot, void *const *g3,
-                  const float *sub = nullptr);
+                  const float *sub = nullptr, double zscale = 1.0);
 // the staggered gradients of two FP64 half spectra a, b as six real FP32
 // fields (a's three, then b's); false where NX has no fused x pass (nothing
 // queued)

@@ -1032,15 +1032,15 @@ __device__ __forceinline__ double block_sum_fft(double v) {
   return s;
 }
 
-// The P points of a line's row of p = z + beta p (update; stored), of p =
+// The P points of a line's row of p = zs z + beta p (update; stored), of p =
 // z - sub (sub; stored) or of z (neither), at the offsets off(n): every
 // load is issued before the first store (a store to p may alias a later
 // point's load for all the compiler knows, which would otherwise hold each
 // point's loads until the store before it is issued).  Zero for an invalid
 // line.
 template <typename T, int P, typename Off>
-__device__ __forceinline__ void p_update_row(V2<T> (&x)[P], const T *z, T *p, const float *sub, bool update, T beta,
-                                             bool valid, Off off) {
+__device__ __forceinline__ void p_update_row(V2<T> (&x)[P], const T *z, T *p, const float *sub, bool update, T zs,
+                                             T beta, bool valid, Off off) {
   if (!valid) {
 #pragma unroll
     for (int n = 0; n < P; n++) x[n] = V2<T>{0, 0};
@@ -1053,7 +1053,7 @@ __device__ __forceinline__ void p_update_row(V2<T> (&x)[P], const T *z, T *p, co
 #pragma unroll
     for (int n = 0; n < P; n++) b[n] = ld2(p + 2 * off(n));
 #pragma unroll
-    for (int n = 0; n < P; n++) x[n] = x[n] + scal(beta, b[n]);
+    for (int n = 0; n < P; n++) x[n] = scal(zs, x[n]) + scal(beta, b[n]);
   } else if (sub) {
     V2<float> d[P];
 #pragma unroll
@@ -1071,12 +1071,15 @@ __device__ __forceinline__ void p_update_row(V2<T> (&x)[P], const T *z, T *p, co
   for (int n = 0; n < P; n++) st2(p + 2 * off(n), x[n]);
 }
 
-// p = z + sc[beta] p (stored), then g_d = IFFT_x(i k_d e^{i k_d dx/2} p), d = x, y, z
-// (thread t of a tile: line t % LPB, its thread s = t / LPB); with sub (and
-// no beta): p = z - sub (stored), the refinement's x -= d before its residual
+// p = zscale z + sc[beta] p (stored), then g_d = IFFT_x(i k_d e^{i k_d dx/2}
+// p), d = x, y, z (thread t of a tile: line t % LPB, its thread s = t /
+// LPB); zscale 2 with beta -1 is the linear extrapolation of a start in
+// time; with sub (and no beta): p = z - sub (stored), the refinement's x -=
+// d before its residual
 template <typename T, int L, int LPB>
 __global__ __launch_bounds__((LPB * LineFFT<T, L>::S)) void fft_x_grad(
-    const T *z, T *p, const double *__restrict__ sc, int beta_slot, FieldPtrs<T> g3, long long Q, long long plane,
+    const T *z, T *p, const double *__restrict__ sc, int beta_slot, T zscale, FieldPtrs<T> g3, long long Q,
+    long long plane,
     int h, int ny, double wx, double wy, double wz, const V2<T> *__restrict__ shx, const V2<T> *__restrict__ shy,
     const V2<T> *__restrict__ shz, const V2<T> *__restrict__ tw, const float *__restrict__ sub) {
   using F = LineFFT<T, L>;
@@ -1093,7 +1096,7 @@ __global__ __launch_bounds__((LPB * LineFFT<T, L>::S)) void fft_x_grad(
   const bool update = beta_slot >= 0;  // else the gradient of z itself, p untouched
   const T beta = update ? (T)sc[beta_slot] : (T)0;
   V2<T> pv[P];
-  p_update_row<T, P>(pv, z, p, sub, update, beta, valid,
+  p_update_row<T, P>(pv, z, p, sub, update, zscale, beta, valid,
                      [&](int n) { return base + (long long)F::in_i(s, n) * plane; });
   V2<T> *sl = sm + l * F::LS;
   V2<T> twr[P];
@@ -1152,7 +1155,8 @@ using DivFFT = LineFFT<T, L, L == 128 && sizeof(T) == 8>;
 // values).
 template <typename T, int L, int LPB>
 __global__ __launch_bounds__((LPB * LineFFT<T, L>::S)) void fft_x_grad_dec(
-    const T *z, T *p, const double *__restrict__ sc, int beta_slot, FieldPtrs<T> g3, unsigned Q, unsigned plane,
+    const T *z, T *p, const double *__restrict__ sc, int beta_slot, T zscale, FieldPtrs<T> g3, unsigned Q,
+    unsigned plane,
     int h, int ny, double wx, double wy, double wz, const V2<T> *__restrict__ shx, const V2<T> *__restrict__ shy,
     const V2<T> *__restrict__ shz, const V2<T> *__restrict__ tw, const float *__restrict__ sub,
     const V2<T> *__restrict__ twy) {
@@ -1173,7 +1177,7 @@ __global__ __launch_bounds__((LPB * LineFFT<T, L>::S)) void fft_x_grad_dec(
   const T beta = update ? (T)sc[beta_slot] : (T)0;
   V2<T> *sl = sm + l * F::LS;
   V2<T> x[P], y[P];
-  p_update_row<T, P>(x, z, p, sub, update, beta, valid, [&](int n) { return own + F::in_i(s, n) * plane; });
+  p_update_row<T, P>(x, z, p, sub, update, zscale, beta, valid, [&](int n) { return own + F::in_i(s, n) * plane; });
   // the x gradient symbol i k_x e^{i k_x dx/2} per input point
 #pragma unroll
   for (int n = 0; n < P; n++) {
@@ -1839,7 +1843,7 @@ void fft_face_mid_t(kapsel_ctx *ctx, void *const *g3, void *const *face3) {
 
 template <typename T, int L>
 void launch_x_grad(kapsel_ctx *ctx, const T *z, T *p, const double *sc, int beta_slot, void *const *g3,
-                   const float *sub) {
+                   const float *sub, double zscale) {
   constexpr int LPB = lines_per_block(sizeof(T));
   const Grid &g = ctx->g;
   const Twiddles &tw = ctx->tw[prec_of<T>()];
@@ -1848,13 +1852,15 @@ void launch_x_grad(kapsel_ctx *ctx, const T *z, T *p, const double *sc, int beta
     constexpr int LPT = LPB / 2;  // row pairs per tile
     const unsigned Q = (unsigned)(g.ny / 2) * g.h;
     hipLaunchKernelGGL((fft_x_grad_dec<T, L, LPB>), dim3((Q + LPT - 1) / LPT), dim3(LPB * LineFFT<T, L>::S), 0,
-                       ctx->stream, z, p, sc, beta_slot, pack<T>(g3, 3), Q, (unsigned)(g.ny * g.h), g.h, g.ny,
+                       ctx->stream, z, p, sc, beta_slot, (T)zscale, pack<T>(g3, 3), Q, (unsigned)(g.ny * g.h),
+                       g.h, g.ny,
                        g.wave_x, g.wave_y, g.wave_z, (const V2<T> *)tw.shx, (const V2<T> *)tw.shy,
                        (const V2<T> *)tw.shz, (const V2<T> *)tw.x, sub, (const V2<T> *)tw.y);
   } else {
     const long long Q = (long long)g.ny * g.h;
     hipLaunchKernelGGL((fft_x_grad<T, L, LPB>), dim3((unsigned)((Q + LPB - 1) / LPB)), dim3(LPB * LineFFT<T, L>::S),
-                       0, ctx->stream, z, p, sc, beta_slot, pack<T>(g3, 3), Q, Q, g.h, g.ny, g.wave_x, g.wave_y,
+                       0, ctx->stream, z, p, sc, beta_slot, (T)zscale, pack<T>(g3, 3), Q, Q, g.h, g.ny, g.wave_x,
+                       g.wave_y,
                        g.wave_z, (const V2<T> *)tw.shx, (const V2<T> *)tw.shy, (const V2<T> *)tw.shz,
                        (const V2<T> *)tw.x, sub);
   }
@@ -1889,10 +1895,10 @@ int launch_x_div(kapsel_ctx *ctx, void *const *g3, const T *p, T *ap, double *pa
 
 template <typename T>
 bool fft_x_grad_t(kapsel_ctx *ctx, const T *z, T *p, const double *sc, int beta_slot, void *const *g3,
-                  const float *sub) {
+                  const float *sub, double zscale) {
   switch (ctx->g.nx) {
-    case 64: launch_x_grad<T, 64>(ctx, z, p, sc, beta_slot, g3, sub); return true;
-    case 128: launch_x_grad<T, 128>(ctx, z, p, sc, beta_slot, g3, sub); return true;
+    case 64: launch_x_grad<T, 64>(ctx, z, p, sc, beta_slot, g3, sub, zscale); return true;
+    case 128: launch_x_grad<T, 128>(ctx, z, p, sc, beta_slot, g3, sub, zscale); return true;
     default: return false;
   }
 }
@@ -1953,9 +1959,9 @@ template void fft_face_mid_t<double>(kapsel_ctx *, void *const *, void *const *)
 template int fft_x_res_t<double>(kapsel_ctx *, void *const *, const double *, double *, double *);
 template void fft_face_mid_t<float>(kapsel_ctx *, void *const *, void *const *);
 template bool fft_x_grad_t<float>(kapsel_ctx *, const float *, float *, const double *, int, void *const *,
-                                  const float *);
+                                  const float *, double);
 template bool fft_x_grad_t<double>(kapsel_ctx *, const double *, double *, const double *, int, void *const *,
-                                   const float *);
+                                   const float *, double);
 template int fft_x_div_t<float>(kapsel_ctx *, void *const *, const float *, float *, double *);
 template int fft_x_div_t<double>(kapsel_ctx *, void *const *, const double *, double *, double *);
 template void fft_face_t<double>(kapsel_ctx *, void *const *, void *const *);

@@ -501,25 +501,37 @@ __global__ void md_position_kernel(double *__restrict__ ps, double *__restrict__
                                    double hdt) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= p.np) return;
+  // every load before the first store (the state is one array, so a store
+  // would hold back the loads after it)
+  double x[3], v[3], vo[3], xn[3];
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    double x = pval(ps, KAPSEL_P_X, d, n, p.np);
-    pref(ps, KAPSEL_P_X_PREVIOUS, d, n, p.np) = x;
-    const double v = pval(ps, KAPSEL_P_V, d, n, p.np);
-    const double dx = euler ? dt * v : hdt * (3.0 * v - pval(ps, KAPSEL_P_V_OLD, d, n, p.np));
-    pref(ps, KAPSEL_P_X_NOPBC, d, n, p.np) += dx;
-    x += dx;
-    pref(ps, KAPSEL_P_X, d, n, p.np) = fmod(x + p.l[d], p.l[d]);
+    x[d] = pval(ps, KAPSEL_P_X, d, n, p.np);
+    v[d] = pval(ps, KAPSEL_P_V, d, n, p.np);
+    vo[d] = euler ? 0.0 : pval(ps, KAPSEL_P_V_OLD, d, n, p.np);
+    xn[d] = pval(ps, KAPSEL_P_X_NOPBC, d, n, p.np);
+  }
+  double q[4], qo[4], om[3], omo[3], dq[4];
+  if (p.rotation) {
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      q[c] = qa[(size_t)c * p.np + n];
+      qo[c] = qa[(size_t)(4 + c) * p.np + n];
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      om[d] = pval(ps, KAPSEL_P_OMEGA, d, n, p.np);
+      omo[d] = euler ? 0.0 : pval(ps, KAPSEL_P_OMEGA_OLD, d, n, p.np);
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const double dx = euler ? dt * v[d] : hdt * (3.0 * v[d] - vo[d]);
+    pref(ps, KAPSEL_P_X_PREVIOUS, d, n, p.np) = x[d];
+    pref(ps, KAPSEL_P_X_NOPBC, d, n, p.np) = xn[d] + dx;
+    pref(ps, KAPSEL_P_X, d, n, p.np) = fmod(x[d] + dx + p.l[d], p.l[d]);
   }
   if (!p.rotation) return;
-  double q[4], qo[4], om[3], dq[4];
-#pragma unroll
-  for (int c = 0; c < 4; c++) {
-    q[c] = qa[(size_t)c * p.np + n];
-    qo[c] = qa[(size_t)(4 + c) * p.np + n];
-  }
-#pragma unroll
-  for (int d = 0; d < 3; d++) om[d] = pval(ps, KAPSEL_P_OMEGA, d, n, p.np);
   if (euler) {
 #pragma unroll
     for (int c = 0; c < 4; c++) qo[c] = q[c];
@@ -528,9 +540,7 @@ __global__ void md_position_kernel(double *__restrict__ ps, double *__restrict__
 #pragma unroll
     for (int c = 0; c < 4; c++) q[c] += dq[c] * dt;
   } else {
-    double omo[3], wb[3], wbo[3];
-#pragma unroll
-    for (int d = 0; d < 3; d++) omo[d] = pval(ps, KAPSEL_P_OMEGA_OLD, d, n, p.np);
+    double wb[3], wbo[3];
     const double qc[4] = {q[0], -q[1], -q[2], -q[3]}, qoc[4] = {qo[0], -qo[1], -qo[2], -qo[3]};
     qvq(wb, qc, om);
     qvq(wbo, qoc, omo);
@@ -564,6 +574,32 @@ __global__ void thermostat_kernel(double *__restrict__ ps, const double *__restr
   }
 }
 
+// The velocity update of component d of particle n from the forces fh
+// (hydrodynamic), fr (the others) and the hydrodynamic torque th, then
+// reset_Forces: every load of the particle's state before the first store
+// (the state is one array, so a store would hold back the loads after it).
+__device__ __forceinline__ void velocity_update(double *__restrict__ ps, const Phys &p, int d, int n, int euler,
+                                                double dmy, double dmy_rot, double fh, double fr, double th) {
+  const double v = pval(ps, KAPSEL_P_V, d, n, p.np), om = pval(ps, KAPSEL_P_OMEGA, d, n, p.np);
+  const double tr = pval(ps, KAPSEL_P_TORQUE_R, d, n, p.np);
+  const double frp = euler ? 0.0 : pval(ps, KAPSEL_P_FR_PREVIOUS, d, n, p.np);
+  const double trp = euler ? 0.0 : pval(ps, KAPSEL_P_TORQUE_R_PREVIOUS, d, n, p.np);
+  const double vn = euler ? v + dmy * (fh + fr) : v + dmy * (2.0 * fh + fr + frp);
+  const double omn = euler ? om + dmy_rot * (th + tr) : om + dmy_rot * (2.0 * th + tr + trp);
+  pref(ps, KAPSEL_P_V_OLD, d, n, p.np) = v;
+  pref(ps, KAPSEL_P_OMEGA_OLD, d, n, p.np) = om;
+  pref(ps, KAPSEL_P_V, d, n, p.np) = vn;
+  pref(ps, KAPSEL_P_OMEGA, d, n, p.np) = omn;
+  pref(ps, KAPSEL_P_FR_PREVIOUS, d, n, p.np) = fr;
+  pref(ps, KAPSEL_P_FR, d, n, p.np) = 0.0;
+  pref(ps, KAPSEL_P_F_HYDRO_PREVIOUS, d, n, p.np) = fh;
+  pref(ps, KAPSEL_P_F_HYDRO, d, n, p.np) = 0.0;
+  pref(ps, KAPSEL_P_TORQUE_R_PREVIOUS, d, n, p.np) = tr;
+  pref(ps, KAPSEL_P_TORQUE_R, d, n, p.np) = 0.0;
+  pref(ps, KAPSEL_P_TORQUE_HYDRO_PREVIOUS, d, n, p.np) = th;
+  pref(ps, KAPSEL_P_TORQUE_HYDRO, d, n, p.np) = 0.0;
+}
+
 // velocity update of MD_solver_velocity_Euler/_AB2_hydro (particle_solver.cxx:
 // 109-150, 252-295) followed by reset_Forces (particle_solver.cxx:11-37)
 __global__ void md_velocity_kernel(double *__restrict__ ps, Phys p, int euler, double dmy, double dmy_rot) {
@@ -571,29 +607,8 @@ __global__ void md_velocity_kernel(double *__restrict__ ps, Phys p, int euler, d
   if (n >= p.np) return;
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    double &v = pref(ps, KAPSEL_P_V, d, n, p.np);
-    double &om = pref(ps, KAPSEL_P_OMEGA, d, n, p.np);
-    pref(ps, KAPSEL_P_V_OLD, d, n, p.np) = v;
-    pref(ps, KAPSEL_P_OMEGA_OLD, d, n, p.np) = om;
-    const double fh = pval(ps, KAPSEL_P_F_HYDRO, d, n, p.np);
-    const double fr = pval(ps, KAPSEL_P_FR, d, n, p.np);
-    const double th = pval(ps, KAPSEL_P_TORQUE_HYDRO, d, n, p.np);
-    const double tr = pval(ps, KAPSEL_P_TORQUE_R, d, n, p.np);
-    if (euler) {
-      v += dmy * (fh + fr);
-      om += dmy_rot * (th + tr);
-    } else {
-      v += dmy * (2.0 * fh + fr + pval(ps, KAPSEL_P_FR_PREVIOUS, d, n, p.np));
-      om += dmy_rot * (2.0 * th + tr + pval(ps, KAPSEL_P_TORQUE_R_PREVIOUS, d, n, p.np));
-    }
-    pref(ps, KAPSEL_P_FR_PREVIOUS, d, n, p.np) = fr;
-    pref(ps, KAPSEL_P_FR, d, n, p.np) = 0.0;
-    pref(ps, KAPSEL_P_F_HYDRO_PREVIOUS, d, n, p.np) = fh;
-    pref(ps, KAPSEL_P_F_HYDRO, d, n, p.np) = 0.0;
-    pref(ps, KAPSEL_P_TORQUE_R_PREVIOUS, d, n, p.np) = tr;
-    pref(ps, KAPSEL_P_TORQUE_R, d, n, p.np) = 0.0;
-    pref(ps, KAPSEL_P_TORQUE_HYDRO_PREVIOUS, d, n, p.np) = th;
-    pref(ps, KAPSEL_P_TORQUE_HYDRO, d, n, p.np) = 0.0;
+    velocity_update(ps, p, d, n, euler, dmy, dmy_rot, pval(ps, KAPSEL_P_F_HYDRO, d, n, p.np),
+                    pval(ps, KAPSEL_P_FR, d, n, p.np), pval(ps, KAPSEL_P_TORQUE_HYDRO, d, n, p.np));
   }
 }
 
@@ -610,11 +625,28 @@ __global__ __launch_bounds__(256) void hydro_force_velocity_kernel(double *__res
   const int i = blockIdx.x;
   __shared__ double fh[6], fr[3];
   __shared__ double red[3][4];
+  // the slices' partials (12 per slice, contiguous) staged in LDS by the
+  // whole workgroup, GS slices at a time, then merged in slice order by
+  // threads 0-5 (the order of hydro_gather_final_kernel)
+  constexpr int GS = 64;
+  __shared__ double stage[12 * GS];
+  const double *src = partial + (size_t)12 * i * nslices;
+  CSum acc{0.0, 0.0};
+  for (int s0 = 0; s0 < nslices; s0 += GS) {
+    const int ns = min(GS, nslices - s0);
+    for (int e = threadIdx.x; e < 12 * ns; e += blockDim.x) stage[e] = src[12 * s0 + e];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+      const int c = threadIdx.x;
+      for (int sl = 0; sl < ns; sl++) {
+        const CSum v{stage[12 * sl + 2 * c], stage[12 * sl + 2 * c + 1]};
+        acc = s0 + sl == 0 ? v : cmerge(acc, v);
+      }
+    }
+    __syncthreads();
+  }
   if (threadIdx.x < 6) {
     const int c = threadIdx.x;
-    const double *src = partial + (size_t)12 * i * nslices + 2 * c;
-    CSum acc{src[0], src[1]};
-    for (int sl = 1; sl < nslices; sl++) acc = cmerge(acc, CSum{src[12 * sl], src[12 * sl + 1]});
     const double v = acc.s + acc.c;
     fh[c] = c < 3 || p.rotation ? dmy_f * v : pval(ps, KAPSEL_P_TORQUE_HYDRO, c - 3, i, p.np);
   }
@@ -661,27 +693,7 @@ __global__ __launch_bounds__(256) void hydro_force_velocity_kernel(double *__res
   __syncthreads();
   if (threadIdx.x < 3) {
     const int d = threadIdx.x;
-    double &v = pref(ps, KAPSEL_P_V, d, i, p.np);
-    double &om = pref(ps, KAPSEL_P_OMEGA, d, i, p.np);
-    pref(ps, KAPSEL_P_V_OLD, d, i, p.np) = v;
-    pref(ps, KAPSEL_P_OMEGA_OLD, d, i, p.np) = om;
-    const double f_h = fh[d], f_r = fr[d], th = fh[3 + d];
-    const double tr = pval(ps, KAPSEL_P_TORQUE_R, d, i, p.np);
-    if (euler) {
-      v += dmy * (f_h + f_r);
-      om += dmy_rot * (th + tr);
-    } else {
-      v += dmy * (2.0 * f_h + f_r + pval(ps, KAPSEL_P_FR_PREVIOUS, d, i, p.np));
-      om += dmy_rot * (2.0 * th + tr + pval(ps, KAPSEL_P_TORQUE_R_PREVIOUS, d, i, p.np));
-    }
-    pref(ps, KAPSEL_P_FR_PREVIOUS, d, i, p.np) = f_r;
-    pref(ps, KAPSEL_P_FR, d, i, p.np) = 0.0;
-    pref(ps, KAPSEL_P_F_HYDRO_PREVIOUS, d, i, p.np) = f_h;
-    pref(ps, KAPSEL_P_F_HYDRO, d, i, p.np) = 0.0;
-    pref(ps, KAPSEL_P_TORQUE_R_PREVIOUS, d, i, p.np) = tr;
-    pref(ps, KAPSEL_P_TORQUE_R, d, i, p.np) = 0.0;
-    pref(ps, KAPSEL_P_TORQUE_HYDRO_PREVIOUS, d, i, p.np) = th;
-    pref(ps, KAPSEL_P_TORQUE_HYDRO, d, i, p.np) = 0.0;
+    velocity_update(ps, p, d, i, euler, dmy, dmy_rot, fh[d], fr[d], fh[3 + d]);
   }
 }
 

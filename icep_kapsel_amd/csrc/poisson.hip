@@ -30,7 +30,7 @@ namespace kapsel {
 
 namespace {
 
-enum { S_BB = 0, S_RZ, S_PAP, S_RR, S_RZN, S_BETA, S_TMP, S_INVEPS, S_N };
+enum { S_BB = 0, S_RZ, S_PAP, S_RR, S_RZN, S_BETA, S_TMP, S_INVEPS, S_M1, S_N };  // S_M1: -1
 constexpr int NPART = 1024;  // blocks of the partial-sum reductions (fixed: deterministic sums)
 constexpr unsigned TPB = 256;
 
@@ -287,8 +287,19 @@ __global__ __launch_bounds__(TPB) void finish_kernel(const double *__restrict__ 
                                                      double *__restrict__ scal, int slot, int mode,
                                                      double *host = nullptr, int hslot = 0,
                                                      unsigned long long *cnt = nullptr, double count = 0.0) {
+  // in rounds of four partials per thread, their loads issued together (the
+  // sum order is that of one partial per thread and round)
   double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += TPB) s += partial[i];
+  for (int i0 = 0; i0 < n; i0 += 4 * TPB) {
+    double v[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int i = i0 + threadIdx.x + k * TPB;
+      v[k] = i < n ? partial[i] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) s += v[k];
+  }
   s = block_sum(s);
   if (threadIdx.x == 0) {
     if (mode == 1) {
@@ -296,7 +307,10 @@ __global__ __launch_bounds__(TPB) void finish_kernel(const double *__restrict__ 
       scal[S_BETA] = old != 0.0 ? s / old : 0.0;
       scal[S_RZ] = s;
     }
-    if (mode == 2) scal[S_INVEPS] = s > 0.0 ? count / s : 1.0;  // 1 / mean of a sum over count points
+    if (mode == 2) {
+      scal[S_INVEPS] = s > 0.0 ? count / s : 1.0;  // 1 / mean of a sum over count points
+      scal[S_M1] = -1.0;                            // the refinement's extrapolation factor
+    }
     scal[slot] = s;
     if (host) {
       host[hslot] = s;
@@ -319,10 +333,18 @@ __global__ __launch_bounds__(TPB) void finish_kernel(const double *__restrict__ 
 __global__ __launch_bounds__(TPB) void finish2_kernel(const double *__restrict__ partial, double *__restrict__ scal,
                                                       int mode, double *host = nullptr,
                                                       unsigned long long *cnt = nullptr) {
+  static_assert(NPART % TPB == 0, "whole rounds");
+  double pa[NPART / TPB], pb[NPART / TPB];
+#pragma unroll
+  for (int k = 0; k < NPART / TPB; k++) {  // all loads issued before the sums
+    pa[k] = partial[threadIdx.x + k * TPB];
+    pb[k] = partial[NPART + threadIdx.x + k * TPB];
+  }
   double a = 0.0, b = 0.0;
-  for (int i = threadIdx.x; i < NPART; i += TPB) {
-    a += partial[i];
-    b += partial[NPART + i];
+#pragma unroll
+  for (int k = 0; k < NPART / TPB; k++) {
+    a += pa[k];
+    b += pb[k];
   }
   a = block_sum(a);
   b = block_sum(b);
@@ -1132,6 +1154,7 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
   // start: the POTENTIAL field if it was written since the last solve; else
   // the last solution, extrapolated linearly in time once two solutions of
   // consecutive solves are known
+  bool extrapolate_in_residual = false;
   if (!ctx->pot64_valid) {
     hipLaunchKernelGGL((convert_real_kernel<double, float>), dim3(fb), dim3(TPB), 0, ctx->stream, xk,
                        fld<float>(ctx, KAPSEL_POTENTIAL), g);
@@ -1140,9 +1163,16 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
     fft_r2c_t<double>(ctx, x1, 1);
     ctx->prev_valid = false;
   } else if (ctx->prev_valid) {
-    LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 3, sizeof(double)));
-    hipLaunchKernelGGL(extrapolate_all_kernel, dim3(fb), dim3(TPB), 0, ctx->stream, xk, ctx->x64_prev, g.f);
-    KAPSEL_LAUNCH_CHECK();
+    // prev <- 2 x - prev, the buffers swapped: where the first residual's
+    // first pass is fused, that pass forms it (reading x and prev, writing
+    // the start it transforms)
+    if (!(g.nx == 64 || g.nx == 128)) {
+      LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 3, sizeof(double)));
+      hipLaunchKernelGGL(extrapolate_all_kernel, dim3(fb), dim3(TPB), 0, ctx->stream, xk, ctx->x64_prev, g.f);
+      KAPSEL_LAUNCH_CHECK();
+    } else {
+      extrapolate_in_residual = true;
+    }
     std::swap(ctx->p64[1], ctx->x64_prev);
     xk = ctx->p64[1];
   } else {
@@ -1188,7 +1218,12 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
       KAPSEL_LAUNCH_CHECK();
       sub = nullptr;
     }
-    if (fft_x_grad_t<double>(ctx, xk, xk, ctx->d_scal, -1, g3, sub)) {
+    // (xk = 2 x64_prev - xk first where the extrapolation was deferred here)
+    const bool fused = extrapolate_in_residual
+                           ? fft_x_grad_t<double>(ctx, ctx->x64_prev, xk, ctx->d_scal, S_M1, g3, nullptr, 2.0)
+                           : fft_x_grad_t<double>(ctx, xk, xk, ctx->d_scal, -1, g3, sub);
+    extrapolate_in_residual = false;
+    if (fused) {
       fft_face_mid_t<double>(ctx, g3, f3);
       const int nb = fft_x_res_t<double>(ctx, g3, bk, rk, ctx->d_partial);
       hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->d_partial, nb, ctx->d_scal, S_RR,
