@@ -369,10 +369,10 @@ bool fft_grad2_f64_to_f32(kapsel_ctx *ctx, const double *a, const double *b, voi
 // g_d; one Parseval-weighted <p, ap> partial per workgroup; returns the
 // number of partials (0: NX has no fused pass)
 template <typename T>
-int fft_x_div_t(kapsel_ctx *ctx, void *const *g3, const T *p, T *ap, double *partial);
+int fft_x_div_t(kapsel_ctx *ctx, void *const *g3, const T *p, T *ap, double *partial, int max_partials);
 // fft_x_div_t's residual form: r = (divergence) - b, partials of |r|^2
 template <typename T>
-int fft_x_res_t(kapsel_ctx *ctx, void *const *g3, const T *b, T *r, double *partial);
+int fft_x_res_t(kapsel_ctx *ctx, void *const *g3, const T *b, T *r, double *partial, int max_partials);
 
 // k-space and real-space pointwise (kspace.hip)
 void truncate(kapsel_ctx *ctx, void *const *fields, int nf);

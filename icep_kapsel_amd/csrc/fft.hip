@@ -1483,6 +1483,8 @@ void init_tw_t(kapsel_ctx *ctx) {
 }
 
 constexpr int lines_per_block(size_t elem) { return elem == 8 ? 16 : 32; }
+// the y-split x passes of the operator (8 row pairs per workgroup)
+constexpr int dec_lines_per_block(size_t) { return 16; }
 
 template <typename T, int L, int DIR>
 void launch_lines(kapsel_ctx *ctx, int cls, const FieldPtrs<T> &fp, const FieldPtrs<T> &out, int nf, long long Q,
@@ -1844,11 +1846,11 @@ void fft_face_mid_t(kapsel_ctx *ctx, void *const *g3, void *const *face3) {
 template <typename T, int L>
 void launch_x_grad(kapsel_ctx *ctx, const T *z, T *p, const double *sc, int beta_slot, void *const *g3,
                    const float *sub, double zscale) {
-  constexpr int LPB = lines_per_block(sizeof(T));
   const Grid &g = ctx->g;
   const Twiddles &tw = ctx->tw[prec_of<T>()];
   LaunchTimer lt(ctx, KAPSEL_K_FFT_X_GRAD, 6.0 * g.f * sizeof(T));
   if (dec_ok<T>(g)) {
+    constexpr int LPB = dec_lines_per_block(sizeof(T));
     constexpr int LPT = LPB / 2;  // row pairs per tile
     const unsigned Q = (unsigned)(g.ny / 2) * g.h;
     hipLaunchKernelGGL((fft_x_grad_dec<T, L, LPB>), dim3((Q + LPT - 1) / LPT), dim3(LPB * LineFFT<T, L>::S), 0,
@@ -1857,34 +1859,46 @@ void launch_x_grad(kapsel_ctx *ctx, const T *z, T *p, const double *sc, int beta
                        g.wave_x, g.wave_y, g.wave_z, (const V2<T> *)tw.shx, (const V2<T> *)tw.shy,
                        (const V2<T> *)tw.shz, (const V2<T> *)tw.x, sub, (const V2<T> *)tw.y);
   } else {
+    constexpr int LPB = lines_per_block(sizeof(T));
     const long long Q = (long long)g.ny * g.h;
     hipLaunchKernelGGL((fft_x_grad<T, L, LPB>), dim3((unsigned)((Q + LPB - 1) / LPB)), dim3(LPB * LineFFT<T, L>::S),
                        0, ctx->stream, z, p, sc, beta_slot, (T)zscale, pack<T>(g3, 3), Q, Q, g.h, g.ny, g.wave_x,
-                       g.wave_y,
-                       g.wave_z, (const V2<T> *)tw.shx, (const V2<T> *)tw.shy, (const V2<T> *)tw.shz,
+                       g.wave_y, g.wave_z, (const V2<T> *)tw.shx, (const V2<T> *)tw.shy, (const V2<T> *)tw.shz,
                        (const V2<T> *)tw.x, sub);
   }
   KAPSEL_LAUNCH_CHECK();
 }
 
+// a divergence pass writes one partial sum per workgroup into a region of
+// max_partials slots (the buffer's next region holds other sums): refuse a
+// launch of more workgroups than that
+void check_partial_slots(int nb, int max_partials) {
+  if (nb > max_partials) {
+    set_error("the divergence pass would write %d partial sums into %d slots", nb, max_partials);
+    throw Error{KAPSEL_ERR_ARG};
+  }
+}
+
 template <typename T, int L, bool RES>
-int launch_x_div(kapsel_ctx *ctx, void *const *g3, const T *p, T *ap, double *partial) {
+int launch_x_div(kapsel_ctx *ctx, void *const *g3, const T *p, T *ap, double *partial, int max_partials) {
   constexpr int LPB = lines_per_block(sizeof(T));
   const Grid &g = ctx->g;
   const Twiddles &tw = ctx->tw[prec_of<T>()];
   LaunchTimer lt(ctx, KAPSEL_K_FFT_X_DIV, 5.0 * g.f * sizeof(T));
   unsigned nb;
   if (dec_ok<T>(g)) {
+    constexpr int LPBD = dec_lines_per_block(sizeof(T)), LPT = LPBD / 2;
     const unsigned Q = (unsigned)(g.ny / 2) * g.h;
-    constexpr int LPT = LPB / 2;
     nb = (Q + LPT - 1) / LPT;
-    hipLaunchKernelGGL((fft_x_div_dec<T, L, LPB, RES>), dim3(nb), dim3(LPB * DivFFT<T, L>::S), 0, ctx->stream,
+    check_partial_slots((int)nb, max_partials);
+    hipLaunchKernelGGL((fft_x_div_dec<T, L, LPBD, RES>), dim3(nb), dim3(LPBD * DivFFT<T, L>::S), 0, ctx->stream,
                        pack<T>(g3, 3), p, ap, partial, Q, (unsigned)(g.ny * g.h), g.h, g.ny, g.nz, g.wave_x,
                        g.wave_y, g.wave_z, (const V2<T> *)tw.shx, (const V2<T> *)tw.shy, (const V2<T> *)tw.shz,
                        (const V2<T> *)tw.x, (const V2<T> *)tw.y);
   } else {
     const long long Q = (long long)g.ny * g.h;
     nb = (unsigned)((Q + LPB - 1) / LPB);
+    check_partial_slots((int)nb, max_partials);
     hipLaunchKernelGGL((fft_x_div<T, L, LPB, RES>), dim3(nb), dim3(LPB * DivFFT<T, L>::S), 0, ctx->stream,
                        pack<T>(g3, 3), p, ap, partial, Q, Q, g.h, g.ny, g.nz, g.wave_x, g.wave_y, g.wave_z,
                        (const V2<T> *)tw.shx, (const V2<T> *)tw.shy, (const V2<T> *)tw.shz, (const V2<T> *)tw.x);
@@ -1918,19 +1932,19 @@ void launch_x_grad2_f64(kapsel_ctx *ctx, const double *a, const double *b, void 
 }
 
 template <typename T>
-int fft_x_div_t(kapsel_ctx *ctx, void *const *g3, const T *p, T *ap, double *partial) {
+int fft_x_div_t(kapsel_ctx *ctx, void *const *g3, const T *p, T *ap, double *partial, int max_partials) {
   switch (ctx->g.nx) {
-    case 64: return launch_x_div<T, 64, false>(ctx, g3, p, ap, partial);
-    case 128: return launch_x_div<T, 128, false>(ctx, g3, p, ap, partial);
+    case 64: return launch_x_div<T, 64, false>(ctx, g3, p, ap, partial, max_partials);
+    case 128: return launch_x_div<T, 128, false>(ctx, g3, p, ap, partial, max_partials);
     default: return 0;
   }
 }
 
 template <typename T>
-int fft_x_res_t(kapsel_ctx *ctx, void *const *g3, const T *b, T *r, double *partial) {
+int fft_x_res_t(kapsel_ctx *ctx, void *const *g3, const T *b, T *r, double *partial, int max_partials) {
   switch (ctx->g.nx) {
-    case 64: return launch_x_div<T, 64, true>(ctx, g3, b, r, partial);
-    case 128: return launch_x_div<T, 128, true>(ctx, g3, b, r, partial);
+    case 64: return launch_x_div<T, 64, true>(ctx, g3, b, r, partial, max_partials);
+    case 128: return launch_x_div<T, 128, true>(ctx, g3, b, r, partial, max_partials);
     default: return 0;
   }
 }
@@ -1956,14 +1970,14 @@ bool fft_grad2_f64_to_f32(kapsel_ctx *ctx, const double *a, const double *b, voi
 }
 
 template void fft_face_mid_t<double>(kapsel_ctx *, void *const *, void *const *);
-template int fft_x_res_t<double>(kapsel_ctx *, void *const *, const double *, double *, double *);
+template int fft_x_res_t<double>(kapsel_ctx *, void *const *, const double *, double *, double *, int);
 template void fft_face_mid_t<float>(kapsel_ctx *, void *const *, void *const *);
 template bool fft_x_grad_t<float>(kapsel_ctx *, const float *, float *, const double *, int, void *const *,
                                   const float *, double);
 template bool fft_x_grad_t<double>(kapsel_ctx *, const double *, double *, const double *, int, void *const *,
                                    const float *, double);
-template int fft_x_div_t<float>(kapsel_ctx *, void *const *, const float *, float *, double *);
-template int fft_x_div_t<double>(kapsel_ctx *, void *const *, const double *, double *, double *);
+template int fft_x_div_t<float>(kapsel_ctx *, void *const *, const float *, float *, double *, int);
+template int fft_x_div_t<double>(kapsel_ctx *, void *const *, const double *, double *, double *, int);
 template void fft_face_t<double>(kapsel_ctx *, void *const *, void *const *);
 template void fft_face_t<float>(kapsel_ctx *, void *const *, void *const *);
 template void fft_r2c_t<double>(kapsel_ctx *, void *const *, int);

@@ -490,8 +490,19 @@ __global__ __launch_bounds__(TPB) void kcg_update_kernel(T *__restrict__ x, T *r
   double pap;
   if (npap > 0) {
     __shared__ double pap_sh;
+    // rounds of four partials per thread, their loads issued together (the
+    // order of one partial per thread and round)
     double s = 0.0;
-    for (int i = threadIdx.x; i < npap; i += TPB) s += pap_part[i];
+    for (int i0 = 0; i0 < npap; i0 += 4 * TPB) {
+      double v[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int i = i0 + threadIdx.x + q * TPB;
+        v[q] = i < npap ? pap_part[i] : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++) s += v[q];
+    }
     s = block_sum(s);
     if (threadIdx.x == 0) pap_sh = s;
     __syncthreads();
@@ -870,7 +881,7 @@ void kcg_iteration(kapsel_ctx *ctx, T *x, const T *eps, int precond, T *const *w
     // <p, Ap> in the last
     fft_face_mid_t<T>(ctx, s3, f3);
     // <p, A p> partials beside the update's own (summed in the update)
-    npap = fft_x_div_t<T>(ctx, s3, p, ap, pap_part);
+    npap = fft_x_div_t<T>(ctx, s3, p, ap, pap_part, NPART);
   } else {
     {
       LaunchTimer lt(ctx, KAPSEL_K_POINTWISE, alg(ctx, 6, sizeof(T)));
@@ -1050,7 +1061,7 @@ int solve_fp64(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
   auto residual = [&]() {
     if (fft_x_grad_t<double>(ctx, xk, xk, ctx->d_scal, -1, g3)) {
       fft_face_mid_t<double>(ctx, g3, f3);
-      const int nb = fft_x_res_t<double>(ctx, g3, bk, rk, ctx->d_partial);
+      const int nb = fft_x_res_t<double>(ctx, g3, bk, rk, ctx->d_partial, NPART);
       hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->d_partial, nb, ctx->d_scal, S_RR,
                          0);
     } else {
@@ -1225,7 +1236,7 @@ int solve_fp32(kapsel_ctx *ctx, const double e0[3], double tol, int maxit, int p
     extrapolate_in_residual = false;
     if (fused) {
       fft_face_mid_t<double>(ctx, g3, f3);
-      const int nb = fft_x_res_t<double>(ctx, g3, bk, rk, ctx->d_partial);
+      const int nb = fft_x_res_t<double>(ctx, g3, bk, rk, ctx->d_partial, NPART);
       hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(TPB), 0, ctx->stream, ctx->d_partial, nb, ctx->d_scal, S_RR,
                          0, ctx->h_slots_dev, 3, ctx->d_count);
       KAPSEL_LAUNCH_CHECK();
