@@ -114,7 +114,8 @@ def run_icep(precision, precond, steps, warmup, learned, timed_class=None, snaps
     k.kernel_timing([])
     dominant = timed_class or max(SINGLE_KERNEL_CLASSES, key=lambda c: shares[c][0])
     # timed region: only the dominant class carries HIP events, on one launch
-    # in TIMING_EVERY (an event pair delays the launch it brackets)
+    # in TIMING_EVERY; the half-plane face launch takes them itself
+    # (hipExtLaunchKernelGGL), so they hold the kernel's own start and end
     k.kernel_timing([dominant])
     k.kernel_timing_every(TIMING_EVERY)
     k.synchronize()

@@ -35,18 +35,18 @@ static hipEvent_t take_event(kapsel_ctx *ctx) {
   return e;
 }
 
-LaunchTimer::LaunchTimer(kapsel_ctx *c, int k, double b)
-    : ctx(c), cls(c->refine64 ? KAPSEL_K_REFINE64 : k), bytes(b) {
+LaunchTimer::LaunchTimer(kapsel_ctx *c, int k, double b, bool kev)
+    : ctx(c), cls(c->refine64 ? KAPSEL_K_REFINE64 : k), bytes(b), kernel_events(kev) {
   if ((ctx->timing_mask & (1 << cls)) && ctx->timing_seen[cls]++ % (unsigned)ctx->timing_every == 0) {
     start = take_event(ctx);
     stop = take_event(ctx);
-    KAPSEL_HIP(hipEventRecord(start, ctx->stream));
+    if (!kernel_events) KAPSEL_HIP(hipEventRecord(start, ctx->stream));
   }
 }
 
 LaunchTimer::~LaunchTimer() noexcept(false) {
   if (stop) {
-    KAPSEL_HIP(hipEventRecord(stop, ctx->stream));
+    if (!kernel_events) KAPSEL_HIP(hipEventRecord(stop, ctx->stream));
     ctx->timing_pending.push_back(TimingRecord{cls, bytes, start, stop});
   }
 }

@@ -225,10 +225,6 @@ struct kapsel_ctx {
   unsigned long long *d_count = nullptr;  // [0], [1]: the device side of the two counts
   unsigned long long seq_cg = 0, seq_res = 0;  // how many of each were queued
   hipEvent_t slot_ev[2] = {};     // real-space CG (FP64 context): copies and events
-  // one k-space PCG iteration captured as a HIP graph, replayed while the
-  // buffers it was captured with (kcg_key) stay the same
-  hipGraphExec_t kcg_exec = nullptr;
-  const void *kcg_key[11] = {};
   int n_partial = 0;
   kapsel::Pnet *pnet = nullptr;   // learned warm-start network, on first use
   // electrolyte (electrolyte.hip): parameters, the conserved ion amounts, and
@@ -283,12 +279,18 @@ namespace kapsel {
 // Brackets one kernel launch with HIP events when its class is being timed;
 // `bytes` is the launch's algorithmic HBM traffic (each input read once,
 // each output written once).
+// kernel_events: the scope launches one kernel with hipExtLaunchKernelGGL
+// and passes it start and stop (null when this launch is not sampled), so
+// that they hold the kernel's own start and end; otherwise the events are
+// recorded on the stream around the scope's launches (their interval then
+// holds the dispatch gap before the first kernel as well)
 struct LaunchTimer {
   kapsel_ctx *ctx;
   int cls;
   double bytes;
   hipEvent_t start = nullptr, stop = nullptr;
-  LaunchTimer(kapsel_ctx *c, int k, double b);
+  bool kernel_events = false;
+  LaunchTimer(kapsel_ctx *c, int k, double b, bool kernel_events = false);
   ~LaunchTimer() noexcept(false);
 };
 

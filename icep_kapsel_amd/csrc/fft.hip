@@ -18,6 +18,8 @@
 // lines are adjacent in memory (lines of a tile differ in the fastest
 // index), so each load/store instruction of a wavefront touches few 64-B
 // segments.  Up to six fields are transformed by one launch (grid.y).
+#include <hip/hip_ext.h>
+
 #include "fft_prims.hpp"
 
 namespace kapsel {
@@ -1797,11 +1799,13 @@ void launch_half_face(kapsel_ctx *ctx, const FieldPtrs<T> &fp, const FieldPtrs<T
   const Grid &g = ctx->g;
   static_assert(PlaneCfg<T, N / 2, N, THREADS>::LDS_BYTES <= 160 * 1024, "half plane does not fit LDS");
   dim3 grid(2u * (unsigned)g.nx, 3);
-  LaunchTimer lt(ctx, KAPSEL_K_FFT_PLANE_FACE, 2.0 * 3 * g.f * sizeof(T) + 3.0 * g.f * sizeof(T));
+  // (a sampled launch carries its timing events itself: their interval is
+  // the kernel's run, without the dispatch gap before it)
+  LaunchTimer lt(ctx, KAPSEL_K_FFT_PLANE_FACE, 2.0 * 3 * g.f * sizeof(T) + 3.0 * g.f * sizeof(T), true);
   const Twiddles &tw = ctx->tw[prec_of<T>()];
   T scale = (T)(1.0 / ((double)g.nx * g.ny * g.nz));
-  hipLaunchKernelGGL((fft_half_face<T, N, N, THREADS>), grid, dim3(THREADS), 0, ctx->stream, fp,
-                     (const V2<T> *)tw.zh, (const V2<T> *)tw.zrot, (const V2<T> *)tw.y, scale, face);
+  hipExtLaunchKernelGGL((fft_half_face<T, N, N, THREADS>), grid, dim3(THREADS), 0, ctx->stream, lt.start, lt.stop, 0,
+                        fp, (const V2<T> *)tw.zh, (const V2<T> *)tw.zrot, (const V2<T> *)tw.y, scale, face);
   KAPSEL_LAUNCH_CHECK();
 }
 
