@@ -1193,8 +1193,8 @@ __global__ __launch_bounds__((LPB * LineFFT<T, L>::S)) void fft_x_grad_dec(
   V2<T> wj = tw_get<+1>(twy, j);
   if (!mb) wj = V2<T>{1, 0};
   const T sg = mb ? (T)-1 : (T)1;
-  // G_x first: its stores are in flight while X is transformed
   F::template run<+1>(y, sl, twr, s, LPB);
+  F::template run<+1>(x, sl, twr, s, LPB);
   T *__restrict__ gx = g3.p[0], *__restrict__ gy = g3.p[1], *__restrict__ gz = g3.p[2];
 #pragma unroll
   for (int m = 0; m < P; m++) {
@@ -1202,7 +1202,6 @@ __global__ __launch_bounds__((LPB * LineFFT<T, L>::S)) void fft_x_grad_dec(
     const V2<T> y0 = mb ? yo : y[m], y1 = mb ? y[m] : yo;
     if (valid) st2(gx + 2 * (own + F::out_i(s, m) * plane), cmul(y0 + scal(sg, y1), wj));
   }
-  F::template run<+1>(x, sl, twr, s, LPB);
   // the y symbols of the pair's rows j' and j' + J, the z symbol
   const T k0 = (T)(signed_k(j, ny) * wy), k1 = (T)(signed_k(j + ny / 2, ny) * wy);
   const V2<T> e0 = shy[j], e1 = shy[j + ny / 2], ez = shz[kz];
