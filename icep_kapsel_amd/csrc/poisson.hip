@@ -865,8 +865,8 @@ struct ConvergenceWatch {
 
 // One k-space PCG iteration: p = z + beta p, ap = A p (3 inverse and 3
 // forward transforms), alpha, x += alpha p, r += alpha ap, z = M^{-1} r,
-// beta.  Every scalar lives in device memory, so the sequence can be
-// captured once and replayed.
+// beta.  Every scalar lives in device memory: the host queues iterations
+// ahead without reading any of them.
 template <typename T>
 void kcg_iteration(kapsel_ctx *ctx, T *x, const T *eps, int precond, T *const *w, T *const *s,
                    T *const *face) {
